@@ -1,0 +1,711 @@
+// CPU oracle for the frovedis per-rank dataframe execution core.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, in plain C++ on the host,
+// the algorithms of the reference's x86 code path so that the HIP library
+// can be checked against it.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py load it; the product library never does.
+//
+// Every function cites the reference file:line it follows (paths relative to
+// the frovedis source tree).  How each function is pinned (closed forms,
+// tutorial known answers, brute force) is stated next to it; the pinning
+// tests live in tests/test_oracle.py.
+//
+// Type codes shared with the HIP library: 0=int32 1=uint32 2=int64 3=uint64
+// 4=float 5=double.
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+enum { T_I32 = 0, T_U32 = 1, T_I64 = 2, T_U64 = 3, T_F32 = 4, T_F64 = 5 };
+
+// ---------------------------------------------------------------------------
+// radix_sort: src/frovedis/core/radix_sort.hpp:197-242 (dispatch, flips,
+// insertion threshold 64 on x86 :13-18) and radix_sort.incl:7-240 (one stable
+// counting pass per byte of the key, least significant first).
+// Pinned by: closed form (numpy stable argsort) for integer keys; for float
+// keys by the quirks read from radix_sort.hpp:101-118,244-266 (tests).
+// ---------------------------------------------------------------------------
+
+constexpr size_t kInsertionThr = 64;  // radix_sort.hpp:17 (x86)
+
+// insertion sort with the reference's comparison (`>`, or `<` for desc),
+// radix_sort.hpp:31-99
+template <class K>
+void insertion(K* k, uint8_t* v, int vb, size_t n, bool desc) {
+  for (size_t i = 1; i < n; i++) {
+    K tmp = k[i];
+    uint8_t vtmp[8];
+    if (vb) std::memcpy(vtmp, v + i * vb, vb);
+    bool out_of_order = desc ? (k[i - 1] < tmp) : (k[i - 1] > tmp);
+    if (!out_of_order) continue;
+    size_t j = i;
+    do {
+      k[j] = k[j - 1];
+      if (vb) std::memcpy(v + j * vb, v + (j - 1) * vb, vb);
+      j--;
+    } while (j > 0 && (desc ? (k[j - 1] < tmp) : (k[j - 1] > tmp)));
+    k[j] = tmp;
+    if (vb) std::memcpy(v + j * vb, vtmp, vb);
+  }
+}
+
+// One LSD pass per byte for bytes [0, nbytes); radix_sort.incl:43-228.
+// Descending uses bucket 255-digit (radix_sort.incl:242-480).
+template <class U>
+void lsd(U* k, uint8_t* v, int vb, size_t n, size_t nbytes, bool desc) {
+  std::vector<U> kt(n);
+  std::vector<uint8_t> vt(vb ? n * vb : 0);
+  U* ks = k; U* kd = kt.data();
+  uint8_t* vs = v; uint8_t* vd = vt.data();
+  for (size_t d = 0; d < nbytes; d++) {
+    size_t cnt[257] = {0};
+    int sh = int(d * 8);
+    for (size_t i = 0; i < n; i++) {
+      unsigned b = unsigned((ks[i] >> sh) & 0xFF);
+      if (desc) b = 255 - b;
+      cnt[b + 1]++;
+    }
+    for (int b = 0; b < 256; b++) cnt[b + 1] += cnt[b];
+    for (size_t i = 0; i < n; i++) {
+      unsigned b = unsigned((ks[i] >> sh) & 0xFF);
+      if (desc) b = 255 - b;
+      size_t to = cnt[b]++;
+      kd[to] = ks[i];
+      if (vb) std::memcpy(vd + to * vb, vs + i * vb, vb);
+    }
+    std::swap(ks, kd);
+    std::swap(vs, vd);
+  }
+  if (ks != k) {  // odd pass count: copy back (radix_sort.incl:230-239)
+    std::memcpy(k, ks, n * sizeof(U));
+    if (vb) std::memcpy(v, vs, n * vb);
+  }
+}
+
+// get_max_key_size: radix_sort.hpp:135-153 (number of bytes of the maximum,
+// maximum initialised to 0)
+template <class K>
+size_t max_key_size(const K* k, size_t n) {
+  K mx = 0;
+  for (size_t i = 0; i < n; i++) if (k[i] > mx) mx = k[i];
+  size_t ks = sizeof(K);
+  unsigned long long tmp = 0xFF;
+  for (size_t i = 1; i < ks + 1; i++) {
+    if ((unsigned long long)mx <= tmp) return i;
+    tmp = (tmp << 8) | 0xFF;
+  }
+  return ks;
+}
+
+template <class K>
+bool contain_negative(const K* k, size_t n) {  // radix_sort.hpp:101-133
+  if (!std::numeric_limits<K>::is_signed && std::numeric_limits<K>::is_integer)
+    return false;
+  for (size_t i = 0; i < n; i++) if (k[i] < 0) return true;
+  return false;
+}
+
+template <class K, class U>
+void radix_sort_int(K* k, uint8_t* v, int vb, size_t n, bool positive_only,
+                    bool desc) {
+  if (n < kInsertionThr) { insertion(k, v, vb, n, desc); return; }
+  U* u = reinterpret_cast<U*>(k);
+  const U top = U(1) << (sizeof(U) * 8 - 1);
+  if (positive_only) {
+    lsd(u, v, vb, n, max_key_size(k, n), desc);
+    return;
+  }
+  bool neg = contain_negative(k, n);
+  if (neg) {
+    for (size_t i = 0; i < n; i++) u[i] ^= top;          // flip_top :155-166
+    lsd(u, v, vb, n, sizeof(U), desc);
+    for (size_t i = 0; i < n; i++) u[i] ^= top;
+  } else {
+    lsd(u, v, vb, n, max_key_size(k, n), desc);
+  }
+}
+
+template <class F, class U>
+void radix_sort_fp(F* k, uint8_t* v, int vb, size_t n, bool positive_only,
+                   bool desc) {
+  if (n < kInsertionThr) { insertion(k, v, vb, n, desc); return; }
+  U* u = reinterpret_cast<U*>(k);
+  const int sh = int(sizeof(U) * 8 - 1);
+  const U top = U(1) << sh;
+  if (positive_only) {  // radix_sort.hpp:250-253: raw bits
+    lsd(u, v, vb, n, max_key_size(u, n), desc);
+    return;
+  }
+  bool neg = contain_negative(k, n);
+  if (neg) {
+    // float_flip / double_flip, radix_sort.hpp:168-195
+    for (size_t i = 0; i < n; i++) u[i] ^= (U(0) - (u[i] >> sh)) | top;
+    lsd(u, v, vb, n, sizeof(U), desc);
+    for (size_t i = 0; i < n; i++) u[i] ^= ((u[i] >> sh) - 1) | top;
+  } else {
+    lsd(u, v, vb, n, max_key_size(u, n), desc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// set_separate / set_unique / set_is_unique: core/set_operations.hpp:146-197
+// (x86 bodies).  Pinned by closed form (run starts of a sorted array).
+// ---------------------------------------------------------------------------
+template <class T>
+size_t separate(const T* k, size_t n, size_t* out) {
+  if (n == 0) { out[0] = 0; return 1; }
+  size_t c = 0;
+  T cur = k[0];
+  out[c++] = 0;
+  for (size_t i = 1; i < n; i++)
+    if (k[i] != cur) { cur = k[i]; out[c++] = i; }
+  out[c++] = n;
+  return c;
+}
+
+template <class T>
+int is_unique(const T* k, size_t n) {
+  if (n == 0) return 1;
+  T cur = k[0];
+  for (size_t i = 1; i < n; i++) {
+    if (k[i] != cur) cur = k[i];
+    else return 0;
+  }
+  return 1;
+}
+
+// lower_bound / upper_bound: core/lower_bound.hpp:13-101,
+// core/upper_bound.hpp (branch-free binary search; same result as
+// std::lower_bound / std::upper_bound).  Pinned by closed form.
+template <class T>
+void lower_bound_t(const T* s, size_t m, const T* q, size_t nq, size_t* out) {
+  for (size_t i = 0; i < nq; i++) out[i] = std::lower_bound(s, s + m, q[i]) - s;
+}
+template <class T>
+void upper_bound_t(const T* s, size_t m, const T* q, size_t nq, size_t* out) {
+  for (size_t i = 0; i < nq; i++) out[i] = std::upper_bound(s, s + m, q[i]) - s;
+}
+
+// ---------------------------------------------------------------------------
+// unique_hashtable<K, size_t>: dataframe/hashtable.hpp.
+//  - table size: smallest listed prime >= 3n (:23, :115-152, :1174-1179)
+//  - home slot: unsigned key bits % table_size (:1190-1212)
+//  - the last row (in index order) to hash to a slot owns it (:1197-1200)
+//  - rows whose key differs from the slot owner's key form the conflict list,
+//    in row order (find_miss_* :155-1165), then stable radix-sorted (:1234)
+//  - uniqueness: a row equal to the owner's key that is not the owner, or a
+//    duplicate inside the sorted conflict list (:1266-1321)
+//  - lookup: slot key, else lower_bound into the conflict list (:1386-1515)
+//  - all_keys: owned slots in slot order, then the conflict keys (:1584-1607)
+// Pinned by: dictionary semantics and brute force (tests).
+// ---------------------------------------------------------------------------
+const long long kPrimes[] = {
+    8 + 3, 16 + 3, 32 + 5, 64 + 3, 128 + 3, 256 + 27, 512 + 9, 1024 + 9,
+    2048 + 5, 4096 + 3, 8192 + 27, 16384 + 43, 32768 + 3, 65536 + 45,
+    131072 + 29, 262144 + 3, 524288 + 21, 1048576 + 7, 2097152 + 17,
+    4194304 + 15, 8388608 + 9, 16777216 + 43, 33554432 + 35, 67108864 + 15,
+    134217728 + 29, 268435456 + 3, 536870912 + 11, 1073741824 + 85,
+    2147483648LL + 11, 4294967296LL + 15, 8589934592LL + 17,
+    17179869184LL + 25, 34359738368LL + 53};
+
+struct HT {
+  int kbytes;
+  size_t table_size;
+  std::vector<uint64_t> key;      // key bits per slot
+  std::vector<uint64_t> val;
+  std::vector<int> filled;
+  std::vector<uint64_t> ckey;     // conflict keys (bits), sorted by typed key
+  std::vector<uint64_t> cval;
+  int ktype;
+};
+
+inline uint64_t load_bits(const void* p, int kb, size_t i) {
+  if (kb == 4) return reinterpret_cast<const uint32_t*>(p)[i];
+  return reinterpret_cast<const uint64_t*>(p)[i];
+}
+
+template <class K>
+K as_typed(uint64_t bits) {
+  K k;
+  if (sizeof(K) == 4) { uint32_t b = uint32_t(bits); std::memcpy(&k, &b, 4); }
+  else std::memcpy(&k, &bits, 8);
+  return k;
+}
+
+template <class K>
+bool typed_less(uint64_t a, uint64_t b) { return as_typed<K>(a) < as_typed<K>(b); }
+template <class K>
+bool typed_eq(uint64_t a, uint64_t b) { return as_typed<K>(a) == as_typed<K>(b); }
+
+size_t table_size_for(size_t n) {
+  size_t target = n * 3;
+  for (long long p : kPrimes) if ((unsigned long long)p >= target) return size_t(p);
+  throw std::runtime_error("unique_hash_table: requested size too large");
+}
+
+template <class K>
+void ht_build_t(HT* h, const void* keys, const uint64_t* vals, size_t n,
+                int check_unique, int* is_unique_ok) {
+  h->table_size = table_size_for(n);
+  size_t ts = h->table_size;
+  h->key.assign(ts, 0);
+  h->val.assign(ts, 0);
+  h->filled.assign(ts, 0);
+  std::vector<size_t> owner(ts, std::numeric_limits<size_t>::max());
+  std::vector<size_t> hash(n);
+  for (size_t i = 0; i < n; i++) {
+    uint64_t b = load_bits(keys, h->kbytes, i);
+    size_t s = size_t(b % ts);
+    h->key[s] = b;
+    h->filled[s] = 1;
+    owner[s] = i;
+    hash[i] = s;
+  }
+  int uniq = 1;
+  std::vector<size_t> missed;
+  for (size_t i = 0; i < n; i++) {
+    uint64_t b = load_bits(keys, h->kbytes, i);
+    size_t s = hash[i];
+    if (!typed_eq<K>(b, h->key[s])) { missed.push_back(i); continue; }
+    if (owner[s] == i) { if (vals) h->val[s] = vals[i]; }
+    else {
+      // duplicate of the owner's key: only the unique-check ctor records it
+      uniq = 0;
+      if (!check_unique && vals) h->val[s] = vals[i];  // unspecified in ref
+    }
+  }
+  // conflict list in row order, then stable sort by typed key
+  std::vector<size_t> order(missed.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return typed_less<K>(load_bits(keys, h->kbytes, missed[a]),
+                         load_bits(keys, h->kbytes, missed[b]));
+  });
+  h->ckey.resize(missed.size());
+  h->cval.resize(missed.size());
+  for (size_t i = 0; i < order.size(); i++) {
+    size_t r = missed[order[i]];
+    h->ckey[i] = load_bits(keys, h->kbytes, r);
+    h->cval[i] = vals ? vals[r] : 0;
+  }
+  for (size_t i = 1; i < h->ckey.size(); i++)
+    if (typed_eq<K>(h->ckey[i], h->ckey[i - 1])) uniq = 0;
+  if (is_unique_ok) *is_unique_ok = uniq;
+}
+
+template <class K>
+size_t conflict_lb(const HT* h, uint64_t b) {
+  size_t lo = 0, hi = h->ckey.size();
+  while (lo < hi) {
+    size_t mid = (lo + hi) / 2;
+    if (typed_less<K>(h->ckey[mid], b)) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// lookup with miss list (hashtable.hpp:1442-1515): misses are returned as the
+// sorted union of "clear" misses (empty slot) and conflict-list misses; the
+// value of a missed row stays 0 (value-initialised output, :1447).
+template <class K>
+size_t ht_lookup_t(const HT* h, const void* keys, size_t n, uint64_t* out,
+                   size_t* miss) {
+  size_t nm = 0;
+  for (size_t i = 0; i < n; i++) {
+    uint64_t b = load_bits(keys, h->kbytes, i);
+    size_t s = size_t(b % h->table_size);
+    out[i] = 0;
+    if (!h->filled[s]) { miss[nm++] = i; continue; }
+    if (typed_eq<K>(h->key[s], b)) { out[i] = h->val[s]; continue; }
+    size_t p = conflict_lb<K>(h, b);
+    if (p < h->ckey.size() && typed_eq<K>(h->ckey[p], b)) out[i] = h->cval[p];
+    else miss[nm++] = i;
+  }
+  return nm;
+}
+
+template <class K>
+void ht_exist_t(const HT* h, const void* keys, size_t n, int* out) {
+  for (size_t i = 0; i < n; i++) {
+    uint64_t b = load_bits(keys, h->kbytes, i);
+    size_t s = size_t(b % h->table_size);
+    if (!h->filled[s]) { out[i] = 0; continue; }
+    if (typed_eq<K>(h->key[s], b)) { out[i] = 1; continue; }
+    size_t p = conflict_lb<K>(h, b);
+    out[i] = (p < h->ckey.size() && typed_eq<K>(h->ckey[p], b)) ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// equi_join: dataframe/join.hpp.
+//  algo 0 (VE body, :304-326): unique_hashtable on RIGHT; if unique, probe
+//    LEFT in order and drop misses (shrink_missed, join.cc:10-61); otherwise
+//    stable-sort both sides and emit, for every left row in sorted order, its
+//    matching right rows in sorted order (multi_equi_join :198-302).
+//  algo 1 (x86 body, :25-45): std::unordered_multimap on RIGHT, probe LEFT in
+//    order; the order inside a key follows libstdc++ and is only compared
+//    after canonical sorting.
+// Pinned by: brute force pair sets and the tutorial join (tests).
+// ---------------------------------------------------------------------------
+template <class K>
+void equi_join_t(const void* left, const size_t* lidx, size_t nl,
+                 const void* right, const size_t* ridx, size_t nr, int algo,
+                 std::vector<size_t>& lo, std::vector<size_t>& ro,
+                 int* used_hash) {
+  const K* l = reinterpret_cast<const K*>(left);
+  const K* r = reinterpret_cast<const K*>(right);
+  lo.clear(); ro.clear();
+  if (algo == 1) {
+    std::unordered_multimap<K, size_t> m;
+    for (size_t i = 0; i < nr; i++) m.insert(std::make_pair(r[i], ridx[i]));
+    for (size_t i = 0; i < nl; i++) {
+      auto range = m.equal_range(l[i]);
+      for (auto it = range.first; it != range.second; ++it) {
+        lo.push_back(lidx[i]);
+        ro.push_back(it->second);
+      }
+    }
+    if (used_hash) *used_hash = 1;
+    return;
+  }
+  HT h;
+  h.kbytes = sizeof(K);
+  int uniq = 1;
+  std::vector<uint64_t> rv(ridx, ridx + nr);
+  ht_build_t<K>(&h, right, rv.data(), nr, 1, &uniq);
+  if (uniq) {
+    std::vector<uint64_t> got(nl);
+    std::vector<size_t> miss(nl);
+    size_t nm = ht_lookup_t<K>(&h, left, nl, got.data(), miss.data());
+    size_t mi = 0;
+    for (size_t i = 0; i < nl; i++) {
+      if (mi < nm && miss[mi] == i) { mi++; continue; }
+      lo.push_back(lidx[i]);
+      ro.push_back(size_t(got[i]));
+    }
+    if (used_hash) *used_hash = 1;
+    return;
+  }
+  // sort-merge: stable sort of (key, idx) on both sides
+  std::vector<size_t> lp(nl), rp(nr);
+  for (size_t i = 0; i < nl; i++) lp[i] = i;
+  for (size_t i = 0; i < nr; i++) rp[i] = i;
+  std::stable_sort(lp.begin(), lp.end(), [&](size_t a, size_t b) { return l[a] < l[b]; });
+  std::stable_sort(rp.begin(), rp.end(), [&](size_t a, size_t b) { return r[a] < r[b]; });
+  size_t j = 0;
+  for (size_t ii = 0; ii < nl; ii++) {
+    K key = l[lp[ii]];
+    while (j < nr && r[rp[j]] < key) j++;
+    for (size_t t = j; t < nr && r[rp[t]] == key; t++) {
+      lo.push_back(lidx[lp[ii]]);
+      ro.push_back(ridx[rp[t]]);
+    }
+  }
+  if (used_hash) *used_hash = 0;
+}
+
+// ---------------------------------------------------------------------------
+// group_by on one rank: dataframe/dfcolumn_groupby.cc:16-105.  With one node,
+// split_by_hash / alltoall_exchange / the second sort are identities and
+// merge_map[0] = iota, so the per-rank result is: newval[i] = val[idx[i]];
+// radix_sort(newval, idx); split = set_separate(newval); group keys =
+// newval[split[i]].  The last group is the NULL group when its key equals
+// numeric_limits<T>::max() (:98-102).
+// ---------------------------------------------------------------------------
+template <class K>
+size_t group_by_t(const void* col, size_t* idx, size_t n, size_t* split,
+                  void* keys_out) {
+  const K* c = reinterpret_cast<const K*>(col);
+  std::vector<K> nv(n);
+  for (size_t i = 0; i < n; i++) nv[i] = c[idx[i]];
+  uint8_t* v = reinterpret_cast<uint8_t*>(idx);
+  if (std::numeric_limits<K>::is_integer) {
+    if (sizeof(K) == 4) {
+      if (std::numeric_limits<K>::is_signed)
+        radix_sort_int<int32_t, uint32_t>(reinterpret_cast<int32_t*>(nv.data()), v, 8, n, false, false);
+      else
+        radix_sort_int<uint32_t, uint32_t>(reinterpret_cast<uint32_t*>(nv.data()), v, 8, n, false, false);
+    } else {
+      if (std::numeric_limits<K>::is_signed)
+        radix_sort_int<int64_t, uint64_t>(reinterpret_cast<int64_t*>(nv.data()), v, 8, n, false, false);
+      else
+        radix_sort_int<uint64_t, uint64_t>(reinterpret_cast<uint64_t*>(nv.data()), v, 8, n, false, false);
+    }
+  } else if (sizeof(K) == 4) {
+    radix_sort_fp<float, uint32_t>(reinterpret_cast<float*>(nv.data()), v, 8, n, false, false);
+  } else {
+    radix_sort_fp<double, uint64_t>(reinterpret_cast<double*>(nv.data()), v, 8, n, false, false);
+  }
+  size_t ns = separate(nv.data(), n, split);
+  K* ko = reinterpret_cast<K*>(keys_out);
+  for (size_t i = 0; i + 1 < ns; i++) ko[i] = nv[split[i]];
+  return ns;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped aggregation on one rank: dataframe/dfcolumn_grouped_agg.cc (x86
+// #else branches).  With one node the exchange is an identity and the merge
+// is newval[j] = T() + partial[j] (0 + x; :213-232), so:
+//   sum  (:105-190):  nulls read as 0; sequential left fold from 0 in
+//                     grouped (stable sorted) row order; result 0 + fold.
+//   count(:719-813):  split[i+1]-split[i] without nulls; else a sum of ones
+//                     with nulls read as 0.
+//   max  (:366-452):  nulls read as lowest(); m = lowest(); if (m < v) m = v.
+//   min  (:513-717):  nulls read as max(); m = max(); if (m > v) m = v.
+//   avg  (:839-889):  double(sum)/double(count); groups whose count is 0 get
+//                     numeric_limits<T>::max().
+// op codes: 0=sum 1=count 2=avg 3=max 4=min.  Outputs: sum/max/min in T,
+// count as uint64, avg as double.  Pinned by: tutorial known answers and
+// closed forms (tests); the FP fold order is the restatement itself.
+// ---------------------------------------------------------------------------
+template <class T>
+void grouped_agg_t(int op, const void* colp, const size_t* gidx, size_t n,
+                   const size_t* split, size_t ns, const size_t* nulls,
+                   size_t nn, void* out) {
+  const T* col = reinterpret_cast<const T*>(colp);
+  std::vector<uint8_t> isnull(nn ? n : 0, 0);
+  for (size_t i = 0; i < nn; i++) isnull[nulls[i]] = 1;
+  size_t G = ns ? ns - 1 : 0;
+  auto val_at = [&](size_t row, T nullval) -> T {
+    return (nn && isnull[row]) ? nullval : col[row];
+  };
+  if (op == 0 || op == 2) {
+    std::vector<T> sums(G);
+    std::vector<uint64_t> cnt(G);
+    for (size_t g = 0; g < G; g++) {
+      T total = 0;
+      uint64_t c = 0;
+      for (size_t j = split[g]; j < split[g + 1]; j++) {
+        total += val_at(gidx[j], T(0));
+        c += (nn && isnull[gidx[j]]) ? 0 : 1;
+      }
+      T merged = T();
+      merged += total;
+      sums[g] = merged;
+      cnt[g] = c;
+    }
+    if (op == 0) {
+      T* o = reinterpret_cast<T*>(out);
+      for (size_t g = 0; g < G; g++)
+        o[g] = (nn && cnt[g] == 0) ? std::numeric_limits<T>::max() : sums[g];
+    } else {
+      double* o = reinterpret_cast<double*>(out);
+      for (size_t g = 0; g < G; g++)
+        o[g] = (cnt[g] == 0 && nn) ? double(std::numeric_limits<T>::max())
+                                   : double(sums[g]) / double(cnt[g]);
+    }
+  } else if (op == 1) {
+    uint64_t* o = reinterpret_cast<uint64_t*>(out);
+    for (size_t g = 0; g < G; g++) {
+      uint64_t c = 0;
+      for (size_t j = split[g]; j < split[g + 1]; j++)
+        c += (nn && isnull[gidx[j]]) ? 0 : 1;
+      o[g] = c;
+    }
+  } else if (op == 3 || op == 4) {
+    // typed max (:454-510): the merge starts from lowest() and keeps the
+    // larger value; with nulls in the column, a result equal to lowest()
+    // is NULL and reads max().  min is the mirror image (:513-717).
+    T* o = reinterpret_cast<T*>(out);
+    const T lo = std::numeric_limits<T>::lowest(), hi = std::numeric_limits<T>::max();
+    for (size_t g = 0; g < G; g++) {
+      T m = (op == 3) ? lo : hi;
+      for (size_t j = split[g]; j < split[g + 1]; j++) {
+        T v = val_at(gidx[j], op == 3 ? lo : hi);
+        if (op == 3) { if (m < v) m = v; } else { if (m > v) m = v; }
+      }
+      T merged = (op == 3) ? lo : hi;
+      if (op == 3) { if (merged < m) merged = m; } else { if (merged > m) merged = m; }
+      if (op == 3 && nn && merged == lo) merged = hi;
+      o[g] = merged;
+    }
+  }
+}
+
+template <class F>
+void dispatch_key(int t, F&& f) {
+  switch (t) {
+    case T_I32: f(int32_t()); break;
+    case T_U32: f(uint32_t()); break;
+    case T_I64: f(int64_t()); break;
+    case T_U64: f(uint64_t()); break;
+    case T_F32: f(float()); break;
+    case T_F64: f(double()); break;
+    default: throw std::runtime_error("bad type code");
+  }
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+int orc_radix_sort(int ktype, void* keys, int vbytes, void* vals, size_t n,
+                   int positive_only, int descending) {
+  uint8_t* v = reinterpret_cast<uint8_t*>(vals);
+  if (vbytes != 0 && vbytes != 4 && vbytes != 8) return -1;
+  bool po = positive_only != 0, de = descending != 0;
+  switch (ktype) {
+    case T_I32: radix_sort_int<int32_t, uint32_t>((int32_t*)keys, v, vbytes, n, po, de); break;
+    case T_U32: radix_sort_int<uint32_t, uint32_t>((uint32_t*)keys, v, vbytes, n, po, de); break;
+    case T_I64: radix_sort_int<int64_t, uint64_t>((int64_t*)keys, v, vbytes, n, po, de); break;
+    case T_U64: radix_sort_int<uint64_t, uint64_t>((uint64_t*)keys, v, vbytes, n, po, de); break;
+    case T_F32: radix_sort_fp<float, uint32_t>((float*)keys, v, vbytes, n, po, de); break;
+    case T_F64: radix_sort_fp<double, uint64_t>((double*)keys, v, vbytes, n, po, de); break;
+    default: return -1;
+  }
+  return 0;
+}
+
+// prefix_sum x86 body, core/prefix_sum.hpp:67-76: sequential inclusive scan.
+int orc_prefix_sum(int type, const void* in, void* out, size_t n) {
+  try {
+    dispatch_key(type, [&](auto z) {
+      using T = decltype(z);
+      const T* a = reinterpret_cast<const T*>(in);
+      T* o = reinterpret_cast<T*>(out);
+      T cur = 0;
+      for (size_t j = 0; j < n; j++) { cur = a[j] + cur; o[j] = cur; }
+    });
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+  return 0;
+}
+
+// returns the number of entries written to out (capacity n + 1)
+size_t orc_set_separate(int type, const void* keys, size_t n, size_t* out) {
+  size_t r = 0;
+  dispatch_key(type, [&](auto z) {
+    using T = decltype(z);
+    r = separate(reinterpret_cast<const T*>(keys), n, out);
+  });
+  return r;
+}
+
+int orc_set_is_unique(int type, const void* keys, size_t n) {
+  int r = 1;
+  dispatch_key(type, [&](auto z) {
+    using T = decltype(z);
+    r = is_unique(reinterpret_cast<const T*>(keys), n);
+  });
+  return r;
+}
+
+void orc_lower_bound(int type, const void* sorted, size_t m, const void* q,
+                     size_t nq, size_t* out, int upper) {
+  dispatch_key(type, [&](auto z) {
+    using T = decltype(z);
+    if (upper) upper_bound_t(reinterpret_cast<const T*>(sorted), m,
+                             reinterpret_cast<const T*>(q), nq, out);
+    else lower_bound_t(reinterpret_cast<const T*>(sorted), m,
+                       reinterpret_cast<const T*>(q), nq, out);
+  });
+}
+
+void* orc_ht_build(int ktype, const void* keys, const uint64_t* vals, size_t n,
+                   int check_unique, int* is_unique_ok) {
+  HT* h = new HT;
+  h->ktype = ktype;
+  h->kbytes = (ktype == T_I32 || ktype == T_U32 || ktype == T_F32) ? 4 : 8;
+  try {
+    dispatch_key(ktype, [&](auto z) {
+      using K = decltype(z);
+      ht_build_t<K>(h, keys, vals, n, check_unique, is_unique_ok);
+    });
+  } catch (std::exception& e) { g_err = e.what(); delete h; return nullptr; }
+  return h;
+}
+
+size_t orc_ht_table_size(void* hp) { return reinterpret_cast<HT*>(hp)->table_size; }
+size_t orc_ht_conflict_size(void* hp) { return reinterpret_cast<HT*>(hp)->ckey.size(); }
+
+size_t orc_ht_lookup(void* hp, const void* keys, size_t n, uint64_t* out,
+                     size_t* miss) {
+  HT* h = reinterpret_cast<HT*>(hp);
+  size_t r = 0;
+  dispatch_key(h->ktype, [&](auto z) {
+    using K = decltype(z);
+    r = ht_lookup_t<K>(h, keys, n, out, miss);
+  });
+  return r;
+}
+
+void orc_ht_check_existence(void* hp, const void* keys, size_t n, int* out) {
+  HT* h = reinterpret_cast<HT*>(hp);
+  dispatch_key(h->ktype, [&](auto z) {
+    using K = decltype(z);
+    ht_exist_t<K>(h, keys, n, out);
+  });
+}
+
+// all_keys (hashtable.hpp:1584-1607); out capacity = number of build rows
+size_t orc_ht_all_keys(void* hp, void* out) {
+  HT* h = reinterpret_cast<HT*>(hp);
+  size_t c = 0;
+  for (size_t s = 0; s < h->table_size; s++) {
+    if (!h->filled[s]) continue;
+    if (h->kbytes == 4) reinterpret_cast<uint32_t*>(out)[c++] = uint32_t(h->key[s]);
+    else reinterpret_cast<uint64_t*>(out)[c++] = h->key[s];
+  }
+  for (size_t i = 0; i < h->ckey.size(); i++) {
+    if (h->kbytes == 4) reinterpret_cast<uint32_t*>(out)[c++] = uint32_t(h->ckey[i]);
+    else reinterpret_cast<uint64_t*>(out)[c++] = h->ckey[i];
+  }
+  return c;
+}
+
+void orc_ht_free(void* hp) { delete reinterpret_cast<HT*>(hp); }
+
+// Returns the pair count; *lo/*ro are malloc'ed (free with orc_free).
+size_t orc_equi_join(int ktype, const void* left, const size_t* lidx, size_t nl,
+                     const void* right, const size_t* ridx, size_t nr, int algo,
+                     size_t** lo, size_t** ro, int* used_hash) {
+  std::vector<size_t> a, b;
+  dispatch_key(ktype, [&](auto z) {
+    using K = decltype(z);
+    equi_join_t<K>(left, lidx, nl, right, ridx, nr, algo, a, b, used_hash);
+  });
+  size_t m = a.size();
+  *lo = static_cast<size_t*>(std::malloc((m ? m : 1) * sizeof(size_t)));
+  *ro = static_cast<size_t*>(std::malloc((m ? m : 1) * sizeof(size_t)));
+  if (m) {
+    std::memcpy(*lo, a.data(), m * sizeof(size_t));
+    std::memcpy(*ro, b.data(), m * sizeof(size_t));
+  }
+  return m;
+}
+
+void orc_free(void* p) { std::free(p); }
+
+// split capacity n + 1, keys_out capacity n; returns split length.
+size_t orc_group_by(int ktype, const void* col, size_t* idx, size_t n,
+                    size_t* split, void* keys_out) {
+  size_t r = 0;
+  dispatch_key(ktype, [&](auto z) {
+    using K = decltype(z);
+    r = group_by_t<K>(col, idx, n, split, keys_out);
+  });
+  return r;
+}
+
+int orc_grouped_agg(int op, int vtype, const void* col, const size_t* gidx,
+                    size_t n, const size_t* split, size_t ns,
+                    const size_t* nulls, size_t nn, void* out) {
+  try {
+    dispatch_key(vtype, [&](auto z) {
+      using T = decltype(z);
+      grouped_agg_t<T>(op, col, gidx, n, split, ns, nulls, nn, out);
+    });
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+  return 0;
+}
+
+}  // extern "C"
