@@ -1,0 +1,193 @@
+// Shared internals of libfrovedis_hip: status codes, error text, type
+// codes, the device workspace pool and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/frovedis_hip.h"
+
+namespace fhip {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define FHIP_CHECK(expr)                                                      \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw ::fhip::Error(FHIP_ERR_HIP, std::string(#expr) + ": " +           \
+                                            hipGetErrorString(e_));           \
+  } while (0)
+
+#define FHIP_REQUIRE(cond, code, msg)                                         \
+  do {                                                                        \
+    if (!(cond)) throw ::fhip::Error((code), (msg));                          \
+  } while (0)
+
+// Runs `body` and converts exceptions into status codes (the C ABI never
+// lets a C++ exception escape).
+template <class F>
+int guarded(F&& body) {
+  try {
+    body();
+    return FHIP_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return FHIP_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return FHIP_ERR_INTERNAL;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// type codes (same numbering as the C header)
+// ---------------------------------------------------------------------------
+inline int type_bytes(int t) {
+  switch (t) {
+    case FHIP_I32: case FHIP_U32: case FHIP_F32: return 4;
+    case FHIP_I64: case FHIP_U64: case FHIP_F64: return 8;
+    default: throw Error(FHIP_ERR_TYPE, "unsupported type code " + std::to_string(t));
+  }
+}
+inline bool type_is_signed_int(int t) { return t == FHIP_I32 || t == FHIP_I64; }
+inline bool type_is_float(int t) { return t == FHIP_F32 || t == FHIP_F64; }
+
+// ---------------------------------------------------------------------------
+// device workspace: a grow-only cache of device blocks keyed by size class.
+// Blocks are handed out for the duration of one library call; calls on one
+// device are serialised by the caller (one host thread per rank, as the
+// reference's MPI_THREAD_SERIALIZED ranks are), so stream order keeps reuse
+// safe.
+// ---------------------------------------------------------------------------
+class Pool {
+ public:
+  static Pool& get();
+  void* alloc(size_t bytes);
+  void release(void* p);
+  void trim();  // frees every cached block (device must be idle)
+  size_t cached_bytes() const { return cached_; }
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> live_;
+  size_t cached_ = 0;
+};
+
+// RAII handle over a pool block.
+class DBuf {
+ public:
+  DBuf() = default;
+  explicit DBuf(size_t bytes) { reset(bytes); }
+  ~DBuf() { free(); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { free(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    return *this;
+  }
+  void reset(size_t bytes) {
+    free();
+    if (bytes) { p_ = Pool::get().alloc(bytes); n_ = bytes; }
+  }
+  void free() {
+    if (p_) Pool::get().release(p_);
+    p_ = nullptr; n_ = 0;
+  }
+  void* get() const { return p_; }
+  template <class T> T* as() const { return static_cast<T*>(p_); }
+  size_t size() const { return n_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Pinned host staging (hipHostMalloc), for the host-pointer entry points.
+class HostStage {
+ public:
+  explicit HostStage(size_t bytes);
+  ~HostStage();
+  void* get() const { return p_; }
+ private:
+  void* p_ = nullptr;
+};
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+constexpr int kWave = 64;
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// inclusive wave prefix sum (64 lanes)
+template <class T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const unsigned lane = __lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T y = __shfl_up(x, o, 64);
+    if (lane >= unsigned(o)) x += y;
+  }
+  return x;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// Exclusive prefix over a block of NT threads; `tmp` holds NT/64 entries in
+// LDS.  Returns the exclusive prefix; *total gets the block total.
+template <int NT, class T>
+__device__ __forceinline__ T block_excl_scan(T v, T* tmp, T* total) {
+  const unsigned lane = __lane_id();
+  const unsigned wave = threadIdx.x / 64;
+  T x = wave_incl_scan(v);
+  if (lane == 63) tmp[wave] = x;
+  __syncthreads();
+  T before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; w++) {
+    T t = tmp[w];
+    if (unsigned(w) < wave) before += t;
+    all += t;
+  }
+  __syncthreads();
+  if (total) *total = all;
+  return before + x - v;
+}
+
+inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
+
+// number of workgroups that fill the device for grid-stride kernels
+int fill_blocks(int threads_per_block, int blocks_per_cu = 8);
+
+}  // namespace fhip

@@ -1,0 +1,105 @@
+// Reusable device building blocks: stable index compaction and the
+// single-workgroup scan of per-tile counts.  Header-only so every kernel
+// file can instantiate them with its own predicates.
+#pragma once
+
+#include "fhip_common.hpp"
+
+namespace fhip {
+
+constexpr int CP_NT = 256;
+constexpr int CP_ITEMS = 8;
+constexpr int CP_TILE = CP_NT * CP_ITEMS;
+
+// In-place exclusive scan of `counts[0..nb)`, one workgroup; writes the
+// grand total to *total.
+template <class T>
+__global__ __launch_bounds__(1024) void scan_counts_kernel(T* counts, size_t nb, T* total) {
+  __shared__ T tmp[16];
+  __shared__ T carry_s;
+  if (threadIdx.x == 0) carry_s = 0;
+  __syncthreads();
+  for (size_t base = 0; base < nb; base += 1024) {
+    size_t i = base + threadIdx.x;
+    T v = i < nb ? counts[i] : T(0);
+    T all;
+    T e = block_excl_scan<1024>(v, tmp, &all);
+    T carry = carry_s;
+    if (i < nb) counts[i] = carry + e;
+    __syncthreads();
+    if (threadIdx.x == 0) carry_s = carry + all;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry_s;
+}
+
+template <class Pred>
+__global__ __launch_bounds__(CP_NT) void compact_count_kernel(Pred pred, size_t n,
+                                                              unsigned long long* counts) {
+  const size_t base = size_t(blockIdx.x) * CP_TILE;
+  unsigned c = 0;
+#pragma unroll
+  for (int j = 0; j < CP_ITEMS; j++) {
+    size_t i = base + size_t(j) * CP_NT + threadIdx.x;
+    if (i < n && pred(i)) c++;
+  }
+  c = wave_sum(c);
+  __shared__ unsigned ws[CP_NT / 64];
+  if (__lane_id() == 0) ws[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned t = 0;
+    for (int w = 0; w < CP_NT / 64; w++) t += ws[w];
+    counts[blockIdx.x] = t;
+  }
+}
+
+// Writes map(i) for every i with pred(i), in increasing i.
+template <class Pred, class Map, class OutT>
+__global__ __launch_bounds__(CP_NT) void compact_write_kernel(Pred pred, Map map, size_t n,
+                                                              const unsigned long long* offs,
+                                                              OutT* out) {
+  __shared__ unsigned tmp[CP_NT / 64];
+  const size_t base = size_t(blockIdx.x) * CP_TILE;
+  unsigned long long run = offs[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < CP_ITEMS; j++) {
+    size_t i = base + size_t(j) * CP_NT + threadIdx.x;
+    unsigned f = (i < n && pred(i)) ? 1u : 0u;
+    unsigned all;
+    unsigned e = block_excl_scan<CP_NT>(f, tmp, &all);
+    if (f) out[run + e] = map(i);
+    run += all;
+  }
+}
+
+struct IdentityMap {
+  __device__ size_t operator()(size_t i) const { return i; }
+};
+
+// Stable compaction; returns the number of selected indices (synchronises
+// the stream).  `out` must hold every selected entry.
+template <class Pred, class Map, class OutT>
+size_t compact(Pred pred, Map map, size_t n, OutT* out, hipStream_t st) {
+  if (n == 0) return 0;
+  const size_t nb = ceil_div(n, CP_TILE);
+  DBuf cb((nb + 1) * sizeof(unsigned long long));
+  unsigned long long* counts = cb.as<unsigned long long>();
+  compact_count_kernel<<<unsigned(nb), CP_NT, 0, st>>>(pred, n, counts);
+  FHIP_CHECK(hipGetLastError());
+  scan_counts_kernel<<<1, 1024, 0, st>>>(counts, nb, counts + nb);
+  FHIP_CHECK(hipGetLastError());
+  compact_write_kernel<<<unsigned(nb), CP_NT, 0, st>>>(pred, map, n, counts, out);
+  FHIP_CHECK(hipGetLastError());
+  unsigned long long total = 0;
+  FHIP_CHECK(hipMemcpyAsync(&total, counts + nb, sizeof(total), hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  return size_t(total);
+}
+
+template <class Pred>
+size_t compact_indices(Pred pred, size_t n, size_t* out, hipStream_t st) {
+  return compact(pred, IdentityMap{}, n, out, st);
+}
+
+}  // namespace fhip

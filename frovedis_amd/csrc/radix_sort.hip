@@ -1,0 +1,541 @@
+// Stable LSD radix sort for gfx950 (one rank's share of
+// core/radix_sort.hpp:197-242 and core/radix_sort.incl:7-240).
+//
+// Design (HBM-bound, no MFMA):
+//  1. rs_hist: one read of the keys builds the 256-bin digit histogram of
+//     every byte position at once (LDS counters, wave-uniform fast path).
+//  2. rs_scan: exclusive scan of each histogram -> global digit bases.
+//  3. rs_onesweep, once per non-trivial byte: a workgroup takes the next tile
+//     (4096 keys) in arrival order, ranks its keys stably with wave64 ballot
+//     matching, publishes per-digit tile counts, finds its global offsets by
+//     decoupled look-back over earlier tiles, reorders the tile in LDS by
+//     digit and writes keys, then payloads, in digit runs (coalesced).
+// Passes whose digit is the same for every key are skipped; that covers the
+// reference's "bytes of the maximum key" rule (get_max_key_size,
+// radix_sort.hpp:135-153) without a separate max pass.
+#include "radix_sort.hpp"
+
+namespace fhip {
+namespace {
+
+constexpr int RS_NT = 256;
+constexpr int RS_WAVES = RS_NT / 64;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_NT * RS_ITEMS;
+constexpr unsigned long long LB_FLAG_A = 1ull << 62;  // tile aggregate only
+constexpr unsigned long long LB_FLAG_P = 2ull << 62;  // inclusive prefix
+constexpr unsigned long long LB_VALUE = (1ull << 62) - 1;
+
+template <int B> struct UInt;
+template <> struct UInt<4> { using T = uint32_t; };
+template <> struct UInt<8> { using T = uint64_t; };
+
+template <class U>
+__device__ __forceinline__ U key_xform(U k, int mode) {
+  constexpr int SH = int(sizeof(U) * 8 - 1);
+  const U top = U(1) << SH;
+  if (mode == KM_FLIPTOP) return k ^ top;
+  if (mode == KM_FLOATFLIP) return k ^ ((U(0) - (k >> SH)) | top);
+  return k;
+}
+
+template <class U>
+__device__ __forceinline__ unsigned key_digit(U k, int mode, int shift, int desc) {
+  unsigned d = unsigned(key_xform(k, mode) >> shift) & 0xFFu;
+  return desc ? 255u - d : d;
+}
+
+__device__ __forceinline__ unsigned long long lb_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- key statistics: max (init 0, as get_max_key_size) and "any < 0" ------
+template <class K>
+__global__ __launch_bounds__(256) void rs_stats(const K* __restrict__ keys, size_t n,
+                                                unsigned long long* __restrict__ out) {
+  using U = typename UInt<sizeof(K)>::T;
+  unsigned long long mx = 0;
+  int neg = 0;
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    K k = keys[i];
+    if (k < K(0)) neg = 1;
+    unsigned long long m;
+    if constexpr (std::is_floating_point<K>::value) {
+      U b;
+      __builtin_memcpy(&b, &k, sizeof(U));
+      m = b;
+    } else {
+      m = k > K(0) ? (unsigned long long)k : 0ull;
+    }
+    mx = m > mx ? m : mx;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long y = __shfl_xor(mx, o, 64);
+    mx = y > mx ? y : mx;
+  }
+  unsigned long long anyneg = __ballot(neg);
+  if (__lane_id() == 0) {
+    if (mx) atomicMax(&out[0], mx);
+    if (anyneg) atomicOr(&out[1], 1ull);
+  }
+}
+
+// ---- all-digit histograms in one read -------------------------------------
+template <class U>
+__global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, size_t n,
+                                                 int mode, int desc, int nbytes,
+                                                 unsigned long long* __restrict__ ghist) {
+  __shared__ unsigned h[sizeof(U) * 256];
+  for (int i = threadIdx.x; i < int(sizeof(U)) * 256; i += RS_NT) h[i] = 0;
+  __syncthreads();
+  const unsigned lane = __lane_id();
+  const size_t stride = size_t(gridDim.x) * RS_NT;
+  for (size_t base = size_t(blockIdx.x) * RS_NT + (threadIdx.x & ~63u); base < n;
+       base += stride) {
+    const size_t i = base + lane;
+    const bool valid = i < n;
+    const U k = valid ? keys[i] : U(0);
+    const unsigned long long vmask = __ballot(valid);
+    for (int b = 0; b < nbytes; b++) {
+      unsigned d = key_digit(k, mode, 8 * b, desc);
+      unsigned d0 = __shfl(d, 0, 64);  // lane 0 is always valid here
+      unsigned long long same = __ballot(valid && d == d0);
+      if (same == vmask) {
+        if (lane == 0) atomicAdd(&h[b * 256 + d0], unsigned(__popcll(vmask)));
+      } else if (valid) {
+        atomicAdd(&h[b * 256 + d], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nbytes * 256; i += RS_NT) {
+    unsigned v = h[i];
+    if (v) atomicAdd(&ghist[i], (unsigned long long)v);
+  }
+}
+
+// exclusive scan of each byte's histogram -> global digit bases
+__global__ __launch_bounds__(256) void rs_scan(const unsigned long long* __restrict__ ghist,
+                                               unsigned long long* __restrict__ gbase) {
+  __shared__ unsigned long long tmp[4];
+  const int b = blockIdx.x;
+  unsigned long long v = ghist[b * 256 + threadIdx.x];
+  unsigned long long e = block_excl_scan<256>(v, tmp, (unsigned long long*)nullptr);
+  gbase[b * 256 + threadIdx.x] = e;
+}
+
+// ---- one onesweep pass -------------------------------------------------------
+template <class U, int VB>
+struct PassLds {
+  static constexpr int EB = (VB > int(sizeof(U))) ? VB : int(sizeof(U));
+  unsigned wcnt[RS_WAVES][256];   // per-wave digit counts -> wave offsets
+  unsigned tstart[256];           // tile-local start of each digit
+  unsigned long long gdst[256];   // global position of tile-local index 0
+  unsigned scan_tmp[RS_WAVES];
+  unsigned tile;
+  alignas(16) unsigned char buf[RS_TILE * EB];
+};
+
+template <class U, int VB>
+__global__ __launch_bounds__(RS_NT) void rs_onesweep(
+    const U* __restrict__ ksrc, U* __restrict__ kdst, const void* __restrict__ vsrc_,
+    void* __restrict__ vdst_, size_t n, int shift, int mode, int desc,
+    const unsigned long long* __restrict__ gbase, unsigned long long* lookback,
+    unsigned long long* lookback_next, unsigned* tile_counter,
+    unsigned* tile_counter_next) {
+  using V = typename UInt<(VB ? VB : 4)>::T;
+  __shared__ PassLds<U, VB> s;
+  const unsigned tid = threadIdx.x;
+  const unsigned lane = __lane_id();
+  const unsigned wave = tid >> 6;
+
+  if (tid == 0) s.tile = atomicAdd(tile_counter, 1u);
+  for (unsigned i = tid; i < RS_WAVES * 256; i += RS_NT) (&s.wcnt[0][0])[i] = 0;
+  __syncthreads();
+  const unsigned tile = s.tile;
+  // clear this tile's look-back slots of the next pass (double buffering)
+  lookback_next[size_t(tile) * 256 + tid] = 0;
+  if (tile == 0 && tid == 0) *tile_counter_next = 0;
+
+  const size_t tile_base = size_t(tile) * RS_TILE;
+  const size_t wave_base = tile_base + size_t(wave) * 64 * RS_ITEMS;
+
+  U keys[RS_ITEMS];
+  V vals[RS_ITEMS];
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; j++) {
+    size_t idx = wave_base + size_t(j) * 64 + lane;
+    keys[j] = idx < n ? ksrc[idx] : U(0);
+  }
+  if constexpr (VB != 0) {
+    const V* vsrc = static_cast<const V*>(vsrc_);
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+      size_t idx = wave_base + size_t(j) * 64 + lane;
+      vals[j] = idx < n ? vsrc[idx] : V(0);
+    }
+  }
+
+  // stable in-wave ranking: lanes with equal digits are matched with 8
+  // ballots; ranks count earlier items first, then lower lanes.
+  unsigned rank[RS_ITEMS];
+  unsigned digit[RS_ITEMS];
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; j++) {
+    size_t idx = wave_base + size_t(j) * 64 + lane;
+    const bool valid = idx < n;
+    unsigned d = key_digit(keys[j], mode, shift, desc);
+    unsigned long long m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1u;
+      unsigned long long bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    digit[j] = d;
+    rank[j] = 0;
+    if (valid) {
+      unsigned cnt = unsigned(__popcll(m));
+      unsigned below = unsigned(__popcll(m & lt_mask));
+      unsigned prior = s.wcnt[wave][d];
+      rank[j] = prior + below;
+      if (below == cnt - 1) s.wcnt[wave][d] = prior + cnt;
+    }
+  }
+  __syncthreads();
+
+  // per-digit totals and wave offsets (thread t owns digit t)
+  unsigned total = 0;
+#pragma unroll
+  for (int w = 0; w < RS_WAVES; w++) {
+    unsigned c = s.wcnt[w][tid];
+    s.wcnt[w][tid] = total;
+    total += c;
+  }
+  unsigned long long* my_lb = lookback + size_t(tile) * 256 + tid;
+  lb_store(my_lb, (tile == 0 ? LB_FLAG_P : LB_FLAG_A) | (unsigned long long)total);
+  unsigned tstart = block_excl_scan<RS_NT>(total, s.scan_tmp, (unsigned*)nullptr);
+  s.tstart[tid] = tstart;
+
+  // decoupled look-back over earlier tiles for this digit
+  unsigned long long excl = 0;
+  if (tile > 0) {
+    long long p = (long long)tile - 1;
+    while (true) {
+      unsigned long long v = lb_load(lookback + size_t(p) * 256 + tid);
+      unsigned long long flag = v & ~LB_VALUE;
+      if (flag == 0) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += v & LB_VALUE;
+      if (flag == LB_FLAG_P) break;
+      --p;
+    }
+    lb_store(my_lb, LB_FLAG_P | (excl + total));
+  }
+  s.gdst[tid] = gbase[tid] + excl - tstart;
+  __syncthreads();
+
+  // reorder keys by digit in LDS
+  U* kbuf = reinterpret_cast<U*>(s.buf);
+  unsigned pos[RS_ITEMS];
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; j++) {
+    size_t idx = wave_base + size_t(j) * 64 + lane;
+    pos[j] = s.tstart[digit[j]] + s.wcnt[wave][digit[j]] + rank[j];
+    if (idx < n) kbuf[pos[j]] = keys[j];
+  }
+  __syncthreads();
+  const unsigned tile_n = unsigned((n - tile_base) < size_t(RS_TILE) ? (n - tile_base) : size_t(RS_TILE));
+  unsigned long long dst[RS_ITEMS];
+#pragma unroll
+  for (int j = 0; j < RS_ITEMS; j++) {
+    unsigned i = unsigned(j) * RS_NT + tid;
+    dst[j] = 0;
+    if (i < tile_n) {
+      U k = kbuf[i];
+      unsigned d = key_digit(k, mode, shift, desc);
+      dst[j] = s.gdst[d] + i;
+      kdst[dst[j]] = k;
+    }
+  }
+  if constexpr (VB != 0) {
+    V* vbuf = reinterpret_cast<V*>(s.buf);
+    V* vdst = static_cast<V*>(vdst_);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+      size_t idx = wave_base + size_t(j) * 64 + lane;
+      if (idx < n) vbuf[pos[j]] = vals[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; j++) {
+      unsigned i = unsigned(j) * RS_NT + tid;
+      if (i < tile_n) vdst[dst[j]] = vbuf[i];
+    }
+  }
+}
+
+// ---- n < 64: the reference's insertion sort (radix_sort.hpp:31-99) -----------
+template <class K, int VB>
+__global__ void rs_insertion(K* __restrict__ k, void* __restrict__ v_, int n, int desc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  using V = typename UInt<(VB ? VB : 4)>::T;
+  V* v = static_cast<V*>(v_);
+  for (int i = 1; i < n; i++) {
+    K tmp = k[i];
+    bool ooo = desc ? (k[i - 1] < tmp) : (k[i - 1] > tmp);
+    if (!ooo) continue;
+    V vtmp = VB ? v[i] : V(0);
+    int j = i;
+    do {
+      k[j] = k[j - 1];
+      if (VB) v[j] = v[j - 1];
+      j--;
+    } while (j > 0 && (desc ? (k[j - 1] < tmp) : (k[j - 1] > tmp)));
+    k[j] = tmp;
+    if (VB) v[j] = vtmp;
+  }
+}
+
+template <class K>
+void launch_insertion(void* k, int vbytes, void* v, size_t n, bool desc, hipStream_t st) {
+  if (vbytes == 0) rs_insertion<K, 0><<<1, 64, 0, st>>>((K*)k, v, int(n), desc);
+  else if (vbytes == 4) rs_insertion<K, 4><<<1, 64, 0, st>>>((K*)k, v, int(n), desc);
+  else rs_insertion<K, 8><<<1, 64, 0, st>>>((K*)k, v, int(n), desc);
+  FHIP_CHECK(hipGetLastError());
+}
+
+template <class K>
+void launch_stats(const void* keys, size_t n, unsigned long long* out, hipStream_t st) {
+  int grid = int(std::min<size_t>(ceil_div(n, 256), size_t(fill_blocks(256, 4))));
+  rs_stats<K><<<grid, 256, 0, st>>>((const K*)keys, n, out);
+  FHIP_CHECK(hipGetLastError());
+}
+
+template <class U, int VB>
+void launch_pass(const void* ks, void* kd, const void* vs, void* vd, size_t n, int shift,
+                 int mode, int desc, const unsigned long long* gbase,
+                 unsigned long long* lb, unsigned long long* lb_next, unsigned* ctr,
+                 unsigned* ctr_next, unsigned tiles, hipStream_t st) {
+  rs_onesweep<U, VB><<<tiles, RS_NT, 0, st>>>((const U*)ks, (U*)kd, vs, vd, n, shift, mode,
+                                              desc, gbase, lb, lb_next, ctr, ctr_next);
+  FHIP_CHECK(hipGetLastError());
+}
+
+using PassFn = void (*)(const void*, void*, const void*, void*, size_t, int, int, int,
+                        const unsigned long long*, unsigned long long*, unsigned long long*,
+                        unsigned*, unsigned*, unsigned, hipStream_t);
+
+PassFn pick_pass(int kbytes, int vbytes) {
+  if (kbytes == 4) {
+    if (vbytes == 0) return launch_pass<uint32_t, 0>;
+    if (vbytes == 4) return launch_pass<uint32_t, 4>;
+    return launch_pass<uint32_t, 8>;
+  }
+  if (vbytes == 0) return launch_pass<uint64_t, 0>;
+  if (vbytes == 4) return launch_pass<uint64_t, 4>;
+  return launch_pass<uint64_t, 8>;
+}
+
+size_t bytes_of_max(unsigned long long mx, int kbytes) {
+  // get_max_key_size, radix_sort.hpp:135-153
+  unsigned long long tmp = 0xFF;
+  for (int i = 1; i <= kbytes; i++) {
+    if (mx <= tmp) return size_t(i);
+    tmp = (tmp << 8) | 0xFF;
+  }
+  return size_t(kbytes);
+}
+
+// Core driver: histogram, pass selection and the pass schedule.
+void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* kout,
+             int vbytes, const void* vin, void* vout, size_t n, hipStream_t st) {
+  DBuf hist(size_t(2) * 8 * 256 * sizeof(unsigned long long));
+  unsigned long long* ghist = hist.as<unsigned long long>();
+  unsigned long long* gbase = ghist + 8 * 256;
+  FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
+  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * 4), size_t(fill_blocks(RS_NT, 4))));
+  if (kbytes == 4)
+    rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist);
+  else
+    rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, mode, desc, nbytes, ghist);
+  FHIP_CHECK(hipGetLastError());
+  rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
+  FHIP_CHECK(hipGetLastError());
+  std::vector<unsigned long long> h(size_t(nbytes) * 256);
+  FHIP_CHECK(hipMemcpyAsync(h.data(), ghist, h.size() * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+
+  std::vector<int> passes;
+  for (int b = 0; b < nbytes; b++) {
+    unsigned long long mx = 0;
+    for (int d = 0; d < 256; d++) mx = std::max(mx, h[size_t(b) * 256 + d]);
+    if (mx < n) passes.push_back(b);  // a byte shared by every key is a no-op
+  }
+  const size_t kbn = n * size_t(kbytes), vbn = n * size_t(vbytes);
+  if (passes.empty()) {
+    if (kout != kin) {
+      FHIP_CHECK(hipMemcpyAsync(kout, kin, kbn, hipMemcpyDeviceToDevice, st));
+      if (vbytes) FHIP_CHECK(hipMemcpyAsync(vout, vin, vbn, hipMemcpyDeviceToDevice, st));
+    }
+    return;
+  }
+  const int P = int(passes.size());
+  const bool inplace = (kout == kin);
+  // destinations so that the last pass lands in (kout, vout) and no pass
+  // reads and writes the same buffer
+  DBuf t1k(kbn), t1v(vbn), t2k, t2v;
+  std::vector<int> dest(P);  // 0 = out, 1 = t1, 2 = t2
+  bool copy_back = false;
+  if (!inplace) {
+    for (int i = 0; i < P; i++) dest[i] = ((P - 1 - i) % 2 == 0) ? 0 : 1;
+  } else if (P == 1) {
+    dest[0] = 1;
+    copy_back = true;
+  } else if (P % 2 == 0) {
+    for (int i = 0; i < P; i++) dest[i] = (i % 2 == 0) ? 1 : 0;
+  } else {
+    t2k.reset(kbn);
+    t2v.reset(vbn);
+    dest[0] = 1;
+    dest[1] = 2;
+    for (int i = 2; i < P; i++) dest[i] = (i % 2 == 0) ? 0 : 1;
+  }
+  auto kbuf = [&](int d) -> void* { return d == 0 ? kout : d == 1 ? t1k.get() : t2k.get(); };
+  auto vbuf = [&](int d) -> void* { return d == 0 ? vout : d == 1 ? t1v.get() : t2v.get(); };
+
+  const unsigned tiles = unsigned(ceil_div(n, RS_TILE));
+  FHIP_REQUIRE(ceil_div(n, RS_TILE) < (1ull << 31), FHIP_ERR_TOO_LARGE, "radix_sort: too many keys");
+  const size_t lb_words = size_t(tiles) * 256;
+  DBuf lbuf(2 * lb_words * sizeof(unsigned long long) + 256);
+  unsigned long long* lb0 = lbuf.as<unsigned long long>();
+  unsigned long long* lb1 = lb0 + lb_words;
+  unsigned* ctr = reinterpret_cast<unsigned*>(lb1 + lb_words);
+  FHIP_CHECK(hipMemsetAsync(lb0, 0, lb_words * sizeof(unsigned long long), st));
+  FHIP_CHECK(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), st));
+
+  PassFn fn = pick_pass(kbytes, vbytes);
+  const void* ks = kin;
+  const void* vs = vin;
+  for (int i = 0; i < P; i++) {
+    void* kd = kbuf(dest[i]);
+    void* vd = vbuf(dest[i]);
+    unsigned long long* lb = (i & 1) ? lb1 : lb0;
+    unsigned long long* lbn = (i & 1) ? lb0 : lb1;
+    fn(ks, kd, vs, vd, n, 8 * passes[i], mode, desc, gbase + size_t(passes[i]) * 256, lb, lbn,
+       ctr + (i & 1), ctr + ((i + 1) & 1), tiles, st);
+    ks = kd;
+    vs = vd;
+  }
+  if (copy_back) {
+    FHIP_CHECK(hipMemcpyAsync(kout, t1k.get(), kbn, hipMemcpyDeviceToDevice, st));
+    if (vbytes) FHIP_CHECK(hipMemcpyAsync(vout, t1v.get(), vbn, hipMemcpyDeviceToDevice, st));
+  }
+  // temporaries go back to the pool; later work on this stream is ordered
+}
+
+}  // namespace
+
+void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* vin,
+                void* vout, size_t n, bool positive_only, bool descending, hipStream_t st) {
+  const int kb = type_bytes(ktype);
+  FHIP_REQUIRE(vbytes == 0 || vbytes == 4 || vbytes == 8, FHIP_ERR_ARG,
+               "radix_sort: payload must be 0, 4 or 8 bytes");
+  if (n == 0) return;
+  FHIP_REQUIRE(vbytes == 0 || (vin && vout), FHIP_ERR_ARG, "radix_sort: null payload");
+  FHIP_REQUIRE(kin && kout, FHIP_ERR_ARG, "radix_sort: null keys");
+  if (n < 64) {  // SWITCH_INSERTION_THR on x86 (radix_sort.hpp:17)
+    if (kout != kin) {
+      FHIP_CHECK(hipMemcpyAsync(kout, kin, n * kb, hipMemcpyDeviceToDevice, st));
+      if (vbytes) FHIP_CHECK(hipMemcpyAsync(vout, vin, n * vbytes, hipMemcpyDeviceToDevice, st));
+    }
+    switch (ktype) {
+      case FHIP_I32: launch_insertion<int32_t>(kout, vbytes, vout, n, descending, st); break;
+      case FHIP_U32: launch_insertion<uint32_t>(kout, vbytes, vout, n, descending, st); break;
+      case FHIP_I64: launch_insertion<int64_t>(kout, vbytes, vout, n, descending, st); break;
+      case FHIP_U64: launch_insertion<uint64_t>(kout, vbytes, vout, n, descending, st); break;
+      case FHIP_F32: launch_insertion<float>(kout, vbytes, vout, n, descending, st); break;
+      default: launch_insertion<double>(kout, vbytes, vout, n, descending, st); break;
+    }
+    return;
+  }
+  int mode = KM_RAW;
+  int nbytes = kb;
+  const bool need_stats =
+      type_is_float(ktype) ? !positive_only : (type_is_signed_int(ktype) && positive_only);
+  if (need_stats) {
+    DBuf sbuf(2 * sizeof(unsigned long long));
+    unsigned long long* sdev = sbuf.as<unsigned long long>();
+    FHIP_CHECK(hipMemsetAsync(sdev, 0, 2 * sizeof(unsigned long long), st));
+    switch (ktype) {
+      case FHIP_I32: launch_stats<int32_t>(kin, n, sdev, st); break;
+      case FHIP_I64: launch_stats<int64_t>(kin, n, sdev, st); break;
+      case FHIP_F32: launch_stats<float>(kin, n, sdev, st); break;
+      default: launch_stats<double>(kin, n, sdev, st); break;
+    }
+    unsigned long long hs[2];
+    FHIP_CHECK(hipMemcpyAsync(hs, sdev, sizeof(hs), hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    if (type_is_float(ktype)) {
+      mode = hs[1] ? KM_FLOATFLIP : KM_RAW;
+    } else {
+      mode = KM_RAW;  // positive_only: raw bits of the low bytes only
+      nbytes = int(bytes_of_max(hs[0], kb));
+    }
+  } else if (type_is_signed_int(ktype)) {
+    mode = KM_FLIPTOP;
+  }
+  run_lsd(kb, mode, descending ? 1 : 0, nbytes, kin, kout, vbytes, vin, vout, n, st);
+}
+
+void sort_pairs_bits(int kbytes, const void* kin, void* kout, int vbytes, const void* vin,
+                     void* vout, size_t n, int key_bits, hipStream_t st) {
+  if (n == 0) return;
+  int nbytes = std::max(1, std::min(kbytes, (key_bits + 7) / 8));
+  run_lsd(kbytes, KM_RAW, 0, nbytes, kin, kout, vbytes, vin, vout, n, st);
+}
+
+}  // namespace fhip
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int fhip_dev_radix_sort(int ktype, void* keys, int vbytes, void* vals, size_t n,
+                        int positive_only, int descending, void* stream) {
+  return fhip::guarded([&] {
+    fhip::sort_pairs(ktype, keys, keys, vbytes, vals, vals, n, positive_only != 0,
+                     descending != 0, fhip::as_stream(stream));
+  });
+}
+
+int fhip_radix_sort(int ktype, void* keys, int vbytes, void* vals, size_t n,
+                    int positive_only, int descending) {
+  return fhip::guarded([&] {
+    if (n == 0) return;
+    const size_t kb = size_t(fhip::type_bytes(ktype)) * n;
+    const size_t vb = size_t(vbytes) * n;
+    hipStream_t st = nullptr;
+    fhip::DBuf dk(kb), dv(vb);
+    FHIP_CHECK(hipMemcpyAsync(dk.get(), keys, kb, hipMemcpyHostToDevice, st));
+    if (vb) FHIP_CHECK(hipMemcpyAsync(dv.get(), vals, vb, hipMemcpyHostToDevice, st));
+    fhip::sort_pairs(ktype, dk.get(), dk.get(), vbytes, dv.get(), dv.get(), n,
+                     positive_only != 0, descending != 0, st);
+    FHIP_CHECK(hipMemcpyAsync(keys, dk.get(), kb, hipMemcpyDeviceToHost, st));
+    if (vb) FHIP_CHECK(hipMemcpyAsync(vals, dv.get(), vb, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

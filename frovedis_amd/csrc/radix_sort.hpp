@@ -1,0 +1,39 @@
+// Internal interface of the gfx950 radix sort, shared by group_by, the
+// joins, the hash table conflict list and the batched/learned sorts.
+#pragma once
+
+#include "fhip_common.hpp"
+
+namespace fhip {
+
+// How the key bits are mapped to an unsigned order-preserving value before
+// digits are extracted (the reference's flips, core/radix_sort.hpp:155-195).
+enum KeyMode : int {
+  KM_RAW = 0,        // unsigned keys, or positive_only, or floats w/o negatives
+  KM_FLIPTOP = 1,    // signed ints: xor the sign bit
+  KM_FLOATFLIP = 2,  // IEEE keys with a negative: total-order flip
+};
+
+struct SortPlan {
+  int kbytes = 8;
+  int mode = KM_RAW;
+  int desc = 0;
+  std::vector<int> passes;  // byte positions to run, ascending
+};
+
+// Stable sort of n keys (+ optional payload of vbytes bytes) from
+// (kin, vin) into (kout, vout).  kout may equal kin (in place).  Follows the
+// reference semantics of radix_sort(K*, V*, size_t, bool) including the
+// positive_only rule and the < 64 insertion path.
+void sort_pairs(int ktype, const void* kin, void* kout, int vbytes,
+                const void* vin, void* vout, size_t n, bool positive_only,
+                bool descending, hipStream_t stream);
+
+// Plain stable ascending sort of unsigned 64-bit (or 32-bit) keys that only
+// uses the bytes in [0, nbytes); for internal callers that already know the
+// key range.  No insertion path, no flips.
+void sort_pairs_bits(int kbytes, const void* kin, void* kout, int vbytes,
+                     const void* vin, void* vout, size_t n, int key_bits,
+                     hipStream_t stream);
+
+}  // namespace fhip

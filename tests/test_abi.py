@@ -1,0 +1,51 @@
+"""CPU-side checks of the C ABI: the library builds, loads without a GPU and
+exports every entry point declared in include/frovedis_hip.h.  No compute
+calls are made here."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "frovedis_hip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fhip_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def libpath():
+    from frovedis_amd import build as b
+    if not os.path.exists(b.LIB):
+        b.build()
+    return b.LIB
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "fhip_dev_radix_sort" in names and "fhip_last_error" in names
+    assert len(names) >= 10
+
+
+def test_library_exports_every_declared_symbol(libpath):
+    L = ctypes.CDLL(libpath)
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+def test_last_error_is_empty_string_initially(libpath):
+    L = ctypes.CDLL(libpath)
+    L.fhip_last_error.restype = ctypes.c_char_p
+    assert L.fhip_last_error() == b""
+
+
+def test_package_imports_and_fails_loudly_without_library(tmp_path, monkeypatch):
+    import frovedis_amd
+    monkeypatch.setattr(frovedis_amd, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(frovedis_amd, "_lib", None)
+    with pytest.raises(frovedis_amd.FhipError):
+        frovedis_amd.lib()
