@@ -1,0 +1,201 @@
+"""Benchmark: dftable group_by + sum/count/mean/max on one MI355X.
+
+Workload (one "step"): 2^28 rows already resident in HBM -- int64 key
+Zipf(s=1.1) over 2^20 values, double value N(0,1), int32 column U[0,1000) --
+grouped by key with sum/count/avg of the double column and max of the int32
+column, groups ascending by key (dftable.cc:346-366,3724-3772 on one rank).
+Prints one JSON line.  Side measurements (the plain radix sort key/value
+configuration) go under "extra".
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows LOG2] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+GROUPBY_BYTES_PER_ROW = 20  # read key 8 + value 8 + int32 4, once
+
+
+def _prof_ms(fa, name):
+    t = ctypes.c_double(0)
+    c = ctypes.c_size_t(0)
+    fa.check(fa.lib().fhip_prof_query(name.encode(), ctypes.byref(t), ctypes.byref(c)))
+    return t.value, c.value
+
+
+def _traffic_from_profiles(name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{name}_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def gen_groupby(n, device="cuda"):
+    import numpy as np
+
+    import frovedis_amd as fa
+    from frovedis_amd import datagen as dg
+    k = fa.DeviceArray(dg.zipf_keys_dev(n, device=device), np.int64)
+    v = fa.DeviceArray(dg.normal_dev(n, device=device), np.float64)
+    iv = fa.DeviceArray(dg.int_uniform_dev(n, 1000, device=device), np.int32)
+    return k, v, iv
+
+
+def step_groupby(fa, k, v, iv):
+    return fa.group_by_agg(k, [("sum", v), ("count", v), ("avg", v), ("max", iv)],
+                           table_hint=1 << 20)
+
+
+def cpu_baseline_groupby(sample_log2=24, reps=3):
+    """The oracle's restatement of the reference's per-rank group_by +
+    sum/count/avg/max lambdas (sort based, x86 fold), one core, on the first
+    2^sample_log2 rows of the same generators."""
+    import numpy as np
+
+    import oracle
+    from frovedis_amd import datagen as dg
+    n = 1 << sample_log2
+    k = dg.zipf_keys_np(n)
+    v = dg.normal_np(n)
+    iv = dg.int_uniform_np(n, 1000)
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        gi, split, keys = oracle.group_by(k)
+        oracle.grouped_agg("sum", v, gi, split)
+        oracle.grouped_agg("count", v, gi, split)
+        oracle.grouped_agg("avg", v, gi, split)
+        oracle.grouped_agg("max", iv, gi, split)
+        best = min(best, time.perf_counter() - t0)
+    return {"value": n / best, "unit": "rows/s", "cores": 1, "kind": "port",
+            "sample": f"first 2^{sample_log2} rows of the same generators, best of {reps}"}
+
+
+def bench_radix(fa, log2n, steps=3):
+    """Plain key/value radix sort: uint64 keys over the full 64-bit range,
+    size_t index payload (the reference's radix_sort(K*, V*, n) case)."""
+    import numpy as np
+    import torch
+
+    from frovedis_amd import datagen as dg
+    n = 1 << log2n
+    k0 = dg.radix_keys_dev(n)
+    v0 = torch.arange(n, dtype=torch.int64, device="cuda")
+    keys = fa.DeviceArray(k0.clone(), np.uint64)
+    vals = fa.DeviceArray(v0.clone(), np.uint64)
+    fa.radix_sort(keys, vals)
+    times = []
+    fa.check(fa.lib().fhip_prof_enable(1))
+    for _ in range(steps):
+        keys.t.copy_(k0)
+        vals.t.copy_(v0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fa.radix_sort(keys, vals)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    pass_ms, passes = _prof_ms(fa, "radix_pass")
+    hist_ms, _ = _prof_ms(fa, "radix_hist")
+    fa.check(fa.lib().fhip_prof_enable(0))
+    t = min(times)
+    P = passes // steps if steps else 0
+    algo = n * (8 + 32 * P)
+    kern_s = (pass_ms + hist_ms) / 1e3 / steps
+    return {"keys": n, "keys_per_s": n / t, "ms": t * 1e3, "passes": P, "digit_bits": 8,
+            "algorithmic_bytes": algo, "kernel_ms": kern_s * 1e3,
+            "hbm_frac": algo / kern_s / HBM_PEAK if kern_s else None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=28, help="log2 of the row count")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the side measurements")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU's share only (--gpus {args.gpus} requested); "
+              "the multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import frovedis_amd as fa
+    torch.cuda.set_device(0)
+    fa.lib()
+    n = 1 << args.rows
+    print(f"# generating {n} rows", file=sys.stderr, flush=True)
+    k, v, iv = gen_groupby(n)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        step_groupby(fa, k, v, iv)
+    torch.cuda.synchronize()
+    fa.check(fa.lib().fhip_prof_enable(1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gk, outs = step_groupby(fa, k, v, iv)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ha_ms, ha_launches = _prof_ms(fa, "hashagg")
+    fa.check(fa.lib().fhip_prof_enable(0))
+    ms_per_step = dt * 1e3 / args.steps
+    groups = len(gk)
+    print(f"# {groups} groups, {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    kern_s = ha_ms / 1e3 / max(ha_launches, 1)
+    algo_bytes = GROUPBY_BYTES_PER_ROW * n
+    achieved = algo_bytes / kern_s if kern_s else 0.0
+    traffic = _traffic_from_profiles("hashagg")
+    result = {
+        "metric": "group-by rows/s (sum/count/mean/max, Zipf keys)",
+        "value": n / (ms_per_step / 1e3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 on device)",
+        "config": {"workload": f"dftable group_by + sum/count/avg/max, 2^{args.rows} rows, "
+                               "int64 key Zipf(1.1) over 2^20, double N(0,1), int32 U[0,1000)",
+                   "rows": n, "groups": groups, "path": "fused hash group-by"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic, "kernel": "hashagg_kernel",
+                     "kernel_ms": kern_s * 1e3,
+                     "algorithmic_bytes_per_launch": algo_bytes},
+        "cpu_baseline": None,
+        "extra": {},
+    }
+    del k, v, iv, gk, outs
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        print("# radix sort side measurements", file=sys.stderr, flush=True)
+        result["extra"]["radix_kv_u64_2^24"] = bench_radix(fa, 24)
+        result["extra"]["radix_kv_u64_2^28"] = bench_radix(fa, 28)
+        torch.cuda.empty_cache()
+    if not args.no_cpu:
+        print("# cpu baseline (oracle, one core)", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline_groupby()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -187,6 +187,34 @@ __device__ __forceinline__ T block_excl_scan(T v, T* tmp, T* total) {
 
 inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
 
+// ---------------------------------------------------------------------------
+// kernel timing (off by default): HIP events recorded on the launch stream
+// around a named kernel; read back through fhip_prof_query.
+// ---------------------------------------------------------------------------
+bool prof_enabled();
+void prof_record(const char* name, hipEvent_t start, hipEvent_t stop);
+
+class ProfScope {
+ public:
+  ProfScope(const char* name, hipStream_t st) : name_(name), st_(st) {
+    if (!prof_enabled()) return;
+    FHIP_CHECK(hipEventCreate(&a_));
+    FHIP_CHECK(hipEventCreate(&b_));
+    FHIP_CHECK(hipEventRecord(a_, st_));
+  }
+  ~ProfScope() {
+    if (!a_) return;
+    if (hipEventRecord(b_, st_) == hipSuccess) prof_record(name_, a_, b_);
+  }
+  ProfScope(const ProfScope&) = delete;
+  ProfScope& operator=(const ProfScope&) = delete;
+
+ private:
+  const char* name_;
+  hipStream_t st_;
+  hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
 // number of workgroups that fill the device for grid-stride kernels
 int fill_blocks(int threads_per_block, int blocks_per_cu = 8);
 

@@ -364,6 +364,7 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   unsigned long long* gbase = ghist + 8 * 256;
   FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
   int grid = int(std::min<size_t>(ceil_div(n, RS_NT * 4), size_t(fill_blocks(RS_NT, 4))));
+  ProfScope prof("radix_hist", st);
   if (kbytes == 4)
     rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist);
   else
@@ -392,6 +393,8 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   }
   const int P = int(passes.size());
   const bool inplace = (kout == kin);
+  FHIP_REQUIRE(vbytes == 0 || inplace == (vout == vin), FHIP_ERR_ARG,
+               "radix_sort: keys and payload must both be in place or both out of place");
   // destinations so that the last pass lands in (kout, vout) and no pass
   // reads and writes the same buffer
   DBuf t1k(kbn), t1v(vbn), t2k, t2v;
@@ -432,6 +435,7 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
     void* vd = vbuf(dest[i]);
     unsigned long long* lb = (i & 1) ? lb1 : lb0;
     unsigned long long* lbn = (i & 1) ? lb0 : lb1;
+    ProfScope prof("radix_pass", st);
     fn(ks, kd, vs, vd, n, 8 * passes[i], mode, desc, gbase + size_t(passes[i]) * 256, lb, lbn,
        ctr + (i & 1), ctr + ((i + 1) & 1), tiles, st);
     ks = kd;

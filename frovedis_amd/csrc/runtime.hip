@@ -70,6 +70,31 @@ HostStage::~HostStage() {
   if (p_) (void)hipHostFree(p_);
 }
 
+namespace {
+struct ProfEntry {
+  std::string name;
+  hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfEntry> g_prof;
+}  // namespace
+
+bool prof_enabled() { return g_prof_on; }
+
+void prof_record(const char* name, hipEvent_t a, hipEvent_t b) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof.push_back(ProfEntry{name, a, b});
+}
+
+static void prof_clear_locked() {
+  for (auto& e : g_prof) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  g_prof.clear();
+}
+
 int fill_blocks(int threads_per_block, int blocks_per_cu) {
   static int cus = 0;
   if (!cus) {
@@ -99,6 +124,33 @@ int fhip_finalize(void) {
   return fhip::guarded([&] {
     FHIP_CHECK(hipDeviceSynchronize());
     fhip::Pool::get().trim();
+  });
+}
+
+int fhip_prof_enable(int on) {
+  return fhip::guarded([&] {
+    std::lock_guard<std::mutex> lk(fhip::g_prof_mu);
+    fhip::g_prof_on = on != 0;
+    fhip::prof_clear_locked();
+  });
+}
+
+int fhip_prof_query(const char* name, double* total_ms, size_t* launches) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(name && total_ms && launches, FHIP_ERR_ARG, "prof_query: null pointer");
+    std::lock_guard<std::mutex> lk(fhip::g_prof_mu);
+    double t = 0;
+    size_t c = 0;
+    for (auto& e : fhip::g_prof) {
+      if (e.name != name) continue;
+      FHIP_CHECK(hipEventSynchronize(e.b));
+      float ms = 0;
+      FHIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+      t += ms;
+      c++;
+    }
+    *total_ms = t;
+    *launches = c;
   });
 }
 

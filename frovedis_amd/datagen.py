@@ -1,0 +1,135 @@
+"""Seeded synthetic inputs for the benchmark configurations.
+
+SplitMix64 is counter based: element i of stream `seed` is
+splitmix64(seed + (i + 1) * golden), so the host (numpy) and device (torch)
+generators produce the same bits and any slice can be generated alone.
+Integer-valued outputs (keys, ranks, int32 columns) are bit-identical on host
+and device; FP outputs that go through log/cos (normal, lognormal) may
+differ in the last ulp between the two.
+"""
+import math
+
+import numpy as np
+
+GOLDEN = 0x9E3779B97F4A7C15
+M1 = 0xBF58476D1CE4E5B9
+M2 = 0x94D049BB133111EB
+
+SEED_RADIX = 0x5EED0001
+SEED_ZIPF = 0x5EED0002
+SEED_NORMAL = 0x5EED0003
+SEED_INT = 0x5EED0004
+SEED_PERM = 0x5EED0005
+SEED_FK = 0x5EED0006
+SEED_SEGS = 0x5EED0007
+SEED_LOGN = 0x5EED0008
+
+
+def _s64(x):
+    """uint64 constant as the int64 with the same bits (torch carrier)."""
+    x &= (1 << 64) - 1
+    return x - (1 << 64) if x >= (1 << 63) else x
+
+
+# ---------------------------------------------------------------- host (numpy)
+def splitmix64_np(seed, start, n):
+    with np.errstate(over="ignore"):
+        i = np.arange(start + 1, start + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * np.uint64(GOLDEN)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(M1)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(M2)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01_np(seed, start, n):
+    return (splitmix64_np(seed, start, n) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
+def zipf_cdf(n_values=1 << 20, s=1.1):
+    k = np.arange(1, n_values + 1, dtype=np.float64)
+    w = k ** (-s)
+    c = np.cumsum(w)
+    return c / c[-1]
+
+
+def zipf_keys_np(n, start=0, n_values=1 << 20, s=1.1, seed=SEED_ZIPF):
+    u = uniform01_np(seed, start, n)
+    return (np.searchsorted(zipf_cdf(n_values, s), u, side="right") + 1).astype(np.int64)
+
+
+def int_uniform_np(n, hi, start=0, seed=SEED_INT):
+    return ((splitmix64_np(seed, start, n) >> np.uint64(32)) % np.uint64(hi)).astype(np.int32)
+
+
+def normal_np(n, start=0, seed=SEED_NORMAL):
+    # Box-Muller on two interleaved uniforms of the same stream
+    u = uniform01_np(seed, 2 * start, 2 * n)
+    u1 = 1.0 - u[0::2]
+    u2 = u[1::2]
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+# --------------------------------------------------------------- device (torch)
+def _torch():
+    import torch
+    return torch
+
+
+def _srl(torch, z, s):
+    # logical right shift on the int64 carrier
+    return (z >> s) & ((1 << (64 - s)) - 1)
+
+
+def splitmix64_dev(seed, start, n, device="cuda"):
+    torch = _torch()
+    i = torch.arange(start + 1, start + n + 1, dtype=torch.int64, device=device)
+    z = i * _s64(GOLDEN) + _s64(seed)
+    z = (z ^ _srl(torch, z, 30)) * _s64(M1)
+    z = (z ^ _srl(torch, z, 27)) * _s64(M2)
+    return z ^ _srl(torch, z, 31)
+
+
+def uniform01_dev(seed, start, n, device="cuda"):
+    torch = _torch()
+    z = splitmix64_dev(seed, start, n, device)
+    return _srl(torch, z, 11).to(torch.float64) * (2.0 ** -53)
+
+
+def zipf_keys_dev(n, start=0, n_values=1 << 20, s=1.1, seed=SEED_ZIPF, device="cuda",
+                  chunk=1 << 26):
+    torch = _torch()
+    cdf = torch.from_numpy(zipf_cdf(n_values, s)).to(device)
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    for b in range(0, n, chunk):
+        m = min(chunk, n - b)
+        u = uniform01_dev(seed, start + b, m, device)
+        out[b:b + m] = torch.searchsorted(cdf, u, right=True) + 1
+        del u
+    return out
+
+
+def int_uniform_dev(n, hi, start=0, seed=SEED_INT, device="cuda", chunk=1 << 26):
+    torch = _torch()
+    out = torch.empty(n, dtype=torch.int32, device=device)
+    for b in range(0, n, chunk):
+        m = min(chunk, n - b)
+        z = splitmix64_dev(seed, start + b, m, device)
+        out[b:b + m] = (_srl(torch, z, 32) % hi).to(torch.int32)
+    return out
+
+
+def normal_dev(n, start=0, seed=SEED_NORMAL, device="cuda", chunk=1 << 25):
+    torch = _torch()
+    out = torch.empty(n, dtype=torch.float64, device=device)
+    for b in range(0, n, chunk):
+        m = min(chunk, n - b)
+        u = uniform01_dev(seed, 2 * (start + b), 2 * m, device)
+        u1 = 1.0 - u[0::2]
+        u2 = u[1::2]
+        out[b:b + m] = torch.sqrt(-2.0 * torch.log(u1)) * torch.cos(2.0 * math.pi * u2)
+    return out
+
+
+def radix_keys_dev(n, seed=SEED_RADIX, device="cuda"):
+    """Full-range uint64 keys (int64 carrier) for the radix kv config."""
+    return splitmix64_dev(seed, 0, n, device)

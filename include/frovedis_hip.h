@@ -57,6 +57,15 @@ const char* fhip_last_error(void);
 /* Frees device memory that an entry point allocated for its output. */
 int fhip_dev_free(void* p);
 
+/* Kernel timing (the role of the reference's time_spent laps,
+ * core/utility.hpp:44-75): when enabled, the library records HIP events on
+ * the launch stream around its main kernels, named "radix_pass",
+ * "radix_hist", "hashagg", "ht_build", "ht_probe", "join_probe",
+ * "batch_sort", "learned_scatter"...  fhip_prof_enable(on) also clears
+ * what was recorded; fhip_prof_query sums the durations of one name. */
+int fhip_prof_enable(int on);
+int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
+
 /* ---- radix_sort ---------------------------------------------------------
  * Stable ascending (or descending) sort of keys with an optional payload,
  * in place.  Replaces radix_sort(K*, V*, size_t, bool positive_only)
@@ -95,6 +104,53 @@ int fhip_dev_set_is_unique(int type, const void* sorted, size_t n,
 int fhip_dev_lower_bound(int type, const void* sorted, size_t m,
                          const void* queries, size_t nq, size_t* out,
                          int upper, void* stream);
+
+/* ---- group_by + grouped aggregation (sort path) ---------------------------
+ * typed_dfcolumn<T>::group_by on one rank (dataframe/dfcolumn_groupby.cc:
+ * 16-120): gathers col[idx[i]], stable-sorts (key, idx) -- idx is permuted
+ * in place into local_grouped_idx -- and writes local_idx_split (n + 1
+ * entries of capacity; *nsplit receives the count) and the group keys
+ * (ascending, nsplit - 1 of them). */
+int fhip_dev_group_by(int ktype, const void* col, size_t* idx, size_t n,
+                      size_t* split, size_t* nsplit, void* keys_out,
+                      void* stream);
+
+/* aggregation codes (the dfaggregator sum/count/avg/max/min,
+ * dataframe/dfaggregator.cc:6-63) */
+#define FHIP_AGG_SUM 0
+#define FHIP_AGG_COUNT 1
+#define FHIP_AGG_AVG 2
+#define FHIP_AGG_MAX 3
+#define FHIP_AGG_MIN 4
+
+/* Per-group aggregate of one column over (grouped_idx, split) from
+ * fhip_dev_group_by (dataframe/dfcolumn_grouped_agg.cc: sum :105-275,
+ * max :278-510, min :513-717, count :719-837, avg :839-889; one rank).
+ * `nulls` lists null row positions (may be NULL when nnull == 0).  Output
+ * types: sum/max/min in vtype, count as uint64, avg as double. */
+int fhip_dev_grouped_agg(int op, int vtype, const void* col,
+                         const size_t* grouped_idx, size_t n,
+                         const size_t* split, size_t nsplit,
+                         const size_t* nulls, size_t nnull, void* out,
+                         void* stream);
+
+/* ---- fused group_by + aggregation (hash path) ----------------------------
+ * One pass over the rows for group_by(k).select({k}, {aggs...})
+ * (dftable.cc:346-366 and :3724-3772 on one rank, no nulls): integer key
+ * column, up to four distinct (column, sum|max|min) accumulators plus the
+ * row count.  Groups come out ascending by key; keys_out and every
+ * aggs[i].out hold `capacity` entries; *ngroups receives the group count.
+ * table_hint (0 = automatic) is the expected number of groups. */
+typedef struct fhip_agg {
+  int op;          /* FHIP_AGG_* */
+  int vtype;       /* FHIP_* type code of col (ignored for count) */
+  const void* col; /* device column of n rows (NULL for count) */
+  void* out;       /* device output */
+} fhip_agg;
+int fhip_dev_group_by_agg(int ktype, const void* keys, size_t n,
+                          const fhip_agg* aggs, int naggs, void* keys_out,
+                          size_t capacity, size_t* ngroups, size_t table_hint,
+                          void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,193 @@
+"""GPU parity of group_by and grouped aggregation against the CPU oracle.
+
+Bit-exact: local_grouped_idx, local_idx_split, group keys, counts, integer
+sums, max/min.  FP64 sums and means: |gpu - ref| <= 1e-12 * max(|ref|,
+sum|x|) per group -- the reference's own fold carries rounding error of that
+order, so the bound is taken relative to the group's absolute mass, which
+stays meaningful for groups whose sum cancels to near zero.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FP64_TOL = 1e-12
+FP32_TOL = 2e-6
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import frovedis_amd
+    frovedis_amd.lib()
+    return frovedis_amd
+
+
+def _abs_mass(v, gi, split):
+    a = np.abs(v[gi.astype(np.int64)])
+    c = np.concatenate([[0.0], np.cumsum(a)])
+    return c[split[1:].astype(np.int64)] - c[split[:-1].astype(np.int64)]
+
+
+def _check_fp(got, ref, mass, tol):
+    bound = tol * np.maximum(np.abs(ref), mass)
+    bad = np.abs(got - ref) > bound
+    assert not bad.any(), (np.flatnonzero(bad)[:5], got[bad][:5], ref[bad][:5])
+
+
+@pytest.mark.parametrize("kt", [np.int32, np.uint32, np.int64, np.uint64, np.float32, np.float64])
+def test_group_by_matches_oracle(fa, kt):
+    rng = np.random.default_rng(11)
+    for n in [0, 1, 63, 64, 5000, 1 << 20]:
+        if np.dtype(kt).kind == "f":
+            k = rng.integers(-500, 500, n).astype(kt) * kt(0.25)
+        elif np.dtype(kt).kind == "u":
+            k = rng.integers(0, 3000, n).astype(kt)
+        else:
+            k = rng.integers(-1500, 1500, n).astype(kt)
+        gi, split, keys = oracle.group_by(k)
+        dk = fa.DeviceArray.from_numpy(k)
+        ggi, gsplit, gkeys = fa.group_by(dk)
+        assert np.array_equal(ggi.numpy(), gi), n
+        assert np.array_equal(gsplit.numpy(), split), n
+        assert np.array_equal(gkeys.numpy(), keys), n
+
+
+def test_group_by_with_local_idx(fa):
+    # local_idx need not be iota (a filtered table): the key column is read
+    # through it (dfcolumn_groupby.cc:39)
+    rng = np.random.default_rng(12)
+    col = rng.integers(0, 100, 50000).astype(np.int64)
+    idx = np.sort(rng.choice(50000, 20000, replace=False)).astype(np.uint64)
+    gi, split, keys = oracle.group_by(col, idx)
+    ggi, gsplit, gkeys = fa.group_by(fa.DeviceArray.from_numpy(col), fa.DeviceArray.from_numpy(idx))
+    assert np.array_equal(ggi.numpy(), gi)
+    assert np.array_equal(gsplit.numpy(), split)
+    assert np.array_equal(gkeys.numpy(), keys)
+
+
+@pytest.mark.parametrize("vt", [np.int32, np.int64, np.uint64, np.float32, np.float64])
+def test_grouped_agg_matches_oracle(fa, vt):
+    rng = np.random.default_rng(13)
+    n = 300000
+    # a skewed key so that some groups exceed the one-wave limit (8192 rows)
+    k = np.where(rng.random(n) < 0.3, 7, rng.integers(0, 2000, n)).astype(np.int64)
+    if np.dtype(vt).kind == "f":
+        v = rng.standard_normal(n).astype(vt)
+    elif np.dtype(vt).kind == "u":
+        v = rng.integers(0, 1 << 40, n).astype(vt)
+    else:
+        v = rng.integers(-(1 << 30), 1 << 30, n).astype(vt)
+    gi, split, keys = oracle.group_by(k)
+    dgi, dsplit = fa.DeviceArray.from_numpy(gi), fa.DeviceArray.from_numpy(split)
+    dv = fa.DeviceArray.from_numpy(v)
+    for nulls in [None, np.sort(rng.choice(n, 5000, replace=False)).astype(np.uint64)]:
+        dn = None if nulls is None else fa.DeviceArray.from_numpy(nulls)
+        for op in ["sum", "count", "avg", "max", "min"]:
+            ref = oracle.grouped_agg(op, v, gi, split, nulls)
+            got = fa.grouped_agg(op, dv, dgi, dsplit, dn).numpy()
+            if op in ("count", "max", "min") or np.dtype(vt).kind != "f" and op == "sum":
+                assert np.array_equal(got, ref), (op, nulls is None)
+            else:
+                vv = v.astype(np.float64)
+                if nulls is not None:
+                    vv = vv.copy()
+                    vv[nulls.astype(np.int64)] = 0
+                mass = _abs_mass(vv, gi, split)
+                if op == "avg":
+                    cnt = oracle.grouped_agg("count", v, gi, split, nulls).astype(np.float64)
+                    mass = mass / np.maximum(cnt, 1)
+                tol = FP32_TOL if vt == np.float32 and op == "sum" else FP64_TOL
+                if vt == np.float32 and op == "avg":
+                    tol = FP32_TOL
+                _check_fp(got, ref, mass, tol)
+
+
+def test_tutorial_group_by_on_gpu(fa):
+    import csv
+    rows = list(csv.reader(open(os.path.join(GOLD, "tut5_6_t.csv"))))
+    kat = json.load(open(os.path.join(GOLD, "tutorial_known_answers.json")))["group_by_c1_agg_c3"]
+    c1 = fa.DeviceArray.from_numpy(np.array([int(r[0]) for r in rows], np.int32))
+    c3 = fa.DeviceArray.from_numpy(np.array([int(r[2]) for r in rows], np.int32))
+    gi, split, keys = fa.group_by(c1)
+    assert keys.numpy().tolist() == kat["c1"]
+    for op in ["sum", "count", "avg", "max", "min"]:
+        assert fa.grouped_agg(op, c3, gi, split).numpy().tolist() == kat[op], op
+    gk, outs = fa.group_by_agg(c1, [("sum", c3), ("count", c3), ("avg", c3), ("max", c3), ("min", c3)])
+    assert gk.numpy().tolist() == kat["c1"]
+    for op, o in zip(["sum", "count", "avg", "max", "min"], outs):
+        assert o.numpy().tolist() == kat[op], op
+
+
+def _zipf_case(n, seed_shift=0):
+    from frovedis_amd import datagen as dg
+    k = dg.zipf_keys_np(n, start=seed_shift)
+    v = dg.normal_np(n, start=seed_shift)
+    iv = dg.int_uniform_np(n, 1000, start=seed_shift)
+    return k, v, iv
+
+
+@pytest.mark.parametrize("n", [1000, 1 << 16, 1 << 21])
+def test_group_by_agg_zipf_matches_oracle(fa, n):
+    k, v, iv = _zipf_case(n)
+    gi, split, keys = oracle.group_by(k)
+    ref_sum = oracle.grouped_agg("sum", v, gi, split)
+    ref_cnt = oracle.grouped_agg("count", v, gi, split)
+    ref_avg = oracle.grouped_agg("avg", v, gi, split)
+    ref_max = oracle.grouped_agg("max", iv, gi, split)
+    dk, dv, div = (fa.DeviceArray.from_numpy(a) for a in (k, v, iv))
+    gk, (s, c, a, m) = fa.group_by_agg(dk, [("sum", dv), ("count", dv), ("avg", dv), ("max", div)])
+    assert np.array_equal(gk.numpy(), keys)
+    assert np.array_equal(c.numpy(), ref_cnt)
+    assert np.array_equal(m.numpy(), ref_max)
+    mass = _abs_mass(v, gi, split)
+    _check_fp(s.numpy(), ref_sum, mass, FP64_TOL)
+    _check_fp(a.numpy(), ref_avg, mass / ref_cnt, FP64_TOL)
+
+
+def test_group_by_agg_table_growth_and_special_keys(fa):
+    # many distinct keys with a tiny table hint: the table is rebuilt larger;
+    # INT64_MIN (the empty-slot marker) and INT64_MAX are ordinary keys
+    rng = np.random.default_rng(21)
+    n = 400000
+    k = rng.integers(-(1 << 62), 1 << 62, n).astype(np.int64) // 3
+    k[::1000] = np.iinfo(np.int64).min
+    k[5::1000] = np.iinfo(np.int64).max
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    gi, split, keys = oracle.group_by(k)
+    dk, dv = fa.DeviceArray.from_numpy(k), fa.DeviceArray.from_numpy(v)
+    gk, (s, c, mx, mn) = fa.group_by_agg(dk, [("sum", dv), ("count", None), ("max", dv), ("min", dv)],
+                                         table_hint=16, capacity=100)
+    assert np.array_equal(gk.numpy(), keys)
+    assert np.array_equal(s.numpy(), oracle.grouped_agg("sum", v, gi, split))
+    assert np.array_equal(c.numpy(), oracle.grouped_agg("count", v, gi, split))
+    assert np.array_equal(mx.numpy(), oracle.grouped_agg("max", v, gi, split))
+    assert np.array_equal(mn.numpy(), oracle.grouped_agg("min", v, gi, split))
+
+
+def test_group_by_agg_full_size_cross_check(fa):
+    # 2^26 Zipf(1.1) rows generated on the device: the hash path and the sort
+    # path are independent implementations; keys/counts/max must agree
+    # exactly and the sums to the FP64 bound; counts add up to n
+    import torch
+    from frovedis_amd import datagen as dg
+    n = 1 << 26
+    k = fa.DeviceArray(dg.zipf_keys_dev(n), np.int64)
+    v = fa.DeviceArray(dg.normal_dev(n), np.float64)
+    iv = fa.DeviceArray(dg.int_uniform_dev(n, 1000), np.int32)
+    gk, (s, c, m) = fa.group_by_agg(k, [("sum", v), ("count", v), ("max", iv)])
+    gi, split, keys = fa.group_by(k)
+    assert torch.equal(gk.t, keys.t)
+    assert int(c.t.sum()) == n
+    assert torch.equal(c.t, (split.t[1:] - split.t[:-1]))
+    assert torch.equal(m.t, fa.grouped_agg("max", iv, gi, split).t)
+    s2 = fa.grouped_agg("sum", v, gi, split).t
+    absm = fa.grouped_agg("sum", fa.DeviceArray(v.t.abs(), np.float64), gi, split).t
+    assert bool(((s.t - s2).abs() <= 2 * FP64_TOL * torch.maximum(s2.abs(), absm)).all())
