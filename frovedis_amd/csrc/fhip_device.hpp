@@ -102,4 +102,62 @@ size_t compact_indices(Pred pred, size_t n, size_t* out, hipStream_t st) {
   return compact(pred, IdentityMap{}, n, out, st);
 }
 
+// Calls writer(pos, i) for every i with pred(i), pos = rank among the
+// selected indices (stable).
+template <class Pred, class Writer>
+__global__ __launch_bounds__(CP_NT) void compact_apply_kernel(Pred pred, Writer writer, size_t n,
+                                                              const unsigned long long* offs) {
+  __shared__ unsigned tmp[CP_NT / 64];
+  const size_t base = size_t(blockIdx.x) * CP_TILE;
+  unsigned long long run = offs[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < CP_ITEMS; j++) {
+    size_t i = base + size_t(j) * CP_NT + threadIdx.x;
+    unsigned f = (i < n && pred(i)) ? 1u : 0u;
+    unsigned all;
+    unsigned e = block_excl_scan<CP_NT>(f, tmp, &all);
+    if (f) writer(size_t(run + e), i);
+    run += all;
+  }
+}
+
+// Counts the selected indices (synchronises); if the count is at most
+// `capacity`, also runs the writer.  Returns the count.
+template <class Pred, class Writer>
+size_t compact_apply(Pred pred, Writer writer, size_t n, size_t capacity, hipStream_t st) {
+  if (n == 0) return 0;
+  const size_t nb = ceil_div(n, CP_TILE);
+  DBuf cb((nb + 1) * sizeof(unsigned long long));
+  unsigned long long* counts = cb.as<unsigned long long>();
+  compact_count_kernel<<<unsigned(nb), CP_NT, 0, st>>>(pred, n, counts);
+  FHIP_CHECK(hipGetLastError());
+  scan_counts_kernel<<<1, 1024, 0, st>>>(counts, nb, counts + nb);
+  FHIP_CHECK(hipGetLastError());
+  unsigned long long total = 0;
+  FHIP_CHECK(hipMemcpyAsync(&total, counts + nb, sizeof(total), hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  if (size_t(total) <= capacity && total) {
+    compact_apply_kernel<<<unsigned(nb), CP_NT, 0, st>>>(pred, writer, n, counts);
+    FHIP_CHECK(hipGetLastError());
+  }
+  return size_t(total);
+}
+
+inline int grid_for_n(size_t n, int per_cu = 8) {
+  return int(std::max<size_t>(1, std::min<size_t>(ceil_div(n, 256), size_t(fill_blocks(256, per_cu)))));
+}
+
+template <class F>
+void dispatch_type(int t, F&& f) {
+  switch (t) {
+    case FHIP_I32: f(int32_t()); break;
+    case FHIP_U32: f(uint32_t()); break;
+    case FHIP_I64: f(int64_t()); break;
+    case FHIP_U64: f(uint64_t()); break;
+    case FHIP_F32: f(float()); break;
+    case FHIP_F64: f(double()); break;
+    default: throw Error(FHIP_ERR_TYPE, "unsupported type code " + std::to_string(t));
+  }
+}
+
 }  // namespace fhip

@@ -343,7 +343,7 @@ void group_by(int ktype, const void* col, size_t* idx, size_t n, size_t* split, 
   FHIP_CHECK(hipGetLastError());
 }
 
-void grouped_agg(int op, int vtype, const void* col, const size_t* gidx, size_t n,
+void grouped_agg(int op, int vtype, const void* col, size_t col_n, const size_t* gidx, size_t n,
                  const size_t* split, size_t nsplit, const size_t* nulls, size_t nn, void* out,
                  hipStream_t st) {
   FHIP_REQUIRE(op >= OP_SUM && op <= OP_MIN, FHIP_ERR_ARG, "grouped_agg: bad op");
@@ -351,10 +351,10 @@ void grouped_agg(int op, int vtype, const void* col, const size_t* gidx, size_t 
   const size_t G = nsplit - 1;
   DBuf maskb;
   const uint8_t* mask = nullptr;
-  if (nn) {
-    maskb.reset(n);
-    FHIP_CHECK(hipMemsetAsync(maskb.get(), 0, n, st));
-    nullmask_kernel<<<grid_for(nn), 256, 0, st>>>(nulls, nn, n, maskb.as<uint8_t>());
+  if (nn) {  // one flag per row of the column (grouped_idx indexes the column)
+    maskb.reset(col_n);
+    FHIP_CHECK(hipMemsetAsync(maskb.get(), 0, col_n, st));
+    nullmask_kernel<<<grid_for(nn), 256, 0, st>>>(nulls, nn, col_n, maskb.as<uint8_t>());
     FHIP_CHECK(hipGetLastError());
     mask = maskb.as<uint8_t>();
   }
@@ -410,14 +410,14 @@ int fhip_dev_group_by(int ktype, const void* col, size_t* idx, size_t n, size_t*
   });
 }
 
-int fhip_dev_grouped_agg(int op, int vtype, const void* col, const size_t* gidx, size_t n,
-                         const size_t* split, size_t nsplit, const size_t* nulls, size_t nnull,
-                         void* out, void* stream) {
+int fhip_dev_grouped_agg(int op, int vtype, const void* col, size_t col_n, const size_t* gidx,
+                         size_t n, const size_t* split, size_t nsplit, const size_t* nulls,
+                         size_t nnull, void* out, void* stream) {
   return fhip::guarded([&] {
     FHIP_REQUIRE(nsplit < 2 || (split && out && gidx && (col || op == 1)), FHIP_ERR_ARG,
                  "grouped_agg: null pointer");
     FHIP_REQUIRE(nnull == 0 || nulls, FHIP_ERR_ARG, "grouped_agg: null nulls list");
-    fhip::grouped_agg(op, vtype, col, gidx, n, split, nsplit, nulls, nnull, out,
+    fhip::grouped_agg(op, vtype, col, col_n, gidx, n, split, nsplit, nulls, nnull, out,
                       fhip::as_stream(stream));
   });
 }

@@ -1,0 +1,173 @@
+// equi_join on one rank's share for gfx950 (dataframe/join.hpp:304-326 VE
+// body, join.cc:10-61 shrink_missed, multi_equi_join :198-302).
+//
+// strategy FHIP_JOIN_AUTO follows the reference: a unique_hashtable is built
+// on RIGHT with the right row ids as values; if the right keys are unique,
+// LEFT is probed in order and the hits are compacted (hash join, output in
+// left order); otherwise both sides are stable-sorted and every left row
+// (in sorted order) is paired with its run of equal right keys (sort-merge,
+// output in (key, left position, right position) order).
+// FHIP_JOIN_HASH requires unique right keys; FHIP_JOIN_SORTMERGE skips the
+// table build.
+#include "fhip_device.hpp"
+#include "hashtable.hpp"
+#include "radix_sort.hpp"
+
+namespace fhip {
+
+void prefix_sum(int type, const void* in, void* out, size_t n, hipStream_t st);
+
+namespace {
+
+struct HitPred {
+  const uint8_t* f;
+  __device__ bool operator()(size_t i) const { return f[i] != 0; }
+};
+
+struct HitWriter {
+  const size_t* lidx;
+  const uint64_t* vals;
+  size_t* lo;
+  size_t* ro;
+  __device__ void operator()(size_t pos, size_t i) const {
+    lo[pos] = lidx[i];
+    ro[pos] = vals[i];
+  }
+};
+
+// number of right matches of every sorted left key, and where they start
+template <class K>
+__global__ void match_count_kernel(const K* __restrict__ lk, size_t nl, const K* __restrict__ rk,
+                                   size_t nr, unsigned long long* __restrict__ cnt,
+                                   unsigned long long* __restrict__ first) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nl; i += stride) {
+    const K x = lk[i];
+    size_t lo = 0, hi = nr;
+    while (lo < hi) {  // lower_bound
+      size_t mid = (lo + hi) >> 1;
+      if (rk[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    size_t a = lo;
+    hi = nr;
+    while (lo < hi) {  // upper_bound
+      size_t mid = (lo + hi) >> 1;
+      if (x < rk[mid]) hi = mid; else lo = mid + 1;
+    }
+    cnt[i] = lo - a;
+    first[i] = a;
+  }
+}
+
+__global__ void pair_write_kernel(const unsigned long long* __restrict__ incl,
+                                  const unsigned long long* __restrict__ cnt,
+                                  const unsigned long long* __restrict__ first,
+                                  const size_t* __restrict__ lidx_sorted,
+                                  const size_t* __restrict__ ridx_sorted, size_t nl,
+                                  size_t* __restrict__ lo, size_t* __restrict__ ro) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nl; i += stride) {
+    const unsigned long long c = cnt[i];
+    if (!c) continue;
+    const size_t off = size_t(incl[i] - c);
+    const size_t l = lidx_sorted[i];
+    const size_t f = size_t(first[i]);
+    for (unsigned long long j = 0; j < c; j++) {
+      lo[off + j] = l;
+      ro[off + j] = ridx_sorted[f + j];
+    }
+  }
+}
+
+template <class K>
+size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, const K* right,
+                    const size_t* ridx, size_t nr, size_t* lo, size_t* ro, size_t capacity,
+                    hipStream_t st) {
+  if (nl == 0 || nr == 0) return 0;
+  DBuf lk(nl * sizeof(K)), li(nl * sizeof(size_t)), rk(nr * sizeof(K)), ri(nr * sizeof(size_t));
+  // radix_sort(right, right_idx); radix_sort(left, left_idx) (join.hpp:320-322)
+  sort_pairs(ktype, right, rk.get(), 8, ridx, ri.get(), nr, false, false, st);
+  sort_pairs(ktype, left, lk.get(), 8, lidx, li.get(), nl, false, false, st);
+  DBuf cnt(nl * sizeof(unsigned long long)), first(nl * sizeof(unsigned long long)),
+      incl(nl * sizeof(unsigned long long));
+  {
+    ProfScope prof("join_merge", st);
+    match_count_kernel<K><<<grid_for_n(nl), 256, 0, st>>>(lk.as<K>(), nl, rk.as<K>(), nr,
+                                                          cnt.as<unsigned long long>(),
+                                                          first.as<unsigned long long>());
+    FHIP_CHECK(hipGetLastError());
+  }
+  prefix_sum(FHIP_U64, cnt.get(), incl.get(), nl, st);
+  unsigned long long total = 0;
+  FHIP_CHECK(hipMemcpyAsync(&total, incl.as<unsigned long long>() + (nl - 1), sizeof(total),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  if (total && size_t(total) <= capacity) {
+    ProfScope prof("join_write", st);
+    pair_write_kernel<<<grid_for_n(nl), 256, 0, st>>>(
+        incl.as<unsigned long long>(), cnt.as<unsigned long long>(),
+        first.as<unsigned long long>(), li.as<size_t>(), ri.as<size_t>(), nl, lo, ro);
+    FHIP_CHECK(hipGetLastError());
+  }
+  return size_t(total);
+}
+
+}  // namespace
+
+size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, const void* right,
+                 const size_t* ridx, size_t nr, int strategy, size_t* lo, size_t* ro,
+                 size_t capacity, int* used_hash, hipStream_t st) {
+  FHIP_REQUIRE(strategy >= FHIP_JOIN_AUTO && strategy <= FHIP_JOIN_SORTMERGE, FHIP_ERR_ARG,
+               "equi_join: bad strategy");
+  if (used_hash) *used_hash = 0;
+  if (strategy != FHIP_JOIN_SORTMERGE) {
+    int uniq = 1;
+    fhip_ht* h = ht_build(ktype, right, reinterpret_cast<const uint64_t*>(ridx), nr, true, &uniq, st);
+    if (uniq) {
+      size_t r = 0;
+      try {
+        DBuf vals(nl * sizeof(uint64_t)), found(nl);
+        ht_probe(h, left, nl, vals.as<uint64_t>(), found.as<uint8_t>(), st);
+        ProfScope prof("join_compact", st);
+        r = compact_apply(HitPred{found.as<uint8_t>()},
+                          HitWriter{lidx, vals.as<uint64_t>(), lo, ro}, nl, capacity, st);
+      } catch (...) {
+        ht_destroy(h);
+        throw;
+      }
+      ht_destroy(h);
+      if (used_hash) *used_hash = 1;
+      return r;
+    }
+    ht_destroy(h);
+    FHIP_REQUIRE(strategy == FHIP_JOIN_AUTO, FHIP_ERR_ARG,
+                 "equi_join: hash strategy needs unique right keys");
+  }
+  size_t r = 0;
+  dispatch_type(ktype, [&](auto z) {
+    using K = decltype(z);
+    r = sort_merge_t<K>(ktype, (const K*)left, lidx, nl, (const K*)right, ridx, nr, lo, ro,
+                        capacity, st);
+  });
+  return r;
+}
+
+}  // namespace fhip
+
+extern "C" int fhip_dev_equi_join(int ktype, const void* left, const size_t* left_idx, size_t nl,
+                                  const void* right, const size_t* right_idx, size_t nr,
+                                  int strategy, size_t* left_out, size_t* right_out,
+                                  size_t capacity, size_t* nout, int* used_hash, void* stream) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(nout, FHIP_ERR_ARG, "equi_join: null nout");
+    FHIP_REQUIRE((nl == 0 || (left && left_idx)) && (nr == 0 || (right && right_idx)),
+                 FHIP_ERR_ARG, "equi_join: null input");
+    FHIP_REQUIRE(capacity == 0 || (left_out && right_out), FHIP_ERR_ARG, "equi_join: null output");
+    size_t r = fhip::equi_join(ktype, left, left_idx, nl, right, right_idx, nr, strategy, left_out,
+                               right_out, capacity, used_hash, fhip::as_stream(stream));
+    *nout = r;
+    FHIP_REQUIRE(r <= capacity, FHIP_ERR_TOO_LARGE,
+                 "equi_join: output capacity " + std::to_string(capacity) + " < pairs " +
+                     std::to_string(r));
+  });
+}

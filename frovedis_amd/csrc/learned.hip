@@ -1,0 +1,612 @@
+// Learned-CDF partition sort for gfx950 (no counterpart in the reference;
+// its output equals radix_sort(K*, size_t) of core/radix_sort.hpp:221-242).
+//
+// Model: an FP32 MLP (mlp.hip) maps a normalised log-key to its CDF; it is
+// trained on (key, rank) samples taken from a plain radix sort of an earlier
+// input, then distilled into B1 coarse splitter keys (monotone by
+// construction) and, like the leaves of a recursive model index, B2 fine
+// splitters per coarse bucket taken from the sample keys inside it.
+//
+// Sort of 64-bit keys (key-only; equal keys are indistinguishable so the
+// scatter need not be stable):
+//   L1 count   : bucket of every key (binary search over the B1 splitters
+//                in LDS), per-bucket counts                    read 8 B/key
+//   L1 scatter : keys to their coarse bucket range (LDS ranks + one global
+//                atomic per block and bucket)                  16 B/key
+//   L2         : one workgroup per coarse bucket splits it into B2 fine
+//                buckets                                       24 B/key
+//   L3         : one workgroup per fine bucket sorts it in LDS (LSD radix
+//                on key - bucket min, 8-bit digits, wave64 ballot ranks)
+//                and writes the final positions                16 B/key
+// Fine buckets over CAP keys (2x the expected size) are sorted afterwards by
+// the plain radix sort (fallback); if too many keys need the fallback the
+// whole input goes through the plain radix sort.
+#include "fhip_device.hpp"
+#include "radix_sort.hpp"
+
+#include <cmath>
+
+namespace fhip {
+
+void mlp_grad(const float* params, const float* xs, const float* ts, size_t S, float* grad,
+              float* loss, hipStream_t st);
+void mlp_forward(const float* params, const float* xs, size_t S, float* ys, hipStream_t st);
+void mlp_adam(float* p, float* g, float* m, float* v, int step, float lr, hipStream_t st);
+extern const int kMlpParams;
+
+namespace {
+
+constexpr int LS_B1 = 4096;        // coarse buckets
+constexpr int LS_B2 = 16;          // fine buckets per coarse bucket
+constexpr int LS_NT = 512;
+constexpr int LS_ITEMS = 32;       // keys per thread in the L1 scatter
+constexpr int L3_NT = 256;
+constexpr int L3_CAP = 8192;       // keys a fine bucket may hold (LDS)
+constexpr int L3_ITEMS = L3_CAP / L3_NT;
+
+__device__ __forceinline__ uint64_t to_ordered(uint64_t k, int is_signed) {
+  return is_signed ? (k ^ 0x8000000000000000ull) : k;
+}
+__device__ __forceinline__ uint64_t from_ordered(uint64_t u, int is_signed) {
+  return is_signed ? (u ^ 0x8000000000000000ull) : u;
+}
+
+// number of splitters <= u, minus one (splitters[0] is 0)
+__device__ __forceinline__ int bucket_of(const uint64_t* spl, int nb, uint64_t u) {
+  int lo = 0, hi = nb;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (spl[mid] <= u) lo = mid + 1; else hi = mid;
+  }
+  return lo - 1;
+}
+
+__global__ __launch_bounds__(LS_NT) void l1_count_kernel(const uint64_t* __restrict__ keys,
+                                                         size_t n, int is_signed,
+                                                         const uint64_t* __restrict__ coarse,
+                                                         unsigned long long* __restrict__ hist) {
+  __shared__ uint64_t spl[LS_B1];
+  __shared__ unsigned h[LS_B1];
+  for (int i = threadIdx.x; i < LS_B1; i += LS_NT) {
+    spl[i] = coarse[i];
+    h[i] = 0;
+  }
+  __syncthreads();
+  const size_t stride = size_t(gridDim.x) * LS_NT;
+  for (size_t i = size_t(blockIdx.x) * LS_NT + threadIdx.x; i < n; i += stride)
+    atomicAdd(&h[bucket_of(spl, LS_B1, to_ordered(keys[i], is_signed))], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < LS_B1; i += LS_NT)
+    if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
+}
+
+__global__ __launch_bounds__(LS_NT) void l1_scatter_kernel(const uint64_t* __restrict__ keys,
+                                                           size_t n, int is_signed,
+                                                           const uint64_t* __restrict__ coarse,
+                                                           unsigned long long* __restrict__ cursor,
+                                                           uint64_t* __restrict__ out) {
+  __shared__ uint64_t spl[LS_B1];
+  __shared__ unsigned cnt[LS_B1];
+  __shared__ unsigned long long base[LS_B1];
+  for (int i = threadIdx.x; i < LS_B1; i += LS_NT) {
+    spl[i] = coarse[i];
+    cnt[i] = 0;
+  }
+  __syncthreads();
+  const size_t tile = size_t(blockIdx.x) * LS_NT * LS_ITEMS;
+  uint64_t k[LS_ITEMS];
+  int b[LS_ITEMS];
+#pragma unroll
+  for (int j = 0; j < LS_ITEMS; j++) {
+    size_t i = tile + size_t(j) * LS_NT + threadIdx.x;
+    k[j] = i < n ? keys[i] : 0;
+    b[j] = -1;
+    if (i < n) {
+      b[j] = bucket_of(spl, LS_B1, to_ordered(k[j], is_signed));
+      atomicAdd(&cnt[b[j]], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < LS_B1; i += LS_NT) {
+    unsigned c = cnt[i];
+    base[i] = c ? atomicAdd(&cursor[i], (unsigned long long)c) : 0;
+    cnt[i] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LS_ITEMS; j++) {
+    if (b[j] < 0) continue;
+    unsigned r = atomicAdd(&cnt[b[j]], 1u);
+    out[base[b[j]] + r] = k[j];
+  }
+}
+
+// one workgroup per coarse bucket: split into LS_B2 fine buckets
+__global__ __launch_bounds__(256) void l2_kernel(const uint64_t* __restrict__ in,
+                                                 const unsigned long long* __restrict__ off1,
+                                                 int is_signed, const uint64_t* __restrict__ fine,
+                                                 unsigned long long* __restrict__ fine_off,
+                                                 uint64_t* __restrict__ out) {
+  __shared__ uint64_t spl[LS_B2];
+  __shared__ unsigned cnt[LS_B2];
+  __shared__ unsigned long long base[LS_B2];
+  const int b = blockIdx.x;
+  const size_t s = off1[b], e = off1[b + 1];
+  if (threadIdx.x < LS_B2) {
+    spl[threadIdx.x] = fine[size_t(b) * LS_B2 + threadIdx.x];
+    cnt[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  for (size_t i = s + threadIdx.x; i < e; i += 256)
+    atomicAdd(&cnt[bucket_of(spl, LS_B2, to_ordered(in[i], is_signed))], 1u);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long run = s;
+    for (int j = 0; j < LS_B2; j++) {
+      base[j] = run;
+      fine_off[size_t(b) * LS_B2 + j] = run;
+      run += cnt[j];
+      cnt[j] = 0;
+    }
+  }
+  __syncthreads();
+  for (size_t i = s + threadIdx.x; i < e; i += 256) {
+    const uint64_t k = in[i];
+    const int f = bucket_of(spl, LS_B2, to_ordered(k, is_signed));
+    out[base[f] + atomicAdd(&cnt[f], 1u)] = k;
+  }
+}
+
+// one workgroup per fine bucket: LSD radix sort in LDS of (key - min)
+__global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ in,
+                                                   const unsigned long long* __restrict__ fine_off,
+                                                   int is_signed, uint64_t* __restrict__ out,
+                                                   unsigned* __restrict__ overflow_list,
+                                                   unsigned* __restrict__ overflow_n) {
+  __shared__ uint64_t buf[2][L3_CAP];
+  __shared__ unsigned wcnt[L3_NT / 64][256];
+  __shared__ unsigned tstart[256];
+  __shared__ unsigned scan_tmp[L3_NT / 64];
+  __shared__ uint64_t red[2][L3_NT / 64];
+  const int f = blockIdx.x;
+  const size_t s = fine_off[f], e = fine_off[f + 1];
+  const unsigned m = unsigned(e - s);
+  if (e - s > size_t(L3_CAP)) {
+    if (threadIdx.x == 0) overflow_list[atomicAdd(overflow_n, 1u)] = unsigned(f);
+    return;
+  }
+  if (m == 0) return;
+  const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  // load, min/max of the ordered keys
+  uint64_t mn = ~0ull, mx = 0;
+  for (unsigned i = tid; i < m; i += L3_NT) {
+    uint64_t u = to_ordered(in[s + i], is_signed);
+    buf[0][i] = u;
+    mn = u < mn ? u : mn;
+    mx = u > mx ? u : mx;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint64_t a = __shfl_xor(mn, o, 64), c = __shfl_xor(mx, o, 64);
+    mn = a < mn ? a : mn;
+    mx = c > mx ? c : mx;
+  }
+  if (lane == 0) {
+    red[0][wave] = mn;
+    red[1][wave] = mx;
+  }
+  __syncthreads();
+  mn = red[0][0];
+  mx = red[1][0];
+  for (int w = 1; w < L3_NT / 64; w++) {
+    mn = red[0][w] < mn ? red[0][w] : mn;
+    mx = red[1][w] > mx ? red[1][w] : mx;
+  }
+  const uint64_t range = mx - mn;
+  const int bits = range ? 64 - __clzll(range) : 0;
+  const int passes = (bits + 7) / 8;
+  int cur = 0;
+  const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int p = 0; p < passes; p++) {
+    const int sh = 8 * p;
+    for (unsigned i = tid; i < (L3_NT / 64) * 256; i += L3_NT) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    unsigned rank[L3_ITEMS], dig[L3_ITEMS];
+#pragma unroll
+    for (int j = 0; j < L3_ITEMS; j++) {
+      const unsigned idx = wave * (64 * L3_ITEMS) + unsigned(j) * 64 + lane;
+      const bool valid = idx < m;
+      const unsigned d = valid ? unsigned(((buf[cur][idx] - mn) >> sh) & 0xFF) : 0u;
+      unsigned long long mk = __ballot(valid);
+#pragma unroll
+      for (int bb = 0; bb < 8; bb++) {
+        const bool bit = (d >> bb) & 1u;
+        unsigned long long bl = __ballot(bit);
+        mk &= bit ? bl : ~bl;
+      }
+      dig[j] = d;
+      rank[j] = 0;
+      if (valid) {
+        const unsigned c = unsigned(__popcll(mk)), below = unsigned(__popcll(mk & lt_mask));
+        const unsigned prior = wcnt[wave][d];
+        rank[j] = prior + below;
+        if (below == c - 1) wcnt[wave][d] = prior + c;
+      }
+    }
+    __syncthreads();
+    unsigned total = 0;
+#pragma unroll
+    for (int w = 0; w < L3_NT / 64; w++) {
+      const unsigned c = wcnt[w][tid];
+      wcnt[w][tid] = total;
+      total += c;
+    }
+    tstart[tid] = block_excl_scan<L3_NT>(total, scan_tmp, (unsigned*)nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < L3_ITEMS; j++) {
+      const unsigned idx = wave * (64 * L3_ITEMS) + unsigned(j) * 64 + lane;
+      if (idx < m) buf[cur ^ 1][tstart[dig[j]] + wcnt[wave][dig[j]] + rank[j]] = buf[cur][idx];
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (unsigned i = tid; i < m; i += L3_NT) out[s + i] = from_ordered(buf[cur][i], is_signed);
+}
+
+__global__ void to_feature_kernel(const uint64_t* __restrict__ sorted, size_t n, size_t step,
+                                  size_t S, int is_signed, uint64_t umin, double inv_span,
+                                  float* __restrict__ x, float* __restrict__ t) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x; k < S; k += stride) {
+    const size_t i = k * step;
+    const uint64_t u = to_ordered(sorted[i], is_signed);
+    x[k] = float(log2(1.0 + double(u - umin)) * inv_span);
+    t[k] = float(double(i) / double(n));
+  }
+}
+
+}  // namespace
+}  // namespace fhip
+
+// Trained and distilled CDF model of a 64-bit key column.
+struct fhip_cdf_model {
+  int ktype = FHIP_I64;
+  uint64_t umin = 0, umax = 0;       // ordered-key range of the training sample
+  double inv_span = 1.0;             // 1 / log2(1 + umax - umin)
+  std::vector<float> params;         // MLP parameters
+  std::vector<double> knots_cdf;     // distilled monotone CDF at K uniform x knots
+  std::vector<uint64_t> coarse;      // B1 coarse splitters (ordered keys)
+  std::vector<uint64_t> fine;        // B1*B2 fine splitters
+  std::vector<uint64_t> sample;      // ordered sample keys (sorted)
+  size_t train_n = 0;                // rows of the training input
+  double final_loss = 0;
+};
+
+namespace fhip {
+namespace {
+
+constexpr int KNOTS = 4096;
+
+uint64_t ordered_host(uint64_t k, bool sgn) { return sgn ? (k ^ 0x8000000000000000ull) : k; }
+
+// smallest ordered key whose model CDF reaches q
+uint64_t cdf_inverse(const fhip_cdf_model& m, double q) {
+  const std::vector<double>& c = m.knots_cdf;
+  size_t k = size_t(std::lower_bound(c.begin(), c.end(), q) - c.begin());
+  double x;
+  if (k == 0) x = 0.0;
+  else if (k >= c.size()) x = 1.0;
+  else {
+    const double c0 = c[k - 1], c1 = c[k];
+    const double fr = c1 > c0 ? (q - c0) / (c1 - c0) : 1.0;
+    x = (double(k - 1) + fr) / double(KNOTS - 1);
+  }
+  const double lg = x / m.inv_span;  // log2(1 + u - umin)
+  const double span = double(m.umax - m.umin);
+  double d = std::exp2(lg) - 1.0;
+  if (d < 0) d = 0;
+  if (d > span) d = span;
+  return m.umin + uint64_t(std::ceil(d));
+}
+
+void distill(fhip_cdf_model& m, const std::vector<float>& y) {
+  m.knots_cdf.assign(KNOTS, 0.0);
+  double run = 0.0;
+  for (int k = 0; k < KNOTS; k++) {
+    double v = std::min(1.0, std::max(0.0, double(y[k])));
+    run = std::max(run, v);  // monotone repair
+    m.knots_cdf[k] = run;
+  }
+  // pin the ends to the sample range
+  const double lo = m.knots_cdf.front(), hi = m.knots_cdf.back();
+  for (auto& v : m.knots_cdf) v = hi > lo ? (v - lo) / (hi - lo) : 0.0;
+  m.coarse.assign(LS_B1, 0);
+  m.coarse[0] = 0;
+  for (int b = 1; b < LS_B1; b++) {
+    uint64_t sp = cdf_inverse(m, double(b) / LS_B1);
+    m.coarse[b] = std::max(sp, m.coarse[b - 1]);
+  }
+  // leaves: sample quantiles inside each coarse bucket
+  m.fine.assign(size_t(LS_B1) * LS_B2, 0);
+  const std::vector<uint64_t>& s = m.sample;
+  for (int b = 0; b < LS_B1; b++) {
+    const uint64_t lo_k = m.coarse[b];
+    const uint64_t hi_k = b + 1 < LS_B1 ? m.coarse[b + 1] : ~0ull;
+    size_t a = size_t(std::lower_bound(s.begin(), s.end(), lo_k) - s.begin());
+    size_t z = b + 1 < LS_B1 ? size_t(std::lower_bound(s.begin(), s.end(), hi_k) - s.begin())
+                             : s.size();
+    m.fine[size_t(b) * LS_B2] = lo_k;
+    for (int j = 1; j < LS_B2; j++) {
+      uint64_t v = lo_k;
+      if (z > a) v = s[a + (z - a) * size_t(j) / LS_B2];
+      v = std::max(v, m.fine[size_t(b) * LS_B2 + j - 1]);
+      if (b + 1 < LS_B1) v = std::min(v, hi_k);
+      m.fine[size_t(b) * LS_B2 + j] = v;
+    }
+  }
+}
+
+}  // namespace
+
+fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t samples, int steps,
+                              float lr, hipStream_t st) {
+  FHIP_REQUIRE(ktype == FHIP_I64 || ktype == FHIP_U64, FHIP_ERR_TYPE,
+               "learned sort: 64-bit integer keys only");
+  FHIP_REQUIRE(n >= 2, FHIP_ERR_ARG, "learned sort: training input too small");
+  const bool sgn = ktype == FHIP_I64;
+  auto* m = new fhip_cdf_model();
+  m->ktype = ktype;
+  m->train_n = n;
+  try {
+    const size_t S = std::max<size_t>(2, std::min(samples, n));
+    const size_t step = n / S;
+    uint64_t first = 0, last = 0;
+    FHIP_CHECK(hipMemcpyAsync(&first, sorted, 8, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipMemcpyAsync(&last, static_cast<const uint64_t*>(sorted) + (n - 1), 8,
+                              hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    m->umin = ordered_host(first, sgn);
+    m->umax = ordered_host(last, sgn);
+    const double span = std::log2(1.0 + double(m->umax - m->umin));
+    m->inv_span = span > 0 ? 1.0 / span : 1.0;
+    // the network trains on at most 2^18 of the samples (full batch)
+    const size_t St = std::min<size_t>(S, size_t(1) << 18);
+    const size_t step_t = n / St;
+    DBuf xb(St * 4), tb(St * 4);
+    to_feature_kernel<<<grid_for_n(St), 256, 0, st>>>(static_cast<const uint64_t*>(sorted), n,
+                                                      step_t, St, sgn ? 1 : 0, m->umin,
+                                                      m->inv_span, xb.as<float>(), tb.as<float>());
+    FHIP_CHECK(hipGetLastError());
+    // the sample keys themselves (for the leaves)
+    std::vector<uint64_t> host_sorted_sample(S);
+    FHIP_CHECK(hipMemcpy2DAsync(host_sorted_sample.data(), 8, sorted, step * 8, 8, S,
+                                hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    m->sample.resize(S);
+    for (size_t i = 0; i < S; i++) m->sample[i] = ordered_host(host_sorted_sample[i], sgn);
+    // initialise: x spread over [0,1), ReLU units with positive slopes
+    const int P = kMlpParams;
+    std::vector<float> p(P, 0.f);
+    uint64_t rs = 0x243F6A8885A308D3ull;
+    auto rnd = [&]() {  // splitmix64 -> [-1, 1)
+      rs += 0x9E3779B97F4A7C15ull;
+      uint64_t z = rs;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      return float(double(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+    };
+    for (int j = 0; j < 64; j++) {
+      p[j] = 1.0f;                              // W1
+      p[64 + j] = -float(j) / 64.f;             // b1: hinge at x = j/64
+    }
+    for (int i = 0; i < 64 * 64; i++) p[128 + i] = rnd() * 0.125f;
+    for (int j = 0; j < 64; j++) p[128 + 4096 + 64 + j] = rnd() * 0.125f;
+    DBuf pb(P * 4), gb(P * 4), mb(P * 4), vb(P * 4), lb(4);
+    FHIP_CHECK(hipMemcpyAsync(pb.get(), p.data(), P * 4, hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipMemsetAsync(gb.get(), 0, P * 4, st));
+    FHIP_CHECK(hipMemsetAsync(mb.get(), 0, P * 4, st));
+    FHIP_CHECK(hipMemsetAsync(vb.get(), 0, P * 4, st));
+    {
+      ProfScope prof("mlp_train", st);
+      for (int it = 1; it <= steps; it++) {
+        FHIP_CHECK(hipMemsetAsync(lb.get(), 0, 4, st));
+        mlp_grad(pb.as<float>(), xb.as<float>(), tb.as<float>(), St, gb.as<float>(),
+                 lb.as<float>(), st);
+        mlp_adam(pb.as<float>(), gb.as<float>(), mb.as<float>(), vb.as<float>(), it, lr, st);
+      }
+    }
+    float loss = 0;
+    FHIP_CHECK(hipMemcpyAsync(&loss, lb.get(), 4, hipMemcpyDeviceToHost, st));
+    // distil: forward at the knots
+    std::vector<float> xk(KNOTS), yk(KNOTS);
+    for (int k = 0; k < KNOTS; k++) xk[k] = float(double(k) / (KNOTS - 1));
+    DBuf xkb(KNOTS * 4), ykb(KNOTS * 4);
+    FHIP_CHECK(hipMemcpyAsync(xkb.get(), xk.data(), KNOTS * 4, hipMemcpyHostToDevice, st));
+    mlp_forward(pb.as<float>(), xkb.as<float>(), KNOTS, ykb.as<float>(), st);
+    FHIP_CHECK(hipMemcpyAsync(yk.data(), ykb.get(), KNOTS * 4, hipMemcpyDeviceToHost, st));
+    m->params.resize(P);
+    FHIP_CHECK(hipMemcpyAsync(m->params.data(), pb.get(), P * 4, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    m->final_loss = loss;
+    distill(*m, yk);
+  } catch (...) {
+    delete m;
+    throw;
+  }
+  return m;
+}
+
+// Sorts n keys in place; returns the number of keys that went through the
+// radix fallback (n if the whole input did).  equal_width: replace the
+// model's coarse splitters by equal-width ranges of the ordered key space
+// (the same pipeline without the network).
+size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_width,
+                    size_t* fallback_buckets, hipStream_t st) {
+  if (fallback_buckets) *fallback_buckets = 0;
+  if (n < 2) return 0;
+  const bool sgn = m->ktype == FHIP_I64;
+  std::vector<uint64_t> coarse = m->coarse, fine = m->fine;
+  if (equal_width) {
+    const double lo = double(m->umin), hi = double(m->umax);
+    for (int b = 0; b < LS_B1; b++)
+      coarse[b] = b == 0 ? 0 : uint64_t(lo + (hi - lo) * double(b) / LS_B1);
+    for (int b = 0; b < LS_B1; b++) {
+      const uint64_t a = coarse[b], z = b + 1 < LS_B1 ? coarse[b + 1] : m->umax;
+      for (int j = 0; j < LS_B2; j++)
+        fine[size_t(b) * LS_B2 + j] = j == 0 ? a : a + uint64_t(double(z - a) * j / LS_B2);
+    }
+  }
+  DBuf sb((coarse.size() + fine.size()) * 8);
+  uint64_t* dco = sb.as<uint64_t>();
+  uint64_t* dfi = dco + coarse.size();
+  FHIP_CHECK(hipMemcpyAsync(dco, coarse.data(), coarse.size() * 8, hipMemcpyHostToDevice, st));
+  FHIP_CHECK(hipMemcpyAsync(dfi, fine.data(), fine.size() * 8, hipMemcpyHostToDevice, st));
+  const size_t nf = size_t(LS_B1) * LS_B2;
+  DBuf hb((LS_B1 + 1) * 8 * 3), fob((nf + 1) * 8), t1(n * 8), t2(n * 8), ovb((nf + 1) * 4);
+  unsigned long long* hist = hb.as<unsigned long long>();
+  unsigned long long* off1 = hist + (LS_B1 + 1);
+  unsigned long long* cursor = off1 + (LS_B1 + 1);
+  unsigned long long* fine_off = fob.as<unsigned long long>();
+  FHIP_CHECK(hipMemsetAsync(hist, 0, (LS_B1 + 1) * 8, st));
+  FHIP_CHECK(hipMemsetAsync(ovb.get(), 0, 4, st));
+  uint64_t* k = static_cast<uint64_t*>(keys);
+  {
+    ProfScope prof("learned_count", st);
+    const int grid = int(std::min<size_t>(ceil_div(n, LS_NT * 8), size_t(fill_blocks(LS_NT, 2))));
+    l1_count_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, hist);
+    FHIP_CHECK(hipGetLastError());
+  }
+  // exclusive offsets of the coarse buckets (hist[B1] = 0 -> off1[B1] = n)
+  {
+    std::vector<unsigned long long> h(LS_B1 + 1), o(LS_B1 + 1);
+    FHIP_CHECK(hipMemcpyAsync(h.data(), hist, LS_B1 * 8, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    unsigned long long run = 0;
+    for (int b = 0; b < LS_B1; b++) {
+      o[b] = run;
+      run += h[b];
+    }
+    o[LS_B1] = run;
+    FHIP_CHECK(hipMemcpyAsync(off1, o.data(), (LS_B1 + 1) * 8, hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipMemcpyAsync(cursor, o.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipStreamSynchronize(st));  // o/h are host locals
+  }
+  {
+    ProfScope prof("learned_scatter", st);
+    const unsigned grid = unsigned(ceil_div(n, size_t(LS_NT) * LS_ITEMS));
+    l1_scatter_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, cursor, t1.as<uint64_t>());
+    FHIP_CHECK(hipGetLastError());
+  }
+  {
+    ProfScope prof("learned_split", st);
+    l2_kernel<<<LS_B1, 256, 0, st>>>(t1.as<uint64_t>(), off1, sgn, dfi, fine_off,
+                                     t2.as<uint64_t>());
+    FHIP_CHECK(hipGetLastError());
+    FHIP_CHECK(hipMemcpyAsync(fine_off + nf, &n, 8, hipMemcpyHostToDevice, st));
+  }
+  {
+    ProfScope prof("learned_leaf_sort", st);
+    l3_kernel<<<unsigned(nf), L3_NT, 0, st>>>(t2.as<uint64_t>(), fine_off, sgn, k,
+                                              ovb.as<unsigned>() + 1, ovb.as<unsigned>());
+    FHIP_CHECK(hipGetLastError());
+  }
+  unsigned nov = 0;
+  FHIP_CHECK(hipMemcpyAsync(&nov, ovb.get(), 4, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  if (fallback_buckets) *fallback_buckets = nov;
+  if (nov == 0) return 0;
+  // radix fallback for the fine buckets that did not fit
+  std::vector<unsigned> ids(nov);
+  std::vector<unsigned long long> fo(nf + 1);
+  FHIP_CHECK(hipMemcpyAsync(ids.data(), ovb.as<unsigned>() + 1, nov * 4, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipMemcpyAsync(fo.data(), fine_off, (nf + 1) * 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  size_t fb_keys = 0;
+  for (unsigned f : ids) fb_keys += size_t(fo[f + 1] - fo[f]);
+  if (fb_keys * 4 > n || nov > 256) {
+    // too much of the input missed the model: plain radix sort of all keys
+    sort_pairs(m->ktype, t2.get(), k, 0, nullptr, nullptr, n, false, false, st);
+    return n;
+  }
+  for (unsigned f : ids) {
+    const size_t a = size_t(fo[f]), z = size_t(fo[f + 1]);
+    sort_pairs(m->ktype, t2.as<uint64_t>() + a, k + a, 0, nullptr, nullptr, z - a, false, false, st);
+  }
+  return fb_keys;
+}
+
+// Expected number of distinct keys among n draws from the model's CDF:
+// sum over knot intervals of w * (1 - exp(-n * mass / w)) with w the number
+// of integer keys in the interval.
+double learned_cardinality(const fhip_cdf_model* m, size_t n) {
+  double total = 0.0;
+  uint64_t prev_key = m->umin;
+  double prev_c = 0.0;
+  for (int k = 1; k < KNOTS; k++) {
+    const double x = double(k) / (KNOTS - 1);
+    const double d = std::min(std::exp2(x / m->inv_span) - 1.0, double(m->umax - m->umin));
+    const uint64_t key = m->umin + uint64_t(d);
+    const double c = m->knots_cdf[k];
+    const double mass = std::max(0.0, c - prev_c);
+    const double w = double(key - prev_key) + (k == 1 ? 1.0 : 0.0);
+    if (w > 0 && mass > 0) total += w * (1.0 - std::exp(-double(n) * mass / w));
+    prev_key = key;
+    prev_c = c;
+  }
+  return std::min(total, double(n));
+}
+
+}  // namespace fhip
+
+extern "C" {
+
+int fhip_learned_train(int ktype, const void* sorted_keys, size_t n, size_t samples, int steps,
+                       float lr, fhip_cdf_model** model, void* stream) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(model && sorted_keys, FHIP_ERR_ARG, "learned_train: null pointer");
+    *model = fhip::learned_train(ktype, sorted_keys, n, samples, steps, lr,
+                                 fhip::as_stream(stream));
+  });
+}
+
+int fhip_learned_model_info(const fhip_cdf_model* m, double* final_loss, double* cdf_knots,
+                            uint64_t* coarse_splitters) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(m, FHIP_ERR_ARG, "learned_model_info: null model");
+    if (final_loss) *final_loss = m->final_loss;
+    if (cdf_knots) std::copy(m->knots_cdf.begin(), m->knots_cdf.end(), cdf_knots);
+    if (coarse_splitters) std::copy(m->coarse.begin(), m->coarse.end(), coarse_splitters);
+  });
+}
+
+int fhip_learned_model_params(const fhip_cdf_model* m, float* params) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(m && params, FHIP_ERR_ARG, "learned_model_params: null pointer");
+    std::copy(m->params.begin(), m->params.end(), params);
+  });
+}
+
+int fhip_dev_learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int equal_width,
+                          size_t* fallback_keys, size_t* fallback_buckets, void* stream) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(m && (n == 0 || keys), FHIP_ERR_ARG, "learned_sort: null pointer");
+    size_t fb = fhip::learned_sort(m, keys, n, equal_width != 0, fallback_buckets,
+                                   fhip::as_stream(stream));
+    if (fallback_keys) *fallback_keys = fb;
+  });
+}
+
+int fhip_learned_cardinality(const fhip_cdf_model* m, size_t n, double* distinct) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(m && distinct, FHIP_ERR_ARG, "learned_cardinality: null pointer");
+    *distinct = fhip::learned_cardinality(m, n);
+  });
+}
+
+int fhip_learned_model_destroy(fhip_cdf_model* m) {
+  return fhip::guarded([&] { delete m; });
+}
+
+}  // extern "C"

@@ -364,12 +364,14 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   unsigned long long* gbase = ghist + 8 * 256;
   FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
   int grid = int(std::min<size_t>(ceil_div(n, RS_NT * 4), size_t(fill_blocks(RS_NT, 4))));
-  ProfScope prof("radix_hist", st);
-  if (kbytes == 4)
-    rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist);
-  else
-    rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, mode, desc, nbytes, ghist);
-  FHIP_CHECK(hipGetLastError());
+  {
+    ProfScope prof("radix_hist", st);
+    if (kbytes == 4)
+      rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist);
+    else
+      rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, mode, desc, nbytes, ghist);
+    FHIP_CHECK(hipGetLastError());
+  }
   rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
   FHIP_CHECK(hipGetLastError());
   std::vector<unsigned long long> h(size_t(nbytes) * 256);

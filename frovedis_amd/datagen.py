@@ -133,3 +133,42 @@ def normal_dev(n, start=0, seed=SEED_NORMAL, device="cuda", chunk=1 << 25):
 def radix_keys_dev(n, seed=SEED_RADIX, device="cuda"):
     """Full-range uint64 keys (int64 carrier) for the radix kv config."""
     return splitmix64_dev(seed, 0, n, device)
+
+
+# ------------------------------------------------ lognormal mixture (learned sort)
+# 8 components: weight, mean and sigma of ln(key)
+LOGN_WEIGHTS = (0.20, 0.15, 0.15, 0.10, 0.10, 0.10, 0.10, 0.10)
+LOGN_MU = (4.0, 8.0, 12.0, 16.0, 20.0, 24.0, 28.0, 34.0)
+LOGN_SIGMA = (0.5, 1.0, 1.5, 0.8, 2.0, 1.2, 0.6, 1.5)
+RUN_LEN = 4096
+RUN_FRACTION = 0.05
+
+
+def lognormal_mixture_dev(n, seed=SEED_LOGN, device="cuda", chunk=1 << 25, runs=True):
+    """int64 keys: 95% floor(exp(N(mu, sigma))) from the 8-component mixture,
+    clipped to [0, 2^62); 5% of the RUN_LEN-aligned runs are pre-sorted."""
+    torch = _torch()
+    cw = torch.tensor(np.cumsum(LOGN_WEIGHTS), dtype=torch.float64, device=device)
+    cw[-1] = 1.0
+    mu = torch.tensor(LOGN_MU, dtype=torch.float64, device=device)
+    sg = torch.tensor(LOGN_SIGMA, dtype=torch.float64, device=device)
+    out = torch.empty(n, dtype=torch.int64, device=device)
+    lim = float(2 ** 62 - 1024)
+    for b in range(0, n, chunk):
+        m = min(chunk, n - b)
+        u = uniform01_dev(seed, 3 * b, 3 * m, device)
+        comp = torch.searchsorted(cw, u[0::3], right=True).clamp_(max=7)
+        z = torch.sqrt(-2.0 * torch.log(1.0 - u[1::3])) * torch.cos(2.0 * math.pi * u[2::3])
+        v = torch.exp(mu[comp] + sg[comp] * z).clamp_(0.0, lim)
+        out[b:b + m] = torch.floor(v).to(torch.int64)
+        del u, comp, z, v
+    if runs and n >= RUN_LEN:
+        nruns = n // RUN_LEN
+        pick = splitmix64_dev(seed + 1, 0, nruns, device)
+        sel = torch.nonzero(_srl(torch, pick, 11).to(torch.float64) * (2.0 ** -53)
+                            < RUN_FRACTION).flatten()
+        view = out[: nruns * RUN_LEN].view(nruns, RUN_LEN)
+        for c in range(0, len(sel), 1024):
+            rows = sel[c:c + 1024]
+            view[rows] = torch.sort(view[rows], dim=1).values
+    return out

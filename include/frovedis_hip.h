@@ -126,9 +126,10 @@ int fhip_dev_group_by(int ktype, const void* col, size_t* idx, size_t n,
 /* Per-group aggregate of one column over (grouped_idx, split) from
  * fhip_dev_group_by (dataframe/dfcolumn_grouped_agg.cc: sum :105-275,
  * max :278-510, min :513-717, count :719-837, avg :839-889; one rank).
- * `nulls` lists null row positions (may be NULL when nnull == 0).  Output
+ * col holds col_n rows; grouped_idx (n entries) indexes it.  `nulls` lists
+ * null row positions of the column (may be NULL when nnull == 0).  Output
  * types: sum/max/min in vtype, count as uint64, avg as double. */
-int fhip_dev_grouped_agg(int op, int vtype, const void* col,
+int fhip_dev_grouped_agg(int op, int vtype, const void* col, size_t col_n,
                          const size_t* grouped_idx, size_t n,
                          const size_t* split, size_t nsplit,
                          const size_t* nulls, size_t nnull, void* out,
@@ -151,6 +152,109 @@ int fhip_dev_group_by_agg(int ktype, const void* keys, size_t n,
                           const fhip_agg* aggs, int naggs, void* keys_out,
                           size_t capacity, size_t* ngroups, size_t table_hint,
                           void* stream);
+
+/* ---- unique_hashtable<K, size_t> -----------------------------------------
+ * Same layout and results as dataframe/hashtable.hpp: table size = smallest
+ * listed prime >= 3n, home slot = key bits % size, last row owns a slot,
+ * other keys go to a sorted conflict list.  Build replaces the (k, v) and
+ * (k, v, int& is_unique_ok) constructors (:1167-1322; vals NULL gives the
+ * set constructor :1324-1384); lookup replaces lookup(k, miss) (:1442-1515:
+ * values of missed keys are 0, misses are returned as sorted row indices);
+ * check_existence (:1517-1582); all_keys (:1584-1607: owned slots in slot
+ * order, then the conflict keys).  miss_out and all_keys' out hold n (resp.
+ * build n) entries. */
+typedef struct fhip_ht fhip_ht;
+int fhip_dev_ht_build(int ktype, const void* keys, const uint64_t* vals,
+                      size_t n, int check_unique, int* is_unique_ok,
+                      fhip_ht** ht, void* stream);
+int fhip_dev_ht_lookup(const fhip_ht* ht, const void* keys, size_t n,
+                       uint64_t* vals_out, size_t* miss_out, size_t* miss_n,
+                       void* stream);
+int fhip_dev_ht_check_existence(const fhip_ht* ht, const void* keys, size_t n,
+                                int* out, void* stream);
+int fhip_dev_ht_all_keys(const fhip_ht* ht, void* out, size_t* out_n,
+                         void* stream);
+int fhip_ht_info(const fhip_ht* ht, size_t* table_size, size_t* conflict_size);
+int fhip_ht_destroy(fhip_ht* ht);
+
+/* ---- equi_join ------------------------------------------------------------
+ * Inner equi-join producing (left_idx, right_idx) pairs, equi_join<T>
+ * (dataframe/join.hpp:304-326, VE body; shrink_missed join.cc:10-61;
+ * multi_equi_join join.hpp:198-302).  AUTO = the reference's choice (hash
+ * join when the right keys are unique, output in left order; else
+ * sort-merge, output in (key, left, right) order).  Outputs hold `capacity`
+ * pairs; *nout receives the pair count (also when it does not fit, with
+ * status FHIP_ERR_TOO_LARGE).  *used_hash (may be NULL) tells which path
+ * ran.  The inputs are not modified. */
+#define FHIP_JOIN_AUTO 0
+#define FHIP_JOIN_HASH 1
+#define FHIP_JOIN_SORTMERGE 2
+int fhip_dev_equi_join(int ktype, const void* left, const size_t* left_idx,
+                       size_t nl, const void* right, const size_t* right_idx,
+                       size_t nr, int strategy, size_t* left_out,
+                       size_t* right_out, size_t capacity, size_t* nout,
+                       int* used_hash, void* stream);
+
+/* ---- host-pointer forms (std::vector storage) ------------------------------
+ * Same semantics as the dev_ forms above; inputs are copied to the device
+ * and results back.  col_n is the length of the column that idx / gidx
+ * index into. */
+int fhip_ht_build(int ktype, const void* keys, const uint64_t* vals, size_t n,
+                  int check_unique, int* is_unique_ok, fhip_ht** ht);
+int fhip_ht_lookup(const fhip_ht* ht, const void* keys, size_t n,
+                   uint64_t* vals_out, size_t* miss_out, size_t* miss_n);
+int fhip_ht_check_existence(const fhip_ht* ht, const void* keys, size_t n,
+                            int* out);
+int fhip_ht_all_keys(const fhip_ht* ht, void* out, size_t* out_n);
+int fhip_equi_join(int ktype, const void* left, const size_t* left_idx,
+                   size_t nl, const void* right, const size_t* right_idx,
+                   size_t nr, int strategy, size_t* left_out, size_t* right_out,
+                   size_t capacity, size_t* nout, int* used_hash);
+int fhip_group_by(int ktype, const void* col, size_t col_n, size_t* idx,
+                  size_t n, size_t* split, size_t* nsplit, void* keys_out);
+int fhip_grouped_agg(int op, int vtype, const void* col, size_t col_n,
+                     const size_t* grouped_idx, size_t n, const size_t* split,
+                     size_t nsplit, const size_t* nulls, size_t nnull,
+                     void* out);
+
+/* ---- learned CDF model and learned partition sort -------------------------
+ * No counterpart in the reference.  The output of fhip_dev_learned_sort is
+ * that of radix_sort(K*, size_t) (core/radix_sort.hpp:221-242) on the same
+ * keys (64-bit integer keys, key only).
+ *  - fhip_learned_train: FP32 MLP 1->64->64->1 trained on the GPU (Adam,
+ *    `steps` full-batch steps, learning rate lr) on (key, rank) samples of an
+ *    already sorted device array (every n/samples-th key), then distilled
+ *    into 4096 coarse splitter keys plus 16 sample-quantile leaves each.
+ *  - fhip_dev_learned_sort: sorts in place; *fallback_keys gets how many
+ *    keys went through the plain radix sort because their fine bucket
+ *    overflowed (n when the whole input fell back).  equal_width = 1 runs
+ *    the same pipeline with equal-width coarse buckets instead of the
+ *    network (for comparison).
+ *  - fhip_learned_cardinality: expected distinct keys among n draws, from
+ *    the model (sizes group-by tables, picks join strategies).
+ *  - fhip_dev_mlp_*: the network's kernels (gradient of the mean squared
+ *    error, forward pass, Adam step) on a parameter vector of
+ *    fhip_mlp_nparam() floats. */
+typedef struct fhip_cdf_model fhip_cdf_model;
+int fhip_learned_train(int ktype, const void* sorted_keys, size_t n,
+                       size_t samples, int steps, float lr,
+                       fhip_cdf_model** model, void* stream);
+int fhip_dev_learned_sort(const fhip_cdf_model* m, void* keys, size_t n,
+                          int equal_width, size_t* fallback_keys,
+                          size_t* fallback_buckets, void* stream);
+int fhip_learned_cardinality(const fhip_cdf_model* m, size_t n,
+                             double* distinct);
+int fhip_learned_model_info(const fhip_cdf_model* m, double* final_loss,
+                            double* cdf_knots, uint64_t* coarse_splitters);
+int fhip_learned_model_params(const fhip_cdf_model* m, float* params);
+int fhip_learned_model_destroy(fhip_cdf_model* m);
+int fhip_mlp_nparam(void);
+int fhip_dev_mlp_grad(const float* params, const float* x, const float* t,
+                      size_t S, float* grad, float* loss, void* stream);
+int fhip_dev_mlp_forward(const float* params, const float* x, size_t S,
+                         float* y, void* stream);
+int fhip_dev_mlp_adam(float* params, float* grad, float* m, float* v,
+                      int step, float lr, void* stream);
 
 #ifdef __cplusplus
 }

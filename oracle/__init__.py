@@ -183,9 +183,11 @@ def group_by(col, idx=None):
     """(grouped_idx, split, group_keys): dataframe/dfcolumn_groupby.cc:16-105
     on one rank."""
     c = np.ascontiguousarray(col)
-    n = len(c)
-    gi = (np.arange(n, dtype=np.uint64) if idx is None
+    gi = (np.arange(len(c), dtype=np.uint64) if idx is None
           else np.ascontiguousarray(idx, dtype=np.uint64).copy())
+    if len(gi) and int(gi.max()) >= len(c):
+        raise IndexError("group_by: index beyond the column")
+    n = len(gi)
     split = np.empty(n + 1, np.uint64)
     keys = np.empty(max(n, 1), c.dtype)
     m = lib().orc_group_by(TYPE_CODE[c.dtype], _p(c), _p(gi), n, _p(split),

@@ -49,3 +49,18 @@ def test_package_imports_and_fails_loudly_without_library(tmp_path, monkeypatch)
     monkeypatch.setattr(frovedis_amd, "_lib", None)
     with pytest.raises(frovedis_amd.FhipError):
         frovedis_amd.lib()
+
+
+def _build_shim(out):
+    import subprocess
+    cmd = ["g++", "-std=c++11", "-O2", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "cpp", "shim_test.cpp"),
+           "-L", os.path.join(ROOT, "frovedis_amd"), "-lfrovedis_hip",
+           "-Wl,-rpath," + os.path.join(ROOT, "frovedis_amd"), "-o", str(out)]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+
+
+def test_cpp_dropin_headers_compile_as_cxx11(libpath, tmp_path):
+    # the reference builds with -std=c++11 (Makefile.in.x86:3-4)
+    _build_shim(tmp_path / "shim_test")
+    assert (tmp_path / "shim_test").exists()

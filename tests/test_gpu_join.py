@@ -1,0 +1,150 @@
+"""GPU parity of unique_hashtable and equi_join against the CPU oracle.
+
+The GPU table keeps the reference's layout, so lookups, miss lists,
+check_existence, is_unique_ok, the conflict count and the all_keys order
+are compared bit-exact.  Join pairs are compared in the reference's own
+orders (probe order for the hash join, (key, left, right) for sort-merge),
+and canonically against the x86 multimap body.
+"""
+import csv
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def fa():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import frovedis_amd
+    frovedis_amd.lib()
+    return frovedis_amd
+
+
+def _keys(rng, dt, n, unique):
+    dt = np.dtype(dt)
+    hi = 4 * max(n, 1)
+    if unique:
+        k = rng.choice(hi, n, replace=False)
+    else:
+        k = rng.integers(0, max(n // 3, 1), n)
+    k = k.astype(np.int64)
+    if dt.kind == "i" and n:
+        k[::5] = -k[::5] - 1
+    if dt.kind == "f":
+        return (k.astype(np.float64) * 0.5).astype(dt)
+    return k.astype(dt)
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.uint32, np.int64, np.uint64, np.float32, np.float64])
+@pytest.mark.parametrize("unique", [True, False])
+def test_hashtable_matches_oracle(fa, dt, unique):
+    rng = np.random.default_rng(31)
+    for n in [0, 1, 100, 5000, 300000]:
+        k = _keys(rng, dt, n, unique)
+        v = rng.integers(0, 1 << 62, n).astype(np.uint64)
+        ref = oracle.UniqueHashtable(k, v, check_unique=True)
+        ht = fa.UniqueHashtable(fa.DeviceArray.from_numpy(k), fa.DeviceArray.from_numpy(v),
+                                check_unique=True)
+        assert ht.table_size == ref.table_size
+        assert ht.conflict_size == ref.conflict_size
+        assert ht.is_unique_ok == ref.is_unique_ok
+        q = np.concatenate([k, _keys(rng, dt, n // 2 + 3, True)])
+        rng.shuffle(q)
+        dq = fa.DeviceArray.from_numpy(q)
+        gv, gm = ht.lookup(dq)
+        ev, em = ref.lookup(q)
+        assert np.array_equal(gm.numpy(), em), n
+        if ref.is_unique_ok:
+            assert np.array_equal(gv.numpy(), ev), n
+        assert np.array_equal(ht.check_existence(dq).numpy(), ref.check_existence(q))
+        assert np.array_equal(ht.all_keys().numpy().view(np.uint8), ref.all_keys().view(np.uint8))
+
+
+def test_hashtable_set_constructor(fa):
+    rng = np.random.default_rng(32)
+    k = rng.integers(0, 1000, 20000).astype(np.int64)
+    ref = oracle.UniqueHashtable(k)
+    ht = fa.UniqueHashtable(fa.DeviceArray.from_numpy(k))
+    assert ht.conflict_size == ref.conflict_size
+    q = rng.integers(-10, 1200, 5000).astype(np.int64)
+    assert np.array_equal(ht.check_existence(fa.DeviceArray.from_numpy(q)).numpy(),
+                          ref.check_existence(q))
+    assert np.array_equal(ht.all_keys().numpy(), ref.all_keys())
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int64, np.uint64, np.float64])
+@pytest.mark.parametrize("unique_right", [True, False])
+def test_equi_join_matches_oracle(fa, dt, unique_right):
+    rng = np.random.default_rng(33)
+    for nl, nr in [(0, 10), (10, 0), (1, 1), (3000, 2000), (200000, 100000)]:
+        r = _keys(rng, dt, nr, unique_right)
+        l = _keys(rng, dt, nl, False) if nl else np.zeros(0, dt)
+        if nl and nr:
+            l[: nl // 2] = rng.choice(r, nl // 2)
+        li = rng.permutation(nl).astype(np.uint64) + np.uint64(7)
+        ri = rng.permutation(nr).astype(np.uint64) + np.uint64(100000)
+        elo, ero, eh = oracle.equi_join(l, li, r, ri, "ve")
+        dl, dr = fa.DeviceArray.from_numpy(l), fa.DeviceArray.from_numpy(r)
+        dli, dri = fa.DeviceArray.from_numpy(li), fa.DeviceArray.from_numpy(ri)
+        lo, ro, used = fa.equi_join(dl, dli, dr, dri)
+        assert used == eh, (nl, nr)
+        assert np.array_equal(lo.numpy(), elo), (nl, nr)
+        assert np.array_equal(ro.numpy(), ero), (nl, nr)
+        # forced sort-merge gives the sort-merge order on any input
+        slo, sro, used = fa.equi_join(dl, dli, dr, dri, strategy="sortmerge")
+        assert not used
+        got = sorted(zip(slo.numpy().tolist(), sro.numpy().tolist()))
+        assert got == sorted(zip(elo.tolist(), ero.tolist()))
+        # x86 body (std::unordered_multimap): same pair set
+        xlo, xro, _ = oracle.equi_join(l, li, r, ri, "x86")
+        assert got == sorted(zip(xlo.tolist(), xro.tolist()))
+
+
+def test_tutorial_join_on_gpu(fa):
+    t1 = list(csv.reader(open(os.path.join(GOLD, "tut5_5_t1.csv"))))
+    t2 = list(csv.reader(open(os.path.join(GOLD, "tut5_5_t2.csv"))))
+    kat = json.load(open(os.path.join(GOLD, "tutorial_known_answers.json")))
+    t1c3 = fa.DeviceArray.from_numpy(np.array([int(r[2]) for r in t1], np.int32))
+    t2c1 = fa.DeviceArray.from_numpy(np.array([int(r[0]) for r in t2], np.int32))
+    lo, ro, used = fa.equi_join(t1c3, fa.DeviceArray.from_numpy(np.arange(8, dtype=np.uint64)),
+                                t2c1, fa.DeviceArray.from_numpy(np.arange(4, dtype=np.uint64)))
+    assert not used  # t2c1 has a duplicated key: sort-merge path
+    rows = sorted([int(t1[a][0]), int(t1[a][1]), int(t1[a][2]), int(t2[b][0]), t2[b][1]]
+                  for a, b in zip(lo.numpy(), ro.numpy()))
+    assert rows == sorted(kat["hash_join_t1c3_eq_t2c1_rows"])
+
+
+def test_equi_join_scaled_join_config_properties(fa):
+    # the join configuration at 1/4 scale: unique build side = a permutation
+    # of 0..nb-1, probe side 90% hits and 10% out-of-range misses
+    import torch
+    from frovedis_amd import datagen as dg
+    nb, npb = 1 << 25, 1 << 27
+    build = torch.argsort(dg.uniform01_dev(dg.SEED_PERM, 0, nb))
+    u = dg.uniform01_dev(dg.SEED_FK, 0, npb)
+    z = dg.splitmix64_dev(dg.SEED_FK + 1, 0, npb)
+    pk = torch.where(u < 0.9, dg._srl(torch, z, 1) % nb, nb + dg._srl(torch, z, 1) % nb)
+    bk = fa.DeviceArray(build.to(torch.int64), np.int64)
+    pkd = fa.DeviceArray(pk, np.int64)
+    bi = fa.DeviceArray(torch.arange(nb, device="cuda", dtype=torch.int64), np.uint64)
+    pi = fa.DeviceArray(torch.arange(npb, device="cuda", dtype=torch.int64), np.uint64)
+    lo, ro, used = fa.equi_join(pkd, pi, bk, bi)  # probe = left, build = right
+    assert used
+    hits = int((pk < nb).sum())
+    assert len(lo) == hits
+    assert torch.equal(pk[lo.t], build[ro.t])
+    assert bool((lo.t[1:] > lo.t[:-1]).all())  # probe order
+    slo, sro, used = fa.equi_join(bk, bi, pkd, pi, strategy="sortmerge")
+    assert len(slo) == hits
+    assert torch.equal(build[slo.t], pk[sro.t])
+    kk = build[slo.t]
+    assert bool((kk[1:] >= kk[:-1]).all())  # sorted by key

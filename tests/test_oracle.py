@@ -321,3 +321,14 @@ def test_grouped_agg_nulls():
     assert oracle.grouped_agg("count", col, gi, split, nulls).tolist() == [1, 0]
     assert oracle.grouped_agg("max", col, gi, split, nulls).tolist() == [1.0, mx]
     assert oracle.grouped_agg("avg", col, gi, split, nulls).tolist() == [1.0, mx]
+
+
+def test_group_by_with_local_idx_closed_form():
+    # the key column is read through local_idx (dfcolumn_groupby.cc:39)
+    rng = np.random.default_rng(12)
+    col = rng.integers(0, 100, 50000).astype(np.int64)
+    idx = np.sort(rng.choice(50000, 20000, replace=False)).astype(np.uint64)
+    gi, split, keys = oracle.group_by(col, idx)
+    o = np.argsort(col[idx.astype(np.int64)], kind="stable")
+    assert np.array_equal(gi, idx[o])
+    assert np.array_equal(keys, np.unique(col[idx.astype(np.int64)]))
