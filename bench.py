@@ -117,6 +117,87 @@ def bench_radix(fa, log2n, steps=3):
             "hbm_frac": algo / kern_s / HBM_PEAK if kern_s else None}
 
 
+def bench_learned(fa, log2n, steps=3):
+    """Learned-CDF partition sort vs the plain radix sort on the same keys:
+    2^log2n int64 keys from the 8-component lognormal mixture with 5%
+    pre-sorted runs; the model is trained on a 1% sample of the plain sort
+    of an earlier input (another seed).  Outputs are compared bit-exact."""
+    import numpy as np
+    import torch
+
+    from frovedis_amd import datagen as dg
+    n = 1 << log2n
+    prev = fa.DeviceArray(dg.lognormal_mixture_dev(n, seed=dg.SEED_LOGN + 1), np.int64)
+    fa.radix_sort(prev)
+    t0 = time.perf_counter()
+    model = fa.CdfModel(prev, steps=300)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del prev
+    k0 = dg.lognormal_mixture_dev(n)
+    keys = fa.DeviceArray(k0.clone(), np.int64)
+
+    def timed(fn):
+        best = float("inf")
+        for _ in range(steps):
+            keys.t.copy_(k0)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best, r
+
+    fa.radix_sort(keys)
+    plain_s, _ = timed(lambda: fa.radix_sort(keys))
+    ref = keys.t.clone()
+    learned_s, fb = timed(lambda: model.sort(keys))
+    exact = bool(torch.equal(keys.t, ref))
+    flat_s, fb_flat = timed(lambda: model.sort(keys, equal_width=True))
+    del ref
+    return {"keys": n, "plain_ms": plain_s * 1e3, "learned_ms": learned_s * 1e3,
+            "speedup": plain_s / learned_s, "bit_exact": exact,
+            "fallback_keys": fb[0], "fallback_buckets": fb[1], "max_coarse_bucket": fb[2],
+            "equal_width_ms": flat_s * 1e3, "equal_width_fallback_keys": fb_flat[0],
+            "equal_width_max_coarse_bucket": fb_flat[2],
+            "plain_keys_per_s": n / plain_s, "learned_keys_per_s": n / learned_s,
+            "train_s": train_s, "model_loss": model.info()["loss"]}
+
+
+def bench_batch(fa, steps=3):
+    """4096 independent sort + group sum/count jobs, row counts log-uniform
+    in [256, 2^20] rescaled to 2^28 rows, int64 keys U[0, rows/4), double
+    values U[0,1); one HIP graph launch per step (replayed)."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from frovedis_amd import datagen as dg
+    sizes = dg.segment_sizes()
+    keys, vals, off = dg.batch_inputs_dev(sizes)
+    k = fa.DeviceArray(keys, np.int64)
+    v = fa.DeviceArray(vals, np.float64)
+    o = fa.DeviceArray(off, np.uint64)
+    out = fa.batch_sort_groupby(k, v, o, key_bits=18, use_graph=True)  # capture
+    best = float("inf")
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fa.batch_sort_groupby(k, v, o, key_bits=18, use_graph=True, out=out)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    n = int(sizes.sum())
+    G = out["ngroups"]
+    # algorithmic bytes: per segment 8 + 32 * ceil(b_s / 8) + 16 per row
+    # (b_s = bits of rows_s / 4), plus 24 per group
+    bits = np.array([max(1, math.ceil(math.log2(max(s // 4, 2)))) for s in sizes])
+    algo = int(np.sum(sizes * (8 + 32 * np.ceil(bits / 8) + 16))) + 24 * G
+    return {"rows": n, "segments": len(sizes), "groups": G, "ms": best * 1e3,
+            "rows_per_s": n / best, "algorithmic_bytes": algo,
+            "hbm_frac": algo / best / HBM_PEAK, "graph": True}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +232,11 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ha_ms, ha_launches = _prof_ms(fa, "hashagg")
+    breakdown = {}
+    for name in ("groupby_part_hist", "groupby_part_scatter", "hashagg", "groupby_merge",
+                 "radix_hist", "radix_pass"):
+        ms, cnt = _prof_ms(fa, name)
+        breakdown[name] = {"ms_per_step": ms / args.steps, "launches_per_step": cnt / args.steps}
     fa.check(fa.lib().fhip_prof_enable(0))
     ms_per_step = dt * 1e3 / args.steps
     groups = len(gk)
@@ -182,7 +268,7 @@ def main():
                      "kernel_ms": kern_s * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": None,
-        "extra": {},
+        "extra": {"groupby_kernels": breakdown},
     }
     del k, v, iv, gk, outs
     torch.cuda.empty_cache()
@@ -190,6 +276,12 @@ def main():
         print("# radix sort side measurements", file=sys.stderr, flush=True)
         result["extra"]["radix_kv_u64_2^24"] = bench_radix(fa, 24)
         result["extra"]["radix_kv_u64_2^28"] = bench_radix(fa, 28)
+        torch.cuda.empty_cache()
+        print("# batched segments", file=sys.stderr, flush=True)
+        result["extra"]["batched_segments_2^28"] = bench_batch(fa)
+        torch.cuda.empty_cache()
+        print("# learned vs plain sort", file=sys.stderr, flush=True)
+        result["extra"]["learned_sort_2^28"] = bench_learned(fa, 28)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         print("# cpu baseline (oracle, one core)", file=sys.stderr, flush=True)

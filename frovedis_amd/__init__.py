@@ -52,3 +52,4 @@ def check(rc):
 from .core import *  # noqa: E402,F401,F403
 from .dataframe import *  # noqa: E402,F401,F403
 from .learned import *  # noqa: E402,F401,F403
+from .batch import *  # noqa: E402,F401,F403

@@ -198,6 +198,8 @@ class ProfScope {
  public:
   ProfScope(const char* name, hipStream_t st) : name_(name), st_(st) {
     if (!prof_enabled()) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
     FHIP_CHECK(hipEventCreate(&a_));
     FHIP_CHECK(hipEventCreate(&b_));
     FHIP_CHECK(hipEventRecord(a_, st_));

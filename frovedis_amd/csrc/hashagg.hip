@@ -22,12 +22,15 @@
 namespace fhip {
 namespace {
 
-constexpr int HA_NT = 512;
-constexpr int HA_UNROLL = 4;
-constexpr int HA_LDS_BITS = 11;
-constexpr int HA_LDS_SLOTS = 1 << HA_LDS_BITS;
-constexpr int HA_LDS_PROBES = 8;
 constexpr int HA_MAX_ACC = 4;
+constexpr int PA_NT = 512;            // threads of the scatter / aggregate kernels
+constexpr int PA_UNROLL = 4;          // rows in flight per thread (aggregate)
+constexpr int PS_NT = 256;            // threads of the count / scatter kernels
+constexpr int PS_ITEMS = 16;          // rows per thread
+constexpr int PS_TILE = PS_NT * PS_ITEMS;  // rows per scatter tile
+constexpr unsigned long long PA_CHUNK = 1ull << 17;  // rows per aggregate workgroup
+constexpr int PA_SLOTS = 2048;        // LDS table slots (power of two)
+constexpr int PA_MAX_PBITS = 12;      // 4096 partitions (LDS of the scatter kernel)
 constexpr unsigned long long HA_EMPTY = 0x8000000000000000ull;
 
 enum { OP_SUM = 0, OP_COUNT = 1, OP_AVG = 2, OP_MAX = 3, OP_MIN = 4 };
@@ -139,145 +142,451 @@ __device__ __forceinline__ void acc_merge(unsigned long long* w, int kind, int v
   }
 }
 
-// Global table: `stride` 64-bit words per entry: key, count, acc[0..n).
-struct GTable {
-  unsigned long long* w;
-  unsigned long long mask;  // capacity - 1; entry `capacity` is the EMPTY-key entry
-  int stride;
-  unsigned long long* inserted;
-  int* overflow;
+// ---- pass 1: partition histogram -------------------------------------------
+// partition of a key: the top pbits of its hash; LDS slots use the low bits
+__device__ __forceinline__ unsigned part_of(unsigned long long h, int pbits) {
+  return pbits ? unsigned(h >> (64 - pbits)) : 0u;
+}
+
+// rank of this lane among the active lanes with the same value of `p`, and
+// the lane count, when all active lanes agree (wave-uniform fast path)
+__device__ __forceinline__ bool wave_uniform(unsigned p, unsigned long long active) {
+  const unsigned p0 = __shfl(p, __builtin_ffsll((long long)active) - 1, 64);
+  return __ballot(p == p0) == active;
+}
+
+// counts per (partition, tile), partition-major: cnt[p * tiles + t]
+template <class K>
+__global__ __launch_bounds__(PS_NT) void pa_count_kernel(const K* __restrict__ keys, size_t n,
+                                                         int pbits, size_t tiles,
+                                                         unsigned long long* __restrict__ cnt) {
+  extern __shared__ unsigned lh[];
+  const int P = 1 << pbits;
+  for (int i = threadIdx.x; i < P; i += PS_NT) lh[i] = 0;
+  __syncthreads();
+  const size_t t0 = size_t(blockIdx.x) * PS_TILE;
+  const unsigned lane = __lane_id();
+#pragma unroll 4
+  for (int j = 0; j < PS_ITEMS; j++) {
+    const size_t i = t0 + size_t(j) * PS_NT + threadIdx.x;
+    const bool ok = i < n;
+    const unsigned pp = ok ? part_of(mix64(key_bits(keys[i])), pbits) : 0u;
+    const unsigned long long act = __ballot(ok);
+    if (!act) continue;
+    if (wave_uniform(pp, act)) {
+      if (lane == unsigned(__builtin_ffsll((long long)act) - 1)) atomicAdd(&lh[pp], unsigned(__popcll(act)));
+    } else if (ok) {
+      atomicAdd(&lh[pp], 1u);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < P; q += PS_NT) cnt[size_t(q) * tiles + blockIdx.x] = lh[q];
+}
+
+// ---- pass 2: scatter rows (key + accumulator columns) into partitions ------
+struct Cols {
+  const void* in[HA_MAX_ACC];
+  void* out[HA_MAX_ACC];
+  int bytes[HA_MAX_ACC];
+  int n;
 };
 
-constexpr unsigned long long HA_MAX_PROBES = 4096;
-
-// Finds or inserts the entry of key bits `kb`; -1 when the table is too full
-// (the caller retries with a larger table).
-__device__ __forceinline__ long long gt_find(const GTable& t, unsigned long long kb,
-                                             unsigned long long h) {
-  if (kb == HA_EMPTY) return (long long)(t.mask + 1);
-  unsigned long long i = h & t.mask;
-  const unsigned long long limit = t.mask < HA_MAX_PROBES ? t.mask + 1 : HA_MAX_PROBES;
-  for (unsigned long long p = 0; p < limit; p++) {
-    unsigned long long* slot = t.w + i * t.stride;
-    unsigned long long cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur == kb) return (long long)i;
-    if (cur == HA_EMPTY) {
-      unsigned long long old = atomicCAS(slot, HA_EMPTY, kb);
-      if (old == HA_EMPTY) {
-        unsigned long long c = atomicAdd(t.inserted, 1ull);
-        if (c + 1 > ((t.mask + 1) / 4) * 3) atomicExch(t.overflow, 1);
-        return (long long)i;
-      }
-      if (old == kb) return (long long)i;
-    }
-    i = (i + 1) & t.mask;
+// One tile of PS_TILE rows: rows are ranked by partition in LDS, staged in
+// partition order and written out column by column in runs, so consecutive
+// lanes store consecutive addresses.  incl/cnt: inclusive scan and counts of
+// the partition-major count matrix (exact offsets, no global atomics).
+template <class K>
+__global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
+    const K* __restrict__ keys, size_t n, int pbits, size_t tiles, Cols cols,
+    const unsigned long long* __restrict__ cnt, const unsigned long long* __restrict__ incl,
+    K* __restrict__ out_keys) {
+  extern __shared__ unsigned long long lds_u64[];
+  const int P = 1 << pbits;
+  unsigned long long* stage = lds_u64;                                     // [PS_TILE]
+  unsigned long long* gbase = stage + PS_TILE;                             // [P]
+  unsigned* pc = reinterpret_cast<unsigned*>(gbase + P);                   // [P]
+  unsigned* ps = pc + P;                                                   // [P]
+  unsigned short* pid = reinterpret_cast<unsigned short*>(ps + P);         // [PS_TILE]
+  __shared__ unsigned scan_tmp[PS_NT / 64];
+  const unsigned tid = threadIdx.x, lane = __lane_id();
+  const size_t t = blockIdx.x;
+  const size_t t0 = t * PS_TILE;
+  const unsigned tn = unsigned(n - t0 < size_t(PS_TILE) ? n - t0 : size_t(PS_TILE));
+  for (int q = tid; q < P; q += PS_NT) {
+    const size_t m = size_t(q) * tiles + t;
+    gbase[q] = incl[m] - cnt[m];  // exclusive offset of (partition q, this tile)
+    pc[q] = 0;
   }
-  atomicExch(t.overflow, 1);
+  __syncthreads();
+  unsigned part[PS_ITEMS];
+  unsigned short slot[PS_ITEMS];
+#pragma unroll
+  for (int j = 0; j < PS_ITEMS; j++) {
+    const unsigned r = unsigned(j) * PS_NT + tid;
+    part[j] = r < tn ? part_of(mix64(key_bits(keys[t0 + r])), pbits) : 0u;
+  }
+  // counts of this tile per partition
+#pragma unroll
+  for (int j = 0; j < PS_ITEMS; j++) {
+    const unsigned r = unsigned(j) * PS_NT + tid;
+    const bool ok = r < tn;
+    const unsigned long long act = __ballot(ok);
+    if (!act) continue;
+    if (wave_uniform(part[j], act)) {
+      if (lane == unsigned(__builtin_ffsll((long long)act) - 1)) atomicAdd(&pc[part[j]], unsigned(__popcll(act)));
+    } else if (ok) {
+      atomicAdd(&pc[part[j]], 1u);
+    }
+  }
+  __syncthreads();
+  // tile-local start of each partition (block scan, P / PS_NT entries per thread)
+  {
+    const int per = (P + PS_NT - 1) / PS_NT;
+    unsigned local = 0;
+    for (int e = 0; e < per; e++) {
+      const int q = tid * per + e;
+      if (q < P) local += pc[q];
+    }
+    unsigned run = block_excl_scan<PS_NT>(local, scan_tmp, (unsigned*)nullptr);
+    for (int e = 0; e < per; e++) {
+      const int q = tid * per + e;
+      if (q < P) {
+        ps[q] = run;
+        run += pc[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < P; q += PS_NT) pc[q] = 0;
+  __syncthreads();
+  // rank within (tile, partition)
+  const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+  for (int j = 0; j < PS_ITEMS; j++) {
+    const unsigned r = unsigned(j) * PS_NT + tid;
+    const bool ok = r < tn;
+    const unsigned long long act = __ballot(ok);
+    slot[j] = 0;
+    if (!act) continue;
+    if (wave_uniform(part[j], act)) {
+      const int leader = __builtin_ffsll((long long)act) - 1;
+      unsigned base = 0;
+      if (lane == unsigned(leader)) base = atomicAdd(&pc[part[j]], unsigned(__popcll(act)));
+      base = __shfl(base, leader, 64);
+      if (ok) slot[j] = (unsigned short)(ps[part[j]] + base + __popcll(act & lt));
+    } else if (ok) {
+      slot[j] = (unsigned short)(ps[part[j]] + atomicAdd(&pc[part[j]], 1u));
+    }
+  }
+  // keys: stage in partition order, then write runs
+#pragma unroll
+  for (int j = 0; j < PS_ITEMS; j++) {
+    const unsigned r = unsigned(j) * PS_NT + tid;
+    if (r < tn) {
+      stage[slot[j]] = (unsigned long long)keys[t0 + r];
+      pid[slot[j]] = (unsigned short)part[j];
+    }
+  }
+  __syncthreads();
+  for (unsigned i = tid; i < tn; i += PS_NT) {
+    const unsigned q = pid[i];
+    out_keys[gbase[q] + (i - ps[q])] = K(stage[i]);
+  }
+  for (int a = 0; a < cols.n; a++) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PS_ITEMS; j++) {
+      const unsigned r = unsigned(j) * PS_NT + tid;
+      if (r < tn)
+        stage[slot[j]] = cols.bytes[a] == 4
+                             ? (unsigned long long)static_cast<const unsigned*>(cols.in[a])[t0 + r]
+                             : static_cast<const unsigned long long*>(cols.in[a])[t0 + r];
+    }
+    __syncthreads();
+    if (cols.bytes[a] == 4) {
+      unsigned* o = static_cast<unsigned*>(cols.out[a]);
+      for (unsigned i = tid; i < tn; i += PS_NT) {
+        const unsigned q = pid[i];
+        o[gbase[q] + (i - ps[q])] = unsigned(stage[i]);
+      }
+    } else {
+      unsigned long long* o = static_cast<unsigned long long*>(cols.out[a]);
+      for (unsigned i = tid; i < tn; i += PS_NT) {
+        const unsigned q = pid[i];
+        o[gbase[q] + (i - ps[q])] = stage[i];
+      }
+    }
+  }
+}
+
+// ---- LDS aggregation table shared by passes 3 and 4 -------------------------
+// S slots + one extra slot for the key whose bits equal the empty marker.
+struct LdsTable {
+  unsigned long long* key;
+  unsigned long long* cnt;  // 64-bit counts
+  unsigned long long* acc;  // [na][S + 1]
+  int S;
+};
+
+__device__ __forceinline__ LdsTable lds_table(unsigned char* raw, int S, int na) {
+  LdsTable t;
+  t.key = reinterpret_cast<unsigned long long*>(raw);
+  t.cnt = t.key + (S + 1);
+  t.acc = t.cnt + (S + 1);
+  t.S = S;
+  (void)na;
+  return t;
+}
+
+__device__ __forceinline__ void lds_table_init(const LdsTable& t, const AccSet& accs) {
+  for (int s = threadIdx.x; s <= t.S; s += blockDim.x) {
+    t.key[s] = HA_EMPTY;
+    t.cnt[s] = 0;
+    for (int a = 0; a < accs.n; a++)
+      t.acc[size_t(a) * (t.S + 1) + s] = acc_identity(accs.a[a].kind, accs.a[a].vtype);
+  }
+}
+
+// slot of key bits kb (inserting it), or -1 when the table is full
+__device__ __forceinline__ int lds_slot(const LdsTable& t, unsigned long long kb,
+                                        unsigned long long h) {
+  if (kb == HA_EMPTY) return t.S;
+  int s = int(h & (unsigned long long)(t.S - 1));
+  for (int p = 0; p < t.S; p++) {
+    const unsigned long long cur = t.key[s];
+    if (cur == kb) return s;
+    if (cur == HA_EMPTY) {
+      const unsigned long long old = atomicCAS(&t.key[s], HA_EMPTY, kb);
+      if (old == HA_EMPTY || old == kb) return s;
+    }
+    s = (s + 1) & (t.S - 1);
+  }
   return -1;
 }
 
-__device__ __forceinline__ bool gt_overflowed(const GTable& t) {
-  return __hip_atomic_load(t.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-}
-
-template <class K>
-__global__ __launch_bounds__(HA_NT) void hashagg_kernel(const K* __restrict__ keys, size_t n,
-                                                        size_t rows_per_block, AccSet accs,
-                                                        GTable gt) {
-  // LDS table: keys, per-slot row counts, then na accumulator arrays
-  extern __shared__ unsigned long long lds[];
-  unsigned long long* lkey = lds;
-  unsigned long long* lacc = lds + HA_LDS_SLOTS;  // [na][slots]
-  const int na = accs.n;
-  unsigned* lcnt = reinterpret_cast<unsigned*>(lacc + size_t(na) * HA_LDS_SLOTS);
-  for (int s = threadIdx.x; s < HA_LDS_SLOTS; s += HA_NT) {
-    lkey[s] = HA_EMPTY;
-    lcnt[s] = 0;
-    for (int a = 0; a < na; a++)
-      lacc[a * HA_LDS_SLOTS + s] = acc_identity(accs.a[a].kind, accs.a[a].vtype);
+// Writes the occupied slots as entries {key, count, acc...}; returns count.
+__device__ __forceinline__ void lds_table_flush(const LdsTable& t, int na, int stride,
+                                                unsigned long long* out, unsigned* out_n,
+                                                unsigned* lds_counter) {
+  for (int s = threadIdx.x; s <= t.S; s += blockDim.x) {
+    const bool used = (s == t.S) ? t.cnt[s] != 0 : t.key[s] != HA_EMPTY;
+    if (!used) continue;
+    const unsigned j = atomicAdd(lds_counter, 1u);
+    unsigned long long* e = out + size_t(j) * stride;
+    e[0] = s == t.S ? HA_EMPTY : t.key[s];
+    e[1] = t.cnt[s];
+    for (int a = 0; a < na; a++) e[2 + a] = t.acc[size_t(a) * (t.S + 1) + s];
   }
   __syncthreads();
+  if (threadIdx.x == 0) *out_n = *lds_counter;
+}
 
-  const size_t beg = size_t(blockIdx.x) * rows_per_block;
-  const size_t end = beg + rows_per_block < n ? beg + rows_per_block : n;
-  for (size_t r0 = beg + threadIdx.x; r0 < end; r0 += size_t(HA_NT) * HA_UNROLL) {
-    if (gt_overflowed(gt)) break;  // result will be discarded and rebuilt
-    unsigned long long kbv[HA_UNROLL];
-    unsigned long long vv[HA_UNROLL][HA_MAX_ACC];
+// Reduces an accumulator word over the lanes in `act` (all lanes end with
+// the result); used when a whole wave updates the same LDS slot.
+__device__ __forceinline__ unsigned long long wave_reduce_acc(unsigned long long w, int kind,
+                                                              int vtype, unsigned long long act,
+                                                              bool ok) {
+  if (!ok) w = acc_identity(kind, vtype);
 #pragma unroll
-    for (int u = 0; u < HA_UNROLL; u++) {
-      const size_t r = r0 + size_t(u) * HA_NT;
-      kbv[u] = r < end ? key_bits(keys[r]) : 0ull;
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long x = __shfl_xor(w, o, 64);
+    switch (vtype) {
+      case FHIP_F64: {
+        double a, b;
+        __builtin_memcpy(&a, &w, 8);
+        __builtin_memcpy(&b, &x, 8);
+        double r = kind == K_SUM ? a + b
+                 : kind == K_MAX ? (a < b || a != a ? b : a)
+                                 : (a > b || a != a ? b : a);
+        if (kind != K_SUM && b != b) r = a;  // NaN is never selected
+        __builtin_memcpy(&w, &r, 8);
+        break;
+      }
+      case FHIP_F32: {
+        float a, b;
+        unsigned ua = unsigned(w), ub = unsigned(x);
+        __builtin_memcpy(&a, &ua, 4);
+        __builtin_memcpy(&b, &ub, 4);
+        float r = kind == K_SUM ? a + b
+                : kind == K_MAX ? (a < b || a != a ? b : a)
+                                : (a > b || a != a ? b : a);
+        if (kind != K_SUM && b != b) r = a;
+        unsigned ur;
+        __builtin_memcpy(&ur, &r, 4);
+        w = ur;
+        break;
+      }
+      case FHIP_I32: {
+        int a = int(unsigned(w)), b = int(unsigned(x));
+        int r = kind == K_SUM ? int(unsigned(a) + unsigned(b)) : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
+        w = unsigned(r);
+        break;
+      }
+      case FHIP_U32: {
+        unsigned a = unsigned(w), b = unsigned(x);
+        w = kind == K_SUM ? a + b : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
+        break;
+      }
+      case FHIP_I64: {
+        long long a = (long long)w, b = (long long)x;
+        long long r = kind == K_SUM ? (long long)(w + x) : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
+        w = (unsigned long long)r;
+        break;
+      }
+      default:
+        w = kind == K_SUM ? w + x : kind == K_MAX ? (w < x ? x : w) : (w > x ? x : w);
+        break;
+    }
+  }
+  (void)act;
+  return w;
+}
+
+struct Item {
+  unsigned long long beg, end;
+};
+
+// ---- pass 3: aggregate one chunk of one partition in LDS --------------------
+template <class K>
+__global__ __launch_bounds__(PA_NT) void pa_agg_kernel(const Item* __restrict__ items,
+                                                       const K* __restrict__ keys, AccSet accs,
+                                                       int S, int stride,
+                                                       unsigned long long* __restrict__ partial,
+                                                       unsigned* __restrict__ partial_n,
+                                                       int* __restrict__ overflow) {
+  extern __shared__ unsigned long long lds_u64[];
+  unsigned char* lds_raw = reinterpret_cast<unsigned char*>(lds_u64);
+  __shared__ unsigned counter;
+  __shared__ int full;
+  const int na = accs.n;
+  LdsTable t = lds_table(lds_raw, S, na);
+  lds_table_init(t, accs);
+  if (threadIdx.x == 0) {
+    counter = 0;
+    full = 0;
+  }
+  __syncthreads();
+  const Item it = items[blockIdx.x];
+  // the loop bound is tested on the wave's first row so that every lane of a
+  // wave stays active through the ballots and shuffles below
+  const size_t wave_first = it.beg + (threadIdx.x & ~63u);
+  for (size_t w0 = wave_first; w0 < it.end; w0 += size_t(PA_NT) * PA_UNROLL) {
+    const size_t r0 = w0 + __lane_id();
+    unsigned long long kbv[PA_UNROLL];
+    unsigned long long vv[PA_UNROLL][HA_MAX_ACC];
+#pragma unroll
+    for (int u = 0; u < PA_UNROLL; u++) {
+      const size_t r = r0 + size_t(u) * PA_NT;
+      kbv[u] = r < it.end ? key_bits(keys[r]) : 0ull;
 #pragma unroll
       for (int a = 0; a < HA_MAX_ACC; a++)
-        vv[u][a] = (a < na && r < end) ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
+        vv[u][a] = (a < na && r < it.end) ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
     }
+    const unsigned lane = __lane_id();
 #pragma unroll
-    for (int u = 0; u < HA_UNROLL; u++) {
-      const size_t r = r0 + size_t(u) * HA_NT;
-      if (r >= end) break;
-      const unsigned long long kb = kbv[u];
-      const unsigned long long h = mix64(kb);
-      int slot = -1;
-      if (kb != HA_EMPTY) {
-        int s = int(h >> (64 - HA_LDS_BITS));
-#pragma unroll 1
-        for (int p = 0; p < HA_LDS_PROBES; p++) {
-          unsigned long long cur = lkey[s];
-          if (cur == kb) { slot = s; break; }
-          if (cur == HA_EMPTY) {
-            unsigned long long old = atomicCAS(&lkey[s], HA_EMPTY, kb);
-            if (old == HA_EMPTY || old == kb) { slot = s; break; }
-          }
-          s = (s + 1) & (HA_LDS_SLOTS - 1);
+    for (int u = 0; u < PA_UNROLL; u++) {
+      const size_t r = r0 + size_t(u) * PA_NT;
+      const bool in = r < it.end;
+      const int s = in ? lds_slot(t, kbv[u], mix64(kbv[u])) : -1;
+      if (in && s < 0) full = 1;
+      const bool ok = s >= 0;
+      const unsigned long long act = __ballot(ok);
+      if (!act) continue;
+      if (wave_uniform(unsigned(s), act)) {
+        // the whole wave updates one slot (a heavy key): reduce, then one
+        // atomic per accumulator
+        const int leader = __builtin_ffsll((long long)act) - 1;
+        if (lane == unsigned(leader)) atomicAdd(&t.cnt[s], (unsigned long long)__popcll(act));
+        for (int a = 0; a < na; a++) {
+          const unsigned long long w =
+              wave_reduce_acc(vv[u][a], accs.a[a].kind, accs.a[a].vtype, act, ok);
+          if (lane == unsigned(leader))
+            acc_merge(&t.acc[size_t(a) * (S + 1) + s], accs.a[a].kind, accs.a[a].vtype, w);
         }
-      }
-      if (slot >= 0) {
-        atomicAdd(&lcnt[slot], 1u);
+      } else if (ok) {
+        atomicAdd(&t.cnt[s], 1ull);
         for (int a = 0; a < na; a++)
-          acc_merge(&lacc[a * HA_LDS_SLOTS + slot], accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
-      } else {
-        long long e = gt_find(gt, kb, h);
-        if (e < 0) continue;
-        unsigned long long* ent = gt.w + size_t(e) * gt.stride;
-        atomicAdd(ent + 1, 1ull);
-        for (int a = 0; a < na; a++)
-          acc_merge(ent + 2 + a, accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
+          acc_merge(&t.acc[size_t(a) * (S + 1) + s], accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
       }
     }
   }
   __syncthreads();
-  for (int s = threadIdx.x; s < HA_LDS_SLOTS; s += HA_NT) {
-    const unsigned long long kb = lkey[s];
-    if (kb == HA_EMPTY || gt_overflowed(gt)) continue;
-    long long e = gt_find(gt, kb, mix64(kb));
-    if (e < 0) continue;
-    unsigned long long* ent = gt.w + size_t(e) * gt.stride;
-    atomicAdd(ent + 1, (unsigned long long)lcnt[s]);
-    for (int a = 0; a < na; a++)
-      acc_merge(ent + 2 + a, accs.a[a].kind, accs.a[a].vtype, lacc[a * HA_LDS_SLOTS + s]);
+  if (full) {  // the partition has more groups than the table: rebuilt with more partitions
+    if (threadIdx.x == 0) {
+      partial_n[blockIdx.x] = 0;
+      atomicExch(overflow, 1);
+    }
+    return;
   }
+  lds_table_flush(t, na, stride, partial + size_t(blockIdx.x) * (S + 1) * stride,
+                  partial_n + blockIdx.x, &counter);
 }
 
-__global__ void gt_init_kernel(unsigned long long* w, size_t entries, int stride, AccSet accs) {
-  const size_t stride_g = size_t(gridDim.x) * blockDim.x;
-  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < entries; e += stride_g) {
-    unsigned long long* ent = w + e * stride;
-    ent[0] = HA_EMPTY;
-    ent[1] = 0;
-    for (int a = 0; a < accs.n; a++) ent[2 + a] = acc_identity(accs.a[a].kind, accs.a[a].vtype);
+// ---- pass 4: merge the chunk partials of one partition ----------------------
+__global__ __launch_bounds__(PA_NT) void pa_merge_kernel(
+    const unsigned* __restrict__ part_items, const unsigned long long* __restrict__ partial,
+    const unsigned* __restrict__ partial_n, AccSet accs, int S, int stride,
+    unsigned long long* __restrict__ final_out, unsigned* __restrict__ final_n,
+    int* __restrict__ overflow) {
+  extern __shared__ unsigned long long lds_u64[];
+  unsigned char* lds_raw = reinterpret_cast<unsigned char*>(lds_u64);
+  __shared__ unsigned counter;
+  __shared__ int full;
+  const int na = accs.n;
+  const unsigned p = blockIdx.x;
+  const unsigned i0 = part_items[p], i1 = part_items[p + 1];
+  unsigned long long* out = final_out + size_t(p) * (S + 1) * stride;
+  const unsigned cap = unsigned(S + 1);
+  if (i1 - i0 == 1) {  // one chunk: its partial is final
+    const unsigned m = partial_n[i0] < cap ? partial_n[i0] : cap;
+    const unsigned long long* src = partial + size_t(i0) * (S + 1) * stride;
+    for (unsigned j = threadIdx.x; j < m * unsigned(stride); j += PA_NT) out[j] = src[j];
+    if (threadIdx.x == 0) final_n[p] = m;
+    return;
   }
+  LdsTable t = lds_table(lds_raw, S, na);
+  lds_table_init(t, accs);
+  if (threadIdx.x == 0) {
+    counter = 0;
+    full = 0;
+  }
+  __syncthreads();
+  for (unsigned it = i0; it < i1; it++) {
+    const unsigned m = partial_n[it] < cap ? partial_n[it] : cap;
+    const unsigned long long* src = partial + size_t(it) * (S + 1) * stride;
+    for (unsigned j = threadIdx.x; j < m; j += PA_NT) {
+      const unsigned long long* e = src + size_t(j) * stride;
+      const unsigned long long kb = e[0];
+      const int s = lds_slot(t, kb, mix64(kb));
+      if (s < 0) {
+        full = 1;
+        continue;
+      }
+      atomicAdd(&t.cnt[s], e[1]);
+      for (int a = 0; a < na; a++)
+        acc_merge(&t.acc[size_t(a) * (S + 1) + s], accs.a[a].kind, accs.a[a].vtype, e[2 + a]);
+    }
+  }
+  __syncthreads();
+  if (full) {
+    if (threadIdx.x == 0) {
+      final_n[p] = 0;
+      atomicExch(overflow, 1);
+    }
+    return;
+  }
+  lds_table_flush(t, na, stride, out, final_n + p, &counter);
 }
 
-struct Occupied {
-  const unsigned long long* w;
-  int stride;
-  size_t cap;
-  __device__ bool operator()(size_t e) const {
-    // the EMPTY-key entry (index cap) is occupied when its count is non-zero
-    return e == cap ? w[e * stride + 1] != 0 : w[e * stride] != HA_EMPTY;
-  }
-};
+__global__ void final_index_kernel(const unsigned* __restrict__ final_n,
+                                   const unsigned long long* __restrict__ final_off, int P, int S,
+                                   size_t* __restrict__ ent) {
+  const int p = blockIdx.x;
+  const unsigned m = final_n[p];
+  const unsigned long long o = final_off[p];
+  for (unsigned j = threadIdx.x; j < m; j += blockDim.x)
+    ent[o + j] = size_t(p) * (S + 1) + j;
+}
 
 template <class K>
 __global__ void gather_group_keys(const unsigned long long* __restrict__ w, int stride,
@@ -336,7 +645,53 @@ int grid_for(size_t n, int per_cu = 8) {
   return int(std::max<size_t>(1, std::min<size_t>(ceil_div(n, 256), size_t(fill_blocks(256, per_cu)))));
 }
 
+template <class K>
+void launch_pa(const void* keys, size_t n, int pbits, size_t tiles, const Cols& cols,
+               unsigned long long* cnt, unsigned long long* incl, void* pkeys, hipStream_t st,
+               int pass) {
+  const int P = 1 << pbits;
+  if (pass == 1) {
+    pa_count_kernel<K><<<unsigned(tiles), PS_NT, P * 4, st>>>((const K*)keys, n, pbits, tiles, cnt);
+  } else {
+    const size_t lds = size_t(PS_TILE) * 8 + size_t(P) * 16 + size_t(PS_TILE) * 2;
+    pa_scatter_kernel<K><<<unsigned(tiles), PS_NT, lds, st>>>((const K*)keys, n, pbits, tiles, cols,
+                                                             cnt, incl, (K*)pkeys);
+  }
+  FHIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
+
+void group_by(int ktype, const void* col, size_t* idx, size_t n, size_t* split, size_t* nsplit,
+              void* keys_out, hipStream_t st);
+void prefix_sum(int type, const void* in, void* out, size_t n, hipStream_t st);
+void grouped_agg(int op, int vtype, const void* col, size_t col_n, const size_t* gidx, size_t n,
+                 const size_t* split, size_t nsplit, const size_t* nulls, size_t nn, void* out,
+                 hipStream_t st);
+
+// More groups than the partitioned tables hold: sort-based group_by plus
+// per-column aggregation (same results).
+void group_by_agg_sorted(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
+                         void* keys_out, size_t capacity, size_t* ngroups, hipStream_t st) {
+  const int kb = type_bytes(ktype);
+  DBuf idx(n * 8), split((n + 1) * 8), gk(n * size_t(kb));
+  std::vector<size_t> iota(n);
+  for (size_t i = 0; i < n; i++) iota[i] = i;
+  FHIP_CHECK(hipMemcpyAsync(idx.get(), iota.data(), n * 8, hipMemcpyHostToDevice, st));
+  size_t ns = 0;
+  group_by(ktype, keys, idx.as<size_t>(), n, split.as<size_t>(), &ns, gk.get(), st);
+  const size_t G = ns - 1;
+  *ngroups = G;
+  FHIP_REQUIRE(G <= capacity, FHIP_ERR_TOO_LARGE,
+               "group_by_agg: output capacity " + std::to_string(capacity) + " < groups " +
+                   std::to_string(G));
+  FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
+  for (int i = 0; i < naggs; i++)
+    grouped_agg(aggs[i].op, aggs[i].op == OP_COUNT ? ktype : aggs[i].vtype,
+                aggs[i].op == OP_COUNT ? keys : aggs[i].col, n, idx.as<size_t>(), n,
+                split.as<size_t>(), ns, nullptr, 0, aggs[i].out, st);
+  FHIP_CHECK(hipStreamSynchronize(st));  // iota is a host local
+}
 
 // Sorted-by-key result of a fused group-by; see fhip_dev_group_by_agg.
 void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
@@ -367,72 +722,159 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   }
   *ngroups = 0;
   if (n == 0) return;
+  const int kb = type_bytes(ktype);
   const int stride = 2 + accs.n;
-  size_t cap = 1024;
-  size_t want = table_hint ? table_hint : std::min<size_t>(n, size_t(1) << 21);
-  while (cap < 2 * want) cap <<= 1;
-  const int grid = fill_blocks(HA_NT, 2);
-  const size_t lds_bytes = size_t(HA_LDS_SLOTS) * (8 + 8 * size_t(accs.n) + 4);
-  static bool lds_attr_set = false;
-  if (!lds_attr_set) {  // allow more than 64 KiB of dynamic LDS (4 accumulators)
-    const int max_lds = int(size_t(HA_LDS_SLOTS) * (8 + 8 * HA_MAX_ACC + 4));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)hashagg_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)hashagg_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)hashagg_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)hashagg_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-    lds_attr_set = true;
+  // LDS table: 4096 slots with up to two accumulators, else 2048
+  const int S = accs.n <= 2 ? 2 * PA_SLOTS : PA_SLOTS;
+  const size_t lds_table_bytes = size_t(S + 1) * 8 * size_t(2 + accs.n);
+  static bool attr_set = false;
+  if (!attr_set) {
+    const int mx = int(std::max(size_t(2 * PA_SLOTS + 1) * 8 * 4,
+                                size_t(PA_SLOTS + 1) * 8 * (2 + HA_MAX_ACC)));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_merge_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    const int ms = int(size_t(PS_TILE) * 10 + (size_t(1) << PA_MAX_PBITS) * 16);
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    attr_set = true;
   }
-  const size_t rows_per_block = ceil_div(n, size_t(grid));
-  DBuf tab, flags(2 * sizeof(unsigned long long));
+  // partitions: about PA_GROUPS_PER_PART expected groups each
+  const size_t want = table_hint ? table_hint : std::min<size_t>(n, size_t(1) << 20);
+  int pbits = 4;
+  while (pbits < PA_MAX_PBITS && (size_t(1) << pbits) * size_t(S / 2) < want) pbits++;
   for (;;) {
-    tab.reset((cap + 1) * stride * sizeof(unsigned long long));
-    GTable gt{tab.as<unsigned long long>(), (unsigned long long)(cap - 1), stride,
-              flags.as<unsigned long long>(), reinterpret_cast<int*>(flags.as<unsigned long long>() + 1)};
-    FHIP_CHECK(hipMemsetAsync(flags.get(), 0, 2 * sizeof(unsigned long long), st));
-    gt_init_kernel<<<grid_for(cap + 1), 256, 0, st>>>(gt.w, cap + 1, stride, accs);
+    const int P = 1 << pbits;
+    const size_t tiles = ceil_div(n, PS_TILE);
+    DBuf cb(size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int));
+    unsigned long long* cnt = cb.as<unsigned long long>();
+    unsigned long long* incl = ib.as<unsigned long long>();
+    FHIP_CHECK(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
+    DBuf pkeys(n * size_t(kb));
+    DBuf pcols[HA_MAX_ACC];
+    Cols cols{};
+    cols.n = accs.n;
+    for (int a = 0; a < accs.n; a++) {
+      cols.bytes[a] = type_bytes(accs.a[a].vtype);
+      cols.in[a] = accs.a[a].col;
+      pcols[a].reset(n * size_t(cols.bytes[a]));
+      cols.out[a] = pcols[a].get();
+    }
+    auto run = [&](int pass) {
+      switch (ktype) {
+        case FHIP_I32: launch_pa<int32_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_U32: launch_pa<uint32_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_I64: launch_pa<int64_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        default: launch_pa<uint64_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+      }
+    };
+    {
+      ProfScope prof("groupby_part_hist", st);
+      run(1);
+      prefix_sum(FHIP_U64, cnt, incl, size_t(P) * tiles, st);
+    }
+    // partition sizes: inclusive offset at the last tile of every partition
+    std::vector<unsigned long long> ends(P);
+    FHIP_CHECK(hipMemcpy2DAsync(ends.data(), 8, incl + (tiles - 1), tiles * 8, 8, P,
+                                hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    // chunks of at most PA_CHUNK rows per partition
+    std::vector<Item> items;
+    std::vector<unsigned> part_items(P + 1);
+    unsigned long long run_off = 0;
+    for (int p = 0; p < P; p++) {
+      part_items[p] = unsigned(items.size());
+      const unsigned long long e = ends[p];
+      for (unsigned long long b = run_off; b < e; b += PA_CHUNK)
+        items.push_back(Item{b, std::min<unsigned long long>(b + PA_CHUNK, e)});
+      if (e == run_off) items.push_back(Item{run_off, run_off});
+      run_off = e;
+    }
+    part_items[P] = unsigned(items.size());
+    {
+      ProfScope prof("groupby_part_scatter", st);
+      run(2);
+    }
+    const size_t NI = items.size();
+    DBuf itb(NI * sizeof(Item)), pib((P + 1) * sizeof(unsigned));
+    FHIP_CHECK(hipMemcpyAsync(itb.get(), items.data(), NI * sizeof(Item), hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipMemcpyAsync(pib.get(), part_items.data(), (P + 1) * sizeof(unsigned),
+                              hipMemcpyHostToDevice, st));
+    DBuf partial(NI * size_t(S + 1) * stride * 8), pn(NI * sizeof(unsigned));
+    FHIP_CHECK(hipMemsetAsync(pn.get(), 0, NI * sizeof(unsigned), st));
+    AccSet paccs = accs;  // the partitioned copies of the columns
+    for (int a = 0; a < accs.n; a++) paccs.a[a].col = pcols[a].get();
+    {
+      ProfScope prof("hashagg", st);
+      switch (ktype) {
+        case FHIP_I32: pa_agg_kernel<int32_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<int32_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
+        case FHIP_U32: pa_agg_kernel<uint32_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<uint32_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
+        case FHIP_I64: pa_agg_kernel<int64_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<int64_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
+        default: pa_agg_kernel<uint64_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<uint64_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
+      }
+      FHIP_CHECK(hipGetLastError());
+    }
+    DBuf fin(size_t(P) * (S + 1) * stride * 8), fnb(size_t(P) * sizeof(unsigned));
+    {
+      ProfScope prof("groupby_merge", st);
+      pa_merge_kernel<<<P, PA_NT, lds_table_bytes, st>>>(
+          pib.as<unsigned>(), partial.as<unsigned long long>(), pn.as<unsigned>(), accs, S, stride,
+          fin.as<unsigned long long>(), fnb.as<unsigned>(), ovf.as<int>());
+      FHIP_CHECK(hipGetLastError());
+    }
+    int ov = 0;
+    std::vector<unsigned> fn(P);
+    FHIP_CHECK(hipMemcpyAsync(&ov, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipMemcpyAsync(fn.data(), fnb.get(), size_t(P) * sizeof(unsigned),
+                              hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    if (ov) {  // a partition had more groups than its LDS table: more partitions
+      if (pbits < PA_MAX_PBITS) {
+        pbits = std::min(pbits + 2, PA_MAX_PBITS);
+        continue;
+      }
+      group_by_agg_sorted(ktype, keys, n, aggs, naggs, keys_out, capacity, ngroups, st);
+      return;
+    }
+    std::vector<unsigned long long> foff(P);
+    size_t G = 0;
+    for (int p = 0; p < P; p++) {
+      foff[p] = G;
+      G += fn[p];
+    }
+    *ngroups = G;  // reported even when it does not fit, so the caller can resize
+    FHIP_REQUIRE(G <= capacity, FHIP_ERR_TOO_LARGE,
+                 "group_by_agg: output capacity " + std::to_string(capacity) + " < groups " +
+                     std::to_string(G));
+    if (G == 0) return;
+    DBuf fob(size_t(P) * 8), entb(G * sizeof(size_t));
+    FHIP_CHECK(hipMemcpyAsync(fob.get(), foff.data(), size_t(P) * 8, hipMemcpyHostToDevice, st));
+    final_index_kernel<<<P, 256, 0, st>>>(fnb.as<unsigned>(), fob.as<unsigned long long>(), P, S,
+                                           entb.as<size_t>());
     FHIP_CHECK(hipGetLastError());
-    ProfScope prof("hashagg", st);
+    const unsigned long long* w = fin.as<unsigned long long>();
+    DBuf gk(G * size_t(kb));
     switch (ktype) {
-      case FHIP_I32: hashagg_kernel<int32_t><<<grid, HA_NT, lds_bytes, st>>>((const int32_t*)keys, n, rows_per_block, accs, gt); break;
-      case FHIP_U32: hashagg_kernel<uint32_t><<<grid, HA_NT, lds_bytes, st>>>((const uint32_t*)keys, n, rows_per_block, accs, gt); break;
-      case FHIP_I64: hashagg_kernel<int64_t><<<grid, HA_NT, lds_bytes, st>>>((const int64_t*)keys, n, rows_per_block, accs, gt); break;
-      default: hashagg_kernel<uint64_t><<<grid, HA_NT, lds_bytes, st>>>((const uint64_t*)keys, n, rows_per_block, accs, gt); break;
+      case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int32_t>()); break;
+      case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint32_t>()); break;
+      case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int64_t>()); break;
+      default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint64_t>()); break;
     }
     FHIP_CHECK(hipGetLastError());
-    int ovf = 0;
-    FHIP_CHECK(hipMemcpyAsync(&ovf, gt.overflow, sizeof(int), hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
-    if (!ovf) break;
-    FHIP_REQUIRE(cap < (size_t(1) << 40), FHIP_ERR_TOO_LARGE, "group_by_agg: table too large");
-    cap <<= 2;  // too many groups for this table: retry 4x larger
-  }
-  // compact occupied entries, then sort them by key
-  const size_t entries = cap + 1;
-  DBuf entb(entries * sizeof(size_t));
-  size_t G = compact_indices(Occupied{tab.as<unsigned long long>(), stride, cap}, entries,
-                             entb.as<size_t>(), st);
-  *ngroups = G;  // reported even when it does not fit, so the caller can resize
-  FHIP_REQUIRE(G <= capacity, FHIP_ERR_TOO_LARGE,
-               "group_by_agg: output capacity " + std::to_string(capacity) + " < groups " +
-                   std::to_string(G));
-  if (G == 0) return;
-  const int kb = type_bytes(ktype);
-  DBuf gk(G * size_t(kb));
-  switch (ktype) {
-    case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(tab.as<unsigned long long>(), stride, entb.as<size_t>(), G, gk.as<int32_t>()); break;
-    case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(tab.as<unsigned long long>(), stride, entb.as<size_t>(), G, gk.as<uint32_t>()); break;
-    case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(tab.as<unsigned long long>(), stride, entb.as<size_t>(), G, gk.as<int64_t>()); break;
-    default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(tab.as<unsigned long long>(), stride, entb.as<size_t>(), G, gk.as<uint64_t>()); break;
-  }
-  FHIP_CHECK(hipGetLastError());
-  // keys are distinct, so any stable order is the reference's ascending order
-  sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st);
-  FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
-  for (int i = 0; i < naggs; i++) {
-    emit_kernel<<<grid_for(G), 256, 0, st>>>(tab.as<unsigned long long>(), stride,
-                                             entb.as<size_t>(), G, aggs[i].op, aggs[i].vtype,
-                                             slot_of[i] < 0 ? 0 : slot_of[i], aggs[i].out);
-    FHIP_CHECK(hipGetLastError());
+    // keys are distinct, so the stable sort gives the reference's ascending order
+    sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st);
+    FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
+    for (int i = 0; i < naggs; i++) {
+      emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, aggs[i].op,
+                                               aggs[i].vtype, slot_of[i] < 0 ? 0 : slot_of[i],
+                                               aggs[i].out);
+      FHIP_CHECK(hipGetLastError());
+    }
+    return;
   }
 }
 

@@ -36,8 +36,9 @@ extern const int kMlpParams;
 
 namespace {
 
-constexpr int LS_B1 = 4096;        // coarse buckets
-constexpr int LS_B2 = 16;          // fine buckets per coarse bucket
+constexpr int LS_B1 = 4096;        // coarse buckets (routed by the network)
+constexpr int LS_NF = 65536;       // leaves (fine buckets, sample quantiles)
+constexpr int L2_LDS_FAN = 2048;   // leaves per coarse bucket kept in LDS
 constexpr int LS_NT = 512;
 constexpr int LS_ITEMS = 32;       // keys per thread in the L1 scatter
 constexpr int L3_NT = 256;
@@ -121,45 +122,72 @@ __global__ __launch_bounds__(LS_NT) void l1_scatter_kernel(const uint64_t* __res
   }
 }
 
-// one workgroup per coarse bucket: split into LS_B2 fine buckets
+// One workgroup per coarse bucket c: split it into its leaves
+// [kidx[c], kidx[c+1]) (global fine buckets; their splitters are `fine`).
+// Counters live in LDS for up to L2_LDS_FAN leaves, else in the global
+// scratch arrays gcnt / gcur, whose [kidx[c], kidx[c+1]) range belongs to
+// this workgroup alone.
 __global__ __launch_bounds__(256) void l2_kernel(const uint64_t* __restrict__ in,
                                                  const unsigned long long* __restrict__ off1,
-                                                 int is_signed, const uint64_t* __restrict__ fine,
+                                                 const unsigned* __restrict__ kidx, int is_signed,
+                                                 const uint64_t* __restrict__ fine,
                                                  unsigned long long* __restrict__ fine_off,
+                                                 unsigned* __restrict__ gcnt,
+                                                 unsigned long long* __restrict__ gcur,
                                                  uint64_t* __restrict__ out) {
-  __shared__ uint64_t spl[LS_B2];
-  __shared__ unsigned cnt[LS_B2];
-  __shared__ unsigned long long base[LS_B2];
-  const int b = blockIdx.x;
-  const size_t s = off1[b], e = off1[b + 1];
-  if (threadIdx.x < LS_B2) {
-    spl[threadIdx.x] = fine[size_t(b) * LS_B2 + threadIdx.x];
-    cnt[threadIdx.x] = 0;
+  __shared__ uint64_t spl[L2_LDS_FAN];
+  __shared__ unsigned lcnt[L2_LDS_FAN];
+  __shared__ unsigned long long lcur[L2_LDS_FAN];
+  const int c = blockIdx.x;
+  const size_t s = off1[c], e = off1[c + 1];
+  const unsigned f0 = kidx[c], m = kidx[c + 1] - f0;
+  if (m == 0 || s == e) {
+    // no keys here; leaves of an empty coarse bucket start (and end) at s
+    for (unsigned j = threadIdx.x; j < m; j += 256) fine_off[f0 + j] = s;
+    return;
   }
+  const bool in_lds = m <= unsigned(L2_LDS_FAN);
+  unsigned* cnt = in_lds ? lcnt : gcnt + f0;
+  unsigned long long* cur = in_lds ? lcur : gcur + f0;
+  const uint64_t* sp = in_lds ? spl : fine + f0;
+  if (in_lds)
+    for (unsigned j = threadIdx.x; j < m; j += 256) spl[j] = fine[f0 + j];
+  for (unsigned j = threadIdx.x; j < m; j += 256) cnt[j] = 0;
   __syncthreads();
   for (size_t i = s + threadIdx.x; i < e; i += 256)
-    atomicAdd(&cnt[bucket_of(spl, LS_B2, to_ordered(in[i], is_signed))], 1u);
+    atomicAdd(&cnt[bucket_of(sp, int(m), to_ordered(in[i], is_signed))], 1u);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long run = s;
-    for (int j = 0; j < LS_B2; j++) {
-      base[j] = run;
-      fine_off[size_t(b) * LS_B2 + j] = run;
-      run += cnt[j];
-      cnt[j] = 0;
+  // exclusive scan of the m counts; leaves start at s + prefix
+  const unsigned per = (m + 255) / 256;
+  unsigned long long local = 0;
+  for (unsigned q = 0; q < per; q++) {
+    const unsigned j = threadIdx.x * per + q;
+    if (j < m) local += cnt[j];
+  }
+  {
+    __shared__ unsigned long long tmp64[4];
+    unsigned long long r = s + block_excl_scan<256>(local, tmp64, (unsigned long long*)nullptr);
+    for (unsigned q = 0; q < per; q++) {
+      const unsigned j = threadIdx.x * per + q;
+      if (j < m) {
+        fine_off[f0 + j] = r;
+        cur[j] = r;
+        r += cnt[j];
+      }
     }
   }
   __syncthreads();
   for (size_t i = s + threadIdx.x; i < e; i += 256) {
     const uint64_t k = in[i];
-    const int f = bucket_of(spl, LS_B2, to_ordered(k, is_signed));
-    out[base[f] + atomicAdd(&cnt[f], 1u)] = k;
+    const int f = bucket_of(sp, int(m), to_ordered(k, is_signed));
+    out[atomicAdd(&cur[f], 1ull)] = k;
   }
 }
 
 // one workgroup per fine bucket: LSD radix sort in LDS of (key - min)
 __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ in,
                                                    const unsigned long long* __restrict__ fine_off,
+                                                   const uint64_t* __restrict__ fine, int nf,
                                                    int is_signed, uint64_t* __restrict__ out,
                                                    unsigned* __restrict__ overflow_list,
                                                    unsigned* __restrict__ overflow_n) {
@@ -172,6 +200,11 @@ __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ 
   const size_t s = fine_off[f], e = fine_off[f + 1];
   const unsigned m = unsigned(e - s);
   if (e - s > size_t(L3_CAP)) {
+    // a single-value bucket [fine[f], fine[f] + 1) holds equal keys: copy it
+    if (f + 1 < nf && fine[f + 1] == fine[f] + 1) {
+      for (size_t i = s + threadIdx.x; i < e; i += L3_NT) out[i] = in[i];
+      return;
+    }
     if (threadIdx.x == 0) overflow_list[atomicAdd(overflow_n, 1u)] = unsigned(f);
     return;
   }
@@ -276,8 +309,9 @@ struct fhip_cdf_model {
   double inv_span = 1.0;             // 1 / log2(1 + umax - umin)
   std::vector<float> params;         // MLP parameters
   std::vector<double> knots_cdf;     // distilled monotone CDF at K uniform x knots
-  std::vector<uint64_t> coarse;      // B1 coarse splitters (ordered keys)
-  std::vector<uint64_t> fine;        // B1*B2 fine splitters
+  std::vector<uint64_t> coarse;      // B1 coarse splitters (ordered keys), each a leaf splitter
+  std::vector<unsigned> kidx;        // B1+1: leaves of coarse bucket c are [kidx[c], kidx[c+1])
+  std::vector<uint64_t> fine;        // NF leaf splitters (sample quantiles)
   std::vector<uint64_t> sample;      // ordered sample keys (sorted)
   size_t train_n = 0;                // rows of the training input
   double final_loss = 0;
@@ -310,6 +344,48 @@ uint64_t cdf_inverse(const fhip_cdf_model& m, double q) {
   return m.umin + uint64_t(std::ceil(d));
 }
 
+// leaves: NF splitters at the sample quantiles; a value that fills two
+// quantile steps gets a leaf of its own, [q, q+1), which the leaf kernel
+// copies without sorting whatever its size
+std::vector<uint64_t> leaf_splitters(const std::vector<uint64_t>& smp) {
+  std::vector<uint64_t> f;
+  f.reserve(LS_NF);
+  f.push_back(0);
+  auto push = [&](uint64_t w) {
+    if (int(f.size()) < LS_NF && w > f.back()) f.push_back(w);
+  };
+  const size_t S = smp.size();
+  uint64_t prev_q = S ? smp[0] : 0;
+  for (size_t j = 1; j < size_t(LS_NF) && S; j++) {
+    const uint64_t q = smp[std::min(S - 1, j * S / LS_NF)];
+    if (q == prev_q) {
+      push(q);
+      if (q != ~0ull) push(q + 1);
+    } else {
+      push(q);
+    }
+    prev_q = q;
+  }
+  while (int(f.size()) < LS_NF) f.push_back(f.back());
+  return f;
+}
+
+// Coarse splitters snapped to leaf splitters, so that every leaf belongs to
+// exactly one coarse bucket.
+void snap_coarse(const std::vector<uint64_t>& want, const std::vector<uint64_t>& f,
+                 std::vector<uint64_t>& coarse, std::vector<unsigned>& kidx) {
+  coarse.assign(LS_B1, 0);
+  kidx.assign(LS_B1 + 1, 0);
+  for (int c = 1; c < LS_B1; c++) {
+    size_t k = size_t(std::lower_bound(f.begin(), f.end(), want[c]) - f.begin());
+    k = std::min(k, size_t(LS_NF - 1));
+    k = std::max<size_t>(k, kidx[c - 1]);
+    kidx[c] = unsigned(k);
+    coarse[c] = f[k];
+  }
+  kidx[LS_B1] = LS_NF;
+}
+
 void distill(fhip_cdf_model& m, const std::vector<float>& y) {
   m.knots_cdf.assign(KNOTS, 0.0);
   double run = 0.0;
@@ -321,30 +397,11 @@ void distill(fhip_cdf_model& m, const std::vector<float>& y) {
   // pin the ends to the sample range
   const double lo = m.knots_cdf.front(), hi = m.knots_cdf.back();
   for (auto& v : m.knots_cdf) v = hi > lo ? (v - lo) / (hi - lo) : 0.0;
-  m.coarse.assign(LS_B1, 0);
-  m.coarse[0] = 0;
-  for (int b = 1; b < LS_B1; b++) {
-    uint64_t sp = cdf_inverse(m, double(b) / LS_B1);
-    m.coarse[b] = std::max(sp, m.coarse[b - 1]);
-  }
-  // leaves: sample quantiles inside each coarse bucket
-  m.fine.assign(size_t(LS_B1) * LS_B2, 0);
-  const std::vector<uint64_t>& s = m.sample;
-  for (int b = 0; b < LS_B1; b++) {
-    const uint64_t lo_k = m.coarse[b];
-    const uint64_t hi_k = b + 1 < LS_B1 ? m.coarse[b + 1] : ~0ull;
-    size_t a = size_t(std::lower_bound(s.begin(), s.end(), lo_k) - s.begin());
-    size_t z = b + 1 < LS_B1 ? size_t(std::lower_bound(s.begin(), s.end(), hi_k) - s.begin())
-                             : s.size();
-    m.fine[size_t(b) * LS_B2] = lo_k;
-    for (int j = 1; j < LS_B2; j++) {
-      uint64_t v = lo_k;
-      if (z > a) v = s[a + (z - a) * size_t(j) / LS_B2];
-      v = std::max(v, m.fine[size_t(b) * LS_B2 + j - 1]);
-      if (b + 1 < LS_B1) v = std::min(v, hi_k);
-      m.fine[size_t(b) * LS_B2 + j] = v;
-    }
-  }
+  std::vector<uint64_t> want(LS_B1, 0);
+  for (int b = 1; b < LS_B1; b++)
+    want[b] = std::max(cdf_inverse(m, double(b) / LS_B1), want[b - 1]);
+  m.fine = leaf_splitters(m.sample);
+  snap_coarse(want, m.fine, m.coarse, m.kidx);
 }
 
 }  // namespace
@@ -443,32 +500,35 @@ fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t sa
 // model's coarse splitters by equal-width ranges of the ordered key space
 // (the same pipeline without the network).
 size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_width,
-                    size_t* fallback_buckets, hipStream_t st) {
+                    size_t* fallback_buckets, size_t* max_coarse, hipStream_t st) {
   if (fallback_buckets) *fallback_buckets = 0;
+  if (max_coarse) *max_coarse = n;
   if (n < 2) return 0;
   const bool sgn = m->ktype == FHIP_I64;
-  std::vector<uint64_t> coarse = m->coarse, fine = m->fine;
-  if (equal_width) {
+  std::vector<uint64_t> coarse = m->coarse;
+  std::vector<unsigned> kidx = m->kidx;
+  if (equal_width) {  // the same pipeline with equal-width coarse buckets, no network
+    std::vector<uint64_t> want(LS_B1, 0);
     const double lo = double(m->umin), hi = double(m->umax);
-    for (int b = 0; b < LS_B1; b++)
-      coarse[b] = b == 0 ? 0 : uint64_t(lo + (hi - lo) * double(b) / LS_B1);
-    for (int b = 0; b < LS_B1; b++) {
-      const uint64_t a = coarse[b], z = b + 1 < LS_B1 ? coarse[b + 1] : m->umax;
-      for (int j = 0; j < LS_B2; j++)
-        fine[size_t(b) * LS_B2 + j] = j == 0 ? a : a + uint64_t(double(z - a) * j / LS_B2);
-    }
+    for (int b = 1; b < LS_B1; b++) want[b] = uint64_t(lo + (hi - lo) * double(b) / LS_B1);
+    snap_coarse(want, m->fine, coarse, kidx);
   }
-  DBuf sb((coarse.size() + fine.size()) * 8);
+  const size_t nf = LS_NF;
+  DBuf sb((LS_B1 + nf) * 8 + (LS_B1 + 1) * 4);
   uint64_t* dco = sb.as<uint64_t>();
-  uint64_t* dfi = dco + coarse.size();
-  FHIP_CHECK(hipMemcpyAsync(dco, coarse.data(), coarse.size() * 8, hipMemcpyHostToDevice, st));
-  FHIP_CHECK(hipMemcpyAsync(dfi, fine.data(), fine.size() * 8, hipMemcpyHostToDevice, st));
-  const size_t nf = size_t(LS_B1) * LS_B2;
-  DBuf hb((LS_B1 + 1) * 8 * 3), fob((nf + 1) * 8), t1(n * 8), t2(n * 8), ovb((nf + 1) * 4);
+  uint64_t* dfi = dco + LS_B1;
+  unsigned* dk = reinterpret_cast<unsigned*>(dfi + nf);
+  FHIP_CHECK(hipMemcpyAsync(dco, coarse.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
+  FHIP_CHECK(hipMemcpyAsync(dfi, m->fine.data(), nf * 8, hipMemcpyHostToDevice, st));
+  FHIP_CHECK(hipMemcpyAsync(dk, kidx.data(), (LS_B1 + 1) * 4, hipMemcpyHostToDevice, st));
+  DBuf hb((LS_B1 + 1) * 8 * 3), fob((nf + 1) * 8), t1(n * 8), t2(n * 8), ovb((nf + 1) * 4),
+      scr(nf * 12);
   unsigned long long* hist = hb.as<unsigned long long>();
   unsigned long long* off1 = hist + (LS_B1 + 1);
   unsigned long long* cursor = off1 + (LS_B1 + 1);
   unsigned long long* fine_off = fob.as<unsigned long long>();
+  unsigned long long* gcur = scr.as<unsigned long long>();
+  unsigned* gcnt = reinterpret_cast<unsigned*>(gcur + nf);
   FHIP_CHECK(hipMemsetAsync(hist, 0, (LS_B1 + 1) * 8, st));
   FHIP_CHECK(hipMemsetAsync(ovb.get(), 0, 4, st));
   uint64_t* k = static_cast<uint64_t*>(keys);
@@ -478,17 +538,19 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
     l1_count_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, hist);
     FHIP_CHECK(hipGetLastError());
   }
-  // exclusive offsets of the coarse buckets (hist[B1] = 0 -> off1[B1] = n)
+  // exclusive offsets of the coarse buckets
   {
     std::vector<unsigned long long> h(LS_B1 + 1), o(LS_B1 + 1);
     FHIP_CHECK(hipMemcpyAsync(h.data(), hist, LS_B1 * 8, hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipStreamSynchronize(st));
-    unsigned long long run = 0;
+    unsigned long long run = 0, mx = 0;
     for (int b = 0; b < LS_B1; b++) {
       o[b] = run;
       run += h[b];
+      mx = std::max(mx, h[b]);
     }
     o[LS_B1] = run;
+    if (max_coarse) *max_coarse = size_t(mx);
     FHIP_CHECK(hipMemcpyAsync(off1, o.data(), (LS_B1 + 1) * 8, hipMemcpyHostToDevice, st));
     FHIP_CHECK(hipMemcpyAsync(cursor, o.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
     FHIP_CHECK(hipStreamSynchronize(st));  // o/h are host locals
@@ -501,14 +563,14 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
   }
   {
     ProfScope prof("learned_split", st);
-    l2_kernel<<<LS_B1, 256, 0, st>>>(t1.as<uint64_t>(), off1, sgn, dfi, fine_off,
+    l2_kernel<<<LS_B1, 256, 0, st>>>(t1.as<uint64_t>(), off1, dk, sgn, dfi, fine_off, gcnt, gcur,
                                      t2.as<uint64_t>());
     FHIP_CHECK(hipGetLastError());
     FHIP_CHECK(hipMemcpyAsync(fine_off + nf, &n, 8, hipMemcpyHostToDevice, st));
   }
   {
     ProfScope prof("learned_leaf_sort", st);
-    l3_kernel<<<unsigned(nf), L3_NT, 0, st>>>(t2.as<uint64_t>(), fine_off, sgn, k,
+    l3_kernel<<<unsigned(nf), L3_NT, 0, st>>>(t2.as<uint64_t>(), fine_off, dfi, int(nf), sgn, k,
                                               ovb.as<unsigned>() + 1, ovb.as<unsigned>());
     FHIP_CHECK(hipGetLastError());
   }
@@ -517,7 +579,7 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
   FHIP_CHECK(hipStreamSynchronize(st));
   if (fallback_buckets) *fallback_buckets = nov;
   if (nov == 0) return 0;
-  // radix fallback for the fine buckets that did not fit
+  // radix fallback for the leaves that did not fit in LDS
   std::vector<unsigned> ids(nov);
   std::vector<unsigned long long> fo(nf + 1);
   FHIP_CHECK(hipMemcpyAsync(ids.data(), ovb.as<unsigned>() + 1, nov * 4, hipMemcpyDeviceToHost, st));
@@ -589,11 +651,12 @@ int fhip_learned_model_params(const fhip_cdf_model* m, float* params) {
 }
 
 int fhip_dev_learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int equal_width,
-                          size_t* fallback_keys, size_t* fallback_buckets, void* stream) {
+                          size_t* fallback_keys, size_t* fallback_buckets,
+                          size_t* max_coarse_bucket, void* stream) {
   return fhip::guarded([&] {
     FHIP_REQUIRE(m && (n == 0 || keys), FHIP_ERR_ARG, "learned_sort: null pointer");
     size_t fb = fhip::learned_sort(m, keys, n, equal_width != 0, fallback_buckets,
-                                   fhip::as_stream(stream));
+                                   max_coarse_bucket, fhip::as_stream(stream));
     if (fallback_keys) *fallback_keys = fb;
   });
 }

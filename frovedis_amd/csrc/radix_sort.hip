@@ -15,6 +15,8 @@
 // radix_sort.hpp:135-153) without a separate max pass.
 #include "radix_sort.hpp"
 
+#include "fhip_device.hpp"
+
 namespace fhip {
 namespace {
 
@@ -450,7 +452,88 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   // temporaries go back to the pool; later work on this stream is ordered
 }
 
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ p, size_t n) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
+struct AsyncLayout {
+  size_t tiles, lb_words;
+  size_t off_hist, off_base, off_lb0, off_lb1, off_ctr, off_t1k, off_t1v, total;
+};
+
+AsyncLayout async_layout(size_t n, int kbytes, int vbytes) {
+  AsyncLayout L{};
+  L.tiles = ceil_div(n, RS_TILE);
+  L.lb_words = L.tiles * 256;
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    size_t r = o;
+    o += (bytes + 255) & ~size_t(255);
+    return r;
+  };
+  L.off_hist = take(8 * 256 * 8);
+  L.off_base = take(8 * 256 * 8);
+  L.off_lb0 = take(L.lb_words * 8);
+  L.off_lb1 = take(L.lb_words * 8);
+  L.off_ctr = take(256);
+  L.off_t1k = take(n * size_t(kbytes));
+  L.off_t1v = take(n * size_t(vbytes));
+  L.total = o;
+  return L;
+}
+
 }  // namespace
+
+size_t sort_async_workspace_bytes(size_t n, int kbytes, int vbytes) {
+  return async_layout(n, kbytes, vbytes).total;
+}
+
+// Stream-ordered stable sort of unsigned keys on bytes [0, nbytes) with no
+// host synchronisation, no memset and no allocation, so that it can be
+// captured in a HIP graph: every byte position runs a pass (no trivial-pass
+// skip).  kout must differ from kin (and vout from vin).
+void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int vbytes,
+                      const void* vin, void* vout, size_t n, void* ws, hipStream_t st) {
+  if (n == 0) return;
+  FHIP_REQUIRE(kout != kin && (vbytes == 0 || vout != vin), FHIP_ERR_ARG,
+               "sort_pairs_async: output must not alias the input");
+  const AsyncLayout L = async_layout(n, kbytes, vbytes);
+  char* w = static_cast<char*>(ws);
+  auto* ghist = reinterpret_cast<unsigned long long*>(w + L.off_hist);
+  auto* gbase = reinterpret_cast<unsigned long long*>(w + L.off_base);
+  auto* lb0 = reinterpret_cast<unsigned long long*>(w + L.off_lb0);
+  auto* lb1 = reinterpret_cast<unsigned long long*>(w + L.off_lb1);
+  auto* ctr = reinterpret_cast<unsigned*>(w + L.off_ctr);
+  void* t1k = w + L.off_t1k;
+  void* t1v = w + L.off_t1v;
+  zero_u64_kernel<<<grid_for_n(8 * 256), 256, 0, st>>>(ghist, 8 * 256);
+  zero_u64_kernel<<<grid_for_n(L.lb_words), 256, 0, st>>>(lb0, L.lb_words);
+  zero_u64_kernel<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(ctr), 1);
+  FHIP_CHECK(hipGetLastError());
+  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * 4), size_t(fill_blocks(RS_NT, 4))));
+  if (kbytes == 4)
+    rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, KM_RAW, 0, nbytes, ghist);
+  else
+    rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, KM_RAW, 0, nbytes, ghist);
+  FHIP_CHECK(hipGetLastError());
+  rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
+  FHIP_CHECK(hipGetLastError());
+  PassFn fn = pick_pass(kbytes, vbytes);
+  const void* ks = kin;
+  const void* vs = vin;
+  for (int i = 0; i < nbytes; i++) {
+    const bool to_out = ((nbytes - 1 - i) % 2) == 0;
+    void* kd = to_out ? kout : t1k;
+    void* vd = to_out ? vout : t1v;
+    unsigned long long* lb = (i & 1) ? lb1 : lb0;
+    unsigned long long* lbn = (i & 1) ? lb0 : lb1;
+    fn(ks, kd, vs, vd, n, 8 * i, KM_RAW, 0, gbase + size_t(i) * 256, lb, lbn, ctr + (i & 1),
+       ctr + ((i + 1) & 1), unsigned(L.tiles), st);
+    ks = kd;
+    vs = vd;
+  }
+}
 
 void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* vin,
                 void* vout, size_t n, bool positive_only, bool descending, hipStream_t st) {

@@ -36,4 +36,12 @@ void sort_pairs_bits(int kbytes, const void* kin, void* kout, int vbytes,
                      const void* vin, void* vout, size_t n, int key_bits,
                      hipStream_t stream);
 
+// Graph-capturable variant (no host synchronisation, memset or allocation):
+// stable sort of unsigned kbytes-wide keys on bytes [0, nbytes) into
+// (kout, vout), using a caller-provided workspace of
+// sort_async_workspace_bytes(n, kbytes, vbytes) bytes.
+size_t sort_async_workspace_bytes(size_t n, int kbytes, int vbytes);
+void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int vbytes,
+                      const void* vin, void* vout, size_t n, void* ws, hipStream_t st);
+
 }  // namespace fhip

@@ -172,3 +172,51 @@ def lognormal_mixture_dev(n, seed=SEED_LOGN, device="cuda", chunk=1 << 25, runs=
             rows = sel[c:c + 1024]
             view[rows] = torch.sort(view[rows], dim=1).values
     return out
+
+
+# ---------------------------------------------------------- batched segments
+def segment_sizes(nseg=4096, total=1 << 28, lo=256, hi=1 << 20, seed=SEED_SEGS):
+    """Row counts log-uniform on [lo, hi], rescaled by one common factor and
+    clamped to [lo, hi] so that they add up to `total` (the last rounding
+    remainder goes to the segments with room, largest first)."""
+    u = uniform01_np(seed, 0, nseg)
+    raw = np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))
+
+    def sized(f):
+        return np.clip(np.rint(raw * f), lo, hi).astype(np.int64)
+
+    a, b = 1e-6, 1e6
+    for _ in range(200):
+        mid = math.sqrt(a * b)
+        if sized(mid).sum() < total:
+            a = mid
+        else:
+            b = mid
+    s = sized(a)
+    rem = total - int(s.sum())
+    order = np.argsort(-s, kind="stable")
+    i = 0
+    while rem != 0 and i < 10 * nseg:
+        j = order[i % nseg]
+        step = 1 if rem > 0 else -1
+        if lo <= s[j] + step <= hi:
+            s[j] += step
+            rem -= step
+        i += 1
+    return s
+
+
+def batch_inputs_dev(sizes, seed=SEED_SEGS + 1, device="cuda"):
+    """keys int64 U[0, rows_s / 4) per segment, values double U[0, 1)."""
+    torch = _torch()
+    sizes_t = torch.from_numpy(np.asarray(sizes, np.int64)).to(device)
+    n = int(sizes_t.sum())
+    seg = torch.repeat_interleave(torch.arange(len(sizes), device=device), sizes_t)
+    span = torch.clamp(sizes_t // 4, min=1)[seg]
+    u = uniform01_dev(seed, 0, n, device)
+    keys = torch.floor(u * span.to(torch.float64)).to(torch.int64)
+    del u, seg, span
+    vals = uniform01_dev(seed + 1, 0, n, device)
+    off = torch.zeros(len(sizes) + 1, dtype=torch.int64, device=device)
+    off[1:] = torch.cumsum(sizes_t, 0)
+    return keys, vals, off

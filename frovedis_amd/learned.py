@@ -47,14 +47,15 @@ class CdfModel:
         return p
 
     def sort(self, keys, equal_width=False, stream=None):
-        """Sorts keys in place; returns (fallback_keys, fallback_buckets)."""
+        """Sorts keys in place; returns (fallback_keys, fallback_buckets,
+        max_coarse_bucket)."""
         if keys.dtype != self.dtype:
             raise FhipError("learned sort: key type differs from the model's")
-        fk, fb = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        fk, fb, mc = ctypes.c_size_t(0), ctypes.c_size_t(0), ctypes.c_size_t(0)
         check(lib().fhip_dev_learned_sort(self._m, keys.ptr, _sz(len(keys)), int(bool(equal_width)),
-                                          ctypes.byref(fk), ctypes.byref(fb),
+                                          ctypes.byref(fk), ctypes.byref(fb), ctypes.byref(mc),
                                           stream or current_stream()))
-        return fk.value, fb.value
+        return fk.value, fb.value, mc.value
 
     def cardinality(self, n):
         d = ctypes.c_double(0)

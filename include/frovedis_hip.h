@@ -224,10 +224,12 @@ int fhip_grouped_agg(int op, int vtype, const void* col, size_t col_n,
  *  - fhip_learned_train: FP32 MLP 1->64->64->1 trained on the GPU (Adam,
  *    `steps` full-batch steps, learning rate lr) on (key, rank) samples of an
  *    already sorted device array (every n/samples-th key), then distilled
- *    into 4096 coarse splitter keys plus 16 sample-quantile leaves each.
+ *    into 4096 coarse splitter keys (the network routes keys to coarse
+ *    buckets) over 65536 leaves at the sample quantiles.
  *  - fhip_dev_learned_sort: sorts in place; *fallback_keys gets how many
- *    keys went through the plain radix sort because their fine bucket
- *    overflowed (n when the whole input fell back).  equal_width = 1 runs
+ *    keys went through the plain radix sort because their leaf overflowed
+ *    (n when the whole input fell back); *max_coarse_bucket the largest
+ *    coarse bucket (load balance of the split stage).  equal_width = 1 runs
  *    the same pipeline with equal-width coarse buckets instead of the
  *    network (for comparison).
  *  - fhip_learned_cardinality: expected distinct keys among n draws, from
@@ -241,7 +243,8 @@ int fhip_learned_train(int ktype, const void* sorted_keys, size_t n,
                        fhip_cdf_model** model, void* stream);
 int fhip_dev_learned_sort(const fhip_cdf_model* m, void* keys, size_t n,
                           int equal_width, size_t* fallback_keys,
-                          size_t* fallback_buckets, void* stream);
+                          size_t* fallback_buckets, size_t* max_coarse_bucket,
+                          void* stream);
 int fhip_learned_cardinality(const fhip_cdf_model* m, size_t n,
                              double* distinct);
 int fhip_learned_model_info(const fhip_cdf_model* m, double* final_loss,
@@ -255,6 +258,24 @@ int fhip_dev_mlp_forward(const float* params, const float* x, size_t S,
                          float* y, void* stream);
 int fhip_dev_mlp_adam(float* params, float* grad, float* m, float* v,
                       int step, float lr, void* stream);
+
+/* ---- batched segments --------------------------------------------------------
+ * No counterpart in the reference: runs, for each of nseg segments
+ * (rows [seg_off[s], seg_off[s+1]) of keys/vals, seg_off on the device), the
+ * reference's radix_sort(key, val) (core/radix_sort.hpp:197-219) and the
+ * per-group sum and count of dfcolumn_grouped_agg.cc (sequential fold,
+ * :153-164), all segments in one HIP graph launch.  Keys must lie in
+ * [0, 2^key_bits).  Outputs: keys_out/vals_out (n, sorted per segment,
+ * stable), per group gkeys/gsum/gcount (capacity entries; groups of segment
+ * s are [gseg_off[s], gseg_off[s+1])), *ngroups.  use_graph = 1 captures
+ * the launch sequence once per shape and replays it. */
+int fhip_dev_batch_sort_groupby(const int64_t* keys, const double* vals,
+                                size_t n, const size_t* seg_off, size_t nseg,
+                                int key_bits, int64_t* keys_out,
+                                double* vals_out, int64_t* gkeys, double* gsum,
+                                uint64_t* gcount, size_t* gseg_off,
+                                size_t capacity, size_t* ngroups,
+                                int use_graph, void* stream);
 
 #ifdef __cplusplus
 }

@@ -191,3 +191,22 @@ def test_group_by_agg_full_size_cross_check(fa):
     s2 = fa.grouped_agg("sum", v, gi, split).t
     absm = fa.grouped_agg("sum", fa.DeviceArray(v.t.abs(), np.float64), gi, split).t
     assert bool(((s.t - s2).abs() <= 2 * FP64_TOL * torch.maximum(s2.abs(), absm)).all())
+
+
+def test_group_by_agg_min_max_small_and_ragged(fa):
+    # chunks whose row count is not a multiple of the wave width, and waves
+    # where every row has the same key (the wave-reduction path)
+    rng = np.random.default_rng(23)
+    for n, nk in [(1, 1), (8, 3), (65, 2), (200, 3), (4099, 5), (100000, 50)]:
+        k = rng.integers(0, nk, n).astype(np.int64)
+        for dt in [np.int32, np.int64, np.float64]:
+            v = rng.integers(-500, 500, n).astype(dt)
+            gk, (mn, mx, sm) = fa.group_by_agg(fa.DeviceArray.from_numpy(k),
+                                               [("min", fa.DeviceArray.from_numpy(v)),
+                                                ("max", fa.DeviceArray.from_numpy(v)),
+                                                ("sum", fa.DeviceArray.from_numpy(v))])
+            keys = np.unique(k)
+            assert np.array_equal(gk.numpy(), keys)
+            assert np.array_equal(mn.numpy(), [v[k == x].min() for x in keys]), (n, dt)
+            assert np.array_equal(mx.numpy(), [v[k == x].max() for x in keys]), (n, dt)
+            assert np.array_equal(sm.numpy(), [v[k == x].sum() for x in keys]), (n, dt)
