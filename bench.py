@@ -151,7 +151,13 @@ def bench_learned(fa, log2n, steps=3):
     fa.radix_sort(keys)
     plain_s, _ = timed(lambda: fa.radix_sort(keys))
     ref = keys.t.clone()
+    fa.check(fa.lib().fhip_prof_enable(1))
     learned_s, fb = timed(lambda: model.sort(keys))
+    stages = {}
+    for name in ("learned_count", "learned_scatter", "learned_split", "learned_leaf_sort"):
+        ms, cnt = _prof_ms(fa, name)
+        stages[name] = {"ms_per_step": ms / steps, "launches_per_step": cnt / steps}
+    fa.check(fa.lib().fhip_prof_enable(0))
     exact = bool(torch.equal(keys.t, ref))
     flat_s, fb_flat = timed(lambda: model.sort(keys, equal_width=True))
     del ref
@@ -161,7 +167,7 @@ def bench_learned(fa, log2n, steps=3):
             "equal_width_ms": flat_s * 1e3, "equal_width_fallback_keys": fb_flat[0],
             "equal_width_max_coarse_bucket": fb_flat[2],
             "plain_keys_per_s": n / plain_s, "learned_keys_per_s": n / learned_s,
-            "train_s": train_s, "model_loss": model.info()["loss"]}
+            "train_s": train_s, "model_loss": model.info()["loss"], "stages": stages}
 
 
 def bench_batch(fa, steps=3):
@@ -196,6 +202,55 @@ def bench_batch(fa, steps=3):
     return {"rows": n, "segments": len(sizes), "groups": G, "ms": best * 1e3,
             "rows_per_s": n / best, "algorithmic_bytes": algo,
             "hbm_frac": algo / best / HBM_PEAK, "graph": True}
+
+
+def bench_join(fa, steps=3, log2_left=27, log2_right=29):
+    """equi_join of a unique 2^27 key column (a random permutation of
+    [0, 2^27)) with a 2^29 column (90% keys in [0, 2^27), 10% misses in
+    [2^27, 2^28)).  "hash": the unique side is the build side
+    (right_df.hash_join(left_df, eq(rk, lk)) -> unique_hashtable, output in
+    probe order); "sortmerge": left_df.hash_join(right_df, eq(lk, rk)) whose
+    build side is not unique (sort both, merge; (key, left, right) order).
+    Algorithmic bytes: 8 per input row + 16 per output pair."""
+    import numpy as np
+    import torch
+
+    from frovedis_amd import datagen as dg
+    nl, nr = 1 << log2_left, 1 << log2_right
+    lk = fa.DeviceArray(dg.perm_keys_dev(nl), np.int64)
+    rk = fa.DeviceArray(dg.fk_keys_dev(nr, domain=nl), np.int64)
+    li = fa.DeviceArray(torch.arange(nl, dtype=torch.int64, device="cuda"), np.uint64)
+    ri = fa.DeviceArray(torch.arange(nr, dtype=torch.int64, device="cuda"), np.uint64)
+    hits = int((rk.t < nl).sum())
+    out = {"left_rows": nl, "right_rows": nr, "pairs": hits}
+    legs = {"hash": (rk, ri, lk, li, "auto", ("ht_build", "ht_probe", "join_compact")),
+            "sortmerge": (lk, li, rk, ri, "auto", ("radix_hist", "radix_pass", "join_merge",
+                                                    "join_write"))}
+    for name, (a, ai, b, bi, strat, kernels) in legs.items():
+        lo, ro, used = fa.equi_join(a, ai, b, bi, strategy=strat, capacity=nr)
+        ok = len(lo) == hits and bool(torch.equal(a.t[lo.t], b.t[ro.t]))
+        ok = ok and used == (name == "hash")
+        del lo, ro
+        torch.cuda.empty_cache()
+        best = float("inf")
+        fa.check(fa.lib().fhip_prof_enable(1))
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lo, ro, used = fa.equi_join(a, ai, b, bi, strategy=strat, capacity=nr)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+            del lo, ro
+        kern = {}
+        for kname in kernels:
+            ms, cnt = _prof_ms(fa, kname)
+            kern[kname] = {"ms_per_step": ms / steps, "launches_per_step": cnt / steps}
+        fa.check(fa.lib().fhip_prof_enable(0))
+        algo = 8 * (nl + nr) + 16 * hits
+        out[name] = {"ms": best * 1e3, "rows_per_s": (nl + nr) / best,
+                     "algorithmic_bytes": algo, "hbm_frac": algo / best / HBM_PEAK,
+                     "pairs_checked": ok, "kernels": kern}
+    return out
 
 
 def main():
@@ -279,6 +334,8 @@ def main():
         torch.cuda.empty_cache()
         print("# batched segments", file=sys.stderr, flush=True)
         result["extra"]["batched_segments_2^28"] = bench_batch(fa)
+        print("# equi_join", file=sys.stderr, flush=True)
+        result["extra"]["join_2^27_x_2^29"] = bench_join(fa)
         torch.cuda.empty_cache()
         print("# learned vs plain sort", file=sys.stderr, flush=True)
         result["extra"]["learned_sort_2^28"] = bench_learned(fa, 28)

@@ -217,6 +217,10 @@ class ProfScope {
   hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
+// Tunable parameters (set by the autotuner through fhip_tune_set); the
+// default is returned when a name was never set.
+long long tune_get(const char* name, long long dflt);
+
 // number of workgroups that fill the device for grid-stride kernels
 int fill_blocks(int threads_per_block, int blocks_per_cu = 8);
 

@@ -147,6 +147,28 @@ inline int grid_for_n(size_t n, int per_cu = 8) {
   return int(std::max<size_t>(1, std::min<size_t>(ceil_div(n, 256), size_t(fill_blocks(256, per_cu)))));
 }
 
+// dst[i] = src[i * stride] for i < count (a strided read gathered into one
+// contiguous block, so that a single linear copy brings it to the host).
+__global__ __launch_bounds__(256) inline void gather_strided_u64_kernel(
+    const unsigned long long* __restrict__ src, size_t stride, size_t count,
+    unsigned long long* __restrict__ dst) {
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < count; i += size_t(gridDim.x) * 256)
+    dst[i] = src[i * stride];
+}
+
+// Copies `count` 64-bit words spaced `stride` words apart from device memory
+// into host memory `dst` and waits for the copy.
+inline void copy_strided_to_host(const void* src, size_t stride, size_t count, void* dst,
+                                 hipStream_t st) {
+  if (count == 0) return;
+  DBuf tmp(count * 8);
+  gather_strided_u64_kernel<<<grid_for_n(count), 256, 0, st>>>(
+      static_cast<const unsigned long long*>(src), stride, count, tmp.as<unsigned long long>());
+  FHIP_CHECK(hipGetLastError());
+  FHIP_CHECK(hipMemcpyAsync(dst, tmp.get(), count * 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+}
+
 template <class F>
 void dispatch_type(int t, F&& f) {
   switch (t) {

@@ -1,19 +1,27 @@
-// Fused one-pass group_by + aggregation (hash path) for gfx950.
+// Fused group_by + aggregation (hash path) for gfx950.
 //
 // Same result as dftable::group_by(k).select({k}, {sum_as(v), count_as(v),
 // avg_as(v), max_as(i), ...}) on one rank (dftable.cc:346-366,3724-3772 ->
 // dfcolumn_groupby.cc:16-120 -> dfcolumn_grouped_agg.cc): groups ascending by
 // key, aggregates in argument order.  Instead of the reference's sort of
-// (key, row) plus gathers, every row is read once:
-//   1. each workgroup walks a contiguous row range and aggregates into an
-//      LDS open-addressing table (heavy keys of skewed data stay on chip);
-//      rows whose key finds no LDS slot go straight to the global table;
-//   2. at the end the workgroup flushes its LDS entries into one global
-//      open-addressing table in HBM (64-bit atomicCAS on the key, hardware
-//      FP64 / integer atomics on the accumulators);
-//   3. occupied entries are compacted, radix-sorted by key and gathered.
-// Integer sums and counts are exact; FP64 sums use FP64 atomics (the order
-// of additions is not fixed, so they match the reference fold to rounding).
+// (key, row) plus gathers, the rows are hash-partitioned once so that every
+// partition's groups fit one workgroup's LDS table:
+//   1. pa_count: per 4096-row tile, rows per partition (P = 2^pbits
+//      partitions by the top bits of a 64-bit mix of the key);
+//   2. prefix_sum over the (partition, tile) counts; pa_scatter: each tile
+//      stages its key and value columns in LDS in partition order and writes
+//      them out as contiguous runs (coalesced);
+//   3. pa_agg: one workgroup per chunk (at most `groupby.chunk` rows of one
+//      partition) aggregates into an LDS open-addressing table; a wave whose
+//      rows all hit one slot (a heavy key) reduces in registers first;
+//   4. pa_merge: one workgroup per partition merges its chunk partials in
+//      LDS; then the groups are compacted, their keys radix-sorted and the
+//      aggregates gathered in key order.
+// A partition with more groups than its table raises a flag; the pipeline
+// reruns with 4x the partitions, and past 4096 partitions falls back to the
+// sort path.  Counts, integer sums and max/min are exact; FP sums use LDS
+// FP64 atomics (the order of additions is not fixed, so they match the
+// reference fold to rounding).
 #include "fhip_device.hpp"
 #include "radix_sort.hpp"
 
@@ -724,8 +732,13 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   if (n == 0) return;
   const int kb = type_bytes(ktype);
   const int stride = 2 + accs.n;
-  // LDS table: 4096 slots with up to two accumulators, else 2048
-  const int S = accs.n <= 2 ? 2 * PA_SLOTS : PA_SLOTS;
+  // LDS table slots (power of two, capped by LDS) and the expected groups per
+  // partition (load factor) come from the tunables
+  int S = int(tune_get("groupby.slots", accs.n <= 2 ? 2 * PA_SLOTS : PA_SLOTS));
+  S = std::max(256, std::min(S, accs.n <= 2 ? 2 * PA_SLOTS : PA_SLOTS));
+  while (S & (S - 1)) S &= S - 1;
+  const long long fill_pct = std::max(5LL, std::min(90LL, tune_get("groupby.fill_pct", 50)));
+  const unsigned long long chunk = std::max(4096LL, tune_get("groupby.chunk", (long long)PA_CHUNK));
   const size_t lds_table_bytes = size_t(S + 1) * 8 * size_t(2 + accs.n);
   static bool attr_set = false;
   if (!attr_set) {
@@ -746,7 +759,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   // partitions: about PA_GROUPS_PER_PART expected groups each
   const size_t want = table_hint ? table_hint : std::min<size_t>(n, size_t(1) << 20);
   int pbits = 4;
-  while (pbits < PA_MAX_PBITS && (size_t(1) << pbits) * size_t(S / 2) < want) pbits++;
+  const size_t per_part = std::max<size_t>(1, size_t(S) * size_t(fill_pct) / 100);
+  while (pbits < PA_MAX_PBITS && (size_t(1) << pbits) * per_part < want) pbits++;
   for (;;) {
     const int P = 1 << pbits;
     const size_t tiles = ceil_div(n, PS_TILE);
@@ -779,9 +793,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     }
     // partition sizes: inclusive offset at the last tile of every partition
     std::vector<unsigned long long> ends(P);
-    FHIP_CHECK(hipMemcpy2DAsync(ends.data(), 8, incl + (tiles - 1), tiles * 8, 8, P,
-                                hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
+    copy_strided_to_host(incl + (tiles - 1), tiles, size_t(P), ends.data(), st);
     // chunks of at most PA_CHUNK rows per partition
     std::vector<Item> items;
     std::vector<unsigned> part_items(P + 1);
@@ -789,8 +801,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     for (int p = 0; p < P; p++) {
       part_items[p] = unsigned(items.size());
       const unsigned long long e = ends[p];
-      for (unsigned long long b = run_off; b < e; b += PA_CHUNK)
-        items.push_back(Item{b, std::min<unsigned long long>(b + PA_CHUNK, e)});
+      for (unsigned long long b = run_off; b < e; b += chunk)
+        items.push_back(Item{b, std::min<unsigned long long>(b + chunk, e)});
       if (e == run_off) items.push_back(Item{run_off, run_off});
       run_off = e;
     }

@@ -33,12 +33,16 @@ void mlp_grad(const float* params, const float* xs, const float* ts, size_t S, f
 void mlp_forward(const float* params, const float* xs, size_t S, float* ys, hipStream_t st);
 void mlp_adam(float* p, float* g, float* m, float* v, int step, float lr, hipStream_t st);
 extern const int kMlpParams;
+void prefix_sum(int type, const void* in, void* out, size_t n, hipStream_t st);
 
 namespace {
 
 constexpr int LS_B1 = 4096;        // coarse buckets (routed by the network)
 constexpr int LS_NF = 65536;       // leaves (fine buckets, sample quantiles)
 constexpr int L2_LDS_FAN = 2048;   // leaves per coarse bucket kept in LDS
+constexpr int L2_NT = 512;
+constexpr int L2_ITEMS = 16;
+constexpr int L2_CHUNK = L2_NT * L2_ITEMS;  // keys per second-level workgroup
 constexpr int LS_NT = 512;
 constexpr int LS_ITEMS = 32;       // keys per thread in the L1 scatter
 constexpr int L3_NT = 256;
@@ -122,65 +126,112 @@ __global__ __launch_bounds__(LS_NT) void l1_scatter_kernel(const uint64_t* __res
   }
 }
 
-// One workgroup per coarse bucket c: split it into its leaves
-// [kidx[c], kidx[c+1]) (global fine buckets; their splitters are `fine`).
-// Counters live in LDS for up to L2_LDS_FAN leaves, else in the global
-// scratch arrays gcnt / gcur, whose [kidx[c], kidx[c+1]) range belongs to
-// this workgroup alone.
-__global__ __launch_bounds__(256) void l2_kernel(const uint64_t* __restrict__ in,
-                                                 const unsigned long long* __restrict__ off1,
-                                                 const unsigned* __restrict__ kidx, int is_signed,
-                                                 const uint64_t* __restrict__ fine,
-                                                 unsigned long long* __restrict__ fine_off,
-                                                 unsigned* __restrict__ gcnt,
-                                                 unsigned long long* __restrict__ gcur,
-                                                 uint64_t* __restrict__ out) {
+// The second level works on chunks of at most L2_CHUNK keys of one coarse
+// bucket c, whose leaves (fine buckets) are [kidx[c], kidx[c+1]) with
+// splitters `fine`; a heavy coarse bucket is spread over many workgroups.
+// Leaf counters live in LDS when the bucket has at most L2_LDS_FAN leaves,
+// else the global ones are used directly.
+struct L2Item {
+  unsigned long long s, e;  // key range [s, e) of the chunk in the L1 output
+  unsigned c, pad;
+};
+
+__global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restrict__ items,
+                                                         const uint64_t* __restrict__ in,
+                                                         const unsigned* __restrict__ kidx,
+                                                         int is_signed,
+                                                         const uint64_t* __restrict__ fine,
+                                                         unsigned long long* __restrict__ leaf_cnt) {
   __shared__ uint64_t spl[L2_LDS_FAN];
-  __shared__ unsigned lcnt[L2_LDS_FAN];
-  __shared__ unsigned long long lcur[L2_LDS_FAN];
-  const int c = blockIdx.x;
-  const size_t s = off1[c], e = off1[c + 1];
-  const unsigned f0 = kidx[c], m = kidx[c + 1] - f0;
-  if (m == 0 || s == e) {
-    // no keys here; leaves of an empty coarse bucket start (and end) at s
-    for (unsigned j = threadIdx.x; j < m; j += 256) fine_off[f0 + j] = s;
+  __shared__ unsigned cnt[L2_LDS_FAN];
+  const L2Item it = items[blockIdx.x];
+  const unsigned f0 = kidx[it.c], m = kidx[it.c + 1] - f0;
+  if (m == 0 || it.s == it.e) return;
+  if (m == 1) {
+    if (threadIdx.x == 0) atomicAdd(&leaf_cnt[f0], it.e - it.s);
     return;
   }
   const bool in_lds = m <= unsigned(L2_LDS_FAN);
-  unsigned* cnt = in_lds ? lcnt : gcnt + f0;
-  unsigned long long* cur = in_lds ? lcur : gcur + f0;
   const uint64_t* sp = in_lds ? spl : fine + f0;
-  if (in_lds)
-    for (unsigned j = threadIdx.x; j < m; j += 256) spl[j] = fine[f0 + j];
-  for (unsigned j = threadIdx.x; j < m; j += 256) cnt[j] = 0;
-  __syncthreads();
-  for (size_t i = s + threadIdx.x; i < e; i += 256)
-    atomicAdd(&cnt[bucket_of(sp, int(m), to_ordered(in[i], is_signed))], 1u);
-  __syncthreads();
-  // exclusive scan of the m counts; leaves start at s + prefix
-  const unsigned per = (m + 255) / 256;
-  unsigned long long local = 0;
-  for (unsigned q = 0; q < per; q++) {
-    const unsigned j = threadIdx.x * per + q;
-    if (j < m) local += cnt[j];
-  }
-  {
-    __shared__ unsigned long long tmp64[4];
-    unsigned long long r = s + block_excl_scan<256>(local, tmp64, (unsigned long long*)nullptr);
-    for (unsigned q = 0; q < per; q++) {
-      const unsigned j = threadIdx.x * per + q;
-      if (j < m) {
-        fine_off[f0 + j] = r;
-        cur[j] = r;
-        r += cnt[j];
-      }
+  if (in_lds) {
+    for (unsigned j = threadIdx.x; j < m; j += L2_NT) {
+      spl[j] = fine[f0 + j];
+      cnt[j] = 0;
     }
   }
   __syncthreads();
-  for (size_t i = s + threadIdx.x; i < e; i += 256) {
-    const uint64_t k = in[i];
-    const int f = bucket_of(sp, int(m), to_ordered(k, is_signed));
-    out[atomicAdd(&cur[f], 1ull)] = k;
+  for (size_t i = it.s + threadIdx.x; i < it.e; i += L2_NT) {
+    const int f = bucket_of(sp, int(m), to_ordered(in[i], is_signed));
+    if (in_lds) atomicAdd(&cnt[f], 1u);
+    else atomicAdd(&leaf_cnt[f0 + f], 1ull);
+  }
+  __syncthreads();
+  if (in_lds)
+    for (unsigned j = threadIdx.x; j < m; j += L2_NT)
+      if (cnt[j]) atomicAdd(&leaf_cnt[f0 + j], (unsigned long long)cnt[j]);
+}
+
+__global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restrict__ items,
+                                                           const uint64_t* __restrict__ in,
+                                                           const unsigned* __restrict__ kidx,
+                                                           int is_signed,
+                                                           const uint64_t* __restrict__ fine,
+                                                           unsigned long long* __restrict__ cursor,
+                                                           uint64_t* __restrict__ out) {
+  __shared__ uint64_t spl[L2_LDS_FAN];
+  __shared__ unsigned cnt[L2_LDS_FAN];
+  __shared__ unsigned long long base[L2_LDS_FAN];
+  const L2Item it = items[blockIdx.x];
+  const unsigned f0 = kidx[it.c], m = kidx[it.c + 1] - f0;
+  if (m == 0 || it.s == it.e) return;
+  const unsigned len = unsigned(it.e - it.s);
+  if (m == 1) {  // one leaf: the chunk moves as a block
+    if (threadIdx.x == 0) base[0] = atomicAdd(&cursor[f0], (unsigned long long)len);
+    __syncthreads();
+    const unsigned long long b0 = base[0];
+    for (unsigned i = threadIdx.x; i < len; i += L2_NT) out[b0 + i] = in[it.s + i];
+    return;
+  }
+  const bool in_lds = m <= unsigned(L2_LDS_FAN);
+  const uint64_t* sp = in_lds ? spl : fine + f0;
+  if (in_lds) {
+    for (unsigned j = threadIdx.x; j < m; j += L2_NT) {
+      spl[j] = fine[f0 + j];
+      cnt[j] = 0;
+    }
+  }
+  __syncthreads();
+  uint64_t k[L2_ITEMS];
+  int b[L2_ITEMS];
+#pragma unroll
+  for (int j = 0; j < L2_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
+    b[j] = -1;
+    k[j] = 0;
+    if (i < len) {
+      k[j] = in[it.s + i];
+      b[j] = bucket_of(sp, int(m), to_ordered(k[j], is_signed));
+      if (in_lds) atomicAdd(&cnt[b[j]], 1u);
+    }
+  }
+  if (!in_lds) {  // rare wide bucket: one global reservation per key
+#pragma unroll
+    for (int j = 0; j < L2_ITEMS; j++)
+      if (b[j] >= 0) out[atomicAdd(&cursor[f0 + b[j]], 1ull)] = k[j];
+    return;
+  }
+  __syncthreads();
+  for (unsigned j = threadIdx.x; j < m; j += L2_NT) {
+    const unsigned c = cnt[j];
+    base[j] = c ? atomicAdd(&cursor[f0 + j], (unsigned long long)c) : 0;
+    cnt[j] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < L2_ITEMS; j++) {
+    if (b[j] < 0) continue;
+    const unsigned r = atomicAdd(&cnt[b[j]], 1u);
+    out[base[b[j]] + r] = k[j];
   }
 }
 
@@ -437,9 +488,7 @@ fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t sa
     FHIP_CHECK(hipGetLastError());
     // the sample keys themselves (for the leaves)
     std::vector<uint64_t> host_sorted_sample(S);
-    FHIP_CHECK(hipMemcpy2DAsync(host_sorted_sample.data(), 8, sorted, step * 8, 8, S,
-                                hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
+    copy_strided_to_host(sorted, step, S, host_sorted_sample.data(), st);
     m->sample.resize(S);
     for (size_t i = 0; i < S; i++) m->sample[i] = ordered_host(host_sorted_sample[i], sgn);
     // initialise: x spread over [0,1), ReLU units with positive slopes
@@ -522,15 +571,17 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
   FHIP_CHECK(hipMemcpyAsync(dfi, m->fine.data(), nf * 8, hipMemcpyHostToDevice, st));
   FHIP_CHECK(hipMemcpyAsync(dk, kidx.data(), (LS_B1 + 1) * 4, hipMemcpyHostToDevice, st));
   DBuf hb((LS_B1 + 1) * 8 * 3), fob((nf + 1) * 8), t1(n * 8), t2(n * 8), ovb((nf + 1) * 4),
-      scr(nf * 12);
+      lcb(nf * 8 * 2);
   unsigned long long* hist = hb.as<unsigned long long>();
   unsigned long long* off1 = hist + (LS_B1 + 1);
   unsigned long long* cursor = off1 + (LS_B1 + 1);
   unsigned long long* fine_off = fob.as<unsigned long long>();
-  unsigned long long* gcur = scr.as<unsigned long long>();
-  unsigned* gcnt = reinterpret_cast<unsigned*>(gcur + nf);
+  unsigned long long* leaf_cnt = lcb.as<unsigned long long>();
+  unsigned long long* leaf_cur = leaf_cnt + nf;
   FHIP_CHECK(hipMemsetAsync(hist, 0, (LS_B1 + 1) * 8, st));
   FHIP_CHECK(hipMemsetAsync(ovb.get(), 0, 4, st));
+  FHIP_CHECK(hipMemsetAsync(leaf_cnt, 0, nf * 8, st));
+  FHIP_CHECK(hipMemsetAsync(fine_off, 0, 8, st));
   uint64_t* k = static_cast<uint64_t*>(keys);
   {
     ProfScope prof("learned_count", st);
@@ -538,7 +589,8 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
     l1_count_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, hist);
     FHIP_CHECK(hipGetLastError());
   }
-  // exclusive offsets of the coarse buckets
+  // exclusive offsets of the coarse buckets, and the second-level chunks
+  std::vector<L2Item> items;
   {
     std::vector<unsigned long long> h(LS_B1 + 1), o(LS_B1 + 1);
     FHIP_CHECK(hipMemcpyAsync(h.data(), hist, LS_B1 * 8, hipMemcpyDeviceToHost, st));
@@ -546,6 +598,9 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
     unsigned long long run = 0, mx = 0;
     for (int b = 0; b < LS_B1; b++) {
       o[b] = run;
+      for (unsigned long long c = run; c < run + h[b]; c += L2_CHUNK)
+        items.push_back(L2Item{c, std::min<unsigned long long>(c + L2_CHUNK, run + h[b]),
+                               unsigned(b), 0u});
       run += h[b];
       mx = std::max(mx, h[b]);
     }
@@ -555,6 +610,9 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
     FHIP_CHECK(hipMemcpyAsync(cursor, o.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
     FHIP_CHECK(hipStreamSynchronize(st));  // o/h are host locals
   }
+  DBuf itb(std::max<size_t>(1, items.size()) * sizeof(L2Item));
+  FHIP_CHECK(hipMemcpyAsync(itb.get(), items.data(), items.size() * sizeof(L2Item),
+                            hipMemcpyHostToDevice, st));
   {
     ProfScope prof("learned_scatter", st);
     const unsigned grid = unsigned(ceil_div(n, size_t(LS_NT) * LS_ITEMS));
@@ -563,10 +621,15 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
   }
   {
     ProfScope prof("learned_split", st);
-    l2_kernel<<<LS_B1, 256, 0, st>>>(t1.as<uint64_t>(), off1, dk, sgn, dfi, fine_off, gcnt, gcur,
-                                     t2.as<uint64_t>());
+    const unsigned ni = unsigned(items.size());
+    l2_count_kernel<<<ni, L2_NT, 0, st>>>(itb.as<L2Item>(), t1.as<uint64_t>(), dk, sgn, dfi,
+                                          leaf_cnt);
     FHIP_CHECK(hipGetLastError());
-    FHIP_CHECK(hipMemcpyAsync(fine_off + nf, &n, 8, hipMemcpyHostToDevice, st));
+    prefix_sum(FHIP_U64, leaf_cnt, fine_off + 1, nf, st);  // fine_off[nf] = n
+    FHIP_CHECK(hipMemcpyAsync(leaf_cur, fine_off, nf * 8, hipMemcpyDeviceToDevice, st));
+    l2_scatter_kernel<<<ni, L2_NT, 0, st>>>(itb.as<L2Item>(), t1.as<uint64_t>(), dk, sgn, dfi,
+                                            leaf_cur, t2.as<uint64_t>());
+    FHIP_CHECK(hipGetLastError());
   }
   {
     ProfScope prof("learned_leaf_sort", st);

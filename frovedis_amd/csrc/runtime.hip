@@ -82,6 +82,21 @@ std::vector<ProfEntry> g_prof;
 
 bool prof_enabled() { return g_prof_on; }
 
+// ---- tunables ---------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, long long>& tune_map() {
+  static auto* m = new std::map<std::string, long long>();
+  return *m;
+}
+}  // namespace
+
+long long tune_get(const char* name, long long dflt) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = tune_map().find(name);
+  return it == tune_map().end() ? dflt : it->second;
+}
+
 void prof_record(const char* name, hipEvent_t a, hipEvent_t b) {
   std::lock_guard<std::mutex> lk(g_prof_mu);
   g_prof.push_back(ProfEntry{name, a, b});
@@ -151,6 +166,22 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches) {
     }
     *total_ms = t;
     *launches = c;
+  });
+}
+
+int fhip_tune_set(const char* name, long long value) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(name, FHIP_ERR_ARG, "tune_set: null name");
+    std::lock_guard<std::mutex> lk(fhip::g_tune_mu);
+    if (value < 0) fhip::tune_map().erase(name);
+    else fhip::tune_map()[name] = value;
+  });
+}
+
+int fhip_tune_get(const char* name, long long dflt, long long* value) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(name && value, FHIP_ERR_ARG, "tune_get: null pointer");
+    *value = fhip::tune_get(name, dflt);
   });
 }
 

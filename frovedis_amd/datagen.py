@@ -220,3 +220,42 @@ def batch_inputs_dev(sizes, seed=SEED_SEGS + 1, device="cuda"):
     off = torch.zeros(len(sizes) + 1, dtype=torch.int64, device=device)
     off[1:] = torch.cumsum(sizes_t, 0)
     return keys, vals, off
+
+
+# ------------------------------------------------------------- join inputs
+def perm_keys_np(nl, seed=SEED_PERM):
+    """A uniformly random permutation of [0, nl): the stable argsort of the
+    stream's values (top 63 bits), identical on host and device."""
+    z = splitmix64_np(seed, 0, nl) >> np.uint64(1)
+    return np.argsort(z.astype(np.int64), kind="stable").astype(np.int64)
+
+
+def perm_keys_dev(nl, seed=SEED_PERM, device="cuda"):
+    torch = _torch()
+    z = _srl(torch, splitmix64_dev(seed, 0, nl, device), 1)
+    return torch.sort(z, stable=True).indices.to(torch.int64)
+
+
+def fk_keys_np(nr, domain=1 << 27, hit=0.9, seed=SEED_FK):
+    """Probe keys: with probability `hit` U[0, domain), else
+    U[domain, 2 * domain) (a non-negative miss domain).  Element i uses
+    draws 2i (hit or miss) and 2i+1 (the key); domain is a power of two."""
+    bits = domain.bit_length() - 1
+    z = splitmix64_np(seed, 0, 2 * nr)
+    u = (z[0::2] >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    k = (z[1::2] >> np.uint64(64 - bits)).astype(np.int64)
+    return np.where(u < hit, k, k + domain)
+
+
+def fk_keys_dev(nr, domain=1 << 27, hit=0.9, seed=SEED_FK, device="cuda", chunk=1 << 26):
+    torch = _torch()
+    bits = domain.bit_length() - 1
+    out = torch.empty(nr, dtype=torch.int64, device=device)
+    for b in range(0, nr, chunk):
+        m = min(chunk, nr - b)
+        z = splitmix64_dev(seed, 2 * b, 2 * m, device)
+        u = _srl(torch, z[0::2], 11).to(torch.float64) * (2.0 ** -53)
+        k = _srl(torch, z[1::2], 64 - bits)
+        out[b:b + m] = torch.where(u < hit, k, k + domain)
+        del z, u, k
+    return out

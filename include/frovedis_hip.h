@@ -66,6 +66,15 @@ int fhip_dev_free(void* p);
 int fhip_prof_enable(int on);
 int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
 
+/* Tunable parameters, set by the autotuner (frovedis_amd/autotune.py) from
+ * measurements on the device and kept in a JSON cache: "groupby.slots"
+ * (LDS hash table slots, power of two), "groupby.fill_pct" (expected groups
+ * per partition as a percentage of the slots, i.e. the load factor),
+ * "groupby.chunk" (rows per aggregation workgroup), "learned.leaf_cap"...
+ * A negative value removes the setting (back to the built-in default). */
+int fhip_tune_set(const char* name, long long value);
+int fhip_tune_get(const char* name, long long dflt, long long* value);
+
 /* ---- radix_sort ---------------------------------------------------------
  * Stable ascending (or descending) sort of keys with an optional payload,
  * in place.  Replaces radix_sort(K*, V*, size_t, bool positive_only)

@@ -26,3 +26,15 @@ def test_segment_sizes_add_up():
 def test_int_uniform_range():
     a = dg.int_uniform_np(100000, 1000)
     assert a.min() >= 0 and a.max() < 1000 and len(np.unique(a)) == 1000
+
+
+def test_join_generators():
+    from frovedis_amd import datagen as dg
+    p = dg.perm_keys_np(1 << 12)
+    assert np.array_equal(np.sort(p), np.arange(1 << 12))
+    k = dg.fk_keys_np(1 << 16, domain=1 << 10)
+    assert k.min() >= 0 and k.max() < 1 << 11
+    hit = (k < 1 << 10).mean()
+    assert abs(hit - 0.9) < 0.01
+    # element i depends only on draws 2i, 2i+1 (chunked generation agrees)
+    assert np.array_equal(dg.fk_keys_np(100, domain=1 << 10), k[:100])

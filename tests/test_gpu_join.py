@@ -129,10 +129,11 @@ def test_equi_join_scaled_join_config_properties(fa):
     import torch
     from frovedis_amd import datagen as dg
     nb, npb = 1 << 25, 1 << 27
-    build = torch.argsort(dg.uniform01_dev(dg.SEED_PERM, 0, nb))
-    u = dg.uniform01_dev(dg.SEED_FK, 0, npb)
-    z = dg.splitmix64_dev(dg.SEED_FK + 1, 0, npb)
-    pk = torch.where(u < 0.9, dg._srl(torch, z, 1) % nb, nb + dg._srl(torch, z, 1) % nb)
+    build = dg.perm_keys_dev(nb)
+    pk = dg.fk_keys_dev(npb, domain=nb)
+    # the device generators give the host generators' values
+    assert np.array_equal(build[:4096].cpu().numpy(), dg.perm_keys_np(nb)[:4096])
+    assert np.array_equal(pk[-4096:].cpu().numpy(), dg.fk_keys_np(npb, domain=nb)[-4096:])
     bk = fa.DeviceArray(build.to(torch.int64), np.int64)
     pkd = fa.DeviceArray(pk, np.int64)
     bi = fa.DeviceArray(torch.arange(nb, device="cuda", dtype=torch.int64), np.uint64)
