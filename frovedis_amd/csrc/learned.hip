@@ -45,9 +45,13 @@ constexpr int L2_ITEMS = 16;
 constexpr int L2_CHUNK = L2_NT * L2_ITEMS;  // keys per second-level workgroup
 constexpr int LS_NT = 512;
 constexpr int LS_ITEMS = 32;       // keys per thread in the L1 scatter
-constexpr int L3_NT = 256;
+constexpr int L3_NT = 512;
 constexpr int L3_CAP = 8192;       // keys a fine bucket may hold (LDS)
 constexpr int L3_ITEMS = L3_CAP / L3_NT;
+constexpr int L3_B = 2048;         // interpolation sub-buckets per leaf
+constexpr int L3_INS = 32;         // largest sub-bucket sorted by one thread
+constexpr int LSD_NT = 256;
+constexpr int LSD_ITEMS = L3_CAP / LSD_NT;
 
 __device__ __forceinline__ uint64_t to_ordered(uint64_t k, int is_signed) {
   return is_signed ? (k ^ 0x8000000000000000ull) : k;
@@ -235,23 +239,30 @@ __global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restr
   }
 }
 
-// one workgroup per fine bucket: LSD radix sort in LDS of (key - min)
+// One workgroup per fine bucket (leaf), sorted in LDS by interpolation:
+// the leaf's keys are spread over L3_B sub-buckets by
+// floor((u - min) * L3_B / (max - min + 1)) -- non-decreasing in u, so the
+// sub-buckets are ordered -- counted, scattered, and every sub-bucket of at
+// most L3_INS keys is insertion-sorted by one thread.  Leaves inside the
+// model's CDF are close to uniform, so a sub-bucket holds a few keys.  A
+// parallel check over the whole leaf then proves it sorted; a leaf that
+// fails (a large sub-bucket of distinct keys) or exceeds L3_CAP keys goes
+// to the overflow list and is radix-sorted from `in` afterwards.  A
+// single-value leaf [fine[f], fine[f] + 1) of any size is copied.
 __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ in,
                                                    const unsigned long long* __restrict__ fine_off,
                                                    const uint64_t* __restrict__ fine, int nf,
                                                    int is_signed, uint64_t* __restrict__ out,
                                                    unsigned* __restrict__ overflow_list,
                                                    unsigned* __restrict__ overflow_n) {
-  __shared__ uint64_t buf[2][L3_CAP];
-  __shared__ unsigned wcnt[L3_NT / 64][256];
-  __shared__ unsigned tstart[256];
+  __shared__ uint64_t buf[L3_CAP];
+  __shared__ unsigned cur[L3_B];
   __shared__ unsigned scan_tmp[L3_NT / 64];
   __shared__ uint64_t red[2][L3_NT / 64];
+  __shared__ int bad;
   const int f = blockIdx.x;
   const size_t s = fine_off[f], e = fine_off[f + 1];
-  const unsigned m = unsigned(e - s);
   if (e - s > size_t(L3_CAP)) {
-    // a single-value bucket [fine[f], fine[f] + 1) holds equal keys: copy it
     if (f + 1 < nf && fine[f + 1] == fine[f] + 1) {
       for (size_t i = s + threadIdx.x; i < e; i += L3_NT) out[i] = in[i];
       return;
@@ -259,11 +270,127 @@ __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ 
     if (threadIdx.x == 0) overflow_list[atomicAdd(overflow_n, 1u)] = unsigned(f);
     return;
   }
+  const unsigned m = unsigned(e - s);
+  if (m == 0) return;
+  const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  uint64_t u[L3_ITEMS];
+  uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+  for (int j = 0; j < L3_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L3_NT + tid;
+    u[j] = i < m ? to_ordered(in[s + i], is_signed) : 0ull;
+    if (i < m) {
+      mn = u[j] < mn ? u[j] : mn;
+      mx = u[j] > mx ? u[j] : mx;
+    }
+  }
+  for (unsigned i = tid; i < unsigned(L3_B); i += L3_NT) cur[i] = 0;
+  if (tid == 0) bad = 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a = __shfl_xor(mn, o, 64), c = __shfl_xor(mx, o, 64);
+    mn = a < mn ? a : mn;
+    mx = c > mx ? c : mx;
+  }
+  if (lane == 0) {
+    red[0][wave] = mn;
+    red[1][wave] = mx;
+  }
+  __syncthreads();
+  mn = red[0][0];
+  mx = red[1][0];
+#pragma unroll
+  for (int w = 1; w < L3_NT / 64; w++) {
+    mn = red[0][w] < mn ? red[0][w] : mn;
+    mx = red[1][w] > mx ? red[1][w] : mx;
+  }
+  if (mn == mx) {  // one value
+#pragma unroll
+    for (int j = 0; j < L3_ITEMS; j++) {
+      const unsigned i = unsigned(j) * L3_NT + tid;
+      if (i < m) out[s + i] = from_ordered(u[j], is_signed);
+    }
+    return;
+  }
+  const double scale = double(L3_B) / (double(mx - mn) + 1.0);
+  unsigned bk[L3_ITEMS];
+#pragma unroll
+  for (int j = 0; j < L3_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L3_NT + tid;
+    bk[j] = 0;
+    if (i < m) {
+      const unsigned b = unsigned(double(u[j] - mn) * scale);
+      bk[j] = b < unsigned(L3_B) ? b : unsigned(L3_B - 1);
+      atomicAdd(&cur[bk[j]], 1u);
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the sub-bucket counts (L3_B / L3_NT consecutive each)
+  constexpr int PER = L3_B / L3_NT;
+  unsigned c[PER], local = 0;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    c[q] = cur[tid * PER + q];
+    local += c[q];
+  }
+  unsigned run = block_excl_scan<L3_NT>(local, scan_tmp, (unsigned*)nullptr);
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    cur[tid * PER + q] = run;
+    run += c[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < L3_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L3_NT + tid;
+    if (i < m) buf[atomicAdd(&cur[bk[j]], 1u)] = u[j];
+  }
+  __syncthreads();
+  // cur[b] is now the end of sub-bucket b (= the start of b + 1)
+  for (unsigned b = tid; b < unsigned(L3_B); b += L3_NT) {
+    const unsigned lo = b ? cur[b - 1] : 0u, hi = cur[b];
+    if (hi - lo < 2 || hi - lo > unsigned(L3_INS)) continue;
+    for (unsigned i = lo + 1; i < hi; i++) {
+      const uint64_t x = buf[i];
+      unsigned j = i;
+      while (j > lo && buf[j - 1] > x) {
+        buf[j] = buf[j - 1];
+        j--;
+      }
+      buf[j] = x;
+    }
+  }
+  __syncthreads();
+  for (unsigned i = tid + 1; i < m; i += L3_NT)
+    if (buf[i - 1] > buf[i]) bad = 1;
+  __syncthreads();
+  if (bad) {
+    if (tid == 0) overflow_list[atomicAdd(overflow_n, 1u)] = unsigned(f);
+    return;
+  }
+  for (unsigned i = tid; i < m; i += L3_NT) out[s + i] = from_ordered(buf[i], is_signed);
+}
+
+// The leaves that failed the interpolation check (at most L3_CAP keys):
+// one workgroup per listed leaf, LSD radix sort in LDS of (key - min) with
+// 8-bit digits and wave64 ballot ranking.
+__global__ __launch_bounds__(LSD_NT) void l3_lsd_kernel(const uint64_t* __restrict__ in,
+                                                       const unsigned long long* __restrict__ fine_off,
+                                                       const unsigned* __restrict__ leaves,
+                                                       int is_signed, uint64_t* __restrict__ out) {
+  __shared__ uint64_t buf[2][L3_CAP];
+  __shared__ unsigned wcnt[LSD_NT / 64][256];
+  __shared__ unsigned tstart[256];
+  __shared__ unsigned scan_tmp[LSD_NT / 64];
+  __shared__ uint64_t red[2][LSD_NT / 64];
+  const unsigned f = leaves[blockIdx.x];
+  const size_t s = fine_off[f], e = fine_off[f + 1];
+  const unsigned m = unsigned(e - s);  // at most L3_CAP (checked by the caller)
   if (m == 0) return;
   const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
   // load, min/max of the ordered keys
   uint64_t mn = ~0ull, mx = 0;
-  for (unsigned i = tid; i < m; i += L3_NT) {
+  for (unsigned i = tid; i < m; i += LSD_NT) {
     uint64_t u = to_ordered(in[s + i], is_signed);
     buf[0][i] = u;
     mn = u < mn ? u : mn;
@@ -282,7 +409,7 @@ __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ 
   __syncthreads();
   mn = red[0][0];
   mx = red[1][0];
-  for (int w = 1; w < L3_NT / 64; w++) {
+  for (int w = 1; w < LSD_NT / 64; w++) {
     mn = red[0][w] < mn ? red[0][w] : mn;
     mx = red[1][w] > mx ? red[1][w] : mx;
   }
@@ -293,12 +420,12 @@ __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ 
   const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (int p = 0; p < passes; p++) {
     const int sh = 8 * p;
-    for (unsigned i = tid; i < (L3_NT / 64) * 256; i += L3_NT) (&wcnt[0][0])[i] = 0;
+    for (unsigned i = tid; i < (LSD_NT / 64) * 256; i += LSD_NT) (&wcnt[0][0])[i] = 0;
     __syncthreads();
-    unsigned rank[L3_ITEMS], dig[L3_ITEMS];
+    unsigned rank[LSD_ITEMS], dig[LSD_ITEMS];
 #pragma unroll
-    for (int j = 0; j < L3_ITEMS; j++) {
-      const unsigned idx = wave * (64 * L3_ITEMS) + unsigned(j) * 64 + lane;
+    for (int j = 0; j < LSD_ITEMS; j++) {
+      const unsigned idx = wave * (64 * LSD_ITEMS) + unsigned(j) * 64 + lane;
       const bool valid = idx < m;
       const unsigned d = valid ? unsigned(((buf[cur][idx] - mn) >> sh) & 0xFF) : 0u;
       unsigned long long mk = __ballot(valid);
@@ -320,22 +447,22 @@ __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ 
     __syncthreads();
     unsigned total = 0;
 #pragma unroll
-    for (int w = 0; w < L3_NT / 64; w++) {
+    for (int w = 0; w < LSD_NT / 64; w++) {
       const unsigned c = wcnt[w][tid];
       wcnt[w][tid] = total;
       total += c;
     }
-    tstart[tid] = block_excl_scan<L3_NT>(total, scan_tmp, (unsigned*)nullptr);
+    tstart[tid] = block_excl_scan<LSD_NT>(total, scan_tmp, (unsigned*)nullptr);
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < L3_ITEMS; j++) {
-      const unsigned idx = wave * (64 * L3_ITEMS) + unsigned(j) * 64 + lane;
+    for (int j = 0; j < LSD_ITEMS; j++) {
+      const unsigned idx = wave * (64 * LSD_ITEMS) + unsigned(j) * 64 + lane;
       if (idx < m) buf[cur ^ 1][tstart[dig[j]] + wcnt[wave][dig[j]] + rank[j]] = buf[cur][idx];
     }
     __syncthreads();
     cur ^= 1;
   }
-  for (unsigned i = tid; i < m; i += L3_NT) out[s + i] = from_ordered(buf[cur][i], is_signed);
+  for (unsigned i = tid; i < m; i += LSD_NT) out[s + i] = from_ordered(buf[cur][i], is_signed);
 }
 
 __global__ void to_feature_kernel(const uint64_t* __restrict__ sorted, size_t n, size_t step,
@@ -648,15 +775,35 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
   FHIP_CHECK(hipMemcpyAsync(ids.data(), ovb.as<unsigned>() + 1, nov * 4, hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipMemcpyAsync(fo.data(), fine_off, (nf + 1) * 8, hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
-  size_t fb_keys = 0;
-  for (unsigned f : ids) fb_keys += size_t(fo[f + 1] - fo[f]);
-  if (fb_keys * 4 > n || nov > 256) {
+  // leaves that fit in LDS: one batched launch of the LSD leaf sort; larger
+  // ones: the device radix sort, leaf by leaf
+  std::vector<unsigned> small;
+  size_t fb_keys = 0, big_keys = 0, nbig = 0;
+  for (unsigned f : ids) {
+    const size_t len = size_t(fo[f + 1] - fo[f]);
+    fb_keys += len;
+    if (len <= size_t(L3_CAP)) small.push_back(f);
+    else {
+      big_keys += len;
+      nbig++;
+    }
+  }
+  if (big_keys * 4 > n || nbig > 256) {
     // too much of the input missed the model: plain radix sort of all keys
     sort_pairs(m->ktype, t2.get(), k, 0, nullptr, nullptr, n, false, false, st);
     return n;
   }
+  if (!small.empty()) {
+    DBuf lb(small.size() * 4);
+    FHIP_CHECK(hipMemcpyAsync(lb.get(), small.data(), small.size() * 4, hipMemcpyHostToDevice, st));
+    l3_lsd_kernel<<<unsigned(small.size()), LSD_NT, 0, st>>>(t2.as<uint64_t>(), fine_off,
+                                                              lb.as<unsigned>(), sgn, k);
+    FHIP_CHECK(hipGetLastError());
+    FHIP_CHECK(hipStreamSynchronize(st));  // `small` is a host local
+  }
   for (unsigned f : ids) {
     const size_t a = size_t(fo[f]), z = size_t(fo[f + 1]);
+    if (z - a <= size_t(L3_CAP)) continue;
     sort_pairs(m->ktype, t2.as<uint64_t>() + a, k + a, 0, nullptr, nullptr, z - a, false, false, st);
   }
   return fb_keys;

@@ -115,6 +115,31 @@ def test_learned_sort_on_unseen_distributions(fa):
         assert torch.equal(k.t, ref)
 
 
+def test_learned_sort_leaf_fallbacks(fa):
+    # a model of uniform keys over [0, 2^40) (leaves about 2^24 wide); the
+    # input adds dense clusters inside single leaves: 3000 distinct keys in a
+    # 2^15-wide range (the interpolation check fails, the leaf goes to the
+    # LDS radix sort) and one 20000-key cluster (larger than a leaf's LDS
+    # capacity, sorted by the device radix sort)
+    import torch
+    from frovedis_amd import datagen as dg
+    n = 1 << 22
+    train = dg.splitmix64_dev(11, 0, n) & ((1 << 40) - 1)
+    model = fa.CdfModel(fa.DeviceArray(_plain_sorted(fa, train), np.int64), steps=300)
+    base = dg.splitmix64_dev(12, 0, n) & ((1 << 40) - 1)
+    parts = [base]
+    for c in range(40):
+        x0 = (c * 0x9E3779B97F4A7C15 % (1 << 40)) & ~((1 << 24) - 1)
+        parts.append(x0 + (1 << 23) + dg.splitmix64_dev(13, c * 4096, 3000) % (1 << 15))
+    parts.append((7 << 32) + (1 << 23) + dg.splitmix64_dev(14, 0, 20000) % (1 << 16))
+    keys = torch.cat(parts)
+    ref = _plain_sorted(fa, keys)
+    k = fa.DeviceArray(keys.clone(), np.int64)
+    fb_keys, fb_buckets, _ = model.sort(k)
+    assert torch.equal(k.t, ref)
+    assert fb_buckets >= 41 and fb_keys >= 40 * 3000 + 20000, (fb_keys, fb_buckets)
+
+
 def test_network_gain_over_equal_width_buckets(fa):
     import torch
     from frovedis_amd import datagen as dg
