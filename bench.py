@@ -8,6 +8,7 @@ Prints one JSON line.  Side measurements (the plain radix sort key/value
 configuration) go under "extra".
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows LOG2] [--no-cpu]
+                  [--no-extra] [--leg groupby|radix|batch|join|learned|copy]
 """
 import argparse
 import ctypes
@@ -253,6 +254,25 @@ def bench_join(fa, steps=3, log2_left=27, log2_right=29):
     return out
 
 
+def bench_copy(log2n, steps=3):
+    """A plain device copy of 2^log2n int64 (reads and writes 8 B each): the
+    reference point for the HBM byte counters and the achievable bandwidth."""
+    import torch
+    n = 1 << log2n
+    src = torch.arange(n, dtype=torch.int64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return {"bytes_read": 8 * n, "bytes_written": 8 * n, "ms": best * 1e3,
+            "gb_per_s": 16 * n / best / 1e9, "hbm_frac": 16 * n / best / HBM_PEAK}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +281,10 @@ def main():
     ap.add_argument("--rows", type=int, default=28, help="log2 of the row count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the side measurements")
+    ap.add_argument("--leg", default="all",
+                    choices=["all", "groupby", "radix", "batch", "join", "learned", "copy"],
+                    help="run one measurement alone (for profiling); 'copy' is a plain "
+                         "device copy of a known size, the counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share only (--gpus {args.gpus} requested); "
@@ -272,6 +296,17 @@ def main():
     import frovedis_amd as fa
     torch.cuda.set_device(0)
     fa.lib()
+    if args.leg not in ("all", "groupby"):
+        legs = {"radix": lambda: bench_radix(fa, args.rows),
+                "batch": lambda: bench_batch(fa),
+                "join": lambda: bench_join(fa),
+                "learned": lambda: bench_learned(fa, args.rows),
+                "copy": lambda: bench_copy(args.rows)}
+        print(f"# leg {args.leg}", file=sys.stderr, flush=True)
+        print(json.dumps({"leg": args.leg, **legs[args.leg]()}))
+        return
+    if args.leg == "groupby":
+        args.no_extra = args.no_cpu = True
     n = 1 << args.rows
     print(f"# generating {n} rows", file=sys.stderr, flush=True)
     k, v, iv = gen_groupby(n)
@@ -319,7 +354,7 @@ def main():
                    "rows": n, "groups": groups, "path": "fused hash group-by"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": traffic, "kernel": "hashagg_kernel",
+                     "traffic": traffic, "kernel": "pa_agg_kernel",
                      "kernel_ms": kern_s * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": None,

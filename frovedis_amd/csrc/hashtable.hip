@@ -9,10 +9,13 @@
 //     row + 1 replaces the reference's sequential overwrite, :1197-1200);
 //   - rows whose key differs from their slot owner's key form the conflict
 //     list in row order, stable radix-sorted by key (:1216-1235);
-//   - probe: slot key, else lower_bound into the conflict list (:1442-1515).
-// In HBM: owner[ts] (4 B), slot_key[ts], slot_val[ts] (8 B) plus the
-// conflict arrays; a probe reads owner, slot key and value of one slot (the
-// three loads depend only on the slot, so they are issued together).
+//   - probe: slot key, else the first equal entry of the conflict list
+//     (lower_bound, :1442-1515).
+// In HBM: one 32-byte slot per table entry holding owner, key and value
+// (a probe reads one sector), the sorted conflict arrays, and for integer
+// keys a secondary open-addressing index over the distinct conflict keys
+// that answers "first equal entry" in one or two sector reads instead of a
+// binary search of ~log2(conflicts) dependent reads.
 #include "hashtable.hpp"
 
 #include "fhip_device.hpp"
@@ -44,44 +47,85 @@ __device__ __forceinline__ unsigned long long home_slot(K k, unsigned long long 
 }
 
 template <class K>
-__global__ void owner_kernel(const K* __restrict__ keys, size_t n, unsigned long long ts,
-                             unsigned* __restrict__ owner) {
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    atomicMax(&owner[home_slot(keys[i], ts)], unsigned(i + 1));
+__device__ __forceinline__ unsigned long long key_to_bits(K k) {
+  if constexpr (sizeof(K) == 4) {
+    unsigned b;
+    __builtin_memcpy(&b, &k, 4);
+    return b;
+  } else {
+    unsigned long long b;
+    __builtin_memcpy(&b, &k, 8);
+    return b;
+  }
 }
 
-// owners write their slot; a non-owner equal to its owner's key is a
-// duplicate (find_miss_unique_check, :302-433)
+template <class K>
+__device__ __forceinline__ K bits_to_key(unsigned long long b) {
+  K k;
+  if constexpr (sizeof(K) == 4) {
+    const unsigned lo = unsigned(b);
+    __builtin_memcpy(&k, &lo, 4);
+  } else {
+    __builtin_memcpy(&k, &b, 8);
+  }
+  return k;
+}
+
+__device__ __forceinline__ unsigned long long cmix(unsigned long long h) {
+  h ^= h >> 33;
+  h *= 0xff51afd7ed558ccdull;
+  h ^= h >> 33;
+  h *= 0xc4ceb9fe1a85ec53ull;
+  h ^= h >> 33;
+  return h;
+}
+
+// the last row in index order to hash to a slot owns it (:1197-1200)
+template <class K>
+__global__ void owner_kernel(const K* __restrict__ keys, size_t n, unsigned long long ts,
+                             fhip_ht_slot* __restrict__ slots) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    atomicMax(&slots[home_slot(keys[i], ts)].owner, unsigned(i + 1));
+}
+
+// owners write their key and value into their slot
 template <class K>
 __global__ void fill_kernel(const K* __restrict__ keys, const uint64_t* __restrict__ vals,
-                            size_t n, unsigned long long ts, const unsigned* __restrict__ owner,
-                            K* __restrict__ slot_key, uint64_t* __restrict__ slot_val,
-                            int* __restrict__ not_unique) {
+                            size_t n, unsigned long long ts, fhip_ht_slot* __restrict__ slots) {
   const size_t stride = size_t(gridDim.x) * blockDim.x;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     const K k = keys[i];
-    const unsigned long long s = home_slot(k, ts);
-    const unsigned o = owner[s];
-    if (o == unsigned(i + 1)) {
-      slot_key[s] = k;
-      if (slot_val) slot_val[s] = vals ? vals[i] : 0;
-    } else if (keys[o - 1] == k) {
-      *not_unique = 1;
+    fhip_ht_slot* sl = &slots[home_slot(k, ts)];
+    if (sl->owner == unsigned(i + 1)) {
+      sl->key = key_to_bits(k);
+      sl->val = vals ? vals[i] : 0;
     }
   }
 }
 
+// a non-owner whose key differs from its slot's goes to the conflict list;
+// one equal to it is a duplicate (find_miss_unique_check, :302-433)
 template <class K>
-struct IsConflict {
-  const K* keys;
-  const unsigned* owner;
-  unsigned long long ts;
-  __device__ bool operator()(size_t i) const {
+__global__ void classify_kernel(const K* __restrict__ keys, size_t n, unsigned long long ts,
+                                const fhip_ht_slot* __restrict__ slots,
+                                uint8_t* __restrict__ conflict, int* __restrict__ not_unique) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
     const K k = keys[i];
-    const unsigned o = owner[home_slot(k, ts)];
-    return o != unsigned(i + 1) && !(keys[o - 1] == k);
+    const fhip_ht_slot* sl = &slots[home_slot(k, ts)];
+    uint8_t c = 0;
+    if (sl->owner != unsigned(i + 1)) {
+      if (bits_to_key<K>(sl->key) == k) *not_unique = 1;
+      else c = 1;
+    }
+    conflict[i] = c;
   }
+}
+
+struct FlagPred {
+  const uint8_t* f;
+  __device__ bool operator()(size_t i) const { return f[i] != 0; }
 };
 
 template <class K>
@@ -103,6 +147,24 @@ __global__ void adjacent_dup_kernel(const K* __restrict__ k, size_t n, int* __re
     if (!(k[i] != k[i - 1])) *flag = 1;
 }
 
+// secondary index: the first row of every run of equal keys in the sorted
+// conflict list; the keys inserted are distinct, so a slot is claimed by
+// its tag alone and readers only run after this kernel
+template <class K>
+__global__ void cidx_kernel(const K* __restrict__ ck, const uint64_t* __restrict__ cv, size_t m,
+                            fhip_ht_cslot* __restrict__ tab, unsigned long long mask) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride) {
+    const K k = ck[i];
+    if (i > 0 && ck[i - 1] == k) continue;
+    const unsigned long long kb = key_to_bits(k);
+    unsigned long long s = cmix(kb) & mask;
+    while (atomicCAS(&tab[s].tag, 0ull, (unsigned long long)(i + 1)) != 0ull) s = (s + 1) & mask;
+    tab[s].key = kb;
+    tab[s].val = cv ? cv[i] : 0;
+  }
+}
+
 // lower_bound exactly as core/lower_bound.hpp:13-101 (`x <= s[mid]` goes left)
 template <class K>
 __device__ __forceinline__ size_t lb_search(const K* s, size_t m, K x) {
@@ -116,28 +178,40 @@ __device__ __forceinline__ size_t lb_search(const K* s, size_t m, K x) {
 
 template <class K>
 __global__ __launch_bounds__(256) void probe_kernel(
-    const K* __restrict__ q, size_t m, unsigned long long ts, const unsigned* __restrict__ owner,
-    const K* __restrict__ slot_key, const uint64_t* __restrict__ slot_val,
-    const K* __restrict__ ck, const uint64_t* __restrict__ cv, size_t nconf,
-    uint64_t* __restrict__ out, uint8_t* __restrict__ found) {
+    const K* __restrict__ q, size_t m, unsigned long long ts,
+    const fhip_ht_slot* __restrict__ slots, const K* __restrict__ ck,
+    const uint64_t* __restrict__ cv, size_t nconf, const fhip_ht_cslot* __restrict__ cidx,
+    unsigned long long cmask, uint64_t* __restrict__ out, uint8_t* __restrict__ found) {
   const size_t stride = size_t(gridDim.x) * blockDim.x;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride) {
     const K k = q[i];
-    const unsigned long long s = home_slot(k, ts);
-    const unsigned o = owner[s];
-    const K sk = slot_key[s];
-    const uint64_t sv = slot_val ? slot_val[s] : 0;
+    const fhip_ht_slot sl = slots[home_slot(k, ts)];
     uint64_t v = 0;
     uint8_t f = 0;
-    if (o) {
-      if (sk == k) {
-        v = sv;
+    if (sl.owner) {
+      if (bits_to_key<K>(sl.key) == k) {
+        v = sl.val;
         f = 1;
       } else if (nconf) {
-        size_t p = lb_search(ck, nconf, k);
-        if (p < nconf && ck[p] == k) {
-          v = cv ? cv[p] : 0;
-          f = 1;
+        if (cidx) {
+          const unsigned long long kb = key_to_bits(k);
+          unsigned long long s = cmix(kb) & cmask;
+          for (;;) {
+            const fhip_ht_cslot c = cidx[s];
+            if (c.tag == 0) break;
+            if (c.key == kb) {
+              v = c.val;
+              f = 1;
+              break;
+            }
+            s = (s + 1) & cmask;
+          }
+        } else {
+          const size_t p = lb_search(ck, nconf, k);
+          if (p < nconf && ck[p] == k) {
+            v = cv ? cv[p] : 0;
+            f = 1;
+          }
         }
       }
     }
@@ -156,17 +230,16 @@ struct NotFound {
   __device__ bool operator()(size_t i) const { return f[i] == 0; }
 };
 
-template <class K>
 struct SlotOwned {
-  const unsigned* owner;
-  __device__ bool operator()(size_t s) const { return owner[s] != 0; }
+  const fhip_ht_slot* slots;
+  __device__ bool operator()(size_t s) const { return slots[s].owner != 0; }
 };
 
 template <class K>
 struct SlotKeyWriter {
-  const K* slot_key;
+  const fhip_ht_slot* slots;
   K* out;
-  __device__ void operator()(size_t pos, size_t s) const { out[pos] = slot_key[s]; }
+  __device__ void operator()(size_t pos, size_t s) const { out[pos] = bits_to_key<K>(slots[s].key); }
 };
 
 template <class T>
@@ -185,25 +258,23 @@ template <class K>
 void build_t(fhip_ht* h, const K* keys, const uint64_t* vals, size_t n, bool check_unique,
              int* is_unique_ok, hipStream_t st) {
   const unsigned long long ts = h->table_size;
-  h->owner = dev_alloc<unsigned>(ts);
-  h->slot_key = dev_alloc<K>(ts);
-  if (h->has_vals) h->slot_val = dev_alloc<uint64_t>(ts);
-  FHIP_CHECK(hipMemsetAsync(h->owner, 0, ts * sizeof(unsigned), st));
-  if (n) {
-    ProfScope prof("ht_build", st);
-    owner_kernel<K><<<grid_for_n(n), 256, 0, st>>>(keys, n, ts, h->owner);
-    FHIP_CHECK(hipGetLastError());
-  }
-  DBuf flag(2 * sizeof(int));
+  h->slots = dev_alloc<fhip_ht_slot>(ts);
+  FHIP_CHECK(hipMemsetAsync(h->slots, 0, ts * sizeof(fhip_ht_slot), st));
+  DBuf flag(2 * sizeof(int)), conflict(std::max<size_t>(n, 1));
   FHIP_CHECK(hipMemsetAsync(flag.get(), 0, 2 * sizeof(int), st));
   if (n) {
-    fill_kernel<K><<<grid_for_n(n), 256, 0, st>>>(keys, vals, n, ts, h->owner, (K*)h->slot_key,
-                                                  h->slot_val, flag.as<int>());
+    ProfScope prof("ht_build", st);
+    owner_kernel<K><<<grid_for_n(n), 256, 0, st>>>(keys, n, ts, h->slots);
+    FHIP_CHECK(hipGetLastError());
+    fill_kernel<K><<<grid_for_n(n), 256, 0, st>>>(keys, vals, n, ts, h->slots);
+    FHIP_CHECK(hipGetLastError());
+    classify_kernel<K><<<grid_for_n(n), 256, 0, st>>>(keys, n, ts, h->slots,
+                                                      conflict.as<uint8_t>(), flag.as<int>());
     FHIP_CHECK(hipGetLastError());
   }
   // conflict list: count, allocate, write in row order
-  IsConflict<K> pred{keys, h->owner, ts};
-  size_t nconf = compact_apply(pred, ConflictWriter<K>{keys, vals, nullptr, nullptr}, n, 0, st);
+  FlagPred pred{conflict.as<uint8_t>()};
+  size_t nconf = n ? compact_apply(pred, ConflictWriter<K>{keys, vals, nullptr, nullptr}, n, 0, st) : 0;
   h->nconf = nconf;
   if (nconf) {
     h->conf_key = dev_alloc<K>(nconf);
@@ -215,6 +286,18 @@ void build_t(fhip_ht* h, const K* keys, const uint64_t* vals, size_t n, bool che
     if (check_unique && nconf > 1) {
       adjacent_dup_kernel<K><<<grid_for_n(nconf), 256, 0, st>>>((const K*)h->conf_key, nconf,
                                                                 flag.as<int>());
+      FHIP_CHECK(hipGetLastError());
+    }
+    if constexpr (std::is_integral<K>::value) {
+      // floating keys keep the binary search: +0.0 and -0.0 compare equal
+      // with different bits, and the reference's sort may place them apart
+      size_t cap = 64;
+      while (cap < 2 * nconf) cap <<= 1;
+      h->cmask = cap - 1;
+      h->cidx = dev_alloc<fhip_ht_cslot>(cap);
+      FHIP_CHECK(hipMemsetAsync(h->cidx, 0, cap * sizeof(fhip_ht_cslot), st));
+      cidx_kernel<K><<<grid_for_n(nconf), 256, 0, st>>>((const K*)h->conf_key, h->conf_val, nconf,
+                                                        h->cidx, h->cmask);
       FHIP_CHECK(hipGetLastError());
     }
   }
@@ -256,11 +339,10 @@ fhip_ht* ht_build(int ktype, const void* keys, const uint64_t* vals, size_t n, b
 
 void ht_destroy(fhip_ht* h) {
   if (!h) return;
-  (void)hipFree(h->owner);
-  (void)hipFree(h->slot_key);
-  (void)hipFree(h->slot_val);
+  (void)hipFree(h->slots);
   (void)hipFree(h->conf_key);
   (void)hipFree(h->conf_val);
+  (void)hipFree(h->cidx);
   delete h;
 }
 
@@ -270,9 +352,9 @@ void ht_probe(const fhip_ht* h, const void* q, size_t m, uint64_t* vals_out, uin
   ProfScope prof("ht_probe", st);
   dispatch_type(h->ktype, [&](auto z) {
     using K = decltype(z);
-    probe_kernel<K><<<grid_for_n(m), 256, 0, st>>>(
-        (const K*)q, m, h->table_size, h->owner, (const K*)h->slot_key, h->slot_val,
-        (const K*)h->conf_key, h->conf_val, h->nconf, vals_out, found);
+    probe_kernel<K><<<grid_for_n(m), 256, 0, st>>>((const K*)q, m, h->table_size, h->slots,
+                                                   (const K*)h->conf_key, h->conf_val, h->nconf,
+                                                   h->cidx, h->cmask, vals_out, found);
   });
   FHIP_CHECK(hipGetLastError());
 }
@@ -338,8 +420,8 @@ int fhip_dev_ht_all_keys(const fhip_ht* ht, void* out, size_t* out_n, void* stre
     size_t owned = 0;
     fhip::dispatch_type(ht->ktype, [&](auto z) {
       using K = decltype(z);
-      owned = fhip::compact_apply(fhip::SlotOwned<K>{ht->owner},
-                                  fhip::SlotKeyWriter<K>{(const K*)ht->slot_key, (K*)out},
+      owned = fhip::compact_apply(fhip::SlotOwned{ht->slots},
+                                  fhip::SlotKeyWriter<K>{ht->slots, (K*)out},
                                   ht->table_size, ht->n, st);
     });
     if (ht->nconf)

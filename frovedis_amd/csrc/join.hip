@@ -2,8 +2,9 @@
 // body, join.cc:10-61 shrink_missed, multi_equi_join :198-302).
 //
 // strategy FHIP_JOIN_AUTO follows the reference: a unique_hashtable is built
-// on RIGHT with the right row ids as values; if the right keys are unique,
-// LEFT is probed in order and the hits are compacted (hash join, output in
+// on RIGHT with the right row ids as values (skipped when an evenly spaced
+// sample of RIGHT already repeats a key, which proves it not unique); if the
+// right keys are unique, LEFT is probed in order and the hits are compacted (hash join, output in
 // left order); otherwise both sides are stable-sorted and every left row
 // (in sorted order) is paired with its run of equal right keys (sort-merge,
 // output in (key, left position, right position) order).
@@ -79,6 +80,44 @@ __global__ void pair_write_kernel(const unsigned long long* __restrict__ incl,
   }
 }
 
+// an evenly spaced sample of the right keys
+template <class K>
+__global__ void sample_kernel(const K* __restrict__ keys, size_t step, size_t m,
+                              K* __restrict__ out) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride)
+    out[i] = keys[i * step];
+}
+
+template <class K>
+__global__ void sample_dup_kernel(const K* __restrict__ k, size_t m, int* __restrict__ flag) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x + 1; i < m; i += stride)
+    if (k[i] == k[i - 1]) *flag = 1;
+}
+
+// True when an evenly spaced sample of the right keys already holds two
+// equal keys: then the right side is certainly not unique and the table
+// build the reference uses to find that out can be skipped.  No duplicate
+// in the sample decides nothing (the table build decides).
+template <class K>
+bool sample_has_duplicate(int ktype, const K* right, size_t nr, hipStream_t st) {
+  constexpr size_t kSample = 1 << 16;
+  if (nr < 2 * kSample) return false;
+  const size_t step = nr / kSample;
+  DBuf smp(kSample * sizeof(K)), flag(sizeof(int));
+  sample_kernel<K><<<grid_for_n(kSample), 256, 0, st>>>(right, step, kSample, smp.as<K>());
+  FHIP_CHECK(hipGetLastError());
+  sort_pairs(ktype, smp.get(), smp.get(), 0, nullptr, nullptr, kSample, false, false, st);
+  FHIP_CHECK(hipMemsetAsync(flag.get(), 0, sizeof(int), st));
+  sample_dup_kernel<K><<<grid_for_n(kSample), 256, 0, st>>>(smp.as<K>(), kSample, flag.as<int>());
+  FHIP_CHECK(hipGetLastError());
+  int f = 0;
+  FHIP_CHECK(hipMemcpyAsync(&f, flag.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  return f != 0;
+}
+
 template <class K>
 size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, const K* right,
                     const size_t* ridx, size_t nr, size_t* lo, size_t* ro, size_t capacity,
@@ -120,7 +159,18 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
   FHIP_REQUIRE(strategy >= FHIP_JOIN_AUTO && strategy <= FHIP_JOIN_SORTMERGE, FHIP_ERR_ARG,
                "equi_join: bad strategy");
   if (used_hash) *used_hash = 0;
-  if (strategy != FHIP_JOIN_SORTMERGE) {
+  bool dup_seen = false;
+  if (strategy == FHIP_JOIN_AUTO) {
+    dispatch_type(ktype, [&](auto z) {
+      using K = decltype(z);
+      // integer keys only: for floating keys the reference's uniqueness
+      // test compares with == and != (NaN, -0.0), which a sample cannot
+      // reproduce
+      if constexpr (std::is_integral<K>::value)
+        dup_seen = sample_has_duplicate<K>(ktype, (const K*)right, nr, st);
+    });
+  }
+  if (strategy != FHIP_JOIN_SORTMERGE && !dup_seen) {
     int uniq = 1;
     fhip_ht* h = ht_build(ktype, right, reinterpret_cast<const uint64_t*>(ridx), nr, true, &uniq, st);
     if (uniq) {

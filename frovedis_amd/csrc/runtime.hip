@@ -2,6 +2,12 @@
 // pool, pinned host staging and the init/finalize entry points.
 #include "fhip_common.hpp"
 
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+
 namespace fhip {
 
 namespace {
@@ -182,6 +188,59 @@ int fhip_tune_get(const char* name, long long dflt, long long* value) {
   return fhip::guarded([&] {
     FHIP_REQUIRE(name && value, FHIP_ERR_ARG, "tune_get: null pointer");
     *value = fhip::tune_get(name, dflt);
+  });
+}
+
+// The tunables as one flat JSON object {"name": integer, ...}.
+int fhip_autotune_save(const char* path) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(path, FHIP_ERR_ARG, "autotune_save: null path");
+    std::lock_guard<std::mutex> lk(fhip::g_tune_mu);
+    FILE* f = std::fopen(path, "w");
+    FHIP_REQUIRE(f, FHIP_ERR_ARG, std::string("autotune_save: cannot open ") + path);
+    std::fputs("{", f);
+    bool first = true;
+    for (auto& kv : fhip::tune_map()) {
+      std::fprintf(f, "%s\n  \"%s\": %lld", first ? "" : ",", kv.first.c_str(), kv.second);
+      first = false;
+    }
+    std::fputs("\n}\n", f);
+    std::fclose(f);
+  });
+}
+
+int fhip_autotune_load(const char* path) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(path, FHIP_ERR_ARG, "autotune_load: null path");
+    FILE* f = std::fopen(path, "r");
+    FHIP_REQUIRE(f, FHIP_ERR_ARG, std::string("autotune_load: cannot open ") + path);
+    std::string text;
+    char buf[4096];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
+    std::fclose(f);
+    // "name": integer pairs; anything else is a format error
+    std::map<std::string, long long> parsed;
+    size_t i = text.find('{');
+    FHIP_REQUIRE(i != std::string::npos, FHIP_ERR_ARG, "autotune_load: not a JSON object");
+    for (;;) {
+      size_t q0 = text.find('"', i + 1);
+      size_t close = text.find('}', i + 1);
+      if (q0 == std::string::npos || (close != std::string::npos && close < q0)) break;
+      size_t q1 = text.find('"', q0 + 1);
+      size_t colon = text.find(':', q1 + 1);
+      FHIP_REQUIRE(q1 != std::string::npos && colon != std::string::npos, FHIP_ERR_ARG,
+                   "autotune_load: malformed entry");
+      char* end = nullptr;
+      const long long v = std::strtoll(text.c_str() + colon + 1, &end, 10);
+      FHIP_REQUIRE(end != text.c_str() + colon + 1, FHIP_ERR_ARG,
+                   "autotune_load: value of " + text.substr(q0 + 1, q1 - q0 - 1) +
+                       " is not an integer");
+      parsed[text.substr(q0 + 1, q1 - q0 - 1)] = v;
+      i = size_t(end - text.c_str());
+    }
+    std::lock_guard<std::mutex> lk(fhip::g_tune_mu);
+    for (auto& kv : parsed) fhip::tune_map()[kv.first] = kv.second;
   });
 }
 

@@ -69,11 +69,16 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
 /* Tunable parameters, set by the autotuner (frovedis_amd/autotune.py) from
  * measurements on the device and kept in a JSON cache: "groupby.slots"
  * (LDS hash table slots, power of two), "groupby.fill_pct" (expected groups
- * per partition as a percentage of the slots, i.e. the load factor),
- * "groupby.chunk" (rows per aggregation workgroup), "learned.leaf_cap"...
- * A negative value removes the setting (back to the built-in default). */
+ * per partition as a percentage of the slots: the hash load factor) and
+ * "groupby.chunk" (rows per aggregation workgroup).  A negative value
+ * removes the setting (back to the built-in default).  No counterpart in
+ * the reference. */
 int fhip_tune_set(const char* name, long long value);
 int fhip_tune_get(const char* name, long long dflt, long long* value);
+/* The tunables as one flat JSON object {"name": integer, ...}: save writes
+ * every value that is set; load sets every value in the file. */
+int fhip_autotune_save(const char* path);
+int fhip_autotune_load(const char* path);
 
 /* ---- radix_sort ---------------------------------------------------------
  * Stable ascending (or descending) sort of keys with an optional payload,

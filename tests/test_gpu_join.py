@@ -145,6 +145,12 @@ def test_equi_join_scaled_join_config_properties(fa):
     assert torch.equal(pk[lo.t], build[ro.t])
     assert bool((lo.t[1:] > lo.t[:-1]).all())  # probe order
     slo, sro, used = fa.equi_join(bk, bi, pkd, pi, strategy="sortmerge")
+    # AUTO on the non-unique side (decided by the sampled duplicate test
+    # without a table build) gives the same pairs in the same order
+    alo, aro, aused = fa.equi_join(bk, bi, pkd, pi)
+    assert not aused
+    assert torch.equal(alo.t, slo.t) and torch.equal(aro.t, sro.t)
+    del alo, aro
     assert len(slo) == hits
     assert torch.equal(build[slo.t], pk[sro.t])
     kk = build[slo.t]
