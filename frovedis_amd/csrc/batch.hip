@@ -25,63 +25,114 @@ __global__ void clear_kernel(unsigned long long* __restrict__ p, size_t n) {
   for (size_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-__global__ void composite_kernel(const int64_t* __restrict__ keys, size_t n,
-                                 const size_t* __restrict__ seg_off, size_t nseg, int key_bits,
-                                 uint64_t* __restrict__ comp, int* __restrict__ err) {
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  const uint64_t lim = key_bits >= 64 ? ~0ull : (1ull << key_bits);
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    // segment: last s with seg_off[s] <= i
-    size_t lo = 0, hi = nseg;
-    while (lo < hi) {
-      size_t mid = (lo + hi + 1) >> 1;
-      if (seg_off[mid] <= i) lo = mid; else hi = mid - 1;
+// Composite keys (segment << key_bits) | key for one tile of CB_TILE
+// consecutive rows per workgroup.  The segment of the tile's first row comes
+// from one binary search over seg_off; the segment boundaries inside the
+// tile (at most CB_MAXB of them) are kept in LDS and every row counts those
+// at or below it.  A tile with more boundaries searches per row.
+constexpr int CB_NT = 256;
+constexpr int CB_TILE = 4096;
+constexpr int CB_MAXB = 64;
+
+__device__ __forceinline__ size_t segment_of(const size_t* seg_off, size_t nseg, size_t i) {
+  // last s < nseg with seg_off[s] <= i
+  size_t lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    size_t mid = (lo + hi + 1) >> 1;
+    if (seg_off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class CT>
+__global__ __launch_bounds__(CB_NT) void composite_kernel(const int64_t* __restrict__ keys,
+                                                          size_t n,
+                                                          const size_t* __restrict__ seg_off,
+                                                          size_t nseg, int key_bits,
+                                                          CT* __restrict__ comp,
+                                                          int* __restrict__ err) {
+  __shared__ size_t bnd[CB_MAXB];
+  __shared__ size_t first_seg;
+  __shared__ int nbnd;
+  const size_t t0 = size_t(blockIdx.x) * CB_TILE;
+  const size_t t1 = t0 + CB_TILE < n ? t0 + CB_TILE : n;
+  if (threadIdx.x == 0) {
+    const size_t s0 = segment_of(seg_off, nseg, t0);
+    int k = 0;
+    for (size_t s = s0 + 1; s < nseg && seg_off[s] < t1; s++) {
+      if (k == CB_MAXB) {
+        k = CB_MAXB + 1;
+        break;
+      }
+      bnd[k++] = seg_off[s];
     }
-    const int64_t k = keys[i];
-    if (k < 0 || uint64_t(k) >= lim) *err = 1;
-    comp[i] = (uint64_t(lo) << key_bits) | (uint64_t(k) & (lim - 1));
+    first_seg = s0;
+    nbnd = k;
+  }
+  __syncthreads();
+  const uint64_t lim = key_bits >= 64 ? ~0ull : (1ull << key_bits);
+  const int k = nbnd;
+  for (size_t i = t0 + threadIdx.x; i < t1; i += CB_NT) {
+    size_t seg;
+    if (k <= CB_MAXB) {
+      seg = first_seg;
+      for (int j = 0; j < k; j++) seg += bnd[j] <= i ? 1 : 0;
+    } else {
+      seg = segment_of(seg_off, nseg, i);
+    }
+    const int64_t key = keys[i];
+    if (key < 0 || uint64_t(key) >= lim) *err = 1;
+    comp[i] = CT((uint64_t(seg) << key_bits) | (uint64_t(key) & (lim - 1)));
   }
 }
 
+template <class CT>
 struct RunStart {
-  const uint64_t* c;
+  const CT* c;
   __device__ bool operator()(size_t i) const { return i == 0 || c[i] != c[i - 1]; }
 };
 
 // one thread per group: fold the run in order (0 + v0 + v1 + ...)
-__global__ void fold_kernel(const uint64_t* __restrict__ comp, const double* __restrict__ vals,
+template <class CT>
+__global__ void fold_kernel(const CT* __restrict__ comp, const double* __restrict__ vals,
                             const size_t* __restrict__ starts, const unsigned long long* G_dev,
                             size_t n, int key_bits, size_t capacity, int64_t* __restrict__ gkeys,
-                            double* __restrict__ gsum, uint64_t* __restrict__ gcount,
-                            int64_t* __restrict__ keys_out) {
+                            double* __restrict__ gsum, uint64_t* __restrict__ gcount) {
   const size_t G = size_t(*G_dev);
   const uint64_t mask = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1);
   const size_t stride = size_t(gridDim.x) * blockDim.x;
   for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < G; g += stride) {
     const size_t s = starts[g], e = g + 1 < G ? starts[g + 1] : n;
     double total = 0;
-    for (size_t i = s; i < e; i++) {
-      total += vals[i];
-      keys_out[i] = int64_t(comp[i] & mask);
-    }
+    for (size_t i = s; i < e; i++) total += vals[i];
     if (g < capacity) {
-      gkeys[g] = int64_t(comp[s] & mask);
+      gkeys[g] = int64_t(uint64_t(comp[s]) & mask);
       gsum[g] = 0.0 + total;
       gcount[g] = e - s;
     }
   }
 }
 
+// the sorted keys without their segment bits (row-parallel, coalesced)
+template <class CT>
+__global__ void strip_kernel(const CT* __restrict__ comp, size_t n, int key_bits,
+                             int64_t* __restrict__ keys_out) {
+  const uint64_t mask = key_bits >= 64 ? ~0ull : ((1ull << key_bits) - 1);
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    keys_out[i] = int64_t(uint64_t(comp[i]) & mask);
+}
+
 // group range of every segment: gseg_off[s] = first group of segment >= s
-__global__ void seg_groups_kernel(const uint64_t* __restrict__ comp,
-                                  const size_t* __restrict__ starts,
+template <class CT>
+__global__ void seg_groups_kernel(const CT* __restrict__ comp, const size_t* __restrict__ starts,
                                   const unsigned long long* G_dev, size_t nseg, int key_bits,
                                   size_t* __restrict__ gseg_off) {
   const size_t G = size_t(*G_dev);
   const size_t stride = size_t(gridDim.x) * blockDim.x;
   for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g <= G; g += stride) {
-    const size_t sg = g < G ? size_t(comp[starts[g]] >> key_bits) : nseg;
-    const size_t sp = g == 0 ? 0 : size_t(comp[starts[g - 1]] >> key_bits) + 1;
+    const size_t sg = g < G ? size_t(uint64_t(comp[starts[g]]) >> key_bits) : nseg;
+    const size_t sp = g == 0 ? 0 : size_t(uint64_t(comp[starts[g - 1]]) >> key_bits) + 1;
     for (size_t s = sp; s <= sg && s <= nseg; s++) gseg_off[s] = g;
   }
 }
@@ -91,6 +142,7 @@ struct Plan {
   const void *keys, *vals, *seg_off;
   size_t n, nseg;
   int key_bits;
+  int cbytes;  // width of the composite key: 4 when segment + key bits fit 32
   void *keys_out, *vals_out, *gkeys, *gsum, *gcount, *gseg_off;
   size_t capacity;
   // owned
@@ -103,7 +155,7 @@ struct Plan {
   }
   bool same(const Plan& o) const {
     return keys == o.keys && vals == o.vals && seg_off == o.seg_off && n == o.n &&
-           nseg == o.nseg && key_bits == o.key_bits && keys_out == o.keys_out &&
+           nseg == o.nseg && key_bits == o.key_bits && cbytes == o.cbytes && keys_out == o.keys_out &&
            vals_out == o.vals_out && gkeys == o.gkeys && gsum == o.gsum && gcount == o.gcount &&
            gseg_off == o.gseg_off && capacity == o.capacity;
   }
@@ -112,25 +164,24 @@ struct Plan {
 std::unique_ptr<Plan> g_plan;
 hipStream_t g_capture_stream = nullptr;
 
-void enqueue(Plan& p, hipStream_t st) {
+template <class CT>
+void enqueue_t(Plan& p, int nbytes, hipStream_t st) {
   const size_t n = p.n;
-  const int total_bits = p.key_bits + (p.nseg > 1 ? 64 - __builtin_clzll(p.nseg - 1) : 0);
-  const int nbytes = std::max(1, (total_bits + 7) / 8);
   unsigned long long* scal = p.scalars.as<unsigned long long>();  // [0]=err [1]=G
   clear_kernel<<<1, 64, 0, st>>>(scal, 4);  // a kernel, not a memset node
-  composite_kernel<<<grid_for_n(n), 256, 0, st>>>(
+  composite_kernel<CT><<<unsigned(ceil_div(n, CB_TILE)), CB_NT, 0, st>>>(
       static_cast<const int64_t*>(p.keys), n, static_cast<const size_t*>(p.seg_off), p.nseg,
-      p.key_bits, p.comp.as<uint64_t>(), reinterpret_cast<int*>(scal));
+      p.key_bits, p.comp.as<CT>(), reinterpret_cast<int*>(scal));
   FHIP_CHECK(hipGetLastError());
   {
     ProfScope prof("batch_sort", st);
-    sort_pairs_async(8, nbytes, p.comp.get(), p.comp_sorted.get(), 8, p.vals, p.vals_out, n,
-                     p.ws.get(), st);
+    sort_pairs_async(int(sizeof(CT)), nbytes, p.comp.get(), p.comp_sorted.get(), 8, p.vals,
+                     p.vals_out, n, p.ws.get(), st);
   }
   // run starts (device-side compaction; the count stays on the device)
   const size_t nb = ceil_div(n, CP_TILE);
   unsigned long long* counts = p.counts.as<unsigned long long>();
-  RunStart pred{p.comp_sorted.as<uint64_t>()};
+  RunStart<CT> pred{p.comp_sorted.as<CT>()};
   compact_count_kernel<<<unsigned(nb), CP_NT, 0, st>>>(pred, n, counts);
   scan_counts_kernel<<<1, 1024, 0, st>>>(counts, nb, scal + 1);
   compact_write_kernel<<<unsigned(nb), CP_NT, 0, st>>>(pred, IdentityMap{}, n, counts,
@@ -138,17 +189,26 @@ void enqueue(Plan& p, hipStream_t st) {
   FHIP_CHECK(hipGetLastError());
   {
     ProfScope prof("batch_fold", st);
-    fold_kernel<<<grid_for_n(n), 256, 0, st>>>(
-        p.comp_sorted.as<uint64_t>(), static_cast<const double*>(p.vals_out),
-        p.starts.as<size_t>(), scal + 1, n, p.key_bits, p.capacity,
-        static_cast<int64_t*>(p.gkeys), static_cast<double*>(p.gsum),
-        static_cast<uint64_t*>(p.gcount), static_cast<int64_t*>(p.keys_out));
+    fold_kernel<CT><<<grid_for_n(n), 256, 0, st>>>(
+        p.comp_sorted.as<CT>(), static_cast<const double*>(p.vals_out), p.starts.as<size_t>(),
+        scal + 1, n, p.key_bits, p.capacity, static_cast<int64_t*>(p.gkeys),
+        static_cast<double*>(p.gsum), static_cast<uint64_t*>(p.gcount));
     FHIP_CHECK(hipGetLastError());
   }
-  seg_groups_kernel<<<grid_for_n(n + 1), 256, 0, st>>>(p.comp_sorted.as<uint64_t>(),
-                                                      p.starts.as<size_t>(), scal + 1, p.nseg,
-                                                      p.key_bits, static_cast<size_t*>(p.gseg_off));
+  strip_kernel<CT><<<grid_for_n(n), 256, 0, st>>>(p.comp_sorted.as<CT>(), n, p.key_bits,
+                                                  static_cast<int64_t*>(p.keys_out));
   FHIP_CHECK(hipGetLastError());
+  seg_groups_kernel<CT><<<grid_for_n(n + 1), 256, 0, st>>>(
+      p.comp_sorted.as<CT>(), p.starts.as<size_t>(), scal + 1, p.nseg, p.key_bits,
+      static_cast<size_t*>(p.gseg_off));
+  FHIP_CHECK(hipGetLastError());
+}
+
+void enqueue(Plan& p, hipStream_t st) {
+  const int total_bits = p.key_bits + (p.nseg > 1 ? 64 - __builtin_clzll(p.nseg - 1) : 0);
+  const int nbytes = std::max(1, (total_bits + 7) / 8);
+  if (p.cbytes == 4) enqueue_t<uint32_t>(p, nbytes, st);
+  else enqueue_t<uint64_t>(p, nbytes, st);
 }
 
 }  // namespace
@@ -173,13 +233,14 @@ void batch_sort_groupby(const void* keys, const void* vals, size_t n, const size
   p->gsum = gsum; p->gcount = gcount; p->gseg_off = gseg_off; p->capacity = capacity;
   const int seg_bits = nseg > 1 ? 64 - __builtin_clzll(nseg - 1) : 0;
   FHIP_REQUIRE(key_bits + seg_bits <= 64, FHIP_ERR_ARG, "batch: key_bits + segment bits > 64");
+  p->cbytes = key_bits + seg_bits <= 32 ? 4 : 8;
   Plan* plan = p.get();
   if (use_graph && g_plan && g_plan->same(*p)) {
     plan = g_plan.get();
   } else {
-    p->ws.reset(sort_async_workspace_bytes(n, 8, 8));
-    p->comp.reset(n * 8);
-    p->comp_sorted.reset(n * 8);
+    p->ws.reset(sort_async_workspace_bytes(n, p->cbytes, 8));
+    p->comp.reset(n * size_t(p->cbytes));
+    p->comp_sorted.reset(n * size_t(p->cbytes));
     p->starts.reset(n * sizeof(size_t));
     p->counts.reset((ceil_div(n, CP_TILE) + 1) * 8);
     p->scalars.reset(4 * 8);

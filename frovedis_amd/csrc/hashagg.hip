@@ -31,11 +31,12 @@ namespace fhip {
 namespace {
 
 constexpr int HA_MAX_ACC = 4;
-constexpr int PA_NT = 512;            // threads of the scatter / aggregate kernels
+constexpr int PA_NT = 1024;           // threads of the aggregate / merge kernels (one per CU: LDS)
 constexpr int PA_UNROLL = 4;          // rows in flight per thread (aggregate)
 constexpr int PS_NT = 256;            // threads of the count / scatter kernels
 constexpr int PS_ITEMS = 16;          // rows per thread
 constexpr int PS_TILE = PS_NT * PS_ITEMS;  // rows per scatter tile
+constexpr int PS_SPAN = 4;             // default tiles per count / scatter workgroup (a "span")
 constexpr unsigned long long PA_CHUNK = 1ull << 17;  // rows per aggregate workgroup
 constexpr int PA_SLOTS = 2048;        // LDS table slots (power of two)
 constexpr int PA_MAX_PBITS = 12;      // 4096 partitions (LDS of the scatter kernel)
@@ -156,39 +157,37 @@ __device__ __forceinline__ unsigned part_of(unsigned long long h, int pbits) {
   return pbits ? unsigned(h >> (64 - pbits)) : 0u;
 }
 
-// rank of this lane among the active lanes with the same value of `p`, and
-// the lane count, when all active lanes agree (wave-uniform fast path)
-__device__ __forceinline__ bool wave_uniform(unsigned p, unsigned long long active) {
-  const unsigned p0 = __shfl(p, __builtin_ffsll((long long)active) - 1, 64);
-  return __ballot(p == p0) == active;
-}
-
-// counts per (partition, tile), partition-major: cnt[p * tiles + t]
+// counts per (partition, span), partition-major: cnt[p * spans + s]; a span
+// is span_tiles consecutive tiles, counted by one workgroup
 template <class K>
 __global__ __launch_bounds__(PS_NT) void pa_count_kernel(const K* __restrict__ keys, size_t n,
-                                                         int pbits, size_t tiles,
+                                                         int pbits, size_t spans, int span_tiles,
                                                          unsigned long long* __restrict__ cnt) {
   extern __shared__ unsigned lh[];
   const int P = 1 << pbits;
   for (int i = threadIdx.x; i < P; i += PS_NT) lh[i] = 0;
   __syncthreads();
-  const size_t t0 = size_t(blockIdx.x) * PS_TILE;
   const unsigned lane = __lane_id();
+  const size_t r0 = size_t(blockIdx.x) * span_tiles * PS_TILE;
+  const size_t r1 = std::min(n, r0 + size_t(span_tiles) * PS_TILE);
+  for (size_t t0 = r0; t0 < r1; t0 += PS_TILE) {
 #pragma unroll 4
-  for (int j = 0; j < PS_ITEMS; j++) {
-    const size_t i = t0 + size_t(j) * PS_NT + threadIdx.x;
-    const bool ok = i < n;
-    const unsigned pp = ok ? part_of(mix64(key_bits(keys[i])), pbits) : 0u;
-    const unsigned long long act = __ballot(ok);
-    if (!act) continue;
-    if (wave_uniform(pp, act)) {
-      if (lane == unsigned(__builtin_ffsll((long long)act) - 1)) atomicAdd(&lh[pp], unsigned(__popcll(act)));
-    } else if (ok) {
-      atomicAdd(&lh[pp], 1u);
+    for (int j = 0; j < PS_ITEMS; j++) {
+      const size_t i = t0 + size_t(j) * PS_NT + threadIdx.x;
+      const bool ok = i < r1;
+      const unsigned pp = ok ? part_of(mix64(key_bits(keys[i])), pbits) : 0u;
+      const unsigned long long act = __ballot(ok);
+      if (!act) continue;
+      // the lanes in the first active lane's partition count with one atomic
+      const int leader = __builtin_ffsll((long long)act) - 1;
+      const unsigned p0 = __shfl(pp, leader, 64);
+      const unsigned long long gm = __ballot(ok && pp == p0);
+      if (lane == unsigned(leader)) atomicAdd(&lh[p0], unsigned(__popcll(gm)));
+      else if (ok && pp != p0) atomicAdd(&lh[pp], 1u);
     }
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < P; q += PS_NT) cnt[size_t(q) * tiles + blockIdx.x] = lh[q];
+  for (int q = threadIdx.x; q < P; q += PS_NT) cnt[size_t(q) * spans + blockIdx.x] = lh[q];
 }
 
 // ---- pass 2: scatter rows (key + accumulator columns) into partitions ------
@@ -199,13 +198,15 @@ struct Cols {
   int n;
 };
 
-// One tile of PS_TILE rows: rows are ranked by partition in LDS, staged in
-// partition order and written out column by column in runs, so consecutive
-// lanes store consecutive addresses.  incl/cnt: inclusive scan and counts of
-// the partition-major count matrix (exact offsets, no global atomics).
+// One span of span_tiles tiles of PS_TILE rows, tile by tile: rows are ranked
+// by partition in LDS, staged in partition order and written out column by
+// column in runs, so consecutive lanes store consecutive addresses.
+// incl/cnt: inclusive scan and counts of the partition-major (partition,
+// span) count matrix (exact offsets, no global atomics); the running offset
+// of every partition advances by each tile's count.
 template <class K>
 __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
-    const K* __restrict__ keys, size_t n, int pbits, size_t tiles, Cols cols,
+    const K* __restrict__ keys, size_t n, int pbits, size_t spans, int span_tiles, Cols cols,
     const unsigned long long* __restrict__ cnt, const unsigned long long* __restrict__ incl,
     K* __restrict__ out_keys) {
   extern __shared__ unsigned long long lds_u64[];
@@ -217,112 +218,122 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
   unsigned short* pid = reinterpret_cast<unsigned short*>(ps + P);         // [PS_TILE]
   __shared__ unsigned scan_tmp[PS_NT / 64];
   const unsigned tid = threadIdx.x, lane = __lane_id();
-  const size_t t = blockIdx.x;
-  const size_t t0 = t * PS_TILE;
-  const unsigned tn = unsigned(n - t0 < size_t(PS_TILE) ? n - t0 : size_t(PS_TILE));
+  const size_t sp = blockIdx.x;
   for (int q = tid; q < P; q += PS_NT) {
-    const size_t m = size_t(q) * tiles + t;
-    gbase[q] = incl[m] - cnt[m];  // exclusive offset of (partition q, this tile)
+    const size_t m = size_t(q) * spans + sp;
+    gbase[q] = incl[m] - cnt[m];  // exclusive offset of (partition q, this span)
     pc[q] = 0;
   }
-  __syncthreads();
-  unsigned part[PS_ITEMS];
-  unsigned short slot[PS_ITEMS];
-#pragma unroll
-  for (int j = 0; j < PS_ITEMS; j++) {
-    const unsigned r = unsigned(j) * PS_NT + tid;
-    part[j] = r < tn ? part_of(mix64(key_bits(keys[t0 + r])), pbits) : 0u;
-  }
-  // counts of this tile per partition
-#pragma unroll
-  for (int j = 0; j < PS_ITEMS; j++) {
-    const unsigned r = unsigned(j) * PS_NT + tid;
-    const bool ok = r < tn;
-    const unsigned long long act = __ballot(ok);
-    if (!act) continue;
-    if (wave_uniform(part[j], act)) {
-      if (lane == unsigned(__builtin_ffsll((long long)act) - 1)) atomicAdd(&pc[part[j]], unsigned(__popcll(act)));
-    } else if (ok) {
-      atomicAdd(&pc[part[j]], 1u);
-    }
-  }
-  __syncthreads();
-  // tile-local start of each partition (block scan, P / PS_NT entries per thread)
-  {
-    const int per = (P + PS_NT - 1) / PS_NT;
-    unsigned local = 0;
-    for (int e = 0; e < per; e++) {
-      const int q = tid * per + e;
-      if (q < P) local += pc[q];
-    }
-    unsigned run = block_excl_scan<PS_NT>(local, scan_tmp, (unsigned*)nullptr);
-    for (int e = 0; e < per; e++) {
-      const int q = tid * per + e;
-      if (q < P) {
-        ps[q] = run;
-        run += pc[q];
-      }
-    }
-  }
-  __syncthreads();
-  for (int q = tid; q < P; q += PS_NT) pc[q] = 0;
-  __syncthreads();
-  // rank within (tile, partition)
+  const size_t r0 = sp * span_tiles * PS_TILE;
+  const size_t r1 = std::min(n, r0 + size_t(span_tiles) * PS_TILE);
   const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-#pragma unroll
-  for (int j = 0; j < PS_ITEMS; j++) {
-    const unsigned r = unsigned(j) * PS_NT + tid;
-    const bool ok = r < tn;
-    const unsigned long long act = __ballot(ok);
-    slot[j] = 0;
-    if (!act) continue;
-    if (wave_uniform(part[j], act)) {
-      const int leader = __builtin_ffsll((long long)act) - 1;
-      unsigned base = 0;
-      if (lane == unsigned(leader)) base = atomicAdd(&pc[part[j]], unsigned(__popcll(act)));
-      base = __shfl(base, leader, 64);
-      if (ok) slot[j] = (unsigned short)(ps[part[j]] + base + __popcll(act & lt));
-    } else if (ok) {
-      slot[j] = (unsigned short)(ps[part[j]] + atomicAdd(&pc[part[j]], 1u));
-    }
-  }
-  // keys: stage in partition order, then write runs
-#pragma unroll
-  for (int j = 0; j < PS_ITEMS; j++) {
-    const unsigned r = unsigned(j) * PS_NT + tid;
-    if (r < tn) {
-      stage[slot[j]] = (unsigned long long)keys[t0 + r];
-      pid[slot[j]] = (unsigned short)part[j];
-    }
-  }
-  __syncthreads();
-  for (unsigned i = tid; i < tn; i += PS_NT) {
-    const unsigned q = pid[i];
-    out_keys[gbase[q] + (i - ps[q])] = K(stage[i]);
-  }
-  for (int a = 0; a < cols.n; a++) {
+  for (size_t t0 = r0; t0 < r1; t0 += PS_TILE) {
     __syncthreads();
+    const unsigned tn = unsigned(r1 - t0 < size_t(PS_TILE) ? r1 - t0 : size_t(PS_TILE));
+    unsigned part[PS_ITEMS];
+    unsigned short slot[PS_ITEMS];
 #pragma unroll
     for (int j = 0; j < PS_ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
-      if (r < tn)
-        stage[slot[j]] = cols.bytes[a] == 4
-                             ? (unsigned long long)static_cast<const unsigned*>(cols.in[a])[t0 + r]
-                             : static_cast<const unsigned long long*>(cols.in[a])[t0 + r];
+      part[j] = r < tn ? part_of(mix64(key_bits(keys[t0 + r])), pbits) : 0u;
+    }
+    // counts of this tile per partition
+#pragma unroll
+    for (int j = 0; j < PS_ITEMS; j++) {
+      const unsigned r = unsigned(j) * PS_NT + tid;
+      const bool ok = r < tn;
+      const unsigned long long act = __ballot(ok);
+      if (!act) continue;
+      const int leader = __builtin_ffsll((long long)act) - 1;
+      const unsigned p0 = __shfl(part[j], leader, 64);
+      const unsigned long long gm = __ballot(ok && part[j] == p0);
+      if (lane == unsigned(leader)) atomicAdd(&pc[p0], unsigned(__popcll(gm)));
+      else if (ok && part[j] != p0) atomicAdd(&pc[part[j]], 1u);
     }
     __syncthreads();
-    if (cols.bytes[a] == 4) {
-      unsigned* o = static_cast<unsigned*>(cols.out[a]);
-      for (unsigned i = tid; i < tn; i += PS_NT) {
-        const unsigned q = pid[i];
-        o[gbase[q] + (i - ps[q])] = unsigned(stage[i]);
+    // tile-local start of each partition (block scan, P / PS_NT entries per thread)
+    {
+      const int per = (P + PS_NT - 1) / PS_NT;
+      unsigned local = 0;
+      for (int e = 0; e < per; e++) {
+        const int q = tid * per + e;
+        if (q < P) local += pc[q];
       }
-    } else {
-      unsigned long long* o = static_cast<unsigned long long*>(cols.out[a]);
-      for (unsigned i = tid; i < tn; i += PS_NT) {
-        const unsigned q = pid[i];
-        o[gbase[q] + (i - ps[q])] = stage[i];
+      unsigned run = block_excl_scan<PS_NT>(local, scan_tmp, (unsigned*)nullptr);
+      for (int e = 0; e < per; e++) {
+        const int q = tid * per + e;
+        if (q < P) {
+          ps[q] = run;
+          run += pc[q];
+        }
       }
+    }
+    __syncthreads();
+    for (int q = tid; q < P; q += PS_NT) pc[q] = 0;
+    __syncthreads();
+    // rank within (tile, partition)
+#pragma unroll
+    for (int j = 0; j < PS_ITEMS; j++) {
+      const unsigned r = unsigned(j) * PS_NT + tid;
+      const bool ok = r < tn;
+      const unsigned long long act = __ballot(ok);
+      slot[j] = 0;
+      if (!act) continue;
+      // the lanes in the first active lane's partition reserve one block
+      const int leader = __builtin_ffsll((long long)act) - 1;
+      const unsigned p0 = __shfl(part[j], leader, 64);
+      const bool in0 = ok && part[j] == p0;
+      const unsigned long long gm = __ballot(in0);
+      unsigned base = 0;
+      if (lane == unsigned(leader)) base = atomicAdd(&pc[p0], unsigned(__popcll(gm)));
+      base = __shfl(base, leader, 64);
+      if (in0) slot[j] = (unsigned short)(ps[p0] + base + __popcll(gm & lt));
+      else if (ok) slot[j] = (unsigned short)(ps[part[j]] + atomicAdd(&pc[part[j]], 1u));
+    }
+    // keys: stage in partition order, then write runs
+#pragma unroll
+    for (int j = 0; j < PS_ITEMS; j++) {
+      const unsigned r = unsigned(j) * PS_NT + tid;
+      if (r < tn) {
+        stage[slot[j]] = (unsigned long long)keys[t0 + r];
+        pid[slot[j]] = (unsigned short)part[j];
+      }
+    }
+    __syncthreads();
+    for (unsigned i = tid; i < tn; i += PS_NT) {
+      const unsigned q = pid[i];
+      out_keys[gbase[q] + (i - ps[q])] = K(stage[i]);
+    }
+    for (int a = 0; a < cols.n; a++) {
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < PS_ITEMS; j++) {
+        const unsigned r = unsigned(j) * PS_NT + tid;
+        if (r < tn)
+          stage[slot[j]] = cols.bytes[a] == 4
+                               ? (unsigned long long)static_cast<const unsigned*>(cols.in[a])[t0 + r]
+                               : static_cast<const unsigned long long*>(cols.in[a])[t0 + r];
+      }
+      __syncthreads();
+      if (cols.bytes[a] == 4) {
+        unsigned* o = static_cast<unsigned*>(cols.out[a]);
+        for (unsigned i = tid; i < tn; i += PS_NT) {
+          const unsigned q = pid[i];
+          o[gbase[q] + (i - ps[q])] = unsigned(stage[i]);
+        }
+      } else {
+        unsigned long long* o = static_cast<unsigned long long*>(cols.out[a]);
+        for (unsigned i = tid; i < tn; i += PS_NT) {
+          const unsigned q = pid[i];
+          o[gbase[q] + (i - ps[q])] = stage[i];
+        }
+      }
+    }
+    __syncthreads();
+    // advance the running offsets by this tile's counts; clear for the next
+    for (int q = tid; q < P; q += PS_NT) {
+      gbase[q] += pc[q];
+      pc[q] = 0;
     }
   }
 }
@@ -500,16 +511,17 @@ __global__ __launch_bounds__(PA_NT) void pa_agg_kernel(const Item* __restrict__ 
       const bool ok = s >= 0;
       const unsigned long long act = __ballot(ok);
       if (!act) continue;
-      if (wave_uniform(unsigned(s), act)) {
+      const int leader = __builtin_ffsll((long long)act) - 1;
+      const int s0 = __shfl(s, leader, 64);
+      if (__ballot(ok && s == s0) == act) {
         // the whole wave updates one slot (a heavy key): reduce, then one
         // atomic per accumulator
-        const int leader = __builtin_ffsll((long long)act) - 1;
-        if (lane == unsigned(leader)) atomicAdd(&t.cnt[s], (unsigned long long)__popcll(act));
+        if (lane == unsigned(leader)) atomicAdd(&t.cnt[s0], (unsigned long long)__popcll(act));
         for (int a = 0; a < na; a++) {
           const unsigned long long w =
               wave_reduce_acc(vv[u][a], accs.a[a].kind, accs.a[a].vtype, act, ok);
           if (lane == unsigned(leader))
-            acc_merge(&t.acc[size_t(a) * (S + 1) + s], accs.a[a].kind, accs.a[a].vtype, w);
+            acc_merge(&t.acc[size_t(a) * (S + 1) + s0], accs.a[a].kind, accs.a[a].vtype, w);
         }
       } else if (ok) {
         atomicAdd(&t.cnt[s], 1ull);
@@ -654,15 +666,17 @@ int grid_for(size_t n, int per_cu = 8) {
 }
 
 template <class K>
-void launch_pa(const void* keys, size_t n, int pbits, size_t tiles, const Cols& cols,
+void launch_pa(const void* keys, size_t n, int pbits, size_t spans, int span_tiles, const Cols& cols,
                unsigned long long* cnt, unsigned long long* incl, void* pkeys, hipStream_t st,
                int pass) {
   const int P = 1 << pbits;
   if (pass == 1) {
-    pa_count_kernel<K><<<unsigned(tiles), PS_NT, P * 4, st>>>((const K*)keys, n, pbits, tiles, cnt);
+    pa_count_kernel<K><<<unsigned(spans), PS_NT, P * 4, st>>>((const K*)keys, n, pbits, spans,
+                                                               span_tiles, cnt);
   } else {
     const size_t lds = size_t(PS_TILE) * 8 + size_t(P) * 16 + size_t(PS_TILE) * 2;
-    pa_scatter_kernel<K><<<unsigned(tiles), PS_NT, lds, st>>>((const K*)keys, n, pbits, tiles, cols,
+    pa_scatter_kernel<K><<<unsigned(spans), PS_NT, lds, st>>>((const K*)keys, n, pbits, spans,
+                                                             span_tiles, cols,
                                                              cnt, incl, (K*)pkeys);
   }
   FHIP_CHECK(hipGetLastError());
@@ -739,6 +753,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   while (S & (S - 1)) S &= S - 1;
   const long long fill_pct = std::max(5LL, std::min(90LL, tune_get("groupby.fill_pct", 50)));
   const unsigned long long chunk = std::max(4096LL, tune_get("groupby.chunk", (long long)PA_CHUNK));
+  const int span_tiles = int(std::max(1LL, std::min(256LL, tune_get("groupby.span", PS_SPAN))));
   const size_t lds_table_bytes = size_t(S + 1) * 8 * size_t(2 + accs.n);
   static bool attr_set = false;
   if (!attr_set) {
@@ -763,7 +778,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   while (pbits < PA_MAX_PBITS && (size_t(1) << pbits) * per_part < want) pbits++;
   for (;;) {
     const int P = 1 << pbits;
-    const size_t tiles = ceil_div(n, PS_TILE);
+    const size_t tiles = ceil_div(ceil_div(n, PS_TILE), size_t(span_tiles));  // spans
     DBuf cb(size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int));
     unsigned long long* cnt = cb.as<unsigned long long>();
     unsigned long long* incl = ib.as<unsigned long long>();
@@ -780,10 +795,10 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     }
     auto run = [&](int pass) {
       switch (ktype) {
-        case FHIP_I32: launch_pa<int32_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        case FHIP_U32: launch_pa<uint32_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        case FHIP_I64: launch_pa<int64_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        default: launch_pa<uint64_t>(keys, n, pbits, tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_I32: launch_pa<int32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_U32: launch_pa<uint32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_I64: launch_pa<int64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        default: launch_pa<uint64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
       }
     };
     {

@@ -70,6 +70,37 @@ __device__ __forceinline__ int bucket_of(const uint64_t* spl, int nb, uint64_t u
   return lo - 1;
 }
 
+// index of the last coarse splitter <= u (spl[0] = 0 <= u, splitters
+// non-decreasing, LS_B1 a power of two): the same as bucket_of over LS_B1
+// splitters, as a fixed-step search whose steps interleave across keys
+__device__ __forceinline__ int coarse_of(const uint64_t* spl, uint64_t u) {
+  int pos = 0;
+#pragma unroll
+  for (int step = LS_B1 / 2; step >= 1; step >>= 1)
+    if (spl[pos + step] <= u) pos += step;
+  return pos;
+}
+
+constexpr int L1C_ITEMS = 8;  // keys per thread per step of the count kernel
+
+// bucket_of over m splitters (sp[0] <= every key) for N keys at once: the
+// steps of the N searches interleave, so their LDS reads overlap
+template <int N>
+__device__ __forceinline__ void leaf_search(const uint64_t* sp, unsigned m, const uint64_t* u,
+                                            int* pos) {
+#pragma unroll
+  for (int j = 0; j < N; j++) pos[j] = 0;
+  unsigned top = 1;
+  while (top * 2 <= m) top *= 2;
+  for (unsigned st = top; st >= 1; st >>= 1) {
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      const unsigned q = unsigned(pos[j]) + st;
+      if (q < m && sp[q] <= u[j]) pos[j] = int(q);
+    }
+  }
+}
+
 __global__ __launch_bounds__(LS_NT) void l1_count_kernel(const uint64_t* __restrict__ keys,
                                                          size_t n, int is_signed,
                                                          const uint64_t* __restrict__ coarse,
@@ -81,9 +112,29 @@ __global__ __launch_bounds__(LS_NT) void l1_count_kernel(const uint64_t* __restr
     h[i] = 0;
   }
   __syncthreads();
-  const size_t stride = size_t(gridDim.x) * LS_NT;
-  for (size_t i = size_t(blockIdx.x) * LS_NT + threadIdx.x; i < n; i += stride)
-    atomicAdd(&h[bucket_of(spl, LS_B1, to_ordered(keys[i], is_signed))], 1u);
+  const size_t step = size_t(gridDim.x) * LS_NT * L1C_ITEMS;
+  for (size_t base = size_t(blockIdx.x) * LS_NT * L1C_ITEMS; base < n; base += step) {
+    uint64_t u[L1C_ITEMS];
+#pragma unroll
+    for (int j = 0; j < L1C_ITEMS; j++) {
+      const size_t i = base + size_t(j) * LS_NT + threadIdx.x;
+      u[j] = i < n ? to_ordered(keys[i], is_signed) : 0ull;
+    }
+    int pos[L1C_ITEMS];
+#pragma unroll
+    for (int j = 0; j < L1C_ITEMS; j++) pos[j] = 0;
+#pragma unroll
+    for (int st = LS_B1 / 2; st >= 1; st >>= 1) {
+#pragma unroll
+      for (int j = 0; j < L1C_ITEMS; j++)
+        if (spl[pos[j] + st] <= u[j]) pos[j] += st;
+    }
+#pragma unroll
+    for (int j = 0; j < L1C_ITEMS; j++) {
+      const size_t i = base + size_t(j) * LS_NT + threadIdx.x;
+      if (i < n) atomicAdd(&h[pos[j]], 1u);
+    }
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < LS_B1; i += LS_NT)
     if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
@@ -107,13 +158,21 @@ __global__ __launch_bounds__(LS_NT) void l1_scatter_kernel(const uint64_t* __res
   int b[LS_ITEMS];
 #pragma unroll
   for (int j = 0; j < LS_ITEMS; j++) {
-    size_t i = tile + size_t(j) * LS_NT + threadIdx.x;
+    const size_t i = tile + size_t(j) * LS_NT + threadIdx.x;
     k[j] = i < n ? keys[i] : 0;
-    b[j] = -1;
-    if (i < n) {
-      b[j] = bucket_of(spl, LS_B1, to_ordered(k[j], is_signed));
-      atomicAdd(&cnt[b[j]], 1u);
-    }
+    b[j] = 0;
+  }
+#pragma unroll
+  for (int st = LS_B1 / 2; st >= 1; st >>= 1) {
+#pragma unroll
+    for (int j = 0; j < LS_ITEMS; j++)
+      if (spl[b[j] + st] <= to_ordered(k[j], is_signed)) b[j] += st;
+  }
+#pragma unroll
+  for (int j = 0; j < LS_ITEMS; j++) {
+    const size_t i = tile + size_t(j) * LS_NT + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[b[j]], 1u);
+    else b[j] = -1;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < LS_B1; i += LS_NT) {
@@ -164,10 +223,21 @@ __global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restric
     }
   }
   __syncthreads();
-  for (size_t i = it.s + threadIdx.x; i < it.e; i += L2_NT) {
-    const int f = bucket_of(sp, int(m), to_ordered(in[i], is_signed));
-    if (in_lds) atomicAdd(&cnt[f], 1u);
-    else atomicAdd(&leaf_cnt[f0 + f], 1ull);
+  const unsigned len = unsigned(it.e - it.s);
+  uint64_t u[L2_ITEMS];
+  int f[L2_ITEMS];
+#pragma unroll
+  for (int j = 0; j < L2_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
+    u[j] = i < len ? to_ordered(in[it.s + i], is_signed) : ~0ull;
+  }
+  leaf_search<L2_ITEMS>(sp, m, u, f);
+#pragma unroll
+  for (int j = 0; j < L2_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
+    if (i >= len) continue;
+    if (in_lds) atomicAdd(&cnt[f[j]], 1u);
+    else atomicAdd(&leaf_cnt[f0 + f[j]], 1ull);
   }
   __syncthreads();
   if (in_lds)
@@ -205,18 +275,20 @@ __global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restr
     }
   }
   __syncthreads();
-  uint64_t k[L2_ITEMS];
+  uint64_t k[L2_ITEMS], u[L2_ITEMS];
   int b[L2_ITEMS];
 #pragma unroll
   for (int j = 0; j < L2_ITEMS; j++) {
     const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
-    b[j] = -1;
-    k[j] = 0;
-    if (i < len) {
-      k[j] = in[it.s + i];
-      b[j] = bucket_of(sp, int(m), to_ordered(k[j], is_signed));
-      if (in_lds) atomicAdd(&cnt[b[j]], 1u);
-    }
+    k[j] = i < len ? in[it.s + i] : 0ull;
+    u[j] = i < len ? to_ordered(k[j], is_signed) : ~0ull;
+  }
+  leaf_search<L2_ITEMS>(sp, m, u, b);
+#pragma unroll
+  for (int j = 0; j < L2_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
+    if (i >= len) b[j] = -1;
+    else if (in_lds) atomicAdd(&cnt[b[j]], 1u);
   }
   if (!in_lds) {  // rare wide bucket: one global reservation per key
 #pragma unroll

@@ -6,7 +6,8 @@
 //     every byte position at once (LDS counters, wave-uniform fast path).
 //  2. rs_scan: exclusive scan of each histogram -> global digit bases.
 //  3. rs_onesweep, once per non-trivial byte: a workgroup takes the next tile
-//     (4096 keys) in arrival order, ranks its keys stably with wave64 ballot
+//     (512 threads x 16 keys by default; the tile shape is tunable) in
+//     arrival order, ranks its keys stably with wave64 ballot
 //     matching, publishes per-digit tile counts, finds its global offsets by
 //     decoupled look-back over earlier tiles, reorders the tile in LDS by
 //     digit and writes keys, then payloads, in digit runs (coalesced).
@@ -21,9 +22,6 @@ namespace fhip {
 namespace {
 
 constexpr int RS_NT = 256;
-constexpr int RS_WAVES = RS_NT / 64;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_NT * RS_ITEMS;
 constexpr unsigned long long LB_FLAG_A = 1ull << 62;  // tile aggregate only
 constexpr unsigned long long LB_FLAG_P = 2ull << 62;  // inclusive prefix
 constexpr unsigned long long LB_VALUE = (1ull << 62) - 1;
@@ -132,52 +130,60 @@ __global__ __launch_bounds__(256) void rs_scan(const unsigned long long* __restr
 }
 
 // ---- one onesweep pass -------------------------------------------------------
-template <class U, int VB>
+// NT threads (a multiple of 64, at least 256) with ITEMS keys each form a
+// tile; the first 256 threads own one digit each for the counts, the scan
+// and the look-back.
+template <class U, int VB, int NT, int ITEMS>
 struct PassLds {
   static constexpr int EB = (VB > int(sizeof(U))) ? VB : int(sizeof(U));
-  unsigned wcnt[RS_WAVES][256];   // per-wave digit counts -> wave offsets
+  static constexpr int WAVES = NT / 64;
+  unsigned wcnt[WAVES][256];      // per-wave digit counts -> wave offsets
   unsigned tstart[256];           // tile-local start of each digit
   unsigned long long gdst[256];   // global position of tile-local index 0
-  unsigned scan_tmp[RS_WAVES];
+  unsigned scan_tmp[WAVES];
   unsigned tile;
-  alignas(16) unsigned char buf[RS_TILE * EB];
+  alignas(16) unsigned char buf[NT * ITEMS * EB];
 };
 
-template <class U, int VB>
-__global__ __launch_bounds__(RS_NT) void rs_onesweep(
+template <class U, int VB, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void rs_onesweep(
     const U* __restrict__ ksrc, U* __restrict__ kdst, const void* __restrict__ vsrc_,
     void* __restrict__ vdst_, size_t n, int shift, int mode, int desc,
     const unsigned long long* __restrict__ gbase, unsigned long long* lookback,
     unsigned long long* lookback_next, unsigned* tile_counter,
     unsigned* tile_counter_next) {
   using V = typename UInt<(VB ? VB : 4)>::T;
-  __shared__ PassLds<U, VB> s;
+  constexpr int WAVES = NT / 64;
+  constexpr int TILE = NT * ITEMS;
+  static_assert(NT >= 256 && NT % 64 == 0, "one thread per digit");
+  __shared__ PassLds<U, VB, NT, ITEMS> s;
   const unsigned tid = threadIdx.x;
   const unsigned lane = __lane_id();
   const unsigned wave = tid >> 6;
+  const bool owns_digit = tid < 256u;
 
   if (tid == 0) s.tile = atomicAdd(tile_counter, 1u);
-  for (unsigned i = tid; i < RS_WAVES * 256; i += RS_NT) (&s.wcnt[0][0])[i] = 0;
+  for (unsigned i = tid; i < WAVES * 256; i += NT) (&s.wcnt[0][0])[i] = 0;
   __syncthreads();
   const unsigned tile = s.tile;
   // clear this tile's look-back slots of the next pass (double buffering)
-  lookback_next[size_t(tile) * 256 + tid] = 0;
+  if (owns_digit) lookback_next[size_t(tile) * 256 + tid] = 0;
   if (tile == 0 && tid == 0) *tile_counter_next = 0;
 
-  const size_t tile_base = size_t(tile) * RS_TILE;
-  const size_t wave_base = tile_base + size_t(wave) * 64 * RS_ITEMS;
+  const size_t tile_base = size_t(tile) * TILE;
+  const size_t wave_base = tile_base + size_t(wave) * 64 * ITEMS;
 
-  U keys[RS_ITEMS];
-  V vals[RS_ITEMS];
+  U keys[ITEMS];
+  V vals[ITEMS];
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; j++) {
+  for (int j = 0; j < ITEMS; j++) {
     size_t idx = wave_base + size_t(j) * 64 + lane;
     keys[j] = idx < n ? ksrc[idx] : U(0);
   }
   if constexpr (VB != 0) {
     const V* vsrc = static_cast<const V*>(vsrc_);
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
       size_t idx = wave_base + size_t(j) * 64 + lane;
       vals[j] = idx < n ? vsrc[idx] : V(0);
     }
@@ -185,11 +191,11 @@ __global__ __launch_bounds__(RS_NT) void rs_onesweep(
 
   // stable in-wave ranking: lanes with equal digits are matched with 8
   // ballots; ranks count earlier items first, then lower lanes.
-  unsigned rank[RS_ITEMS];
-  unsigned digit[RS_ITEMS];
+  unsigned rank[ITEMS];
+  unsigned digit[ITEMS];
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; j++) {
+  for (int j = 0; j < ITEMS; j++) {
     size_t idx = wave_base + size_t(j) * 64 + lane;
     const bool valid = idx < n;
     unsigned d = key_digit(keys[j], mode, shift, desc);
@@ -212,54 +218,57 @@ __global__ __launch_bounds__(RS_NT) void rs_onesweep(
   }
   __syncthreads();
 
-  // per-digit totals and wave offsets (thread t owns digit t)
+  // per-digit totals and wave offsets (thread t < 256 owns digit t)
   unsigned total = 0;
+  if (owns_digit) {
 #pragma unroll
-  for (int w = 0; w < RS_WAVES; w++) {
-    unsigned c = s.wcnt[w][tid];
-    s.wcnt[w][tid] = total;
-    total += c;
+    for (int w = 0; w < WAVES; w++) {
+      unsigned c = s.wcnt[w][tid];
+      s.wcnt[w][tid] = total;
+      total += c;
+    }
   }
   unsigned long long* my_lb = lookback + size_t(tile) * 256 + tid;
-  lb_store(my_lb, (tile == 0 ? LB_FLAG_P : LB_FLAG_A) | (unsigned long long)total);
-  unsigned tstart = block_excl_scan<RS_NT>(total, s.scan_tmp, (unsigned*)nullptr);
-  s.tstart[tid] = tstart;
-
-  // decoupled look-back over earlier tiles for this digit
-  unsigned long long excl = 0;
-  if (tile > 0) {
-    long long p = (long long)tile - 1;
-    while (true) {
-      unsigned long long v = lb_load(lookback + size_t(p) * 256 + tid);
-      unsigned long long flag = v & ~LB_VALUE;
-      if (flag == 0) {
-        __builtin_amdgcn_s_sleep(1);
-        continue;
+  if (owns_digit) lb_store(my_lb, (tile == 0 ? LB_FLAG_P : LB_FLAG_A) | (unsigned long long)total);
+  unsigned tstart = block_excl_scan<NT>(total, s.scan_tmp, (unsigned*)nullptr);
+  if (owns_digit) {
+    s.tstart[tid] = tstart;
+    // decoupled look-back over earlier tiles for this digit
+    unsigned long long excl = 0;
+    if (tile > 0) {
+      long long p = (long long)tile - 1;
+      while (true) {
+        unsigned long long v = lb_load(lookback + size_t(p) * 256 + tid);
+        unsigned long long flag = v & ~LB_VALUE;
+        if (flag == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        excl += v & LB_VALUE;
+        if (flag == LB_FLAG_P) break;
+        --p;
       }
-      excl += v & LB_VALUE;
-      if (flag == LB_FLAG_P) break;
-      --p;
+      lb_store(my_lb, LB_FLAG_P | (excl + total));
     }
-    lb_store(my_lb, LB_FLAG_P | (excl + total));
+    s.gdst[tid] = gbase[tid] + excl - tstart;
   }
-  s.gdst[tid] = gbase[tid] + excl - tstart;
   __syncthreads();
 
   // reorder keys by digit in LDS
   U* kbuf = reinterpret_cast<U*>(s.buf);
-  unsigned pos[RS_ITEMS];
+  unsigned pos[ITEMS];
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; j++) {
+  for (int j = 0; j < ITEMS; j++) {
     size_t idx = wave_base + size_t(j) * 64 + lane;
     pos[j] = s.tstart[digit[j]] + s.wcnt[wave][digit[j]] + rank[j];
     if (idx < n) kbuf[pos[j]] = keys[j];
   }
   __syncthreads();
-  const unsigned tile_n = unsigned((n - tile_base) < size_t(RS_TILE) ? (n - tile_base) : size_t(RS_TILE));
-  unsigned long long dst[RS_ITEMS];
+  const unsigned tile_n = unsigned((n - tile_base) < size_t(TILE) ? (n - tile_base) : size_t(TILE));
+  unsigned long long dst[ITEMS];
 #pragma unroll
-  for (int j = 0; j < RS_ITEMS; j++) {
-    unsigned i = unsigned(j) * RS_NT + tid;
+  for (int j = 0; j < ITEMS; j++) {
+    unsigned i = unsigned(j) * NT + tid;
     dst[j] = 0;
     if (i < tile_n) {
       U k = kbuf[i];
@@ -273,14 +282,14 @@ __global__ __launch_bounds__(RS_NT) void rs_onesweep(
     V* vdst = static_cast<V*>(vdst_);
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
       size_t idx = wave_base + size_t(j) * 64 + lane;
       if (idx < n) vbuf[pos[j]] = vals[j];
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < RS_ITEMS; j++) {
-      unsigned i = unsigned(j) * RS_NT + tid;
+    for (int j = 0; j < ITEMS; j++) {
+      unsigned i = unsigned(j) * NT + tid;
       if (i < tile_n) vdst[dst[j]] = vbuf[i];
     }
   }
@@ -323,13 +332,27 @@ void launch_stats(const void* keys, size_t n, unsigned long long* out, hipStream
   FHIP_CHECK(hipGetLastError());
 }
 
-template <class U, int VB>
-void launch_pass(const void* ks, void* kd, const void* vs, void* vd, size_t n, int shift,
-                 int mode, int desc, const unsigned long long* gbase,
-                 unsigned long long* lb, unsigned long long* lb_next, unsigned* ctr,
-                 unsigned* ctr_next, unsigned tiles, hipStream_t st) {
-  rs_onesweep<U, VB><<<tiles, RS_NT, 0, st>>>((const U*)ks, (U*)kd, vs, vd, n, shift, mode,
-                                              desc, gbase, lb, lb_next, ctr, ctr_next);
+// Tile shapes of the onesweep pass (threads x keys per thread); the tunable
+// "radix.tile" picks one by its keys per tile.
+struct TileShape {
+  int nt, items;
+};
+constexpr TileShape kTileShapes[] = {{256, 16}, {256, 32}, {512, 16}, {512, 24},
+                                     {512, 12},  {1024, 8}, {1024, 16}, {768, 16}};
+// measured on MI355X: 512 x 16 is the fastest key-only shape at 2^24 and
+// 2^28 keys; with a payload, 1024 x 16 at 2^28 (512 x 16 within 2% at 2^24)
+constexpr int kDefaultShape = 2;
+constexpr int kDefaultShapeKV = 6;
+constexpr int kNumShapes = int(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
+
+template <class U, int VB, int NT, int ITEMS>
+void launch_pass_t(const void* ks, void* kd, const void* vs, void* vd, size_t n, int shift,
+                   int mode, int desc, const unsigned long long* gbase, unsigned long long* lb,
+                   unsigned long long* lb_next, unsigned* ctr, unsigned* ctr_next,
+                   unsigned tiles, hipStream_t st) {
+  rs_onesweep<U, VB, NT, ITEMS><<<tiles, NT, 0, st>>>((const U*)ks, (U*)kd, vs, vd, n, shift,
+                                                      mode, desc, gbase, lb, lb_next, ctr,
+                                                      ctr_next);
   FHIP_CHECK(hipGetLastError());
 }
 
@@ -337,15 +360,37 @@ using PassFn = void (*)(const void*, void*, const void*, void*, size_t, int, int
                         const unsigned long long*, unsigned long long*, unsigned long long*,
                         unsigned*, unsigned*, unsigned, hipStream_t);
 
-PassFn pick_pass(int kbytes, int vbytes) {
+template <int NT, int ITEMS>
+PassFn pick_pass_t(int kbytes, int vbytes) {
   if (kbytes == 4) {
-    if (vbytes == 0) return launch_pass<uint32_t, 0>;
-    if (vbytes == 4) return launch_pass<uint32_t, 4>;
-    return launch_pass<uint32_t, 8>;
+    if (vbytes == 0) return launch_pass_t<uint32_t, 0, NT, ITEMS>;
+    if (vbytes == 4) return launch_pass_t<uint32_t, 4, NT, ITEMS>;
+    return launch_pass_t<uint32_t, 8, NT, ITEMS>;
   }
-  if (vbytes == 0) return launch_pass<uint64_t, 0>;
-  if (vbytes == 4) return launch_pass<uint64_t, 4>;
-  return launch_pass<uint64_t, 8>;
+  if (vbytes == 0) return launch_pass_t<uint64_t, 0, NT, ITEMS>;
+  if (vbytes == 4) return launch_pass_t<uint64_t, 4, NT, ITEMS>;
+  return launch_pass_t<uint64_t, 8, NT, ITEMS>;
+}
+
+PassFn pick_pass(int shape, int kbytes, int vbytes) {
+  switch (shape) {
+    case 1: return pick_pass_t<256, 32>(kbytes, vbytes);
+    case 2: return pick_pass_t<512, 16>(kbytes, vbytes);
+    case 3: return pick_pass_t<512, 24>(kbytes, vbytes);
+    case 4: return pick_pass_t<512, 12>(kbytes, vbytes);
+    case 5: return pick_pass_t<1024, 8>(kbytes, vbytes);
+    case 6: return pick_pass_t<1024, 16>(kbytes, vbytes);
+    case 7: return pick_pass_t<768, 16>(kbytes, vbytes);
+    default: return pick_pass_t<256, 16>(kbytes, vbytes);
+  }
+}
+
+// the tunables "radix.shape_k" (keys only) / "radix.shape_kv" (with a
+// payload): an index into kTileShapes; else the measured defaults
+int chosen_shape(int vbytes) {
+  const int dflt = vbytes ? kDefaultShapeKV : kDefaultShape;
+  const long long idx = tune_get(vbytes ? "radix.shape_kv" : "radix.shape_k", dflt);
+  return (idx >= 0 && idx < kNumShapes) ? int(idx) : dflt;
 }
 
 size_t bytes_of_max(unsigned long long mx, int kbytes) {
@@ -421,8 +466,10 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   auto kbuf = [&](int d) -> void* { return d == 0 ? kout : d == 1 ? t1k.get() : t2k.get(); };
   auto vbuf = [&](int d) -> void* { return d == 0 ? vout : d == 1 ? t1v.get() : t2v.get(); };
 
-  const unsigned tiles = unsigned(ceil_div(n, RS_TILE));
-  FHIP_REQUIRE(ceil_div(n, RS_TILE) < (1ull << 31), FHIP_ERR_TOO_LARGE, "radix_sort: too many keys");
+  const int shape = chosen_shape(vbytes);
+  const size_t tile_keys = size_t(kTileShapes[shape].nt) * kTileShapes[shape].items;
+  const unsigned tiles = unsigned(ceil_div(n, tile_keys));
+  FHIP_REQUIRE(ceil_div(n, tile_keys) < (1ull << 31), FHIP_ERR_TOO_LARGE, "radix_sort: too many keys");
   const size_t lb_words = size_t(tiles) * 256;
   DBuf lbuf(2 * lb_words * sizeof(unsigned long long) + 256);
   unsigned long long* lb0 = lbuf.as<unsigned long long>();
@@ -431,7 +478,7 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   FHIP_CHECK(hipMemsetAsync(lb0, 0, lb_words * sizeof(unsigned long long), st));
   FHIP_CHECK(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), st));
 
-  PassFn fn = pick_pass(kbytes, vbytes);
+  PassFn fn = pick_pass(shape, kbytes, vbytes);
   const void* ks = kin;
   const void* vs = vin;
   for (int i = 0; i < P; i++) {
@@ -464,7 +511,7 @@ struct AsyncLayout {
 
 AsyncLayout async_layout(size_t n, int kbytes, int vbytes) {
   AsyncLayout L{};
-  L.tiles = ceil_div(n, RS_TILE);
+  L.tiles = ceil_div(n, size_t(kTileShapes[kDefaultShape].nt) * kTileShapes[kDefaultShape].items);
   L.lb_words = L.tiles * 256;
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -519,7 +566,7 @@ void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int v
   FHIP_CHECK(hipGetLastError());
   rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
   FHIP_CHECK(hipGetLastError());
-  PassFn fn = pick_pass(kbytes, vbytes);
+  PassFn fn = pick_pass(kDefaultShape, kbytes, vbytes);  // the workspace's tile count
   const void* ks = kin;
   const void* vs = vin;
   for (int i = 0; i < nbytes; i++) {

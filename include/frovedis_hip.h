@@ -67,12 +67,17 @@ int fhip_prof_enable(int on);
 int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
 
 /* Tunable parameters, set by the autotuner (frovedis_amd/autotune.py) from
- * measurements on the device and kept in a JSON cache: "groupby.slots"
- * (LDS hash table slots, power of two), "groupby.fill_pct" (expected groups
- * per partition as a percentage of the slots: the hash load factor) and
- * "groupby.chunk" (rows per aggregation workgroup).  A negative value
- * removes the setting (back to the built-in default).  No counterpart in
- * the reference. */
+ * measurements on the device and kept in a JSON cache:
+ *   "radix.shape_k", "radix.shape_kv"  tile shape of the radix sort pass
+ *       (index: 0 256x16, 1 256x32, 2 512x16, 3 512x24, 4 512x12, 5 1024x8,
+ *       6 1024x16, 7 768x16 threads x keys) for key-only / key-value sorts;
+ *   "groupby.slots"     LDS hash table slots of the fused group-by (power of 2);
+ *   "groupby.fill_pct"  expected groups per partition as a percentage of the
+ *       slots (the hash load factor);
+ *   "groupby.chunk"     rows aggregated by one workgroup;
+ *   "groupby.span"      4096-row tiles partitioned by one workgroup.
+ * A negative value removes the setting (back to the built-in default).  No
+ * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);
 int fhip_tune_get(const char* name, long long dflt, long long* value);
 /* The tunables as one flat JSON object {"name": integer, ...}: save writes
