@@ -91,6 +91,8 @@ def tune_groupby(keys, aggs, space=None, reps=3, table_hint=None):
         tune_set(nm, None)
 
     def run():
+        if table_hint is None:
+            return group_by_agg(keys, aggs)
         return group_by_agg(keys, aggs, table_hint=table_hint)
 
     base_s, (ref_keys, _) = _timed(run, reps)

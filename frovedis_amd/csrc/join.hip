@@ -36,47 +36,93 @@ struct HitWriter {
   }
 };
 
-// number of right matches of every sorted left key, and where they start
+// number of right matches of every sorted left key, and where they start.
+// One workgroup per block of MC_NT sorted left keys: the block's right range
+// comes from two binary searches over all of RIGHT (its first and last
+// key); each key then searches only that range, which stays in cache.
+constexpr int MC_NT = 256;
+
 template <class K>
-__global__ void match_count_kernel(const K* __restrict__ lk, size_t nl, const K* __restrict__ rk,
-                                   size_t nr, unsigned long long* __restrict__ cnt,
-                                   unsigned long long* __restrict__ first) {
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nl; i += stride) {
-    const K x = lk[i];
-    size_t lo = 0, hi = nr;
-    while (lo < hi) {  // lower_bound
-      size_t mid = (lo + hi) >> 1;
-      if (rk[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    size_t a = lo;
-    hi = nr;
-    while (lo < hi) {  // upper_bound
-      size_t mid = (lo + hi) >> 1;
-      if (x < rk[mid]) hi = mid; else lo = mid + 1;
-    }
-    cnt[i] = lo - a;
-    first[i] = a;
+__device__ __forceinline__ size_t lower_in(const K* r, size_t lo, size_t hi, K x) {
+  while (lo < hi) {
+    const size_t mid = (lo + hi) >> 1;
+    if (r[mid] < x) lo = mid + 1; else hi = mid;
   }
+  return lo;
 }
 
-__global__ void pair_write_kernel(const unsigned long long* __restrict__ incl,
-                                  const unsigned long long* __restrict__ cnt,
-                                  const unsigned long long* __restrict__ first,
-                                  const size_t* __restrict__ lidx_sorted,
-                                  const size_t* __restrict__ ridx_sorted, size_t nl,
-                                  size_t* __restrict__ lo, size_t* __restrict__ ro) {
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nl; i += stride) {
-    const unsigned long long c = cnt[i];
-    if (!c) continue;
-    const size_t off = size_t(incl[i] - c);
-    const size_t l = lidx_sorted[i];
-    const size_t f = size_t(first[i]);
-    for (unsigned long long j = 0; j < c; j++) {
-      lo[off + j] = l;
-      ro[off + j] = ridx_sorted[f + j];
+template <class K>
+__device__ __forceinline__ size_t upper_in(const K* r, size_t lo, size_t hi, K x) {
+  while (lo < hi) {
+    const size_t mid = (lo + hi) >> 1;
+    if (x < r[mid]) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+template <class K>
+__global__ __launch_bounds__(MC_NT) void match_count_kernel(const K* __restrict__ lk, size_t nl,
+                                                            const K* __restrict__ rk, size_t nr,
+                                                            unsigned long long* __restrict__ cnt,
+                                                            unsigned long long* __restrict__ first) {
+  __shared__ size_t range[2];
+  const size_t b0 = size_t(blockIdx.x) * MC_NT;
+  const size_t b1 = b0 + MC_NT < nl ? b0 + MC_NT : nl;
+  if constexpr (std::is_integral<K>::value) {
+    if (threadIdx.x == 0) range[0] = lower_in(rk, 0, nr, lk[b0]);
+    if (threadIdx.x == 64) range[1] = upper_in(rk, 0, nr, lk[b1 - 1]);
+  } else {
+    // floating keys (NaN, -0.0) are not totally ordered by <: search all
+    if (threadIdx.x == 0) {
+      range[0] = 0;
+      range[1] = nr;
     }
+  }
+  __syncthreads();
+  const size_t i = b0 + threadIdx.x;
+  if (i >= b1) return;
+  const K x = lk[i];
+  const size_t a = lower_in(rk, range[0], range[1], x);
+  const size_t e = upper_in(rk, a, range[1], x);
+  cnt[i] = e - a;
+  first[i] = a;
+}
+
+// Writes the pairs of one block of PW_NT sorted left rows: the block's
+// output is the contiguous range [incl[b0] - cnt[b0], incl[b1 - 1]); lanes
+// take consecutive output positions (coalesced stores) and find their left
+// row by a binary search over the block's inclusive counts in LDS.
+constexpr int PW_NT = 256;
+
+__global__ __launch_bounds__(PW_NT) void pair_write_kernel(
+    const unsigned long long* __restrict__ incl, const unsigned long long* __restrict__ cnt,
+    const unsigned long long* __restrict__ first, const size_t* __restrict__ lidx_sorted,
+    const size_t* __restrict__ ridx_sorted, size_t nl, size_t* __restrict__ lo,
+    size_t* __restrict__ ro) {
+  __shared__ unsigned long long s_incl[PW_NT];
+  __shared__ unsigned long long s_first[PW_NT];
+  __shared__ size_t s_l[PW_NT];
+  const size_t b0 = size_t(blockIdx.x) * PW_NT;
+  const unsigned m = unsigned(b0 + PW_NT < nl ? PW_NT : nl - b0);
+  const unsigned t = threadIdx.x;
+  if (t < m) {
+    s_incl[t] = incl[b0 + t];
+    s_first[t] = first[b0 + t];
+    s_l[t] = lidx_sorted[b0 + t];
+  }
+  __syncthreads();
+  const unsigned long long out0 = s_incl[0] - cnt[b0];
+  const unsigned long long out1 = s_incl[m - 1];
+  for (unsigned long long p = out0 + t; p < out1; p += PW_NT) {
+    // first row j of the block with s_incl[j] > p
+    unsigned a = 0, z = m;
+    while (a < z) {
+      const unsigned mid = (a + z) >> 1;
+      if (s_incl[mid] > p) z = mid; else a = mid + 1;
+    }
+    const unsigned long long row_start = a ? s_incl[a - 1] : out0;
+    lo[p] = s_l[a];
+    ro[p] = ridx_sorted[s_first[a] + (p - row_start)];
   }
 }
 
@@ -131,9 +177,9 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
       incl(nl * sizeof(unsigned long long));
   {
     ProfScope prof("join_merge", st);
-    match_count_kernel<K><<<grid_for_n(nl), 256, 0, st>>>(lk.as<K>(), nl, rk.as<K>(), nr,
-                                                          cnt.as<unsigned long long>(),
-                                                          first.as<unsigned long long>());
+    match_count_kernel<K><<<unsigned(ceil_div(nl, MC_NT)), MC_NT, 0, st>>>(
+        lk.as<K>(), nl, rk.as<K>(), nr, cnt.as<unsigned long long>(),
+        first.as<unsigned long long>());
     FHIP_CHECK(hipGetLastError());
   }
   prefix_sum(FHIP_U64, cnt.get(), incl.get(), nl, st);
@@ -143,7 +189,7 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
   FHIP_CHECK(hipStreamSynchronize(st));
   if (total && size_t(total) <= capacity) {
     ProfScope prof("join_write", st);
-    pair_write_kernel<<<grid_for_n(nl), 256, 0, st>>>(
+    pair_write_kernel<<<unsigned(ceil_div(nl, PW_NT)), PW_NT, 0, st>>>(
         incl.as<unsigned long long>(), cnt.as<unsigned long long>(),
         first.as<unsigned long long>(), li.as<size_t>(), ri.as<size_t>(), nl, lo, ro);
     FHIP_CHECK(hipGetLastError());
