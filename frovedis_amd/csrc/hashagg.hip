@@ -72,17 +72,6 @@ __device__ __forceinline__ unsigned long long key_bits(K k) {
   else return (unsigned long long)k;
 }
 
-// order-preserving unsigned image of a key (signed keys: sign bit flipped)
-template <class K>
-__device__ __forceinline__ unsigned long long key_ord(K k) {
-  return key_bits(k) ^ (std::is_signed<K>::value ? 0x8000000000000000ull : 0ull);
-}
-
-template <class K>
-__device__ __forceinline__ K key_from_ord(unsigned long long o) {
-  return K(o ^ (std::is_signed<K>::value ? 0x8000000000000000ull : 0ull));
-}
-
 // accumulator identity as 64-bit pattern
 __host__ __device__ inline unsigned long long acc_identity(int kind, int vtype) {
   if (kind == K_SUM) return 0ull;  // 0 for ints, +0.0 for FP
@@ -170,16 +159,11 @@ __device__ __forceinline__ unsigned part_of(unsigned long long h, int pbits) {
 
 // counts per (partition, span), partition-major: cnt[p * spans + s]; a span
 // is span_tiles consecutive tiles, counted by one workgroup
-// The same pass finds the range of the keys (mm[0] = min, mm[1] = max of
-// key_ord), which decides whether the partitioned copy can hold 32-bit
-// offsets from the minimum instead of the keys.
 template <class K>
 __global__ __launch_bounds__(PS_NT) void pa_count_kernel(const K* __restrict__ keys, size_t n,
                                                          int pbits, size_t spans, int span_tiles,
-                                                         unsigned long long* __restrict__ cnt,
-                                                         unsigned long long* __restrict__ mm) {
+                                                         unsigned long long* __restrict__ cnt) {
   extern __shared__ unsigned lh[];
-  unsigned long long kmin = ~0ull, kmax = 0ull;
   const int P = 1 << pbits;
   for (int i = threadIdx.x; i < P; i += PS_NT) lh[i] = 0;
   __syncthreads();
@@ -191,13 +175,7 @@ __global__ __launch_bounds__(PS_NT) void pa_count_kernel(const K* __restrict__ k
     for (int j = 0; j < PS_ITEMS; j++) {
       const size_t i = t0 + size_t(j) * PS_NT + threadIdx.x;
       const bool ok = i < r1;
-      const K key = ok ? keys[i] : K(0);
-      const unsigned pp = ok ? part_of(mix64(key_bits(key)), pbits) : 0u;
-      if (ok) {
-        const unsigned long long o = key_ord(key);
-        kmin = o < kmin ? o : kmin;
-        kmax = o > kmax ? o : kmax;
-      }
+      const unsigned pp = ok ? part_of(mix64(key_bits(keys[i])), pbits) : 0u;
       const unsigned long long act = __ballot(ok);
       if (!act) continue;
       // the lanes in the first active lane's partition count with one atomic
@@ -207,16 +185,6 @@ __global__ __launch_bounds__(PS_NT) void pa_count_kernel(const K* __restrict__ k
       if (lane == unsigned(leader)) atomicAdd(&lh[p0], unsigned(__popcll(gm)));
       else if (ok && pp != p0) atomicAdd(&lh[pp], 1u);
     }
-  }
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned long long a = __shfl_xor(kmin, o, 64), b = __shfl_xor(kmax, o, 64);
-    kmin = a < kmin ? a : kmin;
-    kmax = b > kmax ? b : kmax;
-  }
-  if (lane == 0) {
-    atomicMin(&mm[0], kmin);
-    atomicMax(&mm[1], kmax);
   }
   __syncthreads();
   for (int q = threadIdx.x; q < P; q += PS_NT) cnt[size_t(q) * spans + blockIdx.x] = lh[q];
@@ -236,12 +204,11 @@ struct Cols {
 // incl/cnt: inclusive scan and counts of the partition-major (partition,
 // span) count matrix (exact offsets, no global atomics); the running offset
 // of every partition advances by each tile's count.
-// KO: the partitioned key type, K itself or uint32_t offsets key_ord - omin.
-template <class K, class KO>
+template <class K>
 __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     const K* __restrict__ keys, size_t n, int pbits, size_t spans, int span_tiles, Cols cols,
     const unsigned long long* __restrict__ cnt, const unsigned long long* __restrict__ incl,
-    unsigned long long omin, KO* __restrict__ out_keys) {
+    K* __restrict__ out_keys) {
   extern __shared__ unsigned long long lds_u64[];
   const int P = 1 << pbits;
   unsigned long long* stage = lds_u64;                                     // [PS_TILE]
@@ -328,15 +295,14 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     for (int j = 0; j < PS_ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
       if (r < tn) {
-        if constexpr (std::is_same<K, KO>::value) stage[slot[j]] = (unsigned long long)keys[t0 + r];
-        else stage[slot[j]] = key_ord(keys[t0 + r]) - omin;
+        stage[slot[j]] = (unsigned long long)keys[t0 + r];
         pid[slot[j]] = (unsigned short)part[j];
       }
     }
     __syncthreads();
     for (unsigned i = tid; i < tn; i += PS_NT) {
       const unsigned q = pid[i];
-      out_keys[gbase[q] + (i - ps[q])] = KO(stage[i]);
+      out_keys[gbase[q] + (i - ps[q])] = K(stage[i]);
     }
     for (int a = 0; a < cols.n; a++) {
       __syncthreads();
@@ -650,16 +616,6 @@ __global__ void gather_group_keys(const unsigned long long* __restrict__ w, int 
     keys[g] = K(w[ent[g] * stride]);
 }
 
-// group keys from 32-bit offsets: key_from_ord(omin + offset)
-template <class K>
-__global__ void gather_group_keys_off(const unsigned long long* __restrict__ w, int stride,
-                                      const size_t* __restrict__ ent, size_t G,
-                                      unsigned long long omin, K* __restrict__ keys) {
-  const size_t st = size_t(gridDim.x) * blockDim.x;
-  for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < G; g += st)
-    keys[g] = key_from_ord<K>(omin + (w[ent[g] * stride] & 0xFFFFFFFFull));
-}
-
 // Writes one aggregate in sorted-group order.
 __global__ void emit_kernel(const unsigned long long* __restrict__ w, int stride,
                             const size_t* __restrict__ ent, size_t G, int op, int vtype, int slot,
@@ -711,20 +667,17 @@ int grid_for(size_t n, int per_cu = 8) {
 
 template <class K>
 void launch_pa(const void* keys, size_t n, int pbits, size_t spans, int span_tiles, const Cols& cols,
-               unsigned long long* cnt, unsigned long long* incl, unsigned long long* mm,
-               bool compress, unsigned long long omin, void* pkeys, hipStream_t st, int pass) {
+               unsigned long long* cnt, unsigned long long* incl, void* pkeys, hipStream_t st,
+               int pass) {
   const int P = 1 << pbits;
   if (pass == 1) {
     pa_count_kernel<K><<<unsigned(spans), PS_NT, P * 4, st>>>((const K*)keys, n, pbits, spans,
-                                                               span_tiles, cnt, mm);
+                                                               span_tiles, cnt);
   } else {
     const size_t lds = size_t(PS_TILE) * 8 + size_t(P) * 16 + size_t(PS_TILE) * 2;
-    if (compress)
-      pa_scatter_kernel<K, uint32_t><<<unsigned(spans), PS_NT, lds, st>>>(
-          (const K*)keys, n, pbits, spans, span_tiles, cols, cnt, incl, omin, (uint32_t*)pkeys);
-    else
-      pa_scatter_kernel<K, K><<<unsigned(spans), PS_NT, lds, st>>>(
-          (const K*)keys, n, pbits, spans, span_tiles, cols, cnt, incl, 0ull, (K*)pkeys);
+    pa_scatter_kernel<K><<<unsigned(spans), PS_NT, lds, st>>>((const K*)keys, n, pbits, spans,
+                                                             span_tiles, cols,
+                                                             cnt, incl, (K*)pkeys);
   }
   FHIP_CHECK(hipGetLastError());
 }
@@ -812,14 +765,10 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     FHIP_CHECK(hipFuncSetAttribute((const void*)pa_merge_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     const int ms = int(size_t(PS_TILE) * 10 + (size_t(1) << PA_MAX_PBITS) * 16);
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int32_t, int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int32_t, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint32_t, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint32_t, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int64_t, int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int64_t, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint64_t, uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint64_t, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
     attr_set = true;
   }
   // partitions: about PA_GROUPS_PER_PART expected groups each
@@ -830,15 +779,11 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   for (;;) {
     const int P = 1 << pbits;
     const size_t tiles = ceil_div(ceil_div(n, PS_TILE), size_t(span_tiles));  // spans
-    DBuf cb(size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int)), mmb(16);
+    DBuf cb(size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int));
     unsigned long long* cnt = cb.as<unsigned long long>();
     unsigned long long* incl = ib.as<unsigned long long>();
-    unsigned long long* mm = mmb.as<unsigned long long>();
     FHIP_CHECK(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
-    FHIP_CHECK(hipMemsetAsync(mm, 0xFF, 8, st));      // min: all ones
-    FHIP_CHECK(hipMemsetAsync(mm + 1, 0, 8, st));     // max: zero
-    bool compress = false;
-    unsigned long long omin = 0;
+    DBuf pkeys(n * size_t(kb));
     DBuf pcols[HA_MAX_ACC];
     Cols cols{};
     cols.n = accs.n;
@@ -848,13 +793,12 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       pcols[a].reset(n * size_t(cols.bytes[a]));
       cols.out[a] = pcols[a].get();
     }
-    DBuf pkeys;
     auto run = [&](int pass) {
       switch (ktype) {
-        case FHIP_I32: launch_pa<int32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, mm, compress, omin, pkeys.get(), st, pass); break;
-        case FHIP_U32: launch_pa<uint32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, mm, compress, omin, pkeys.get(), st, pass); break;
-        case FHIP_I64: launch_pa<int64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, mm, compress, omin, pkeys.get(), st, pass); break;
-        default: launch_pa<uint64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, mm, compress, omin, pkeys.get(), st, pass); break;
+        case FHIP_I32: launch_pa<int32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_U32: launch_pa<uint32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_I64: launch_pa<int64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        default: launch_pa<uint64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
       }
     };
     {
@@ -865,12 +809,6 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     // partition sizes: inclusive offset at the last tile of every partition
     std::vector<unsigned long long> ends(P);
     copy_strided_to_host(incl + (tiles - 1), tiles, size_t(P), ends.data(), st);
-    // a key range below 2^32 is partitioned as 32-bit offsets from the minimum
-    unsigned long long hmm[2];
-    FHIP_CHECK(hipMemcpy(hmm, mm, 16, hipMemcpyDeviceToHost));
-    compress = kb == 8 && hmm[1] >= hmm[0] && hmm[1] - hmm[0] < (1ull << 32);
-    omin = hmm[0];
-    pkeys.reset(n * size_t(compress ? 4 : kb));
     // chunks of at most PA_CHUNK rows per partition
     std::vector<Item> items;
     std::vector<unsigned> part_items(P + 1);
@@ -899,7 +837,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     for (int a = 0; a < accs.n; a++) paccs.a[a].col = pcols[a].get();
     {
       ProfScope prof("hashagg", st);
-      switch (compress ? FHIP_U32 : ktype) {
+      switch (ktype) {
         case FHIP_I32: pa_agg_kernel<int32_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<int32_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
         case FHIP_U32: pa_agg_kernel<uint32_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<uint32_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
         case FHIP_I64: pa_agg_kernel<int64_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<int64_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
@@ -947,12 +885,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     FHIP_CHECK(hipGetLastError());
     const unsigned long long* w = fin.as<unsigned long long>();
     DBuf gk(G * size_t(kb));
-    if (compress) {
-      if (ktype == FHIP_I64)
-        gather_group_keys_off<int64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, omin, gk.as<int64_t>());
-      else
-        gather_group_keys_off<uint64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, omin, gk.as<uint64_t>());
-    } else switch (ktype) {
+    switch (ktype) {
       case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int32_t>()); break;
       case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint32_t>()); break;
       case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int64_t>()); break;

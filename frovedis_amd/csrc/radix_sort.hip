@@ -233,20 +233,36 @@ __global__ __launch_bounds__(NT) void rs_onesweep(
   unsigned tstart = block_excl_scan<NT>(total, s.scan_tmp, (unsigned*)nullptr);
   if (owns_digit) {
     s.tstart[tid] = tstart;
-    // decoupled look-back over earlier tiles for this digit
+    // decoupled look-back over earlier tiles for this digit; LB_WINDOW
+    // predecessors are read at once (independent loads), then consumed in
+    // order up to the first inclusive prefix or the first tile that has not
+    // published yet.  Tile 0 always holds an inclusive prefix, so the walk
+    // never goes below it.
     unsigned long long excl = 0;
     if (tile > 0) {
+      constexpr int LB_WINDOW = 8;
       long long p = (long long)tile - 1;
       while (true) {
-        unsigned long long v = lb_load(lookback + size_t(p) * 256 + tid);
-        unsigned long long flag = v & ~LB_VALUE;
-        if (flag == 0) {
-          __builtin_amdgcn_s_sleep(1);
-          continue;
+        unsigned long long v[LB_WINDOW];
+#pragma unroll
+        for (int w = 0; w < LB_WINDOW; w++)
+          v[w] = (p - w >= 0) ? lb_load(lookback + size_t(p - w) * 256 + tid) : 0ull;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int w = 0; w < LB_WINDOW; w++) {
+          const unsigned long long flag = v[w] & ~LB_VALUE;
+          if (flag == 0) break;
+          excl += v[w] & LB_VALUE;
+          used = w + 1;
+          if (flag == LB_FLAG_P) {
+            done = true;
+            break;
+          }
         }
-        excl += v & LB_VALUE;
-        if (flag == LB_FLAG_P) break;
-        --p;
+        if (done) break;
+        if (used == 0) __builtin_amdgcn_s_sleep(1);
+        p -= used;
       }
       lb_store(my_lb, LB_FLAG_P | (excl + total));
     }

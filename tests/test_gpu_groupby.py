@@ -210,3 +210,28 @@ def test_group_by_agg_min_max_small_and_ragged(fa):
             assert np.array_equal(mn.numpy(), [v[k == x].min() for x in keys]), (n, dt)
             assert np.array_equal(mx.numpy(), [v[k == x].max() for x in keys]), (n, dt)
             assert np.array_equal(sm.numpy(), [v[k == x].sum() for x in keys]), (n, dt)
+
+
+@pytest.mark.parametrize("kt,base", [(np.int64, -(1 << 40)), (np.int64, -(1 << 31)),
+                                     (np.uint64, (1 << 64) - (1 << 33)), (np.int64, (1 << 62))])
+def test_group_by_agg_narrow_key_range(fa, kt, base):
+    # keys within a 2^32-wide range far from zero (the partitioned copy holds
+    # 32-bit offsets from the minimum), and one 2^33-wide range (full keys)
+    rng = np.random.default_rng(29)
+    n = 300000
+    for width in [1 << 20, (1 << 32) - 1, 1 << 33]:
+        off = rng.integers(0, width, n, dtype=np.uint64)
+        if kt == np.uint64:
+            k = (np.uint64(base) + off).astype(np.uint64)
+        else:
+            k = (np.int64(base) + off.astype(np.int64)).astype(np.int64)
+        k[::7] = k[0]
+        v = rng.integers(-1000, 1000, n).astype(np.int64)
+        gi, split, keys = oracle.group_by(k)
+        gk, (s, c, mx) = fa.group_by_agg(fa.DeviceArray.from_numpy(k),
+                                         [("sum", fa.DeviceArray.from_numpy(v)), ("count", None),
+                                          ("max", fa.DeviceArray.from_numpy(v))])
+        assert np.array_equal(gk.numpy(), keys), width
+        assert np.array_equal(s.numpy(), oracle.grouped_agg("sum", v, gi, split))
+        assert np.array_equal(c.numpy(), oracle.grouped_agg("count", v, gi, split))
+        assert np.array_equal(mx.numpy(), oracle.grouped_agg("max", v, gi, split))
