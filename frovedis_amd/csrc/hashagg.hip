@@ -33,8 +33,8 @@ namespace {
 constexpr int HA_MAX_ACC = 4;
 constexpr int PA_NT = 1024;           // threads of the aggregate / merge kernels (one per CU: LDS)
 constexpr int PA_UNROLL = 4;          // rows in flight per thread (aggregate)
-constexpr int PS_NT = 256;            // threads of the count / scatter kernels
-constexpr int PS_ITEMS = 16;          // rows per thread
+constexpr int PS_NT = 512;            // threads of the count / scatter kernels
+constexpr int PS_ITEMS = 8;           // rows per thread
 constexpr int PS_TILE = PS_NT * PS_ITEMS;  // rows per scatter tile
 constexpr int PS_SPAN = 4;             // default tiles per count / scatter workgroup (a "span")
 constexpr unsigned long long PA_CHUNK = 1ull << 17;  // rows per aggregate workgroup
@@ -232,10 +232,30 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     const unsigned tn = unsigned(r1 - t0 < size_t(PS_TILE) ? r1 - t0 : size_t(PS_TILE));
     unsigned part[PS_ITEMS];
     unsigned short slot[PS_ITEMS];
+    // the key and the first two columns of the tile are loaded together
+    K kreg[PS_ITEMS];
+    unsigned long long creg[2][PS_ITEMS];
 #pragma unroll
     for (int j = 0; j < PS_ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
-      part[j] = r < tn ? part_of(mix64(key_bits(keys[t0 + r])), pbits) : 0u;
+      kreg[j] = r < tn ? keys[t0 + r] : K(0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+      if (a >= cols.n) break;
+#pragma unroll
+      for (int j = 0; j < PS_ITEMS; j++) {
+        const unsigned r = unsigned(j) * PS_NT + tid;
+        creg[a][j] = r >= tn ? 0ull
+                     : cols.bytes[a] == 4
+                         ? (unsigned long long)static_cast<const unsigned*>(cols.in[a])[t0 + r]
+                         : static_cast<const unsigned long long*>(cols.in[a])[t0 + r];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < PS_ITEMS; j++) {
+      const unsigned r = unsigned(j) * PS_NT + tid;
+      part[j] = r < tn ? part_of(mix64(key_bits(kreg[j])), pbits) : 0u;
     }
     // counts of this tile per partition
 #pragma unroll
@@ -290,19 +310,27 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
       if (in0) slot[j] = (unsigned short)(ps[p0] + base + __popcll(gm & lt));
       else if (ok) slot[j] = (unsigned short)(ps[part[j]] + atomicAdd(&pc[part[j]], 1u));
     }
-    // keys: stage in partition order, then write runs
+    // keys: stage in partition order, then write runs; every thread keeps
+    // the destinations of its staged positions for all the columns
 #pragma unroll
     for (int j = 0; j < PS_ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
       if (r < tn) {
-        stage[slot[j]] = (unsigned long long)keys[t0 + r];
+        stage[slot[j]] = (unsigned long long)kreg[j];
         pid[slot[j]] = (unsigned short)part[j];
       }
     }
     __syncthreads();
-    for (unsigned i = tid; i < tn; i += PS_NT) {
-      const unsigned q = pid[i];
-      out_keys[gbase[q] + (i - ps[q])] = K(stage[i]);
+    unsigned long long dst[PS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < PS_ITEMS; k++) {
+      const unsigned i = unsigned(k) * PS_NT + tid;
+      dst[k] = 0;
+      if (i < tn) {
+        const unsigned q = pid[i];
+        dst[k] = gbase[q] + (i - ps[q]);
+        out_keys[dst[k]] = K(stage[i]);
+      }
     }
     for (int a = 0; a < cols.n; a++) {
       __syncthreads();
@@ -310,22 +338,26 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
       for (int j = 0; j < PS_ITEMS; j++) {
         const unsigned r = unsigned(j) * PS_NT + tid;
         if (r < tn)
-          stage[slot[j]] = cols.bytes[a] == 4
+          stage[slot[j]] = a == 0 ? creg[0][j]
+                           : a == 1 ? creg[1][j]
+                           : cols.bytes[a] == 4
                                ? (unsigned long long)static_cast<const unsigned*>(cols.in[a])[t0 + r]
                                : static_cast<const unsigned long long*>(cols.in[a])[t0 + r];
       }
       __syncthreads();
       if (cols.bytes[a] == 4) {
         unsigned* o = static_cast<unsigned*>(cols.out[a]);
-        for (unsigned i = tid; i < tn; i += PS_NT) {
-          const unsigned q = pid[i];
-          o[gbase[q] + (i - ps[q])] = unsigned(stage[i]);
+#pragma unroll
+        for (int k = 0; k < PS_ITEMS; k++) {
+          const unsigned i = unsigned(k) * PS_NT + tid;
+          if (i < tn) o[dst[k]] = unsigned(stage[i]);
         }
       } else {
         unsigned long long* o = static_cast<unsigned long long*>(cols.out[a]);
-        for (unsigned i = tid; i < tn; i += PS_NT) {
-          const unsigned q = pid[i];
-          o[gbase[q] + (i - ps[q])] = stage[i];
+#pragma unroll
+        for (int k = 0; k < PS_ITEMS; k++) {
+          const unsigned i = unsigned(k) * PS_NT + tid;
+          if (i < tn) o[dst[k]] = stage[i];
         }
       }
     }
