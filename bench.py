@@ -322,8 +322,8 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     breakdown = {}
-    for name in ("groupby_part_hist", "groupby_part_scatter", "hashagg", "groupby_merge",
-                 "radix_hist", "radix_pass"):
+    for name in ("groupby_hot", "groupby_hot_merge", "groupby_part_hist",
+                 "groupby_part_scatter", "hashagg", "groupby_merge", "radix_hist", "radix_pass"):
         ms, cnt = _prof_ms(fa, name)
         breakdown[name] = {"ms_per_step": ms / args.steps, "launches_per_step": cnt / args.steps}
     fa.check(fa.lib().fhip_prof_enable(0))
@@ -332,8 +332,9 @@ def main():
     print(f"# {groups} groups, {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
 
     # the roofline is taken on the step's dominant kernel (the longest one)
-    kernel_of = {"groupby_part_hist": "pa_count_kernel", "groupby_part_scatter": "pa_scatter_kernel",
-                 "hashagg": "pa_agg_kernel", "groupby_merge": "pa_merge_kernel"}
+    kernel_of = {"groupby_hot": "hot_agg_kernel", "groupby_part_hist": "pa_count_kernel",
+                 "groupby_part_scatter": "pa_scatter_kernel", "hashagg": "pa_agg_kernel",
+                 "groupby_merge": "pa_merge_kernel"}
     dom = max(kernel_of, key=lambda nm: breakdown[nm]["ms_per_step"])
     kern_s = breakdown[dom]["ms_per_step"] / 1e3 / max(breakdown[dom]["launches_per_step"], 1)
     algo_bytes = GROUPBY_BYTES_PER_ROW * n

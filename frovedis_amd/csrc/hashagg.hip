@@ -56,7 +56,7 @@ struct AccSet {
   int n;
 };
 
-__device__ __forceinline__ unsigned long long mix64(unsigned long long h) {
+__host__ __device__ __forceinline__ unsigned long long mix64(unsigned long long h) {
   h ^= h >> 33;
   h *= 0xff51afd7ed558ccdull;
   h ^= h >> 33;
@@ -432,6 +432,55 @@ __device__ __forceinline__ void lds_table_flush(const LdsTable& t, int na, int s
   if (threadIdx.x == 0) *out_n = *lds_counter;
 }
 
+// Combines two accumulator words (the same operation as acc_merge, on
+// registers); NaN is never selected by max/min.
+__device__ __forceinline__ unsigned long long acc_combine(unsigned long long w,
+                                                          unsigned long long x, int kind,
+                                                          int vtype) {
+  switch (vtype) {
+    case FHIP_F64: {
+      double a, b;
+      __builtin_memcpy(&a, &w, 8);
+      __builtin_memcpy(&b, &x, 8);
+      double r = kind == K_SUM ? a + b
+               : kind == K_MAX ? (a < b || a != a ? b : a)
+                               : (a > b || a != a ? b : a);
+      if (kind != K_SUM && b != b) r = a;  // NaN is never selected
+      __builtin_memcpy(&w, &r, 8);
+      return w;
+    }
+    case FHIP_F32: {
+      float a, b;
+      unsigned ua = unsigned(w), ub = unsigned(x);
+      __builtin_memcpy(&a, &ua, 4);
+      __builtin_memcpy(&b, &ub, 4);
+      float r = kind == K_SUM ? a + b
+              : kind == K_MAX ? (a < b || a != a ? b : a)
+                              : (a > b || a != a ? b : a);
+      if (kind != K_SUM && b != b) r = a;
+      unsigned ur;
+      __builtin_memcpy(&ur, &r, 4);
+      return ur;
+    }
+    case FHIP_I32: {
+      int a = int(unsigned(w)), b = int(unsigned(x));
+      int r = kind == K_SUM ? int(unsigned(a) + unsigned(b)) : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
+      return unsigned(r);
+    }
+    case FHIP_U32: {
+      unsigned a = unsigned(w), b = unsigned(x);
+      return kind == K_SUM ? a + b : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
+    }
+    case FHIP_I64: {
+      long long a = (long long)w, b = (long long)x;
+      long long r = kind == K_SUM ? (long long)(w + x) : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
+      return (unsigned long long)r;
+    }
+    default:
+      return kind == K_SUM ? w + x : kind == K_MAX ? (w < x ? x : w) : (w > x ? x : w);
+  }
+}
+
 // Reduces an accumulator word over the lanes in `act` (all lanes end with
 // the result); used when a whole wave updates the same LDS slot.
 __device__ __forceinline__ unsigned long long wave_reduce_acc(unsigned long long w, int kind,
@@ -439,56 +488,7 @@ __device__ __forceinline__ unsigned long long wave_reduce_acc(unsigned long long
                                                               bool ok) {
   if (!ok) w = acc_identity(kind, vtype);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const unsigned long long x = __shfl_xor(w, o, 64);
-    switch (vtype) {
-      case FHIP_F64: {
-        double a, b;
-        __builtin_memcpy(&a, &w, 8);
-        __builtin_memcpy(&b, &x, 8);
-        double r = kind == K_SUM ? a + b
-                 : kind == K_MAX ? (a < b || a != a ? b : a)
-                                 : (a > b || a != a ? b : a);
-        if (kind != K_SUM && b != b) r = a;  // NaN is never selected
-        __builtin_memcpy(&w, &r, 8);
-        break;
-      }
-      case FHIP_F32: {
-        float a, b;
-        unsigned ua = unsigned(w), ub = unsigned(x);
-        __builtin_memcpy(&a, &ua, 4);
-        __builtin_memcpy(&b, &ub, 4);
-        float r = kind == K_SUM ? a + b
-                : kind == K_MAX ? (a < b || a != a ? b : a)
-                                : (a > b || a != a ? b : a);
-        if (kind != K_SUM && b != b) r = a;
-        unsigned ur;
-        __builtin_memcpy(&ur, &r, 4);
-        w = ur;
-        break;
-      }
-      case FHIP_I32: {
-        int a = int(unsigned(w)), b = int(unsigned(x));
-        int r = kind == K_SUM ? int(unsigned(a) + unsigned(b)) : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
-        w = unsigned(r);
-        break;
-      }
-      case FHIP_U32: {
-        unsigned a = unsigned(w), b = unsigned(x);
-        w = kind == K_SUM ? a + b : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
-        break;
-      }
-      case FHIP_I64: {
-        long long a = (long long)w, b = (long long)x;
-        long long r = kind == K_SUM ? (long long)(w + x) : kind == K_MAX ? (a < b ? b : a) : (a > b ? b : a);
-        w = (unsigned long long)r;
-        break;
-      }
-      default:
-        w = kind == K_SUM ? w + x : kind == K_MAX ? (w < x ? x : w) : (w > x ? x : w);
-        break;
-    }
-  }
+  for (int o = 32; o > 0; o >>= 1) w = acc_combine(w, __shfl_xor(w, o, 64), kind, vtype);
   (void)act;
   return w;
 }
@@ -630,6 +630,155 @@ __global__ __launch_bounds__(PA_NT) void pa_merge_kernel(
   lds_table_flush(t, na, stride, out, final_n + p, &counter);
 }
 
+// ---- hot keys: heavy keys of skewed data, aggregated without partitioning --
+// The heavy keys come from a sample (at most half of the Sh slots, so that a
+// lookup ends within a few probes); hot_keys holds them at their
+// open-addressing slots (HA_EMPTY elsewhere).  Every workgroup walks a
+// contiguous chunk of HOT_CHUNK rows: a row with a heavy key is aggregated
+// into the workgroup's LDS table, any other row is appended to the cold
+// buffers (one global reservation per 4096-row step of the workgroup), which
+// then go through the partitioned path.  The per-workgroup tables share the
+// slot layout, so the merge is a column-wise reduction.
+constexpr int HOT_NT = 1024;
+constexpr int HOT_UNROLL = 4;
+constexpr int HOT_STEP = HOT_NT * HOT_UNROLL;
+constexpr size_t HOT_CHUNK = size_t(1) << 18;
+
+__device__ __forceinline__ int hot_slot(const unsigned long long* hk, int Sh,
+                                        unsigned long long kb) {
+  if (kb == HA_EMPTY) return -1;
+  int s = int(mix64(kb) & (unsigned long long)(Sh - 1));
+  for (int p = 0; p < Sh; p++) {
+    const unsigned long long c = hk[s];
+    if (c == kb) return s;
+    if (c == HA_EMPTY) return -1;
+    s = (s + 1) & (Sh - 1);
+  }
+  return -1;
+}
+
+template <class K>
+__global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
+    const K* __restrict__ keys, size_t n, AccSet accs, const unsigned long long* __restrict__ hot_keys,
+    int Sh, unsigned long long* __restrict__ partial, K* __restrict__ cold_keys, Cols cold,
+    unsigned long long* __restrict__ cold_n) {
+  extern __shared__ unsigned long long lds_u64[];
+  __shared__ unsigned scan_tmp[HOT_NT / 64];
+  __shared__ unsigned long long cbase;
+  const int na = accs.n;
+  unsigned long long* hk = lds_u64;        // [Sh]
+  unsigned long long* hc = hk + Sh;        // [Sh]
+  unsigned long long* ha = hc + Sh;        // [na][Sh]
+  for (int i = threadIdx.x; i < Sh; i += HOT_NT) {
+    hk[i] = hot_keys[i];
+    hc[i] = 0;
+    for (int a = 0; a < na; a++) ha[size_t(a) * Sh + i] = acc_identity(accs.a[a].kind, accs.a[a].vtype);
+  }
+  __syncthreads();
+  const unsigned tid = threadIdx.x, lane = __lane_id();
+  const size_t r0 = size_t(blockIdx.x) * HOT_CHUNK;
+  const size_t r1 = r0 + HOT_CHUNK < n ? r0 + HOT_CHUNK : n;
+  for (size_t t0 = r0; t0 < r1; t0 += HOT_STEP) {
+    K kv[HOT_UNROLL];
+    unsigned long long vv[HOT_UNROLL][HA_MAX_ACC];
+    int sl[HOT_UNROLL];
+#pragma unroll
+    for (int u = 0; u < HOT_UNROLL; u++) {
+      const size_t r = t0 + size_t(u) * HOT_NT + tid;
+      kv[u] = r < r1 ? keys[r] : K(0);
+#pragma unroll
+      for (int a = 0; a < HA_MAX_ACC; a++)
+        vv[u][a] = (a < na && r < r1) ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
+    }
+    unsigned ncold = 0;
+#pragma unroll
+    for (int u = 0; u < HOT_UNROLL; u++) {
+      const size_t r = t0 + size_t(u) * HOT_NT + tid;
+      const bool in = r < r1;
+      sl[u] = in ? hot_slot(hk, Sh, key_bits(kv[u])) : -1;
+      if (in && sl[u] < 0) ncold++;
+      const bool ok = sl[u] >= 0;
+      const unsigned long long act = __ballot(ok);
+      if (!act) continue;
+      const int leader = __builtin_ffsll((long long)act) - 1;
+      const int s0 = __shfl(sl[u], leader, 64);
+      if (__ballot(ok && sl[u] == s0) == act) {
+        if (lane == unsigned(leader)) atomicAdd(&hc[s0], (unsigned long long)__popcll(act));
+        for (int a = 0; a < na; a++) {
+          const unsigned long long w = wave_reduce_acc(vv[u][a], accs.a[a].kind, accs.a[a].vtype, act, ok);
+          if (lane == unsigned(leader)) acc_merge(&ha[size_t(a) * Sh + s0], accs.a[a].kind, accs.a[a].vtype, w);
+        }
+      } else if (ok) {
+        atomicAdd(&hc[sl[u]], 1ull);
+        for (int a = 0; a < na; a++)
+          acc_merge(&ha[size_t(a) * Sh + sl[u]], accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
+      }
+    }
+    // cold rows of this step: one reservation for the workgroup
+    unsigned total = 0;
+    const unsigned off = block_excl_scan<HOT_NT>(ncold, scan_tmp, &total);
+    if (tid == 0) cbase = total ? atomicAdd(cold_n, (unsigned long long)total) : 0ull;
+    __syncthreads();
+    unsigned long long pos = cbase + off;
+    __syncthreads();  // cbase is rewritten by the next step
+#pragma unroll
+    for (int u = 0; u < HOT_UNROLL; u++) {
+      const size_t r = t0 + size_t(u) * HOT_NT + tid;
+      if (r >= r1 || sl[u] >= 0) continue;
+      cold_keys[pos] = kv[u];
+      for (int a = 0; a < cold.n; a++) {
+        if (cold.bytes[a] == 4) static_cast<unsigned*>(cold.out[a])[pos] = unsigned(vv[u][a]);
+        else static_cast<unsigned long long*>(cold.out[a])[pos] = vv[u][a];
+      }
+      pos++;
+    }
+  }
+  __syncthreads();
+  unsigned long long* out = partial + size_t(blockIdx.x) * size_t(1 + na) * Sh;
+  for (int i = threadIdx.x; i < Sh; i += HOT_NT) {
+    out[i] = hc[i];
+    for (int a = 0; a < na; a++) out[size_t(1 + a) * Sh + i] = ha[size_t(a) * Sh + i];
+  }
+}
+
+// Column-wise reduction of the per-workgroup hot tables (in workgroup order);
+// every used slot becomes one entry {key, count, acc...} of `out`.
+__global__ __launch_bounds__(256) void hot_merge_kernel(
+    const unsigned long long* __restrict__ partial, int nparts, int Sh, AccSet accs,
+    const unsigned long long* __restrict__ hot_keys, int stride,
+    unsigned long long* __restrict__ out, unsigned* __restrict__ out_n) {
+  __shared__ unsigned long long red[4][64][1 + HA_MAX_ACC];
+  const int na = accs.n;
+  const int slot = blockIdx.x * 64 + int(threadIdx.x & 63);
+  const int part = int(threadIdx.x >> 6);
+  unsigned long long c = 0, acc[HA_MAX_ACC];
+#pragma unroll
+  for (int a = 0; a < HA_MAX_ACC; a++) acc[a] = a < na ? acc_identity(accs.a[a].kind, accs.a[a].vtype) : 0ull;
+  if (slot < Sh) {
+    for (int g = part; g < nparts; g += 4) {
+      const unsigned long long* t = partial + size_t(g) * size_t(1 + na) * Sh;
+      c += t[slot];
+      for (int a = 0; a < na; a++)
+        acc[a] = acc_combine(acc[a], t[size_t(1 + a) * Sh + slot], accs.a[a].kind, accs.a[a].vtype);
+    }
+  }
+  red[part][threadIdx.x & 63][0] = c;
+  for (int a = 0; a < na; a++) red[part][threadIdx.x & 63][1 + a] = acc[a];
+  __syncthreads();
+  if (part != 0 || slot >= Sh) return;
+  for (int q = 1; q < 4; q++) {
+    c += red[q][threadIdx.x][0];
+    for (int a = 0; a < na; a++)
+      acc[a] = acc_combine(acc[a], red[q][threadIdx.x][1 + a], accs.a[a].kind, accs.a[a].vtype);
+  }
+  if (c == 0) return;
+  const unsigned j = atomicAdd(out_n, 1u);
+  unsigned long long* e = out + size_t(j) * stride;
+  e[0] = hot_keys[slot];
+  e[1] = c;
+  for (int a = 0; a < na; a++) e[2 + a] = acc[a];
+}
+
 __global__ void final_index_kernel(const unsigned* __restrict__ final_n,
                                    const unsigned long long* __restrict__ final_off, int P, int S,
                                    size_t* __restrict__ ent) {
@@ -725,6 +874,72 @@ void grouped_agg(int op, int vtype, const void* col, size_t col_n, const size_t*
 
 // More groups than the partitioned tables hold: sort-based group_by plus
 // per-column aggregation (same results).
+template <class K>
+__global__ void sample_bits_kernel(const K* __restrict__ keys, size_t step, size_t m,
+                                   unsigned long long* __restrict__ out) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride)
+    out[i] = key_bits(keys[i * step]);
+}
+
+// The heavy keys of a skewed key column: keys seen at least 4 times in an
+// evenly spaced sample of 2^16 rows (expected frequency >= 2^-14), at most
+// Sh / 2 of them, placed in an open-addressing table of Sh slots.  Not used
+// when they cover less than groupby.hot_min_pct percent of the sample.
+struct HotPlan {
+  bool used = false;
+  int Sh = 0;
+  std::vector<unsigned long long> table;
+};
+
+HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
+  HotPlan h;
+  long long want = tune_get("groupby.hot", S);
+  constexpr size_t m = size_t(1) << 16;
+  if (want < 256 || n < 16 * m) return h;
+  int Sh = int(std::min<long long>(want, S));
+  while (Sh & (Sh - 1)) Sh &= Sh - 1;
+  const size_t step = n / m;
+  DBuf sb(m * 8);
+  switch (ktype) {
+    case FHIP_I32: sample_bits_kernel<int32_t><<<grid_for_n(m), 256, 0, st>>>((const int32_t*)keys, step, m, sb.as<unsigned long long>()); break;
+    case FHIP_U32: sample_bits_kernel<uint32_t><<<grid_for_n(m), 256, 0, st>>>((const uint32_t*)keys, step, m, sb.as<unsigned long long>()); break;
+    case FHIP_I64: sample_bits_kernel<int64_t><<<grid_for_n(m), 256, 0, st>>>((const int64_t*)keys, step, m, sb.as<unsigned long long>()); break;
+    default: sample_bits_kernel<uint64_t><<<grid_for_n(m), 256, 0, st>>>((const uint64_t*)keys, step, m, sb.as<unsigned long long>()); break;
+  }
+  FHIP_CHECK(hipGetLastError());
+  // sorted on the device (the runs are counted on the host)
+  sort_pairs_bits(8, sb.get(), sb.get(), 0, nullptr, nullptr, m, 64, st);
+  std::vector<unsigned long long> smp(m);
+  FHIP_CHECK(hipMemcpyAsync(smp.data(), sb.get(), m * 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  std::vector<std::pair<size_t, unsigned long long>> runs;  // (count, key)
+  for (size_t i = 0; i < m;) {
+    size_t j = i + 1;
+    while (j < m && smp[j] == smp[i]) j++;
+    if (j - i >= 4 && smp[i] != HA_EMPTY) runs.emplace_back(j - i, smp[i]);
+    i = j;
+  }
+  std::sort(runs.begin(), runs.end(), [](const std::pair<size_t, unsigned long long>& a,
+                                         const std::pair<size_t, unsigned long long>& b) {
+    return a.first > b.first || (a.first == b.first && a.second < b.second);
+  });
+  if (runs.size() > size_t(Sh / 2)) runs.resize(size_t(Sh / 2));
+  size_t covered = 0;
+  for (auto& r : runs) covered += r.first;
+  const long long min_pct = tune_get("groupby.hot_min_pct", 30);
+  if (runs.empty() || covered * 100 < size_t(std::max(0LL, min_pct)) * m) return h;
+  h.Sh = Sh;
+  h.table.assign(size_t(Sh), HA_EMPTY);
+  for (auto& r : runs) {
+    size_t slot = size_t(mix64(r.second) & (unsigned long long)(Sh - 1));
+    while (h.table[slot] != HA_EMPTY) slot = (slot + 1) & size_t(Sh - 1);
+    h.table[slot] = r.second;
+  }
+  h.used = true;
+  return h;
+}
+
 void group_by_agg_sorted(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
                          void* keys_out, size_t capacity, size_t* ngroups, hipStream_t st) {
   const int kb = type_bytes(ktype);
@@ -803,34 +1018,149 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
     attr_set = true;
   }
-  // partitions: about PA_GROUPS_PER_PART expected groups each
-  const size_t want = table_hint ? table_hint : std::min<size_t>(n, size_t(1) << 20);
+  // ---- heavy keys: aggregated in one pass, the other rows go on ----------
+  HotPlan hp = plan_hot(ktype, keys, n, S, st);
+  const void* ckeys = keys;  // the rows the partitioned path sees
+  size_t cn = n;
+  AccSet caccs = accs;
+  DBuf hot_tab, cold_k, cold_c[HA_MAX_ACC], hot_part, hot_cnt, hot_fin;
+  unsigned hot_groups = 0;
+  if (hp.used) {
+    const int Sh = hp.Sh;
+    static bool hot_attr = false;
+    if (!hot_attr) {
+      // (2 + accumulators) words per slot: at most 4 x 4096 or 6 x 2048 words
+      const int mx = int(size_t(4) * 2 * PA_SLOTS * 8);
+      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      hot_attr = true;
+    }
+    FHIP_REQUIRE(size_t(2 + accs.n) * Sh * 8 <= size_t(160) * 1024, FHIP_ERR_ARG,
+                 "group_by_agg: heavy-key table exceeds LDS");
+    hot_tab.reset(size_t(Sh) * 8);
+    FHIP_CHECK(hipMemcpyAsync(hot_tab.get(), hp.table.data(), size_t(Sh) * 8,
+                              hipMemcpyHostToDevice, st));
+    cold_k.reset(n * size_t(kb));
+    Cols cold{};
+    cold.n = accs.n;
+    for (int a = 0; a < accs.n; a++) {
+      cold.bytes[a] = type_bytes(accs.a[a].vtype);
+      cold_c[a].reset(n * size_t(cold.bytes[a]));
+      cold.out[a] = cold_c[a].get();
+      cold.in[a] = nullptr;
+    }
+    const size_t nparts = ceil_div(n, HOT_CHUNK);
+    hot_part.reset(nparts * size_t(1 + accs.n) * Sh * 8);
+    hot_cnt.reset(16);
+    FHIP_CHECK(hipMemsetAsync(hot_cnt.get(), 0, 16, st));
+    unsigned long long* cold_n = hot_cnt.as<unsigned long long>();
+    unsigned* hot_n = reinterpret_cast<unsigned*>(cold_n + 1);
+    const size_t lds = size_t(2 + accs.n) * Sh * 8;
+    {
+      ProfScope prof("groupby_hot", st);
+      switch (ktype) {
+        case FHIP_I32: hot_agg_kernel<int32_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const int32_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<int32_t>(), cold, cold_n); break;
+        case FHIP_U32: hot_agg_kernel<uint32_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const uint32_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<uint32_t>(), cold, cold_n); break;
+        case FHIP_I64: hot_agg_kernel<int64_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const int64_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<int64_t>(), cold, cold_n); break;
+        default: hot_agg_kernel<uint64_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const uint64_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<uint64_t>(), cold, cold_n); break;
+      }
+      FHIP_CHECK(hipGetLastError());
+    }
+    hot_fin.reset(size_t(S + 1) * stride * 8);
+    {
+      ProfScope prof("groupby_hot_merge", st);
+      hot_merge_kernel<<<unsigned(Sh / 64), 256, 0, st>>>(
+          hot_part.as<unsigned long long>(), int(nparts), Sh, accs, hot_tab.as<unsigned long long>(),
+          stride, hot_fin.as<unsigned long long>(), hot_n);
+      FHIP_CHECK(hipGetLastError());
+    }
+    unsigned long long hc[2] = {0, 0};
+    FHIP_CHECK(hipMemcpyAsync(hc, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    cn = size_t(hc[0]);
+    hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
+    ckeys = cold_k.get();
+    for (int a = 0; a < accs.n; a++) caccs.a[a].col = cold_c[a].get();
+  }
+
+  // Final assembly over Ptot partitions' tables (the heavy keys are the
+  // last one when used): compact, sort the group keys, gather the outputs.
+  auto assemble = [&](const unsigned long long* w, const unsigned* fn_dev,
+                      const std::vector<unsigned>& fn, int Ptot) {
+    std::vector<unsigned long long> foff(Ptot);
+    size_t G = 0;
+    for (int p = 0; p < Ptot; p++) {
+      foff[p] = G;
+      G += fn[p];
+    }
+    *ngroups = G;  // reported even when it does not fit, so the caller can resize
+    FHIP_REQUIRE(G <= capacity, FHIP_ERR_TOO_LARGE,
+                 "group_by_agg: output capacity " + std::to_string(capacity) + " < groups " +
+                     std::to_string(G));
+    if (G == 0) return;
+    DBuf fob(size_t(Ptot) * 8), entb(G * sizeof(size_t));
+    FHIP_CHECK(hipMemcpyAsync(fob.get(), foff.data(), size_t(Ptot) * 8, hipMemcpyHostToDevice, st));
+    final_index_kernel<<<Ptot, 256, 0, st>>>(fn_dev, fob.as<unsigned long long>(), Ptot, S,
+                                              entb.as<size_t>());
+    FHIP_CHECK(hipGetLastError());
+    DBuf gk(G * size_t(kb));
+    switch (ktype) {
+      case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int32_t>()); break;
+      case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint32_t>()); break;
+      case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int64_t>()); break;
+      default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint64_t>()); break;
+    }
+    FHIP_CHECK(hipGetLastError());
+    // keys are distinct, so the stable sort gives the reference's ascending order
+    sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st);
+    FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
+    for (int i = 0; i < naggs; i++) {
+      emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, aggs[i].op,
+                                               aggs[i].vtype, slot_of[i] < 0 ? 0 : slot_of[i],
+                                               aggs[i].out);
+      FHIP_CHECK(hipGetLastError());
+    }
+    FHIP_CHECK(hipStreamSynchronize(st));  // foff is a host local
+  };
+
+  if (hp.used && cn == 0) {  // every row had a heavy key
+    DBuf fnb(sizeof(unsigned));
+    FHIP_CHECK(hipMemcpyAsync(fnb.get(), &hot_groups, sizeof(unsigned), hipMemcpyHostToDevice, st));
+    assemble(hot_fin.as<unsigned long long>(), fnb.as<unsigned>(), std::vector<unsigned>{hot_groups}, 1);
+    return;
+  }
+
+  // partitions: about S * fill_pct / 100 expected groups each
+  const size_t want = table_hint ? table_hint : std::min<size_t>(cn, size_t(1) << 20);
   int pbits = 4;
   const size_t per_part = std::max<size_t>(1, size_t(S) * size_t(fill_pct) / 100);
   while (pbits < PA_MAX_PBITS && (size_t(1) << pbits) * per_part < want) pbits++;
   for (;;) {
     const int P = 1 << pbits;
-    const size_t tiles = ceil_div(ceil_div(n, PS_TILE), size_t(span_tiles));  // spans
+    const int Ptot = P + (hp.used ? 1 : 0);
+    const size_t tiles = ceil_div(ceil_div(cn, PS_TILE), size_t(span_tiles));  // spans
     DBuf cb(size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int));
     unsigned long long* cnt = cb.as<unsigned long long>();
     unsigned long long* incl = ib.as<unsigned long long>();
     FHIP_CHECK(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
-    DBuf pkeys(n * size_t(kb));
+    DBuf pkeys(cn * size_t(kb));
     DBuf pcols[HA_MAX_ACC];
     Cols cols{};
-    cols.n = accs.n;
-    for (int a = 0; a < accs.n; a++) {
-      cols.bytes[a] = type_bytes(accs.a[a].vtype);
-      cols.in[a] = accs.a[a].col;
-      pcols[a].reset(n * size_t(cols.bytes[a]));
+    cols.n = caccs.n;
+    for (int a = 0; a < caccs.n; a++) {
+      cols.bytes[a] = type_bytes(caccs.a[a].vtype);
+      cols.in[a] = caccs.a[a].col;
+      pcols[a].reset(cn * size_t(cols.bytes[a]));
       cols.out[a] = pcols[a].get();
     }
     auto run = [&](int pass) {
       switch (ktype) {
-        case FHIP_I32: launch_pa<int32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        case FHIP_U32: launch_pa<uint32_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        case FHIP_I64: launch_pa<int64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        default: launch_pa<uint64_t>(keys, n, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_I32: launch_pa<int32_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_U32: launch_pa<uint32_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_I64: launch_pa<int64_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        default: launch_pa<uint64_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
       }
     };
     {
@@ -838,10 +1168,10 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       run(1);
       prefix_sum(FHIP_U64, cnt, incl, size_t(P) * tiles, st);
     }
-    // partition sizes: inclusive offset at the last tile of every partition
+    // partition sizes: inclusive offset at the last span of every partition
     std::vector<unsigned long long> ends(P);
     copy_strided_to_host(incl + (tiles - 1), tiles, size_t(P), ends.data(), st);
-    // chunks of at most PA_CHUNK rows per partition
+    // chunks of at most `chunk` rows per partition
     std::vector<Item> items;
     std::vector<unsigned> part_items(P + 1);
     unsigned long long run_off = 0;
@@ -865,8 +1195,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
                               hipMemcpyHostToDevice, st));
     DBuf partial(NI * size_t(S + 1) * stride * 8), pn(NI * sizeof(unsigned));
     FHIP_CHECK(hipMemsetAsync(pn.get(), 0, NI * sizeof(unsigned), st));
-    AccSet paccs = accs;  // the partitioned copies of the columns
-    for (int a = 0; a < accs.n; a++) paccs.a[a].col = pcols[a].get();
+    AccSet paccs = caccs;  // the partitioned copies of the columns
+    for (int a = 0; a < caccs.n; a++) paccs.a[a].col = pcols[a].get();
     {
       ProfScope prof("hashagg", st);
       switch (ktype) {
@@ -877,16 +1207,16 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       }
       FHIP_CHECK(hipGetLastError());
     }
-    DBuf fin(size_t(P) * (S + 1) * stride * 8), fnb(size_t(P) * sizeof(unsigned));
+    DBuf fin(size_t(Ptot) * (S + 1) * stride * 8), fnb(size_t(Ptot) * sizeof(unsigned));
     {
       ProfScope prof("groupby_merge", st);
       pa_merge_kernel<<<P, PA_NT, lds_table_bytes, st>>>(
-          pib.as<unsigned>(), partial.as<unsigned long long>(), pn.as<unsigned>(), accs, S, stride,
+          pib.as<unsigned>(), partial.as<unsigned long long>(), pn.as<unsigned>(), caccs, S, stride,
           fin.as<unsigned long long>(), fnb.as<unsigned>(), ovf.as<int>());
       FHIP_CHECK(hipGetLastError());
     }
     int ov = 0;
-    std::vector<unsigned> fn(P);
+    std::vector<unsigned> fn(Ptot);
     FHIP_CHECK(hipMemcpyAsync(&ov, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipMemcpyAsync(fn.data(), fnb.get(), size_t(P) * sizeof(unsigned),
                               hipMemcpyDeviceToHost, st));
@@ -899,40 +1229,15 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       group_by_agg_sorted(ktype, keys, n, aggs, naggs, keys_out, capacity, ngroups, st);
       return;
     }
-    std::vector<unsigned long long> foff(P);
-    size_t G = 0;
-    for (int p = 0; p < P; p++) {
-      foff[p] = G;
-      G += fn[p];
+    if (hp.used) {  // the heavy keys' table is the last partition
+      FHIP_CHECK(hipMemcpyAsync(fin.as<unsigned long long>() + size_t(P) * (S + 1) * stride,
+                                hot_fin.get(), size_t(hot_groups) * stride * 8,
+                                hipMemcpyDeviceToDevice, st));
+      FHIP_CHECK(hipMemcpyAsync(fnb.as<unsigned>() + P, &hot_groups, sizeof(unsigned),
+                                hipMemcpyHostToDevice, st));
+      fn[P] = hot_groups;
     }
-    *ngroups = G;  // reported even when it does not fit, so the caller can resize
-    FHIP_REQUIRE(G <= capacity, FHIP_ERR_TOO_LARGE,
-                 "group_by_agg: output capacity " + std::to_string(capacity) + " < groups " +
-                     std::to_string(G));
-    if (G == 0) return;
-    DBuf fob(size_t(P) * 8), entb(G * sizeof(size_t));
-    FHIP_CHECK(hipMemcpyAsync(fob.get(), foff.data(), size_t(P) * 8, hipMemcpyHostToDevice, st));
-    final_index_kernel<<<P, 256, 0, st>>>(fnb.as<unsigned>(), fob.as<unsigned long long>(), P, S,
-                                           entb.as<size_t>());
-    FHIP_CHECK(hipGetLastError());
-    const unsigned long long* w = fin.as<unsigned long long>();
-    DBuf gk(G * size_t(kb));
-    switch (ktype) {
-      case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int32_t>()); break;
-      case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint32_t>()); break;
-      case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int64_t>()); break;
-      default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint64_t>()); break;
-    }
-    FHIP_CHECK(hipGetLastError());
-    // keys are distinct, so the stable sort gives the reference's ascending order
-    sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st);
-    FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
-    for (int i = 0; i < naggs; i++) {
-      emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, aggs[i].op,
-                                               aggs[i].vtype, slot_of[i] < 0 ? 0 : slot_of[i],
-                                               aggs[i].out);
-      FHIP_CHECK(hipGetLastError());
-    }
+    assemble(fin.as<unsigned long long>(), fnb.as<unsigned>(), fn, Ptot);
     return;
   }
 }

@@ -235,3 +235,35 @@ def test_group_by_agg_narrow_key_range(fa, kt, base):
         assert np.array_equal(s.numpy(), oracle.grouped_agg("sum", v, gi, split))
         assert np.array_equal(c.numpy(), oracle.grouped_agg("count", v, gi, split))
         assert np.array_equal(mx.numpy(), oracle.grouped_agg("max", v, gi, split))
+
+
+def test_group_by_agg_heavy_keys_and_rest(fa):
+    # a skewed column (heavy keys aggregated in one pass, the other rows
+    # partitioned): Zipf heads plus uniform tails, INT64_MIN among the heavy
+    # rows (never a heavy-table key), and an all-heavy column
+    from frovedis_amd import autotune as at
+    rng = np.random.default_rng(31)
+    n = 1 << 21
+    from frovedis_amd import datagen as dg
+    k = dg.zipf_keys_np(n, n_values=1 << 16) * 3 - 50000
+    k[::9] = np.iinfo(np.int64).min
+    v = dg.normal_np(n)
+    iv = rng.integers(-7, 7, n).astype(np.int32)
+    for kk in [k, (k % 5).astype(np.int64)]:
+        gi, split, keys = oracle.group_by(kk)
+        for pct in [None, 101]:  # default, and the heavy-key pass switched off
+            at.tune_set("groupby.hot_min_pct", pct)
+            gk, (s, c, a, mx, mn) = fa.group_by_agg(
+                fa.DeviceArray.from_numpy(kk),
+                [("sum", fa.DeviceArray.from_numpy(v)), ("count", None),
+                 ("avg", fa.DeviceArray.from_numpy(v)), ("max", fa.DeviceArray.from_numpy(iv)),
+                 ("min", fa.DeviceArray.from_numpy(iv))])
+            assert np.array_equal(gk.numpy(), keys)
+            assert np.array_equal(c.numpy(), oracle.grouped_agg("count", v, gi, split))
+            assert np.array_equal(mx.numpy(), oracle.grouped_agg("max", iv, gi, split))
+            assert np.array_equal(mn.numpy(), oracle.grouped_agg("min", iv, gi, split))
+            mass = _abs_mass(v, gi, split)
+            _check_fp(s.numpy(), oracle.grouped_agg("sum", v, gi, split), mass, FP64_TOL)
+            cnt = oracle.grouped_agg("count", v, gi, split).astype(np.float64)
+            _check_fp(a.numpy(), oracle.grouped_agg("avg", v, gi, split), mass / cnt, FP64_TOL)
+    at.tune_set("groupby.hot_min_pct", None)
