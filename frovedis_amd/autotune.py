@@ -9,6 +9,8 @@ saved and loaded as one flat JSON object (fhip_autotune_save / _load):
   the slots (the hash load factor);
 - ``groupby.chunk``: rows aggregated by one workgroup;
 - ``groupby.span``: 4096-row tiles partitioned by one workgroup;
+- ``groupby.hot``: slots of the heavy-key table (0 turns the heavy-key pass
+  off); ``groupby.hot_min_pct``: the sample share the heavy keys must cover;
 - ``radix.shape_k`` / ``radix.shape_kv``: the radix sort pass's tile shape
   (threads x keys per thread, an index into RADIX_SHAPES) for key-only and
   key/value sorts.
@@ -35,6 +37,7 @@ GROUPBY_SPACE = {
     "groupby.fill_pct": (40, 50, 70),
     "groupby.chunk": (1 << 16, 1 << 17, 1 << 18),
     "groupby.span": (4, 16),
+    "groupby.hot": (0, 4096),
 }
 
 RADIX_SHAPES = ((256, 16), (256, 32), (512, 16), (512, 24), (512, 12), (1024, 8), (1024, 16),

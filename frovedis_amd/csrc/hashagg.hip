@@ -36,7 +36,7 @@ constexpr int PA_UNROLL = 4;          // rows in flight per thread (aggregate)
 constexpr int PS_NT = 512;            // threads of the count / scatter kernels
 constexpr int PS_ITEMS = 8;           // rows per thread
 constexpr int PS_TILE = PS_NT * PS_ITEMS;  // rows per scatter tile
-constexpr int PS_SPAN = 4;             // default tiles per count / scatter workgroup (a "span")
+constexpr int PS_SPAN = 16;            // default tiles per count / scatter workgroup (a "span")
 constexpr unsigned long long PA_CHUNK = 1ull << 17;  // rows per aggregate workgroup
 constexpr int PA_SLOTS = 2048;        // LDS table slots (power of two)
 constexpr int PA_MAX_PBITS = 12;      // 4096 partitions (LDS of the scatter kernel)

@@ -75,7 +75,10 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *   "groupby.fill_pct"  expected groups per partition as a percentage of the
  *       slots (the hash load factor);
  *   "groupby.chunk"     rows aggregated by one workgroup;
- *   "groupby.span"      4096-row tiles partitioned by one workgroup.
+ *   "groupby.span"      4096-row tiles partitioned by one workgroup;
+ *   "groupby.hot"       slots of the heavy-key table (0 switches the heavy-key
+ *       pass off); "groupby.hot_min_pct" the sample share the heavy keys
+ *       must cover for the pass to run.
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);
