@@ -288,7 +288,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU's share only (--gpus {args.gpus} requested); "
-              "the multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+              "sharding the rows over several GPUs is not part of this build.", file=sys.stderr)
         sys.exit(2)
 
     import torch
