@@ -37,7 +37,7 @@ void prefix_sum(int type, const void* in, void* out, size_t n, hipStream_t st);
 
 namespace {
 
-constexpr int LS_B1 = 4096;        // coarse buckets (routed by the network)
+constexpr int LS_B1 = 1024;        // coarse buckets (routed by the network)
 constexpr int LS_NF = 65536;       // leaves (fine buckets, sample quantiles)
 constexpr int L2_LDS_FAN = 2048;   // leaves per coarse bucket kept in LDS
 constexpr int L2_NT = 512;

@@ -17,7 +17,7 @@ from .core import DeviceArray, _sz, current_stream
 __all__ = ["CdfModel", "mlp_nparam", "mlp_grad", "mlp_forward", "mlp_adam"]
 
 KNOTS = 4096
-COARSE = 4096
+COARSE = 1024
 
 
 class CdfModel:

@@ -246,7 +246,7 @@ int fhip_grouped_agg(int op, int vtype, const void* col, size_t col_n,
  *  - fhip_learned_train: FP32 MLP 1->64->64->1 trained on the GPU (Adam,
  *    `steps` full-batch steps, learning rate lr) on (key, rank) samples of an
  *    already sorted device array (every n/samples-th key), then distilled
- *    into 4096 coarse splitter keys (the network routes keys to coarse
+ *    into 1024 coarse splitter keys (the network routes keys to coarse
  *    buckets) over 65536 leaves at the sample quantiles.
  *  - fhip_dev_learned_sort: sorts in place; *fallback_keys gets how many
  *    keys went through the plain radix sort because their leaf overflowed
