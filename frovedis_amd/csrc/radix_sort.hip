@@ -86,6 +86,11 @@ __global__ __launch_bounds__(256) void rs_stats(const K* __restrict__ keys, size
 }
 
 // ---- all-digit histograms in one read -------------------------------------
+// Every thread loads RH_ITEMS keys per step (independent loads in flight),
+// then counts each byte position into LDS; a wave whose keys share a digit
+// adds them with one atomic.
+constexpr int RH_ITEMS = 8;
+
 template <class U>
 __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, size_t n,
                                                  int mode, int desc, int nbytes,
@@ -94,21 +99,30 @@ __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, siz
   for (int i = threadIdx.x; i < int(sizeof(U)) * 256; i += RS_NT) h[i] = 0;
   __syncthreads();
   const unsigned lane = __lane_id();
-  const size_t stride = size_t(gridDim.x) * RS_NT;
-  for (size_t base = size_t(blockIdx.x) * RS_NT + (threadIdx.x & ~63u); base < n;
-       base += stride) {
-    const size_t i = base + lane;
-    const bool valid = i < n;
-    const U k = valid ? keys[i] : U(0);
-    const unsigned long long vmask = __ballot(valid);
-    for (int b = 0; b < nbytes; b++) {
-      unsigned d = key_digit(k, mode, 8 * b, desc);
-      unsigned d0 = __shfl(d, 0, 64);  // lane 0 is always valid here
-      unsigned long long same = __ballot(valid && d == d0);
-      if (same == vmask) {
-        if (lane == 0) atomicAdd(&h[b * 256 + d0], unsigned(__popcll(vmask)));
-      } else if (valid) {
-        atomicAdd(&h[b * 256 + d], 1u);
+  const size_t step = size_t(gridDim.x) * RS_NT * RH_ITEMS;
+  for (size_t base = size_t(blockIdx.x) * RS_NT * RH_ITEMS + (threadIdx.x & ~63u); base < n;
+       base += step) {
+    U k[RH_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RH_ITEMS; j++) {
+      const size_t i = base + size_t(j) * RS_NT + lane;
+      k[j] = i < n ? keys[i] : U(0);
+    }
+#pragma unroll
+    for (int j = 0; j < RH_ITEMS; j++) {
+      const size_t w0 = base + size_t(j) * RS_NT;  // the wave's first key of item j
+      if (w0 >= n) break;                          // the same for the whole wave
+      const bool valid = w0 + lane < n;
+      const unsigned long long vmask = __ballot(valid);
+      for (int b = 0; b < nbytes; b++) {
+        unsigned d = key_digit(k[j], mode, 8 * b, desc);
+        unsigned d0 = __shfl(d, 0, 64);  // lane 0 is always valid here
+        unsigned long long same = __ballot(valid && d == d0);
+        if (same == vmask) {
+          if (lane == 0) atomicAdd(&h[b * 256 + d0], unsigned(__popcll(vmask)));
+        } else if (valid) {
+          atomicAdd(&h[b * 256 + d], 1u);
+        }
       }
     }
   }
@@ -426,7 +440,7 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   unsigned long long* ghist = hist.as<unsigned long long>();
   unsigned long long* gbase = ghist + 8 * 256;
   FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
-  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * 4), size_t(fill_blocks(RS_NT, 4))));
+  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), size_t(fill_blocks(RS_NT, 4))));
   {
     ProfScope prof("radix_hist", st);
     if (kbytes == 4)
@@ -574,7 +588,7 @@ void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int v
   zero_u64_kernel<<<grid_for_n(L.lb_words), 256, 0, st>>>(lb0, L.lb_words);
   zero_u64_kernel<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(ctr), 1);
   FHIP_CHECK(hipGetLastError());
-  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * 4), size_t(fill_blocks(RS_NT, 4))));
+  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), size_t(fill_blocks(RS_NT, 4))));
   if (kbytes == 4)
     rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, KM_RAW, 0, nbytes, ghist);
   else
