@@ -20,7 +20,8 @@ int fhip_dev_grouped_agg(int op, int vtype, const void* col, size_t col_n, const
 namespace fhip {
 size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, const void* right,
                  const size_t* ridx, size_t nr, int strategy, size_t* lo, size_t* ro,
-                 size_t capacity, int* used_hash, hipStream_t st);
+                 size_t capacity, int* used_hash, size_t* only_left, size_t* n_only,
+                 hipStream_t st);
 namespace {
 
 DBuf to_dev(const void* h, size_t bytes) {
@@ -104,7 +105,7 @@ int fhip_equi_join(int ktype, const void* left, const size_t* left_idx, size_t n
     fhip::DBuf dlo(capacity * 8), dro(capacity * 8);
     size_t r = fhip::equi_join(ktype, dl.get(), dli.as<size_t>(), nl, dr.get(), dri.as<size_t>(),
                                nr, strategy, dlo.as<size_t>(), dro.as<size_t>(), capacity,
-                               used_hash, nullptr);
+                               used_hash, nullptr, nullptr, nullptr);
     *nout = r;
     FHIP_REQUIRE(r <= capacity, FHIP_ERR_TOO_LARGE, "equi_join: output capacity too small");
     fhip::to_host(left_out, dlo.get(), r * 8);

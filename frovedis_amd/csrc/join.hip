@@ -36,6 +36,22 @@ struct HitWriter {
   }
 };
 
+struct MissPred {
+  const uint8_t* f;
+  __device__ bool operator()(size_t i) const { return f[i] == 0; }
+};
+
+struct NoMatchPred {
+  const unsigned long long* cnt;
+  __device__ bool operator()(size_t i) const { return cnt[i] == 0; }
+};
+
+// i -> the row id at position i
+struct RowIdMap {
+  const size_t* ids;
+  __device__ size_t operator()(size_t i) const { return ids[i]; }
+};
+
 // number of right matches of every sorted left key, and where they start.
 // One workgroup per block of MC_NT sorted left keys: the block's right range
 // comes from two binary searches over all of RIGHT (its first and last
@@ -164,15 +180,27 @@ bool sample_has_duplicate(int ktype, const K* right, size_t nr, hipStream_t st) 
   return f != 0;
 }
 
+// only_left (may be null): receives the left row ids without a match in
+// stable key order of LEFT (outer_equi_join, join.hpp:352-378), *n_only
+// their count.
 template <class K>
 size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, const K* right,
                     const size_t* ridx, size_t nr, size_t* lo, size_t* ro, size_t capacity,
-                    hipStream_t st) {
-  if (nl == 0 || nr == 0) return 0;
+                    size_t* only_left, size_t* n_only, hipStream_t st) {
+  if (n_only) *n_only = 0;
+  if (nl == 0) return 0;
+  if (nr == 0 && !only_left) return 0;
   DBuf lk(nl * sizeof(K)), li(nl * sizeof(size_t)), rk(nr * sizeof(K)), ri(nr * sizeof(size_t));
   // radix_sort(right, right_idx); radix_sort(left, left_idx) (join.hpp:320-322)
-  sort_pairs(ktype, right, rk.get(), 8, ridx, ri.get(), nr, false, false, st);
+  if (nr) sort_pairs(ktype, right, rk.get(), 8, ridx, ri.get(), nr, false, false, st);
   sort_pairs(ktype, left, lk.get(), 8, lidx, li.get(), nl, false, false, st);
+  if (nr == 0) {
+    FHIP_CHECK(hipMemcpyAsync(only_left, li.get(), nl * sizeof(size_t), hipMemcpyDeviceToDevice,
+                              st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    *n_only = nl;
+    return 0;
+  }
   DBuf cnt(nl * sizeof(unsigned long long)), first(nl * sizeof(unsigned long long)),
       incl(nl * sizeof(unsigned long long));
   {
@@ -194,14 +222,22 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
         first.as<unsigned long long>(), li.as<size_t>(), ri.as<size_t>(), nl, lo, ro);
     FHIP_CHECK(hipGetLastError());
   }
+  if (only_left)
+    *n_only = compact(NoMatchPred{cnt.as<unsigned long long>()}, RowIdMap{li.as<size_t>()}, nl,
+                      only_left, st);
   return size_t(total);
 }
 
 }  // namespace
 
+// Inner join, or LEFT outer join when only_left is given (outer_equi_join,
+// join.hpp:328-379): then only_left (nl entries) receives the left row ids
+// without a match and *n_only their count.
 size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, const void* right,
                  const size_t* ridx, size_t nr, int strategy, size_t* lo, size_t* ro,
-                 size_t capacity, int* used_hash, hipStream_t st) {
+                 size_t capacity, int* used_hash, size_t* only_left, size_t* n_only,
+                 hipStream_t st) {
+  if (n_only) *n_only = 0;
   FHIP_REQUIRE(strategy >= FHIP_JOIN_AUTO && strategy <= FHIP_JOIN_SORTMERGE, FHIP_ERR_ARG,
                "equi_join: bad strategy");
   if (used_hash) *used_hash = 0;
@@ -227,6 +263,9 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
         ProfScope prof("join_compact", st);
         r = compact_apply(HitPred{found.as<uint8_t>()},
                           HitWriter{lidx, vals.as<uint64_t>(), lo, ro}, nl, capacity, st);
+        // left_idx[missed], missed ascending (join.hpp:340-350)
+        if (only_left)
+          *n_only = compact(MissPred{found.as<uint8_t>()}, RowIdMap{lidx}, nl, only_left, st);
       } catch (...) {
         ht_destroy(h);
         throw;
@@ -243,7 +282,7 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
   dispatch_type(ktype, [&](auto z) {
     using K = decltype(z);
     r = sort_merge_t<K>(ktype, (const K*)left, lidx, nl, (const K*)right, ridx, nr, lo, ro,
-                        capacity, st);
+                        capacity, only_left, n_only, st);
   });
   return r;
 }
@@ -260,10 +299,34 @@ extern "C" int fhip_dev_equi_join(int ktype, const void* left, const size_t* lef
                  FHIP_ERR_ARG, "equi_join: null input");
     FHIP_REQUIRE(capacity == 0 || (left_out && right_out), FHIP_ERR_ARG, "equi_join: null output");
     size_t r = fhip::equi_join(ktype, left, left_idx, nl, right, right_idx, nr, strategy, left_out,
-                               right_out, capacity, used_hash, fhip::as_stream(stream));
+                               right_out, capacity, used_hash, nullptr, nullptr,
+                               fhip::as_stream(stream));
     *nout = r;
     FHIP_REQUIRE(r <= capacity, FHIP_ERR_TOO_LARGE,
                  "equi_join: output capacity " + std::to_string(capacity) + " < pairs " +
+                     std::to_string(r));
+  });
+}
+
+extern "C" int fhip_dev_outer_equi_join(int ktype, const void* left, const size_t* left_idx,
+                                        size_t nl, const void* right, const size_t* right_idx,
+                                        size_t nr, int strategy, size_t* left_out,
+                                        size_t* right_out, size_t capacity, size_t* nout,
+                                        size_t* only_left, size_t* n_only_left, int* used_hash,
+                                        void* stream) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(nout && n_only_left, FHIP_ERR_ARG, "outer_equi_join: null count output");
+    FHIP_REQUIRE((nl == 0 || (left && left_idx && only_left)) &&
+                     (nr == 0 || (right && right_idx)),
+                 FHIP_ERR_ARG, "outer_equi_join: null input");
+    FHIP_REQUIRE(capacity == 0 || (left_out && right_out), FHIP_ERR_ARG,
+                 "outer_equi_join: null output");
+    size_t r = fhip::equi_join(ktype, left, left_idx, nl, right, right_idx, nr, strategy, left_out,
+                               right_out, capacity, used_hash, only_left, n_only_left,
+                               fhip::as_stream(stream));
+    *nout = r;
+    FHIP_REQUIRE(r <= capacity, FHIP_ERR_TOO_LARGE,
+                 "outer_equi_join: output capacity " + std::to_string(capacity) + " < pairs " +
                      std::to_string(r));
   });
 }

@@ -1,7 +1,9 @@
 """Dataframe layer on one rank's share (the reference's
-src/frovedis/dataframe): group_by and grouped aggregation, the fused hash
-group-by, unique_hashtable and equi_join.  Each call goes to one C-ABI entry
-point of libfrovedis_hip."""
+src/frovedis/dataframe): group_by over one or several columns and grouped
+aggregation, the fused hash group-by, unique_hashtable, equi_join and the
+outer join, and the column operations around them (global_extract with
+NULLs, filter, sort).  Each call goes to one C-ABI entry point of
+libfrovedis_hip."""
 import ctypes
 
 import numpy as np
@@ -10,7 +12,8 @@ from . import TYPE_CODE, FhipError, check, lib
 from .core import DeviceArray, _sz, current_stream
 
 __all__ = ["group_by", "grouped_agg", "group_by_agg", "AGG_OPS", "UniqueHashtable",
-           "equi_join", "JOIN_STRATEGY"]
+           "equi_join", "outer_equi_join", "JOIN_STRATEGY", "NULL_INDEX", "global_extract",
+           "extract_nulls", "filter", "CMP_OPS", "sort", "sort_desc", "multi_group_by"]
 
 AGG_OPS = {"sum": 0, "count": 1, "avg": 2, "max": 3, "min": 4}
 
@@ -186,3 +189,157 @@ def equi_join(left, left_idx, right, right_idx, strategy="auto", capacity=None, 
         m = nout.value
         return lo[:m], ro[:m], bool(used.value)
     raise FhipError("equi_join: output did not fit after resizing")
+
+
+def outer_equi_join(left, left_idx, right, right_idx, strategy="auto", capacity=None,
+                    stream=None):
+    """outer_equi_join<T> (dataframe/join.hpp:328-379): returns
+    (left_idx_out, right_idx_out, only_left, used_hash); only_left are the
+    left row ids without a match, whose right columns are NULL."""
+    if left.dtype != right.dtype:
+        raise FhipError("column types are different")
+    dev = left.t.device
+    cap = int(capacity) if capacity is not None else max(1, len(left))
+    only = DeviceArray.empty(max(1, len(left)), np.uint64, dev)
+    for _ in range(2):
+        lo = DeviceArray.empty(cap, np.uint64, dev)
+        ro = DeviceArray.empty(cap, np.uint64, dev)
+        nout = ctypes.c_size_t(0)
+        nonly = ctypes.c_size_t(0)
+        used = ctypes.c_int(0)
+        rc = lib().fhip_dev_outer_equi_join(left.code, left.ptr, left_idx.ptr, _sz(len(left)),
+                                            right.ptr, right_idx.ptr, _sz(len(right)),
+                                            JOIN_STRATEGY[strategy], lo.ptr, ro.ptr, _sz(cap),
+                                            ctypes.byref(nout), only.ptr, ctypes.byref(nonly),
+                                            ctypes.byref(used), stream or current_stream())
+        if rc == 6 and nout.value > cap:
+            cap = nout.value
+            continue
+        check(rc)
+        m = nout.value
+        return lo[:m], ro[:m], only[: nonly.value], bool(used.value)
+    raise FhipError("outer_equi_join: output did not fit after resizing")
+
+
+NULL_INDEX = (1 << 64) - 1
+"""A row index that selects NULL (numeric_limits<T>::max()) in
+global_extract: an outer join's missing right row."""
+
+
+def global_extract(col, idx, stream=None):
+    """col[idx] on one rank (global_extract, dfcolumn_misc.cc:714-761, with
+    append_nulls :1052-1078 for indices equal to NULL_INDEX)."""
+    dev = idx.t.device
+    out = DeviceArray.empty(len(idx), col.dtype, dev)
+    check(lib().fhip_dev_gather(col.code, col.ptr, _sz(len(col)), idx.ptr, _sz(len(idx)),
+                                out.ptr, stream or current_stream()))
+    return out
+
+
+def extract_nulls(idx, src_nulls=None, src_len=None, stream=None):
+    """Null positions of global_extract(col, idx): where idx is NULL_INDEX
+    or points at a null row of the source (dfcolumn.cc:223-233)."""
+    dev = idx.t.device
+    out = DeviceArray.empty(max(1, len(idx)), np.uint64, dev)
+    nn = 0 if src_nulls is None else len(src_nulls)
+    if nn and src_len is None:
+        raise FhipError("extract_nulls: src_len is needed with src_nulls")
+    nout = ctypes.c_size_t(0)
+    check(lib().fhip_dev_gather_nulls(idx.ptr, _sz(len(idx)), _sz(src_len or 0),
+                                      src_nulls.ptr if nn else None, _sz(nn), out.ptr,
+                                      ctypes.byref(nout), stream or current_stream()))
+    return out[: nout.value]
+
+
+CMP_OPS = {"eq": 0, "neq": 1, "lt": 2, "le": 3, "gt": 4, "ge": 5}
+
+
+def filter(col, op, other, nulls=None, other_nulls=None, stream=None):
+    """filter_<op> (other a DeviceArray of the same length) or
+    filter_<op>_immed (other a numpy scalar; its dtype is the scalar's C++
+    type) on one rank (dataframe/dfcolumn_filter_*.cc): the ascending row
+    positions where the comparison holds, null rows removed."""
+    if op not in CMP_OPS:
+        raise FhipError(f"unknown comparison {op!r}")
+    n = len(col)
+    dev = col.t.device
+    out = DeviceArray.empty(max(1, n), np.uint64, dev)
+    nout = ctypes.c_size_t(0)
+    nn = 0 if nulls is None else len(nulls)
+    if isinstance(other, DeviceArray):
+        if len(other) != n:
+            raise FhipError("filter: columns of different sizes")
+        on = 0 if other_nulls is None else len(other_nulls)
+        check(lib().fhip_dev_filter_pair(col.code, col.ptr, other.code, other.ptr, _sz(n),
+                                         CMP_OPS[op], nulls.ptr if nn else None, _sz(nn),
+                                         other_nulls.ptr if on else None, _sz(on), out.ptr,
+                                         ctypes.byref(nout), stream or current_stream()))
+    else:
+        sc = np.asarray(other)
+        if sc.dtype not in TYPE_CODE:
+            raise FhipError(f"unsupported scalar type {sc.dtype}")
+        buf = np.array([sc], dtype=sc.dtype)
+        check(lib().fhip_dev_filter(col.code, col.ptr, _sz(n), CMP_OPS[op], TYPE_CODE[sc.dtype],
+                                    buf.ctypes.data_as(ctypes.c_void_p),
+                                    nulls.ptr if nn else None, _sz(nn), out.ptr,
+                                    ctypes.byref(nout), stream or current_stream()))
+    return out[: nout.value]
+
+
+def _sort(col, idx, descending, stream):
+    if idx is not None and len(idx) and int(idx.t.max()) >= len(col):
+        raise FhipError("sort: index points beyond the column")
+    n = len(col) if idx is None else len(idx)
+    dev = col.t.device
+    keys = DeviceArray.empty(n, col.dtype, dev)
+    out = DeviceArray.empty(n, np.uint64, dev)
+    check(lib().fhip_dev_sort_index(col.code, col.ptr, _sz(len(col)),
+                                    idx.ptr if idx is not None else None, _sz(n),
+                                    int(descending), keys.ptr, out.ptr,
+                                    stream or current_stream()))
+    return keys, out
+
+
+def sort(col, idx=None, stream=None):
+    """dftable::sort (idx None: rows 0..n-1) or sorted_dftable::sort (idx:
+    the current order) on one rank (dfcolumn_sort.cc:141-180, :299-340):
+    returns (sorted_keys, row_order).  Stable, so sorting by c2 and then by
+    c1 orders by (c1, c2)."""
+    return _sort(col, idx, False, stream)
+
+
+def sort_desc(col, idx=None, stream=None):
+    """sort_desc (dfcolumn_sort.cc:254-296): stable descending order."""
+    return _sort(col, idx, True, stream)
+
+
+def multi_group_by(cols, local_idx=None, stream=None):
+    """dftable::group_by over several key columns on one rank (dftable.cc:
+    367-487): returns (local_grouped_idx, local_idx_split, [group keys of
+    every column]); groups in lexicographic key order (column 0 first).
+    The result feeds grouped_agg unchanged."""
+    if not cols:
+        raise FhipError("multi_group_by: no key column")
+    dev = cols[0].t.device
+    n0 = len(cols[0])
+    if any(len(c) != n0 for c in cols):
+        raise FhipError("multi_group_by: columns of different sizes")
+    if local_idx is None:
+        import torch
+        local_idx = DeviceArray(torch.arange(n0, dtype=torch.int64, device=dev), np.uint64)
+    if local_idx.dtype != np.dtype(np.uint64):
+        raise FhipError("multi_group_by: local_idx must be size_t (uint64)")
+    n = len(local_idx)
+    if n and int(local_idx.t.max()) >= n0:
+        raise FhipError("multi_group_by: local_idx points beyond the columns")
+    split = DeviceArray.empty(n + 1, np.uint64, dev)
+    keys = [DeviceArray.empty(max(n, 1), c.dtype, dev) for c in cols]
+    k = len(cols)
+    kt = (ctypes.c_int * k)(*[c.code for c in cols])
+    cp = (ctypes.c_void_p * k)(*[c.ptr.value for c in cols])
+    kp = (ctypes.c_void_p * k)(*[x.ptr.value for x in keys])
+    ns = ctypes.c_size_t(0)
+    check(lib().fhip_dev_multi_group_by(k, kt, cp, local_idx.ptr, _sz(n), split.ptr,
+                                        ctypes.byref(ns), kp, stream or current_stream()))
+    G = ns.value - 1
+    return local_idx, split[: ns.value], [x[:G] for x in keys]

@@ -217,6 +217,98 @@ int fhip_dev_equi_join(int ktype, const void* left, const size_t* left_idx,
                        size_t* right_out, size_t capacity, size_t* nout,
                        int* used_hash, void* stream);
 
+/* Outer equi-join (LEFT outer), outer_equi_join<T> (dataframe/join.hpp:
+ * 328-379, VE body): the matched pairs as fhip_dev_equi_join, plus
+ * only_left, the left row ids without a match (the rows whose right columns
+ * become NULL, hash_joined_dftable, dftable.hpp:874-888).  Hash path:
+ * only_left in left order (left_idx[missed], :340-350); sort-merge path:
+ * in stable key order of LEFT (:352-378).  only_left holds nl entries;
+ * *n_only_left receives the count. */
+int fhip_dev_outer_equi_join(int ktype, const void* left,
+                             const size_t* left_idx, size_t nl,
+                             const void* right, const size_t* right_idx,
+                             size_t nr, int strategy, size_t* left_out,
+                             size_t* right_out, size_t capacity, size_t* nout,
+                             size_t* only_left, size_t* n_only_left,
+                             int* used_hash, void* stream);
+
+/* ---- materialisation, filter and sort of dftable columns -------------------
+ * FHIP_NULL_INDEX in a row index selects NULL (an outer join's missing
+ * right row).  NULL values are numeric_limits<T>::max(), the reference's
+ * NULL value (dfcolumn_misc.cc:1052-1078, append_nulls). */
+#define FHIP_NULL_INDEX ((size_t)-1)
+
+/* dst[i] = src[idx[i]], or numeric_limits<T>::max() where idx[i] ==
+ * FHIP_NULL_INDEX: global_extract on one rank (dfcolumn_misc.cc:714-735 and
+ * the store step :743-761) followed by append_nulls (:1052-1078) when the
+ * outer rows' indices are FHIP_NULL_INDEX.  An index >= src_n is an
+ * FHIP_ERR_ARG error. */
+int fhip_dev_gather(int type, const void* src, size_t src_n,
+                    const size_t* idx, size_t n, void* dst, void* stream);
+/* The null list of a gathered column: ascending positions i with idx[i] ==
+ * FHIP_NULL_INDEX or idx[i] in src_nulls (the source column's null rows):
+ * global_extract_null_helper2 (dfcolumn.cc:223-233) plus the positions
+ * append_nulls adds.  out holds n entries. */
+int fhip_dev_gather_nulls(const size_t* idx, size_t n, size_t src_n,
+                          const size_t* src_nulls, size_t nnull, size_t* out,
+                          size_t* nout, void* stream);
+
+/* comparison codes of filter (dataframe/dfcolumn_filter_{eq,lt,le,gt,ge}.cc
+ * predicates; eq and neq live in dfcolumn_filter_eq.cc) */
+#define FHIP_CMP_EQ 0
+#define FHIP_CMP_NEQ 1
+#define FHIP_CMP_LT 2
+#define FHIP_CMP_LE 3
+#define FHIP_CMP_GT 4
+#define FHIP_CMP_GE 5
+
+/* filter_*_immed on one rank: the ascending row positions i < n where
+ * col[i] <op> scalar holds, compared with C++'s usual arithmetic
+ * conversions between the column type ctype and the scalar type stype
+ * (find_condition, core/find_condition.hpp:23-152, with the templated
+ * predicates of dfcolumn_filter_eq.cc:24-35), minus the column's null rows
+ * (set_difference with nulls, :100-101).  scalar is a host pointer to one
+ * value of stype.  out holds n entries. */
+int fhip_dev_filter(int ctype, const void* col, size_t n, int op, int stype,
+                    const void* scalar, const size_t* nulls, size_t nnull,
+                    size_t* out, size_t* nout, void* stream);
+/* filter_eq / neq / lt / le / gt / ge between two columns of one table
+ * (find_condition_pair, core/find_condition.hpp:412-546, dfcolumn_filter_eq.cc:
+ * 12-21 and :60-102, :160-214).  Null rows
+ * of a are removed; those of b too except for eq (the reference skips them
+ * there: NULL never equals a non-NULL value). */
+int fhip_dev_filter_pair(int atype, const void* a, int btype, const void* b,
+                         size_t n, int op, const size_t* a_nulls,
+                         size_t na_null, const size_t* b_nulls, size_t nb_null,
+                         size_t* out, size_t* nout, void* stream);
+
+/* dftable::sort / sort_desc on one rank (dftable.cc:96-122 ->
+ * dfcolumn_sort.cc:69-88 and :141-180): the stable order of the column,
+ * ascending (radix_sort) or descending (radix_sort_desc).  idx_in NULL:
+ * the rows 0..n-1 of col (n == col_n), the dftable case.  idx_in given: the
+ * current order of a sorted_dftable (sorted_dftable::sort, dftable.cc:
+ * 2778-2800 -> sort_with_idx, dfcolumn_sort.cc:299-340); the keys are
+ * col[idx_in[i]], and sorting again by another column is the reference's
+ * multi-column sort ("sort c2, then c1").  Outputs: sorted_out (n keys) and
+ * idx_out (n row ids); idx_out may equal idx_in. */
+int fhip_dev_sort_index(int ktype, const void* col, size_t col_n,
+                        const size_t* idx_in, size_t n, int descending,
+                        void* sorted_out, size_t* idx_out, void* stream);
+
+/* dftable::group_by over several key columns on one rank (dftable.cc:
+ * 367-487 -> dfcolumn_groupby.cc:122-234): idx (n row ids into every
+ * column, permuted in place) is stably sorted by columns ncols-1 .. 1 and
+ * then 0 (multi_group_by_sort / _sort_split), so the groups come out in
+ * lexicographic key order; split receives the union of every column's run
+ * boundaries (merge_split = set_union, dftable.cc:336-339), n + 1 entries
+ * of capacity; keys_out[c] (n entries of capacity each) the key of column
+ * c of every group (multi_group_by_extract).  The result feeds
+ * fhip_dev_grouped_agg unchanged. */
+int fhip_dev_multi_group_by(int ncols, const int* ktypes,
+                            const void* const* cols, size_t* idx, size_t n,
+                            size_t* split, size_t* nsplit,
+                            void* const* keys_out, void* stream);
+
 /* ---- host-pointer forms (std::vector storage) ------------------------------
  * Same semantics as the dev_ forms above; inputs are copied to the device
  * and results back.  col_n is the length of the column that idx / gidx

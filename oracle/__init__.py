@@ -3,7 +3,8 @@
 numpy front end over ``liboracle.so`` (built from ``oracle/oracle.cpp``).  It
 restates the reference's x86 algorithms for the dataframe execution core
 (radix_sort, prefix_sum, set_separate, lower_bound, unique_hashtable,
-equi_join, group_by and the grouped aggregates).  Only ``tests/``,
+equi_join and the outer join, group_by over one or several columns, the
+grouped aggregates, and the gather / filter / sort of dftable columns).  Only ``tests/``,
 ``__graft_entry__.smoke()`` and the cpu_baseline leg of ``bench.py`` use it,
 as the checker; the product library never calls it.
 """
@@ -62,6 +63,20 @@ def lib():
         L.orc_group_by.argtypes = [i, vp, vp, sz, vp, vp]
         L.orc_group_by.restype = sz
         L.orc_grouped_agg.argtypes = [i, i, vp, vp, sz, vp, sz, vp, sz, vp]
+        psz = ctypes.POINTER(ctypes.POINTER(ctypes.c_size_t))
+        L.orc_outer_equi_join.argtypes = [i, vp, vp, sz, vp, vp, sz, i, psz, psz, psz,
+                                          ctypes.POINTER(sz), ctypes.POINTER(i)]
+        L.orc_outer_equi_join.restype = sz
+        L.orc_gather.argtypes = [i, vp, sz, vp, sz, vp]
+        L.orc_gather_nulls.argtypes = [vp, sz, vp, sz, vp]
+        L.orc_gather_nulls.restype = sz
+        L.orc_filter.argtypes = [i, vp, sz, i, i, vp, vp, sz, vp]
+        L.orc_filter.restype = sz
+        L.orc_filter_pair.argtypes = [i, vp, i, vp, sz, i, vp, sz, vp, sz, vp]
+        L.orc_filter_pair.restype = sz
+        L.orc_sort_index.argtypes = [i, vp, vp, sz, i, vp, vp]
+        L.orc_multi_group_by.argtypes = [i, vp, vp, vp, sz, vp, vp]
+        L.orc_multi_group_by.restype = sz
         L.orc_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -159,9 +174,13 @@ class UniqueHashtable:
             self.h = None
 
 
+JOIN_ALGO = {"ve": 0, "x86": 1, "ve_sortmerge": 2}
+
+
 def equi_join(left, left_idx, right, right_idx, algo="ve"):
     """(left_idx_out, right_idx_out, used_hash).  algo "ve" follows
-    dataframe/join.hpp:304-326, "x86" follows :25-45."""
+    dataframe/join.hpp:304-326, "x86" follows :25-45, "ve_sortmerge" the
+    sort-merge branch of "ve"."""
     l = np.ascontiguousarray(left)
     r = np.ascontiguousarray(right, dtype=l.dtype)
     li = np.ascontiguousarray(left_idx, dtype=np.uint64)
@@ -170,7 +189,7 @@ def equi_join(left, left_idx, right, right_idx, algo="ve"):
     ro = ctypes.POINTER(ctypes.c_size_t)()
     used = ctypes.c_int(0)
     m = lib().orc_equi_join(TYPE_CODE[l.dtype], _p(l), _p(li), len(l), _p(r),
-                            _p(ri), len(r), 1 if algo == "x86" else 0,
+                            _p(ri), len(r), JOIN_ALGO[algo],
                             ctypes.byref(lo), ctypes.byref(ro), ctypes.byref(used))
     a = np.ctypeslib.as_array(lo, shape=(max(m, 1),))[:m].astype(np.uint64)
     b = np.ctypeslib.as_array(ro, shape=(max(m, 1),))[:m].astype(np.uint64)
@@ -213,3 +232,112 @@ def grouped_agg(op, col, grouped_idx, split, nulls=None):
                                _p(nu) if len(nu) else None, len(nu), _p(out))
     assert rc == 0
     return out
+
+
+def _take(ptr, m):
+    a = np.ctypeslib.as_array(ptr, shape=(max(m, 1),))[:m].astype(np.uint64)
+    lib().orc_free(ctypes.cast(ptr, ctypes.c_void_p))
+    return a
+
+
+def outer_equi_join(left, left_idx, right, right_idx, algo="ve"):
+    """(left_idx_out, right_idx_out, only_left, used_hash): outer_equi_join
+    (dataframe/join.hpp:328-379 "ve", :73-99 "x86")."""
+    l = np.ascontiguousarray(left)
+    r = np.ascontiguousarray(right, dtype=l.dtype)
+    li = np.ascontiguousarray(left_idx, dtype=np.uint64)
+    ri = np.ascontiguousarray(right_idx, dtype=np.uint64)
+    P = ctypes.POINTER(ctypes.c_size_t)
+    lo, ro, ol = P(), P(), P()
+    no = ctypes.c_size_t(0)
+    used = ctypes.c_int(0)
+    m = lib().orc_outer_equi_join(TYPE_CODE[l.dtype], _p(l), _p(li), len(l), _p(r), _p(ri),
+                                  len(r), JOIN_ALGO[algo], ctypes.byref(lo), ctypes.byref(ro),
+                                  ctypes.byref(ol), ctypes.byref(no), ctypes.byref(used))
+    return _take(lo, m), _take(ro, m), _take(ol, no.value), bool(used.value)
+
+
+NULL_INDEX = np.uint64(2**64 - 1)
+CMP_CODE = {"eq": 0, "neq": 1, "lt": 2, "le": 3, "gt": 4, "ge": 5}
+
+
+def gather(src, idx):
+    """src[idx] with NULL (max of the type) where idx == NULL_INDEX."""
+    s = np.ascontiguousarray(src)
+    ix = np.ascontiguousarray(idx, dtype=np.uint64)
+    out = np.empty(len(ix), s.dtype)
+    if lib().orc_gather(TYPE_CODE[s.dtype], _p(s), len(s), _p(ix), len(ix), _p(out)):
+        raise IndexError(lib().orc_last_error().decode())
+    return out
+
+
+def gather_nulls(idx, src_nulls=None):
+    ix = np.ascontiguousarray(idx, dtype=np.uint64)
+    nu = np.zeros(0, np.uint64) if src_nulls is None else np.ascontiguousarray(src_nulls,
+                                                                                 dtype=np.uint64)
+    out = np.empty(max(len(ix), 1), np.uint64)
+    m = lib().orc_gather_nulls(_p(ix), len(ix), _p(nu) if len(nu) else None, len(nu), _p(out))
+    return out[:m].copy()
+
+
+def filter(col, op, scalar, nulls=None):
+    """Row positions where ``col op scalar`` (scalar: a numpy scalar whose
+    dtype is the scalar's C++ type), nulls removed."""
+    c = np.ascontiguousarray(col)
+    sc = np.array([scalar], dtype=np.asarray(scalar).dtype)
+    nu = np.zeros(0, np.uint64) if nulls is None else np.ascontiguousarray(nulls, dtype=np.uint64)
+    out = np.empty(max(len(c), 1), np.uint64)
+    m = lib().orc_filter(TYPE_CODE[c.dtype], _p(c), len(c), CMP_CODE[op], TYPE_CODE[sc.dtype],
+                         _p(sc), _p(nu) if len(nu) else None, len(nu), _p(out))
+    if m == 2**64 - 1:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out[:m].copy()
+
+
+def filter_pair(a, b, op, a_nulls=None, b_nulls=None):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    an = np.zeros(0, np.uint64) if a_nulls is None else np.ascontiguousarray(a_nulls,
+                                                                             dtype=np.uint64)
+    bn = np.zeros(0, np.uint64) if b_nulls is None else np.ascontiguousarray(b_nulls,
+                                                                             dtype=np.uint64)
+    out = np.empty(max(len(a), 1), np.uint64)
+    m = lib().orc_filter_pair(TYPE_CODE[a.dtype], _p(a), TYPE_CODE[b.dtype], _p(b), len(a),
+                              CMP_CODE[op], _p(an) if len(an) else None, len(an),
+                              _p(bn) if len(bn) else None, len(bn), _p(out))
+    if m == 2**64 - 1:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out[:m].copy()
+
+
+def sort_index(col, idx=None, descending=False):
+    """(sorted_keys, idx_out): dftable::sort / sort_desc (idx None) or
+    sorted_dftable::sort (idx = the current order)."""
+    c = np.ascontiguousarray(col)
+    ix = None if idx is None else np.ascontiguousarray(idx, dtype=np.uint64)
+    n = len(c) if ix is None else len(ix)
+    if ix is not None and n and int(ix.max()) >= len(c):
+        raise IndexError("sort_index: index beyond the column")
+    keys = np.empty(max(n, 1), c.dtype)
+    out = np.empty(max(n, 1), np.uint64)
+    rc = lib().orc_sort_index(TYPE_CODE[c.dtype], _p(c), _p(ix) if ix is not None else None, n,
+                              int(descending), _p(keys), _p(out))
+    assert rc == 0
+    return keys[:n].copy(), out[:n].copy()
+
+
+def multi_group_by(cols, idx=None):
+    """(grouped_idx, split, [group keys per column]): dftable::group_by over
+    several columns on one rank."""
+    cs = [np.ascontiguousarray(c) for c in cols]
+    n0 = len(cs[0])
+    gi = (np.arange(n0, dtype=np.uint64) if idx is None
+          else np.ascontiguousarray(idx, dtype=np.uint64).copy())
+    n = len(gi)
+    split = np.empty(n + 1, np.uint64)
+    keys = [np.empty(max(n, 1), c.dtype) for c in cs]
+    kt = (ctypes.c_int * len(cs))(*[TYPE_CODE[c.dtype] for c in cs])
+    cp = (ctypes.c_void_p * len(cs))(*[c.ctypes.data for c in cs])
+    kp = (ctypes.c_void_p * len(cs))(*[k.ctypes.data for k in keys])
+    m = lib().orc_multi_group_by(len(cs), kt, cp, _p(gi), n, _p(split), kp)
+    return gi, split[:m].copy(), [k[:max(m - 1, 0)].copy() for k in keys]

@@ -15,7 +15,9 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <limits>
 #include <stdexcept>
 #include <unordered_map>
@@ -151,6 +153,31 @@ void radix_sort_fp(F* k, uint8_t* v, int vb, size_t n, bool positive_only,
     for (size_t i = 0; i < n; i++) u[i] ^= ((u[i] >> sh) - 1) | top;
   } else {
     lsd(u, v, vb, n, max_key_size(u, n), desc);
+  }
+}
+
+// radix_sort(K*, size_t*, size_t) / radix_sort_desc of a key column with a
+// row-id payload, by key type (core/radix_sort.hpp:197-219, :244-340,
+// :342-487).
+template <class K>
+void sort_kv(K* k, size_t* v, size_t n, bool desc) {
+  uint8_t* vb = reinterpret_cast<uint8_t*>(v);
+  if (std::numeric_limits<K>::is_integer) {
+    if (sizeof(K) == 4) {
+      if (std::numeric_limits<K>::is_signed)
+        radix_sort_int<int32_t, uint32_t>(reinterpret_cast<int32_t*>(k), vb, 8, n, false, desc);
+      else
+        radix_sort_int<uint32_t, uint32_t>(reinterpret_cast<uint32_t*>(k), vb, 8, n, false, desc);
+    } else {
+      if (std::numeric_limits<K>::is_signed)
+        radix_sort_int<int64_t, uint64_t>(reinterpret_cast<int64_t*>(k), vb, 8, n, false, desc);
+      else
+        radix_sort_int<uint64_t, uint64_t>(reinterpret_cast<uint64_t*>(k), vb, 8, n, false, desc);
+    }
+  } else if (sizeof(K) == 4) {
+    radix_sort_fp<float, uint32_t>(reinterpret_cast<float*>(k), vb, 8, n, false, desc);
+  } else {
+    radix_sort_fp<double, uint64_t>(reinterpret_cast<double*>(k), vb, 8, n, false, desc);
   }
 }
 
@@ -351,21 +378,29 @@ void ht_exist_t(const HT* h, const void* keys, size_t n, int* out) {
 //  algo 1 (x86 body, :25-45): std::unordered_multimap on RIGHT, probe LEFT in
 //    order; the order inside a key follows libstdc++ and is only compared
 //    after canonical sorting.
-// Pinned by: brute force pair sets and the tutorial join (tests).
+//  algo 2: the sort-merge branch of algo 0 without the table build.
+// With only_left, the LEFT outer join outer_equi_join (VE :328-379, x86
+// :73-99): the left row ids without a match -- in left order on the hash
+// paths, in stable key order of LEFT on the sort-merge path.
+// Pinned by: brute force pair sets, the tutorial join and outer join
+// (doc/tutorial/tutorial.md:2362-2375, :2456-2466), and for only_left the
+// closed form "left rows whose key is absent from RIGHT" (tests).
 // ---------------------------------------------------------------------------
 template <class K>
 void equi_join_t(const void* left, const size_t* lidx, size_t nl,
                  const void* right, const size_t* ridx, size_t nr, int algo,
                  std::vector<size_t>& lo, std::vector<size_t>& ro,
-                 int* used_hash) {
+                 int* used_hash, std::vector<size_t>* only_left) {
   const K* l = reinterpret_cast<const K*>(left);
   const K* r = reinterpret_cast<const K*>(right);
   lo.clear(); ro.clear();
+  if (only_left) only_left->clear();
   if (algo == 1) {
     std::unordered_multimap<K, size_t> m;
     for (size_t i = 0; i < nr; i++) m.insert(std::make_pair(r[i], ridx[i]));
     for (size_t i = 0; i < nl; i++) {
       auto range = m.equal_range(l[i]);
+      if (range.first == range.second && only_left) only_left->push_back(lidx[i]);
       for (auto it = range.first; it != range.second; ++it) {
         lo.push_back(lidx[i]);
         ro.push_back(it->second);
@@ -374,38 +409,48 @@ void equi_join_t(const void* left, const size_t* lidx, size_t nl,
     if (used_hash) *used_hash = 1;
     return;
   }
-  HT h;
-  h.kbytes = sizeof(K);
-  int uniq = 1;
-  std::vector<uint64_t> rv(ridx, ridx + nr);
-  ht_build_t<K>(&h, right, rv.data(), nr, 1, &uniq);
-  if (uniq) {
-    std::vector<uint64_t> got(nl);
-    std::vector<size_t> miss(nl);
-    size_t nm = ht_lookup_t<K>(&h, left, nl, got.data(), miss.data());
-    size_t mi = 0;
-    for (size_t i = 0; i < nl; i++) {
-      if (mi < nm && miss[mi] == i) { mi++; continue; }
-      lo.push_back(lidx[i]);
-      ro.push_back(size_t(got[i]));
+  int uniq = 0;
+  if (algo != 2) {
+    HT h;
+    h.kbytes = sizeof(K);
+    uniq = 1;
+    std::vector<uint64_t> rv(ridx, ridx + nr);
+    ht_build_t<K>(&h, right, rv.data(), nr, 1, &uniq);
+    if (uniq) {
+      std::vector<uint64_t> got(nl);
+      std::vector<size_t> miss(nl);
+      size_t nm = ht_lookup_t<K>(&h, left, nl, got.data(), miss.data());
+      size_t mi = 0;
+      for (size_t i = 0; i < nl; i++) {
+        if (mi < nm && miss[mi] == i) { mi++; continue; }
+        lo.push_back(lidx[i]);
+        ro.push_back(size_t(got[i]));
+      }
+      // outer: left_idx[missed] (join.hpp:340-350)
+      if (only_left)
+        for (size_t t = 0; t < nm; t++) only_left->push_back(lidx[miss[t]]);
+      if (used_hash) *used_hash = 1;
+      return;
     }
-    if (used_hash) *used_hash = 1;
-    return;
   }
-  // sort-merge: stable sort of (key, idx) on both sides
-  std::vector<size_t> lp(nl), rp(nr);
-  for (size_t i = 0; i < nl; i++) lp[i] = i;
-  for (size_t i = 0; i < nr; i++) rp[i] = i;
-  std::stable_sort(lp.begin(), lp.end(), [&](size_t a, size_t b) { return l[a] < l[b]; });
-  std::stable_sort(rp.begin(), rp.end(), [&](size_t a, size_t b) { return r[a] < r[b]; });
+  // sort-merge: radix_sort(right, right_idx); radix_sort(left, left_idx)
+  // (join.hpp:320-322), then every sorted left row against its run of equal
+  // right keys (multi_equi_join :198-302); outer: the sorted left rows whose
+  // run is empty (:352-378)
+  std::vector<K> lk(l, l + nl), rk(r, r + nr);
+  std::vector<size_t> li(lidx, lidx + nl), ri(ridx, ridx + nr);
+  sort_kv(rk.data(), ri.data(), nr, false);
+  sort_kv(lk.data(), li.data(), nl, false);
   size_t j = 0;
   for (size_t ii = 0; ii < nl; ii++) {
-    K key = l[lp[ii]];
-    while (j < nr && r[rp[j]] < key) j++;
-    for (size_t t = j; t < nr && r[rp[t]] == key; t++) {
-      lo.push_back(lidx[lp[ii]]);
-      ro.push_back(ridx[rp[t]]);
+    K key = lk[ii];
+    while (j < nr && rk[j] < key) j++;
+    size_t t = j;
+    for (; t < nr && rk[t] == key; t++) {
+      lo.push_back(li[ii]);
+      ro.push_back(ri[t]);
     }
+    if (t == j && only_left) only_left->push_back(li[ii]);
   }
   if (used_hash) *used_hash = 0;
 }
@@ -424,24 +469,7 @@ size_t group_by_t(const void* col, size_t* idx, size_t n, size_t* split,
   const K* c = reinterpret_cast<const K*>(col);
   std::vector<K> nv(n);
   for (size_t i = 0; i < n; i++) nv[i] = c[idx[i]];
-  uint8_t* v = reinterpret_cast<uint8_t*>(idx);
-  if (std::numeric_limits<K>::is_integer) {
-    if (sizeof(K) == 4) {
-      if (std::numeric_limits<K>::is_signed)
-        radix_sort_int<int32_t, uint32_t>(reinterpret_cast<int32_t*>(nv.data()), v, 8, n, false, false);
-      else
-        radix_sort_int<uint32_t, uint32_t>(reinterpret_cast<uint32_t*>(nv.data()), v, 8, n, false, false);
-    } else {
-      if (std::numeric_limits<K>::is_signed)
-        radix_sort_int<int64_t, uint64_t>(reinterpret_cast<int64_t*>(nv.data()), v, 8, n, false, false);
-      else
-        radix_sort_int<uint64_t, uint64_t>(reinterpret_cast<uint64_t*>(nv.data()), v, 8, n, false, false);
-    }
-  } else if (sizeof(K) == 4) {
-    radix_sort_fp<float, uint32_t>(reinterpret_cast<float*>(nv.data()), v, 8, n, false, false);
-  } else {
-    radix_sort_fp<double, uint64_t>(reinterpret_cast<double*>(nv.data()), v, 8, n, false, false);
-  }
+  sort_kv(nv.data(), idx, n, false);
   size_t ns = separate(nv.data(), n, split);
   K* ko = reinterpret_cast<K*>(keys_out);
   for (size_t i = 0; i + 1 < ns; i++) ko[i] = nv[split[i]];
@@ -539,6 +567,174 @@ void dispatch_key(int t, F&& f) {
     case T_F64: f(double()); break;
     default: throw std::runtime_error("bad type code");
   }
+}
+
+// ---------------------------------------------------------------------------
+// Materialisation of joined / filtered tables on one rank:
+//  gather: global_extract (dataframe/dfcolumn_misc.cc:714-761) is
+//    ret[i] = val[idx[i]] once the exchange is an identity; append_nulls
+//    (:1052-1078) adds numeric_limits<T>::max() rows, here selected by the
+//    index ~0 (FHIP_NULL_INDEX).  An index past the column is an error.
+//  gather_nulls: positions i whose source row is null
+//    (global_extract_null_helper2, dfcolumn.cc:223-233: check_existence of
+//    idx[i] in the null set) or whose index is ~0 (the appended rows).
+// Pinned by: closed forms (numpy take / isin, tests).
+// ---------------------------------------------------------------------------
+constexpr size_t kNullIndex = ~size_t(0);
+
+template <class T>
+int gather_t(const T* src, size_t src_n, const size_t* idx, size_t n, T* dst) {
+  for (size_t i = 0; i < n; i++) {
+    if (idx[i] == kNullIndex) dst[i] = std::numeric_limits<T>::max();
+    else if (idx[i] < src_n) dst[i] = src[idx[i]];
+    else return -1;
+  }
+  return 0;
+}
+
+size_t gather_nulls_t(const size_t* idx, size_t n, const size_t* nulls, size_t nn,
+                      size_t* out) {
+  std::vector<size_t> s(nulls, nulls + nn);
+  std::sort(s.begin(), s.end());
+  size_t c = 0;
+  for (size_t i = 0; i < n; i++)
+    if (idx[i] == kNullIndex || std::binary_search(s.begin(), s.end(), idx[i])) out[c++] = i;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// filter on one rank: find_condition / find_condition_pair
+// (core/find_condition.hpp:23-152, :412-546) with the predicates of
+// dataframe/dfcolumn_filter_{eq,lt,le,gt,ge}.cc -- `a op b` on T and U
+// with C++'s usual arithmetic conversions -- then set_difference with the
+// sorted null lists: the column's own nulls for every filter, the other
+// column's nulls too for neq/lt/le/gt/ge (dfcolumn_filter_eq.cc:60-102,
+// :160-214; dfcolumn_filter_lt.cc:37-88).
+// op codes: 0 eq, 1 neq, 2 lt, 3 le, 4 gt, 5 ge.
+// Pinned by: the tutorial filters (doc/tutorial/src/tut5.3: int c1 against
+// double c4, double c2 against 30.0) and numpy comparisons in the common
+// type that C++ picks (tests).
+// ---------------------------------------------------------------------------
+template <class A, class B>
+bool cmp_op(A a, B b, int op) {
+  switch (op) {
+    case 0: return a == b;
+    case 1: return a != b;
+    case 2: return a < b;
+    case 3: return a <= b;
+    case 4: return a > b;
+    case 5: return a >= b;
+    default: throw std::runtime_error("bad comparison code");
+  }
+}
+
+size_t minus_nulls(std::vector<size_t>& hit, std::vector<size_t> nulls, size_t* out) {
+  std::sort(nulls.begin(), nulls.end());
+  nulls.erase(std::unique(nulls.begin(), nulls.end()), nulls.end());
+  size_t* e = std::set_difference(hit.begin(), hit.end(), nulls.begin(), nulls.end(), out);
+  return size_t(e - out);
+}
+
+template <class T, class U>
+size_t filter_t(const T* col, size_t n, int op, U c, const size_t* nulls, size_t nn,
+                size_t* out) {
+  std::vector<size_t> hit;
+  for (size_t i = 0; i < n; i++) if (cmp_op(col[i], c, op)) hit.push_back(i);
+  return minus_nulls(hit, std::vector<size_t>(nulls, nulls + nn), out);
+}
+
+template <class T, class U>
+size_t filter_pair_t(const T* a, const U* b, size_t n, int op, const size_t* an, size_t na,
+                     const size_t* bn, size_t nb, size_t* out) {
+  std::vector<size_t> hit;
+  for (size_t i = 0; i < n; i++) if (cmp_op(a[i], b[i], op)) hit.push_back(i);
+  std::vector<size_t> nulls(an, an + na);
+  if (op != 0) nulls.insert(nulls.end(), bn, bn + nb);
+  return minus_nulls(hit, nulls, out);
+}
+
+// ---------------------------------------------------------------------------
+// dftable::sort / sort_desc and sorted_dftable::sort on one rank
+// (dataframe/dfcolumn_sort.cc:69-88, :141-180, :299-340): the keys
+// val[idx[i]] (idx = iota for a dftable) are sorted with idx as payload by
+// radix_sort / radix_sort_desc; with one node the partition, exchange and
+// set_multimerge_pair leave that order unchanged.
+// Pinned by: numpy stable argsort (closed form) and the tutorial's two-key
+// sort (doc/tutorial/tutorial.md:2321-2341, tut5.4/t.csv) (tests).
+// ---------------------------------------------------------------------------
+template <class K>
+void sort_index_t(const K* col, const size_t* idx_in, size_t n, bool desc, K* sorted,
+                  size_t* idx_out) {
+  for (size_t i = 0; i < n; i++) {
+    idx_out[i] = idx_in ? idx_in[i] : i;
+    sorted[i] = col[idx_out[i]];
+  }
+  sort_kv(sorted, idx_out, n, desc);
+}
+
+// ---------------------------------------------------------------------------
+// dftable::group_by over several columns on one rank (dataframe/dftable.cc:
+// 367-487 -> dfcolumn_groupby.cc:122-234): multi_group_by_sort of columns
+// size-1 .. 1, multi_group_by_sort_split of column 0 (sort, then
+// set_separate), merge_split = set_union with multi_group_by_split of the
+// other columns (dftable.cc:336-339), multi_group_by_extract of the keys
+// (val[idx[split[g]]]).  With one node the hash exchange and the second
+// round of sorts keep that result (the keys are already sorted and unique).
+// Pinned by: the tutorial's two-column group_by (doc/tutorial/tutorial.md:
+// 2535-2552, tut5.6/t.csv, as a set of rows) and a closed form (Python
+// sort by key tuples) (tests).
+// ---------------------------------------------------------------------------
+// sorts (col[idx], idx) by one column; returns set_separate of the sorted
+// keys when `separate_out` is set
+void mg_sort(int ktype, const void* col, size_t* idx, size_t n, std::vector<size_t>* sep) {
+  dispatch_key(ktype, [&](auto z) {
+    using K = decltype(z);
+    const K* c = reinterpret_cast<const K*>(col);
+    std::vector<K> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = c[idx[i]];
+    sort_kv(v.data(), idx, n, false);
+    if (sep) {
+      sep->resize(n + 1);
+      sep->resize(separate(v.data(), n, sep->data()));
+    }
+  });
+}
+
+// set_separate of col[idx] without sorting (multi_group_by_split)
+std::vector<size_t> mg_split(int ktype, const void* col, const size_t* idx, size_t n) {
+  std::vector<size_t> sep(n + 1);
+  dispatch_key(ktype, [&](auto z) {
+    using K = decltype(z);
+    const K* c = reinterpret_cast<const K*>(col);
+    std::vector<K> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = c[idx[i]];
+    sep.resize(separate(v.data(), n, sep.data()));
+  });
+  return sep;
+}
+
+size_t multi_group_by_t(int ncols, const int* ktypes, const void* const* cols, size_t* idx,
+                        size_t n, size_t* split, void* const* keys_out) {
+  for (int i = 0; i < ncols - 1; i++)
+    mg_sort(ktypes[ncols - i - 1], cols[ncols - i - 1], idx, n, nullptr);
+  std::vector<size_t> sp;
+  mg_sort(ktypes[0], cols[0], idx, n, &sp);
+  for (int i = 1; i < ncols; i++) {
+    std::vector<size_t> t = mg_split(ktypes[i], cols[i], idx, n);
+    std::vector<size_t> u;
+    std::set_union(sp.begin(), sp.end(), t.begin(), t.end(), std::back_inserter(u));
+    sp.swap(u);
+  }
+  for (int c = 0; c < ncols; c++) {
+    dispatch_key(ktypes[c], [&](auto z) {
+      using K = decltype(z);
+      const K* col = reinterpret_cast<const K*>(cols[c]);
+      K* ko = reinterpret_cast<K*>(keys_out[c]);
+      for (size_t g = 0; g + 1 < sp.size(); g++) ko[g] = col[idx[sp[g]]];
+    });
+  }
+  std::copy(sp.begin(), sp.end(), split);
+  return sp.size();
 }
 
 thread_local std::string g_err;
@@ -671,7 +867,7 @@ size_t orc_equi_join(int ktype, const void* left, const size_t* lidx, size_t nl,
   std::vector<size_t> a, b;
   dispatch_key(ktype, [&](auto z) {
     using K = decltype(z);
-    equi_join_t<K>(left, lidx, nl, right, ridx, nr, algo, a, b, used_hash);
+    equi_join_t<K>(left, lidx, nl, right, ridx, nr, algo, a, b, used_hash, nullptr);
   });
   size_t m = a.size();
   *lo = static_cast<size_t*>(std::malloc((m ? m : 1) * sizeof(size_t)));
@@ -706,6 +902,101 @@ int orc_grouped_agg(int op, int vtype, const void* col, const size_t* gidx,
     });
   } catch (std::exception& e) { g_err = e.what(); return -1; }
   return 0;
+}
+
+
+// Outer join: returns the pair count; *lo/*ro/*only are malloc'ed (free with
+// orc_free).  algo 0 VE, 1 x86, 2 VE sort-merge branch.
+size_t orc_outer_equi_join(int ktype, const void* left, const size_t* lidx, size_t nl,
+                           const void* right, const size_t* ridx, size_t nr, int algo,
+                           size_t** lo, size_t** ro, size_t** only, size_t* n_only,
+                           int* used_hash) {
+  std::vector<size_t> a, b, c;
+  dispatch_key(ktype, [&](auto z) {
+    using K = decltype(z);
+    equi_join_t<K>(left, lidx, nl, right, ridx, nr, algo, a, b, used_hash, &c);
+  });
+  auto out = [](const std::vector<size_t>& v) {
+    size_t* p = static_cast<size_t*>(std::malloc((v.empty() ? 1 : v.size()) * sizeof(size_t)));
+    if (!v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(size_t));
+    return p;
+  };
+  *lo = out(a);
+  *ro = out(b);
+  *only = out(c);
+  *n_only = c.size();
+  return a.size();
+}
+
+int orc_gather(int type, const void* src, size_t src_n, const size_t* idx, size_t n, void* dst) {
+  int rc = 0;
+  try {
+    dispatch_key(type, [&](auto z) {
+      using T = decltype(z);
+      rc = gather_t(reinterpret_cast<const T*>(src), src_n, idx, n, reinterpret_cast<T*>(dst));
+    });
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+  if (rc) g_err = "gather: row index out of range";
+  return rc;
+}
+
+size_t orc_gather_nulls(const size_t* idx, size_t n, const size_t* nulls, size_t nn,
+                        size_t* out) {
+  return gather_nulls_t(idx, n, nulls, nn, out);
+}
+
+// out capacity n; returns the count, or (size_t)-1 on a bad code
+size_t orc_filter(int ctype, const void* col, size_t n, int op, int stype, const void* scalar,
+                  const size_t* nulls, size_t nn, size_t* out) {
+  size_t r = 0;
+  try {
+    dispatch_key(ctype, [&](auto zc) {
+      using T = decltype(zc);
+      dispatch_key(stype, [&](auto zs) {
+        using U = decltype(zs);
+        U c;
+        std::memcpy(&c, scalar, sizeof(U));
+        r = filter_t(reinterpret_cast<const T*>(col), n, op, c, nulls, nn, out);
+      });
+    });
+  } catch (std::exception& e) { g_err = e.what(); return ~size_t(0); }
+  return r;
+}
+
+size_t orc_filter_pair(int atype, const void* a, int btype, const void* b, size_t n, int op,
+                       const size_t* an, size_t na, const size_t* bn, size_t nb, size_t* out) {
+  size_t r = 0;
+  try {
+    dispatch_key(atype, [&](auto za) {
+      using T = decltype(za);
+      dispatch_key(btype, [&](auto zb) {
+        using U = decltype(zb);
+        r = filter_pair_t(reinterpret_cast<const T*>(a), reinterpret_cast<const U*>(b), n, op,
+                          an, na, bn, nb, out);
+      });
+    });
+  } catch (std::exception& e) { g_err = e.what(); return ~size_t(0); }
+  return r;
+}
+
+int orc_sort_index(int ktype, const void* col, const size_t* idx_in, size_t n, int desc,
+                   void* sorted, size_t* idx_out) {
+  try {
+    dispatch_key(ktype, [&](auto z) {
+      using K = decltype(z);
+      sort_index_t(reinterpret_cast<const K*>(col), idx_in, n, desc != 0,
+                   reinterpret_cast<K*>(sorted), idx_out);
+    });
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+  return 0;
+}
+
+// split capacity n + 1, keys_out[c] capacity n; returns the split length
+size_t orc_multi_group_by(int ncols, const int* ktypes, const void* const* cols, size_t* idx,
+                          size_t n, size_t* split, void* const* keys_out) {
+  try {
+    return multi_group_by_t(ncols, ktypes, cols, idx, n, split, keys_out);
+  } catch (std::exception& e) { g_err = e.what(); return 0; }
 }
 
 }  // extern "C"
