@@ -8,7 +8,7 @@ Prints one JSON line.  Side measurements (the plain radix sort key/value
 configuration) go under "extra".
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows LOG2] [--no-cpu]
-                  [--no-extra] [--leg groupby|radix|batch|join|learned|copy]
+                  [--no-extra] [--leg groupby|radix|batch|join|learned|copy|relational]
 """
 import argparse
 import ctypes
@@ -254,6 +254,103 @@ def bench_join(fa, steps=3, log2_left=27, log2_right=29):
     return out
 
 
+def _timed_min(fn, steps):
+    import torch
+    best, out = float("inf"), None
+    for _ in range(steps):
+        out = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best, out
+
+
+def bench_relational(fa, log2n=28, steps=3):
+    """The column operations around the core on the group-by table (2^log2n
+    rows: int64 key Zipf(1.1) over 2^20, double N(0,1), int32 U[0,1000)) and
+    on the join tables (2^29 foreign keys, 10% without a partner, outer-joined
+    to 2^27 unique keys):
+      filter      iv < 500 (filter_lt_immed): reads 4 B/row, writes 8 B per hit;
+      extract     v at the filter's rows (global_extract): 8 B index + 8 B
+                  value read + 8 B written per hit;
+      sort        dftable::sort on the key column: key/value radix sort of
+                  (key, row id), 3 significant key bytes -> 3 passes of 32 B/row
+                  plus the 8 B/row histogram read;
+      group_by2   group_by({key, iv}): the key ranges (16 + 12 B/row: row id
+                  and column), the 30-bit composite key (8 + 8 + 4 read, 4
+                  written), its key/value radix sort (4 B/row histogram, 4
+                  passes of 24 B/row) and the run boundaries (2 x 4 B/row):
+                  160 B/row;
+      outer_join  outer_hash_join of the 2^29 side to the unique 2^27 side:
+                  8 B per input row + 16 B per pair + 8 B per left-only row."""
+    import numpy as np
+    import torch
+
+    from frovedis_amd import datagen as dg
+    n = 1 << log2n
+    k, v, iv = gen_groupby(n)
+    out = {"rows": n}
+
+    def stage(name, fn, algo_bytes, kernels, check=None):
+        fn()  # warm-up: the workspace pool reaches its size
+        torch.cuda.synchronize()
+        fa.check(fa.lib().fhip_prof_enable(1))
+        t, res = _timed_min(fn, steps)
+        kern = {}
+        for kname in kernels:
+            ms, cnt = _prof_ms(fa, kname)
+            kern[kname] = {"ms_per_step": ms / steps, "launches_per_step": cnt / steps}
+        fa.check(fa.lib().fhip_prof_enable(0))
+        ab = algo_bytes(res)
+        out[name] = {"ms": t * 1e3, "rows_per_s": n / t, "algorithmic_bytes": ab,
+                     "hbm_frac": ab / t / HBM_PEAK, "kernels": kern}
+        if check is not None:
+            out[name]["checked"] = bool(check(res))
+        return res
+
+    sel = stage("filter", lambda: fa.filter(iv, "lt", np.int32(500)),
+                lambda r: 4 * n + 8 * len(r), ["filter"],
+                lambda r: len(r) == int((iv.t < 500).sum()))
+    stage("extract", lambda: fa.global_extract(v, sel), lambda r: 24 * len(sel), ["gather"],
+          lambda r: bool(torch.equal(r.t, v.t[sel.t])))
+    del sel
+    stage("sort", lambda: fa.sort(k), lambda r: 8 * n + 3 * 32 * n,
+          ["radix_hist", "radix_pass"],
+          lambda r: bool((r[0].t[1:] >= r[0].t[:-1]).all()) and
+          bool(torch.equal(k.t[r[1].t], r[0].t)))
+    torch.cuda.empty_cache()
+    g = stage("group_by2", lambda: fa.multi_group_by([k, iv]),
+              lambda r: 160 * n,
+              ["multi_group_range", "multi_group_composite", "radix_hist", "radix_pass",
+               "multi_group_split"])
+    out["group_by2"]["groups"] = len(g[1]) - 1
+    del g, k, v, iv
+    torch.cuda.empty_cache()
+
+    nl, nr = 1 << 27, 1 << 29
+    lk = fa.DeviceArray(dg.perm_keys_dev(nl), np.int64)
+    rk = fa.DeviceArray(dg.fk_keys_dev(nr, domain=nl), np.int64)
+    li = fa.DeviceArray(torch.arange(nl, dtype=torch.int64, device="cuda"), np.uint64)
+    ri = fa.DeviceArray(torch.arange(nr, dtype=torch.int64, device="cuda"), np.uint64)
+    hits = int((rk.t < nl).sum())
+
+    def outer():
+        return fa.outer_equi_join(rk, ri, lk, li, capacity=nr)
+
+    res = stage("outer_join", outer, lambda r: 8 * (nl + nr) + 16 * hits + 8 * (nr - hits),
+                ["ht_build", "ht_probe", "join_compact"],
+                lambda r: len(r[0]) == hits and len(r[2]) == nr - hits and r[3] and
+                bool(torch.equal(rk.t[r[0].t], lk.t[r[1].t])) and
+                bool((rk.t[r[2].t] >= nl).all()))
+    out["outer_join"].update({"left_rows": nr, "right_rows": nl, "pairs": hits,
+                              "left_only": nr - hits})
+    out["outer_join"]["rows_per_s"] = (nl + nr) / (out["outer_join"]["ms"] / 1e3)
+    del res
+    return out
+
+
 def bench_copy(log2n, steps=3):
     """A plain device copy of 2^log2n int64 (reads and writes 8 B each): the
     reference point for the HBM byte counters and the achievable bandwidth."""
@@ -282,7 +379,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the side measurements")
     ap.add_argument("--leg", default="all",
-                    choices=["all", "groupby", "radix", "batch", "join", "learned", "copy"],
+                    choices=["all", "groupby", "radix", "batch", "join", "learned", "copy",
+                             "relational"],
                     help="run one measurement alone (for profiling); 'copy' is a plain "
                          "device copy of a known size, the counter calibration")
     args = ap.parse_args()
@@ -301,6 +399,7 @@ def main():
                 "batch": lambda: bench_batch(fa),
                 "join": lambda: bench_join(fa),
                 "learned": lambda: bench_learned(fa, args.rows),
+                "relational": lambda: bench_relational(fa, args.rows),
                 "copy": lambda: bench_copy(args.rows)}
         print(f"# leg {args.leg}", file=sys.stderr, flush=True)
         print(json.dumps({"leg": args.leg, **legs[args.leg]()}))
@@ -378,6 +477,9 @@ def main():
         torch.cuda.empty_cache()
         print("# learned vs plain sort", file=sys.stderr, flush=True)
         result["extra"]["learned_sort_2^28"] = bench_learned(fa, 28)
+        torch.cuda.empty_cache()
+        print("# filter / extract / sort / group_by2 / outer join", file=sys.stderr, flush=True)
+        result["extra"]["relational_2^28"] = bench_relational(fa, 28)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         print("# cpu baseline (oracle, one core)", file=sys.stderr, flush=True)

@@ -13,7 +13,11 @@ saved and loaded as one flat JSON object (fhip_autotune_save / _load):
   off); ``groupby.hot_min_pct``: the sample share the heavy keys must cover;
 - ``radix.shape_k`` / ``radix.shape_kv``: the radix sort pass's tile shape
   (threads x keys per thread, an index into RADIX_SHAPES) for key-only and
-  key/value sorts.
+  key/value sorts;
+- ``compact.shape``: the tile of the one-pass stream compaction (filter,
+  set_separate, join outputs), an index into COMPACT_SHAPES;
+- ``multi_group_by.composite``: 0 switches off the composite-key path of
+  the multi-column group_by.
 
 ``tune_groupby`` and ``tune_radix`` measure candidate settings on the device for a given key
 column and aggregates, keeps the fastest whose output equals the default
@@ -28,7 +32,7 @@ import time
 from . import check, lib
 
 __all__ = ["tune_set", "tune_get", "save", "load", "tune_groupby", "tune_radix", "DEFAULT_CACHE",
-           "RADIX_SHAPES"]
+           "RADIX_SHAPES", "COMPACT_SHAPES"]
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "frovedis_amd", "tune.json")
 
@@ -42,6 +46,8 @@ GROUPBY_SPACE = {
 
 RADIX_SHAPES = ((256, 16), (256, 32), (512, 16), (512, 24), (512, 12), (1024, 8), (1024, 16),
                 (768, 16))
+
+COMPACT_SHAPES = ((256, 16), (1024, 16))
 
 
 def tune_set(name, value):

@@ -17,7 +17,14 @@
 //   - multi_group_by (dftable.cc:367-487 with one rank ->
 //     dfcolumn_groupby.cc:122-234): stable sort of the row index by every
 //     key column, last column first, the union of the run boundaries of all
-//     columns, and the group keys of every column.
+//     columns, and the group keys of every column.  When every key column
+//     is an integer column and their value ranges fit in 64 bits together,
+//     the columns are packed into one order-preserving composite key
+//     (column 0 in the high bits) and sorted once: a stable sort by the
+//     composite is the same permutation as the stable sorts column by
+//     column, and two rows differ in the composite exactly when they differ
+//     in some column, so the splits are the composite's run boundaries.
+//     This replaces the per-column random gathers with one coalesced pass.
 // Comparisons between a column of type T and a scalar or column of type U
 // use C++'s usual arithmetic conversions, as the reference's templated
 // predicates do.  Every kernel is a grid-stride loop over rows: these are
@@ -127,18 +134,131 @@ __global__ __launch_bounds__(RL_NT) void gather_kernel(const T* __restrict__ src
     dst[i] = src[idx[i]];
 }
 
-// Run boundary flags of one column over the current row order: flag[i] = 1
-// where col[idx[i]] != col[idx[i - 1]] (the comparison of set_separate,
+// Run boundary flags of one column already in the final row order: flag[i]
+// = 1 where v[i] != v[i - 1] (the comparison of set_separate,
 // core/set_operations.hpp:146-163).  `first` overwrites, later columns OR.
 template <class T>
-__global__ __launch_bounds__(RL_NT) void boundary_kernel(const T* __restrict__ col,
-                                                         const size_t* __restrict__ idx, size_t n,
+__global__ __launch_bounds__(RL_NT) void boundary_kernel(const T* __restrict__ v, size_t n,
                                                          uint8_t* __restrict__ flag, int first) {
   const size_t stride = size_t(gridDim.x) * RL_NT;
   for (size_t i = size_t(blockIdx.x) * RL_NT + threadIdx.x; i < n; i += stride) {
-    const bool b = i > 0 && col[idx[i]] != col[idx[i - 1]];
+    const bool b = i > 0 && v[i] != v[i - 1];
     if (first) flag[i] = b ? 1 : 0;
     else if (b) flag[i] = 1;
+  }
+}
+
+// ---- composite keys for integer key columns --------------------------------
+constexpr int MG_MAXC = 8;
+
+// Order-preserving map of an integer value into uint64 (signed values get
+// their sign bit flipped after sign extension).
+template <class T>
+__host__ __device__ __forceinline__ unsigned long long to_ord(T v) {
+  if constexpr (std::is_signed<T>::value)
+    return static_cast<unsigned long long>(static_cast<long long>(v)) ^ (1ull << 63);
+  else
+    return static_cast<unsigned long long>(v);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T from_ord(unsigned long long u) {
+  if constexpr (std::is_signed<T>::value)
+    return static_cast<T>(static_cast<long long>(u ^ (1ull << 63)));
+  else
+    return static_cast<T>(u);
+}
+
+struct MgCols {
+  const void* col[MG_MAXC];
+  int type[MG_MAXC];
+  unsigned long long lo[MG_MAXC];
+  int shift[MG_MAXC];
+  int ncols;
+};
+
+__device__ __forceinline__ unsigned long long load_ord(const void* p, int type, size_t j) {
+  switch (type) {
+    case FHIP_I32: return to_ord(static_cast<const int32_t*>(p)[j]);
+    case FHIP_U32: return to_ord(static_cast<const uint32_t*>(p)[j]);
+    case FHIP_I64: return to_ord(static_cast<const int64_t*>(p)[j]);
+    default: return to_ord(static_cast<const uint64_t*>(p)[j]);
+  }
+}
+
+// [min, max] of the ordered values col[idx[i]]; mm = {min, max}, preset to
+// {~0, 0}
+template <class T>
+__global__ __launch_bounds__(RL_NT) void range_kernel(const T* __restrict__ col,
+                                                      const size_t* __restrict__ idx, size_t n,
+                                                      unsigned long long* __restrict__ mm) {
+  unsigned long long lo = ~0ull, hi = 0;
+  const size_t stride = size_t(gridDim.x) * RL_NT;
+  for (size_t i = size_t(blockIdx.x) * RL_NT + threadIdx.x; i < n; i += stride) {
+    const unsigned long long u = to_ord(col[idx[i]]);
+    lo = u < lo ? u : lo;
+    hi = u > hi ? u : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  __shared__ unsigned long long s_lo[RL_NT / 64], s_hi[RL_NT / 64];
+  if (__lane_id() == 0) {
+    s_lo[threadIdx.x / 64] = lo;
+    s_hi[threadIdx.x / 64] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < RL_NT / 64; w++) {
+      lo = s_lo[w] < lo ? s_lo[w] : lo;
+      hi = s_hi[w] > hi ? s_hi[w] : hi;
+    }
+    atomicMin(mm, lo);
+    atomicMax(mm + 1, hi);
+  }
+}
+
+__global__ void range_init_kernel(unsigned long long* mm, int ncols) {
+  const int c = threadIdx.x;
+  if (c < ncols) {
+    mm[2 * c] = ~0ull;
+    mm[2 * c + 1] = 0;
+  }
+}
+
+// comp[i] = sum over columns of (ord(col_c[idx[i]]) - lo_c) << shift_c
+template <class K>
+__global__ __launch_bounds__(RL_NT) void composite_kernel(MgCols c, const size_t* __restrict__ idx,
+                                                          size_t n, K* __restrict__ comp) {
+  const size_t stride = size_t(gridDim.x) * RL_NT;
+  for (size_t i = size_t(blockIdx.x) * RL_NT + threadIdx.x; i < n; i += stride) {
+    const size_t j = idx[i];
+    unsigned long long key = 0;
+    for (int k = 0; k < c.ncols; k++)
+      key |= (load_ord(c.col[k], c.type[k], j) - c.lo[k]) << c.shift[k];
+    comp[i] = static_cast<K>(key);
+  }
+}
+
+template <class K>
+struct CompRunStart {
+  const K* k;
+  __device__ bool operator()(size_t i) const { return i == 0 || k[i] != k[i - 1]; }
+};
+
+// keys[g] of one column from the composite key at the group's first row
+template <class T, class K>
+__global__ __launch_bounds__(RL_NT) void decode_kernel(const K* __restrict__ comp,
+                                                       const size_t* __restrict__ split, size_t G,
+                                                       unsigned long long lo, int shift,
+                                                       unsigned long long mask,
+                                                       T* __restrict__ keys) {
+  const size_t stride = size_t(gridDim.x) * RL_NT;
+  for (size_t g = size_t(blockIdx.x) * RL_NT + threadIdx.x; g < G; g += stride) {
+    const unsigned long long u = (static_cast<unsigned long long>(comp[split[g]]) >> shift) & mask;
+    keys[g] = from_ord<T>(u + lo);
   }
 }
 
@@ -271,6 +391,86 @@ void sort_index(int ktype, const void* col, size_t col_n, const size_t* idx_in, 
   sort_pairs(ktype, sorted_out, sorted_out, 8, idx_out, idx_out, n, false, desc, st);
 }
 
+namespace {
+
+// The composite-key path; returns false (and does nothing visible) when a
+// key column is not an integer column or the ranges need more than 64 bits.
+bool multi_group_by_composite(int ncols, const int* ktypes, const void* const* cols, size_t* idx,
+                              size_t n, size_t* split, size_t* nsplit, void* const* keys_out,
+                              hipStream_t st) {
+  if (ncols > MG_MAXC) return false;
+  for (int c = 0; c < ncols; c++)
+    if (ktypes[c] == FHIP_F32 || ktypes[c] == FHIP_F64) return false;
+  DBuf mm(2 * sizeof(unsigned long long) * size_t(ncols));
+  unsigned long long* dmm = mm.as<unsigned long long>();
+  range_init_kernel<<<1, 64, 0, st>>>(dmm, ncols);
+  FHIP_CHECK(hipGetLastError());
+  {
+    ProfScope prof("multi_group_range", st);
+    for (int c = 0; c < ncols; c++) {
+      dispatch_type(ktypes[c], [&](auto z) {
+        using T = decltype(z);
+        if constexpr (std::is_integral<T>::value)
+          range_kernel<T><<<grid_for_n(n), RL_NT, 0, st>>>((const T*)cols[c], idx, n,
+                                                           dmm + 2 * c);
+      });
+      FHIP_CHECK(hipGetLastError());
+    }
+  }
+  unsigned long long hmm[2 * MG_MAXC];
+  FHIP_CHECK(hipMemcpyAsync(hmm, dmm, 2 * sizeof(unsigned long long) * size_t(ncols),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  int bits[MG_MAXC], total = 0;
+  for (int c = 0; c < ncols; c++) {
+    const unsigned long long span = hmm[2 * c + 1] - hmm[2 * c];
+    bits[c] = span ? 64 - __builtin_clzll(span) : 0;
+    total += bits[c];
+  }
+  if (total > 64) return false;
+  MgCols mc{};
+  mc.ncols = ncols;
+  int sh = 0;
+  for (int c = ncols - 1; c >= 0; c--) {
+    mc.col[c] = cols[c];
+    mc.type[c] = ktypes[c];
+    mc.lo[c] = hmm[2 * c];
+    mc.shift[c] = bits[c] ? sh : 0;
+    sh += bits[c];
+  }
+  auto run = [&](auto kz) {
+    using K = decltype(kz);
+    DBuf comp(n * sizeof(K));
+    {
+      ProfScope prof("multi_group_composite", st);
+      composite_kernel<K><<<grid_for_n(n), RL_NT, 0, st>>>(mc, idx, n, comp.as<K>());
+      FHIP_CHECK(hipGetLastError());
+    }
+    // one stable sort by the composite = the column-by-column stable sorts
+    sort_pairs_bits(int(sizeof(K)), comp.get(), comp.get(), 8, idx, idx, n, std::max(total, 1),
+                    st);
+    const size_t G = compact_indices(CompRunStart<K>{comp.as<K>()}, n, split, st);
+    put_end_kernel<<<1, 1, 0, st>>>(split, G, n);
+    FHIP_CHECK(hipGetLastError());
+    *nsplit = G + 1;
+    for (int c = 0; c < ncols; c++) {
+      const unsigned long long mask = bits[c] == 64 ? ~0ull : ((1ull << bits[c]) - 1);
+      dispatch_type(ktypes[c], [&](auto z) {
+        using T = decltype(z);
+        if constexpr (std::is_integral<T>::value)
+          decode_kernel<T, K><<<grid_for_n(G), RL_NT, 0, st>>>(comp.as<K>(), split, G, mc.lo[c],
+                                                               mc.shift[c], mask, (T*)keys_out[c]);
+      });
+      FHIP_CHECK(hipGetLastError());
+    }
+  };
+  if (total <= 32) run(uint32_t());
+  else run(uint64_t());
+  return true;
+}
+
+}  // namespace
+
 void multi_group_by(int ncols, const int* ktypes, const void* const* cols, size_t* idx, size_t n,
                     size_t* split, size_t* nsplit, void* const* keys_out, hipStream_t st) {
   FHIP_REQUIRE(ncols >= 1, FHIP_ERR_ARG, "multi_group_by: no key column");
@@ -279,6 +479,9 @@ void multi_group_by(int ncols, const int* ktypes, const void* const* cols, size_
     *nsplit = set_separate(ktypes[0], nullptr, 0, split, st);
     return;
   }
+  if (tune_get("multi_group_by.composite", 1) &&
+      multi_group_by_composite(ncols, ktypes, cols, idx, n, split, nsplit, keys_out, st))
+    return;
   int kbmax = 4;
   for (int c = 0; c < ncols; c++) kbmax = std::max(kbmax, type_bytes(ktypes[c]));
   DBuf tmp(n * size_t(kbmax)), flag(n);
@@ -293,14 +496,18 @@ void multi_group_by(int ncols, const int* ktypes, const void* const* cols, size_
     sort_pairs(ktypes[c], tmp.get(), tmp.get(), 8, idx, idx, n, false, false, st);
   }
   // union of the run boundaries of every column in the final order
-  // (set_separate per column, merged by set_union)
+  // (set_separate per column, merged by set_union); tmp holds column 0
+  // sorted, the other columns are gathered into it once each
   {
     ProfScope prof("multi_group_split", st);
     for (int c = 0; c < ncols; c++) {
       dispatch_type(ktypes[c], [&](auto z) {
         using T = decltype(z);
-        boundary_kernel<T><<<grid_for_n(n), RL_NT, 0, st>>>((const T*)cols[c], idx, n,
-                                                            flag.as<uint8_t>(), c == 0);
+        if (c > 0)
+          gather_kernel<T><<<grid_for_n(n), RL_NT, 0, st>>>((const T*)cols[c], idx, n,
+                                                            tmp.as<T>());
+        boundary_kernel<T><<<grid_for_n(n), RL_NT, 0, st>>>(tmp.as<T>(), n, flag.as<uint8_t>(),
+                                                            c == 0);
       });
       FHIP_CHECK(hipGetLastError());
     }

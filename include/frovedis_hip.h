@@ -78,7 +78,12 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *   "groupby.span"      4096-row tiles partitioned by one workgroup;
  *   "groupby.hot"       slots of the heavy-key table (0 switches the heavy-key
  *       pass off); "groupby.hot_min_pct" the sample share the heavy keys
- *       must cover for the pass to run.
+ *       must cover for the pass to run;
+ *   "compact.shape"     tile of the one-pass stream compaction behind filter,
+ *       set_separate and the join outputs (0 256x16, 1 1024x16 threads x
+ *       rows);
+ *   "multi_group_by.composite"  0 switches off the composite-key path of
+ *       fhip_dev_multi_group_by (the column-by-column sorts run instead).
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);

@@ -104,6 +104,27 @@ def test_filter_pair_matches_oracle(fa, at, bt):
         np.testing.assert_array_equal(got, oracle.filter_pair(a, b, op, an, bn), err_msg=op)
 
 
+@pytest.mark.parametrize("shape", [0, 1])
+def test_compaction_tile_shapes(fa, shape):
+    """Both tiles of the one-pass compaction, on partial last tiles and on
+    many tiles (the look-back crosses more than 64 of them)."""
+    from frovedis_amd import autotune
+    rng = np.random.default_rng(22)
+    autotune.tune_set("compact.shape", shape)
+    try:
+        for n in [1, 4095, 4097, 16385, 3 << 20]:
+            col = rng.integers(0, 100, n).astype(np.int32)
+            got = fa.filter(_d(fa, col), "lt", np.int32(37)).numpy()
+            np.testing.assert_array_equal(got, np.flatnonzero(col < 37))
+            none = fa.filter(_d(fa, col), "gt", np.int32(1000)).numpy()
+            assert len(none) == 0
+            ks = np.sort(rng.integers(0, max(n // 3, 1), n)).astype(np.int64)
+            np.testing.assert_array_equal(fa.set_separate(_d(fa, ks)).numpy(),
+                                          oracle.set_separate(ks))
+    finally:
+        autotune.tune_set("compact.shape", None)
+
+
 def test_filter_tutorial_and_errors(fa):
     c1 = _d(fa, np.arange(1, 9, dtype=np.int32))
     c2 = _d(fa, np.arange(10, 90, 10).astype(np.float64))
@@ -165,9 +186,20 @@ def test_sort_large_properties(fa):
 
 # ---- multi-column group_by ---------------------------------------------------------
 
+@pytest.fixture(params=["composite", "column_sorts"])
+def mg_path(fa, request):
+    """Runs a test with the composite-key path allowed (the default) and
+    with it switched off (the column-by-column sorts)."""
+    from frovedis_amd import autotune
+    autotune.tune_set("multi_group_by.composite", 1 if request.param == "composite" else 0)
+    yield request.param
+    autotune.tune_set("multi_group_by.composite", None)
+
+
 @pytest.mark.parametrize("types", [(np.int64, np.int32), (np.uint32, np.float64, np.int64),
-                                   (np.float32,), (np.int32, np.int32, np.uint64, np.int64)])
-def test_multi_group_by_matches_oracle(fa, types):
+                                   (np.float32,), (np.int32, np.int32, np.uint64, np.int64),
+                                   (np.uint64, np.int32, np.uint32)])
+def test_multi_group_by_matches_oracle(fa, mg_path, types):
     rng = np.random.default_rng(16)
     for n in [0, 1, 40, 5000, 400000]:
         cols = [_col(rng, t, n, -6, 6) for t in types]
@@ -182,6 +214,40 @@ def test_multi_group_by_matches_oracle(fa, types):
             for op in ["sum", "count", "max", "min", "avg"]:
                 got = fa.grouped_agg(op, _d(fa, v), gi, split).numpy()
                 np.testing.assert_array_equal(got, oracle.grouped_agg(op, v, rgi, rsplit))
+
+
+@pytest.mark.parametrize("case", ["full64", "int32_uint32", "over64", "constant"])
+def test_multi_group_by_key_ranges(fa, mg_path, case):
+    """Ranges that fill the 64-bit composite exactly, overflow it (the
+    column sorts run instead) or collapse to one value."""
+    rng = np.random.default_rng(21)
+    n = 100000
+    if case == "full64":
+        a = rng.integers(np.iinfo(np.int64).min, np.iinfo(np.int64).max, n, dtype=np.int64)
+        a[:3] = [np.iinfo(np.int64).min, np.iinfo(np.int64).max, 0]
+        a[3:1000] = a[1000:1997]  # repeated keys
+        cols = [a]
+    elif case == "int32_uint32":
+        a = rng.integers(-2**31, 2**31, n).astype(np.int32)
+        b = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+        a[:2], b[:2] = [-2**31, 2**31 - 1], [0, 2**32 - 1]
+        a[2:5000] = a[5000:9998]
+        b[2:5000] = b[5000:9998]
+        cols = [a, b]
+    elif case == "over64":
+        a = rng.integers(0, 1 << 40, n).astype(np.uint64)
+        b = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+        a[:4000] = a[4000:8000]
+        b[:4000] = b[4000:8000]
+        cols = [a, b]
+    else:
+        cols = [np.full(n, -7, np.int64), np.full(n, 3, np.uint32)]
+    gi, split, keys = fa.multi_group_by([_d(fa, c) for c in cols])
+    rgi, rsplit, rkeys = oracle.multi_group_by(cols)
+    np.testing.assert_array_equal(gi.numpy(), rgi)
+    np.testing.assert_array_equal(split.numpy(), rsplit)
+    for k, rk in zip(keys, rkeys):
+        np.testing.assert_array_equal(k.numpy(), rk)
 
 
 def test_multi_group_by_subset_and_tutorial(fa):
