@@ -87,8 +87,11 @@ __global__ __launch_bounds__(256) void rs_stats(const K* __restrict__ keys, size
 
 // ---- all-digit histograms in one read -------------------------------------
 // Every thread loads RH_ITEMS keys per step (independent loads in flight),
-// then counts each byte position into LDS; a wave whose keys share a digit
-// adds them with one atomic.
+// then counts each byte position into LDS; the lanes of a wave that share
+// lane 0's digit, and those that share the digit of the first lane with
+// another digit, add them with one atomic per digit (skewed keys put most
+// of a wave on one or two bins, and same-address atomics of one wave
+// serialise).
 constexpr int RH_ITEMS = 8;
 
 template <class U>
@@ -115,13 +118,21 @@ __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, siz
       const bool valid = w0 + lane < n;
       const unsigned long long vmask = __ballot(valid);
       for (int b = 0; b < nbytes; b++) {
+        // lanes holding lane 0's digit, and then those holding the digit of
+        // the first other lane, are each counted with one atomic (skewed
+        // keys put most of a wave on one or two bins: same-address atomics
+        // from one wave serialise); the rest add one each
         unsigned d = key_digit(k[j], mode, 8 * b, desc);
         unsigned d0 = __shfl(d, 0, 64);  // lane 0 is always valid here
         unsigned long long same = __ballot(valid && d == d0);
-        if (same == vmask) {
-          if (lane == 0) atomicAdd(&h[b * 256 + d0], unsigned(__popcll(vmask)));
-        } else if (valid) {
-          atomicAdd(&h[b * 256 + d], 1u);
+        if (lane == 0) atomicAdd(&h[b * 256 + d0], unsigned(__popcll(same)));
+        const unsigned long long rest = vmask & ~same;
+        if (rest) {
+          const int l1 = __ffsll((long long)rest) - 1;
+          const unsigned d1 = __shfl(d, l1, 64);
+          const unsigned long long same1 = __ballot(valid && d == d1);
+          if (int(lane) == l1) atomicAdd(&h[b * 256 + d1], unsigned(__popcll(same1)));
+          if (valid && d != d0 && d != d1) atomicAdd(&h[b * 256 + d], 1u);
         }
       }
     }
