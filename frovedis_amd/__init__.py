@@ -53,3 +53,4 @@ from .core import *  # noqa: E402,F401,F403
 from .dataframe import *  # noqa: E402,F401,F403
 from .learned import *  # noqa: E402,F401,F403
 from .batch import *  # noqa: E402,F401,F403
+from .dftable import *  # noqa: E402,F401,F403

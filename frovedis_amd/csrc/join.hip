@@ -14,6 +14,8 @@
 #include "hashtable.hpp"
 #include "radix_sort.hpp"
 
+#include <vector>
+
 namespace fhip {
 
 void prefix_sum(int type, const void* in, void* out, size_t n, hipStream_t st);
@@ -228,6 +230,35 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
   return size_t(total);
 }
 
+// ---- star join ---------------------------------------------------------------
+constexpr int SJ_MAXD = 8;
+
+template <class K>
+__global__ __launch_bounds__(256) void sj_gather_kernel(const K* __restrict__ col,
+                                                        const size_t* __restrict__ idx, size_t n,
+                                                        K* __restrict__ out) {
+  const size_t stride = size_t(gridDim.x) * 256;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) out[i] = col[idx[i]];
+}
+
+__global__ __launch_bounds__(256) void sj_and_kernel(uint8_t* __restrict__ acc,
+                                                     const uint8_t* __restrict__ f, size_t n) {
+  const size_t stride = size_t(gridDim.x) * 256;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) acc[i] &= f[i];
+}
+
+struct StarWriter {
+  const size_t* lidx;  // null: the left rows are 0..nl-1
+  size_t* lo;
+  const uint64_t* vals[SJ_MAXD];
+  size_t* ro[SJ_MAXD];
+  int nd;
+  __device__ void operator()(size_t pos, size_t i) const {
+    lo[pos] = lidx ? lidx[i] : i;
+    for (int d = 0; d < nd; d++) ro[d][pos] = vals[d][i];
+  }
+};
+
 }  // namespace
 
 // Inner join, or LEFT outer join when only_left is given (outer_equi_join,
@@ -328,5 +359,88 @@ extern "C" int fhip_dev_outer_equi_join(int ktype, const void* left, const size_
     FHIP_REQUIRE(r <= capacity, FHIP_ERR_TOO_LARGE,
                  "outer_equi_join: output capacity " + std::to_string(capacity) + " < pairs " +
                      std::to_string(r));
+  });
+}
+
+namespace fhip {
+
+// star_join on one rank (dftable.cc:295-333 -> star_join_eq,
+// dfcolumn_join.cc:561-605 -> unique_equi_join2, join.hpp:103-117): for
+// every dimension d a unique_hashtable over its keys with the right row ids
+// as values ("star_join: right column is not unique" otherwise) is probed
+// with the left keys; the left rows found in every dimension survive, in
+// left order, with the looked-up right ids.  The reference probes only the
+// survivors of the earlier dimensions and shrinks the outputs after each
+// (shrink_missed_inplace), which keeps the same rows in the same order with
+// the same looked-up ids.
+size_t star_join(int nd, const int* ktypes, const void* const* left_cols, const size_t* lidx,
+                 size_t nl, const void* const* right_cols, const size_t* const* ridx,
+                 const size_t* nr, size_t* lo, size_t* const* ro, hipStream_t st) {
+  FHIP_REQUIRE(nd >= 1 && nd <= SJ_MAXD, FHIP_ERR_ARG,
+               "star_join: 1 to " + std::to_string(SJ_MAXD) + " dimension tables");
+  for (int d = 0; d < nd; d++) type_bytes(ktypes[d]);
+  if (nl == 0) return 0;
+  std::vector<DBuf> vals(nd);
+  DBuf found(nl), fd(nl), keys;
+  StarWriter w{};
+  w.lidx = lidx;
+  w.lo = lo;
+  w.nd = nd;
+  for (int d = 0; d < nd; d++) {
+    int uniq = 1;
+    fhip_ht* h = ht_build(ktypes[d], right_cols[d], reinterpret_cast<const uint64_t*>(ridx[d]),
+                          nr[d], true, &uniq, st);
+    try {
+      FHIP_REQUIRE(uniq, FHIP_ERR_ARG, "star_join: right column is not unique");
+      const void* q = left_cols[d];
+      if (lidx) {
+        keys.reset(nl * size_t(type_bytes(ktypes[d])));
+        dispatch_type(ktypes[d], [&](auto z) {
+          using K = decltype(z);
+          sj_gather_kernel<K><<<grid_for_n(nl), 256, 0, st>>>((const K*)left_cols[d], lidx, nl,
+                                                              keys.as<K>());
+        });
+        FHIP_CHECK(hipGetLastError());
+        q = keys.get();
+      }
+      vals[d].reset(nl * sizeof(uint64_t));
+      uint8_t* f = d == 0 ? found.as<uint8_t>() : fd.as<uint8_t>();
+      ht_probe(h, q, nl, vals[d].as<uint64_t>(), f, st);
+      if (d > 0) {
+        sj_and_kernel<<<grid_for_n(nl), 256, 0, st>>>(found.as<uint8_t>(), f, nl);
+        FHIP_CHECK(hipGetLastError());
+      }
+    } catch (...) {
+      ht_destroy(h);
+      throw;
+    }
+    ht_destroy(h);
+    w.vals[d] = vals[d].as<uint64_t>();
+    w.ro[d] = ro[d];
+  }
+  ProfScope prof("join_compact", st);
+  return compact_apply(HitPred{found.as<uint8_t>()}, w, nl, nl, st);
+}
+
+}  // namespace fhip
+
+extern "C" int fhip_dev_star_join(int ndims, const int* ktypes, const void* const* left_cols,
+                                  const size_t* left_idx, size_t nl,
+                                  const void* const* right_cols,
+                                  const size_t* const* right_idx, const size_t* nr,
+                                  size_t* left_out, size_t* const* right_out, size_t* nout,
+                                  void* stream) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(nout && ktypes && left_cols && right_cols && right_idx && nr && right_out,
+                 FHIP_ERR_ARG, "star_join: null pointer");
+    FHIP_REQUIRE(nl == 0 || left_out, FHIP_ERR_ARG, "star_join: null output");
+    for (int d = 0; d < ndims; d++) {
+      FHIP_REQUIRE(nl == 0 || (left_cols[d] && right_out[d]), FHIP_ERR_ARG,
+                   "star_join: null left column or output");
+      FHIP_REQUIRE(nr[d] == 0 || (right_cols[d] && right_idx[d]), FHIP_ERR_ARG,
+                   "star_join: null right column");
+    }
+    *nout = fhip::star_join(ndims, ktypes, left_cols, left_idx, nl, right_cols, right_idx, nr,
+                            left_out, right_out, fhip::as_stream(stream));
   });
 }

@@ -12,7 +12,7 @@ from . import TYPE_CODE, FhipError, check, lib
 from .core import DeviceArray, _sz, current_stream
 
 __all__ = ["group_by", "grouped_agg", "group_by_agg", "AGG_OPS", "UniqueHashtable",
-           "equi_join", "outer_equi_join", "JOIN_STRATEGY", "NULL_INDEX", "global_extract",
+           "equi_join", "outer_equi_join", "star_join", "JOIN_STRATEGY", "NULL_INDEX", "global_extract",
            "extract_nulls", "filter", "CMP_OPS", "sort", "sort_desc", "multi_group_by"]
 
 AGG_OPS = {"sum": 0, "count": 1, "avg": 2, "max": 3, "min": 4}
@@ -343,3 +343,35 @@ def multi_group_by(cols, local_idx=None, stream=None):
                                         ctypes.byref(ns), kp, stream or current_stream()))
     G = ns.value - 1
     return local_idx, split[: ns.value], [x[:G] for x in keys]
+
+
+def star_join(left_cols, rights, left_idx=None, stream=None):
+    """star_join on one rank (dftable.cc:295-333): left_cols[d] is joined
+    to dimension d, rights[d] = (right key column, right ids) with unique
+    keys.  Returns (left_rows, [right ids per dimension]) for the left rows
+    found in every dimension, in left order."""
+    nd = len(left_cols)
+    if nd == 0 or nd != len(rights):
+        raise FhipError("star_join: one right table per left key column")
+    for lc, (rc, ri) in zip(left_cols, rights):
+        if lc.dtype != rc.dtype:
+            raise FhipError("star_join_eq: column types are different")
+        if len(rc) != len(ri):
+            raise FhipError("star_join: right keys and ids of different sizes")
+    dev = left_cols[0].t.device
+    nl = len(left_cols[0]) if left_idx is None else len(left_idx)
+    if left_idx is not None and nl and int(left_idx.t.max()) >= min(len(c) for c in left_cols):
+        raise FhipError("star_join: left index points beyond a column")
+    lo = DeviceArray.empty(max(nl, 1), np.uint64, dev)
+    ros = [DeviceArray.empty(max(nl, 1), np.uint64, dev) for _ in range(nd)]
+    P = ctypes.c_void_p
+    nout = ctypes.c_size_t(0)
+    check(lib().fhip_dev_star_join(
+        nd, (ctypes.c_int * nd)(*[c.code for c in left_cols]),
+        (P * nd)(*[c.ptr.value for c in left_cols]),
+        left_idx.ptr if left_idx is not None else None, _sz(nl),
+        (P * nd)(*[r.ptr.value for r, _ in rights]), (P * nd)(*[i.ptr.value for _, i in rights]),
+        (ctypes.c_size_t * nd)(*[len(r) for r, _ in rights]), lo.ptr,
+        (P * nd)(*[r.ptr.value for r in ros]), ctypes.byref(nout), stream or current_stream()))
+    m = nout.value
+    return lo[:m], [r[:m] for r in ros]

@@ -237,6 +237,22 @@ int fhip_dev_outer_equi_join(int ktype, const void* left,
                              size_t* only_left, size_t* n_only_left,
                              int* used_hash, void* stream);
 
+/* star_join on one rank (dftable.cc:295-333 -> star_join_eq,
+ * dfcolumn_join.cc:561-605 -> unique_equi_join2, join.hpp:103-117): the
+ * left table (rows left_idx, or 0..nl-1 when NULL) is joined to ndims
+ * dimension tables at once, dimension d on left_cols[d] = right_cols[d]
+ * (type ktypes[d], nr[d] right rows with ids right_idx[d]).  Every right
+ * column must hold unique keys ("star_join: right column is not unique",
+ * FHIP_ERR_ARG).  The left rows found in every dimension come out in left
+ * order: left_out (nl entries of capacity) and right_out[d] (nl entries
+ * each); *nout receives the row count.  At most 8 dimensions. */
+int fhip_dev_star_join(int ndims, const int* ktypes,
+                       const void* const* left_cols, const size_t* left_idx,
+                       size_t nl, const void* const* right_cols,
+                       const size_t* const* right_idx, const size_t* nr,
+                       size_t* left_out, size_t* const* right_out,
+                       size_t* nout, void* stream);
+
 /* ---- materialisation, filter and sort of dftable columns -------------------
  * FHIP_NULL_INDEX in a row index selects NULL (an outer join's missing
  * right row).  NULL values are numeric_limits<T>::max(), the reference's

@@ -77,6 +77,8 @@ def lib():
         L.orc_sort_index.argtypes = [i, vp, vp, sz, i, vp, vp]
         L.orc_multi_group_by.argtypes = [i, vp, vp, vp, sz, vp, vp]
         L.orc_multi_group_by.restype = sz
+        L.orc_star_join.argtypes = [i, vp, vp, vp, sz, vp, vp, vp, vp, vp]
+        L.orc_star_join.restype = sz
         L.orc_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -341,3 +343,27 @@ def multi_group_by(cols, idx=None):
     kp = (ctypes.c_void_p * len(cs))(*[k.ctypes.data for k in keys])
     m = lib().orc_multi_group_by(len(cs), kt, cp, _p(gi), n, _p(split), kp)
     return gi, split[:m].copy(), [k[:max(m - 1, 0)].copy() for k in keys]
+
+
+def star_join(left_cols, rights, left_idx=None):
+    """(left_rows, [right ids per dimension]): star_join on one rank.
+    rights: list of (right key column, right ids)."""
+    lcs = [np.ascontiguousarray(c) for c in left_cols]
+    rcs = [np.ascontiguousarray(r, dtype=l.dtype) for l, (r, _) in zip(lcs, rights)]
+    ris = [np.ascontiguousarray(i, dtype=np.uint64) for _, i in rights]
+    nd = len(lcs)
+    li = None if left_idx is None else np.ascontiguousarray(left_idx, dtype=np.uint64)
+    nl = len(lcs[0]) if li is None else len(li)
+    lo = np.empty(max(nl, 1), np.uint64)
+    ros = [np.empty(max(nl, 1), np.uint64) for _ in range(nd)]
+    P = ctypes.c_void_p
+    m = lib().orc_star_join(nd, (ctypes.c_int * nd)(*[TYPE_CODE[c.dtype] for c in lcs]),
+                            (P * nd)(*[c.ctypes.data for c in lcs]),
+                            _p(li) if li is not None else None, nl,
+                            (P * nd)(*[r.ctypes.data if len(r) else None for r in rcs]),
+                            (P * nd)(*[r.ctypes.data if len(r) else None for r in ris]),
+                            (ctypes.c_size_t * nd)(*[len(r) for r in rcs]), _p(lo),
+                            (P * nd)(*[r.ctypes.data for r in ros]))
+    if m == 2**64 - 1:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return lo[:m].copy(), [r[:m].copy() for r in ros]

@@ -570,6 +570,45 @@ void dispatch_key(int t, F&& f) {
 }
 
 // ---------------------------------------------------------------------------
+// star_join on one rank: dftable.cc:295-333.  ret_left_idx starts as the
+// left rows; for every dimension in order, unique_equi_join2
+// (join.hpp:103-117: unique_hashtable on the right keys with the right ids,
+// "star_join: right column is not unique" otherwise, lookup of the left keys
+// at ret_left_idx) gives the right ids and the missed positions, and
+// shrink_missed_inplace removes the missed rows from ret_left_idx, this
+// dimension's right ids and every earlier dimension's.
+// Pinned by: brute force (the left rows whose key is present in every
+// dimension, in left order, with the position of that key) (tests).
+// ---------------------------------------------------------------------------
+template <class K>
+void star_dim_t(const void* left_col, const void* right_col, const size_t* ridx, size_t nr,
+                std::vector<size_t>& lrows, std::vector<std::vector<size_t>>& rids) {
+  const K* l = reinterpret_cast<const K*>(left_col);
+  HT h;
+  h.kbytes = sizeof(K);
+  int uniq = 1;
+  std::vector<uint64_t> rv(ridx, ridx + nr);
+  ht_build_t<K>(&h, right_col, rv.data(), nr, 1, &uniq);
+  if (!uniq) throw std::runtime_error("star_join: right column is not unique");
+  std::vector<K> q(lrows.size());
+  for (size_t i = 0; i < lrows.size(); i++) q[i] = l[lrows[i]];
+  std::vector<uint64_t> got(q.size());
+  std::vector<size_t> miss(q.size());
+  size_t nm = ht_lookup_t<K>(&h, q.data(), q.size(), got.data(), miss.data());
+  rids.emplace_back(got.begin(), got.end());
+  // shrink_missed_inplace on every output so far
+  std::vector<char> drop(lrows.size(), 0);
+  for (size_t t = 0; t < nm; t++) drop[miss[t]] = 1;
+  auto shrink = [&](std::vector<size_t>& v) {
+    size_t c = 0;
+    for (size_t i = 0; i < v.size(); i++) if (!drop[i]) v[c++] = v[i];
+    v.resize(c);
+  };
+  shrink(lrows);
+  for (auto& r : rids) shrink(r);
+}
+
+// ---------------------------------------------------------------------------
 // Materialisation of joined / filtered tables on one rank:
 //  gather: global_extract (dataframe/dfcolumn_misc.cc:714-761) is
 //    ret[i] = val[idx[i]] once the exchange is an identity; append_nulls
@@ -997,6 +1036,28 @@ size_t orc_multi_group_by(int ncols, const int* ktypes, const void* const* cols,
   try {
     return multi_group_by_t(ncols, ktypes, cols, idx, n, split, keys_out);
   } catch (std::exception& e) { g_err = e.what(); return 0; }
+}
+
+// star join: returns the row count (or (size_t)-1 with the message in
+// orc_last_error); left_out and right_out[d] hold nl entries.
+size_t orc_star_join(int nd, const int* ktypes, const void* const* left_cols,
+                     const size_t* lidx, size_t nl, const void* const* right_cols,
+                     const size_t* const* ridx, const size_t* nr, size_t* left_out,
+                     size_t* const* right_out) {
+  try {
+    std::vector<size_t> lrows(nl);
+    for (size_t i = 0; i < nl; i++) lrows[i] = lidx ? lidx[i] : i;
+    std::vector<std::vector<size_t>> rids;
+    for (int d = 0; d < nd; d++) {
+      dispatch_key(ktypes[d], [&](auto z) {
+        using K = decltype(z);
+        star_dim_t<K>(left_cols[d], right_cols[d], ridx[d], nr[d], lrows, rids);
+      });
+    }
+    std::copy(lrows.begin(), lrows.end(), left_out);
+    for (int d = 0; d < nd; d++) std::copy(rids[d].begin(), rids[d].end(), right_out[d]);
+    return lrows.size();
+  } catch (std::exception& e) { g_err = e.what(); return ~size_t(0); }
 }
 
 }  // extern "C"

@@ -344,3 +344,36 @@ def test_outer_join_large_conservation(fa):
     assert len(lo) == int(hit.sum()) and len(only) == nl - int(hit.sum())
     np.testing.assert_array_equal(only.numpy(), np.flatnonzero(~hit))
     assert bool(torch.equal(dl.t[lo.t], dr.t[ro.t]))
+
+
+# ---- star join ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("with_idx", [False, True])
+def test_star_join_matches_oracle(fa, with_idx):
+    rng = np.random.default_rng(23)
+    for nl in [0, 1, 3000, 500000]:
+        dims, left = [], []
+        for nr, t in [(1000, np.int64), (77, np.int32), (20000, np.uint64), (5, np.uint32)]:
+            keys = rng.choice(3 * nr, nr, replace=False).astype(t)
+            ids = rng.permutation(nr).astype(np.uint64)
+            dims.append((keys, ids))
+            left.append(rng.integers(0, 3 * nr, nl).astype(t))
+        lidx = (np.sort(rng.choice(nl, nl // 2, replace=False)).astype(np.uint64)
+                if with_idx else None)
+        rlo, rros = oracle.star_join(left, dims, lidx)
+        lo, ros = fa.star_join([_d(fa, c) for c in left],
+                               [(_d(fa, k), _d(fa, i)) for k, i in dims],
+                               left_idx=_d(fa, lidx) if with_idx else None)
+        np.testing.assert_array_equal(lo.numpy(), rlo, err_msg=f"nl={nl}")
+        for r, rr in zip(ros, rros):
+            np.testing.assert_array_equal(r.numpy(), rr)
+
+
+def test_star_join_errors_and_empty_dimension(fa):
+    left = _d(fa, np.arange(10, dtype=np.int64))
+    with pytest.raises(fa.FhipError, match="not unique"):
+        fa.star_join([left], [(_d(fa, np.array([1, 1], np.int64)),
+                               _d(fa, np.arange(2, dtype=np.uint64)))])
+    lo, ros = fa.star_join([left], [(_d(fa, np.zeros(0, np.int64)),
+                                     _d(fa, np.zeros(0, np.uint64)))])
+    assert len(lo) == 0 and len(ros[0]) == 0

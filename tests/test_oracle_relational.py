@@ -1,5 +1,6 @@
 """Pins the oracle's column operations (gather with NULLs, filter, sort,
-multi-column group_by, outer join) before the GPU is checked against them.
+multi-column group_by, outer and star joins) before the GPU is checked
+against them.
 
 The checks come from outside the oracle's own code: numpy take / isin /
 stable argsort, comparisons evaluated in the common type C++ picks for the
@@ -295,3 +296,35 @@ def test_outer_join_empty_sides():
         assert sorted(only.tolist()) == [0, 1, 2]
         lo, ro, only, _ = oracle.outer_equi_join(e, np.zeros(0), l, np.arange(3), algo)
         assert len(lo) == 0 and len(only) == 0
+
+
+# ---- star join -------------------------------------------------------------------
+
+def test_star_join_brute_force():
+    rng = np.random.default_rng(10)
+    nl = 2000
+    dims = []
+    left = []
+    for nr, t in [(300, np.int64), (50, np.int32), (700, np.uint64)]:
+        keys = rng.choice(4 * nr, nr, replace=False).astype(t)
+        ids = rng.permutation(nr).astype(np.uint64) + np.uint64(100)
+        dims.append((keys, ids))
+        left.append(rng.integers(0, 4 * nr, nl).astype(t))
+    lidx = np.sort(rng.choice(nl, 1500, replace=False)).astype(np.uint64)
+    lo, ros = oracle.star_join(left, dims, lidx)
+    want_rows, want_ids = [], [[] for _ in dims]
+    for i in lidx.tolist():
+        pos = [np.flatnonzero(k == c[i]) for c, (k, _) in zip(left, dims)]
+        if all(len(p) for p in pos):
+            want_rows.append(i)
+            for d, p in enumerate(pos):
+                want_ids[d].append(int(dims[d][1][p[0]]))
+    np.testing.assert_array_equal(lo, want_rows)
+    for d in range(len(dims)):
+        np.testing.assert_array_equal(ros[d], want_ids[d])
+
+
+def test_star_join_requires_unique_right():
+    with pytest.raises(RuntimeError, match="not unique"):
+        oracle.star_join([np.arange(5, dtype=np.int64)],
+                         [(np.array([1, 2, 2], np.int64), np.arange(3))])
