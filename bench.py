@@ -283,6 +283,9 @@ def bench_relational(fa, log2n=28, steps=3):
                   written), its key/value radix sort (4 B/row histogram, 4
                   passes of 24 B/row) and the run boundaries (2 x 4 B/row):
                   160 B/row;
+      star_join   star join of the table on (key, iv) to a dimension of 2^20
+                  unique keys and one of 900 int32 keys: 12 B/row read, 24 B
+                  per surviving row written, 16 B per dimension row built;
       outer_join  outer_hash_join of the 2^29 side to the unique 2^27 side:
                   8 B per input row + 16 B per pair + 8 B per left-only row."""
     import numpy as np
@@ -326,7 +329,21 @@ def bench_relational(fa, log2n=28, steps=3):
               ["multi_group_range", "multi_group_composite", "radix_hist", "radix_pass",
                "multi_group_split"])
     out["group_by2"]["groups"] = len(g[1]) - 1
-    del g, k, v, iv
+    del g
+    # star join of the table to two dimensions on (key, iv): 2^20 unique
+    # keys, and 900 of the 1000 iv values
+    d1 = fa.DeviceArray(dg.perm_keys_dev(1 << 20), np.int64)
+    d1i = fa.DeviceArray(torch.arange(1 << 20, dtype=torch.int64, device="cuda"), np.uint64)
+    d2 = fa.DeviceArray(torch.arange(900, dtype=torch.int32, device="cuda").flip(0), np.int32)
+    d2i = fa.DeviceArray(torch.arange(900, dtype=torch.int64, device="cuda"), np.uint64)
+    want = int(((k.t < (1 << 20)) & (iv.t < 900)).sum())
+    stage("star_join", lambda: fa.star_join([k, iv], [(d1, d1i), (d2, d2i)]),
+          lambda r: 12 * n + 24 * len(r[0]) + 16 * ((1 << 20) + 900),
+          ["ht_build", "ht_probe", "join_compact"],
+          lambda r: len(r[0]) == want and
+          bool(torch.equal(d1.t[r[1][0].t], k.t[r[0].t])) and
+          bool(torch.equal(d2.t[r[1][1].t], iv.t[r[0].t])))
+    del k, v, iv, d1, d1i, d2, d2i
     torch.cuda.empty_cache()
 
     nl, nr = 1 << 27, 1 << 29

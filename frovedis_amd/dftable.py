@@ -5,8 +5,9 @@ dftable.hpp:32-260) for numeric columns, with every column held in HBM
 between calls, so a chain of operations moves no data over PCIe:
 
 - ``filter`` / ``sort`` / ``sort_desc`` return views that keep the base
-  columns and a row index, as the reference's filtered_dftable and
-  sorted_dftable do (late materialization, dftable.hpp:552-760);
+  columns and a row index, as the reference's filtered_dftable
+  (dfoperator.hpp:2443) and sorted_dftable (dftable.hpp:784-855) do (late
+  materialization);
 - ``group_by(cols).select(cols, aggs)``, ``hash_join``, ``outer_hash_join``
   and ``star_join`` produce materialised tables (dftable.cc:164-333,
   :346-487, :3724-3772);
@@ -14,9 +15,9 @@ between calls, so a chain of operations moves no data over PCIe:
   (global_extract on one rank).
 
 NULL is numeric_limits<T>::max() with a per-column list of null row
-positions, the reference's representation (dfcolumn.hpp:262-300).  Every
-step calls one of the library's entry points; nothing is computed in
-Python.
+positions, the reference's representation (typed_dfcolumn<T>::val and
+nulls, dfcolumn.hpp:1012).  Every step calls one of the library's entry
+points; nothing is computed in Python.
 """
 import numpy as np
 
