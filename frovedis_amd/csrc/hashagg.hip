@@ -644,10 +644,17 @@ constexpr int HOT_UNROLL = 4;
 constexpr int HOT_STEP = HOT_NT * HOT_UNROLL;
 constexpr size_t HOT_CHUNK = size_t(1) << 18;
 
-__device__ __forceinline__ int hot_slot(const unsigned long long* hk, int Sh,
+// Home slot of a key in the heavy-key table: Fibonacci multiply-shift (one
+// 64-bit multiply; the table is at most half full, so the linear probe stays
+// short), `shift` = 64 - log2(Sh).
+__host__ __device__ __forceinline__ int hot_home(unsigned long long kb, int shift) {
+  return int((kb * 0x9E3779B97F4A7C15ull) >> shift);
+}
+
+__device__ __forceinline__ int hot_slot(const unsigned long long* hk, int Sh, int shift,
                                         unsigned long long kb) {
   if (kb == HA_EMPTY) return -1;
-  int s = int(mix64(kb) & (unsigned long long)(Sh - 1));
+  int s = hot_home(kb, shift);
   for (int p = 0; p < Sh; p++) {
     const unsigned long long c = hk[s];
     if (c == kb) return s;
@@ -657,7 +664,9 @@ __device__ __forceinline__ int hot_slot(const unsigned long long* hk, int Sh,
   return -1;
 }
 
-template <class K>
+// NA = accs.n, 0 to 4 (a template parameter, so that the row registers are
+// sized to the accumulators actually loaded)
+template <class K, int NA>
 __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
     const K* __restrict__ keys, size_t n, AccSet accs, const unsigned long long* __restrict__ hot_keys,
     int Sh, unsigned long long* __restrict__ partial, K* __restrict__ cold_keys, Cols cold,
@@ -665,7 +674,8 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
   extern __shared__ unsigned long long lds_u64[];
   __shared__ unsigned scan_tmp[HOT_NT / 64];
   __shared__ unsigned long long cbase;
-  const int na = accs.n;
+  constexpr int na = NA;
+  const int shift = 64 - (31 - __builtin_clz(unsigned(Sh)));
   unsigned long long* hk = lds_u64;        // [Sh]
   unsigned long long* hc = hk + Sh;        // [Sh]
   unsigned long long* ha = hc + Sh;        // [na][Sh]
@@ -680,22 +690,22 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
   const size_t r1 = r0 + HOT_CHUNK < n ? r0 + HOT_CHUNK : n;
   for (size_t t0 = r0; t0 < r1; t0 += HOT_STEP) {
     K kv[HOT_UNROLL];
-    unsigned long long vv[HOT_UNROLL][HA_MAX_ACC];
+    unsigned long long vv[HOT_UNROLL][NA > 0 ? NA : 1];
     int sl[HOT_UNROLL];
 #pragma unroll
     for (int u = 0; u < HOT_UNROLL; u++) {
       const size_t r = t0 + size_t(u) * HOT_NT + tid;
       kv[u] = r < r1 ? keys[r] : K(0);
 #pragma unroll
-      for (int a = 0; a < HA_MAX_ACC; a++)
-        vv[u][a] = (a < na && r < r1) ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
+      for (int a = 0; a < NA; a++)
+        vv[u][a] = r < r1 ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
     }
     unsigned ncold = 0;
 #pragma unroll
     for (int u = 0; u < HOT_UNROLL; u++) {
       const size_t r = t0 + size_t(u) * HOT_NT + tid;
       const bool in = r < r1;
-      sl[u] = in ? hot_slot(hk, Sh, key_bits(kv[u])) : -1;
+      sl[u] = in ? hot_slot(hk, Sh, shift, key_bits(kv[u])) : -1;
       if (in && sl[u] < 0) ncold++;
       const bool ok = sl[u] >= 0;
       const unsigned long long act = __ballot(ok);
@@ -704,12 +714,14 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
       const int s0 = __shfl(sl[u], leader, 64);
       if (__ballot(ok && sl[u] == s0) == act) {
         if (lane == unsigned(leader)) atomicAdd(&hc[s0], (unsigned long long)__popcll(act));
+#pragma unroll
         for (int a = 0; a < na; a++) {
           const unsigned long long w = wave_reduce_acc(vv[u][a], accs.a[a].kind, accs.a[a].vtype, act, ok);
           if (lane == unsigned(leader)) acc_merge(&ha[size_t(a) * Sh + s0], accs.a[a].kind, accs.a[a].vtype, w);
         }
       } else if (ok) {
         atomicAdd(&hc[sl[u]], 1ull);
+#pragma unroll
         for (int a = 0; a < na; a++)
           acc_merge(&ha[size_t(a) * Sh + sl[u]], accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
       }
@@ -726,7 +738,8 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
       const size_t r = t0 + size_t(u) * HOT_NT + tid;
       if (r >= r1 || sl[u] >= 0) continue;
       cold_keys[pos] = kv[u];
-      for (int a = 0; a < cold.n; a++) {
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
         if (cold.bytes[a] == 4) static_cast<unsigned*>(cold.out[a])[pos] = unsigned(vv[u][a]);
         else static_cast<unsigned long long*>(cold.out[a])[pos] = vv[u][a];
       }
@@ -931,8 +944,9 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   if (runs.empty() || covered * 100 < size_t(std::max(0LL, min_pct)) * m) return h;
   h.Sh = Sh;
   h.table.assign(size_t(Sh), HA_EMPTY);
+  const int shift = 64 - __builtin_ctz(unsigned(Sh));
   for (auto& r : runs) {
-    size_t slot = size_t(mix64(r.second) & (unsigned long long)(Sh - 1));
+    size_t slot = size_t(hot_home(r.second, shift));
     while (h.table[slot] != HA_EMPTY) slot = (slot + 1) & size_t(Sh - 1);
     h.table[slot] = r.second;
   }
@@ -1031,10 +1045,18 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     if (!hot_attr) {
       // (2 + accumulators) words per slot: at most 4 x 4096 or 6 x 2048 words
       const int mx = int(size_t(4) * 2 * PA_SLOTS * 8);
-      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-      FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      auto set_all = [&](auto kz) {
+        using KK = decltype(kz);
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      };
+      set_all(int32_t());
+      set_all(uint32_t());
+      set_all(int64_t());
+      set_all(uint64_t());
       hot_attr = true;
     }
     FHIP_REQUIRE(size_t(2 + accs.n) * Sh * 8 <= size_t(160) * 1024, FHIP_ERR_ARG,
@@ -1060,11 +1082,25 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     const size_t lds = size_t(2 + accs.n) * Sh * 8;
     {
       ProfScope prof("groupby_hot", st);
+      auto launch = [&](auto kz) {
+        using KK = decltype(kz);
+        const KK* kp = static_cast<const KK*>(keys);
+        KK* ck = cold_k.as<KK>();
+        unsigned long long* ht = hot_tab.as<unsigned long long>();
+        unsigned long long* hpart = hot_part.as<unsigned long long>();
+        switch (accs.n) {
+          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
+          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
+          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
+          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
+          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
+        }
+      };
       switch (ktype) {
-        case FHIP_I32: hot_agg_kernel<int32_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const int32_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<int32_t>(), cold, cold_n); break;
-        case FHIP_U32: hot_agg_kernel<uint32_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const uint32_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<uint32_t>(), cold, cold_n); break;
-        case FHIP_I64: hot_agg_kernel<int64_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const int64_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<int64_t>(), cold, cold_n); break;
-        default: hot_agg_kernel<uint64_t><<<unsigned(nparts), HOT_NT, lds, st>>>((const uint64_t*)keys, n, accs, hot_tab.as<unsigned long long>(), Sh, hot_part.as<unsigned long long>(), cold_k.as<uint64_t>(), cold, cold_n); break;
+        case FHIP_I32: launch(int32_t()); break;
+        case FHIP_U32: launch(uint32_t()); break;
+        case FHIP_I64: launch(int64_t()); break;
+        default: launch(uint64_t()); break;
       }
       FHIP_CHECK(hipGetLastError());
     }
