@@ -644,24 +644,26 @@ constexpr int HOT_UNROLL = 4;
 constexpr int HOT_STEP = HOT_NT * HOT_UNROLL;
 constexpr size_t HOT_CHUNK = size_t(1) << 18;
 
-// Home slot of a key in the heavy-key table: Fibonacci multiply-shift (one
-// 64-bit multiply; the table is at most half full, so the linear probe stays
-// short), `shift` = 64 - log2(Sh).
-__host__ __device__ __forceinline__ int hot_home(unsigned long long kb, int shift) {
+// The heavy-key table is two-choice (cuckoo) hashed with buckets of two
+// slots: a key sits in one of the slots 2*b1, 2*b1+1, 2*b2, 2*b2+1, the
+// buckets b1, b2 coming from two multiply-shift hashes (`shift` = 64 -
+// log2(Sh / 2)).  A lookup reads those four slots (two 16-byte LDS reads)
+// and compares, with no probe loop: every lane of a wave does the same work,
+// hit or miss.  A key that the host cannot place stays off the table and
+// its rows take the partitioned path.
+__host__ __device__ __forceinline__ int hot_bucket1(unsigned long long kb, int shift) {
   return int((kb * 0x9E3779B97F4A7C15ull) >> shift);
 }
+__host__ __device__ __forceinline__ int hot_bucket2(unsigned long long kb, int shift) {
+  return int((kb * 0xC2B2AE3D27D4EB4Full) >> shift);
+}
 
-__device__ __forceinline__ int hot_slot(const unsigned long long* hk, int Sh, int shift,
+__device__ __forceinline__ int hot_slot(const unsigned long long* hk, int shift,
                                         unsigned long long kb) {
   if (kb == HA_EMPTY) return -1;
-  int s = hot_home(kb, shift);
-  for (int p = 0; p < Sh; p++) {
-    const unsigned long long c = hk[s];
-    if (c == kb) return s;
-    if (c == HA_EMPTY) return -1;
-    s = (s + 1) & (Sh - 1);
-  }
-  return -1;
+  const int s1 = 2 * hot_bucket1(kb, shift), s2 = 2 * hot_bucket2(kb, shift);
+  const unsigned long long a0 = hk[s1], a1 = hk[s1 + 1], b0 = hk[s2], b1 = hk[s2 + 1];
+  return a0 == kb ? s1 : a1 == kb ? s1 + 1 : b0 == kb ? s2 : b1 == kb ? s2 + 1 : -1;
 }
 
 // NA = accs.n, 0 to 4 (a template parameter, so that the row registers are
@@ -675,7 +677,13 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
   __shared__ unsigned scan_tmp[HOT_NT / 64];
   __shared__ unsigned long long cbase;
   constexpr int na = NA;
-  const int shift = 64 - (31 - __builtin_clz(unsigned(Sh)));
+  // the accumulator loops are unrolled for up to two accumulators; wider
+  // sets keep the loop (unrolled, they hold too many values in registers)
+  constexpr int AGG_UNROLL = NA <= 2 ? 2 : 1;
+  // rows per thread and step: 4, or 2 with three or four accumulators
+  constexpr int U = NA <= 2 ? HOT_UNROLL : 2;
+  constexpr int STEP = HOT_NT * U;
+  const int shift = 64 - (30 - __builtin_clz(unsigned(Sh)));  // 64 - log2(Sh / 2)
   unsigned long long* hk = lds_u64;        // [Sh]
   unsigned long long* hc = hk + Sh;        // [Sh]
   unsigned long long* ha = hc + Sh;        // [na][Sh]
@@ -688,24 +696,29 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
   const unsigned tid = threadIdx.x, lane = __lane_id();
   const size_t r0 = size_t(blockIdx.x) * HOT_CHUNK;
   const size_t r1 = r0 + HOT_CHUNK < n ? r0 + HOT_CHUNK : n;
-  for (size_t t0 = r0; t0 < r1; t0 += HOT_STEP) {
-    K kv[HOT_UNROLL];
-    unsigned long long vv[HOT_UNROLL][NA > 0 ? NA : 1];
-    int sl[HOT_UNROLL];
+  constexpr int NA1 = NA > 0 ? NA : 1;
+  // the rows of the step starting at t (row r >= r1 reads nothing)
+  auto load_step = [&](size_t t, K (&kk)[U], unsigned long long (&vals)[U][NA1]) {
 #pragma unroll
-    for (int u = 0; u < HOT_UNROLL; u++) {
-      const size_t r = t0 + size_t(u) * HOT_NT + tid;
-      kv[u] = r < r1 ? keys[r] : K(0);
+    for (int u = 0; u < U; u++) {
+      const size_t r = t + size_t(u) * HOT_NT + tid;
+      kk[u] = r < r1 ? keys[r] : K(0);
 #pragma unroll
       for (int a = 0; a < NA; a++)
-        vv[u][a] = r < r1 ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
+        vals[u][a] = r < r1 ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
     }
+  };
+  for (size_t t0 = r0; t0 < r1; t0 += STEP) {
+    K kv[U];
+    unsigned long long vv[U][NA1];
+    load_step(t0, kv, vv);
+    int sl[U];
     unsigned ncold = 0;
 #pragma unroll
-    for (int u = 0; u < HOT_UNROLL; u++) {
+    for (int u = 0; u < U; u++) {
       const size_t r = t0 + size_t(u) * HOT_NT + tid;
       const bool in = r < r1;
-      sl[u] = in ? hot_slot(hk, Sh, shift, key_bits(kv[u])) : -1;
+      sl[u] = in ? hot_slot(hk, shift, key_bits(kv[u])) : -1;
       if (in && sl[u] < 0) ncold++;
       const bool ok = sl[u] >= 0;
       const unsigned long long act = __ballot(ok);
@@ -714,14 +727,14 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
       const int s0 = __shfl(sl[u], leader, 64);
       if (__ballot(ok && sl[u] == s0) == act) {
         if (lane == unsigned(leader)) atomicAdd(&hc[s0], (unsigned long long)__popcll(act));
-#pragma unroll
+#pragma unroll AGG_UNROLL
         for (int a = 0; a < na; a++) {
           const unsigned long long w = wave_reduce_acc(vv[u][a], accs.a[a].kind, accs.a[a].vtype, act, ok);
           if (lane == unsigned(leader)) acc_merge(&ha[size_t(a) * Sh + s0], accs.a[a].kind, accs.a[a].vtype, w);
         }
       } else if (ok) {
         atomicAdd(&hc[sl[u]], 1ull);
-#pragma unroll
+#pragma unroll AGG_UNROLL
         for (int a = 0; a < na; a++)
           acc_merge(&ha[size_t(a) * Sh + sl[u]], accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
       }
@@ -734,7 +747,7 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
     unsigned long long pos = cbase + off;
     __syncthreads();  // cbase is rewritten by the next step
 #pragma unroll
-    for (int u = 0; u < HOT_UNROLL; u++) {
+    for (int u = 0; u < U; u++) {
       const size_t r = t0 + size_t(u) * HOT_NT + tid;
       if (r >= r1 || sl[u] >= 0) continue;
       cold_keys[pos] = kv[u];
@@ -944,11 +957,31 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   if (runs.empty() || covered * 100 < size_t(std::max(0LL, min_pct)) * m) return h;
   h.Sh = Sh;
   h.table.assign(size_t(Sh), HA_EMPTY);
-  const int shift = 64 - __builtin_ctz(unsigned(Sh));
+  // two-choice placement, heaviest keys first: a key goes to a free slot of
+  // either bucket, or displaces a key of the bucket it was not just evicted
+  // from, which then moves on; a key still unplaced after the displacement
+  // limit is left out (its rows take the partitioned path)
+  const int shift = 64 - (__builtin_ctz(unsigned(Sh)) - 1);
   for (auto& r : runs) {
-    size_t slot = size_t(hot_home(r.second, shift));
-    while (h.table[slot] != HA_EMPTY) slot = (slot + 1) & size_t(Sh - 1);
-    h.table[slot] = r.second;
+    unsigned long long k = r.second;
+    int from = -1;  // the bucket k was evicted from
+    for (int kick = 0; kick < 256; kick++) {
+      const int c1 = hot_bucket1(k, shift), c2 = hot_bucket2(k, shift);
+      int placed = -1;
+      for (int s : {2 * c1, 2 * c1 + 1, 2 * c2, 2 * c2 + 1})
+        if (h.table[size_t(s)] == HA_EMPTY) {
+          placed = s;
+          break;
+        }
+      if (placed >= 0) {
+        h.table[size_t(placed)] = k;
+        k = HA_EMPTY;
+        break;
+      }
+      const int eb = (from == c1) ? c2 : c1;
+      std::swap(k, h.table[size_t(2 * eb + (kick & 1))]);
+      from = eb;
+    }
   }
   h.used = true;
   return h;

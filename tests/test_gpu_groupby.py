@@ -267,3 +267,34 @@ def test_group_by_agg_heavy_keys_and_rest(fa):
             cnt = oracle.grouped_agg("count", v, gi, split).astype(np.float64)
             _check_fp(a.numpy(), oracle.grouped_agg("avg", v, gi, split), mass / cnt, FP64_TOL)
     at.tune_set("groupby.hot_min_pct", None)
+
+
+@pytest.mark.parametrize("aggs", [("count",), ("sum",), ("sum", "maxi"),
+                                  ("sum", "maxv", "minv", "maxi")])
+def test_group_by_agg_heavy_keys_accumulator_counts(fa, aggs):
+    """The heavy-key pass with zero to four distinct accumulators (its
+    kernel is specialised on that count)."""
+    from frovedis_amd import datagen as dg
+    n = 1 << 21
+    k = dg.zipf_keys_np(n, n_values=1 << 16)
+    v = dg.normal_np(n)
+    iv = np.random.default_rng(32).integers(-1000, 1000, n).astype(np.int32)
+    dk, dv, div = (fa.DeviceArray.from_numpy(x) for x in (k, v, iv))
+    spec = {"count": ("count", None), "sum": ("sum", dv), "maxi": ("max", div),
+            "maxv": ("max", dv), "minv": ("min", dv)}
+    gk, outs = fa.group_by_agg(dk, [spec[a] for a in aggs])
+    gi, split, keys = oracle.group_by(k)
+    assert np.array_equal(gk.numpy(), keys)
+    mass = _abs_mass(v, gi, split)
+    for name, out in zip(aggs, outs):
+        got = out.numpy()
+        if name == "count":
+            assert np.array_equal(got, oracle.grouped_agg("count", v, gi, split))
+        elif name == "sum":
+            _check_fp(got, oracle.grouped_agg("sum", v, gi, split), mass, FP64_TOL)
+        elif name == "maxi":
+            assert np.array_equal(got, oracle.grouped_agg("max", iv, gi, split))
+        elif name == "maxv":
+            assert np.array_equal(got, oracle.grouped_agg("max", v, gi, split))
+        else:
+            assert np.array_equal(got, oracle.grouped_agg("min", v, gi, split))
