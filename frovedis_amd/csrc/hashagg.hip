@@ -631,16 +631,15 @@ __global__ __launch_bounds__(PA_NT) void pa_merge_kernel(
 }
 
 // ---- hot keys: heavy keys of skewed data, aggregated without partitioning --
-// The heavy keys come from a sample (at most half of the Sh slots, so that a
-// lookup ends within a few probes); hot_keys holds them at their
-// open-addressing slots (HA_EMPTY elsewhere).  Every workgroup walks a
+// The heavy keys come from a sample (at most 80 % of the Sh slots); hot_keys
+// holds them at their two-choice slots (HA_EMPTY elsewhere).  Every workgroup walks a
 // contiguous chunk of HOT_CHUNK rows: a row with a heavy key is aggregated
 // into the workgroup's LDS table, any other row is appended to the cold
 // buffers (one global reservation per 4096-row step of the workgroup), which
 // then go through the partitioned path.  The per-workgroup tables share the
 // slot layout, so the merge is a column-wise reduction.
 constexpr int HOT_NT = 1024;
-constexpr int HOT_UNROLL = 4;
+constexpr int HOT_UNROLL = 2;
 constexpr int HOT_STEP = HOT_NT * HOT_UNROLL;
 constexpr size_t HOT_CHUNK = size_t(1) << 18;
 
@@ -668,8 +667,10 @@ __device__ __forceinline__ int hot_slot(const unsigned long long* hk, int shift,
 
 // NA = accs.n, 0 to 4 (a template parameter, so that the row registers are
 // sized to the accumulators actually loaded)
+// Up to two accumulators, the registers fit two 1024-thread workgroups per
+// CU (8 waves per SIMD); the heavy-key table of 2048 slots leaves LDS for both.
 template <class K, int NA>
-__global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
+__global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     const K* __restrict__ keys, size_t n, AccSet accs, const unsigned long long* __restrict__ hot_keys,
     int Sh, unsigned long long* __restrict__ partial, K* __restrict__ cold_keys, Cols cold,
     unsigned long long* __restrict__ cold_n) {
@@ -680,8 +681,8 @@ __global__ __launch_bounds__(HOT_NT) void hot_agg_kernel(
   // the accumulator loops are unrolled for up to two accumulators; wider
   // sets keep the loop (unrolled, they hold too many values in registers)
   constexpr int AGG_UNROLL = NA <= 2 ? 2 : 1;
-  // rows per thread and step: 4, or 2 with three or four accumulators
-  constexpr int U = NA <= 2 ? HOT_UNROLL : 2;
+  // rows per thread and step
+  constexpr int U = HOT_UNROLL;
   constexpr int STEP = HOT_NT * U;
   const int shift = 64 - (30 - __builtin_clz(unsigned(Sh)));  // 64 - log2(Sh / 2)
   unsigned long long* hk = lds_u64;        // [Sh]
@@ -920,7 +921,10 @@ struct HotPlan {
 
 HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   HotPlan h;
-  long long want = tune_get("groupby.hot", S);
+  // 2048 slots by default: the sampled heavy keys of skewed data fit (about
+  // a thousand keys occur 4+ times in a 2^16 sample of Zipf(1.1) keys), and
+  // two workgroups' tables fit one CU's LDS
+  long long want = tune_get("groupby.hot", std::min(S, 2048));
   constexpr size_t m = size_t(1) << 16;
   if (want < 256 || n < 16 * m) return h;
   int Sh = int(std::min<long long>(want, S));
@@ -950,7 +954,9 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
                                          const std::pair<size_t, unsigned long long>& b) {
     return a.first > b.first || (a.first == b.first && a.second < b.second);
   });
-  if (runs.size() > size_t(Sh / 2)) runs.resize(size_t(Sh / 2));
+  // two-choice buckets of two slots place keys up to a load near 90 %; the
+  // plan fills at most 80 % of the slots
+  if (runs.size() > size_t(Sh) * 4 / 5) runs.resize(size_t(Sh) * 4 / 5);
   size_t covered = 0;
   for (auto& r : runs) covered += r.first;
   const long long min_pct = tune_get("groupby.hot_min_pct", 30);
