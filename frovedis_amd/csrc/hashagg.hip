@@ -33,8 +33,8 @@ namespace {
 constexpr int HA_MAX_ACC = 4;
 constexpr int PA_NT = 1024;           // threads of the aggregate / merge kernels (one per CU: LDS)
 constexpr int PA_UNROLL = 4;          // rows in flight per thread (aggregate)
-constexpr int PS_NT = 512;            // threads of the count / scatter kernels
-constexpr int PS_ITEMS = 8;           // rows per thread
+constexpr int PS_NT = 1024;           // threads of the count / scatter kernels
+constexpr int PS_ITEMS = 4;           // rows per thread
 constexpr int PS_TILE = PS_NT * PS_ITEMS;  // rows per scatter tile
 constexpr int PS_SPAN = 16;            // default tiles per count / scatter workgroup (a "span")
 constexpr unsigned long long PA_CHUNK = 1ull << 17;  // rows per aggregate workgroup
