@@ -498,18 +498,20 @@ struct Item {
 };
 
 // ---- pass 3: aggregate one chunk of one partition in LDS --------------------
-template <class K>
-__global__ __launch_bounds__(PA_NT) void pa_agg_kernel(const Item* __restrict__ items,
-                                                       const K* __restrict__ keys, AccSet accs,
-                                                       int S, int stride,
-                                                       unsigned long long* __restrict__ partial,
-                                                       unsigned* __restrict__ partial_n,
-                                                       int* __restrict__ overflow) {
+// NA = accs.n, 0 to 4: the row registers are sized to the accumulators
+// loaded; up to two accumulators two 1024-thread workgroups share a CU.
+template <class K, int NA>
+__global__ __launch_bounds__(PA_NT, NA <= 2 ? 8 : 4) void pa_agg_kernel(
+    const Item* __restrict__ items, const K* __restrict__ keys, AccSet accs, int S, int stride,
+    unsigned long long* __restrict__ partial, unsigned* __restrict__ partial_n,
+    int* __restrict__ overflow) {
   extern __shared__ unsigned long long lds_u64[];
   unsigned char* lds_raw = reinterpret_cast<unsigned char*>(lds_u64);
   __shared__ unsigned counter;
   __shared__ int full;
-  const int na = accs.n;
+  constexpr int na = NA;
+  constexpr int NA1 = NA > 0 ? NA : 1;
+  constexpr int AGG_UNROLL = NA <= 2 ? 2 : 1;
   LdsTable t = lds_table(lds_raw, S, na);
   lds_table_init(t, accs);
   if (threadIdx.x == 0) {
@@ -524,14 +526,14 @@ __global__ __launch_bounds__(PA_NT) void pa_agg_kernel(const Item* __restrict__ 
   for (size_t w0 = wave_first; w0 < it.end; w0 += size_t(PA_NT) * PA_UNROLL) {
     const size_t r0 = w0 + __lane_id();
     unsigned long long kbv[PA_UNROLL];
-    unsigned long long vv[PA_UNROLL][HA_MAX_ACC];
+    unsigned long long vv[PA_UNROLL][NA1];
 #pragma unroll
     for (int u = 0; u < PA_UNROLL; u++) {
       const size_t r = r0 + size_t(u) * PA_NT;
       kbv[u] = r < it.end ? key_bits(keys[r]) : 0ull;
 #pragma unroll
-      for (int a = 0; a < HA_MAX_ACC; a++)
-        vv[u][a] = (a < na && r < it.end) ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
+      for (int a = 0; a < NA; a++)
+        vv[u][a] = r < it.end ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
     }
     const unsigned lane = __lane_id();
 #pragma unroll
@@ -549,6 +551,7 @@ __global__ __launch_bounds__(PA_NT) void pa_agg_kernel(const Item* __restrict__ 
         // the whole wave updates one slot (a heavy key): reduce, then one
         // atomic per accumulator
         if (lane == unsigned(leader)) atomicAdd(&t.cnt[s0], (unsigned long long)__popcll(act));
+#pragma unroll AGG_UNROLL
         for (int a = 0; a < na; a++) {
           const unsigned long long w =
               wave_reduce_acc(vv[u][a], accs.a[a].kind, accs.a[a].vtype, act, ok);
@@ -557,6 +560,7 @@ __global__ __launch_bounds__(PA_NT) void pa_agg_kernel(const Item* __restrict__ 
         }
       } else if (ok) {
         atomicAdd(&t.cnt[s], 1ull);
+#pragma unroll AGG_UNROLL
         for (int a = 0; a < na; a++)
           acc_merge(&t.acc[size_t(a) * (S + 1) + s], accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
       }
@@ -1059,10 +1063,18 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   if (!attr_set) {
     const int mx = int(std::max(size_t(2 * PA_SLOTS + 1) * 8 * 4,
                                 size_t(PA_SLOTS + 1) * 8 * (2 + HA_MAX_ACC)));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    auto set_agg = [&](auto kz) {
+      using KK = decltype(kz);
+      FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<KK, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<KK, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<KK, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<KK, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)pa_agg_kernel<KK, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+    };
+    set_agg(int32_t());
+    set_agg(uint32_t());
+    set_agg(int64_t());
+    set_agg(uint64_t());
     FHIP_CHECK(hipFuncSetAttribute((const void*)pa_merge_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     const int ms = int(size_t(PS_TILE) * 10 + (size_t(1) << PA_MAX_PBITS) * 16);
     FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
@@ -1274,11 +1286,23 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     for (int a = 0; a < caccs.n; a++) paccs.a[a].col = pcols[a].get();
     {
       ProfScope prof("hashagg", st);
+      auto launch_agg = [&](auto kz) {
+        using KK = decltype(kz);
+        const KK* pk = pkeys.as<KK>();
+        unsigned long long* pp = partial.as<unsigned long long>();
+        switch (paccs.n) {
+          case 0: pa_agg_kernel<KK, 0><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pk, paccs, S, stride, pp, pn.as<unsigned>(), ovf.as<int>()); break;
+          case 1: pa_agg_kernel<KK, 1><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pk, paccs, S, stride, pp, pn.as<unsigned>(), ovf.as<int>()); break;
+          case 2: pa_agg_kernel<KK, 2><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pk, paccs, S, stride, pp, pn.as<unsigned>(), ovf.as<int>()); break;
+          case 3: pa_agg_kernel<KK, 3><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pk, paccs, S, stride, pp, pn.as<unsigned>(), ovf.as<int>()); break;
+          default: pa_agg_kernel<KK, 4><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pk, paccs, S, stride, pp, pn.as<unsigned>(), ovf.as<int>()); break;
+        }
+      };
       switch (ktype) {
-        case FHIP_I32: pa_agg_kernel<int32_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<int32_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
-        case FHIP_U32: pa_agg_kernel<uint32_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<uint32_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
-        case FHIP_I64: pa_agg_kernel<int64_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<int64_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
-        default: pa_agg_kernel<uint64_t><<<unsigned(NI), PA_NT, lds_table_bytes, st>>>(itb.as<Item>(), pkeys.as<uint64_t>(), paccs, S, stride, partial.as<unsigned long long>(), pn.as<unsigned>(), ovf.as<int>()); break;
+        case FHIP_I32: launch_agg(int32_t()); break;
+        case FHIP_U32: launch_agg(uint32_t()); break;
+        case FHIP_I64: launch_agg(int64_t()); break;
+        default: launch_agg(uint64_t()); break;
       }
       FHIP_CHECK(hipGetLastError());
     }
