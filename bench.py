@@ -224,7 +224,7 @@ def bench_join(fa, steps=3, log2_left=27, log2_right=29):
     ri = fa.DeviceArray(torch.arange(nr, dtype=torch.int64, device="cuda"), np.uint64)
     hits = int((rk.t < nl).sum())
     out = {"left_rows": nl, "right_rows": nr, "pairs": hits}
-    legs = {"hash": (rk, ri, lk, li, "auto", ("ht_build", "ht_probe", "join_compact")),
+    legs = {"hash": (rk, ri, lk, li, "auto", ("join_build", "join_probe", "join_compact")),
             "sortmerge": (lk, li, rk, ri, "auto", ("radix_hist", "radix_pass", "join_merge",
                                                     "join_write"))}
     for name, (a, ai, b, bi, strat, kernels) in legs.items():
@@ -339,7 +339,7 @@ def bench_relational(fa, log2n=28, steps=3):
     want = int(((k.t < (1 << 20)) & (iv.t < 900)).sum())
     stage("star_join", lambda: fa.star_join([k, iv], [(d1, d1i), (d2, d2i)]),
           lambda r: 12 * n + 24 * len(r[0]) + 16 * ((1 << 20) + 900),
-          ["ht_build", "ht_probe", "join_compact"],
+          ["join_build", "join_probe", "join_compact"],
           lambda r: len(r[0]) == want and
           bool(torch.equal(d1.t[r[1][0].t], k.t[r[0].t])) and
           bool(torch.equal(d2.t[r[1][1].t], iv.t[r[0].t])))
@@ -357,7 +357,7 @@ def bench_relational(fa, log2n=28, steps=3):
         return fa.outer_equi_join(rk, ri, lk, li, capacity=nr)
 
     res = stage("outer_join", outer, lambda r: 8 * (nl + nr) + 16 * hits + 8 * (nr - hits),
-                ["ht_build", "ht_probe", "join_compact"],
+                ["join_build", "join_probe", "join_compact"],
                 lambda r: len(r[0]) == hits and len(r[2]) == nr - hits and r[3] and
                 bool(torch.equal(rk.t[r[0].t], lk.t[r[1].t])) and
                 bool((rk.t[r[2].t] >= nl).all()))

@@ -17,7 +17,9 @@ saved and loaded as one flat JSON object (fhip_autotune_save / _load):
 - ``compact.shape``: the tile of the one-pass stream compaction (filter,
   set_separate, join outputs), an index into COMPACT_SHAPES;
 - ``multi_group_by.composite``: 0 switches off the composite-key path of
-  the multi-column group_by.
+  the multi-column group_by;
+- ``join.load_pct``: load factor (10 to 50 %) of the joins' hash table for
+  integer keys.
 
 ``tune_groupby`` and ``tune_radix`` measure candidate settings on the device for a given key
 column and aggregates, keeps the fastest whose output equals the default

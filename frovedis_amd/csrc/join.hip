@@ -1,13 +1,15 @@
 // equi_join on one rank's share for gfx950 (dataframe/join.hpp:304-326 VE
 // body, join.cc:10-61 shrink_missed, multi_equi_join :198-302).
 //
-// strategy FHIP_JOIN_AUTO follows the reference: a unique_hashtable is built
-// on RIGHT with the right row ids as values (skipped when an evenly spaced
-// sample of RIGHT already repeats a key, which proves it not unique); if the
-// right keys are unique, LEFT is probed in order and the hits are compacted (hash join, output in
-// left order); otherwise both sides are stable-sorted and every left row
-// (in sorted order) is paired with its run of equal right keys (sort-merge,
-// output in (key, left position, right position) order).
+// strategy FHIP_JOIN_AUTO follows the reference: a hash table (for integer
+// keys the joins' own open-addressing table below, otherwise a
+// unique_hashtable) is built on RIGHT with the right row ids as values
+// (skipped when an evenly spaced sample of RIGHT already repeats a key,
+// which proves it not unique); if the right keys are unique, LEFT is probed
+// in order and the hits are compacted (hash join, output in left order);
+// otherwise both sides are stable-sorted and every left row (in sorted
+// order) is paired with its run of equal right keys (sort-merge, output in
+// (key, left position, right position) order).
 // FHIP_JOIN_HASH requires unique right keys; FHIP_JOIN_SORTMERGE skips the
 // table build.
 #include "fhip_device.hpp"
@@ -230,6 +232,128 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
   return size_t(total);
 }
 
+// ---- the joins' own table for integer keys -------------------------------------
+// A join only looks keys up, so for integer keys it does not need the
+// unique_hashtable layout: open addressing in HBM, 16-byte {key, value}
+// slots, at most 25 % full (the tunable "join.load_pct", 10 to 50), home slot by a multiply-shift hash, linear
+// probing, insertion by a 64-bit atomicCAS on the key word (which also finds
+// a repeated key: then the right side is not unique, as the reference's
+// is_unique_ok reports).  The all-ones key, the empty marker, is kept aside
+// in a side record.  Floating keys keep the reference layout, whose bit
+// pattern versus == behaviour (-0.0, NaN) the result depends on.
+constexpr unsigned long long JT_EMPTY = ~0ull;
+constexpr int JT_NT = 256;
+
+template <class K>
+__device__ __forceinline__ unsigned long long jt_bits(K k) {
+  return static_cast<unsigned long long>(static_cast<typename std::make_unsigned<K>::type>(k));
+}
+
+__device__ __forceinline__ unsigned long long jt_home(unsigned long long k, int shift) {
+  return (k * 0x9E3779B97F4A7C15ull) >> shift;
+}
+
+// side = {rows with the empty-marker key, the value of the first, repeated key}
+template <class K>
+__global__ __launch_bounds__(JT_NT) void jt_insert_kernel(const K* __restrict__ keys,
+                                                          const unsigned long long* __restrict__ vals,
+                                                          size_t n, unsigned long long* slots,
+                                                          unsigned long long mask, int shift,
+                                                          unsigned long long* side) {
+  const size_t stride = size_t(gridDim.x) * JT_NT;
+  for (size_t i = size_t(blockIdx.x) * JT_NT + threadIdx.x; i < n; i += stride) {
+    const unsigned long long k = jt_bits(keys[i]);
+    const unsigned long long v = vals[i];
+    if (k == JT_EMPTY) {
+      if (atomicAdd(&side[0], 1ull) == 0) side[1] = v;
+      else atomicOr(&side[2], 1ull);
+      continue;
+    }
+    unsigned long long h = jt_home(k, shift);
+    while (true) {
+      const unsigned long long prev = atomicCAS(&slots[2 * h], JT_EMPTY, k);
+      if (prev == JT_EMPTY) {
+        slots[2 * h + 1] = v;
+        break;
+      }
+      if (prev == k) {
+        atomicOr(&side[2], 1ull);
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+template <class K>
+__global__ __launch_bounds__(JT_NT) void jt_probe_kernel(const K* __restrict__ q, size_t m,
+                                                         const unsigned long long* __restrict__ slots,
+                                                         unsigned long long mask, int shift,
+                                                         const unsigned long long* __restrict__ side,
+                                                         unsigned long long* __restrict__ vals_out,
+                                                         uint8_t* __restrict__ found) {
+  const size_t stride = size_t(gridDim.x) * JT_NT;
+  for (size_t j = size_t(blockIdx.x) * JT_NT + threadIdx.x; j < m; j += stride) {
+    const unsigned long long k = jt_bits(q[j]);
+    bool f = false;
+    unsigned long long v = 0;
+    if (k == JT_EMPTY) {
+      f = side[0] != 0;
+      v = f ? side[1] : 0ull;
+    } else {
+      unsigned long long h = jt_home(k, shift);
+      while (true) {
+        const ulonglong2 e = *reinterpret_cast<const ulonglong2*>(slots + 2 * h);
+        if (e.x == k) {
+          f = true;
+          v = e.y;
+          break;
+        }
+        if (e.x == JT_EMPTY) break;
+        h = (h + 1) & mask;
+      }
+    }
+    vals_out[j] = v;
+    found[j] = f ? 1 : 0;
+  }
+}
+
+// Builds the table over (keys, vals) and probes q with it; returns false
+// (without probing) when the keys repeat.
+template <class K>
+bool jt_build_probe(const K* keys, const unsigned long long* vals, size_t n, const K* q, size_t m,
+                    unsigned long long* vals_out, uint8_t* found, hipStream_t st) {
+  const long long load_pct = std::max(10LL, std::min(50LL, tune_get("join.load_pct", 25)));
+  int bits = 10;
+  while ((size_t(1) << bits) * size_t(load_pct) < 100 * n) bits++;
+  const unsigned long long nslots = 1ull << bits;
+  DBuf tab(size_t(nslots) * 16), side(3 * sizeof(unsigned long long));
+  FHIP_CHECK(hipMemsetAsync(tab.get(), 0xFF, size_t(nslots) * 16, st));
+  FHIP_CHECK(hipMemsetAsync(side.get(), 0, 3 * sizeof(unsigned long long), st));
+  {
+    ProfScope prof("join_build", st);
+    if (n)
+      jt_insert_kernel<K><<<grid_for_n(n), JT_NT, 0, st>>>(
+          keys, vals, n, tab.as<unsigned long long>(), nslots - 1, 64 - bits,
+          side.as<unsigned long long>());
+    FHIP_CHECK(hipGetLastError());
+  }
+  unsigned long long dup = 0;
+  FHIP_CHECK(hipMemcpyAsync(&dup, side.as<unsigned long long>() + 2, sizeof(dup),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  if (dup) return false;
+  if (m) {
+    ProfScope prof("join_probe", st);
+    jt_probe_kernel<K><<<grid_for_n(m), JT_NT, 0, st>>>(q, m, tab.as<unsigned long long>(),
+                                                        nslots - 1, 64 - bits,
+                                                        side.as<unsigned long long>(), vals_out,
+                                                        found);
+    FHIP_CHECK(hipGetLastError());
+  }
+  return true;
+}
+
 // ---- star join ---------------------------------------------------------------
 constexpr int SJ_MAXD = 8;
 
@@ -284,28 +408,50 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
     });
   }
   if (strategy != FHIP_JOIN_SORTMERGE && !dup_seen) {
-    int uniq = 1;
-    fhip_ht* h = ht_build(ktype, right, reinterpret_cast<const uint64_t*>(ridx), nr, true, &uniq, st);
-    if (uniq) {
-      size_t r = 0;
-      try {
+    // the probe's hits in left order, then the left rows without a partner
+    auto emit = [&](DBuf& vals, DBuf& found) {
+      ProfScope prof("join_compact", st);
+      const size_t r = compact_apply(HitPred{found.as<uint8_t>()},
+                                     HitWriter{lidx, vals.as<uint64_t>(), lo, ro}, nl, capacity,
+                                     st);
+      // left_idx[missed], missed ascending (join.hpp:340-350)
+      if (only_left)
+        *n_only = compact(MissPred{found.as<uint8_t>()}, RowIdMap{lidx}, nl, only_left, st);
+      return r;
+    };
+    bool uniq = true, done = false;
+    size_t r = 0;
+    dispatch_type(ktype, [&](auto z) {
+      using K = decltype(z);
+      if constexpr (std::is_integral<K>::value) {
         DBuf vals(nl * sizeof(uint64_t)), found(nl);
-        ht_probe(h, left, nl, vals.as<uint64_t>(), found.as<uint8_t>(), st);
-        ProfScope prof("join_compact", st);
-        r = compact_apply(HitPred{found.as<uint8_t>()},
-                          HitWriter{lidx, vals.as<uint64_t>(), lo, ro}, nl, capacity, st);
-        // left_idx[missed], missed ascending (join.hpp:340-350)
-        if (only_left)
-          *n_only = compact(MissPred{found.as<uint8_t>()}, RowIdMap{lidx}, nl, only_left, st);
-      } catch (...) {
-        ht_destroy(h);
-        throw;
+        uniq = jt_build_probe<K>((const K*)right, reinterpret_cast<const unsigned long long*>(ridx),
+                                 nr, (const K*)left, nl, vals.as<unsigned long long>(),
+                                 found.as<uint8_t>(), st);
+        if (uniq) r = emit(vals, found);
+        done = true;
+      }
+    });
+    if (!done) {
+      int u = 1;
+      fhip_ht* h = ht_build(ktype, right, reinterpret_cast<const uint64_t*>(ridx), nr, true, &u, st);
+      uniq = u != 0;
+      if (uniq) {
+        try {
+          DBuf vals(nl * sizeof(uint64_t)), found(nl);
+          ht_probe(h, left, nl, vals.as<uint64_t>(), found.as<uint8_t>(), st);
+          r = emit(vals, found);
+        } catch (...) {
+          ht_destroy(h);
+          throw;
+        }
       }
       ht_destroy(h);
+    }
+    if (uniq) {
       if (used_hash) *used_hash = 1;
       return r;
     }
-    ht_destroy(h);
     FHIP_REQUIRE(strategy == FHIP_JOIN_AUTO, FHIP_ERR_ARG,
                  "equi_join: hash strategy needs unique right keys");
   }
@@ -387,34 +533,47 @@ size_t star_join(int nd, const int* ktypes, const void* const* left_cols, const 
   w.lo = lo;
   w.nd = nd;
   for (int d = 0; d < nd; d++) {
-    int uniq = 1;
-    fhip_ht* h = ht_build(ktypes[d], right_cols[d], reinterpret_cast<const uint64_t*>(ridx[d]),
-                          nr[d], true, &uniq, st);
-    try {
-      FHIP_REQUIRE(uniq, FHIP_ERR_ARG, "star_join: right column is not unique");
-      const void* q = left_cols[d];
-      if (lidx) {
-        keys.reset(nl * size_t(type_bytes(ktypes[d])));
-        dispatch_type(ktypes[d], [&](auto z) {
-          using K = decltype(z);
-          sj_gather_kernel<K><<<grid_for_n(nl), 256, 0, st>>>((const K*)left_cols[d], lidx, nl,
-                                                              keys.as<K>());
-        });
-        FHIP_CHECK(hipGetLastError());
-        q = keys.get();
-      }
-      vals[d].reset(nl * sizeof(uint64_t));
-      uint8_t* f = d == 0 ? found.as<uint8_t>() : fd.as<uint8_t>();
-      ht_probe(h, q, nl, vals[d].as<uint64_t>(), f, st);
-      if (d > 0) {
-        sj_and_kernel<<<grid_for_n(nl), 256, 0, st>>>(found.as<uint8_t>(), f, nl);
-        FHIP_CHECK(hipGetLastError());
-      }
-    } catch (...) {
-      ht_destroy(h);
-      throw;
+    const void* q = left_cols[d];
+    if (lidx) {
+      keys.reset(nl * size_t(type_bytes(ktypes[d])));
+      dispatch_type(ktypes[d], [&](auto z) {
+        using K = decltype(z);
+        sj_gather_kernel<K><<<grid_for_n(nl), 256, 0, st>>>((const K*)left_cols[d], lidx, nl,
+                                                            keys.as<K>());
+      });
+      FHIP_CHECK(hipGetLastError());
+      q = keys.get();
     }
-    ht_destroy(h);
+    vals[d].reset(nl * sizeof(uint64_t));
+    uint8_t* f = d == 0 ? found.as<uint8_t>() : fd.as<uint8_t>();
+    bool done = false;
+    dispatch_type(ktypes[d], [&](auto z) {
+      using K = decltype(z);
+      if constexpr (std::is_integral<K>::value) {
+        const bool uniq = jt_build_probe<K>(
+            (const K*)right_cols[d], reinterpret_cast<const unsigned long long*>(ridx[d]), nr[d],
+            (const K*)q, nl, vals[d].as<unsigned long long>(), f, st);
+        FHIP_REQUIRE(uniq, FHIP_ERR_ARG, "star_join: right column is not unique");
+        done = true;
+      }
+    });
+    if (!done) {
+      int uniq = 1;
+      fhip_ht* h = ht_build(ktypes[d], right_cols[d], reinterpret_cast<const uint64_t*>(ridx[d]),
+                            nr[d], true, &uniq, st);
+      try {
+        FHIP_REQUIRE(uniq, FHIP_ERR_ARG, "star_join: right column is not unique");
+        ht_probe(h, q, nl, vals[d].as<uint64_t>(), f, st);
+      } catch (...) {
+        ht_destroy(h);
+        throw;
+      }
+      ht_destroy(h);
+    }
+    if (d > 0) {
+      sj_and_kernel<<<grid_for_n(nl), 256, 0, st>>>(found.as<uint8_t>(), f, nl);
+      FHIP_CHECK(hipGetLastError());
+    }
     w.vals[d] = vals[d].as<uint64_t>();
     w.ro[d] = ro[d];
   }

@@ -83,7 +83,9 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *       set_separate and the join outputs (0 256x16, 1 1024x16 threads x
  *       rows);
  *   "multi_group_by.composite"  0 switches off the composite-key path of
- *       fhip_dev_multi_group_by (the column-by-column sorts run instead).
+ *       fhip_dev_multi_group_by (the column-by-column sorts run instead);
+ *   "join.load_pct"     load factor (10 to 50 %) of the joins' hash table for
+ *       integer keys.
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);
