@@ -247,7 +247,8 @@ int fhip_dev_outer_equi_join(int ktype, const void* left,
  * column must hold unique keys ("star_join: right column is not unique",
  * FHIP_ERR_ARG).  The left rows found in every dimension come out in left
  * order: left_out (nl entries of capacity) and right_out[d] (nl entries
- * each); *nout receives the row count.  At most 8 dimensions. */
+ * each); *nout receives the row count.  At most 8 dimensions; every
+ * left_idx entry must be below the left columns' length. */
 int fhip_dev_star_join(int ndims, const int* ktypes,
                        const void* const* left_cols, const size_t* left_idx,
                        size_t nl, const void* const* right_cols,
@@ -313,7 +314,8 @@ int fhip_dev_filter_pair(int atype, const void* a, int btype, const void* b,
  * 2778-2800 -> sort_with_idx, dfcolumn_sort.cc:299-340); the keys are
  * col[idx_in[i]], and sorting again by another column is the reference's
  * multi-column sort ("sort c2, then c1").  Outputs: sorted_out (n keys) and
- * idx_out (n row ids); idx_out may equal idx_in. */
+ * idx_out (n row ids); idx_out may equal idx_in.  Every idx_in entry must be
+ * below col_n (the caller's contract, as for fhip_dev_group_by's idx). */
 int fhip_dev_sort_index(int ktype, const void* col, size_t col_n,
                         const size_t* idx_in, size_t n, int descending,
                         void* sorted_out, size_t* idx_out, void* stream);
@@ -326,7 +328,8 @@ int fhip_dev_sort_index(int ktype, const void* col, size_t col_n,
  * boundaries (merge_split = set_union, dftable.cc:336-339), n + 1 entries
  * of capacity; keys_out[c] (n entries of capacity each) the key of column
  * c of every group (multi_group_by_extract).  The result feeds
- * fhip_dev_grouped_agg unchanged. */
+ * fhip_dev_grouped_agg unchanged.  Every idx entry must be below the
+ * columns' length (the caller's contract, as for fhip_dev_group_by). */
 int fhip_dev_multi_group_by(int ncols, const int* ktypes,
                             const void* const* cols, size_t* idx, size_t n,
                             size_t* split, size_t* nsplit,
