@@ -97,7 +97,8 @@ constexpr int RH_ITEMS = 8;
 template <class U>
 __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, size_t n,
                                                  int mode, int desc, int nbytes,
-                                                 unsigned long long* __restrict__ ghist) {
+                                                 unsigned long long* __restrict__ ghist,
+                                                 int two_leaders) {
   __shared__ unsigned h[sizeof(U) * 256];
   for (int i = threadIdx.x; i < int(sizeof(U)) * 256; i += RS_NT) h[i] = 0;
   __syncthreads();
@@ -127,12 +128,14 @@ __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, siz
         unsigned long long same = __ballot(valid && d == d0);
         if (lane == 0) atomicAdd(&h[b * 256 + d0], unsigned(__popcll(same)));
         const unsigned long long rest = vmask & ~same;
-        if (rest) {
+        if (rest && (two_leaders || (same & (same - 1)))) {
           const int l1 = __ffsll((long long)rest) - 1;
           const unsigned d1 = __shfl(d, l1, 64);
           const unsigned long long same1 = __ballot(valid && d == d1);
           if (int(lane) == l1) atomicAdd(&h[b * 256 + d1], unsigned(__popcll(same1)));
           if (valid && d != d0 && d != d1) atomicAdd(&h[b * 256 + d], 1u);
+        } else if (valid && d != d0) {
+          atomicAdd(&h[b * 256 + d], 1u);
         }
       }
     }
@@ -445,19 +448,30 @@ size_t bytes_of_max(unsigned long long mx, int kbytes) {
 }
 
 // Core driver: histogram, pass selection and the pass schedule.
+// Launch shape of the histogram pass: workgroups per CU (tunable
+// "radix.hist_bpc") and whether a wave also aggregates the lanes sharing a
+// second digit ("radix.hist_two", 1 = always, 0 = only when lane 0's digit
+// was shared).
+int hist_grid(size_t n, int* two) {
+  const int hbpc = int(std::max(1LL, std::min(64LL, tune_get("radix.hist_bpc", 64))));
+  *two = int(tune_get("radix.hist_two", 1));
+  return int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), size_t(fill_blocks(RS_NT, hbpc))));
+}
+
 void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* kout,
              int vbytes, const void* vin, void* vout, size_t n, hipStream_t st) {
   DBuf hist(size_t(2) * 8 * 256 * sizeof(unsigned long long));
   unsigned long long* ghist = hist.as<unsigned long long>();
   unsigned long long* gbase = ghist + 8 * 256;
   FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
-  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), size_t(fill_blocks(RS_NT, 4))));
+  int two = 1;
+  const int grid = hist_grid(n, &two);
   {
     ProfScope prof("radix_hist", st);
     if (kbytes == 4)
-      rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist);
+      rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist, two);
     else
-      rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, mode, desc, nbytes, ghist);
+      rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, mode, desc, nbytes, ghist, two);
     FHIP_CHECK(hipGetLastError());
   }
   rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
@@ -599,11 +613,12 @@ void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int v
   zero_u64_kernel<<<grid_for_n(L.lb_words), 256, 0, st>>>(lb0, L.lb_words);
   zero_u64_kernel<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(ctr), 1);
   FHIP_CHECK(hipGetLastError());
-  int grid = int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), size_t(fill_blocks(RS_NT, 4))));
+  int two = 1;
+  const int grid = hist_grid(n, &two);
   if (kbytes == 4)
-    rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, KM_RAW, 0, nbytes, ghist);
+    rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, KM_RAW, 0, nbytes, ghist, two);
   else
-    rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, KM_RAW, 0, nbytes, ghist);
+    rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, KM_RAW, 0, nbytes, ghist, two);
   FHIP_CHECK(hipGetLastError());
   rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
   FHIP_CHECK(hipGetLastError());

@@ -85,7 +85,10 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *   "multi_group_by.composite"  0 switches off the composite-key path of
  *       fhip_dev_multi_group_by (the column-by-column sorts run instead);
  *   "join.load_pct"     load factor (10 to 50 %) of the joins' hash table for
- *       integer keys.
+ *       integer keys;
+ *   "radix.hist_bpc", "radix.hist_two"  workgroups per CU of the radix
+ *       histogram pass (1 to 64), and 0 to aggregate a second shared digit
+ *       per wave only when lane 0's digit was shared.
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);
