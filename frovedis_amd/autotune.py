@@ -11,6 +11,7 @@ saved and loaded as one flat JSON object (fhip_autotune_save / _load):
 - ``groupby.span``: 4096-row tiles partitioned by one workgroup;
 - ``groupby.hot``: slots of the heavy-key table (0 turns the heavy-key pass
   off); ``groupby.hot_min_pct``: the sample share the heavy keys must cover;
+  ``groupby.hot_sample``: log2 of the rows sampled to find them (12 to 20);
 - ``radix.shape_k`` / ``radix.shape_kv``: the radix sort pass's tile shape
   (threads x keys per thread, an index into RADIX_SHAPES) for key-only and
   key/value sorts;

@@ -929,7 +929,9 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   // a thousand keys occur 4+ times in a 2^16 sample of Zipf(1.1) keys), and
   // two workgroups' tables fit one CU's LDS
   long long want = tune_get("groupby.hot", std::min(S, 2048));
-  constexpr size_t m = size_t(1) << 16;
+  // sample of 2^groupby.hot_sample rows (default 2^16)
+  const int mbits = int(std::max(12LL, std::min(20LL, tune_get("groupby.hot_sample", 16))));
+  const size_t m = size_t(1) << mbits;
   if (want < 256 || n < 16 * m) return h;
   int Sh = int(std::min<long long>(want, S));
   while (Sh & (Sh - 1)) Sh &= Sh - 1;
