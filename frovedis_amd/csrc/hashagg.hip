@@ -162,14 +162,16 @@ __device__ __forceinline__ unsigned part_of(unsigned long long h, int pbits) {
 template <class K>
 __global__ __launch_bounds__(PS_NT) void pa_count_kernel(const K* __restrict__ keys, size_t n,
                                                          int pbits, size_t spans, int span_tiles,
-                                                         unsigned long long* __restrict__ cnt) {
+                                                         unsigned long long* __restrict__ cnt,
+                                                         const unsigned long long* __restrict__ seg_n,
+                                                         size_t seg_stride) {
   extern __shared__ unsigned lh[];
   const int P = 1 << pbits;
   for (int i = threadIdx.x; i < P; i += PS_NT) lh[i] = 0;
   __syncthreads();
   const unsigned lane = __lane_id();
-  const size_t r0 = size_t(blockIdx.x) * span_tiles * PS_TILE;
-  const size_t r1 = std::min(n, r0 + size_t(span_tiles) * PS_TILE);
+  const size_t r0 = seg_n ? size_t(blockIdx.x) * seg_stride : size_t(blockIdx.x) * span_tiles * PS_TILE;
+  const size_t r1 = seg_n ? r0 + seg_n[blockIdx.x] : std::min(n, r0 + size_t(span_tiles) * PS_TILE);
   for (size_t t0 = r0; t0 < r1; t0 += PS_TILE) {
 #pragma unroll 4
     for (int j = 0; j < PS_ITEMS; j++) {
@@ -208,7 +210,7 @@ template <class K>
 __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     const K* __restrict__ keys, size_t n, int pbits, size_t spans, int span_tiles, Cols cols,
     const unsigned long long* __restrict__ cnt, const unsigned long long* __restrict__ incl,
-    K* __restrict__ out_keys) {
+    K* __restrict__ out_keys, const unsigned long long* __restrict__ seg_n, size_t seg_stride) {
   extern __shared__ unsigned long long lds_u64[];
   const int P = 1 << pbits;
   unsigned long long* stage = lds_u64;                                     // [PS_TILE]
@@ -224,8 +226,8 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     gbase[q] = incl[m] - cnt[m];  // exclusive offset of (partition q, this span)
     pc[q] = 0;
   }
-  const size_t r0 = sp * span_tiles * PS_TILE;
-  const size_t r1 = std::min(n, r0 + size_t(span_tiles) * PS_TILE);
+  const size_t r0 = seg_n ? sp * seg_stride : sp * span_tiles * PS_TILE;
+  const size_t r1 = seg_n ? r0 + seg_n[sp] : std::min(n, r0 + size_t(span_tiles) * PS_TILE);
   const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   for (size_t t0 = r0; t0 < r1; t0 += PS_TILE) {
     __syncthreads();
@@ -638,10 +640,12 @@ __global__ __launch_bounds__(PA_NT) void pa_merge_kernel(
 // The heavy keys come from a sample (at most 80 % of the Sh slots); hot_keys
 // holds them at their two-choice slots (HA_EMPTY elsewhere).  Every workgroup walks a
 // contiguous chunk of HOT_CHUNK rows: a row with a heavy key is aggregated
-// into the workgroup's LDS table, any other row is appended to the cold
-// buffers (one global reservation per 4096-row step of the workgroup), which
-// then go through the partitioned path.  The per-workgroup tables share the
-// slot layout, so the merge is a column-wise reduction.
+// into the workgroup's LDS table, any other row is appended to the
+// workgroup's region of the cold buffers (rows r0 .. r0 + seg_n - 1 for a
+// chunk starting at row r0; each wave reserves its cold rows of a step with
+// one LDS atomic, so the loop has no barrier), and the regions then go
+// through the partitioned path as its spans.  The per-workgroup tables share
+// the slot layout, so the merge is a column-wise reduction.
 constexpr int HOT_NT = 1024;
 constexpr int HOT_UNROLL = 2;
 constexpr int HOT_STEP = HOT_NT * HOT_UNROLL;
@@ -677,10 +681,9 @@ template <class K, int NA>
 __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     const K* __restrict__ keys, size_t n, AccSet accs, const unsigned long long* __restrict__ hot_keys,
     int Sh, unsigned long long* __restrict__ partial, K* __restrict__ cold_keys, Cols cold,
-    unsigned long long* __restrict__ cold_n) {
+    unsigned long long* __restrict__ cold_n, unsigned long long* __restrict__ seg_n) {
   extern __shared__ unsigned long long lds_u64[];
-  __shared__ unsigned scan_tmp[HOT_NT / 64];
-  __shared__ unsigned long long cbase;
+  __shared__ unsigned cold_cnt;  // rows in this workgroup's cold region
   constexpr int na = NA;
   // the accumulator loops are unrolled for up to two accumulators; wider
   // sets keep the loop (unrolled, they hold too many values in registers)
@@ -697,10 +700,12 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     hc[i] = 0;
     for (int a = 0; a < na; a++) ha[size_t(a) * Sh + i] = acc_identity(accs.a[a].kind, accs.a[a].vtype);
   }
+  if (threadIdx.x == 0) cold_cnt = 0;
   __syncthreads();
   const unsigned tid = threadIdx.x, lane = __lane_id();
   const size_t r0 = size_t(blockIdx.x) * HOT_CHUNK;
   const size_t r1 = r0 + HOT_CHUNK < n ? r0 + HOT_CHUNK : n;
+  const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
   constexpr int NA1 = NA > 0 ? NA : 1;
   // the rows of the step starting at t (row r >= r1 reads nothing)
   auto load_step = [&](size_t t, K (&kk)[U], unsigned long long (&vals)[U][NA1]) {
@@ -718,13 +723,15 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     unsigned long long vv[U][NA1];
     load_step(t0, kv, vv);
     int sl[U];
+    unsigned long long cm[U];  // lanes of the wave with a cold row, per row of the step
     unsigned ncold = 0;
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const size_t r = t0 + size_t(u) * HOT_NT + tid;
       const bool in = r < r1;
       sl[u] = in ? hot_slot(hk, shift, key_bits(kv[u])) : -1;
-      if (in && sl[u] < 0) ncold++;
+      cm[u] = __ballot(in && sl[u] < 0);
+      ncold += unsigned(__popcll(cm[u]));
       const bool ok = sl[u] >= 0;
       const unsigned long long act = __ballot(ok);
       if (!act) continue;
@@ -744,32 +751,72 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
           acc_merge(&ha[size_t(a) * Sh + sl[u]], accs.a[a].kind, accs.a[a].vtype, vv[u][a]);
       }
     }
-    // cold rows of this step: one reservation for the workgroup
-    unsigned total = 0;
-    const unsigned off = block_excl_scan<HOT_NT>(ncold, scan_tmp, &total);
-    if (tid == 0) cbase = total ? atomicAdd(cold_n, (unsigned long long)total) : 0ull;
-    __syncthreads();
-    unsigned long long pos = cbase + off;
-    __syncthreads();  // cbase is rewritten by the next step
+    // cold rows of this step: one LDS reservation per wave in the
+    // workgroup's region (no barrier, so the waves drift apart freely)
+    if (ncold == 0) continue;
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&cold_cnt, ncold);
+    base = unsigned(__shfl(int(base), 0, 64));
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const size_t r = t0 + size_t(u) * HOT_NT + tid;
-      if (r >= r1 || sl[u] >= 0) continue;
-      cold_keys[pos] = kv[u];
+      if ((cm[u] >> lane) & 1ull) {
+        const size_t pos = r0 + base + unsigned(__popcll(cm[u] & lt));
+        cold_keys[pos] = kv[u];
 #pragma unroll
-      for (int a = 0; a < NA; a++) {
-        if (cold.bytes[a] == 4) static_cast<unsigned*>(cold.out[a])[pos] = unsigned(vv[u][a]);
-        else static_cast<unsigned long long*>(cold.out[a])[pos] = vv[u][a];
+        for (int a = 0; a < NA; a++) {
+          if (cold.bytes[a] == 4) static_cast<unsigned*>(cold.out[a])[pos] = unsigned(vv[u][a]);
+          else static_cast<unsigned long long*>(cold.out[a])[pos] = vv[u][a];
+        }
       }
-      pos++;
+      base += unsigned(__popcll(cm[u]));
     }
   }
   __syncthreads();
+  if (tid == 0) {
+    seg_n[blockIdx.x] = cold_cnt;
+    if (cold_cnt) atomicAdd(cold_n, (unsigned long long)cold_cnt);
+  }
   unsigned long long* out = partial + size_t(blockIdx.x) * size_t(1 + na) * Sh;
   for (int i = threadIdx.x; i < Sh; i += HOT_NT) {
     out[i] = hc[i];
     for (int a = 0; a < na; a++) out[size_t(1 + a) * Sh + i] = ha[size_t(a) * Sh + i];
   }
+}
+
+// First level of the hot-table reduction: workgroup (x, y) folds the slots
+// 64x .. 64x+63 of the per-workgroup tables y*per .. y*per+per-1 into table
+// y of `out` (same layout), so the final merge reads gridDim.y tables.
+__global__ __launch_bounds__(256) void hot_fold_kernel(
+    const unsigned long long* __restrict__ partial, int nparts, int per, int Sh, AccSet accs,
+    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long red[4][64][1 + HA_MAX_ACC];
+  const int na = accs.n;
+  const int slot = blockIdx.x * 64 + int(threadIdx.x & 63);
+  const int part = int(threadIdx.x >> 6);
+  const int g0 = int(blockIdx.y) * per, g1 = g0 + per < nparts ? g0 + per : nparts;
+  unsigned long long c = 0, acc[HA_MAX_ACC];
+#pragma unroll
+  for (int a = 0; a < HA_MAX_ACC; a++) acc[a] = a < na ? acc_identity(accs.a[a].kind, accs.a[a].vtype) : 0ull;
+  if (slot < Sh) {
+    for (int g = g0 + part; g < g1; g += 4) {
+      const unsigned long long* t = partial + size_t(g) * size_t(1 + na) * Sh;
+      c += t[slot];
+      for (int a = 0; a < na; a++)
+        acc[a] = acc_combine(acc[a], t[size_t(1 + a) * Sh + slot], accs.a[a].kind, accs.a[a].vtype);
+    }
+  }
+  red[part][threadIdx.x & 63][0] = c;
+  for (int a = 0; a < na; a++) red[part][threadIdx.x & 63][1 + a] = acc[a];
+  __syncthreads();
+  if (part != 0 || slot >= Sh) return;
+  for (int q = 1; q < 4; q++) {
+    c += red[q][threadIdx.x][0];
+    for (int a = 0; a < na; a++)
+      acc[a] = acc_combine(acc[a], red[q][threadIdx.x][1 + a], accs.a[a].kind, accs.a[a].vtype);
+  }
+  unsigned long long* t = out + size_t(blockIdx.y) * size_t(1 + na) * Sh;
+  t[slot] = c;
+  for (int a = 0; a < na; a++) t[size_t(1 + a) * Sh + slot] = acc[a];
 }
 
 // Column-wise reduction of the per-workgroup hot tables (in workgroup order);
@@ -828,47 +875,60 @@ __global__ void gather_group_keys(const unsigned long long* __restrict__ w, int 
     keys[g] = K(w[ent[g] * stride]);
 }
 
-// Writes one aggregate in sorted-group order.
+// The aggregations of one fused group-by, for the final write.
+struct EmitSet {
+  int n;
+  int op[16];
+  int vtype[16];
+  int slot[16];  // accumulator of the aggregation (count has none)
+  void* out[16];
+};
+
+// Writes every aggregate in sorted-group order (one thread per group reads
+// the group's entry once for all of them).
 __global__ void emit_kernel(const unsigned long long* __restrict__ w, int stride,
-                            const size_t* __restrict__ ent, size_t G, int op, int vtype, int slot,
-                            void* __restrict__ out) {
+                            const size_t* __restrict__ ent, size_t G, EmitSet es) {
   const size_t st = size_t(gridDim.x) * blockDim.x;
   for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < G; g += st) {
     const unsigned long long* e = w + ent[g] * stride;
     const unsigned long long cnt = e[1];
-    if (op == OP_COUNT) {
-      static_cast<unsigned long long*>(out)[g] = cnt;
-      continue;
-    }
-    const unsigned long long a = e[2 + slot];
-    if (op == OP_AVG) {
-      double s;
+    for (int i = 0; i < es.n; i++) {
+      const int op = es.op[i], vtype = es.vtype[i];
+      void* out = es.out[i];
+      if (op == OP_COUNT) {
+        static_cast<unsigned long long*>(out)[g] = cnt;
+        continue;
+      }
+      const unsigned long long a = e[2 + es.slot[i]];
+      if (op == OP_AVG) {
+        double s;
+        switch (vtype) {
+          case FHIP_I32: s = double(int(unsigned(a))); break;
+          case FHIP_U32: s = double(unsigned(a)); break;
+          case FHIP_I64: s = double((long long)a); break;
+          case FHIP_U64: s = double(a); break;
+          case FHIP_F32: { float f; unsigned b = unsigned(a); __builtin_memcpy(&f, &b, 4); s = double(f); break; }
+          default: __builtin_memcpy(&s, &a, 8); break;
+        }
+        static_cast<double*>(out)[g] = s / double(cnt);
+        continue;
+      }
       switch (vtype) {
-        case FHIP_I32: s = double(int(unsigned(a))); break;
-        case FHIP_U32: s = double(unsigned(a)); break;
-        case FHIP_I64: s = double((long long)a); break;
-        case FHIP_U64: s = double(a); break;
-        case FHIP_F32: { float f; unsigned b = unsigned(a); __builtin_memcpy(&f, &b, 4); s = double(f); break; }
-        default: __builtin_memcpy(&s, &a, 8); break;
+        case FHIP_I32: case FHIP_U32: static_cast<unsigned*>(out)[g] = unsigned(a); break;
+        case FHIP_F32: {
+          float f; unsigned b = unsigned(a); __builtin_memcpy(&f, &b, 4);
+          if (op == OP_SUM) f = 0.0f + f;
+          static_cast<float*>(out)[g] = f;
+          break;
+        }
+        case FHIP_F64: {
+          double d; __builtin_memcpy(&d, &a, 8);
+          if (op == OP_SUM) d = 0.0 + d;
+          static_cast<double*>(out)[g] = d;
+          break;
+        }
+        default: static_cast<unsigned long long*>(out)[g] = a; break;
       }
-      static_cast<double*>(out)[g] = s / double(cnt);
-      continue;
-    }
-    switch (vtype) {
-      case FHIP_I32: case FHIP_U32: static_cast<unsigned*>(out)[g] = unsigned(a); break;
-      case FHIP_F32: {
-        float f; unsigned b = unsigned(a); __builtin_memcpy(&f, &b, 4);
-        if (op == OP_SUM) f = 0.0f + f;
-        static_cast<float*>(out)[g] = f;
-        break;
-      }
-      case FHIP_F64: {
-        double d; __builtin_memcpy(&d, &a, 8);
-        if (op == OP_SUM) d = 0.0 + d;
-        static_cast<double*>(out)[g] = d;
-        break;
-      }
-      default: static_cast<unsigned long long*>(out)[g] = a; break;
     }
   }
 }
@@ -880,16 +940,16 @@ int grid_for(size_t n, int per_cu = 8) {
 template <class K>
 void launch_pa(const void* keys, size_t n, int pbits, size_t spans, int span_tiles, const Cols& cols,
                unsigned long long* cnt, unsigned long long* incl, void* pkeys, hipStream_t st,
-               int pass) {
+               int pass, const unsigned long long* seg_n, size_t seg_stride) {
   const int P = 1 << pbits;
   if (pass == 1) {
     pa_count_kernel<K><<<unsigned(spans), PS_NT, P * 4, st>>>((const K*)keys, n, pbits, spans,
-                                                               span_tiles, cnt);
+                                                               span_tiles, cnt, seg_n, seg_stride);
   } else {
     const size_t lds = size_t(PS_TILE) * 8 + size_t(P) * 16 + size_t(PS_TILE) * 2;
     pa_scatter_kernel<K><<<unsigned(spans), PS_NT, lds, st>>>((const K*)keys, n, pbits, spans,
                                                              span_tiles, cols,
-                                                             cnt, incl, (K*)pkeys);
+                                                             cnt, incl, (K*)pkeys, seg_n, seg_stride);
   }
   FHIP_CHECK(hipGetLastError());
 }
@@ -903,14 +963,74 @@ void grouped_agg(int op, int vtype, const void* col, size_t col_n, const size_t*
                  const size_t* split, size_t nsplit, const size_t* nulls, size_t nn, void* out,
                  hipStream_t st);
 
-// More groups than the partitioned tables hold: sort-based group_by plus
-// per-column aggregation (same results).
+// Heavy-key candidates.  The keys of an evenly spaced sample of m rows are
+// counted in an open-addressing table of 2m slots (linear probing from a
+// mixed hash; at most half the slots fill, so every probe ends), then the
+// keys seen at least HOT_MIN_SEEN times are listed as (count, key) pairs
+// after a leading entry count.  The list is in arrival order; the host sorts
+// it by (count descending, key ascending), which fixes the plan.
+constexpr unsigned HOT_MIN_SEEN = 4;
+
+__global__ void hot_count_init_kernel(unsigned long long* __restrict__ tk, unsigned* __restrict__ tc,
+                                      unsigned size, unsigned long long* __restrict__ list) {
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
+    tk[i] = HA_EMPTY;
+    tc[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) list[0] = 0;
+}
+
 template <class K>
-__global__ void sample_bits_kernel(const K* __restrict__ keys, size_t step, size_t m,
-                                   unsigned long long* __restrict__ out) {
+__global__ void hot_count_kernel(const K* __restrict__ keys, size_t step, size_t m,
+                                 unsigned long long* __restrict__ tk, unsigned* __restrict__ tc,
+                                 unsigned mask) {
   const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride)
-    out[i] = key_bits(keys[i * step]);
+  const unsigned lane = __lane_id();
+  // the loop runs a whole number of times for every lane of a wave, so the
+  // ballots below see all of them
+  const size_t m_up = (m + stride - 1) / stride * stride;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m_up; i += stride) {
+    const unsigned long long kb = i < m ? key_bits(keys[i * step]) : HA_EMPTY;
+    // the empty marker is never a heavy key
+    const bool valid = kb != HA_EMPTY;
+    unsigned h = unsigned(mix64(kb)) & mask;
+    while (valid) {
+      // a slot's key never changes once set: most lanes of a skewed sample
+      // find theirs with a plain read and leave the CAS to the first one
+      const unsigned long long cur = __hip_atomic_load(&tk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == kb) break;
+      if (cur == HA_EMPTY) {
+        const unsigned long long prev = atomicCAS(&tk[h], HA_EMPTY, kb);
+        if (prev == HA_EMPTY || prev == kb) break;
+      }
+      h = (h + 1) & mask;
+    }
+    // lanes with the same slot add their count with one atomic
+    unsigned long long pending = __ballot(valid);
+    while (pending) {
+      const int leader = __builtin_ffsll((long long)pending) - 1;
+      const unsigned hs = unsigned(__shfl(int(h), leader, 64));
+      const unsigned long long same = __ballot(valid && h == hs) & pending;
+      if (lane == unsigned(leader)) atomicAdd(&tc[hs], unsigned(__popcll(same)));
+      pending &= ~same;
+    }
+  }
+}
+
+__global__ void hot_list_kernel(const unsigned long long* __restrict__ tk,
+                                const unsigned* __restrict__ tc, unsigned size,
+                                unsigned long long* __restrict__ list, unsigned cap) {
+  const unsigned stride = gridDim.x * blockDim.x;
+  for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < size; s += stride) {
+    const unsigned c = tc[s];
+    if (c < HOT_MIN_SEEN) continue;
+    const unsigned long long j = atomicAdd(&list[0], 1ull);
+    if (j < cap) {
+      list[1 + 2 * j] = c;
+      list[2 + 2 * j] = tk[s];
+    }
+  }
 }
 
 // The heavy keys of a skewed key column: keys seen at least 4 times in an
@@ -936,26 +1056,33 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   int Sh = int(std::min<long long>(want, S));
   while (Sh & (Sh - 1)) Sh &= Sh - 1;
   const size_t step = n / m;
-  DBuf sb(m * 8);
+  const unsigned tsize = unsigned(2 * m), cap = unsigned(m / HOT_MIN_SEEN);
+  DBuf tk(size_t(tsize) * 8), tc(size_t(tsize) * 4), lb((1 + 2 * size_t(cap)) * 8);
+  unsigned long long* list = lb.as<unsigned long long>();
+  hot_count_init_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tk.as<unsigned long long>(), tc.as<unsigned>(), tsize, list);
   switch (ktype) {
-    case FHIP_I32: sample_bits_kernel<int32_t><<<grid_for_n(m), 256, 0, st>>>((const int32_t*)keys, step, m, sb.as<unsigned long long>()); break;
-    case FHIP_U32: sample_bits_kernel<uint32_t><<<grid_for_n(m), 256, 0, st>>>((const uint32_t*)keys, step, m, sb.as<unsigned long long>()); break;
-    case FHIP_I64: sample_bits_kernel<int64_t><<<grid_for_n(m), 256, 0, st>>>((const int64_t*)keys, step, m, sb.as<unsigned long long>()); break;
-    default: sample_bits_kernel<uint64_t><<<grid_for_n(m), 256, 0, st>>>((const uint64_t*)keys, step, m, sb.as<unsigned long long>()); break;
+    case FHIP_I32: hot_count_kernel<int32_t><<<grid_for_n(m), 256, 0, st>>>((const int32_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    case FHIP_U32: hot_count_kernel<uint32_t><<<grid_for_n(m), 256, 0, st>>>((const uint32_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    case FHIP_I64: hot_count_kernel<int64_t><<<grid_for_n(m), 256, 0, st>>>((const int64_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    default: hot_count_kernel<uint64_t><<<grid_for_n(m), 256, 0, st>>>((const uint64_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
   }
+  hot_list_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tk.as<unsigned long long>(), tc.as<unsigned>(), tsize, list, cap);
   FHIP_CHECK(hipGetLastError());
-  // sorted on the device (the runs are counted on the host)
-  sort_pairs_bits(8, sb.get(), sb.get(), 0, nullptr, nullptr, m, 64, st);
-  std::vector<unsigned long long> smp(m);
-  FHIP_CHECK(hipMemcpyAsync(smp.data(), sb.get(), m * 8, hipMemcpyDeviceToHost, st));
+  // the count and the first 4096 candidates in one copy (skewed data has
+  // about a thousand), the rest only when there are more
+  const size_t first = std::min<size_t>(cap, 4096);
+  std::vector<unsigned long long> lst(1 + 2 * first);
+  FHIP_CHECK(hipMemcpyAsync(lst.data(), list, lst.size() * 8, hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
-  std::vector<std::pair<size_t, unsigned long long>> runs;  // (count, key)
-  for (size_t i = 0; i < m;) {
-    size_t j = i + 1;
-    while (j < m && smp[j] == smp[i]) j++;
-    if (j - i >= 4 && smp[i] != HA_EMPTY) runs.emplace_back(j - i, smp[i]);
-    i = j;
+  const size_t nc = std::min<size_t>(size_t(lst[0]), cap);
+  if (nc > first) {
+    lst.resize(1 + 2 * nc);
+    FHIP_CHECK(hipMemcpyAsync(lst.data() + 1 + 2 * first, list + 1 + 2 * first,
+                              2 * (nc - first) * 8, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
   }
+  std::vector<std::pair<size_t, unsigned long long>> runs(nc);  // (count, key)
+  for (size_t j = 0; j < nc; j++) runs[j] = {size_t(lst[1 + 2 * j]), lst[2 + 2 * j]};
   std::sort(runs.begin(), runs.end(), [](const std::pair<size_t, unsigned long long>& a,
                                          const std::pair<size_t, unsigned long long>& b) {
     return a.first > b.first || (a.first == b.first && a.second < b.second);
@@ -999,6 +1126,8 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   return h;
 }
 
+// More groups than the partitioned tables hold: sort-based group_by plus
+// per-column aggregation (same results).
 void group_by_agg_sorted(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
                          void* keys_out, size_t capacity, size_t* ngroups, hipStream_t st) {
   const int kb = type_bytes(ktype);
@@ -1086,11 +1215,20 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     attr_set = true;
   }
   // ---- heavy keys: aggregated in one pass, the other rows go on ----------
-  HotPlan hp = plan_hot(ktype, keys, n, S, st);
+  HotPlan hp;
+  {
+    ProfScope prof("groupby_plan", st);
+    hp = plan_hot(ktype, keys, n, S, st);
+  }
   const void* ckeys = keys;  // the rows the partitioned path sees
   size_t cn = n;
   AccSet caccs = accs;
-  DBuf hot_tab, cold_k, cold_c[HA_MAX_ACC], hot_part, hot_cnt, hot_fin;
+  DBuf hot_tab, cold_k, cold_c[HA_MAX_ACC], hot_part, hot_cnt, hot_fin, hot_seg;
+  // with the heavy-key pass, the cold rows of hot workgroup b sit at rows
+  // b * HOT_CHUNK .. b * HOT_CHUNK + seg[b] - 1 of the cold buffers, and the
+  // partitioned path takes each such region as one span
+  const unsigned long long* seg_n = nullptr;
+  size_t seg_spans = 0;
   unsigned hot_groups = 0;
   if (hp.used) {
     const int Sh = hp.Sh;
@@ -1130,6 +1268,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     hot_part.reset(nparts * size_t(1 + accs.n) * Sh * 8);
     hot_cnt.reset(16);
     FHIP_CHECK(hipMemsetAsync(hot_cnt.get(), 0, 16, st));
+    hot_seg.reset(nparts * 8);
     unsigned long long* cold_n = hot_cnt.as<unsigned long long>();
     unsigned* hot_n = reinterpret_cast<unsigned*>(cold_n + 1);
     const size_t lds = size_t(2 + accs.n) * Sh * 8;
@@ -1141,12 +1280,13 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
         KK* ck = cold_k.as<KK>();
         unsigned long long* ht = hot_tab.as<unsigned long long>();
         unsigned long long* hpart = hot_part.as<unsigned long long>();
+        unsigned long long* sg = hot_seg.as<unsigned long long>();
         switch (accs.n) {
-          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
-          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
-          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
-          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
-          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n); break;
+          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
+          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
+          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
+          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
+          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
         }
       };
       switch (ktype) {
@@ -1158,10 +1298,16 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       FHIP_CHECK(hipGetLastError());
     }
     hot_fin.reset(size_t(S + 1) * stride * 8);
+    // two levels: groups of 16 tables, then the groups' tables
+    constexpr int per = 16;
+    const int ng = int(ceil_div(nparts, size_t(per)));
+    DBuf fold(size_t(ng) * size_t(1 + accs.n) * Sh * 8);
     {
       ProfScope prof("groupby_hot_merge", st);
+      hot_fold_kernel<<<dim3(unsigned(Sh / 64), unsigned(ng)), 256, 0, st>>>(
+          hot_part.as<unsigned long long>(), int(nparts), per, Sh, accs, fold.as<unsigned long long>());
       hot_merge_kernel<<<unsigned(Sh / 64), 256, 0, st>>>(
-          hot_part.as<unsigned long long>(), int(nparts), Sh, accs, hot_tab.as<unsigned long long>(),
+          fold.as<unsigned long long>(), ng, Sh, accs, hot_tab.as<unsigned long long>(),
           stride, hot_fin.as<unsigned long long>(), hot_n);
       FHIP_CHECK(hipGetLastError());
     }
@@ -1172,6 +1318,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
     ckeys = cold_k.get();
     for (int a = 0; a < accs.n; a++) caccs.a[a].col = cold_c[a].get();
+    seg_n = hot_seg.as<unsigned long long>();
+    seg_spans = nparts;
   }
 
   // Final assembly over Ptot partitions' tables (the heavy keys are the
@@ -1205,10 +1353,16 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     // keys are distinct, so the stable sort gives the reference's ascending order
     sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st);
     FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
-    for (int i = 0; i < naggs; i++) {
-      emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, aggs[i].op,
-                                               aggs[i].vtype, slot_of[i] < 0 ? 0 : slot_of[i],
-                                               aggs[i].out);
+    if (naggs > 0) {
+      EmitSet es{};
+      es.n = naggs;
+      for (int i = 0; i < naggs; i++) {
+        es.op[i] = aggs[i].op;
+        es.vtype[i] = aggs[i].vtype;
+        es.slot[i] = slot_of[i] < 0 ? 0 : slot_of[i];
+        es.out[i] = aggs[i].out;
+      }
+      emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, es);
       FHIP_CHECK(hipGetLastError());
     }
     FHIP_CHECK(hipStreamSynchronize(st));  // foff is a host local
@@ -1229,7 +1383,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   for (;;) {
     const int P = 1 << pbits;
     const int Ptot = P + (hp.used ? 1 : 0);
-    const size_t tiles = ceil_div(ceil_div(cn, PS_TILE), size_t(span_tiles));  // spans
+    const size_t tiles = seg_n ? seg_spans : ceil_div(ceil_div(cn, PS_TILE), size_t(span_tiles));  // spans
     DBuf cb(size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int));
     unsigned long long* cnt = cb.as<unsigned long long>();
     unsigned long long* incl = ib.as<unsigned long long>();
@@ -1246,10 +1400,10 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     }
     auto run = [&](int pass) {
       switch (ktype) {
-        case FHIP_I32: launch_pa<int32_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        case FHIP_U32: launch_pa<uint32_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        case FHIP_I64: launch_pa<int64_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
-        default: launch_pa<uint64_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass); break;
+        case FHIP_I32: launch_pa<int32_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass, seg_n, HOT_CHUNK); break;
+        case FHIP_U32: launch_pa<uint32_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass, seg_n, HOT_CHUNK); break;
+        case FHIP_I64: launch_pa<int64_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass, seg_n, HOT_CHUNK); break;
+        default: launch_pa<uint64_t>(ckeys, cn, pbits, tiles, span_tiles, cols, cnt, incl, pkeys.get(), st, pass, seg_n, HOT_CHUNK); break;
       }
     };
     {
@@ -1260,7 +1414,11 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     // partition sizes: inclusive offset at the last span of every partition
     std::vector<unsigned long long> ends(P);
     copy_strided_to_host(incl + (tiles - 1), tiles, size_t(P), ends.data(), st);
-    // chunks of at most `chunk` rows per partition
+    {
+      ProfScope prof("groupby_part_scatter", st);
+      run(2);
+    }
+    // chunks of at most `chunk` rows per partition (built while the scatter runs)
     std::vector<Item> items;
     std::vector<unsigned> part_items(P + 1);
     unsigned long long run_off = 0;
@@ -1273,10 +1431,6 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       run_off = e;
     }
     part_items[P] = unsigned(items.size());
-    {
-      ProfScope prof("groupby_part_scatter", st);
-      run(2);
-    }
     const size_t NI = items.size();
     DBuf itb(NI * sizeof(Item)), pib((P + 1) * sizeof(unsigned));
     FHIP_CHECK(hipMemcpyAsync(itb.get(), items.data(), NI * sizeof(Item), hipMemcpyHostToDevice, st));
