@@ -34,8 +34,12 @@ constexpr int HA_MAX_ACC = 4;
 constexpr int PA_NT = 1024;           // threads of the aggregate / merge kernels (one per CU: LDS)
 constexpr int PA_UNROLL = 4;          // rows in flight per thread (aggregate)
 constexpr int PS_NT = 1024;           // threads of the count / scatter kernels
-constexpr int PS_ITEMS = 4;           // rows per thread
-constexpr int PS_TILE = PS_NT * PS_ITEMS;  // rows per scatter tile
+constexpr int PS_ITEMS = 4;           // rows per thread of the count kernel
+constexpr int PS_TILE = PS_NT * PS_ITEMS;  // rows per count tile; spans are whole count tiles
+// rows per thread of the scatter kernel: 12 (12288-row tiles, longer runs
+// per partition) while the partitions leave LDS for it, else 4
+constexpr int PS_ITEMS_WIDE = 12;
+constexpr int PS_WIDE_MAX_P = 2048;
 constexpr int PS_SPAN = 16;            // default tiles per count / scatter workgroup (a "span")
 constexpr unsigned long long PA_CHUNK = 1ull << 17;  // rows per aggregate workgroup
 constexpr int PA_SLOTS = 2048;        // LDS table slots (power of two)
@@ -200,24 +204,29 @@ struct Cols {
   int n;
 };
 
-// One span of span_tiles tiles of PS_TILE rows, tile by tile: rows are ranked
+// One span (span_tiles * PS_TILE rows, or a cold region) in tiles of
+// PS_NT * ITEMS rows, tile by tile: rows are ranked
 // by partition in LDS, staged in partition order and written out column by
 // column in runs, so consecutive lanes store consecutive addresses.
 // incl/cnt: inclusive scan and counts of the partition-major (partition,
 // span) count matrix (exact offsets, no global atomics); the running offset
 // of every partition advances by each tile's count.
-template <class K>
+template <class K, int ITEMS>
 __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     const K* __restrict__ keys, size_t n, int pbits, size_t spans, int span_tiles, Cols cols,
     const unsigned long long* __restrict__ cnt, const unsigned long long* __restrict__ incl,
     K* __restrict__ out_keys, const unsigned long long* __restrict__ seg_n, size_t seg_stride) {
   extern __shared__ unsigned long long lds_u64[];
+  constexpr int TILE = PS_NT * ITEMS;
+  // value columns held in registers from the start of a tile (the others
+  // are loaded when their turn to be staged comes)
+  constexpr int PRELOAD = ITEMS <= 4 ? 2 : 0;
   const int P = 1 << pbits;
-  unsigned long long* stage = lds_u64;                                     // [PS_TILE]
-  unsigned long long* gbase = stage + PS_TILE;                             // [P]
+  unsigned long long* stage = lds_u64;                                     // [TILE]
+  unsigned long long* gbase = stage + TILE;                                // [P]
   unsigned* pc = reinterpret_cast<unsigned*>(gbase + P);                   // [P]
   unsigned* ps = pc + P;                                                   // [P]
-  unsigned short* pid = reinterpret_cast<unsigned short*>(ps + P);         // [PS_TILE]
+  unsigned short* pid = reinterpret_cast<unsigned short*>(ps + P);         // [TILE]
   __shared__ unsigned scan_tmp[PS_NT / 64];
   const unsigned tid = threadIdx.x, lane = __lane_id();
   const size_t sp = blockIdx.x;
@@ -229,24 +238,24 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
   const size_t r0 = seg_n ? sp * seg_stride : sp * span_tiles * PS_TILE;
   const size_t r1 = seg_n ? r0 + seg_n[sp] : std::min(n, r0 + size_t(span_tiles) * PS_TILE);
   const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (size_t t0 = r0; t0 < r1; t0 += PS_TILE) {
+  for (size_t t0 = r0; t0 < r1; t0 += TILE) {
     __syncthreads();
-    const unsigned tn = unsigned(r1 - t0 < size_t(PS_TILE) ? r1 - t0 : size_t(PS_TILE));
-    unsigned part[PS_ITEMS];
-    unsigned short slot[PS_ITEMS];
-    // the key and the first two columns of the tile are loaded together
-    K kreg[PS_ITEMS];
-    unsigned long long creg[2][PS_ITEMS];
+    const unsigned tn = unsigned(r1 - t0 < size_t(TILE) ? r1 - t0 : size_t(TILE));
+    unsigned part[ITEMS];
+    unsigned short slot[ITEMS];
+    // the key and the first PRELOAD columns of the tile are loaded together
+    K kreg[ITEMS];
+    unsigned long long creg[PRELOAD > 0 ? PRELOAD : 1][ITEMS];
 #pragma unroll
-    for (int j = 0; j < PS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
       kreg[j] = r < tn ? keys[t0 + r] : K(0);
     }
 #pragma unroll
-    for (int a = 0; a < 2; a++) {
+    for (int a = 0; a < PRELOAD; a++) {
       if (a >= cols.n) break;
 #pragma unroll
-      for (int j = 0; j < PS_ITEMS; j++) {
+      for (int j = 0; j < ITEMS; j++) {
         const unsigned r = unsigned(j) * PS_NT + tid;
         creg[a][j] = r >= tn ? 0ull
                      : cols.bytes[a] == 4
@@ -255,13 +264,13 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
       }
     }
 #pragma unroll
-    for (int j = 0; j < PS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
       part[j] = r < tn ? part_of(mix64(key_bits(kreg[j])), pbits) : 0u;
     }
     // counts of this tile per partition
 #pragma unroll
-    for (int j = 0; j < PS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
       const bool ok = r < tn;
       const unsigned long long act = __ballot(ok);
@@ -295,7 +304,7 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     __syncthreads();
     // rank within (tile, partition)
 #pragma unroll
-    for (int j = 0; j < PS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
       const bool ok = r < tn;
       const unsigned long long act = __ballot(ok);
@@ -315,7 +324,7 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     // keys: stage in partition order, then write runs; every thread keeps
     // the destinations of its staged positions for all the columns
 #pragma unroll
-    for (int j = 0; j < PS_ITEMS; j++) {
+    for (int j = 0; j < ITEMS; j++) {
       const unsigned r = unsigned(j) * PS_NT + tid;
       if (r < tn) {
         stage[slot[j]] = (unsigned long long)kreg[j];
@@ -323,9 +332,9 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
       }
     }
     __syncthreads();
-    unsigned long long dst[PS_ITEMS];
+    unsigned long long dst[ITEMS];
 #pragma unroll
-    for (int k = 0; k < PS_ITEMS; k++) {
+    for (int k = 0; k < ITEMS; k++) {
       const unsigned i = unsigned(k) * PS_NT + tid;
       dst[k] = 0;
       if (i < tn) {
@@ -337,11 +346,10 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
     for (int a = 0; a < cols.n; a++) {
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < PS_ITEMS; j++) {
+      for (int j = 0; j < ITEMS; j++) {
         const unsigned r = unsigned(j) * PS_NT + tid;
         if (r < tn)
-          stage[slot[j]] = a == 0 ? creg[0][j]
-                           : a == 1 ? creg[1][j]
+          stage[slot[j]] = a < PRELOAD ? creg[a < PRELOAD ? a : 0][j]
                            : cols.bytes[a] == 4
                                ? (unsigned long long)static_cast<const unsigned*>(cols.in[a])[t0 + r]
                                : static_cast<const unsigned long long*>(cols.in[a])[t0 + r];
@@ -350,14 +358,14 @@ __global__ __launch_bounds__(PS_NT) void pa_scatter_kernel(
       if (cols.bytes[a] == 4) {
         unsigned* o = static_cast<unsigned*>(cols.out[a]);
 #pragma unroll
-        for (int k = 0; k < PS_ITEMS; k++) {
+        for (int k = 0; k < ITEMS; k++) {
           const unsigned i = unsigned(k) * PS_NT + tid;
           if (i < tn) o[dst[k]] = unsigned(stage[i]);
         }
       } else {
         unsigned long long* o = static_cast<unsigned long long*>(cols.out[a]);
 #pragma unroll
-        for (int k = 0; k < PS_ITEMS; k++) {
+        for (int k = 0; k < ITEMS; k++) {
           const unsigned i = unsigned(k) * PS_NT + tid;
           if (i < tn) o[dst[k]] = stage[i];
         }
@@ -647,7 +655,7 @@ __global__ __launch_bounds__(PA_NT) void pa_merge_kernel(
 // through the partitioned path as its spans.  The per-workgroup tables share
 // the slot layout, so the merge is a column-wise reduction.
 constexpr int HOT_NT = 1024;
-constexpr int HOT_UNROLL = 2;
+constexpr int HOT_UNROLL = 1;
 constexpr int HOT_STEP = HOT_NT * HOT_UNROLL;
 constexpr size_t HOT_CHUNK = size_t(1) << 18;
 
@@ -946,10 +954,16 @@ void launch_pa(const void* keys, size_t n, int pbits, size_t spans, int span_til
     pa_count_kernel<K><<<unsigned(spans), PS_NT, P * 4, st>>>((const K*)keys, n, pbits, spans,
                                                                span_tiles, cnt, seg_n, seg_stride);
   } else {
-    const size_t lds = size_t(PS_TILE) * 8 + size_t(P) * 16 + size_t(PS_TILE) * 2;
-    pa_scatter_kernel<K><<<unsigned(spans), PS_NT, lds, st>>>((const K*)keys, n, pbits, spans,
-                                                             span_tiles, cols,
-                                                             cnt, incl, (K*)pkeys, seg_n, seg_stride);
+    if (P <= PS_WIDE_MAX_P) {
+      const size_t tile = size_t(PS_NT) * PS_ITEMS_WIDE;
+      const size_t lds = tile * 8 + size_t(P) * 16 + tile * 2;
+      pa_scatter_kernel<K, PS_ITEMS_WIDE><<<unsigned(spans), PS_NT, lds, st>>>(
+          (const K*)keys, n, pbits, spans, span_tiles, cols, cnt, incl, (K*)pkeys, seg_n, seg_stride);
+    } else {
+      const size_t lds = size_t(PS_TILE) * 8 + size_t(P) * 16 + size_t(PS_TILE) * 2;
+      pa_scatter_kernel<K, PS_ITEMS><<<unsigned(spans), PS_NT, lds, st>>>(
+          (const K*)keys, n, pbits, spans, span_tiles, cols, cnt, incl, (K*)pkeys, seg_n, seg_stride);
+    }
   }
   FHIP_CHECK(hipGetLastError());
 }
@@ -1208,10 +1222,16 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     set_agg(uint64_t());
     FHIP_CHECK(hipFuncSetAttribute((const void*)pa_merge_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
     const int ms = int(size_t(PS_TILE) * 10 + (size_t(1) << PA_MAX_PBITS) * 16);
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<int64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
-    FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+    const int mw = int(size_t(PS_NT) * PS_ITEMS_WIDE * 10 + size_t(PS_WIDE_MAX_P) * 16);
+    auto set_scatter = [&](auto kz) {
+      using KK = decltype(kz);
+      FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<KK, PS_ITEMS>, hipFuncAttributeMaxDynamicSharedMemorySize, ms));
+      FHIP_CHECK(hipFuncSetAttribute((const void*)pa_scatter_kernel<KK, PS_ITEMS_WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize, mw));
+    };
+    set_scatter(int32_t());
+    set_scatter(uint32_t());
+    set_scatter(int64_t());
+    set_scatter(uint64_t());
     attr_set = true;
   }
   // ---- heavy keys: aggregated in one pass, the other rows go on ----------

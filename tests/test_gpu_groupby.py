@@ -172,6 +172,23 @@ def test_group_by_agg_table_growth_and_special_keys(fa):
     assert np.array_equal(mn.numpy(), oracle.grouped_agg("min", v, gi, split))
 
 
+def test_group_by_agg_many_partitions(fa):
+    # a table hint of 2^23 groups asks for 4096 partitions, beyond what the
+    # wide scatter tile leaves LDS for: the narrow tile takes over
+    rng = np.random.default_rng(25)
+    n = 1 << 20
+    k = rng.integers(0, 1 << 21, n).astype(np.int64)
+    v = rng.integers(-1000, 1000, n).astype(np.int64)
+    gi, split, keys = oracle.group_by(k)
+    dk, dv = fa.DeviceArray.from_numpy(k), fa.DeviceArray.from_numpy(v)
+    gk, (s, c, mx) = fa.group_by_agg(dk, [("sum", dv), ("count", None), ("max", dv)],
+                                     table_hint=1 << 23)
+    assert np.array_equal(gk.numpy(), keys)
+    assert np.array_equal(s.numpy(), oracle.grouped_agg("sum", v, gi, split))
+    assert np.array_equal(c.numpy(), oracle.grouped_agg("count", v, gi, split))
+    assert np.array_equal(mx.numpy(), oracle.grouped_agg("max", v, gi, split))
+
+
 def test_group_by_agg_full_size_cross_check(fa):
     # 2^26 Zipf(1.1) rows generated on the device: the hash path and the sort
     # path are independent implementations; keys/counts/max must agree
