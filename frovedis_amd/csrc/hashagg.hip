@@ -658,6 +658,7 @@ constexpr int HOT_NT = 1024;
 constexpr int HOT_UNROLL = 1;
 constexpr int HOT_STEP = HOT_NT * HOT_UNROLL;
 constexpr size_t HOT_CHUNK = size_t(1) << 18;
+constexpr int HOT_MAX_LDS = 159 * 1024;  // dynamic LDS of hot_agg_kernel
 
 // The heavy-key table is two-choice (cuckoo) hashed with buckets of two
 // slots: a key sits in one of the slots 2*b1, 2*b1+1, 2*b2, 2*b2+1, the
@@ -689,7 +690,8 @@ template <class K, int NA>
 __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     const K* __restrict__ keys, size_t n, AccSet accs, const unsigned long long* __restrict__ hot_keys,
     int Sh, unsigned long long* __restrict__ partial, K* __restrict__ cold_keys, Cols cold,
-    unsigned long long* __restrict__ cold_n, unsigned long long* __restrict__ seg_n) {
+    unsigned long long* __restrict__ cold_n, unsigned long long* __restrict__ seg_n, int cpbits,
+    unsigned long long* __restrict__ ccnt) {
   extern __shared__ unsigned long long lds_u64[];
   __shared__ unsigned cold_cnt;  // rows in this workgroup's cold region
   constexpr int na = NA;
@@ -703,11 +705,16 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
   unsigned long long* hk = lds_u64;        // [Sh]
   unsigned long long* hc = hk + Sh;        // [Sh]
   unsigned long long* ha = hc + Sh;        // [na][Sh]
+  // with ccnt: the cold rows' partition counts (2^cpbits partitions) of this
+  // workgroup's region, for the partitioned path (it then skips its count pass)
+  unsigned* chist = reinterpret_cast<unsigned*>(ha + size_t(na) * Sh);  // [2^cpbits]
+  const int CP = ccnt ? 1 << cpbits : 0;
   for (int i = threadIdx.x; i < Sh; i += HOT_NT) {
     hk[i] = hot_keys[i];
     hc[i] = 0;
     for (int a = 0; a < na; a++) ha[size_t(a) * Sh + i] = acc_identity(accs.a[a].kind, accs.a[a].vtype);
   }
+  for (int q = threadIdx.x; q < CP; q += HOT_NT) chist[q] = 0;
   if (threadIdx.x == 0) cold_cnt = 0;
   __syncthreads();
   const unsigned tid = threadIdx.x, lane = __lane_id();
@@ -769,6 +776,7 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     for (int u = 0; u < U; u++) {
       if ((cm[u] >> lane) & 1ull) {
         const size_t pos = r0 + base + unsigned(__popcll(cm[u] & lt));
+        if (CP) atomicAdd(&chist[part_of(mix64(key_bits(kv[u])), cpbits)], 1u);
         cold_keys[pos] = kv[u];
 #pragma unroll
         for (int a = 0; a < NA; a++) {
@@ -784,6 +792,8 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     seg_n[blockIdx.x] = cold_cnt;
     if (cold_cnt) atomicAdd(cold_n, (unsigned long long)cold_cnt);
   }
+  // partition-major, like the count kernel's matrix (one span per workgroup)
+  for (int q = tid; q < CP; q += HOT_NT) ccnt[size_t(q) * gridDim.x + blockIdx.x] = chist[q];
   unsigned long long* out = partial + size_t(blockIdx.x) * size_t(1 + na) * Sh;
   for (int i = threadIdx.x; i < Sh; i += HOT_NT) {
     out[i] = hc[i];
@@ -1234,6 +1244,15 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     set_scatter(uint64_t());
     attr_set = true;
   }
+  // partitions of the cold path: about S * fill_pct / 100 expected groups
+  // each; with a table hint the count is known before the heavy-key pass,
+  // which then also counts the cold rows per partition
+  const size_t per_part = std::max<size_t>(1, size_t(S) * size_t(fill_pct) / 100);
+  auto pbits_for = [&](size_t want) {
+    int pb = 4;
+    while (pb < PA_MAX_PBITS && (size_t(1) << pb) * per_part < want) pb++;
+    return pb;
+  };
   // ---- heavy keys: aggregated in one pass, the other rows go on ----------
   HotPlan hp;
   {
@@ -1243,7 +1262,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   const void* ckeys = keys;  // the rows the partitioned path sees
   size_t cn = n;
   AccSet caccs = accs;
-  DBuf hot_tab, cold_k, cold_c[HA_MAX_ACC], hot_part, hot_cnt, hot_fin, hot_seg;
+  DBuf hot_tab, cold_k, cold_c[HA_MAX_ACC], hot_part, hot_cnt, hot_fin, hot_seg, hot_ccnt;
+  int hot_cpbits = -1;  // partition bits of the cold-row counts made by the heavy-key pass
   // with the heavy-key pass, the cold rows of hot workgroup b sit at rows
   // b * HOT_CHUNK .. b * HOT_CHUNK + seg[b] - 1 of the cold buffers, and the
   // partitioned path takes each such region as one span
@@ -1254,8 +1274,9 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     const int Sh = hp.Sh;
     static bool hot_attr = false;
     if (!hot_attr) {
-      // (2 + accumulators) words per slot: at most 4 x 4096 or 6 x 2048 words
-      const int mx = int(size_t(4) * 2 * PA_SLOTS * 8);
+      // (2 + accumulators) words per slot (at most 4 x 4096 or 6 x 2048
+      // words), plus the cold-row partition counts
+      const int mx = HOT_MAX_LDS;
       auto set_all = [&](auto kz) {
         using KK = decltype(kz);
         FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
@@ -1270,7 +1291,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       set_all(uint64_t());
       hot_attr = true;
     }
-    FHIP_REQUIRE(size_t(2 + accs.n) * Sh * 8 <= size_t(160) * 1024, FHIP_ERR_ARG,
+    FHIP_REQUIRE(size_t(2 + accs.n) * Sh * 8 <= size_t(HOT_MAX_LDS), FHIP_ERR_ARG,
                  "group_by_agg: heavy-key table exceeds LDS");
     hot_tab.reset(size_t(Sh) * 8);
     FHIP_CHECK(hipMemcpyAsync(hot_tab.get(), hp.table.data(), size_t(Sh) * 8,
@@ -1291,7 +1312,15 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     hot_seg.reset(nparts * 8);
     unsigned long long* cold_n = hot_cnt.as<unsigned long long>();
     unsigned* hot_n = reinterpret_cast<unsigned*>(cold_n + 1);
-    const size_t lds = size_t(2 + accs.n) * Sh * 8;
+    size_t lds = size_t(2 + accs.n) * Sh * 8;
+    if (table_hint) {
+      const int cp = pbits_for(table_hint);
+      if (lds + (size_t(4) << cp) <= size_t(HOT_MAX_LDS)) {
+        hot_cpbits = cp;
+        lds += size_t(4) << cp;
+        hot_ccnt.reset((size_t(8) << cp) * nparts);
+      }
+    }
     {
       ProfScope prof("groupby_hot", st);
       auto launch = [&](auto kz) {
@@ -1301,12 +1330,13 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
         unsigned long long* ht = hot_tab.as<unsigned long long>();
         unsigned long long* hpart = hot_part.as<unsigned long long>();
         unsigned long long* sg = hot_seg.as<unsigned long long>();
+        unsigned long long* cc = hot_cpbits >= 0 ? hot_ccnt.as<unsigned long long>() : nullptr;
         switch (accs.n) {
-          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
-          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
-          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
-          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
-          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg); break;
+          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
+          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
+          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
+          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
+          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
         }
       };
       switch (ktype) {
@@ -1395,17 +1425,15 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     return;
   }
 
-  // partitions: about S * fill_pct / 100 expected groups each
-  const size_t want = table_hint ? table_hint : std::min<size_t>(cn, size_t(1) << 20);
-  int pbits = 4;
-  const size_t per_part = std::max<size_t>(1, size_t(S) * size_t(fill_pct) / 100);
-  while (pbits < PA_MAX_PBITS && (size_t(1) << pbits) * per_part < want) pbits++;
+  int pbits = pbits_for(table_hint ? table_hint : std::min<size_t>(cn, size_t(1) << 20));
   for (;;) {
     const int P = 1 << pbits;
     const int Ptot = P + (hp.used ? 1 : 0);
     const size_t tiles = seg_n ? seg_spans : ceil_div(ceil_div(cn, PS_TILE), size_t(span_tiles));  // spans
-    DBuf cb(size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int));
-    unsigned long long* cnt = cb.as<unsigned long long>();
+    // the heavy-key pass counted the cold rows for this partition count
+    const bool counted = seg_n && pbits == hot_cpbits;
+    DBuf cb(counted ? 0 : size_t(P) * tiles * 8), ib(size_t(P) * tiles * 8), ovf(sizeof(int));
+    unsigned long long* cnt = counted ? hot_ccnt.as<unsigned long long>() : cb.as<unsigned long long>();
     unsigned long long* incl = ib.as<unsigned long long>();
     FHIP_CHECK(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
     DBuf pkeys(cn * size_t(kb));
@@ -1428,7 +1456,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     };
     {
       ProfScope prof("groupby_part_hist", st);
-      run(1);
+      if (!counted) run(1);
       prefix_sum(FHIP_U64, cnt, incl, size_t(P) * tiles, st);
     }
     // partition sizes: inclusive offset at the last span of every partition
