@@ -1270,6 +1270,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   const unsigned long long* seg_n = nullptr;
   size_t seg_spans = 0;
   unsigned hot_groups = 0;
+  unsigned long long hc[2] = {0, 0};  // cold rows, heavy-key groups
+  bool cn_pending = false;            // hc not read back yet
   if (hp.used) {
     const int Sh = hp.Sh;
     static bool hot_attr = false;
@@ -1361,11 +1363,16 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
           stride, hot_fin.as<unsigned long long>(), hot_n);
       FHIP_CHECK(hipGetLastError());
     }
-    unsigned long long hc[2] = {0, 0};
-    FHIP_CHECK(hipMemcpyAsync(hc, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
-    cn = size_t(hc[0]);
-    hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
+    if (hot_cpbits >= 0) {
+      // the partitioned path's first steps need neither: both come back
+      // with the partition sizes, so the host does not wait here
+      cn_pending = true;
+    } else {
+      FHIP_CHECK(hipMemcpyAsync(hc, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
+      FHIP_CHECK(hipStreamSynchronize(st));
+      cn = size_t(hc[0]);
+      hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
+    }
     ckeys = cold_k.get();
     for (int a = 0; a < accs.n; a++) caccs.a[a].col = cold_c[a].get();
     seg_n = hot_seg.as<unsigned long long>();
@@ -1418,10 +1425,13 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     FHIP_CHECK(hipStreamSynchronize(st));  // foff is a host local
   };
 
-  if (hp.used && cn == 0) {  // every row had a heavy key
+  auto assemble_hot_only = [&]() {  // every row had a heavy key
     DBuf fnb(sizeof(unsigned));
     FHIP_CHECK(hipMemcpyAsync(fnb.get(), &hot_groups, sizeof(unsigned), hipMemcpyHostToDevice, st));
     assemble(hot_fin.as<unsigned long long>(), fnb.as<unsigned>(), std::vector<unsigned>{hot_groups}, 1);
+  };
+  if (hp.used && !cn_pending && cn == 0) {
+    assemble_hot_only();
     return;
   }
 
@@ -1436,15 +1446,13 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     unsigned long long* cnt = counted ? hot_ccnt.as<unsigned long long>() : cb.as<unsigned long long>();
     unsigned long long* incl = ib.as<unsigned long long>();
     FHIP_CHECK(hipMemsetAsync(ovf.get(), 0, sizeof(int), st));
-    DBuf pkeys(cn * size_t(kb));
+    DBuf pkeys;  // the partitioned copy, allocated once the cold-row count is known
     DBuf pcols[HA_MAX_ACC];
     Cols cols{};
     cols.n = caccs.n;
     for (int a = 0; a < caccs.n; a++) {
       cols.bytes[a] = type_bytes(caccs.a[a].vtype);
       cols.in[a] = caccs.a[a].col;
-      pcols[a].reset(cn * size_t(cols.bytes[a]));
-      cols.out[a] = pcols[a].get();
     }
     auto run = [&](int pass) {
       switch (ktype) {
@@ -1461,7 +1469,30 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     }
     // partition sizes: inclusive offset at the last span of every partition
     std::vector<unsigned long long> ends(P);
-    copy_strided_to_host(incl + (tiles - 1), tiles, size_t(P), ends.data(), st);
+    {
+      DBuf eb(size_t(P) * 8);
+      gather_strided_u64_kernel<<<grid_for_n(size_t(P)), 256, 0, st>>>(
+          incl + (tiles - 1), tiles, size_t(P), eb.as<unsigned long long>());
+      FHIP_CHECK(hipGetLastError());
+      FHIP_CHECK(hipMemcpyAsync(ends.data(), eb.get(), size_t(P) * 8, hipMemcpyDeviceToHost, st));
+      if (cn_pending)
+        FHIP_CHECK(hipMemcpyAsync(hc, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
+      FHIP_CHECK(hipStreamSynchronize(st));
+    }
+    if (cn_pending) {
+      cn_pending = false;
+      cn = size_t(hc[0]);
+      hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
+      if (cn == 0) {
+        assemble_hot_only();
+        return;
+      }
+    }
+    pkeys.reset(cn * size_t(kb));
+    for (int a = 0; a < caccs.n; a++) {
+      pcols[a].reset(cn * size_t(cols.bytes[a]));
+      cols.out[a] = pcols[a].get();
+    }
     {
       ProfScope prof("groupby_part_scatter", st);
       run(2);
