@@ -192,16 +192,18 @@ inline size_t ceil_div(size_t a, size_t b) { return (a + b - 1) / b; }
 // around a named kernel; read back through fhip_prof_query.
 // ---------------------------------------------------------------------------
 bool prof_enabled();
+bool prof_wanted(const char* name);  // enabled, and the name passes the filter
+hipEvent_t prof_event();  // from the recycled events when there are any
 void prof_record(const char* name, hipEvent_t start, hipEvent_t stop);
 
 class ProfScope {
  public:
   ProfScope(const char* name, hipStream_t st) : name_(name), st_(st) {
-    if (!prof_enabled()) return;
+    if (!prof_wanted(name)) return;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
-    FHIP_CHECK(hipEventCreate(&a_));
-    FHIP_CHECK(hipEventCreate(&b_));
+    a_ = prof_event();
+    b_ = prof_event();
     FHIP_CHECK(hipEventRecord(a_, st_));
   }
   ~ProfScope() {

@@ -84,7 +84,35 @@ struct ProfEntry {
 std::mutex g_prof_mu;
 bool g_prof_on = false;
 std::vector<ProfEntry> g_prof;
+// recorded events go back to this list when the records are cleared, so a
+// timed region creates no events (enabling fills it with kProfEvents)
+std::vector<hipEvent_t> g_prof_free;
+constexpr size_t kProfEvents = 512;
+std::vector<std::string> g_prof_only;  // names recorded (empty: all)
 }  // namespace
+
+bool prof_wanted(const char* name) {
+  if (!g_prof_on) return false;
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  if (g_prof_only.empty()) return true;
+  for (auto& s : g_prof_only)
+    if (s == name) return true;
+  return false;
+}
+
+hipEvent_t prof_event() {
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    if (!g_prof_free.empty()) {
+      hipEvent_t e = g_prof_free.back();
+      g_prof_free.pop_back();
+      return e;
+    }
+  }
+  hipEvent_t e = nullptr;
+  FHIP_CHECK(hipEventCreate(&e));
+  return e;
+}
 
 bool prof_enabled() { return g_prof_on; }
 
@@ -110,8 +138,9 @@ void prof_record(const char* name, hipEvent_t a, hipEvent_t b) {
 
 static void prof_clear_locked() {
   for (auto& e : g_prof) {
-    (void)hipEventDestroy(e.a);
-    (void)hipEventDestroy(e.b);
+    (void)hipEventSynchronize(e.b);  // both may still be pending on their stream
+    g_prof_free.push_back(e.a);
+    g_prof_free.push_back(e.b);
   }
   g_prof.clear();
 }
@@ -153,6 +182,27 @@ int fhip_prof_enable(int on) {
     std::lock_guard<std::mutex> lk(fhip::g_prof_mu);
     fhip::g_prof_on = on != 0;
     fhip::prof_clear_locked();
+    while (on && fhip::g_prof_free.size() < fhip::kProfEvents) {
+      hipEvent_t e = nullptr;
+      FHIP_CHECK(hipEventCreate(&e));
+      fhip::g_prof_free.push_back(e);
+    }
+  });
+}
+
+int fhip_prof_filter(const char* names) {
+  return fhip::guarded([&] {
+    std::lock_guard<std::mutex> lk(fhip::g_prof_mu);
+    fhip::g_prof_only.clear();
+    if (!names) return;
+    std::string all(names);
+    size_t i = 0;
+    while (i <= all.size()) {
+      size_t j = all.find(',', i);
+      if (j == std::string::npos) j = all.size();
+      if (j > i) fhip::g_prof_only.push_back(all.substr(i, j - i));
+      i = j + 1;
+    }
   });
 }
 

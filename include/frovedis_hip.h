@@ -62,8 +62,12 @@ int fhip_dev_free(void* p);
  * the launch stream around its main kernels, named "radix_pass",
  * "radix_hist", "hashagg", "ht_build", "ht_probe", "join_probe",
  * "batch_sort", "learned_scatter"...  fhip_prof_enable(on) also clears
- * what was recorded; fhip_prof_query sums the durations of one name. */
+ * what was recorded; fhip_prof_query sums the durations of one name.
+ * fhip_prof_filter(names) records only the comma-separated names given
+ * (NULL or "" records every name), so that timing one kernel costs a
+ * timed region two event records per launch of it. */
 int fhip_prof_enable(int on);
+int fhip_prof_filter(const char* names);
 int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
 
 /* Tunable parameters, set by the autotuner (frovedis_amd/autotune.py) from
