@@ -987,13 +987,19 @@ void grouped_agg(int op, int vtype, const void* col, size_t col_n, const size_t*
                  const size_t* split, size_t nsplit, const size_t* nulls, size_t nn, void* out,
                  hipStream_t st);
 
-// Heavy-key candidates.  The keys of an evenly spaced sample of m rows are
-// counted in an open-addressing table of 2m slots (linear probing from a
-// mixed hash; at most half the slots fill, so every probe ends), then the
-// keys seen at least HOT_MIN_SEEN times are listed as (count, key) pairs
-// after a leading entry count.  The list is in arrival order; the host sorts
-// it by (count descending, key ascending), which fixes the plan.
+// Heavy-key candidates.  The sample is m rows in m / HOT_GROUP evenly spaced
+// groups of HOT_GROUP consecutive rows (few memory pages, so the reads do
+// not wait on address translation).  A key counts once per group that holds
+// it, so a run of equal keys (sorted or clustered data) does not look heavy;
+// for keys of frequency p << 1 / HOT_GROUP the count expects m * p, as for m
+// single rows.  The counts go to an open-addressing table of 2m slots
+// (linear probing from a mixed hash; at most half the slots fill, so every
+// probe ends), then the keys counted at least HOT_MIN_SEEN times are listed
+// as (count, key) pairs after a leading entry count.  The list is in
+// arrival order; the host sorts it by (count descending, key ascending),
+// which fixes the plan.
 constexpr unsigned HOT_MIN_SEEN = 4;
+constexpr unsigned HOT_GROUP = 64;  // the wave width (or a divisor of it)
 
 __global__ void hot_count_init_kernel(unsigned long long* __restrict__ tk, unsigned* __restrict__ tc,
                                       unsigned size, unsigned long long* __restrict__ list) {
@@ -1005,19 +1011,31 @@ __global__ void hot_count_init_kernel(unsigned long long* __restrict__ tk, unsig
   if (blockIdx.x == 0 && threadIdx.x == 0) list[0] = 0;
 }
 
+// gstep: rows between the starts of consecutive sample groups (>= HOT_GROUP)
 template <class K>
-__global__ void hot_count_kernel(const K* __restrict__ keys, size_t step, size_t m,
+__global__ void hot_count_kernel(const K* __restrict__ keys, size_t gstep, size_t m,
                                  unsigned long long* __restrict__ tk, unsigned* __restrict__ tc,
                                  unsigned mask) {
   const size_t stride = size_t(gridDim.x) * blockDim.x;
   const unsigned lane = __lane_id();
+  const unsigned o = lane % HOT_GROUP;  // position in the group (groups never straddle waves)
   // the loop runs a whole number of times for every lane of a wave, so the
-  // ballots below see all of them
+  // ballots and shuffles below see all of them
   const size_t m_up = (m + stride - 1) / stride * stride;
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m_up; i += stride) {
-    const unsigned long long kb = i < m ? key_bits(keys[i * step]) : HA_EMPTY;
+    const size_t row = (i / HOT_GROUP) * gstep + o;
+    const unsigned long long kb = i < m ? key_bits(keys[row]) : HA_EMPTY;
+    // only the group's first lane with a key counts it
+    bool first = true;
+    const int lo = int(unsigned(kb)), hi = int(unsigned(kb >> 32));
+    for (unsigned d = 1; d < HOT_GROUP; d++) {
+      const int src = int(lane) - int(d);
+      const unsigned olo = unsigned(__shfl(lo, src < 0 ? 0 : src, 64));
+      const unsigned ohi = unsigned(__shfl(hi, src < 0 ? 0 : src, 64));
+      if (o >= d && ((unsigned long long)ohi << 32 | olo) == kb) first = false;
+    }
     // the empty marker is never a heavy key
-    const bool valid = kb != HA_EMPTY;
+    const bool valid = kb != HA_EMPTY && first;
     unsigned h = unsigned(mix64(kb)) & mask;
     while (valid) {
       // a slot's key never changes once set: most lanes of a skewed sample
@@ -1057,10 +1075,11 @@ __global__ void hot_list_kernel(const unsigned long long* __restrict__ tk,
   }
 }
 
-// The heavy keys of a skewed key column: keys seen at least 4 times in an
-// evenly spaced sample of 2^16 rows (expected frequency >= 2^-14), at most
-// Sh / 2 of them, placed in an open-addressing table of Sh slots.  Not used
-// when they cover less than groupby.hot_min_pct percent of the sample.
+// The heavy keys of a skewed key column: keys found in at least 4 of the
+// 2^11 groups of a 2^17-row sample (expected frequency >= 2^-15), at most
+// 80 % of Sh of them (the most frequent first), placed in a two-choice
+// table of Sh slots.  Not used when their counts cover less than
+// groupby.hot_min_pct percent of the sample.
 struct HotPlan {
   bool used = false;
   int Sh = 0;
@@ -1069,26 +1088,28 @@ struct HotPlan {
 
 HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   HotPlan h;
-  // 2048 slots by default: the sampled heavy keys of skewed data fit (about
-  // a thousand keys occur 4+ times in a 2^16 sample of Zipf(1.1) keys), and
-  // two workgroups' tables fit one CU's LDS
+  // 2048 slots by default: the 80 % the plan fills hold the sampled heavy
+  // keys of skewed data (about 1,900 keys of 2^20 Zipf(1.1) values reach 4
+  // groups of a 2^17-row sample; the 1,638 most counted are kept), and two
+  // workgroups' tables fit one CU's LDS
   long long want = tune_get("groupby.hot", std::min(S, 2048));
-  // sample of 2^groupby.hot_sample rows (default 2^16)
-  const int mbits = int(std::max(12LL, std::min(20LL, tune_get("groupby.hot_sample", 16))));
+  // sample of 2^groupby.hot_sample rows (default 2^17)
+  const int mbits = int(std::max(12LL, std::min(20LL, tune_get("groupby.hot_sample", 17))));
   const size_t m = size_t(1) << mbits;
   if (want < 256 || n < 16 * m) return h;
   int Sh = int(std::min<long long>(want, S));
   while (Sh & (Sh - 1)) Sh &= Sh - 1;
-  const size_t step = n / m;
+  // n >= 16 m, so the groups (HOT_GROUP rows every gstep rows) never overlap
+  const size_t gstep = n / (m / HOT_GROUP);
   const unsigned tsize = unsigned(2 * m), cap = unsigned(m / HOT_MIN_SEEN);
   DBuf tk(size_t(tsize) * 8), tc(size_t(tsize) * 4), lb((1 + 2 * size_t(cap)) * 8);
   unsigned long long* list = lb.as<unsigned long long>();
   hot_count_init_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tk.as<unsigned long long>(), tc.as<unsigned>(), tsize, list);
   switch (ktype) {
-    case FHIP_I32: hot_count_kernel<int32_t><<<grid_for_n(m), 256, 0, st>>>((const int32_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
-    case FHIP_U32: hot_count_kernel<uint32_t><<<grid_for_n(m), 256, 0, st>>>((const uint32_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
-    case FHIP_I64: hot_count_kernel<int64_t><<<grid_for_n(m), 256, 0, st>>>((const int64_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
-    default: hot_count_kernel<uint64_t><<<grid_for_n(m), 256, 0, st>>>((const uint64_t*)keys, step, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    case FHIP_I32: hot_count_kernel<int32_t><<<grid_for_n(m), 256, 0, st>>>((const int32_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    case FHIP_U32: hot_count_kernel<uint32_t><<<grid_for_n(m), 256, 0, st>>>((const uint32_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    case FHIP_I64: hot_count_kernel<int64_t><<<grid_for_n(m), 256, 0, st>>>((const int64_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    default: hot_count_kernel<uint64_t><<<grid_for_n(m), 256, 0, st>>>((const uint64_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
   }
   hot_list_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tk.as<unsigned long long>(), tc.as<unsigned>(), tsize, list, cap);
   FHIP_CHECK(hipGetLastError());
@@ -1105,12 +1126,26 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
                               2 * (nc - first) * 8, hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipStreamSynchronize(st));
   }
-  std::vector<std::pair<size_t, unsigned long long>> runs(nc);  // (count, key)
-  for (size_t j = 0; j < nc; j++) runs[j] = {size_t(lst[1 + 2 * j]), lst[2 + 2 * j]};
-  std::sort(runs.begin(), runs.end(), [](const std::pair<size_t, unsigned long long>& a,
-                                         const std::pair<size_t, unsigned long long>& b) {
-    return a.first > b.first || (a.first == b.first && a.second < b.second);
-  });
+  // (count, key) by count descending, then key ascending: a counting sort on
+  // the count (at most m / HOT_GROUP), then the keys of each count sorted
+  std::vector<std::pair<size_t, unsigned long long>> runs(nc);
+  {
+    size_t maxc = 0;
+    for (size_t j = 0; j < nc; j++) maxc = std::max<size_t>(maxc, size_t(lst[1 + 2 * j]));
+    std::vector<size_t> start(maxc + 2, 0);  // start[c]: first position of count c
+    for (size_t j = 0; j < nc; j++) start[maxc - size_t(lst[1 + 2 * j]) + 1]++;
+    for (size_t c = 1; c < start.size(); c++) start[c] += start[c - 1];
+    std::vector<unsigned long long> ks(nc);
+    for (size_t j = 0; j < nc; j++) ks[start[maxc - size_t(lst[1 + 2 * j])]++] = lst[2 + 2 * j];
+    // start[maxc - c] now ends the run of count c
+    size_t b = 0;
+    for (size_t r = 0; r <= maxc; r++) {
+      const size_t e = start[r];
+      if (e - b > 1) std::sort(ks.begin() + b, ks.begin() + e);
+      for (size_t j = b; j < e; j++) runs[j] = {maxc - r, ks[j]};
+      b = e;
+    }
+  }
   // two-choice buckets of two slots place keys up to a load near 90 %; the
   // plan fills at most 80 % of the slots
   if (runs.size() > size_t(Sh) * 4 / 5) runs.resize(size_t(Sh) * 4 / 5);

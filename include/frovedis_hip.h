@@ -83,7 +83,7 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *   "groupby.hot"       slots of the heavy-key table (0 switches the heavy-key
  *       pass off); "groupby.hot_min_pct" the sample share the heavy keys
  *       must cover for the pass to run; "groupby.hot_sample" log2 of the
- *       rows sampled to find them (12 to 20, default 16);
+ *       rows sampled to find them (12 to 20, default 17);
  *   "compact.shape"     tile of the one-pass stream compaction behind filter,
  *       set_separate and the join outputs (0 256x16, 1 1024x16 threads x
  *       rows);
