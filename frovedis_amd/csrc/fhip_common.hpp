@@ -136,6 +136,15 @@ class HostStage {
   void* p_ = nullptr;
 };
 
+// Pinned host memory for small stream-ordered copies: buffer `slot` (0 to
+// 3) of the calling thread, at least `bytes` long.  A thread's buffers grow
+// on demand and are kept until the process ends; a caller reuses a slot only
+// once the copies that used it have completed.
+void* pinned_scratch(int slot, size_t bytes);
+// An event of the calling thread (no timing), for host waits on a point of
+// a stream; kept until the process ends.
+hipEvent_t thread_event();
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 // ---------------------------------------------------------------------------

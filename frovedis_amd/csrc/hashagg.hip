@@ -1418,7 +1418,10 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   // last one when used): compact, sort the group keys, gather the outputs.
   auto assemble = [&](const unsigned long long* w, const unsigned* fn_dev,
                       const std::vector<unsigned>& fn, int Ptot) {
-    std::vector<unsigned long long> foff(Ptot);
+    // partition offsets, uploaded from pinned memory (the lambda ends with
+    // a synchronisation, so the slot is free again on return)
+    unsigned long long* foff =
+        static_cast<unsigned long long*>(pinned_scratch(3, size_t(Ptot) * 8));
     size_t G = 0;
     for (int p = 0; p < Ptot; p++) {
       foff[p] = G;
@@ -1430,7 +1433,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
                      std::to_string(G));
     if (G == 0) return;
     DBuf fob(size_t(Ptot) * 8), entb(G * sizeof(size_t));
-    FHIP_CHECK(hipMemcpyAsync(fob.get(), foff.data(), size_t(Ptot) * 8, hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipMemcpyAsync(fob.get(), foff, size_t(Ptot) * 8, hipMemcpyHostToDevice, st));
     final_index_kernel<<<Ptot, 256, 0, st>>>(fn_dev, fob.as<unsigned long long>(), Ptot, S,
                                               entb.as<size_t>());
     FHIP_CHECK(hipGetLastError());
@@ -1457,7 +1460,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, es);
       FHIP_CHECK(hipGetLastError());
     }
-    FHIP_CHECK(hipStreamSynchronize(st));  // foff is a host local
+    FHIP_CHECK(hipStreamSynchronize(st));  // the pinned slot of foff is reused by the next call
   };
 
   auto assemble_hot_only = [&]() {  // every row had a heavy key
@@ -1502,53 +1505,73 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       if (!counted) run(1);
       prefix_sum(FHIP_U64, cnt, incl, size_t(P) * tiles, st);
     }
-    // partition sizes: inclusive offset at the last span of every partition
-    std::vector<unsigned long long> ends(P);
-    {
-      DBuf eb(size_t(P) * 8);
-      gather_strided_u64_kernel<<<grid_for_n(size_t(P)), 256, 0, st>>>(
-          incl + (tiles - 1), tiles, size_t(P), eb.as<unsigned long long>());
-      FHIP_CHECK(hipGetLastError());
-      FHIP_CHECK(hipMemcpyAsync(ends.data(), eb.get(), size_t(P) * 8, hipMemcpyDeviceToHost, st));
-      if (cn_pending)
-        FHIP_CHECK(hipMemcpyAsync(hc, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
-      FHIP_CHECK(hipStreamSynchronize(st));
-    }
-    if (cn_pending) {
-      cn_pending = false;
-      cn = size_t(hc[0]);
-      hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
-      if (cn == 0) {
-        assemble_hot_only();
-        return;
-      }
-    }
-    pkeys.reset(cn * size_t(kb));
+    // partition sizes (inclusive offset at the last span of every partition)
+    // and, while pending, the cold-row count come back through pinned
+    // memory; the scatter is queued before the host waits for them
+    unsigned long long* ends =
+        static_cast<unsigned long long*>(pinned_scratch(0, (size_t(P) + 2) * 8));
+    DBuf eb(size_t(P) * 8);
+    gather_strided_u64_kernel<<<grid_for_n(size_t(P)), 256, 0, st>>>(
+        incl + (tiles - 1), tiles, size_t(P), eb.as<unsigned long long>());
+    FHIP_CHECK(hipGetLastError());
+    FHIP_CHECK(hipMemcpyAsync(ends, eb.get(), size_t(P) * 8, hipMemcpyDeviceToHost, st));
+    if (cn_pending)
+      FHIP_CHECK(hipMemcpyAsync(ends + P, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
+    hipEvent_t sizes_ready = thread_event();
+    FHIP_CHECK(hipEventRecord(sizes_ready, st));
+    // the partitioned copy holds the cold rows (every row, while their count
+    // is still on its way: the cold buffers' own bound)
+    const size_t prows = cn_pending ? n : cn;
+    pkeys.reset(prows * size_t(kb));
     for (int a = 0; a < caccs.n; a++) {
-      pcols[a].reset(cn * size_t(cols.bytes[a]));
+      pcols[a].reset(prows * size_t(cols.bytes[a]));
       cols.out[a] = pcols[a].get();
     }
     {
       ProfScope prof("groupby_part_scatter", st);
       run(2);
     }
-    // chunks of at most `chunk` rows per partition (built while the scatter runs)
-    std::vector<Item> items;
-    std::vector<unsigned> part_items(P + 1);
-    unsigned long long run_off = 0;
-    for (int p = 0; p < P; p++) {
-      part_items[p] = unsigned(items.size());
-      const unsigned long long e = ends[p];
-      for (unsigned long long b = run_off; b < e; b += chunk)
-        items.push_back(Item{b, std::min<unsigned long long>(b + chunk, e)});
-      if (e == run_off) items.push_back(Item{run_off, run_off});
-      run_off = e;
+    FHIP_CHECK(hipEventSynchronize(sizes_ready));
+    if (cn_pending) {
+      cn_pending = false;
+      cn = size_t(ends[P]);
+      hot_groups = unsigned(ends[P + 1] & 0xFFFFFFFFull);
+      if (cn == 0) {
+        assemble_hot_only();
+        return;
+      }
     }
-    part_items[P] = unsigned(items.size());
-    const size_t NI = items.size();
+    // chunks of at most `chunk` rows per partition, built while the scatter
+    // runs, in pinned memory (items, then the first item of every partition)
+    size_t NI = 0;
+    {
+      unsigned long long prev = 0;
+      for (int p = 0; p < P; p++) {
+        const unsigned long long e = ends[p];
+        NI += e == prev ? 1 : size_t(ceil_div(size_t(e - prev), size_t(chunk)));
+        prev = e;
+      }
+    }
+    const size_t items_bytes = (NI * sizeof(Item) + 15) & ~size_t(15);
+    char* stage = static_cast<char*>(pinned_scratch(1, items_bytes + (size_t(P) + 1) * sizeof(unsigned)));
+    Item* items = reinterpret_cast<Item*>(stage);
+    unsigned* part_items = reinterpret_cast<unsigned*>(stage + items_bytes);
+    {
+      size_t ni = 0;
+      unsigned long long run_off = 0;
+      for (int p = 0; p < P; p++) {
+        part_items[p] = unsigned(ni);
+        const unsigned long long e = ends[p];
+        for (unsigned long long b = run_off; b < e; b += chunk)
+          items[ni++] = Item{b, std::min<unsigned long long>(b + chunk, e)};
+        if (e == run_off) items[ni++] = Item{run_off, run_off};
+        run_off = e;
+      }
+      part_items[P] = unsigned(ni);
+    }
     DBuf itb(NI * sizeof(Item)), pib((P + 1) * sizeof(unsigned));
-    FHIP_CHECK(hipMemcpyAsync(itb.get(), items.data(), NI * sizeof(Item), hipMemcpyHostToDevice, st));
-    FHIP_CHECK(hipMemcpyAsync(pib.get(), part_items.data(), (P + 1) * sizeof(unsigned),
+    FHIP_CHECK(hipMemcpyAsync(itb.get(), items, NI * sizeof(Item), hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipMemcpyAsync(pib.get(), part_items, (P + 1) * sizeof(unsigned),
                               hipMemcpyHostToDevice, st));
     DBuf partial(NI * size_t(S + 1) * stride * 8), pn(NI * sizeof(unsigned));
     FHIP_CHECK(hipMemsetAsync(pn.get(), 0, NI * sizeof(unsigned), st));
@@ -1584,12 +1607,15 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
           fin.as<unsigned long long>(), fnb.as<unsigned>(), ovf.as<int>());
       FHIP_CHECK(hipGetLastError());
     }
-    int ov = 0;
-    std::vector<unsigned> fn(Ptot);
-    FHIP_CHECK(hipMemcpyAsync(&ov, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipMemcpyAsync(fn.data(), fnb.get(), size_t(P) * sizeof(unsigned),
+    // the overflow flag, then the group count of every partition
+    unsigned* rb = static_cast<unsigned*>(pinned_scratch(2, (size_t(P) + 1) * sizeof(unsigned)));
+    FHIP_CHECK(hipMemcpyAsync(rb, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipMemcpyAsync(rb + 1, fnb.get(), size_t(P) * sizeof(unsigned),
                               hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipStreamSynchronize(st));
+    const int ov = int(rb[0]);
+    std::vector<unsigned> fn(rb + 1, rb + 1 + P);
+    fn.resize(Ptot);
     if (ov) {  // a partition had more groups than its LDS table: more partitions
       if (pbits < PA_MAX_PBITS) {
         pbits = std::min(pbits + 2, PA_MAX_PBITS);

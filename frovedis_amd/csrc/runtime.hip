@@ -77,6 +77,41 @@ HostStage::~HostStage() {
 }
 
 namespace {
+constexpr int kPinSlots = 4;
+struct PinSlots {
+  void* p[kPinSlots] = {};
+  size_t cap[kPinSlots] = {};
+  hipEvent_t ev = nullptr;
+};
+// per thread, intentionally leaked (with its pinned buffers and event): it
+// must outlive static destructors that run after the HIP runtime has gone
+thread_local PinSlots* g_pin = nullptr;
+}  // namespace
+
+void* pinned_scratch(int slot, size_t bytes) {
+  FHIP_REQUIRE(slot >= 0 && slot < kPinSlots, FHIP_ERR_ARG, "pinned_scratch: bad slot");
+  if (!g_pin) g_pin = new PinSlots();
+  if (g_pin->cap[slot] < bytes) {
+    size_t c = size_t(64) << 10;
+    while (c < bytes) c *= 2;
+    if (g_pin->p[slot]) {
+      (void)hipHostFree(g_pin->p[slot]);
+      g_pin->p[slot] = nullptr;
+      g_pin->cap[slot] = 0;
+    }
+    FHIP_CHECK(hipHostMalloc(&g_pin->p[slot], c, hipHostMallocDefault));
+    g_pin->cap[slot] = c;
+  }
+  return g_pin->p[slot];
+}
+
+hipEvent_t thread_event() {
+  if (!g_pin) g_pin = new PinSlots();
+  if (!g_pin->ev) FHIP_CHECK(hipEventCreateWithFlags(&g_pin->ev, hipEventDisableTiming));
+  return g_pin->ev;
+}
+
+namespace {
 struct ProfEntry {
   std::string name;
   hipEvent_t a, b;
