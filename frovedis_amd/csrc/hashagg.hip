@@ -25,6 +25,7 @@
 #include "fhip_device.hpp"
 #include "radix_sort.hpp"
 
+#include <cstring>
 #include <limits>
 
 namespace fhip {
@@ -1116,13 +1117,15 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   // the count and the first 4096 candidates in one copy (skewed data has
   // about a thousand), the rest only when there are more
   const size_t first = std::min<size_t>(cap, 4096);
-  std::vector<unsigned long long> lst(1 + 2 * first);
-  FHIP_CHECK(hipMemcpyAsync(lst.data(), list, lst.size() * 8, hipMemcpyDeviceToHost, st));
+  // (pinned slot 2: its next user, the group-by's final readback, comes
+  // after this copy has been waited for)
+  unsigned long long* lst =
+      static_cast<unsigned long long*>(pinned_scratch(2, (1 + 2 * size_t(cap)) * 8));
+  FHIP_CHECK(hipMemcpyAsync(lst, list, (1 + 2 * first) * 8, hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
   const size_t nc = std::min<size_t>(size_t(lst[0]), cap);
   if (nc > first) {
-    lst.resize(1 + 2 * nc);
-    FHIP_CHECK(hipMemcpyAsync(lst.data() + 1 + 2 * first, list + 1 + 2 * first,
+    FHIP_CHECK(hipMemcpyAsync(lst + 1 + 2 * first, list + 1 + 2 * first,
                               2 * (nc - first) * 8, hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipStreamSynchronize(st));
   }
@@ -1331,8 +1334,11 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     FHIP_REQUIRE(size_t(2 + accs.n) * Sh * 8 <= size_t(HOT_MAX_LDS), FHIP_ERR_ARG,
                  "group_by_agg: heavy-key table exceeds LDS");
     hot_tab.reset(size_t(Sh) * 8);
-    FHIP_CHECK(hipMemcpyAsync(hot_tab.get(), hp.table.data(), size_t(Sh) * 8,
-                              hipMemcpyHostToDevice, st));
+    // uploaded from pinned slot 3 (next used by the final assembly, after
+    // several waits on the stream)
+    void* tab_h = pinned_scratch(3, size_t(Sh) * 8);
+    std::memcpy(tab_h, hp.table.data(), size_t(Sh) * 8);
+    FHIP_CHECK(hipMemcpyAsync(hot_tab.get(), tab_h, size_t(Sh) * 8, hipMemcpyHostToDevice, st));
     cold_k.reset(n * size_t(kb));
     Cols cold{};
     cold.n = accs.n;
