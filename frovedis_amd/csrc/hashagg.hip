@@ -1244,8 +1244,10 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   const int kb = type_bytes(ktype);
   const int stride = 2 + accs.n;
   // LDS table slots (power of two, capped by LDS) and the expected groups per
-  // partition (load factor) come from the tunables
-  int S = int(tune_get("groupby.slots", accs.n <= 2 ? 2 * PA_SLOTS : PA_SLOTS));
+  // partition (load factor) come from the tunables.  2048 slots by default:
+  // two aggregation workgroups share a CU, which outweighs the longer runs
+  // per partition that half as many partitions would give the scatter
+  int S = int(tune_get("groupby.slots", PA_SLOTS));
   S = std::max(256, std::min(S, accs.n <= 2 ? 2 * PA_SLOTS : PA_SLOTS));
   while (S & (S - 1)) S &= S - 1;
   const long long fill_pct = std::max(5LL, std::min(90LL, tune_get("groupby.fill_pct", 50)));
