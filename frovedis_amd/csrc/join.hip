@@ -243,6 +243,7 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
 // pattern versus == behaviour (-0.0, NaN) the result depends on.
 constexpr unsigned long long JT_EMPTY = ~0ull;
 constexpr int JT_NT = 256;
+constexpr int JT_PROBE_U = 4;  // queries per thread and step of the probe
 
 template <class K>
 __device__ __forceinline__ unsigned long long jt_bits(K k) {
@@ -254,34 +255,74 @@ __device__ __forceinline__ unsigned long long jt_home(unsigned long long k, int 
 }
 
 // side = {rows with the empty-marker key, the value of the first, repeated key}
+// The rest of an insertion whose home slot h held another key.
+__device__ __noinline__ void jt_insert_rest(unsigned long long* slots, unsigned long long mask,
+                                            unsigned long long k, unsigned long long v,
+                                            unsigned long long h, unsigned long long* side) {
+  while (true) {
+    h = (h + 1) & mask;
+    const unsigned long long prev = atomicCAS(&slots[2 * h], JT_EMPTY, k);
+    if (prev == JT_EMPTY) {
+      slots[2 * h + 1] = v;
+      return;
+    }
+    if (prev == k) {
+      atomicOr(&side[2], 1ull);
+      return;
+    }
+  }
+}
+
 template <class K>
 __global__ __launch_bounds__(JT_NT) void jt_insert_kernel(const K* __restrict__ keys,
                                                           const unsigned long long* __restrict__ vals,
                                                           size_t n, unsigned long long* slots,
                                                           unsigned long long mask, int shift,
                                                           unsigned long long* side) {
-  const size_t stride = size_t(gridDim.x) * JT_NT;
-  for (size_t i = size_t(blockIdx.x) * JT_NT + threadIdx.x; i < n; i += stride) {
-    const unsigned long long k = jt_bits(keys[i]);
-    const unsigned long long v = vals[i];
-    if (k == JT_EMPTY) {
-      if (atomicAdd(&side[0], 1ull) == 0) side[1] = v;
-      else atomicOr(&side[2], 1ull);
-      continue;
+  // JT_PROBE_U keys per thread and step, their first CAS issued together
+  constexpr int U = JT_PROBE_U;
+  const size_t stride = size_t(gridDim.x) * JT_NT * U;
+  for (size_t i0 = size_t(blockIdx.x) * JT_NT * U + threadIdx.x; i0 < n; i0 += stride) {
+    unsigned long long k[U], v[U], h[U], prev[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const size_t i = i0 + size_t(u) * JT_NT;
+      k[u] = i < n ? jt_bits(keys[i]) : JT_EMPTY;
+      v[u] = i < n ? vals[i] : 0ull;
     }
-    unsigned long long h = jt_home(k, shift);
-    while (true) {
-      const unsigned long long prev = atomicCAS(&slots[2 * h], JT_EMPTY, k);
-      if (prev == JT_EMPTY) {
-        slots[2 * h + 1] = v;
-        break;
-      }
-      if (prev == k) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      h[u] = jt_home(k[u], shift);
+      prev[u] = k[u] != JT_EMPTY ? atomicCAS(&slots[2 * h[u]], JT_EMPTY, k[u]) : JT_EMPTY;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const size_t i = i0 + size_t(u) * JT_NT;
+      if (i >= n) continue;
+      if (k[u] == JT_EMPTY) {
+        if (atomicAdd(&side[0], 1ull) == 0) side[1] = v[u];
+        else atomicOr(&side[2], 1ull);
+      } else if (prev[u] == JT_EMPTY) {
+        slots[2 * h[u] + 1] = v[u];
+      } else if (prev[u] == k[u]) {
         atomicOr(&side[2], 1ull);
-        break;
+      } else {  // a collision: walk on from the next slot
+        jt_insert_rest(slots, mask, k[u], v[u], h[u], side);
       }
-      h = (h + 1) & mask;
     }
+  }
+}
+
+// The rest of a linear probe that started at slot h and did not end there:
+// {1, value} when k is found, {0, 0} at the first empty slot.
+__device__ __noinline__ ulonglong2 jt_probe_rest(const unsigned long long* __restrict__ slots,
+                                                 unsigned long long mask, unsigned long long k,
+                                                 unsigned long long h) {
+  while (true) {
+    h = (h + 1) & mask;
+    const ulonglong2 e = *reinterpret_cast<const ulonglong2*>(slots + 2 * h);
+    if (e.x == k) return make_ulonglong2(1ull, e.y);
+    if (e.x == JT_EMPTY) return make_ulonglong2(0ull, 0ull);
   }
 }
 
@@ -292,29 +333,45 @@ __global__ __launch_bounds__(JT_NT) void jt_probe_kernel(const K* __restrict__ q
                                                          const unsigned long long* __restrict__ side,
                                                          unsigned long long* __restrict__ vals_out,
                                                          uint8_t* __restrict__ found) {
-  const size_t stride = size_t(gridDim.x) * JT_NT;
-  for (size_t j = size_t(blockIdx.x) * JT_NT + threadIdx.x; j < m; j += stride) {
-    const unsigned long long k = jt_bits(q[j]);
-    bool f = false;
-    unsigned long long v = 0;
-    if (k == JT_EMPTY) {
-      f = side[0] != 0;
-      v = f ? side[1] : 0ull;
-    } else {
-      unsigned long long h = jt_home(k, shift);
-      while (true) {
-        const ulonglong2 e = *reinterpret_cast<const ulonglong2*>(slots + 2 * h);
-        if (e.x == k) {
-          f = true;
-          v = e.y;
-          break;
-        }
-        if (e.x == JT_EMPTY) break;
-        h = (h + 1) & mask;
+  // JT_PROBE_U queries per thread and step, their first slot reads issued
+  // together (each lookup is a dependent read chain: query, then slot)
+  constexpr int U = JT_PROBE_U;
+  const size_t stride = size_t(gridDim.x) * JT_NT * U;
+  for (size_t j0 = size_t(blockIdx.x) * JT_NT * U + threadIdx.x; j0 < m; j0 += stride) {
+    unsigned long long k[U], h[U];
+    ulonglong2 e[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const size_t j = j0 + size_t(u) * JT_NT;
+      k[u] = j < m ? jt_bits(q[j]) : JT_EMPTY;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      h[u] = jt_home(k[u], shift);
+      e[u] = k[u] != JT_EMPTY ? *reinterpret_cast<const ulonglong2*>(slots + 2 * h[u])
+                              : make_ulonglong2(JT_EMPTY, 0ull);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const size_t j = j0 + size_t(u) * JT_NT;
+      bool f = false;
+      unsigned long long v = 0;
+      if (k[u] == JT_EMPTY) {
+        f = side[0] != 0;
+        v = f ? side[1] : 0ull;
+      } else if (e[u].x == k[u]) {
+        f = true;
+        v = e[u].y;
+      } else if (e[u].x != JT_EMPTY) {  // a collision: walk on from the next slot
+        const ulonglong2 r = jt_probe_rest(slots, mask, k[u], h[u]);
+        f = r.x != 0;
+        v = r.y;
+      }
+      if (j < m) {
+        vals_out[j] = v;
+        found[j] = f ? 1 : 0;
       }
     }
-    vals_out[j] = v;
-    found[j] = f ? 1 : 0;
   }
 }
 
