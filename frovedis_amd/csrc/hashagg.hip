@@ -876,6 +876,43 @@ __global__ __launch_bounds__(256) void hot_merge_kernel(
   for (int a = 0; a < na; a++) e[2 + a] = acc[a];
 }
 
+// Range of the group keys of the final tables in the radix sort's order (the
+// sign bit of signed keys flipped): range[0] min, range[1] max, updated with
+// one atomic each per partition.  The group-key sort then needs no
+// histogram readback.
+__global__ __launch_bounds__(256) void key_range_kernel(const unsigned long long* __restrict__ w,
+                                                        int stride, int S,
+                                                        const unsigned* __restrict__ fn, int kb,
+                                                        int is_signed,
+                                                        unsigned long long* __restrict__ range) {
+  __shared__ unsigned long long smin[256], smax[256];
+  const int p = blockIdx.x;
+  const unsigned m = fn[p];
+  const unsigned long long mask = kb == 8 ? ~0ull : (1ull << (8 * kb)) - 1;
+  const unsigned long long top = 1ull << (8 * kb - 1);
+  unsigned long long lo = ~0ull, hi = 0;
+  for (unsigned j = threadIdx.x; j < m; j += 256) {
+    unsigned long long b = w[(size_t(p) * (S + 1) + j) * size_t(stride)] & mask;
+    if (is_signed) b ^= top;
+    lo = b < lo ? b : lo;
+    hi = b > hi ? b : hi;
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (unsigned h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      smin[threadIdx.x] = smin[threadIdx.x + h] < smin[threadIdx.x] ? smin[threadIdx.x + h] : smin[threadIdx.x];
+      smax[threadIdx.x] = smax[threadIdx.x + h] > smax[threadIdx.x] ? smax[threadIdx.x + h] : smax[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && m) {
+    atomicMin(&range[0], smin[0]);
+    atomicMax(&range[1], smax[0]);
+  }
+}
+
 __global__ void final_index_kernel(const unsigned* __restrict__ final_n,
                                    const unsigned long long* __restrict__ final_off, int P, int S,
                                    size_t* __restrict__ ent) {
@@ -1424,8 +1461,9 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
 
   // Final assembly over Ptot partitions' tables (the heavy keys are the
   // last one when used): compact, sort the group keys, gather the outputs.
+  // low_bytes: the key bytes the group-key sort must run (-1: unknown)
   auto assemble = [&](const unsigned long long* w, const unsigned* fn_dev,
-                      const std::vector<unsigned>& fn, int Ptot) {
+                      const std::vector<unsigned>& fn, int Ptot, int low_bytes) {
     // partition offsets, uploaded from pinned memory (the lambda ends with
     // a synchronisation, so the slot is free again on return)
     unsigned long long* foff =
@@ -1454,7 +1492,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     }
     FHIP_CHECK(hipGetLastError());
     // keys are distinct, so the stable sort gives the reference's ascending order
-    sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st);
+    sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st,
+               low_bytes);
     FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
     if (naggs > 0) {
       EmitSet es{};
@@ -1474,7 +1513,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   auto assemble_hot_only = [&]() {  // every row had a heavy key
     DBuf fnb(sizeof(unsigned));
     FHIP_CHECK(hipMemcpyAsync(fnb.get(), &hot_groups, sizeof(unsigned), hipMemcpyHostToDevice, st));
-    assemble(hot_fin.as<unsigned long long>(), fnb.as<unsigned>(), std::vector<unsigned>{hot_groups}, 1);
+    assemble(hot_fin.as<unsigned long long>(), fnb.as<unsigned>(), std::vector<unsigned>{hot_groups}, 1, -1);
   };
   if (hp.used && !cn_pending && cn == 0) {
     assemble_hot_only();
@@ -1615,11 +1654,29 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
           fin.as<unsigned long long>(), fnb.as<unsigned>(), ovf.as<int>());
       FHIP_CHECK(hipGetLastError());
     }
-    // the overflow flag, then the group count of every partition
-    unsigned* rb = static_cast<unsigned*>(pinned_scratch(2, (size_t(P) + 1) * sizeof(unsigned)));
+    if (hp.used) {  // the heavy keys' table is the last partition
+      FHIP_CHECK(hipMemcpyAsync(fin.as<unsigned long long>() + size_t(P) * (S + 1) * stride,
+                                hot_fin.get(), size_t(hot_groups) * stride * 8,
+                                hipMemcpyDeviceToDevice, st));
+      FHIP_CHECK(hipMemcpyAsync(fnb.as<unsigned>() + P, &hot_groups, sizeof(unsigned),
+                                hipMemcpyHostToDevice, st));
+    }
+    // the key range of all the final tables (read back with the counts)
+    DBuf rng(2 * sizeof(unsigned long long));
+    FHIP_CHECK(hipMemsetAsync(rng.get(), 0xFF, sizeof(unsigned long long), st));
+    FHIP_CHECK(hipMemsetAsync(rng.as<unsigned long long>() + 1, 0, sizeof(unsigned long long), st));
+    key_range_kernel<<<unsigned(Ptot), 256, 0, st>>>(
+        fin.as<unsigned long long>(), stride, S, fnb.as<unsigned>(), kb,
+        (ktype == FHIP_I32 || ktype == FHIP_I64) ? 1 : 0, rng.as<unsigned long long>());
+    FHIP_CHECK(hipGetLastError());
+    // the overflow flag, the group count of every partition, then the range
+    const size_t rb_words = (size_t(P) + 2) & ~size_t(1);  // u32 words before the range
+    unsigned* rb = static_cast<unsigned*>(pinned_scratch(2, rb_words * 4 + 16));
+    unsigned long long* rng_h = reinterpret_cast<unsigned long long*>(rb + rb_words);
     FHIP_CHECK(hipMemcpyAsync(rb, ovf.get(), sizeof(int), hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipMemcpyAsync(rb + 1, fnb.get(), size_t(P) * sizeof(unsigned),
                               hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipMemcpyAsync(rng_h, rng.get(), 16, hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipStreamSynchronize(st));
     const int ov = int(rb[0]);
     std::vector<unsigned> fn(rb + 1, rb + 1 + P);
@@ -1632,15 +1689,14 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       group_by_agg_sorted(ktype, keys, n, aggs, naggs, keys_out, capacity, ngroups, st);
       return;
     }
-    if (hp.used) {  // the heavy keys' table is the last partition
-      FHIP_CHECK(hipMemcpyAsync(fin.as<unsigned long long>() + size_t(P) * (S + 1) * stride,
-                                hot_fin.get(), size_t(hot_groups) * stride * 8,
-                                hipMemcpyDeviceToDevice, st));
-      FHIP_CHECK(hipMemcpyAsync(fnb.as<unsigned>() + P, &hot_groups, sizeof(unsigned),
-                                hipMemcpyHostToDevice, st));
-      fn[P] = hot_groups;
+    if (hp.used) fn[P] = hot_groups;
+    // the bytes below the highest one where the smallest and largest keys differ
+    int low_bytes = -1;
+    if (rng_h[0] <= rng_h[1]) {
+      const unsigned long long d = rng_h[0] ^ rng_h[1];
+      low_bytes = d ? (63 - __builtin_clzll(d)) / 8 + 1 : 0;
     }
-    assemble(fin.as<unsigned long long>(), fnb.as<unsigned>(), fn, Ptot);
+    assemble(fin.as<unsigned long long>(), fnb.as<unsigned>(), fn, Ptot, low_bytes);
     return;
   }
 }

@@ -458,34 +458,41 @@ int hist_grid(size_t n, int* two) {
   return int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), size_t(fill_blocks(RS_NT, hbpc))));
 }
 
+// known >= 0: bytes [0, known) are the passes (the caller knows the higher
+// bytes are shared by every key); the host reads no histogram back
 void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* kout,
-             int vbytes, const void* vin, void* vout, size_t n, hipStream_t st) {
+             int vbytes, const void* vin, void* vout, size_t n, hipStream_t st,
+             int known = -1) {
+  if (known >= 0) nbytes = std::min(nbytes, known);
   DBuf hist(size_t(2) * 8 * 256 * sizeof(unsigned long long));
   unsigned long long* ghist = hist.as<unsigned long long>();
   unsigned long long* gbase = ghist + 8 * 256;
   FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
   int two = 1;
   const int grid = hist_grid(n, &two);
-  {
+  if (nbytes > 0) {
     ProfScope prof("radix_hist", st);
     if (kbytes == 4)
       rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist, two);
     else
       rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, mode, desc, nbytes, ghist, two);
     FHIP_CHECK(hipGetLastError());
+    rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
+    FHIP_CHECK(hipGetLastError());
   }
-  rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
-  FHIP_CHECK(hipGetLastError());
-  std::vector<unsigned long long> h(size_t(nbytes) * 256);
-  FHIP_CHECK(hipMemcpyAsync(h.data(), ghist, h.size() * sizeof(unsigned long long),
-                            hipMemcpyDeviceToHost, st));
-  FHIP_CHECK(hipStreamSynchronize(st));
-
   std::vector<int> passes;
-  for (int b = 0; b < nbytes; b++) {
-    unsigned long long mx = 0;
-    for (int d = 0; d < 256; d++) mx = std::max(mx, h[size_t(b) * 256 + d]);
-    if (mx < n) passes.push_back(b);  // a byte shared by every key is a no-op
+  if (known >= 0) {
+    for (int b = 0; b < nbytes; b++) passes.push_back(b);
+  } else {
+    std::vector<unsigned long long> h(size_t(nbytes) * 256);
+    FHIP_CHECK(hipMemcpyAsync(h.data(), ghist, h.size() * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    for (int b = 0; b < nbytes; b++) {
+      unsigned long long mx = 0;
+      for (int d = 0; d < 256; d++) mx = std::max(mx, h[size_t(b) * 256 + d]);
+      if (mx < n) passes.push_back(b);  // a byte shared by every key is a no-op
+    }
   }
   const size_t kbn = n * size_t(kbytes), vbn = n * size_t(vbytes);
   if (passes.empty()) {
@@ -639,7 +646,8 @@ void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int v
 }
 
 void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* vin,
-                void* vout, size_t n, bool positive_only, bool descending, hipStream_t st) {
+                void* vout, size_t n, bool positive_only, bool descending, hipStream_t st,
+                int low_bytes) {
   const int kb = type_bytes(ktype);
   FHIP_REQUIRE(vbytes == 0 || vbytes == 4 || vbytes == 8, FHIP_ERR_ARG,
                "radix_sort: payload must be 0, 4 or 8 bytes");
@@ -687,7 +695,9 @@ void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* 
   } else if (type_is_signed_int(ktype)) {
     mode = KM_FLIPTOP;
   }
-  run_lsd(kb, mode, descending ? 1 : 0, nbytes, kin, kout, vbytes, vin, vout, n, st);
+  const bool known = low_bytes >= 0 && !type_is_float(ktype) && !positive_only;
+  run_lsd(kb, mode, descending ? 1 : 0, nbytes, kin, kout, vbytes, vin, vout, n, st,
+          known ? std::min(low_bytes, kb) : -1);
 }
 
 void sort_pairs_bits(int kbytes, const void* kin, void* kout, int vbytes, const void* vin,

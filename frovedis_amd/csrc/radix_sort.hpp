@@ -24,10 +24,14 @@ struct SortPlan {
 // Stable sort of n keys (+ optional payload of vbytes bytes) from
 // (kin, vin) into (kout, vout).  kout may equal kin (in place).  Follows the
 // reference semantics of radix_sort(K*, V*, size_t, bool) including the
-// positive_only rule and the < 64 insertion path.
+// positive_only rule and the < 64 insertion path.  low_bytes >= 0 (integer
+// keys, positive_only false): the caller knows that every key has the same
+// bytes above the lowest low_bytes (in the sort's order, the sign bit of
+// signed keys flipped), so those passes run without the host reading back
+// the histograms.
 void sort_pairs(int ktype, const void* kin, void* kout, int vbytes,
                 const void* vin, void* vout, size_t n, bool positive_only,
-                bool descending, hipStream_t stream);
+                bool descending, hipStream_t stream, int low_bytes = -1);
 
 // Plain stable ascending sort of unsigned 64-bit (or 32-bit) keys that only
 // uses the bytes in [0, nbytes); for internal callers that already know the
