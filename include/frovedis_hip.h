@@ -92,8 +92,10 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *   "join.load_pct"     load factor (10 to 50 %) of the joins' hash table for
  *       integer keys;
  *   "radix.hist_bpc", "radix.hist_two"  workgroups per CU of the radix
- *       histogram pass (1 to 64), and 0 to aggregate a second shared digit
- *       per wave only when lane 0's digit was shared.
+ *       histogram pass (1 to 64), and how a wave counts its digits: 2
+ *       (default) one atomic for lane 0's digit when at least 8 lanes hold
+ *       it, else one per lane; 1 leaders for two digits; 0 the second
+ *       leader only when lane 0's digit was shared.
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);
