@@ -257,7 +257,8 @@ def test_group_by_agg_narrow_key_range(fa, kt, base):
 def test_group_by_agg_heavy_keys_and_rest(fa):
     # a skewed column (heavy keys aggregated in one pass, the other rows
     # partitioned): Zipf heads plus uniform tails, INT64_MIN among the heavy
-    # rows (never a heavy-table key), and an all-heavy column
+    # rows (never a heavy-table key), an all-heavy column, and the skewed
+    # column sorted (runs of equal keys fill whole sample groups)
     from frovedis_amd import autotune as at
     rng = np.random.default_rng(31)
     n = 1 << 21
@@ -266,7 +267,7 @@ def test_group_by_agg_heavy_keys_and_rest(fa):
     k[::9] = np.iinfo(np.int64).min
     v = dg.normal_np(n)
     iv = rng.integers(-7, 7, n).astype(np.int32)
-    for kk in [k, (k % 5).astype(np.int64)]:
+    for kk in [k, (k % 5).astype(np.int64), np.sort(k)]:
         gi, split, keys = oracle.group_by(kk)
         for pct in [None, 101]:  # default, and the heavy-key pass switched off
             at.tune_set("groupby.hot_min_pct", pct)
