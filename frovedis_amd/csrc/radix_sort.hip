@@ -118,12 +118,18 @@ __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, siz
       if (w0 >= n) break;                          // the same for the whole wave
       const bool valid = w0 + lane < n;
       const unsigned long long vmask = __ballot(valid);
-      for (int b = 0; b < nbytes; b++) {
+      // the key in the sort's order once; the byte loop then shifts by
+      // constants (a descending sort counts 255 - digit)
+      const U xk = key_xform(k[j], mode);
+      const unsigned dmask = desc ? 0xFFu : 0u;
+#pragma unroll
+      for (int b = 0; b < int(sizeof(U)); b++) {
+        if (b >= nbytes) break;
         // lanes holding lane 0's digit, and then those holding the digit of
         // the first other lane, are each counted with one atomic (skewed
         // keys put most of a wave on one or two bins: same-address atomics
         // from one wave serialise); the rest add one each
-        unsigned d = key_digit(k[j], mode, 8 * b, desc);
+        const unsigned d = (unsigned(xk >> (8 * b)) & 0xFFu) ^ dmask;
         unsigned d0 = __shfl(d, 0, 64);  // lane 0 is always valid here
         unsigned long long same = __ballot(valid && d == d0);
         if (two_leaders == 2) {
