@@ -89,7 +89,7 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *       rows);
  *   "multi_group_by.composite"  0 switches off the composite-key path of
  *       fhip_dev_multi_group_by (the column-by-column sorts run instead);
- *   "join.load_pct"     load factor (10 to 50 %) of the joins' hash table for
+ *   "join.load_pct"     load factor (10 to 50 %, default 50) of the joins' hash table for
  *       integer keys;
  *   "radix.hist_bpc", "radix.hist_two"  workgroups per CU of the radix
  *       histogram pass (1 to 64), and how a wave counts its digits: 2
