@@ -111,9 +111,12 @@ __global__ void bound_kernel(const T* __restrict__ s, size_t m, const T* __restr
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nq; i += stride) {
     const T x = q[i];
     size_t lo = 0, hi = m;
+    // the reference's comparisons (core/lower_bound.hpp:44-48,
+    // core/upper_bound.hpp:44-48): lower goes left when x <= s[mid], upper
+    // goes right when x >= s[mid] (a NaN query: m and 0)
     while (lo < hi) {
       size_t mid = (lo + hi) >> 1;
-      bool go_right = UPPER ? !(x < s[mid]) : (s[mid] < x);
+      bool go_right = UPPER ? (x >= s[mid]) : !(x <= s[mid]);
       if (go_right) lo = mid + 1; else hi = mid;
     }
     out[i] = lo;

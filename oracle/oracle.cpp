@@ -209,15 +209,35 @@ int is_unique(const T* k, size_t n) {
 }
 
 // lower_bound / upper_bound: core/lower_bound.hpp:13-101,
-// core/upper_bound.hpp (branch-free binary search; same result as
-// std::lower_bound / std::upper_bound).  Pinned by closed form.
+// core/upper_bound.hpp:13-101.  The reference halves [low, high) with
+// mid = (low + high) / 2 and its own comparisons: lower goes left when
+// `x <= sorted[mid]`, upper goes right when `x >= sorted[mid]`.  On ordered
+// inputs that is std::lower_bound / std::upper_bound; a NaN query goes
+// right in lower_bound (result m) and left in upper_bound (result 0).
+// Pinned by closed form (numpy searchsorted) and those NaN special cases.
+template <class T>
+size_t ref_lower(const T* s, size_t m, T x) {
+  size_t lo = 0, hi = m;
+  while (lo < hi) {
+    size_t mid = (lo + hi) / 2;
+    if (x <= s[mid]) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
 template <class T>
 void lower_bound_t(const T* s, size_t m, const T* q, size_t nq, size_t* out) {
-  for (size_t i = 0; i < nq; i++) out[i] = std::lower_bound(s, s + m, q[i]) - s;
+  for (size_t i = 0; i < nq; i++) out[i] = ref_lower(s, m, q[i]);
 }
 template <class T>
 void upper_bound_t(const T* s, size_t m, const T* q, size_t nq, size_t* out) {
-  for (size_t i = 0; i < nq; i++) out[i] = std::upper_bound(s, s + m, q[i]) - s;
+  for (size_t i = 0; i < nq; i++) {
+    size_t lo = 0, hi = m;
+    while (lo < hi) {
+      size_t mid = (lo + hi) / 2;
+      if (q[i] >= s[mid]) lo = mid + 1; else hi = mid;
+    }
+    out[i] = lo;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -267,8 +287,6 @@ K as_typed(uint64_t bits) {
 }
 
 template <class K>
-bool typed_less(uint64_t a, uint64_t b) { return as_typed<K>(a) < as_typed<K>(b); }
-template <class K>
 bool typed_eq(uint64_t a, uint64_t b) { return as_typed<K>(a) == as_typed<K>(b); }
 
 size_t table_size_for(size_t n) {
@@ -308,13 +326,17 @@ void ht_build_t(HT* h, const void* keys, const uint64_t* vals, size_t n,
       if (!check_unique && vals) h->val[s] = vals[i];  // unspecified in ref
     }
   }
-  // conflict list in row order, then stable sort by typed key
+  // conflict list in row order, then radix_sort(conflict_key, conflict_val)
+  // (:1234): the reference's own sort rules, so float keys follow its
+  // flips (-0.0 after the positives when no conflict key is negative, NaN
+  // by bit pattern) and a list under 64 keys its insertion sort
+  std::vector<K> ck(missed.size());
   std::vector<size_t> order(missed.size());
-  for (size_t i = 0; i < order.size(); i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    return typed_less<K>(load_bits(keys, h->kbytes, missed[a]),
-                         load_bits(keys, h->kbytes, missed[b]));
-  });
+  for (size_t i = 0; i < order.size(); i++) {
+    ck[i] = as_typed<K>(load_bits(keys, h->kbytes, missed[i]));
+    order[i] = i;
+  }
+  sort_kv<K>(ck.data(), order.data(), ck.size(), false);
   h->ckey.resize(missed.size());
   h->cval.resize(missed.size());
   for (size_t i = 0; i < order.size(); i++) {
@@ -327,12 +349,14 @@ void ht_build_t(HT* h, const void* keys, const uint64_t* vals, size_t n,
   if (is_unique_ok) *is_unique_ok = uniq;
 }
 
+// lower_bound(conflict_key, missed_key) (:1430, core/lower_bound.hpp:13-101)
 template <class K>
 size_t conflict_lb(const HT* h, uint64_t b) {
   size_t lo = 0, hi = h->ckey.size();
+  const K x = as_typed<K>(b);
   while (lo < hi) {
     size_t mid = (lo + hi) / 2;
-    if (typed_less<K>(h->ckey[mid], b)) lo = mid + 1; else hi = mid;
+    if (x <= as_typed<K>(h->ckey[mid])) hi = mid; else lo = mid + 1;
   }
   return lo;
 }

@@ -332,3 +332,62 @@ def test_group_by_with_local_idx_closed_form():
     o = np.argsort(col[idx.astype(np.int64)], kind="stable")
     assert np.array_equal(gi, idx[o])
     assert np.array_equal(keys, np.unique(col[idx.astype(np.int64)]))
+
+
+def _colliding_positive(target_bits, ts, k=1 << 20):
+    """A positive double whose bits share the home slot of target_bits."""
+    return (np.uint64(target_bits % ts) + np.uint64(ts) * np.uint64(k)).view(np.float64)
+
+
+@pytest.mark.parametrize("with_negative", [False, True])
+def test_hashtable_float_conflict_list_follows_radix_sort(with_negative):
+    # The conflict list is radix_sort(conflict_key, conflict_val)
+    # (hashtable.hpp:1234), so float keys follow the radix sort's flips: with
+    # no negative conflict key the list is in raw-bit order (-0.0 after NaN
+    # after the positives); with one, in flipped total order.
+    rng = np.random.default_rng(77)
+    n = 2000  # about 240 conflicts: the radix path, not the insertion sort
+    ts = min(p for p in PRIMES if p >= 3 * n)
+    keys = rng.random(n) * 1000.0
+    neg0 = np.uint64(1) << np.uint64(63)
+    nan = np.uint64(0x7FF8000000000000)
+    keys[0] = -0.0
+    keys[1] = np.uint64(nan).view(np.float64)
+    # later rows own those slots, so rows 0 and 1 are conflicts
+    keys[n - 1] = _colliding_positive(int(neg0), ts)
+    keys[n - 2] = _colliding_positive(int(nan), ts, k=1 << 21)
+    if with_negative:
+        keys[2] = -5.0
+        keys[n - 3] = _colliding_positive(int(keys[2:3].view(np.uint64)[0]), ts, k=1 << 22)
+    vals = np.arange(n, dtype=np.uint64)
+    ht = oracle.UniqueHashtable(keys, vals)
+    bits = keys.view(np.uint64)
+    slots = bits % np.uint64(ts)
+    last = {}
+    for i, s in enumerate(slots.tolist()):
+        last[s] = i
+    conf = np.array(sorted(set(range(n)) - set(last.values())), np.int64)
+    assert 0 in conf and 1 in conf and len(conf) >= 64
+    ck, cv = oracle.radix_sort(keys[conf], conf.astype(np.uint64))
+    owned = np.array([keys[last[s]] for s in sorted(last)])
+    got = ht.all_keys()
+    assert np.array_equal(got.view(np.uint64), np.concatenate([owned, ck]).view(np.uint64))
+    tail = ck.view(np.uint64)
+    if not with_negative:
+        assert tail[-1] == neg0 and tail[-2] == nan
+    else:
+        assert tail[0] == keys[2:3].view(np.uint64)[0]
+        pos_neg0 = list(tail).index(neg0)
+        assert pos_neg0 == 1  # right after -5.0, before every positive
+    # a NaN query never compares equal: a miss either way
+    _, miss = ht.lookup(np.array([np.nan]))
+    assert miss.tolist() == [0]
+
+
+def test_lower_upper_bound_nan_query():
+    # lower_bound.hpp:44-48: a NaN query fails `x <= s[mid]` everywhere and
+    # ends at m; upper_bound.hpp:44-48: it fails `x >= s[mid]` and ends at 0
+    s = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
+    q = np.array([np.nan, 2.0, -np.inf, np.inf])
+    assert oracle.lower_bound(s, q).tolist() == [5, 1, 0, 5]
+    assert oracle.lower_bound(s, q, upper=True).tolist() == [0, 2, 0, 5]
