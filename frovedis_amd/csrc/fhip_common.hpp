@@ -75,11 +75,15 @@ inline bool type_is_signed_int(int t) { return t == FHIP_I32 || t == FHIP_I64; }
 inline bool type_is_float(int t) { return t == FHIP_F32 || t == FHIP_F64; }
 
 // ---------------------------------------------------------------------------
-// device workspace: a grow-only cache of device blocks keyed by size class.
-// Blocks are handed out for the duration of one library call; calls on one
-// device are serialised by the caller (one host thread per rank, as the
-// reference's MPI_THREAD_SERIALIZED ranks are), so stream order keeps reuse
-// safe.
+// device workspace: a grow-only cache of device blocks keyed by (device,
+// size class).  Blocks are handed out for the duration of one library call
+// and go back to the cache when the call returns, while its kernels may
+// still be queued.  Reuse is safe because every entry point orders its
+// stream after the previous call's stream on that device (stream_order
+// below), so a block is only handed to work that runs after all the work of
+// the call that released it.  Calls on one device are serialised by the
+// caller (one host thread per rank, as the reference's
+// MPI_THREAD_SERIALIZED ranks are).
 // ---------------------------------------------------------------------------
 class Pool {
  public:
@@ -90,11 +94,21 @@ class Pool {
   size_t cached_bytes() const { return cached_; }
 
  private:
+  struct Key {
+    int dev;
+    size_t size;
+    bool operator<(const Key& o) const { return dev != o.dev ? dev < o.dev : size < o.size; }
+  };
   std::mutex mu_;
-  std::multimap<size_t, void*> free_;
-  std::map<void*, size_t> live_;
+  std::multimap<Key, void*> free_;
+  std::map<void*, Key> live_;
   size_t cached_ = 0;
 };
+
+// Makes stream `st` wait for everything queued so far on the stream of the
+// previous library call on the current device (when that was another
+// stream), then records `st` as the device's last stream.
+void stream_order(hipStream_t st);
 
 // RAII handle over a pool block.
 class DBuf {
@@ -145,7 +159,18 @@ void* pinned_scratch(int slot, size_t bytes);
 // a stream; kept until the process ends.
 hipEvent_t thread_event();
 
-inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+// The stream of an entry point, ordered after the previous call's stream.
+inline hipStream_t as_stream(void* s) {
+  hipStream_t st = static_cast<hipStream_t>(s);
+  stream_order(st);
+  return st;
+}
+
+// Host <-> device copies of host-pointer entry points, staged through two
+// pinned buffers of the calling thread so that the host copy of one chunk
+// overlaps the DMA of the other.  Both return once the data has arrived.
+void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t st);
+void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // device helpers

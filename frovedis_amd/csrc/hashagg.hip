@@ -1036,16 +1036,20 @@ void grouped_agg(int op, int vtype, const void* col, size_t col_n, const size_t*
 // probe ends), then the keys counted at least HOT_MIN_SEEN times are listed
 // as (count, key) pairs after a leading entry count.  The list is in
 // arrival order; the host sorts it by (count descending, key ascending),
-// which fixes the plan.
+// which fixes the plan.  Every slot also counts the sample rows that hold
+// its key (not once per group), so the plan measures how many of the
+// sample's rows the kept keys cover in rows, as its threshold is stated.
 constexpr unsigned HOT_MIN_SEEN = 4;
 constexpr unsigned HOT_GROUP = 64;  // the wave width (or a divisor of it)
 
 __global__ void hot_count_init_kernel(unsigned long long* __restrict__ tk, unsigned* __restrict__ tc,
-                                      unsigned size, unsigned long long* __restrict__ list) {
+                                      unsigned* __restrict__ tr, unsigned size,
+                                      unsigned long long* __restrict__ list) {
   const unsigned stride = gridDim.x * blockDim.x;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < size; i += stride) {
     tk[i] = HA_EMPTY;
     tc[i] = 0;
+    tr[i] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) list[0] = 0;
 }
@@ -1054,7 +1058,7 @@ __global__ void hot_count_init_kernel(unsigned long long* __restrict__ tk, unsig
 template <class K>
 __global__ void hot_count_kernel(const K* __restrict__ keys, size_t gstep, size_t m,
                                  unsigned long long* __restrict__ tk, unsigned* __restrict__ tc,
-                                 unsigned mask) {
+                                 unsigned* __restrict__ tr, unsigned mask) {
   const size_t stride = size_t(gridDim.x) * blockDim.x;
   const unsigned lane = __lane_id();
   const unsigned o = lane % HOT_GROUP;  // position in the group (groups never straddle waves)
@@ -1073,10 +1077,12 @@ __global__ void hot_count_kernel(const K* __restrict__ keys, size_t gstep, size_
       const unsigned ohi = unsigned(__shfl(hi, src < 0 ? 0 : src, 64));
       if (o >= d && ((unsigned long long)ohi << 32 | olo) == kb) first = false;
     }
-    // the empty marker is never a heavy key
-    const bool valid = kb != HA_EMPTY && first;
+    // the empty marker is never a heavy key; every lane with a key finds its
+    // slot (rows), the group's first one also counts the group (presence)
+    const bool has = kb != HA_EMPTY;
+    const bool valid = has && first;
     unsigned h = unsigned(mix64(kb)) & mask;
-    while (valid) {
+    while (has) {
       // a slot's key never changes once set: most lanes of a skewed sample
       // find theirs with a plain read and leave the CAS to the first one
       const unsigned long long cur = __hip_atomic_load(&tk[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1087,29 +1093,37 @@ __global__ void hot_count_kernel(const K* __restrict__ keys, size_t gstep, size_
       }
       h = (h + 1) & mask;
     }
-    // lanes with the same slot add their count with one atomic
-    unsigned long long pending = __ballot(valid);
+    // lanes with the same slot add their counts with one atomic each
+    const unsigned long long firsts = __ballot(valid);
+    unsigned long long pending = __ballot(has);
     while (pending) {
       const int leader = __builtin_ffsll((long long)pending) - 1;
       const unsigned hs = unsigned(__shfl(int(h), leader, 64));
-      const unsigned long long same = __ballot(valid && h == hs) & pending;
-      if (lane == unsigned(leader)) atomicAdd(&tc[hs], unsigned(__popcll(same)));
+      const unsigned long long same = __ballot(has && h == hs) & pending;
+      if (lane == unsigned(leader)) {
+        const unsigned c = unsigned(__popcll(same & firsts));
+        if (c) atomicAdd(&tc[hs], c);
+        atomicAdd(&tr[hs], unsigned(__popcll(same)));
+      }
       pending &= ~same;
     }
   }
 }
 
+// list entries are (presence count, key, sample rows) triples
 __global__ void hot_list_kernel(const unsigned long long* __restrict__ tk,
-                                const unsigned* __restrict__ tc, unsigned size,
-                                unsigned long long* __restrict__ list, unsigned cap) {
+                                const unsigned* __restrict__ tc, const unsigned* __restrict__ tr,
+                                unsigned size, unsigned long long* __restrict__ list,
+                                unsigned cap) {
   const unsigned stride = gridDim.x * blockDim.x;
   for (unsigned s = blockIdx.x * blockDim.x + threadIdx.x; s < size; s += stride) {
     const unsigned c = tc[s];
     if (c < HOT_MIN_SEEN) continue;
     const unsigned long long j = atomicAdd(&list[0], 1ull);
     if (j < cap) {
-      list[1 + 2 * j] = c;
-      list[2 + 2 * j] = tk[s];
+      list[1 + 3 * j] = c;
+      list[2 + 3 * j] = tk[s];
+      list[3 + 3 * j] = tr[s];
     }
   }
 }
@@ -1141,16 +1155,20 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   // n >= 16 m, so the groups (HOT_GROUP rows every gstep rows) never overlap
   const size_t gstep = n / (m / HOT_GROUP);
   const unsigned tsize = unsigned(2 * m), cap = unsigned(m / HOT_MIN_SEEN);
-  DBuf tk(size_t(tsize) * 8), tc(size_t(tsize) * 4), lb((1 + 2 * size_t(cap)) * 8);
+  DBuf tk(size_t(tsize) * 8), tc(size_t(tsize) * 4), trb(size_t(tsize) * 4),
+      lb((1 + 3 * size_t(cap)) * 8);
   unsigned long long* list = lb.as<unsigned long long>();
-  hot_count_init_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tk.as<unsigned long long>(), tc.as<unsigned>(), tsize, list);
+  unsigned long long* tkp = tk.as<unsigned long long>();
+  unsigned* tcp = tc.as<unsigned>();
+  unsigned* trp = trb.as<unsigned>();
+  hot_count_init_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tkp, tcp, trp, tsize, list);
   switch (ktype) {
-    case FHIP_I32: hot_count_kernel<int32_t><<<grid_for_n(m), 256, 0, st>>>((const int32_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
-    case FHIP_U32: hot_count_kernel<uint32_t><<<grid_for_n(m), 256, 0, st>>>((const uint32_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
-    case FHIP_I64: hot_count_kernel<int64_t><<<grid_for_n(m), 256, 0, st>>>((const int64_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
-    default: hot_count_kernel<uint64_t><<<grid_for_n(m), 256, 0, st>>>((const uint64_t*)keys, gstep, m, tk.as<unsigned long long>(), tc.as<unsigned>(), tsize - 1); break;
+    case FHIP_I32: hot_count_kernel<int32_t><<<grid_for_n(m), 256, 0, st>>>((const int32_t*)keys, gstep, m, tkp, tcp, trp, tsize - 1); break;
+    case FHIP_U32: hot_count_kernel<uint32_t><<<grid_for_n(m), 256, 0, st>>>((const uint32_t*)keys, gstep, m, tkp, tcp, trp, tsize - 1); break;
+    case FHIP_I64: hot_count_kernel<int64_t><<<grid_for_n(m), 256, 0, st>>>((const int64_t*)keys, gstep, m, tkp, tcp, trp, tsize - 1); break;
+    default: hot_count_kernel<uint64_t><<<grid_for_n(m), 256, 0, st>>>((const uint64_t*)keys, gstep, m, tkp, tcp, trp, tsize - 1); break;
   }
-  hot_list_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tk.as<unsigned long long>(), tc.as<unsigned>(), tsize, list, cap);
+  hot_list_kernel<<<grid_for_n(tsize), 256, 0, st>>>(tkp, tcp, trp, tsize, list, cap);
   FHIP_CHECK(hipGetLastError());
   // the count and the first 4096 candidates in one copy (skewed data has
   // about a thousand), the rest only when there are more
@@ -1158,40 +1176,47 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   // (pinned slot 2: its next user, the group-by's final readback, comes
   // after this copy has been waited for)
   unsigned long long* lst =
-      static_cast<unsigned long long*>(pinned_scratch(2, (1 + 2 * size_t(cap)) * 8));
-  FHIP_CHECK(hipMemcpyAsync(lst, list, (1 + 2 * first) * 8, hipMemcpyDeviceToHost, st));
+      static_cast<unsigned long long*>(pinned_scratch(2, (1 + 3 * size_t(cap)) * 8));
+  FHIP_CHECK(hipMemcpyAsync(lst, list, (1 + 3 * first) * 8, hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
   const size_t nc = std::min<size_t>(size_t(lst[0]), cap);
   if (nc > first) {
-    FHIP_CHECK(hipMemcpyAsync(lst + 1 + 2 * first, list + 1 + 2 * first,
-                              2 * (nc - first) * 8, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipMemcpyAsync(lst + 1 + 3 * first, list + 1 + 3 * first,
+                              3 * (nc - first) * 8, hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipStreamSynchronize(st));
   }
   // (count, key) by count descending, then key ascending: a counting sort on
-  // the count (at most m / HOT_GROUP), then the keys of each count sorted
+  // the count (at most m / HOT_GROUP), then the keys of each count sorted;
+  // the sample rows of every key ride along for the coverage below
   std::vector<std::pair<size_t, unsigned long long>> runs(nc);
+  std::vector<unsigned long long> rows_of(nc);
   {
     size_t maxc = 0;
-    for (size_t j = 0; j < nc; j++) maxc = std::max<size_t>(maxc, size_t(lst[1 + 2 * j]));
+    for (size_t j = 0; j < nc; j++) maxc = std::max<size_t>(maxc, size_t(lst[1 + 3 * j]));
     std::vector<size_t> start(maxc + 2, 0);  // start[c]: first position of count c
-    for (size_t j = 0; j < nc; j++) start[maxc - size_t(lst[1 + 2 * j]) + 1]++;
+    for (size_t j = 0; j < nc; j++) start[maxc - size_t(lst[1 + 3 * j]) + 1]++;
     for (size_t c = 1; c < start.size(); c++) start[c] += start[c - 1];
-    std::vector<unsigned long long> ks(nc);
-    for (size_t j = 0; j < nc; j++) ks[start[maxc - size_t(lst[1 + 2 * j])]++] = lst[2 + 2 * j];
+    std::vector<std::pair<unsigned long long, unsigned long long>> ks(nc);  // (key, rows)
+    for (size_t j = 0; j < nc; j++)
+      ks[start[maxc - size_t(lst[1 + 3 * j])]++] = {lst[2 + 3 * j], lst[3 + 3 * j]};
     // start[maxc - c] now ends the run of count c
     size_t b = 0;
     for (size_t r = 0; r <= maxc; r++) {
       const size_t e = start[r];
       if (e - b > 1) std::sort(ks.begin() + b, ks.begin() + e);
-      for (size_t j = b; j < e; j++) runs[j] = {maxc - r, ks[j]};
+      for (size_t j = b; j < e; j++) {
+        runs[j] = {maxc - r, ks[j].first};
+        rows_of[j] = ks[j].second;
+      }
       b = e;
     }
   }
   // two-choice buckets of two slots place keys up to a load near 90 %; the
   // plan fills at most 80 % of the slots
   if (runs.size() > size_t(Sh) * 4 / 5) runs.resize(size_t(Sh) * 4 / 5);
+  // sample rows holding a kept key, against the sample's m rows
   size_t covered = 0;
-  for (auto& r : runs) covered += r.first;
+  for (size_t j = 0; j < runs.size(); j++) covered += size_t(rows_of[j]);
   const long long min_pct = tune_get("groupby.hot_min_pct", 30);
   if (runs.empty() || covered * 100 < size_t(std::max(0LL, min_pct)) * m) return h;
   h.Sh = Sh;
@@ -1226,15 +1251,19 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   return h;
 }
 
+static __global__ void ha_iota_kernel(size_t* __restrict__ p, size_t n) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = i;
+}
+
 // More groups than the partitioned tables hold: sort-based group_by plus
 // per-column aggregation (same results).
 void group_by_agg_sorted(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
                          void* keys_out, size_t capacity, size_t* ngroups, hipStream_t st) {
   const int kb = type_bytes(ktype);
   DBuf idx(n * 8), split((n + 1) * 8), gk(n * size_t(kb));
-  std::vector<size_t> iota(n);
-  for (size_t i = 0; i < n; i++) iota[i] = i;
-  FHIP_CHECK(hipMemcpyAsync(idx.get(), iota.data(), n * 8, hipMemcpyHostToDevice, st));
+  ha_iota_kernel<<<grid_for_n(n), 256, 0, st>>>(idx.as<size_t>(), n);
+  FHIP_CHECK(hipGetLastError());
   size_t ns = 0;
   group_by(ktype, keys, idx.as<size_t>(), n, split.as<size_t>(), &ns, gk.get(), st);
   const size_t G = ns - 1;
@@ -1247,7 +1276,6 @@ void group_by_agg_sorted(int ktype, const void* keys, size_t n, const fhip_agg* 
     grouped_agg(aggs[i].op, aggs[i].op == OP_COUNT ? ktype : aggs[i].vtype,
                 aggs[i].op == OP_COUNT ? keys : aggs[i].col, n, idx.as<size_t>(), n,
                 split.as<size_t>(), ns, nullptr, 0, aggs[i].out, st);
-  FHIP_CHECK(hipStreamSynchronize(st));  // iota is a host local
 }
 
 // Sorted-by-key result of a fused group-by; see fhip_dev_group_by_agg.

@@ -1,6 +1,7 @@
 // Host-pointer entry points (the reference's std::vector storage) over the
-// device entry points: inputs are copied to the device, the device path
-// runs on the default stream, results are copied back.
+// device entry points: inputs are copied to the device through pinned
+// staging, the device path runs on the default stream, results are copied
+// back the same way.
 #include "fhip_common.hpp"
 #include "hashtable.hpp"
 
@@ -24,15 +25,15 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
                  hipStream_t st);
 namespace {
 
+// the host entry points run on the default stream, ordered after the
+// previous call's stream, and copy through the pinned staging chunks
 DBuf to_dev(const void* h, size_t bytes) {
   DBuf d(bytes);
-  if (bytes) FHIP_CHECK(hipMemcpy(d.get(), h, bytes, hipMemcpyHostToDevice));
+  copy_h2d(d.get(), h, bytes, as_stream(nullptr));
   return d;
 }
 
-void to_host(void* h, const void* d, size_t bytes) {
-  if (bytes) FHIP_CHECK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
-}
+void to_host(void* h, const void* d, size_t bytes) { copy_d2h(h, d, bytes, as_stream(nullptr)); }
 
 void rethrow(int rc) {
   if (rc != FHIP_OK) throw Error(rc, fhip_last_error());

@@ -24,12 +24,14 @@ Pool& Pool::get() {
 
 void* Pool::alloc(size_t bytes) {
   size_t sz = ((bytes + kPoolAlign - 1) / kPoolAlign) * kPoolAlign;
+  int dev = 0;
+  FHIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(mu_);
-  // best fit among cached blocks up to 2x the request
-  auto it = free_.lower_bound(sz);
-  if (it != free_.end() && it->first <= 2 * sz) {
+  // best fit among this device's cached blocks up to 2x the request
+  auto it = free_.lower_bound(Key{dev, sz});
+  if (it != free_.end() && it->first.dev == dev && it->first.size <= 2 * sz) {
     void* p = it->second;
-    size_t have = it->first;
+    const Key have = it->first;
     free_.erase(it);
     live_[p] = have;
     return p;
@@ -37,11 +39,15 @@ void* Pool::alloc(size_t bytes) {
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, sz);
   if (e != hipSuccess) {
-    // drop the cache and retry once
+    // drop this device's cache and retry once
     (void)hipGetLastError();
     FHIP_CHECK(hipDeviceSynchronize());
-    for (auto& kv : free_) { (void)hipFree(kv.second); cached_ -= kv.first; }
-    free_.clear();
+    for (auto i = free_.begin(); i != free_.end();) {
+      if (i->first.dev != dev) { ++i; continue; }
+      (void)hipFree(i->second);
+      cached_ -= i->first.size;
+      i = free_.erase(i);
+    }
     e = hipMalloc(&p, sz);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -49,7 +55,7 @@ void* Pool::alloc(size_t bytes) {
                                       " bytes failed");
     }
   }
-  live_[p] = sz;
+  live_[p] = Key{dev, sz};
   cached_ += sz;
   return p;
 }
@@ -63,9 +69,44 @@ void Pool::release(void* p) {
 }
 
 void Pool::trim() {
+  int cur = 0;
+  FHIP_CHECK(hipGetDevice(&cur));
   std::lock_guard<std::mutex> lk(mu_);
-  for (auto& kv : free_) { (void)hipFree(kv.second); cached_ -= kv.first; }
+  for (auto& kv : free_) {
+    // each block is freed with its own device current
+    if (kv.first.dev != cur) (void)hipSetDevice(kv.first.dev);
+    (void)hipFree(kv.second);
+    if (kv.first.dev != cur) (void)hipSetDevice(cur);
+    cached_ -= kv.first.size;
+  }
   free_.clear();
+}
+
+namespace {
+struct StreamState {
+  bool used = false;
+  hipStream_t last = nullptr;
+  hipEvent_t ev = nullptr;
+};
+std::mutex g_stream_mu;
+std::map<int, StreamState>& stream_states() {
+  static auto* m = new std::map<int, StreamState>();  // leaked: outlives static dtors
+  return *m;
+}
+}  // namespace
+
+void stream_order(hipStream_t st) {
+  int dev = 0;
+  FHIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_stream_mu);
+  StreamState& s = stream_states()[dev];
+  if (s.used && s.last != st) {
+    if (!s.ev) FHIP_CHECK(hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    FHIP_CHECK(hipEventRecord(s.ev, s.last));
+    FHIP_CHECK(hipStreamWaitEvent(st, s.ev, 0));
+  }
+  s.used = true;
+  s.last = st;
 }
 
 HostStage::HostStage(size_t bytes) {
@@ -103,6 +144,76 @@ void* pinned_scratch(int slot, size_t bytes) {
     g_pin->cap[slot] = c;
   }
   return g_pin->p[slot];
+}
+
+namespace {
+// two pinned chunks and their events per thread (leaked like the slots)
+constexpr size_t kStageChunk = size_t(16) << 20;
+constexpr size_t kStageDirect = size_t(1) << 20;  // smaller copies go straight
+struct Staging {
+  void* buf[2] = {};
+  hipEvent_t ev[2] = {};
+};
+thread_local Staging* g_stage = nullptr;
+
+Staging& staging() {
+  if (!g_stage) {
+    g_stage = new Staging();
+    for (int i = 0; i < 2; i++) {
+      FHIP_CHECK(hipHostMalloc(&g_stage->buf[i], kStageChunk, hipHostMallocDefault));
+      FHIP_CHECK(hipEventCreateWithFlags(&g_stage->ev[i], hipEventDisableTiming));
+      FHIP_CHECK(hipEventRecord(g_stage->ev[i], nullptr));
+    }
+  }
+  return *g_stage;
+}
+}  // namespace
+
+void copy_h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return;
+  if (bytes <= kStageDirect) {
+    FHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
+  Staging& s = staging();
+  char* d = static_cast<char*>(dst);
+  const char* h = static_cast<const char*>(src);
+  int b = 0;
+  for (size_t off = 0; off < bytes; off += kStageChunk, b ^= 1) {
+    const size_t len = std::min(kStageChunk, bytes - off);
+    FHIP_CHECK(hipEventSynchronize(s.ev[b]));  // the chunk's previous DMA is done
+    std::memcpy(s.buf[b], h + off, len);
+    FHIP_CHECK(hipMemcpyAsync(d + off, s.buf[b], len, hipMemcpyHostToDevice, st));
+    FHIP_CHECK(hipEventRecord(s.ev[b], st));
+  }
+  FHIP_CHECK(hipStreamSynchronize(st));
+}
+
+void copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return;
+  if (bytes <= kStageDirect) {
+    FHIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
+  Staging& s = staging();
+  char* h = static_cast<char*>(dst);
+  const char* d = static_cast<const char*>(src);
+  const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t c) {
+    const size_t off = c * kStageChunk, len = std::min(kStageChunk, bytes - off);
+    FHIP_CHECK(hipMemcpyAsync(s.buf[c & 1], d + off, len, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipEventRecord(s.ev[c & 1], st));
+  };
+  FHIP_CHECK(hipStreamSynchronize(st));  // earlier users of both chunks are done
+  issue(0);
+  for (size_t c = 0; c < chunks; c++) {
+    if (c + 1 < chunks) issue(c + 1);  // its buffer was emptied in step c - 1
+    FHIP_CHECK(hipEventSynchronize(s.ev[c & 1]));
+    const size_t off = c * kStageChunk, len = std::min(kStageChunk, bytes - off);
+    std::memcpy(h + off, s.buf[c & 1], len);
+  }
 }
 
 hipEvent_t thread_event() {

@@ -164,11 +164,10 @@ int fhip_prefix_sum(int type, const void* in, void* out, size_t n) {
     if (n == 0) return;
     const size_t b = size_t(fhip::type_bytes(type)) * n;
     fhip::DBuf di(b), dout(b);
-    hipStream_t st = nullptr;
-    FHIP_CHECK(hipMemcpyAsync(di.get(), in, b, hipMemcpyHostToDevice, st));
+    hipStream_t st = fhip::as_stream(nullptr);
+    fhip::copy_h2d(di.get(), in, b, st);
     fhip::prefix_sum(type, di.get(), dout.get(), n, st);
-    FHIP_CHECK(hipMemcpyAsync(out, dout.get(), b, hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
+    fhip::copy_d2h(out, dout.get(), b, st);
   });
 }
 
@@ -185,12 +184,23 @@ int fhip_set_separate(int type, const void* sorted, size_t n, size_t* out, size_
     FHIP_REQUIRE(out && out_n, FHIP_ERR_ARG, "set_separate: null pointer");
     const size_t b = size_t(fhip::type_bytes(type)) * n;
     fhip::DBuf dk(b), dout((n + 1) * sizeof(size_t));
-    hipStream_t st = nullptr;
-    if (b) FHIP_CHECK(hipMemcpyAsync(dk.get(), sorted, b, hipMemcpyHostToDevice, st));
+    hipStream_t st = fhip::as_stream(nullptr);
+    fhip::copy_h2d(dk.get(), sorted, b, st);
     size_t c = fhip::set_separate(type, dk.get(), n, dout.as<size_t>(), st);
-    FHIP_CHECK(hipMemcpyAsync(out, dout.get(), c * sizeof(size_t), hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
+    fhip::copy_d2h(out, dout.get(), c * sizeof(size_t), st);
     *out_n = c;
+  });
+}
+
+int fhip_set_is_unique(int type, const void* sorted, size_t n, int* is_unique) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(is_unique && (n == 0 || sorted), FHIP_ERR_ARG, "set_is_unique: null pointer");
+    const size_t b = size_t(fhip::type_bytes(type)) * n;
+    fhip::DBuf dk(b);
+    hipStream_t st = fhip::as_stream(nullptr);
+    fhip::copy_h2d(dk.get(), sorted, b, st);
+    const int rc = fhip_dev_set_is_unique(type, dk.get(), n, is_unique, nullptr);
+    if (rc != FHIP_OK) throw fhip::Error(rc, fhip_last_error());
   });
 }
 

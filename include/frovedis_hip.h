@@ -120,7 +120,11 @@ int fhip_dev_radix_sort(int ktype, void* keys, int vbytes, void* vals,
 /* ---- prefix_sum ---------------------------------------------------------
  * Inclusive scan, prefix_sum(const T*, T*, size_t) (core/prefix_sum.hpp:
  * 67-88).  Integer types are exact; float types are summed in a fixed tree
- * order (not the reference's sequential order). */
+ * order (not the reference's sequential order).
+ * Host-pointer forms (no fhip_dev_ prefix) run on the default stream and
+ * copy through pinned staging buffers of the calling thread; every entry
+ * point orders its stream after the previous call's stream on the device,
+ * so workspace blocks reused across streams never overlap in time. */
 int fhip_prefix_sum(int type, const void* in, void* out, size_t n);
 int fhip_dev_prefix_sum(int type, const void* in, void* out, size_t n,
                         void* stream);
@@ -133,7 +137,9 @@ int fhip_set_separate(int type, const void* sorted, size_t n, size_t* out,
                       size_t* out_n);
 int fhip_dev_set_separate(int type, const void* sorted, size_t n,
                           size_t* out, size_t* out_n, void* stream);
-/* set_is_unique (core/set_operations.hpp:184-197) of a sorted array. */
+/* set_is_unique (core/set_operations.hpp:184-197) of a sorted array; the
+ * host form copies the keys to the device and runs the same kernel. */
+int fhip_set_is_unique(int type, const void* sorted, size_t n, int* is_unique);
 int fhip_dev_set_is_unique(int type, const void* sorted, size_t n,
                            int* is_unique, void* stream);
 

@@ -59,7 +59,9 @@ std::vector<T> set_unique(const std::vector<T>& key) {
 
 template <class T>
 int set_is_unique(const T* keyp, size_t size) {
-  return set_separate(keyp, size).size() == size + 1 || size == 0;
+  int u = 1;
+  fhip_detail::check(fhip_set_is_unique(fhip_detail::key_code<T>(), keyp, size, &u));
+  return u;
 }
 
 template <class T>

@@ -45,6 +45,14 @@ int main() {
   EXPECT(sep.front() == 0 && sep.back() == ks.size());
   for (size_t i = 1; i + 1 < sep.size(); i++) EXPECT(ks[sep[i]] != ks[sep[i] - 1]);
   EXPECT(frovedis::set_separate(std::vector<int>()).size() == 1);
+  // set_is_unique through the device kernel
+  EXPECT(frovedis::set_is_unique(ks) == (sep.size() == ks.size() + 1));
+  std::vector<long> inc(3000);
+  std::iota(inc.begin(), inc.end(), -1000);
+  EXPECT(frovedis::set_is_unique(inc) == 1);
+  inc[2999] = inc[2998];
+  EXPECT(frovedis::set_is_unique(inc) == 0);
+  EXPECT(frovedis::set_is_unique(std::vector<int>()) == 1);
   // unique_hashtable with is_unique_ok
   std::vector<long> hk(5000);
   for (size_t i = 0; i < hk.size(); i++) hk[i] = long(i) * 7919 % 100003;

@@ -2,6 +2,7 @@
 type, payload width, positive_only, descending, the < 64 insertion path),
 prefix_sum, set_separate, set_is_unique and lower/upper_bound.  Integer and
 index results are compared bit-exact."""
+import ctypes
 import zlib
 
 import numpy as np
@@ -170,3 +171,31 @@ def test_lower_upper_bound(fa, dt):
     assert np.array_equal(fa.upper_bound(ds, dq).numpy(), oracle.lower_bound(s, q, upper=True))
     empty = fa.DeviceArray.from_numpy(np.zeros(0, dt))
     assert np.array_equal(fa.lower_bound(empty, dq).numpy(), np.zeros(len(q), np.uint64))
+
+
+def test_back_to_back_sorts_on_two_streams(fa):
+    # workspace blocks go back to the cache while a call's kernels are still
+    # queued; a call on another stream is ordered after them, so two sorts
+    # issued on two streams without a host wait both stay exact
+    import torch
+    rng = np.random.default_rng(404)
+    n = 1 << 22
+    ka = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+    kb = rng.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+    va = np.arange(n, dtype=np.uint64)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = []
+    for rep in range(3):
+        da, dva = fa.DeviceArray.from_numpy(ka), fa.DeviceArray.from_numpy(va)
+        db, dvb = fa.DeviceArray.from_numpy(kb), fa.DeviceArray.from_numpy(va)
+        torch.cuda.synchronize()
+        fa.radix_sort(da, dva, stream=ctypes.c_void_p(s1.cuda_stream))
+        fa.radix_sort(db, dvb, stream=ctypes.c_void_p(s2.cuda_stream))
+        fa.radix_sort(da, dva, descending=True, stream=ctypes.c_void_p(s2.cuda_stream))
+        torch.cuda.synchronize()
+        outs.append((da.numpy(), dva.numpy(), db.numpy(), dvb.numpy()))
+    ea, eva = oracle.radix_sort(ka, va, descending=True)
+    eb, evb = oracle.radix_sort(kb, va)
+    for a, av, b, bv in outs:
+        assert np.array_equal(a, ea) and np.array_equal(av, eva)
+        assert np.array_equal(b, eb) and np.array_equal(bv, evb)

@@ -320,3 +320,77 @@ def test_group_by_agg_heavy_keys_accumulator_counts(fa, aggs, hint):
             assert np.array_equal(got, oracle.grouped_agg("max", v, gi, split))
         else:
             assert np.array_equal(got, oracle.grouped_agg("min", v, gi, split))
+
+
+def _prof_count(fa, name):
+    import ctypes
+    t, c = ctypes.c_double(0), ctypes.c_size_t(0)
+    fa.check(fa.lib().fhip_prof_query(name.encode(), ctypes.byref(t), ctypes.byref(c)))
+    return c.value
+
+
+@pytest.mark.parametrize("hint", [0, 16])
+def test_group_by_agg_low_cardinality_runs_heavy_pass_only(fa, hint):
+    # five keys: the heavy-key plan covers every sampled row, so the one
+    # heavy-key pass aggregates the whole column and the partitioned
+    # aggregation never runs (the assembly of the heavy table alone)
+    n = 1 << 21
+    rng = np.random.default_rng(55)
+    k = (rng.integers(0, 1 << 40, n) % 5).astype(np.int64) - 2
+    v = rng.standard_normal(n)
+    iv = rng.integers(-100, 100, n).astype(np.int32)
+    fa.check(fa.lib().fhip_prof_enable(1))
+    gk, (s, c, mx) = fa.group_by_agg(fa.DeviceArray.from_numpy(k),
+                                     [("sum", fa.DeviceArray.from_numpy(v)), ("count", None),
+                                      ("max", fa.DeviceArray.from_numpy(iv))], table_hint=hint)
+    hot, agg = _prof_count(fa, "groupby_hot"), _prof_count(fa, "hashagg")
+    fa.check(fa.lib().fhip_prof_enable(0))
+    # (with a hint the scatter is queued before the cold-row count is back,
+    # and runs over empty regions; no cold row is ever aggregated)
+    assert hot == 1 and agg == 0
+    gi, split, keys = oracle.group_by(k)
+    assert np.array_equal(gk.numpy(), keys)
+    assert np.array_equal(c.numpy(), oracle.grouped_agg("count", v, gi, split))
+    assert np.array_equal(mx.numpy(), oracle.grouped_agg("max", iv, gi, split))
+    _check_fp(s.numpy(), oracle.grouped_agg("sum", v, gi, split), _abs_mass(v, gi, split),
+              FP64_TOL)
+
+
+def test_group_by_agg_skewed_column_runs_heavy_pass(fa):
+    # Zipf(1.1) over 2^16 values: the sampled heavy keys cover most rows
+    from frovedis_amd import datagen as dg
+    n = 1 << 21
+    k = dg.zipf_keys_np(n, n_values=1 << 16)
+    fa.check(fa.lib().fhip_prof_enable(1))
+    gk, (c,) = fa.group_by_agg(fa.DeviceArray.from_numpy(k), [("count", None)])
+    hot = _prof_count(fa, "groupby_hot")
+    fa.check(fa.lib().fhip_prof_enable(0))
+    assert hot == 1
+    gi, split, keys = oracle.group_by(k)
+    assert np.array_equal(gk.numpy(), keys)
+    assert np.array_equal(c.numpy(), oracle.grouped_agg("count", k, gi, split))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_fp_max_min_with_signed_zeros_and_nan(fa, dt):
+    # groups holding both -0.0 and +0.0 (and NaN rows, never selected):
+    # the sort path reduces in row order, so it keeps the reference's first
+    # zero bit for bit (`if (m < v)`, dfcolumn_grouped_agg.cc:366-452); the
+    # hash path returns an equal value (either zero)
+    n = 1 << 16
+    rng = np.random.default_rng(66)
+    k = rng.integers(0, 300, n).astype(np.int64)
+    v = np.where(rng.random(n) < 0.5, dt(-0.0), dt(0.0)).astype(dt)
+    v[rng.random(n) < 0.05] = np.nan
+    v[rng.random(n) < 0.3] = -dt(1.5)
+    gi, split, keys = oracle.group_by(k)
+    for op in ("max", "min"):
+        ref = oracle.grouped_agg(op, v, gi, split)
+        got = fa.grouped_agg(op, fa.DeviceArray.from_numpy(v), fa.DeviceArray.from_numpy(gi),
+                             fa.DeviceArray.from_numpy(split)).numpy()
+        assert np.array_equal(got.view(np.uint32 if dt == np.float32 else np.uint64),
+                              ref.view(np.uint32 if dt == np.float32 else np.uint64)), op
+        gk, (h,) = fa.group_by_agg(fa.DeviceArray.from_numpy(k),
+                                   [(op, fa.DeviceArray.from_numpy(v))])
+        assert np.array_equal(gk.numpy(), keys)
+        assert np.array_equal(h.numpy(), ref), op  # == on values: -0.0 == 0.0

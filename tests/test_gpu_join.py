@@ -155,3 +155,38 @@ def test_equi_join_scaled_join_config_properties(fa):
     assert torch.equal(build[slo.t], pk[sro.t])
     kk = build[slo.t]
     assert bool((kk[1:] >= kk[:-1]).all())  # sorted by key
+
+
+@pytest.mark.parametrize("with_negative", [False, True])
+def test_hashtable_float_signed_zero_and_nan_conflicts(fa, with_negative):
+    # -0.0 and NaN in the conflict list: the list follows the radix sort's
+    # flips (hashtable.hpp:1234), which all_keys exposes, and lookups use
+    # the reference's lower_bound comparisons; compared with the oracle
+    n = 2000
+    rng = np.random.default_rng(88)
+    ts = 8192 + 27
+    keys = rng.random(n) * 1000.0
+    neg0, nan = (1 << 63), 0x7FF8000000000000
+
+    def colliding(bits, k):
+        return np.array([bits % ts + ts * k], np.uint64).view(np.float64)[0]
+    keys[0] = -0.0
+    keys[1] = np.array([nan], np.uint64).view(np.float64)[0]
+    keys[n - 1] = colliding(neg0, 1 << 20)
+    keys[n - 2] = colliding(nan, 1 << 21)
+    if with_negative:
+        keys[2] = -5.0
+        keys[n - 3] = colliding(int(keys[2:3].view(np.uint64)[0]), 1 << 22)
+    vals = np.arange(n, dtype=np.uint64) + np.uint64(10)
+    ref = oracle.UniqueHashtable(keys, vals)
+    assert ref.table_size == ts
+    ht = fa.UniqueHashtable(fa.DeviceArray.from_numpy(keys), fa.DeviceArray.from_numpy(vals))
+    assert ht.conflict_size == ref.conflict_size
+    assert np.array_equal(ht.all_keys().numpy().view(np.uint64), ref.all_keys().view(np.uint64))
+    q = np.concatenate([keys, [0.0, -0.0, np.nan, 7.25]])
+    gv, gm = ht.lookup(fa.DeviceArray.from_numpy(q))
+    ev, em = ref.lookup(q)
+    assert np.array_equal(gm.numpy(), em)
+    assert np.array_equal(gv.numpy(), ev)
+    assert np.array_equal(ht.check_existence(fa.DeviceArray.from_numpy(q)).numpy(),
+                          ref.check_existence(q))
