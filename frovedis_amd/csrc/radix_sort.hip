@@ -482,12 +482,21 @@ int hist_grid(size_t n, int* two) {
 
 // known >= 0: bytes [0, known) are the passes (the caller knows the higher
 // bytes are shared by every key); the host reads no histogram back
+// ar: carve the temporaries from a caller's region (one temporary buffer:
+// an in-place sort with an odd pass count ends with a copy back)
 void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* kout,
              int vbytes, const void* vin, void* vout, size_t n, hipStream_t st,
-             int known = -1) {
+             int known = -1, Arena* ar = nullptr) {
   if (known >= 0) nbytes = std::min(nbytes, known);
-  DBuf hist(size_t(2) * 8 * 256 * sizeof(unsigned long long));
-  unsigned long long* ghist = hist.as<unsigned long long>();
+  const size_t hist_bytes = size_t(2) * 8 * 256 * sizeof(unsigned long long);
+  DBuf hist;
+  unsigned long long* ghist;
+  if (ar) {
+    ghist = static_cast<unsigned long long*>(ar->take(hist_bytes));
+  } else {
+    hist.reset(hist_bytes);
+    ghist = hist.as<unsigned long long>();
+  }
   unsigned long long* gbase = ghist + 8 * 256;
   FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
   int two = 1;
@@ -502,41 +511,53 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
     rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
     FHIP_CHECK(hipGetLastError());
   }
-  std::vector<int> passes;
+  int passes[8];
+  int P = 0;
   if (known >= 0) {
-    for (int b = 0; b < nbytes; b++) passes.push_back(b);
-  } else {
-    std::vector<unsigned long long> h(size_t(nbytes) * 256);
-    FHIP_CHECK(hipMemcpyAsync(h.data(), ghist, h.size() * sizeof(unsigned long long),
+    for (int b = 0; b < nbytes; b++) passes[P++] = b;
+  } else if (nbytes > 0) {
+    unsigned long long h[8 * 256];  // on the stack: this path allocates nothing
+    FHIP_CHECK(hipMemcpyAsync(h, ghist, size_t(nbytes) * 256 * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
     FHIP_CHECK(hipStreamSynchronize(st));
     for (int b = 0; b < nbytes; b++) {
       unsigned long long mx = 0;
       for (int d = 0; d < 256; d++) mx = std::max(mx, h[size_t(b) * 256 + d]);
-      if (mx < n) passes.push_back(b);  // a byte shared by every key is a no-op
+      if (mx < n) passes[P++] = b;  // a byte shared by every key is a no-op
     }
   }
   const size_t kbn = n * size_t(kbytes), vbn = n * size_t(vbytes);
-  if (passes.empty()) {
+  if (P == 0) {
     if (kout != kin) {
       FHIP_CHECK(hipMemcpyAsync(kout, kin, kbn, hipMemcpyDeviceToDevice, st));
       if (vbytes) FHIP_CHECK(hipMemcpyAsync(vout, vin, vbn, hipMemcpyDeviceToDevice, st));
     }
     return;
   }
-  const int P = int(passes.size());
   const bool inplace = (kout == kin);
   FHIP_REQUIRE(vbytes == 0 || inplace == (vout == vin), FHIP_ERR_ARG,
                "radix_sort: keys and payload must both be in place or both out of place");
   // destinations so that the last pass lands in (kout, vout) and no pass
   // reads and writes the same buffer
-  DBuf t1k(kbn), t1v(vbn), t2k, t2v;
-  std::vector<int> dest(P);  // 0 = out, 1 = t1, 2 = t2
+  DBuf t1k_own, t1v_own, t2k, t2v;
+  void* t1k = nullptr;
+  void* t1v = nullptr;
+  if (ar) {
+    t1k = ar->take(kbn);
+    t1v = vbn ? ar->take(vbn) : nullptr;
+  } else {
+    t1k_own.reset(kbn);
+    t1v_own.reset(vbn);
+    t1k = t1k_own.get();
+    t1v = t1v_own.get();
+  }
+  int dest[8];  // 0 = out, 1 = t1, 2 = t2
   bool copy_back = false;
   if (!inplace) {
     for (int i = 0; i < P; i++) dest[i] = ((P - 1 - i) % 2 == 0) ? 0 : 1;
-  } else if (P == 1) {
-    dest[0] = 1;
+  } else if (P == 1 || (P % 2 == 1 && ar)) {
+    // one temporary: alternate t1 / out from t1, the last pass lands in t1
+    for (int i = 0; i < P; i++) dest[i] = (i % 2 == 0) ? 1 : 0;
     copy_back = true;
   } else if (P % 2 == 0) {
     for (int i = 0; i < P; i++) dest[i] = (i % 2 == 0) ? 1 : 0;
@@ -547,16 +568,23 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
     dest[1] = 2;
     for (int i = 2; i < P; i++) dest[i] = (i % 2 == 0) ? 0 : 1;
   }
-  auto kbuf = [&](int d) -> void* { return d == 0 ? kout : d == 1 ? t1k.get() : t2k.get(); };
-  auto vbuf = [&](int d) -> void* { return d == 0 ? vout : d == 1 ? t1v.get() : t2v.get(); };
+  auto kbuf = [&](int d) -> void* { return d == 0 ? kout : d == 1 ? t1k : t2k.get(); };
+  auto vbuf = [&](int d) -> void* { return d == 0 ? vout : d == 1 ? t1v : t2v.get(); };
 
   const int shape = chosen_shape(vbytes);
   const size_t tile_keys = size_t(kTileShapes[shape].nt) * kTileShapes[shape].items;
   const unsigned tiles = unsigned(ceil_div(n, tile_keys));
   FHIP_REQUIRE(ceil_div(n, tile_keys) < (1ull << 31), FHIP_ERR_TOO_LARGE, "radix_sort: too many keys");
   const size_t lb_words = size_t(tiles) * 256;
-  DBuf lbuf(2 * lb_words * sizeof(unsigned long long) + 256);
-  unsigned long long* lb0 = lbuf.as<unsigned long long>();
+  const size_t lb_bytes = 2 * lb_words * sizeof(unsigned long long) + 256;
+  DBuf lbuf;
+  unsigned long long* lb0;
+  if (ar) {
+    lb0 = static_cast<unsigned long long*>(ar->take(lb_bytes));
+  } else {
+    lbuf.reset(lb_bytes);
+    lb0 = lbuf.as<unsigned long long>();
+  }
   unsigned long long* lb1 = lb0 + lb_words;
   unsigned* ctr = reinterpret_cast<unsigned*>(lb1 + lb_words);
   FHIP_CHECK(hipMemsetAsync(lb0, 0, lb_words * sizeof(unsigned long long), st));
@@ -577,8 +605,8 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
     vs = vd;
   }
   if (copy_back) {
-    FHIP_CHECK(hipMemcpyAsync(kout, t1k.get(), kbn, hipMemcpyDeviceToDevice, st));
-    if (vbytes) FHIP_CHECK(hipMemcpyAsync(vout, t1v.get(), vbn, hipMemcpyDeviceToDevice, st));
+    FHIP_CHECK(hipMemcpyAsync(kout, t1k, kbn, hipMemcpyDeviceToDevice, st));
+    if (vbytes) FHIP_CHECK(hipMemcpyAsync(vout, t1v, vbn, hipMemcpyDeviceToDevice, st));
   }
   // temporaries go back to the pool; later work on this stream is ordered
 }
@@ -667,9 +695,19 @@ void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int v
   }
 }
 
+size_t sort_scratch_bytes(size_t n, int kbytes, int vbytes) {
+  const size_t tk = size_t(kTileShapes[kDefaultShape].nt) * kTileShapes[kDefaultShape].items;
+  const size_t tkv = size_t(kTileShapes[kDefaultShapeKV].nt) * kTileShapes[kDefaultShapeKV].items;
+  // the smaller tile of either shape bounds the look-back words
+  const size_t tiles = ceil_div(n, std::min(tk, tkv));
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  return up(16) + up(size_t(2) * 8 * 256 * 8) + up(n * size_t(kbytes)) + up(n * size_t(vbytes)) +
+         up(2 * tiles * 256 * 8 + 256);
+}
+
 void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* vin,
                 void* vout, size_t n, bool positive_only, bool descending, hipStream_t st,
-                int low_bytes) {
+                int low_bytes, Arena* ar) {
   const int kb = type_bytes(ktype);
   FHIP_REQUIRE(vbytes == 0 || vbytes == 4 || vbytes == 8, FHIP_ERR_ARG,
                "radix_sort: payload must be 0, 4 or 8 bytes");
@@ -696,8 +734,14 @@ void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* 
   const bool need_stats =
       type_is_float(ktype) ? !positive_only : (type_is_signed_int(ktype) && positive_only);
   if (need_stats) {
-    DBuf sbuf(2 * sizeof(unsigned long long));
-    unsigned long long* sdev = sbuf.as<unsigned long long>();
+    DBuf sbuf;
+    unsigned long long* sdev;
+    if (ar) {
+      sdev = static_cast<unsigned long long*>(ar->take(2 * sizeof(unsigned long long)));
+    } else {
+      sbuf.reset(2 * sizeof(unsigned long long));
+      sdev = sbuf.as<unsigned long long>();
+    }
     FHIP_CHECK(hipMemsetAsync(sdev, 0, 2 * sizeof(unsigned long long), st));
     switch (ktype) {
       case FHIP_I32: launch_stats<int32_t>(kin, n, sdev, st); break;
@@ -719,7 +763,7 @@ void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* 
   }
   const bool known = low_bytes >= 0 && !type_is_float(ktype) && !positive_only;
   run_lsd(kb, mode, descending ? 1 : 0, nbytes, kin, kout, vbytes, vin, vout, n, st,
-          known ? std::min(low_bytes, kb) : -1);
+          known ? std::min(low_bytes, kb) : -1, ar);
 }
 
 void sort_pairs_bits(int kbytes, const void* kin, void* kout, int vbytes, const void* vin,
@@ -732,9 +776,67 @@ void sort_pairs_bits(int kbytes, const void* kin, void* kout, int vbytes, const 
 }  // namespace fhip
 
 // ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point: a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in symbol): the
+// device address comes from hipGetSymbolAddress, once per device, with that
+// device current.
+// ---------------------------------------------------------------------------
+namespace fhip {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+}  // namespace fhip
+
+static __device__ __align__(256) unsigned char scratch[fhip::kNoAllocScratchBytes];
+
+namespace fhip {
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*>& scratch_ptrs() {
+  static auto* m = new std::map<int, void*>();  // leaked: outlives static dtors
+  return *m;
+}
+}  // namespace
+
+// Device address of the current device's scratch array.
+void* noalloc_scratch() {
+  int dev = 0;
+  FHIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = scratch_ptrs().find(dev);
+  if (it != scratch_ptrs().end()) return it->second;
+  void* ptr = nullptr;
+  FHIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  FHIP_REQUIRE(ptr, FHIP_ERR_HIP, "radix_sort_noalloc: no device address for the scratch array");
+  scratch_ptrs()[dev] = ptr;
+  return ptr;
+}
+}  // namespace fhip
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+int fhip_dev_radix_sort_noalloc(int ktype, void* keys, int vbytes, void* vals, size_t n,
+                                int positive_only, int descending, void* stream) {
+  return fhip::guarded([&] {
+    const int kb = fhip::type_bytes(ktype);
+    FHIP_REQUIRE(vbytes == 0 || vbytes == 4 || vbytes == 8, FHIP_ERR_ARG,
+                 "radix_sort: payload must be 0, 4 or 8 bytes");
+    const size_t need = fhip::sort_scratch_bytes(n, kb, vbytes);
+    FHIP_REQUIRE(need <= fhip::kNoAllocScratchBytes, FHIP_ERR_TOO_LARGE,
+                 "radix_sort_noalloc: " + std::to_string(n) + " keys need " +
+                     std::to_string(need) + " bytes of scratch; the limit is " +
+                     std::to_string(fhip::kNoAllocScratchBytes) +
+                     " bytes (use fhip_dev_radix_sort for larger inputs)");
+    hipStream_t st = fhip::as_stream(stream);
+    fhip::Arena ar;
+    ar.base = static_cast<char*>(fhip::noalloc_scratch());
+    ar.cap = fhip::kNoAllocScratchBytes;
+    fhip::sort_pairs(ktype, keys, keys, vbytes, vals, vals, n, positive_only != 0,
+                     descending != 0, st, -1, &ar);
+  });
+}
 
 int fhip_dev_radix_sort(int ktype, void* keys, int vbytes, void* vals, size_t n,
                         int positive_only, int descending, void* stream) {
@@ -750,15 +852,14 @@ int fhip_radix_sort(int ktype, void* keys, int vbytes, void* vals, size_t n,
     if (n == 0) return;
     const size_t kb = size_t(fhip::type_bytes(ktype)) * n;
     const size_t vb = size_t(vbytes) * n;
-    hipStream_t st = nullptr;
+    hipStream_t st = fhip::as_stream(nullptr);
     fhip::DBuf dk(kb), dv(vb);
-    FHIP_CHECK(hipMemcpyAsync(dk.get(), keys, kb, hipMemcpyHostToDevice, st));
-    if (vb) FHIP_CHECK(hipMemcpyAsync(dv.get(), vals, vb, hipMemcpyHostToDevice, st));
+    fhip::copy_h2d(dk.get(), keys, kb, st);
+    fhip::copy_h2d(dv.get(), vals, vb, st);
     fhip::sort_pairs(ktype, dk.get(), dk.get(), vbytes, dv.get(), dv.get(), n,
                      positive_only != 0, descending != 0, st);
-    FHIP_CHECK(hipMemcpyAsync(keys, dk.get(), kb, hipMemcpyDeviceToHost, st));
-    if (vb) FHIP_CHECK(hipMemcpyAsync(vals, dv.get(), vb, hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
+    fhip::copy_d2h(keys, dk.get(), kb, st);
+    fhip::copy_d2h(vals, dv.get(), vb, st);
   });
 }
 

@@ -29,9 +29,27 @@ struct SortPlan {
 // bytes above the lowest low_bytes (in the sort's order, the sign bit of
 // signed keys flipped), so those passes run without the host reading back
 // the histograms.
+// A caller-provided device region that temporaries are carved from instead
+// of the workspace pool (no allocation); take() throws when it runs out.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+  void* take(size_t bytes) {
+    const size_t at = (used + 255) & ~size_t(255);
+    FHIP_REQUIRE(at + bytes <= cap, FHIP_ERR_TOO_LARGE, "radix_sort: scratch region too small");
+    used = at + bytes;
+    return base + at;
+  }
+};
+
 void sort_pairs(int ktype, const void* kin, void* kout, int vbytes,
                 const void* vin, void* vout, size_t n, bool positive_only,
-                bool descending, hipStream_t stream, int low_bytes = -1);
+                bool descending, hipStream_t stream, int low_bytes = -1,
+                Arena* arena = nullptr);
+
+// Bytes of scratch sort_pairs needs with an Arena (in place, one temporary
+// key/payload buffer, the histograms and the look-back words).
+size_t sort_scratch_bytes(size_t n, int kbytes, int vbytes);
 
 // Plain stable ascending sort of unsigned 64-bit (or 32-bit) keys that only
 // uses the bytes in [0, nbytes); for internal callers that already know the

@@ -116,6 +116,17 @@ int fhip_radix_sort(int ktype, void* keys, int vbytes, void* vals, size_t n,
 int fhip_dev_radix_sort(int ktype, void* keys, int vbytes, void* vals,
                         size_t n, int positive_only, int descending,
                         void* stream);
+/* No counterpart in the reference: fhip_dev_radix_sort for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops).  Same arguments and results; it calls no hipMalloc /
+ * hipFree: its temporaries live in a file-scope 64 MiB device array, one
+ * per GPU, kept until the process ends.  Inputs whose temporaries do not
+ * fit (more than about 4M 8+8-byte pairs, 8M 8-byte keys) return
+ * FHIP_ERR_TOO_LARGE.  Every call on a GPU shares that array: calls that
+ * overlap on one GPU (two streams at once, two threads) race. */
+int fhip_dev_radix_sort_noalloc(int ktype, void* keys, int vbytes, void* vals,
+                                size_t n, int positive_only, int descending,
+                                void* stream);
 
 /* ---- prefix_sum ---------------------------------------------------------
  * Inclusive scan, prefix_sum(const T*, T*, size_t) (core/prefix_sum.hpp:
