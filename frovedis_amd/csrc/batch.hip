@@ -137,6 +137,628 @@ __global__ void seg_groups_kernel(const CT* __restrict__ comp, const size_t* __r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Work-queue path: four launches in one stream-ordered sequence (captured
+// in a HIP graph), so that bulk data only crosses kernel boundaries (within
+// a kernel, rows written on one XCD are not visible to the others' L2
+// without a whole-cache write-back):
+//   bq_hist  one workgroup per chunk of a large segment (more rows than
+//            one LDS job holds): histogram of its keys' fine bins (the top
+//            BQ_FINE_BITS bits below key_bits), written per chunk;
+//   bq_plan  one workgroup per large segment: sums its chunk histograms and
+//            cuts the fine bins into nb buckets of about `batch.bucket_rows`
+//            rows (consecutive bins, so the buckets are consecutive key
+//            ranges): a bin -> bucket map and the bucket starts;
+//   bq_tile  BQ_CAP rows of a large segment per workgroup, taken by ticket:
+//            stable scatter by bucket into the temporary arrays (per-tile
+//            wave64 ballot ranks, decoupled look-back over the segment's
+//            earlier tiles), the onesweep pass of the plain radix sort with
+//            the bucket as digit;
+//   bq_jobs  the persistent launch: workgroups take jobs from the queue by
+//            ticket, in queue order -- a small segment, or one bucket of a
+//            large one: its rows sorted in LDS by key (rows packed as
+//            (key - min) << 13 | row, LSD passes of at most
+//            `batch.digit_bits` bits, wave64 ballot ranks: stable), written
+//            out, and every group's values folded in row order by one thread
+//            (the reference's sequential fold, so the sums are bit-exact);
+//            each job's group count goes through a decoupled look-back over
+//            the jobs, which gives every job its output offset in (segment,
+//            key) order.  A job only waits on jobs with smaller tickets,
+//            which running workgroups hold, so every wait ends, and every
+//            workgroup leaves once the ticket passes the queue's end.
+// A job whose bucket exceeds BQ_CAP rows, or whose keys span 2^19 or more,
+// sets the overflow flag, and the host then runs the composite-key path.
+// ---------------------------------------------------------------------------
+constexpr int BQ_NT = 1024;                 // threads of the histogram, plan and tile kernels
+constexpr int BQ_WAVES = BQ_NT / 64;
+constexpr int BQ_TITEMS = 16;               // rows per thread of a scatter tile
+constexpr int BQ_TILE_ROWS = BQ_NT * BQ_TITEMS;
+// threads of a job: one 1024-thread workgroup per CU (its LDS leaves no
+// room for a second).  Two co-resident 512-thread workgroups of this kernel
+// gave wrong rows now and then on the MI355X; one workgroup per CU never did.
+constexpr int BJ_NT = 1024;
+constexpr int BJ_WAVES = BJ_NT / 64;
+constexpr int BQ_ITEMS = 4;                 // rows per thread of a job
+constexpr int BQ_CAP = BJ_NT * BQ_ITEMS;    // rows one LDS job holds
+constexpr int BQ_FINE_BITS = 12;
+constexpr int BQ_FINE = 1 << BQ_FINE_BITS;
+constexpr int BQ_IDX_BITS = 12;  // row within a job (BQ_CAP = 2^12)
+constexpr unsigned long long BQ_REL_LIMIT = 1ull << (32 - BQ_IDX_BITS);
+constexpr int BQ_EARLY_BITS = 15;  // key span of a job whose group count is published early
+static_assert((1 << BQ_EARLY_BITS) / 8 <= BQ_CAP * 8, "the early bitmap lives in the value buffer");
+static_assert(BQ_CAP == (1 << BQ_IDX_BITS), "job rows must fit the packed row field");
+
+enum : unsigned { BQ_HIST = 0, BQ_TILE = 1, BQ_JOB = 2 };
+constexpr unsigned BQ_SMALL = 0xFFFFFFFFu;
+
+// HIST: seg = large ordinal, idx = global chunk, rows [a, b)
+// TILE: seg = large ordinal, idx = tile within the segment, rows [a, b)
+// JOB:  seg = segment, idx = job number, aux = bucket (BQ_SMALL: a whole
+//       small segment, rows [a, b)); for a bucket a = large ordinal
+struct BqItem {
+  unsigned type, seg, idx, aux;
+  unsigned long long a, b;
+};
+
+struct BqSeg {
+  unsigned long long start, len;
+  unsigned chunk0, nchunks, tile0, ntiles, nb;
+  int fshift;
+};
+
+struct BqArgs {
+  const int64_t* keys;
+  const double* vals;
+  const BqItem* items;
+  unsigned njobs;
+  const BqSeg* lsegs;
+  unsigned* fine_hist;           // [chunks][BQ_FINE]
+  unsigned char* bmap;           // [large][BQ_FINE]: bin -> bucket
+  unsigned* bstart;              // [large][257]: bucket starts within the segment
+  unsigned long long* lookback;  // [tiles][256]
+  unsigned long long* job_state; // [jobs]
+  int64_t* tk;
+  double* tv;
+  int64_t* keys_out;
+  double* vals_out;
+  int64_t* gkeys;
+  double* gsum;
+  uint64_t* gcount;
+  size_t* gseg_off;
+  size_t capacity, nseg;
+  unsigned long long key_lim;    // keys must lie in [0, key_lim)
+  int digit_bits;
+  int early_bits;                // jobs spanning at most this many key bits publish early
+  unsigned* ticket;              // [0] tiles, [1] jobs
+  unsigned long long* scal;      // [0] key error, [1] groups, [2] overflow
+};
+
+// LDS (bytes) of the scatter tile
+constexpr int BT_OFF_BUF = 0;                                 // BQ_TILE_ROWS x 8
+constexpr int BT_OFF_WCNT = BT_OFF_BUF + BQ_TILE_ROWS * 8;    // BQ_WAVES x 256 x 4
+constexpr int BT_OFF_TSTART = BT_OFF_WCNT + BQ_WAVES * 256 * 4;
+constexpr int BT_OFF_GDST = BT_OFF_TSTART + 256 * 4;          // 256 x 8
+constexpr int BT_OFF_MAP = BT_OFF_GDST + 256 * 8;             // BQ_FINE x 1
+constexpr int BT_OFF_SCAN = BT_OFF_MAP + BQ_FINE;             // BQ_WAVES x 4
+constexpr int BT_LDS = BT_OFF_SCAN + 64;
+// LDS (bytes) of a job: two packed-row buffers, the sorted values, ranks
+constexpr int BJ_OFF_PK0 = 0;                                 // BQ_CAP x 4
+constexpr int BJ_OFF_PK1 = BJ_OFF_PK0 + BQ_CAP * 4;           // BQ_CAP x 4
+constexpr int BJ_OFF_VS = BJ_OFF_PK1 + BQ_CAP * 4;            // BQ_CAP x 8
+constexpr int BJ_OFF_WCNT = BJ_OFF_VS + BQ_CAP * 8;           // BJ_WAVES x 256 x 4
+constexpr int BJ_OFF_TSTART = BJ_OFF_WCNT + BJ_WAVES * 256 * 4;
+constexpr int BJ_OFF_SCAN = BJ_OFF_TSTART + 256 * 4;          // BQ_WAVES x 4
+constexpr int BJ_OFF_MISC = BJ_OFF_SCAN + 64;                 // 8 x 8
+constexpr int BJ_LDS = BJ_OFF_MISC + 64;
+static_assert(2 * BJ_LDS > 160 * 1024, "one job workgroup per CU");
+// LDS (bytes) of the histogram and of the plan
+constexpr int BH_LDS = BQ_FINE * 4;
+constexpr int BP_LDS = 256 * 4 + 64;
+
+// Stable ranks of the workgroup's IT x NT elements by a digit of dbits <= 8
+// bits; element (wave w, item j, lane l) is position w * 64 * IT + j * 64
+// + l.  Leaves the per-wave digit offsets in wcnt and returns the digit's
+// total for thread t < 256 (NT >= 256).
+template <int NT, int IT>
+__device__ __forceinline__ unsigned bq_rank(const unsigned (&dig)[IT], const bool (&valid)[IT],
+                                            int dbits, unsigned* wcnt, unsigned (&rank)[IT]) {
+  constexpr int WAVES = NT / 64;
+  static_assert(NT >= 256, "one thread per digit");
+  const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  for (unsigned i = tid; i < WAVES * 256; i += NT) wcnt[i] = 0;
+  __syncthreads();
+  const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    unsigned long long m = __ballot(valid[j]);
+    for (int b = 0; b < dbits; b++) {
+      const bool bit = (dig[j] >> b) & 1u;
+      const unsigned long long bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    rank[j] = 0;
+    if (valid[j]) {
+      const unsigned cnt = unsigned(__popcll(m));
+      const unsigned below = unsigned(__popcll(m & lt));
+      const unsigned prior = wcnt[wave * 256 + dig[j]];
+      rank[j] = prior + below;
+      if (below == cnt - 1) wcnt[wave * 256 + dig[j]] = prior + cnt;
+    }
+  }
+  __syncthreads();
+  unsigned total = 0;
+  if (tid < 256) {
+#pragma unroll
+    for (int w = 0; w < WAVES; w++) {
+      const unsigned c = wcnt[w * 256 + tid];
+      wcnt[w * 256 + tid] = total;
+      total += c;
+    }
+  }
+  return total;
+}
+
+__device__ void bq_hist(const BqArgs& A, const BqItem& it, unsigned char* lds) {
+  unsigned* h = reinterpret_cast<unsigned*>(lds);
+  const BqSeg S = A.lsegs[it.seg];
+  for (int i = threadIdx.x; i < BQ_FINE; i += BQ_NT) h[i] = 0;
+  __syncthreads();
+  int bad = 0;
+  constexpr int U = 8;  // rows per thread in flight
+  for (unsigned long long r0 = it.a + threadIdx.x; r0 < it.b; r0 += BQ_NT * U) {
+    int64_t k[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const unsigned long long r = r0 + (unsigned long long)u * BQ_NT;
+      k[u] = r < it.b ? A.keys[r] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (r0 + (unsigned long long)u * BQ_NT >= it.b) break;
+      if (k[u] < 0 || (unsigned long long)k[u] >= A.key_lim) bad = 1;
+      const unsigned long long f = (unsigned long long)k[u] >> S.fshift;
+      atomicAdd(&h[f < BQ_FINE ? unsigned(f) : unsigned(BQ_FINE - 1)], 1u);
+    }
+  }
+  if (__ballot(bad) && __lane_id() == 0) atomicOr(&A.scal[0], 1ull);
+  __syncthreads();
+  unsigned* out = A.fine_hist + size_t(it.idx) * BQ_FINE;
+  for (int i = threadIdx.x; i < BQ_FINE; i += BQ_NT) out[i] = h[i];
+}
+
+__device__ void bq_plan(const BqArgs& A, unsigned L, unsigned char* lds) {
+  const BqSeg S = A.lsegs[L];
+  unsigned* bcnt = reinterpret_cast<unsigned*>(lds);
+  unsigned* scan = reinterpret_cast<unsigned*>(lds + 256 * 4);
+  const unsigned tid = threadIdx.x;
+  constexpr int PER = BQ_FINE / BQ_NT;  // consecutive bins per thread
+  unsigned c[PER];
+  unsigned mine = 0;
+#pragma unroll
+  for (int e = 0; e < PER; e++) c[e] = 0;
+  for (unsigned ch = 0; ch < S.nchunks; ch++) {
+    const uint4 v = *reinterpret_cast<const uint4*>(A.fine_hist + size_t(S.chunk0 + ch) * BQ_FINE +
+                                                     tid * PER);
+    c[0] += v.x;
+    c[1] += v.y;
+    c[2] += v.z;
+    c[3] += v.w;
+  }
+#pragma unroll
+  for (int e = 0; e < PER; e++) mine += c[e];
+  for (unsigned i = tid; i < 256; i += BQ_NT) bcnt[i] = 0;
+  unsigned run = block_excl_scan<BQ_NT>(mine, scan, (unsigned*)nullptr);
+  const unsigned long long target = (S.len + S.nb - 1) / S.nb;
+  unsigned char* map = A.bmap + size_t(L) * BQ_FINE;
+#pragma unroll
+  for (int e = 0; e < PER; e++) {
+    unsigned long long b = run / target;
+    if (b >= S.nb) b = S.nb - 1;
+    map[tid * PER + e] = (unsigned char)b;
+    if (c[e]) atomicAdd(&bcnt[b], c[e]);
+    run += c[e];
+  }
+  __syncthreads();
+  const unsigned v = tid < S.nb ? bcnt[tid] : 0u;
+  const unsigned ex = block_excl_scan<BQ_NT>(v, scan, (unsigned*)nullptr);
+  if (tid <= 256) A.bstart[size_t(L) * 257 + tid] = ex;
+}
+static_assert(BQ_FINE / BQ_NT == 4, "bq_plan reads four bins per thread");
+
+__device__ __forceinline__ unsigned long long bq_lb_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void bq_lb_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void bq_tile(const BqArgs& A, const BqItem& it, unsigned char* lds) {
+  constexpr int IT = BQ_TITEMS;
+  constexpr unsigned long long FA = 1ull << 62, FP = 2ull << 62, VAL = (1ull << 62) - 1;
+  const BqSeg S = A.lsegs[it.seg];
+  unsigned char* map = lds + BT_OFF_MAP;
+  unsigned* wcnt = reinterpret_cast<unsigned*>(lds + BT_OFF_WCNT);
+  unsigned* tstart = reinterpret_cast<unsigned*>(lds + BT_OFF_TSTART);
+  unsigned long long* gdst = reinterpret_cast<unsigned long long*>(lds + BT_OFF_GDST);
+  unsigned* scan = reinterpret_cast<unsigned*>(lds + BT_OFF_SCAN);
+  const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  {
+    const unsigned* src = reinterpret_cast<const unsigned*>(A.bmap + size_t(it.seg) * BQ_FINE);
+    unsigned* dst = reinterpret_cast<unsigned*>(map);
+    for (int i = tid; i < BQ_FINE / 4; i += BQ_NT) dst[i] = src[i];
+  }
+  const unsigned tile_n = unsigned(it.b - it.a);
+  const unsigned wave_base = wave * 64 * IT;
+  int64_t key[IT];
+  double val[IT];
+  unsigned dig[IT], rank[IT];
+  bool valid[IT];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned p = wave_base + unsigned(j) * 64 + lane;
+    valid[j] = p < tile_n;
+    key[j] = valid[j] ? A.keys[it.a + p] : 0;
+    val[j] = valid[j] ? A.vals[it.a + p] : 0.0;
+  }
+  __syncthreads();  // the map is in LDS
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned long long f = (unsigned long long)key[j] >> S.fshift;
+    dig[j] = map[f < BQ_FINE ? unsigned(f) : unsigned(BQ_FINE - 1)];
+  }
+  const unsigned total = bq_rank<BQ_NT, IT>(dig, valid, 8, wcnt, rank);
+  const unsigned ex = block_excl_scan<BQ_NT>(tid < 256 ? total : 0u, scan, (unsigned*)nullptr);
+  if (tid < 256) {
+    tstart[tid] = ex;
+    unsigned long long* lbs = A.lookback + size_t(S.tile0) * 256;
+    unsigned long long* mine = lbs + size_t(it.idx) * 256 + tid;
+    bq_lb_store(mine, (it.idx == 0 ? FP : FA) | total);
+    unsigned long long excl = 0;
+    if (it.idx > 0) {
+      long long p = (long long)it.idx - 1;
+      for (;;) {
+        unsigned long long v[8];
+#pragma unroll
+        for (int w = 0; w < 8; w++) v[w] = (p - w >= 0) ? bq_lb_load(lbs + size_t(p - w) * 256 + tid) : 0ull;
+        int used = 0;
+        bool done = false;
+#pragma unroll
+        for (int w = 0; w < 8; w++) {
+          const unsigned long long flag = v[w] & ~VAL;
+          if (flag == 0) break;
+          excl += v[w] & VAL;
+          used = w + 1;
+          if (flag == FP) {
+            done = true;
+            break;
+          }
+        }
+        if (done) break;
+        if (used == 0) __builtin_amdgcn_s_sleep(1);
+        p -= used;
+      }
+      bq_lb_store(mine, FP | (excl + total));
+    }
+    // bucket tid's rows start at bstart[tid] within the segment (read only
+    // for buckets that hold rows of this tile)
+    const unsigned long long bs = total ? A.bstart[size_t(it.seg) * 257 + tid] : 0ull;
+    gdst[tid] = S.start + bs + excl - ex;
+  }
+  __syncthreads();
+  int64_t* kbuf = reinterpret_cast<int64_t*>(lds + BT_OFF_BUF);
+  unsigned pos[IT];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    pos[j] = tstart[dig[j]] + wcnt[wave * 256 + dig[j]] + rank[j];
+    if (valid[j]) kbuf[pos[j]] = key[j];
+  }
+  __syncthreads();
+  unsigned long long dst[IT];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned i = unsigned(j) * BQ_NT + tid;
+    dst[j] = 0;
+    if (i < tile_n) {
+      const int64_t k = kbuf[i];
+      const unsigned long long f = (unsigned long long)k >> S.fshift;
+      const unsigned d = map[f < BQ_FINE ? unsigned(f) : unsigned(BQ_FINE - 1)];
+      dst[j] = gdst[d] + i;
+      A.tk[dst[j]] = k;
+    }
+  }
+  __syncthreads();
+  double* vbuf = reinterpret_cast<double*>(lds + BT_OFF_BUF);
+#pragma unroll
+  for (int j = 0; j < IT; j++)
+    if (valid[j]) vbuf[pos[j]] = val[j];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned i = unsigned(j) * BQ_NT + tid;
+    if (i < tile_n) A.tv[dst[j]] = vbuf[i];
+  }
+}
+
+// The second half of compact_lookback for a job that has already published
+// its count: sums earlier jobs' counts (64 at a time) up to the first
+// inclusive prefix, then publishes its own; all 64 lanes of one wave.
+__device__ __forceinline__ unsigned long long bq_job_resolve(unsigned long long* state,
+                                                             unsigned idx, unsigned long long run) {
+  const unsigned lane = __lane_id();
+  if (idx == 0) return 0;
+  unsigned long long excl = 0;
+  long long p = (long long)idx - 1;  // lane l looks at job p - l
+  while (true) {
+    const unsigned long long v = (p - (long long)lane >= 0) ? bq_lb_load(state + (p - lane)) : CP_FLAG_P;
+    const unsigned long long flag = v & ~CP_VALUE;
+    const unsigned long long not_ready = __ballot(flag == 0);
+    const unsigned long long prefix = __ballot(flag == CP_FLAG_P);
+    const int first_gap = not_ready ? __ffsll((long long)not_ready) - 1 : 64;
+    const int first_p = prefix ? __ffsll((long long)prefix) - 1 : 64;
+    const bool done = first_p < first_gap;
+    const int take = done ? first_p + 1 : first_gap;
+    unsigned long long part = (int(lane) < take) ? (v & CP_VALUE) : 0ull;
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    excl += part;
+    if (done) break;
+    if (take == 0) __builtin_amdgcn_s_sleep(1);
+    p -= take;
+  }
+  if (lane == 0) bq_lb_store(state + idx, CP_FLAG_P | (excl + run));
+  return excl;
+}
+
+__device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
+  constexpr int IT = BQ_ITEMS;
+  const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+  // the packed-row buffer c (0 or 1): an offset from the LDS base, so the
+  // accesses stay LDS instructions (a pointer picked from an array of
+  // pointers would become a generic one, and FLAT accesses to LDS are not
+  // ordered by the barriers' LDS waits)
+  auto pk = [lds](int c) {
+    return reinterpret_cast<unsigned*>(lds + BJ_OFF_PK0 + (c ? BQ_CAP * 4 : 0));
+  };
+  double* vs = reinterpret_cast<double*>(lds + BJ_OFF_VS);
+  unsigned* wcnt = reinterpret_cast<unsigned*>(lds + BJ_OFF_WCNT);
+  unsigned* tstart = reinterpret_cast<unsigned*>(lds + BJ_OFF_TSTART);
+  unsigned* scan = reinterpret_cast<unsigned*>(lds + BJ_OFF_SCAN);
+  unsigned long long* misc = reinterpret_cast<unsigned long long*>(lds + BJ_OFF_MISC);
+  const bool small = it.aux == BQ_SMALL;
+  unsigned long long a, m;
+  const int64_t* ksrc;
+  const double* vsrc;
+  if (small) {
+    a = it.a;
+    m = it.b - it.a;
+    ksrc = A.keys + a;
+    vsrc = A.vals + a;
+  } else {
+    const BqSeg S = A.lsegs[it.a];
+    const unsigned* bs = A.bstart + size_t(it.a) * 257;
+    a = S.start + bs[it.aux];
+    m = bs[it.aux + 1] - bs[it.aux];
+    ksrc = A.tk + a;
+    vsrc = A.tv + a;
+  }
+  if (tid == 0) {
+    misc[0] = ~0ull;  // min
+    misc[1] = 0;      // max
+  }
+  __syncthreads();
+  const bool fits = m <= BQ_CAP;
+  const unsigned wave_base = wave * 64 * IT;
+  unsigned long long kv[IT];
+  bool valid[IT];
+  int bad = 0;
+  unsigned long long lo = ~0ull, hi = 0;
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned p = wave_base + unsigned(j) * 64 + lane;
+    valid[j] = fits && p < m;
+    const int64_t k = valid[j] ? ksrc[p] : 0;
+    if (valid[j] && (k < 0 || (unsigned long long)k >= A.key_lim)) bad = 1;
+    kv[j] = (unsigned long long)k;
+    if (valid[j]) {
+      lo = kv[j] < lo ? kv[j] : lo;
+      hi = kv[j] > hi ? kv[j] : hi;
+    }
+  }
+  if (small && __ballot(bad) && lane == 0) atomicOr(&A.scal[0], 1ull);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long l2 = __shfl_xor(lo, o, 64), h2 = __shfl_xor(hi, o, 64);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+  }
+  if (lane == 0) {
+    atomicMin(&misc[0], lo);
+    atomicMax(&misc[1], hi);
+  }
+  __syncthreads();
+  const unsigned long long kmin = misc[0];
+  const unsigned long long range = m ? misc[1] - kmin : 0;
+  const bool ok = fits && range < BQ_REL_LIMIT;
+  if (!ok && tid == 0) atomicOr(&A.scal[2], 1ull);
+  // sort the packed rows by their key bits (stable LSD passes in LDS)
+  const int kbits = range ? 64 - __builtin_clzll(range) : 0;
+  // the job's group count up front when its keys span at most
+  // BQ_EARLY_BITS bits (a bitmap of the keys present, in the value buffer):
+  // publishing it now lets later jobs' look-backs pass this job at once
+  const bool early = ok && kbits <= A.early_bits;
+  if (early) {
+    unsigned* bits = reinterpret_cast<unsigned*>(vs);
+    for (unsigned w = tid; w < (1u << BQ_EARLY_BITS) / 32; w += BJ_NT) bits[w] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < IT; j++)
+      if (valid[j]) {
+        const unsigned r = unsigned(kv[j] - kmin);
+        atomicOr(&bits[r >> 5], 1u << (r & 31));
+      }
+    __syncthreads();
+    unsigned c = 0;
+    for (unsigned w = tid; w < (1u << BQ_EARLY_BITS) / 32; w += BJ_NT) c += unsigned(__popc(bits[w]));
+    unsigned ge = 0;
+    (void)block_excl_scan<BJ_NT>(c, scan, &ge);
+    if (tid == 0)
+      bq_lb_store(A.job_state + it.idx, (it.idx == 0 ? CP_FLAG_P : CP_FLAG_A) | (unsigned long long)ge);
+  }
+  const int npass = ok ? (kbits + A.digit_bits - 1) / A.digit_bits : 0;
+  const int dbits = npass ? (kbits + npass - 1) / npass : 0;
+  unsigned pv[IT];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned p = wave_base + unsigned(j) * 64 + lane;
+    pv[j] = valid[j] && ok ? unsigned(((kv[j] - kmin) << BQ_IDX_BITS) | p) : 0u;
+    if (!ok) valid[j] = false;
+  }
+  int cur = 0;
+  for (int q = 0; q < npass; q++) {
+    const int sh = BQ_IDX_BITS + q * dbits;
+    const unsigned dm = (1u << dbits) - 1;
+    unsigned dig[IT], rank[IT];
+#pragma unroll
+    for (int j = 0; j < IT; j++) dig[j] = (pv[j] >> sh) & dm;
+    const unsigned total = bq_rank<BJ_NT, IT>(dig, valid, dbits, wcnt, rank);
+    const unsigned ex = block_excl_scan<BJ_NT>(tid < 256 ? total : 0u, scan, (unsigned*)nullptr);
+    if (tid < 256) tstart[tid] = ex;
+    __syncthreads();
+    unsigned* out = pk(cur ^ 1);
+#pragma unroll
+    for (int j = 0; j < IT; j++)
+      if (valid[j]) out[tstart[dig[j]] + wcnt[wave * 256 + dig[j]] + rank[j]] = pv[j];
+    __syncthreads();
+    cur ^= 1;
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const unsigned p = wave_base + unsigned(j) * 64 + lane;
+      if (valid[j]) pv[j] = out[p];
+    }
+  }
+  unsigned* srt = pk(cur);
+  if (npass == 0) {
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const unsigned p = wave_base + unsigned(j) * 64 + lane;
+      if (valid[j]) srt[p] = pv[j];
+    }
+  }
+  __syncthreads();
+  // the sorted rows go out first (they need no group offset); their values
+  // are kept in LDS for the folds
+  const unsigned mm = ok ? unsigned(m) : 0u;
+  for (unsigned i = tid; i < mm; i += BJ_NT) {
+    const unsigned v = srt[i];
+    const double x = vsrc[v & (BQ_CAP - 1)];
+    A.keys_out[a + i] = int64_t(kmin + (v >> BQ_IDX_BITS));
+    A.vals_out[a + i] = x;
+    vs[i] = x;
+  }
+  // group starts: a row whose key differs from the row before it
+  unsigned long long fl[IT];
+  unsigned wtot = 0;
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned p = wave_base + unsigned(j) * 64 + lane;
+    const bool st = valid[j] && (p == 0 || (srt[p] >> BQ_IDX_BITS) != (srt[p - 1] >> BQ_IDX_BITS));
+    fl[j] = __ballot(st);
+    wtot += unsigned(__popcll(fl[j]));
+  }
+  unsigned* wsum = tstart;  // per-wave group counts
+  if (lane == 0) wsum[wave] = wtot;
+  __syncthreads();
+  unsigned before = 0, G = 0;
+#pragma unroll
+  for (int w = 0; w < BJ_WAVES; w++) {
+    const unsigned c = wsum[w];
+    if (unsigned(w) < wave) before += c;
+    G += c;
+  }
+  // group offset: decoupled look-back over the jobs (wave 0)
+  if (wave == 0) {
+    const unsigned long long off = early ? bq_job_resolve(A.job_state, it.idx, G)
+                                         : compact_lookback(A.job_state, it.idx, G);
+    if (lane == 0) misc[2] = off;
+  }
+  unsigned* starts = pk(cur ^ 1);  // G entries (the end of the last group is m)
+  {
+    unsigned run = before;
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const unsigned p = wave_base + unsigned(j) * 64 + lane;
+      if ((fl[j] >> lane) & 1ull) starts[run + unsigned(__popcll(fl[j] & lt))] = p;
+      run += unsigned(__popcll(fl[j]));
+    }
+  }
+  __syncthreads();
+  const unsigned long long goff = misc[2];
+  if (tid == 0) {
+    if (small || it.aux == 0) A.gseg_off[it.seg] = goff;
+    if (it.idx == A.njobs - 1) {
+      A.gseg_off[A.nseg] = goff + G;
+      A.scal[1] = goff + G;
+    }
+  }
+  for (unsigned g = tid; g < G; g += BJ_NT) {
+    const unsigned s = starts[g], e = g + 1 < G ? starts[g + 1] : mm;
+    double total = 0;
+    for (unsigned i = s; i < e; i++) total += vs[i];
+    const unsigned long long o = goff + g;
+    if (o < A.capacity) {
+      A.gkeys[o] = int64_t(kmin + (srt[s] >> BQ_IDX_BITS));
+      A.gsum[o] = 0.0 + total;
+      A.gcount[o] = e - s;
+    }
+  }
+}
+
+// A.items holds [chunks | tiles | jobs]; the kernels get their section's
+// first item
+__global__ __launch_bounds__(BQ_NT) void bq_hist_kernel(BqArgs A, unsigned base) {
+  extern __shared__ unsigned long long bq_lds_raw[];
+  bq_hist(A, A.items[base + blockIdx.x], reinterpret_cast<unsigned char*>(bq_lds_raw));
+}
+
+__global__ __launch_bounds__(BQ_NT) void bq_plan_kernel(BqArgs A) {
+  extern __shared__ unsigned long long bq_lds_raw[];
+  bq_plan(A, blockIdx.x, reinterpret_cast<unsigned char*>(bq_lds_raw));
+}
+
+// tiles in arrival order (a ticket), so the look-back never waits on a tile
+// that has not started
+__global__ __launch_bounds__(BQ_NT) void bq_tile_kernel(BqArgs A, unsigned base) {
+  extern __shared__ unsigned long long bq_lds_raw[];
+  __shared__ unsigned item_s;
+  if (threadIdx.x == 0) item_s = atomicAdd(A.ticket, 1u);
+  __syncthreads();
+  bq_tile(A, A.items[base + item_s], reinterpret_cast<unsigned char*>(bq_lds_raw));
+}
+
+// The persistent launch: each workgroup takes jobs by ticket until the
+// queue ends (the next ticket is fetched while the current job runs).
+__global__ __launch_bounds__(BJ_NT) void bq_jobs_kernel(BqArgs A, unsigned base, unsigned njobs) {
+  extern __shared__ unsigned long long bq_lds_raw[];
+  unsigned char* lds = reinterpret_cast<unsigned char*>(bq_lds_raw);
+  __shared__ unsigned item_s;
+  if (threadIdx.x == 0) item_s = atomicAdd(A.ticket + 1, 1u);
+  for (;;) {
+    __syncthreads();
+    const unsigned i = item_s;
+    __syncthreads();
+    if (i >= njobs) break;
+    unsigned next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(A.ticket + 1, 1u);
+    bq_job(A, A.items[base + i], lds);
+    if (threadIdx.x == 0) item_s = next;
+  }
+}
+
+__global__ void bq_zero_kernel(unsigned long long* __restrict__ p, size_t n) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0;
+}
+
 struct Plan {
   // shape and pointers the graph was captured for
   const void *keys, *vals, *seg_off;
@@ -161,7 +783,8 @@ struct Plan {
   }
 };
 
-std::unique_ptr<Plan> g_plan;
+std::unique_ptr<Plan> g_plan;  // the composite-key path
+
 hipStream_t g_capture_stream = nullptr;
 
 template <class CT>
@@ -213,7 +836,7 @@ void enqueue(Plan& p, hipStream_t st) {
 
 }  // namespace
 
-void batch_sort_groupby(const void* keys, const void* vals, size_t n, const size_t* seg_off,
+void composite_sort_groupby(const void* keys, const void* vals, size_t n, const size_t* seg_off,
                         size_t nseg, int key_bits, void* keys_out, void* vals_out, void* gkeys,
                         void* gsum, void* gcount, size_t* gseg_off, size_t capacity,
                         size_t* ngroups, int use_graph, hipStream_t st) {
@@ -275,7 +898,241 @@ void batch_sort_groupby(const void* keys, const void* vals, size_t n, const size
   FHIP_REQUIRE(*ngroups <= capacity, FHIP_ERR_TOO_LARGE, "batch: group capacity too small");
 }
 
-void batch_release() { g_plan.reset(); }
+
+namespace {
+
+// Queue and buffers of the work-queue path for one shape (and one content
+// of seg_off, which the queue is built from); the HIP graph replays it.
+struct QPlan {
+  const void *keys = nullptr, *vals = nullptr, *seg_off = nullptr;
+  size_t n = 0, nseg = 0;
+  int key_bits = 0;
+  void *keys_out = nullptr, *vals_out = nullptr, *gkeys = nullptr, *gsum = nullptr,
+       *gcount = nullptr, *gseg_off = nullptr;
+  size_t capacity = 0;
+  std::vector<size_t> off;  // seg_off contents
+  long long tunables[4] = {0, 0, 0, 0};
+  unsigned nitems = 0, njobs = 0, nlarge = 0, nchunks = 0, ntiles = 0;
+  size_t lb_words = 0;
+  DBuf items, lsegs, fine_hist, bmap, bstart, lookback, job_state, tk, tv, scal;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  ~QPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  bool same(const QPlan& o) const {
+    return keys == o.keys && vals == o.vals && seg_off == o.seg_off && n == o.n &&
+           nseg == o.nseg && key_bits == o.key_bits && keys_out == o.keys_out &&
+           vals_out == o.vals_out && gkeys == o.gkeys && gsum == o.gsum && gcount == o.gcount &&
+           gseg_off == o.gseg_off && capacity == o.capacity && off == o.off &&
+           tunables[0] == o.tunables[0] && tunables[1] == o.tunables[1] &&
+           tunables[2] == o.tunables[2] && tunables[3] == o.tunables[3];
+  }
+};
+
+std::unique_ptr<QPlan> g_qplan;
+
+// Rows per bucket job of a large segment of m rows: the tunable of its size
+// class (log2 of m, rounded up) if set, else "batch.bucket_rows".
+unsigned long long bucket_rows_for(size_t m, long long dflt) {
+  int c = 0;
+  while ((size_t(1) << c) < m) c++;
+  const std::string name = "batch.bucket_rows.c" + std::to_string(c);
+  const long long v = tune_get(name.c_str(), dflt);
+  return (unsigned long long)std::max(256LL, std::min(v, (long long)BQ_CAP));
+}
+
+void build_queue(QPlan& q, hipStream_t st) {
+  const long long bucket_rows = tune_get("batch.bucket_rows", 3072);
+  const long long chunk = std::max(1024LL, tune_get("batch.hist_chunk", 1 << 16));
+  const int fshift = std::max(0, q.key_bits - BQ_FINE_BITS);
+  std::vector<BqSeg> ls;
+  std::vector<BqItem> hist, tiles, jobs;
+  unsigned chunk0 = 0, tile0 = 0, job = 0;
+  for (size_t s = 0; s < q.nseg; s++) {
+    const unsigned long long a = q.off[s], b = q.off[s + 1], m = b - a;
+    if (m <= (unsigned long long)BQ_CAP) {
+      jobs.push_back(BqItem{BQ_JOB, unsigned(s), job++, BQ_SMALL, a, b});
+      continue;
+    }
+    const unsigned L = unsigned(ls.size());
+    BqSeg g{};
+    g.start = a;
+    g.len = m;
+    g.chunk0 = chunk0;
+    g.nchunks = unsigned(ceil_div(m, size_t(chunk)));
+    g.tile0 = tile0;
+    g.ntiles = unsigned(ceil_div(m, size_t(BQ_TILE_ROWS)));
+    g.nb = unsigned(std::min<unsigned long long>(256, ceil_div(m, bucket_rows_for(m, bucket_rows))));
+    g.fshift = fshift;
+    for (unsigned c = 0; c < g.nchunks; c++)
+      hist.push_back(BqItem{BQ_HIST, L, chunk0 + c, 0, a + c * chunk,
+                            std::min<unsigned long long>(b, a + (c + 1) * chunk)});
+    for (unsigned t = 0; t < g.ntiles; t++)
+      tiles.push_back(BqItem{BQ_TILE, L, t, 0, a + size_t(t) * BQ_TILE_ROWS,
+                             std::min<unsigned long long>(b, a + size_t(t + 1) * BQ_TILE_ROWS)});
+    for (unsigned k = 0; k < g.nb; k++) jobs.push_back(BqItem{BQ_JOB, unsigned(s), job++, k, L, 0});
+    chunk0 += g.nchunks;
+    tile0 += g.ntiles;
+    ls.push_back(g);
+  }
+  std::vector<BqItem> all;
+  all.reserve(hist.size() + tiles.size() + jobs.size());
+  for (auto* v : {&hist, &tiles, &jobs}) all.insert(all.end(), v->begin(), v->end());
+  q.nitems = unsigned(all.size());
+  q.nchunks = chunk0;
+  q.ntiles = tile0;
+  q.njobs = job;
+  q.nlarge = unsigned(ls.size());
+  q.lb_words = size_t(tile0) * 256;
+  q.items.reset(std::max<size_t>(1, all.size()) * sizeof(BqItem));
+  q.lsegs.reset(std::max<size_t>(1, ls.size()) * sizeof(BqSeg));
+  q.fine_hist.reset(std::max<size_t>(1, chunk0) * BQ_FINE * 4);
+  q.bmap.reset(std::max<size_t>(1, ls.size()) * BQ_FINE);
+  q.bstart.reset(std::max<size_t>(1, ls.size()) * 257 * 4);
+  q.lookback.reset(std::max<size_t>(1, q.lb_words) * 8);
+  q.job_state.reset(std::max<size_t>(1, q.njobs) * 8);
+  q.tk.reset(std::max<size_t>(1, q.n) * 8);
+  q.tv.reset(std::max<size_t>(1, q.n) * 8);
+  q.scal.reset(64);
+  // stream-ordered after the earlier users of these pool blocks
+  FHIP_CHECK(hipMemcpyAsync(q.items.get(), all.data(), all.size() * sizeof(BqItem),
+                            hipMemcpyHostToDevice, st));
+  if (!ls.empty())
+    FHIP_CHECK(hipMemcpyAsync(q.lsegs.get(), ls.data(), ls.size() * sizeof(BqSeg),
+                              hipMemcpyHostToDevice, st));
+  FHIP_CHECK(hipStreamSynchronize(st));  // the host vectors go out of scope
+}
+
+void enqueue_queue(QPlan& q, hipStream_t st) {
+  unsigned long long* scal = q.scal.as<unsigned long long>();  // [0..2] scalars, [4] ticket
+  bq_zero_kernel<<<1, 64, 0, st>>>(scal, 8);
+  bq_zero_kernel<<<grid_for_n(q.lb_words), 256, 0, st>>>(q.lookback.as<unsigned long long>(), q.lb_words);
+  bq_zero_kernel<<<grid_for_n(size_t(q.njobs)), 256, 0, st>>>(q.job_state.as<unsigned long long>(), q.njobs);
+  FHIP_CHECK(hipGetLastError());
+  BqArgs A{};
+  A.keys = static_cast<const int64_t*>(q.keys);
+  A.vals = static_cast<const double*>(q.vals);
+  A.items = q.items.as<BqItem>();
+  A.njobs = q.njobs;
+  A.lsegs = q.lsegs.as<BqSeg>();
+  A.fine_hist = q.fine_hist.as<unsigned>();
+  A.bmap = q.bmap.as<unsigned char>();
+  A.bstart = q.bstart.as<unsigned>();
+  A.lookback = q.lookback.as<unsigned long long>();
+  A.job_state = q.job_state.as<unsigned long long>();
+  A.tk = q.tk.as<int64_t>();
+  A.tv = q.tv.as<double>();
+  A.keys_out = static_cast<int64_t*>(q.keys_out);
+  A.vals_out = static_cast<double*>(q.vals_out);
+  A.gkeys = static_cast<int64_t*>(q.gkeys);
+  A.gsum = static_cast<double*>(q.gsum);
+  A.gcount = static_cast<uint64_t*>(q.gcount);
+  A.gseg_off = static_cast<size_t*>(q.gseg_off);
+  A.capacity = q.capacity;
+  A.nseg = q.nseg;
+  A.key_lim = q.key_bits >= 64 ? ~0ull : (1ull << q.key_bits);
+  A.digit_bits = int(q.tunables[0]);
+  A.early_bits = int(q.tunables[3]);
+  A.ticket = reinterpret_cast<unsigned*>(scal + 4);
+  A.scal = scal;
+  {
+    ProfScope prof("batch_tiles", st);
+    if (q.nchunks) bq_hist_kernel<<<q.nchunks, BQ_NT, BH_LDS, st>>>(A, 0);
+    if (q.nlarge) bq_plan_kernel<<<q.nlarge, BQ_NT, BP_LDS, st>>>(A);
+    if (q.ntiles) bq_tile_kernel<<<q.ntiles, BQ_NT, BT_LDS, st>>>(A, q.nchunks);
+    FHIP_CHECK(hipGetLastError());
+  }
+  ProfScope prof("batch_jobs", st);
+  const unsigned grid = std::max(1u, std::min(q.njobs, unsigned(fill_blocks(BJ_NT, 1))));
+  bq_jobs_kernel<<<grid, BJ_NT, BJ_LDS, st>>>(A, q.nchunks + q.ntiles, q.njobs);
+  FHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+// The work-queue path; returns false when a bucket overflowed its job (the
+// caller then runs the composite path).  Errors throw.
+bool queue_sort_groupby(const void* keys, const void* vals, size_t n, const size_t* seg_off,
+                        size_t nseg, int key_bits, void* keys_out, void* vals_out, void* gkeys,
+                        void* gsum, void* gcount, size_t* gseg_off, size_t capacity,
+                        size_t* ngroups, int use_graph, hipStream_t st) {
+  static bool attr = false;
+  if (!attr) {
+    FHIP_CHECK(hipFuncSetAttribute((const void*)bq_tile_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, BT_LDS));
+    FHIP_CHECK(hipFuncSetAttribute((const void*)bq_jobs_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, BJ_LDS));
+    attr = true;
+  }
+  auto q = std::make_unique<QPlan>();
+  q->keys = keys; q->vals = vals; q->seg_off = seg_off; q->n = n; q->nseg = nseg;
+  q->key_bits = key_bits; q->keys_out = keys_out; q->vals_out = vals_out; q->gkeys = gkeys;
+  q->gsum = gsum; q->gcount = gcount; q->gseg_off = gseg_off; q->capacity = capacity;
+  q->off.resize(nseg + 1);
+  FHIP_CHECK(hipMemcpyAsync(q->off.data(), seg_off, (nseg + 1) * sizeof(size_t),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  FHIP_REQUIRE(q->off[0] == 0 && q->off[nseg] == n, FHIP_ERR_ARG,
+               "batch: seg_off must run from 0 to n");
+  for (size_t s = 0; s < nseg; s++)
+    FHIP_REQUIRE(q->off[s] <= q->off[s + 1], FHIP_ERR_ARG, "batch: seg_off must not decrease");
+  q->tunables[0] = std::max(2LL, std::min(8LL, tune_get("batch.digit_bits", 8)));
+  q->tunables[1] = tune_get("batch.bucket_rows", 3072);
+  q->tunables[2] = tune_get("batch.hist_chunk", 1 << 16);
+  q->tunables[3] = std::max(-1LL, std::min((long long)BQ_EARLY_BITS, tune_get("batch.early_bits", BQ_EARLY_BITS)));
+  if (!(g_qplan && g_qplan->same(*q))) {
+    build_queue(*q, st);
+    g_qplan = std::move(q);
+  }
+  QPlan* plan = g_qplan.get();
+  if (use_graph && !plan->exec) {  // captured once per plan, replayed after
+    if (!g_capture_stream) FHIP_CHECK(hipStreamCreateWithFlags(&g_capture_stream, hipStreamNonBlocking));
+    FHIP_CHECK(hipStreamBeginCapture(g_capture_stream, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_queue(*plan, g_capture_stream);
+    } catch (...) {
+      hipGraph_t dropped = nullptr;
+      (void)hipStreamEndCapture(g_capture_stream, &dropped);
+      if (dropped) (void)hipGraphDestroy(dropped);
+      throw;
+    }
+    FHIP_CHECK(hipStreamEndCapture(g_capture_stream, &plan->graph));
+    FHIP_CHECK(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+  }
+  if (use_graph) FHIP_CHECK(hipGraphLaunch(plan->exec, st));
+  else enqueue_queue(*plan, st);
+  unsigned long long sc[3] = {0, 0, 0};
+  FHIP_CHECK(hipMemcpyAsync(sc, plan->scal.get(), sizeof(sc), hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  FHIP_REQUIRE(sc[0] == 0, FHIP_ERR_ARG, "batch: a key is outside [0, 2^key_bits)");
+  if (sc[2]) return false;
+  *ngroups = size_t(sc[1]);
+  FHIP_REQUIRE(*ngroups <= capacity, FHIP_ERR_TOO_LARGE, "batch: group capacity too small");
+  return true;
+}
+
+void batch_sort_groupby(const void* keys, const void* vals, size_t n, const size_t* seg_off,
+                        size_t nseg, int key_bits, void* keys_out, void* vals_out, void* gkeys,
+                        void* gsum, void* gcount, size_t* gseg_off, size_t capacity,
+                        size_t* ngroups, int use_graph, hipStream_t st) {
+  FHIP_REQUIRE(key_bits >= 1 && key_bits <= 48, FHIP_ERR_ARG, "batch: key_bits must be 1..48");
+  FHIP_REQUIRE(nseg >= 1 && nseg < (size_t(1) << 20), FHIP_ERR_ARG, "batch: 1 to 2^20 segments");
+  *ngroups = 0;
+  // "batch.path": 0 (default) the work queue, 1 the composite-key sort
+  if (n > 0 && tune_get("batch.path", 0) == 0 &&
+      queue_sort_groupby(keys, vals, n, seg_off, nseg, key_bits, keys_out, vals_out, gkeys, gsum,
+                         gcount, gseg_off, capacity, ngroups, use_graph, st))
+    return;
+  composite_sort_groupby(keys, vals, n, seg_off, nseg, key_bits, keys_out, vals_out, gkeys, gsum,
+                         gcount, gseg_off, capacity, ngroups, use_graph, st);
+}
+
+void batch_release() {
+  g_plan.reset();
+  g_qplan.reset();
+}
 
 }  // namespace fhip
 

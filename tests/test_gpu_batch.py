@@ -96,3 +96,62 @@ def test_batch_rejects_out_of_range_keys(fa):
     with pytest.raises(fa.FhipError):
         fa.batch_sort_groupby(fa.DeviceArray.from_numpy(keys), fa.DeviceArray.from_numpy(vals),
                               fa.DeviceArray.from_numpy(off), key_bits=18, use_graph=False)
+
+
+def _run_and_check(fa, keys, vals, off, key_bits, use_graph=False):
+    res = fa.batch_sort_groupby(fa.DeviceArray.from_numpy(keys), fa.DeviceArray.from_numpy(vals),
+                                fa.DeviceArray.from_numpy(off), key_bits=key_bits,
+                                use_graph=use_graph)
+    _check_against_oracle(res, keys, vals, off)
+
+
+def test_batch_overflowing_bucket_falls_back(fa):
+    # a large segment whose keys fill one fine bin (every key equal) cannot be
+    # cut into buckets that fit one LDS job: the composite path takes over
+    rng = np.random.default_rng(3)
+    sizes = np.array([20000, 300, 50000], np.int64)
+    keys = np.concatenate([np.full(20000, 7), rng.integers(0, 75, 300),
+                           rng.integers(0, 12500, 50000)]).astype(np.int64)
+    vals = rng.random(len(keys))
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+    _run_and_check(fa, keys, vals, off, 18)
+
+
+def test_batch_wide_key_range_in_small_segment_falls_back(fa):
+    # a small segment whose keys span more than 2^19 values
+    rng = np.random.default_rng(4)
+    keys = np.concatenate([[0, 1 << 25, 5, 1 << 25], rng.integers(0, 1000, 3000)]).astype(np.int64)
+    vals = rng.random(len(keys))
+    off = np.array([0, 4, len(keys)], np.uint64)
+    _run_and_check(fa, keys, vals, off, 30)
+
+
+@pytest.mark.parametrize("digit_bits", [3, 5, 8])
+def test_batch_digit_width_and_bucket_rows(fa, digit_bits):
+    # the tunables the autotuner sets: LDS sort digit width and bucket rows
+    # per size class; every setting gives the reference's result
+    from frovedis_amd import autotune as at
+    from frovedis_amd import datagen as dg
+    sizes = dg.segment_sizes(nseg=64, total=1 << 20, lo=256, hi=1 << 17)
+    rng = np.random.default_rng(digit_bits)
+    keys, vals, off = _case(sizes, seed=digit_bits)
+    at.tune_set("batch.digit_bits", digit_bits)
+    at.tune_set("batch.bucket_rows", 2048 if digit_bits == 3 else 6144)
+    at.tune_set("batch.bucket_rows.c17", 1024)
+    try:
+        for use_graph in (False, True):
+            _run_and_check(fa, keys, vals, off, 18, use_graph)
+    finally:
+        for name in ("batch.digit_bits", "batch.bucket_rows", "batch.bucket_rows.c17"):
+            at.tune_set(name, None)
+
+
+def test_batch_composite_path_matches_oracle(fa):
+    from frovedis_amd import autotune as at
+    keys, vals, off = _case(np.array([256, 9000, 3, 70000], np.int64), seed=9)
+    at.tune_set("batch.path", 1)
+    try:
+        for use_graph in (False, True):
+            _run_and_check(fa, keys, vals, off, 18, use_graph)
+    finally:
+        at.tune_set("batch.path", None)
