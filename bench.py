@@ -53,9 +53,27 @@ def gen_groupby(n, device="cuda"):
     return k, v, iv
 
 
-def step_groupby(fa, k, v, iv):
+def groupby_model(fa, n, sample_log2=24):
+    """A CdfModel of the key column trained on an earlier input of the same
+    generator (another seed, 2^sample_log2 rows, sorted): its cardinality
+    estimate for n rows sizes the group-by's tables.  Returns the model and
+    the seconds spent (outside the timed region, as a model is trained once
+    per column, not per query)."""
+    import numpy as np
+    import torch
+
+    from frovedis_amd import datagen as dg
+    t0 = time.perf_counter()
+    prev = fa.DeviceArray(dg.zipf_keys_dev(1 << sample_log2, seed=dg.SEED_ZIPF + 1), np.int64)
+    fa.radix_sort(prev)
+    model = fa.CdfModel(prev, steps=300)
+    torch.cuda.synchronize()
+    return model, time.perf_counter() - t0
+
+
+def step_groupby(fa, k, v, iv, model):
     return fa.group_by_agg(k, [("sum", v), ("count", v), ("avg", v), ("max", iv)],
-                           table_hint=1 << 20)
+                           model=model)
 
 
 def cpu_baseline_groupby(sample_log2=24, reps=3):
@@ -160,13 +178,21 @@ def bench_learned(fa, log2n, steps=3):
         stages[name] = {"ms_per_step": ms / steps, "launches_per_step": cnt / steps}
     fa.check(fa.lib().fhip_prof_enable(0))
     exact = bool(torch.equal(keys.t, ref))
-    flat_s, fb_flat = timed(lambda: model.sort(keys, equal_width=True))
+    # the same pipeline without the network: equal-width coarse ranges, and
+    # the training sample's quantiles as coarse splitters
+    flat_s, fb_flat = timed(lambda: model.sort(keys, coarse="equal_width"))
+    exact = exact and bool(torch.equal(keys.t, ref))
+    quant_s, fb_quant = timed(lambda: model.sort(keys, coarse="quantile"))
+    exact = exact and bool(torch.equal(keys.t, ref))
     del ref
     return {"keys": n, "plain_ms": plain_s * 1e3, "learned_ms": learned_s * 1e3,
             "speedup": plain_s / learned_s, "bit_exact": exact,
             "fallback_keys": fb[0], "fallback_buckets": fb[1], "max_coarse_bucket": fb[2],
             "equal_width_ms": flat_s * 1e3, "equal_width_fallback_keys": fb_flat[0],
             "equal_width_max_coarse_bucket": fb_flat[2],
+            "quantile_ms": quant_s * 1e3, "quantile_fallback_keys": fb_quant[0],
+            "quantile_max_coarse_bucket": fb_quant[2],
+            "network_gain_over_quantile": quant_s / learned_s,
             "plain_keys_per_s": n / plain_s, "learned_keys_per_s": n / learned_s,
             "train_s": train_s, "model_loss": model.info()["loss"], "stages": stages}
 
@@ -426,15 +452,17 @@ def main():
     n = 1 << args.rows
     print(f"# generating {n} rows", file=sys.stderr, flush=True)
     k, v, iv = gen_groupby(n)
+    model, train_s = groupby_model(fa, n)
+    hint = model.table_hint(n)
     torch.cuda.synchronize()
     for _ in range(args.warmup):
-        step_groupby(fa, k, v, iv)
+        step_groupby(fa, k, v, iv, model)
     torch.cuda.synchronize()
     # per-kernel breakdown of the step (every named kernel timed), outside
     # the timed region: it finds the dominant kernel
     fa.check(fa.lib().fhip_prof_enable(1))
     for _ in range(args.steps):
-        step_groupby(fa, k, v, iv)
+        step_groupby(fa, k, v, iv, model)
     torch.cuda.synchronize()
     breakdown = {}
     for name in ("groupby_hot", "groupby_hot_merge", "groupby_part_hist",
@@ -454,7 +482,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        gk, outs = step_groupby(fa, k, v, iv)
+        gk, outs = step_groupby(fa, k, v, iv, model)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     dom_ms, dom_cnt = _prof_ms(fa, dom)
@@ -489,7 +517,10 @@ def main():
                      "kernel_ms": kern_s * 1e3,
                      "algorithmic_bytes_per_launch": algo_bytes},
         "cpu_baseline": None,
-        "extra": {"groupby_kernels": breakdown},
+        "extra": {"groupby_kernels": breakdown,
+                  "table_hint": {"from": "CdfModel of an earlier 2^24-row input (another seed)",
+                                 "estimated_groups": hint, "groups": groups,
+                                 "train_s": train_s}},
     }
     del k, v, iv, gk, outs
     torch.cuda.empty_cache()

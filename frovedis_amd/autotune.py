@@ -20,9 +20,15 @@ saved and loaded as one flat JSON object (fhip_autotune_save / _load):
 - ``multi_group_by.composite``: 0 switches off the composite-key path of
   the multi-column group_by;
 - ``join.load_pct``: load factor (10 to 50 %) of the joins' hash table for
-  integer keys.
+  integer keys;
+- ``batch.digit_bits``: digit width (2 to 8 bits) of the LDS sort passes of
+  the batched segments' jobs; ``batch.bucket_rows``: rows per bucket job of a
+  large segment, and ``batch.bucket_rows.c<k>`` the same for the size class
+  of segments with 2^(k-1) < rows <= 2^k (these decide how many bucket jobs,
+  the batch path's tiles, a segment is cut into); ``batch.path``: 1 selects
+  the composite-key sort instead of the work queue.
 
-``tune_groupby`` and ``tune_radix`` measure candidate settings on the device for a given key
+``tune_groupby``, ``tune_radix`` and ``tune_batch`` measure candidate settings on the device for a given key
 column and aggregates, keeps the fastest whose output equals the default
 setting's, and sets it.  ``load`` / ``save`` keep the result between runs,
 for example in ``~/.cache/frovedis_amd/tune.json``.
@@ -34,8 +40,9 @@ import time
 
 from . import check, lib
 
-__all__ = ["tune_set", "tune_get", "save", "load", "tune_groupby", "tune_radix", "DEFAULT_CACHE",
-           "RADIX_SHAPES", "COMPACT_SHAPES"]
+__all__ = ["tune_set", "tune_get", "save", "load", "tune_groupby", "tune_radix", "tune_batch",
+           "DEFAULT_CACHE", "RADIX_SHAPES", "COMPACT_SHAPES", "BATCH_DIGIT_BITS",
+           "BATCH_BUCKET_ROWS", "batch_size_class"]
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "frovedis_amd", "tune.json")
 
@@ -51,6 +58,18 @@ RADIX_SHAPES = ((256, 16), (256, 32), (512, 16), (512, 24), (512, 12), (1024, 8)
                 (768, 16))
 
 COMPACT_SHAPES = ((256, 16), (1024, 16))
+
+BATCH_DIGIT_BITS = (4, 6, 8)
+BATCH_BUCKET_ROWS = (2048, 3072, 3584)
+
+
+def batch_size_class(rows):
+    """The size class k of a segment of `rows` rows: 2^(k-1) < rows <= 2^k
+    (the suffix of the per-class tunable ``batch.bucket_rows.c<k>``)."""
+    k = 0
+    while (1 << k) < rows:
+        k += 1
+    return k
 
 
 def tune_set(name, value):
@@ -167,3 +186,68 @@ def tune_radix(keys, vals=None, reps=3):
     if vals is not None:
         vals.t.copy_(v0)
     return best[0], best[1], results
+
+
+def tune_batch(keys, vals, seg_off, key_bits, reps=2, classes=None):
+    """Autotunes the batched segments (fhip_dev_batch_sort_groupby) on the
+    given batch: first the LDS sort's digit width (BATCH_DIGIT_BITS), then,
+    one size class at a time (the classes present among the segments with
+    more rows than one job holds, largest rows first, or `classes`), the
+    bucket rows of that class (BATCH_BUCKET_ROWS).  A candidate is kept only
+    if it is faster and its output equals the default setting's (sorted
+    rows, group keys, sums and counts).  Returns (best_seconds, setting,
+    all_results)."""
+    import numpy as np
+    import torch
+
+    from .batch import batch_sort_groupby
+    sizes = np.diff(seg_off.numpy().astype(np.int64))
+    if classes is None:
+        classes = sorted({batch_size_class(int(m)) for m in sizes if m > 4096}, reverse=True)
+    names = ["batch.digit_bits"] + [f"batch.bucket_rows.c{c}" for c in classes]
+    for nm in names:
+        tune_set(nm, None)
+    out = {}
+
+    def run():
+        return batch_sort_groupby(keys, vals, seg_off, key_bits, use_graph=False, out=out.get("o"))
+
+    ref = run()
+    out["o"] = None
+    ref = {k: (v.t.clone() if hasattr(v, "t") else v) for k, v in ref.items()}
+
+    def same(r):
+        G = ref["ngroups"]
+        if r["ngroups"] != G:
+            return False
+        for k in ("keys", "vals"):
+            if not torch.equal(r[k].t, ref[k]):
+                return False
+        return all(torch.equal(r[k].t[:G], ref[k][:G]) for k in ("group_keys", "group_sum",
+                                                                   "group_count"))
+
+    def measure():
+        best = float("inf")
+        r = None
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = run()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        return best, same(r)
+
+    best_s, _ = measure()
+    setting = {nm: None for nm in names}
+    results = [(best_s, dict(setting), True)]
+    for nm, cands in [("batch.digit_bits", BATCH_DIGIT_BITS)] + \
+            [(f"batch.bucket_rows.c{c}", BATCH_BUCKET_ROWS) for c in classes]:
+        for v in cands:
+            tune_set(nm, v)
+            t, ok = measure()
+            cand = dict(setting, **{nm: v})
+            results.append((t, cand, ok))
+            if ok and t < best_s:
+                best_s, setting = t, cand
+        tune_set(nm, setting[nm])
+    return best_s, setting, results

@@ -912,6 +912,7 @@ struct QPlan {
   size_t capacity = 0;
   std::vector<size_t> off;  // seg_off contents
   long long tunables[4] = {0, 0, 0, 0};
+  std::vector<long long> class_rows;  // batch.bucket_rows.c<k> for every k (-1: unset)
   unsigned nitems = 0, njobs = 0, nlarge = 0, nchunks = 0, ntiles = 0;
   size_t lb_words = 0;
   DBuf items, lsegs, fine_hist, bmap, bstart, lookback, job_state, tk, tv, scal;
@@ -927,7 +928,8 @@ struct QPlan {
            vals_out == o.vals_out && gkeys == o.gkeys && gsum == o.gsum && gcount == o.gcount &&
            gseg_off == o.gseg_off && capacity == o.capacity && off == o.off &&
            tunables[0] == o.tunables[0] && tunables[1] == o.tunables[1] &&
-           tunables[2] == o.tunables[2] && tunables[3] == o.tunables[3];
+           tunables[2] == o.tunables[2] && tunables[3] == o.tunables[3] &&
+           class_rows == o.class_rows;
   }
 };
 
@@ -1082,6 +1084,8 @@ bool queue_sort_groupby(const void* keys, const void* vals, size_t n, const size
   q->tunables[1] = tune_get("batch.bucket_rows", 3072);
   q->tunables[2] = tune_get("batch.hist_chunk", 1 << 16);
   q->tunables[3] = std::max(-1LL, std::min((long long)BQ_EARLY_BITS, tune_get("batch.early_bits", BQ_EARLY_BITS)));
+  for (int c = 0; c < 64; c++)
+    q->class_rows.push_back(tune_get(("batch.bucket_rows.c" + std::to_string(c)).c_str(), -1));
   if (!(g_qplan && g_qplan->same(*q))) {
     build_queue(*q, st);
     g_qplan = std::move(q);

@@ -453,11 +453,14 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
                  size_t capacity, int* used_hash, size_t* only_left, size_t* n_only,
                  hipStream_t st) {
   if (n_only) *n_only = 0;
-  FHIP_REQUIRE(strategy >= FHIP_JOIN_AUTO && strategy <= FHIP_JOIN_SORTMERGE, FHIP_ERR_ARG,
+  FHIP_REQUIRE(strategy >= FHIP_JOIN_AUTO && strategy <= FHIP_JOIN_AUTO_UNIQUE, FHIP_ERR_ARG,
                "equi_join: bad strategy");
   if (used_hash) *used_hash = 0;
   bool dup_seen = false;
-  if (strategy == FHIP_JOIN_AUTO) {
+  // AUTO_UNIQUE: AUTO for a build side the caller expects unique (the
+  // learned model's prediction); the table build alone decides
+  if (strategy == FHIP_JOIN_AUTO_UNIQUE) strategy = FHIP_JOIN_AUTO;
+  else if (strategy == FHIP_JOIN_AUTO) {
     dispatch_type(ktype, [&](auto z) {
       using K = decltype(z);
       // integer keys only: for floating keys the reference's uniqueness

@@ -549,6 +549,21 @@ __global__ void to_feature_kernel(const uint64_t* __restrict__ sorted, size_t n,
   }
 }
 
+// distinct keys of a sorted column: rows whose key differs from the row
+// before them (plus the first), one atomic per workgroup
+__global__ __launch_bounds__(256) void count_runs_kernel(const uint64_t* __restrict__ k, size_t n,
+                                                         unsigned long long* __restrict__ out) {
+  const size_t stride = size_t(gridDim.x) * 256;
+  unsigned long long c = 0;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+    c += (i == 0 || k[i] != k[i - 1]) ? 1ull : 0ull;
+  c = wave_sum(c);
+  __shared__ unsigned long long ws[4];
+  if (__lane_id() == 0) ws[threadIdx.x / 64] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, ws[0] + ws[1] + ws[2] + ws[3]);
+}
+
 }  // namespace
 }  // namespace fhip
 
@@ -564,6 +579,7 @@ struct fhip_cdf_model {
   std::vector<uint64_t> fine;        // NF leaf splitters (sample quantiles)
   std::vector<uint64_t> sample;      // ordered sample keys (sorted)
   size_t train_n = 0;                // rows of the training input
+  size_t train_distinct = 0;         // distinct keys of the training input
   double final_loss = 0;
 };
 
@@ -675,6 +691,17 @@ fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t sa
     FHIP_CHECK(hipStreamSynchronize(st));
     m->umin = ordered_host(first, sgn);
     m->umax = ordered_host(last, sgn);
+    {
+      DBuf cb(8);
+      FHIP_CHECK(hipMemsetAsync(cb.get(), 0, 8, st));
+      count_runs_kernel<<<grid_for_n(n), 256, 0, st>>>(static_cast<const uint64_t*>(sorted), n,
+                                                      cb.as<unsigned long long>());
+      FHIP_CHECK(hipGetLastError());
+      unsigned long long d = 0;
+      FHIP_CHECK(hipMemcpyAsync(&d, cb.get(), 8, hipMemcpyDeviceToHost, st));
+      FHIP_CHECK(hipStreamSynchronize(st));
+      m->train_distinct = size_t(d);
+    }
     const double span = std::log2(1.0 + double(m->umax - m->umin));
     m->inv_span = span > 0 ? 1.0 / span : 1.0;
     // the network trains on at most 2^18 of the samples (full batch)
@@ -747,7 +774,7 @@ fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t sa
 // radix fallback (n if the whole input did).  equal_width: replace the
 // model's coarse splitters by equal-width ranges of the ordered key space
 // (the same pipeline without the network).
-size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_width,
+size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mode,
                     size_t* fallback_buckets, size_t* max_coarse, hipStream_t st) {
   if (fallback_buckets) *fallback_buckets = 0;
   if (max_coarse) *max_coarse = n;
@@ -755,10 +782,21 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
   const bool sgn = m->ktype == FHIP_I64;
   std::vector<uint64_t> coarse = m->coarse;
   std::vector<unsigned> kidx = m->kidx;
-  if (equal_width) {  // the same pipeline with equal-width coarse buckets, no network
+  FHIP_REQUIRE(coarse_mode >= FHIP_COARSE_NETWORK && coarse_mode <= FHIP_COARSE_QUANTILE,
+               FHIP_ERR_ARG, "learned_sort: bad coarse splitter mode");
+  if (coarse_mode == FHIP_COARSE_EQUAL_WIDTH) {
+    // the same pipeline with equal-width coarse buckets, no network
     std::vector<uint64_t> want(LS_B1, 0);
     const double lo = double(m->umin), hi = double(m->umax);
     for (int b = 1; b < LS_B1; b++) want[b] = uint64_t(lo + (hi - lo) * double(b) / LS_B1);
+    snap_coarse(want, m->fine, coarse, kidx);
+  } else if (coarse_mode == FHIP_COARSE_QUANTILE) {
+    // the same pipeline without the network: the coarse splitters are the
+    // training sample's quantiles b / LS_B1 (snapped to leaf splitters)
+    std::vector<uint64_t> want(LS_B1, 0);
+    const size_t S = m->sample.size();
+    for (int b = 1; b < LS_B1 && S; b++)
+      want[b] = std::max(want[b - 1], m->sample[std::min(S - 1, size_t(b) * S / LS_B1)]);
     snap_coarse(want, m->fine, coarse, kidx);
   }
   const size_t nf = LS_NF;
@@ -885,6 +923,10 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, bool equal_wi
 // sum over knot intervals of w * (1 - exp(-n * mass / w)) with w the number
 // of integer keys in the interval.
 double learned_cardinality(const fhip_cdf_model* m, size_t n) {
+  // a training column whose keys were all distinct (row ids, surrogate
+  // keys): its draws were not independent, and n new rows of it are
+  // expected to be distinct too
+  if (m->train_n > 1 && m->train_distinct == m->train_n) return double(n);
   double total = 0.0;
   uint64_t prev_key = m->umin;
   double prev_c = 0.0;
@@ -932,12 +974,12 @@ int fhip_learned_model_params(const fhip_cdf_model* m, float* params) {
   });
 }
 
-int fhip_dev_learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int equal_width,
+int fhip_dev_learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mode,
                           size_t* fallback_keys, size_t* fallback_buckets,
                           size_t* max_coarse_bucket, void* stream) {
   return fhip::guarded([&] {
     FHIP_REQUIRE(m && (n == 0 || keys), FHIP_ERR_ARG, "learned_sort: null pointer");
-    size_t fb = fhip::learned_sort(m, keys, n, equal_width != 0, fallback_buckets,
+    size_t fb = fhip::learned_sort(m, keys, n, coarse_mode, fallback_buckets,
                                    max_coarse_bucket, fhip::as_stream(stream));
     if (fallback_keys) *fallback_keys = fb;
   });
@@ -947,6 +989,24 @@ int fhip_learned_cardinality(const fhip_cdf_model* m, size_t n, double* distinct
   return fhip::guarded([&] {
     FHIP_REQUIRE(m && distinct, FHIP_ERR_ARG, "learned_cardinality: null pointer");
     *distinct = fhip::learned_cardinality(m, n);
+  });
+}
+
+int fhip_learned_table_hint(const fhip_cdf_model* m, size_t n, size_t* table_hint) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(m && table_hint, FHIP_ERR_ARG, "learned_table_hint: null pointer");
+    const double d = fhip::learned_cardinality(m, n);
+    *table_hint = size_t(std::ceil(std::max(1.0, d)));
+  });
+}
+
+int fhip_learned_join_strategy(const fhip_cdf_model* m, size_t n_build, int* strategy) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(m && strategy, FHIP_ERR_ARG, "learned_join_strategy: null pointer");
+    // predicted unique build keys: AUTO without the sampled duplicate test
+    // (the table build decides, as in the reference); else AUTO with it
+    const double d = fhip::learned_cardinality(m, n_build);
+    *strategy = d >= 0.999 * double(n_build) ? FHIP_JOIN_AUTO_UNIQUE : FHIP_JOIN_AUTO;
   });
 }
 

@@ -63,14 +63,18 @@ class _Agg(ctypes.Structure):
                 ("col", ctypes.c_void_p), ("out", ctypes.c_void_p)]
 
 
-def group_by_agg(keys, aggs, table_hint=0, capacity=None, stream=None):
+def group_by_agg(keys, aggs, table_hint=0, capacity=None, stream=None, model=None):
     """Fused group_by(keys).select({keys}, aggs) on the hash path.
 
     ``aggs`` is a list of (op, column) with op in sum/count/avg/max/min
     (column may be None for count).  Returns (group_keys, [outputs]) as
     device arrays, groups ascending by key like grouped_dftable::select
-    (dftable.cc:3724-3772)."""
+    (dftable.cc:3724-3772).  ``model``: a CdfModel of the key column
+    (trained on an earlier input) whose cardinality estimate becomes the
+    table hint that sizes the partitioned hash tables."""
     n = len(keys)
+    if model is not None and not table_hint:
+        table_hint = model.table_hint(n)
     dev = keys.t.device
     cap = int(capacity) if capacity is not None else max(1, min(n, 1 << 22))
     for _ in range(2):
@@ -162,14 +166,20 @@ class UniqueHashtable:
             pass
 
 
-JOIN_STRATEGY = {"auto": 0, "hash": 1, "sortmerge": 2}
+JOIN_STRATEGY = {"auto": 0, "hash": 1, "sortmerge": 2, "auto_unique": 3}
 
 
-def equi_join(left, left_idx, right, right_idx, strategy="auto", capacity=None, stream=None):
+def equi_join(left, left_idx, right, right_idx, strategy="auto", capacity=None, stream=None,
+              model=None):
     """equi_join<T> (dataframe/join.hpp:304-326): returns
-    (left_idx_out, right_idx_out, used_hash)."""
+    (left_idx_out, right_idx_out, used_hash).  With a CdfModel of the right
+    key column (trained on an earlier input) and strategy "auto", the model
+    picks the AUTO variant (fhip_learned_join_strategy); the result is the
+    same."""
     if left.dtype != right.dtype:
         raise FhipError("column types are different")
+    if model is not None and strategy == "auto":
+        strategy = model.join_strategy(len(right))
     dev = left.t.device
     cap = int(capacity) if capacity is not None else max(1, len(left))
     for _ in range(2):

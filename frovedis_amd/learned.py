@@ -18,6 +18,7 @@ __all__ = ["CdfModel", "mlp_nparam", "mlp_grad", "mlp_forward", "mlp_adam"]
 
 KNOTS = 4096
 COARSE = 1024
+COARSE_MODE = {"network": 0, "equal_width": 1, "quantile": 2}
 
 
 class CdfModel:
@@ -46,13 +47,17 @@ class CdfModel:
         check(lib().fhip_learned_model_params(self._m, p.ctypes.data_as(ctypes.c_void_p)))
         return p
 
-    def sort(self, keys, equal_width=False, stream=None):
+    def sort(self, keys, equal_width=False, coarse=None, stream=None):
         """Sorts keys in place; returns (fallback_keys, fallback_buckets,
-        max_coarse_bucket)."""
+        max_coarse_bucket).  coarse: "network" (default), or the same
+        pipeline without the network: "equal_width" (also equal_width=True)
+        or "quantile" (the training sample's quantiles)."""
         if keys.dtype != self.dtype:
             raise FhipError("learned sort: key type differs from the model's")
+        coarse = coarse or ("equal_width" if equal_width else "network")
+        mode = COARSE_MODE[coarse]
         fk, fb, mc = ctypes.c_size_t(0), ctypes.c_size_t(0), ctypes.c_size_t(0)
-        check(lib().fhip_dev_learned_sort(self._m, keys.ptr, _sz(len(keys)), int(bool(equal_width)),
+        check(lib().fhip_dev_learned_sort(self._m, keys.ptr, _sz(len(keys)), mode,
                                           ctypes.byref(fk), ctypes.byref(fb), ctypes.byref(mc),
                                           stream or current_stream()))
         return fk.value, fb.value, mc.value
@@ -61,6 +66,19 @@ class CdfModel:
         d = ctypes.c_double(0)
         check(lib().fhip_learned_cardinality(self._m, _sz(n), ctypes.byref(d)))
         return d.value
+
+    def table_hint(self, n):
+        """The group-by table hint for n rows of this column."""
+        h = ctypes.c_size_t(0)
+        check(lib().fhip_learned_table_hint(self._m, _sz(n), ctypes.byref(h)))
+        return h.value
+
+    def join_strategy(self, n_build):
+        """"auto_unique" when the model predicts n_build distinct keys on the
+        build side, else "auto" (the join's result is the same)."""
+        s = ctypes.c_int(0)
+        check(lib().fhip_learned_join_strategy(self._m, _sz(n_build), ctypes.byref(s)))
+        return {0: "auto", 3: "auto_unique"}[s.value]
 
     def __del__(self):
         try:

@@ -245,6 +245,10 @@ int fhip_ht_destroy(fhip_ht* ht);
 #define FHIP_JOIN_AUTO 0
 #define FHIP_JOIN_HASH 1
 #define FHIP_JOIN_SORTMERGE 2
+/* AUTO for a right side the caller expects unique (fhip_learned_join_strategy):
+ * the sampled duplicate test is skipped and the table build decides, so the
+ * result is AUTO's. */
+#define FHIP_JOIN_AUTO_UNIQUE 3
 int fhip_dev_equi_join(int ktype, const void* left, const size_t* left_idx,
                        size_t nl, const void* right, const size_t* right_idx,
                        size_t nr, int strategy, size_t* left_out,
@@ -396,24 +400,40 @@ int fhip_grouped_agg(int op, int vtype, const void* col, size_t col_n,
  *  - fhip_dev_learned_sort: sorts in place; *fallback_keys gets how many
  *    keys went through the plain radix sort because their leaf overflowed
  *    (n when the whole input fell back); *max_coarse_bucket the largest
- *    coarse bucket (load balance of the split stage).  equal_width = 1 runs
- *    the same pipeline with equal-width coarse buckets instead of the
- *    network (for comparison).
- *  - fhip_learned_cardinality: expected distinct keys among n draws, from
- *    the model (sizes group-by tables, picks join strategies).
+ *    coarse bucket (load balance of the split stage).  coarse_mode selects
+ *    where the coarse splitters come from: FHIP_COARSE_NETWORK (the
+ *    trained network), or, for comparison, the same pipeline without it:
+ *    FHIP_COARSE_EQUAL_WIDTH (equal-width ranges of the key space) or
+ *    FHIP_COARSE_QUANTILE (the training sample's quantiles b / 1024,
+ *    snapped to leaf splitters).
+ *  - fhip_learned_cardinality: expected distinct keys among n rows, from
+ *    the model: the training column's own count when its keys were all
+ *    distinct (n), else n independent draws from the model's CDF.
+ *  - fhip_learned_table_hint: that estimate as the table_hint of
+ *    fhip_dev_group_by_agg (it sizes the partitioned hash tables).
+ *  - fhip_learned_join_strategy: FHIP_JOIN_AUTO_UNIQUE when the model
+ *    predicts n_build distinct keys, else FHIP_JOIN_AUTO; the join's output
+ *    is the same either way.
  *  - fhip_dev_mlp_*: the network's kernels (gradient of the mean squared
  *    error, forward pass, Adam step) on a parameter vector of
  *    fhip_mlp_nparam() floats. */
+#define FHIP_COARSE_NETWORK 0
+#define FHIP_COARSE_EQUAL_WIDTH 1
+#define FHIP_COARSE_QUANTILE 2
 typedef struct fhip_cdf_model fhip_cdf_model;
 int fhip_learned_train(int ktype, const void* sorted_keys, size_t n,
                        size_t samples, int steps, float lr,
                        fhip_cdf_model** model, void* stream);
 int fhip_dev_learned_sort(const fhip_cdf_model* m, void* keys, size_t n,
-                          int equal_width, size_t* fallback_keys,
+                          int coarse_mode, size_t* fallback_keys,
                           size_t* fallback_buckets, size_t* max_coarse_bucket,
                           void* stream);
 int fhip_learned_cardinality(const fhip_cdf_model* m, size_t n,
                              double* distinct);
+int fhip_learned_table_hint(const fhip_cdf_model* m, size_t n,
+                            size_t* table_hint);
+int fhip_learned_join_strategy(const fhip_cdf_model* m, size_t n_build,
+                               int* strategy);
 int fhip_learned_model_info(const fhip_cdf_model* m, double* final_loss,
                             double* cdf_knots, uint64_t* coarse_splitters);
 int fhip_learned_model_params(const fhip_cdf_model* m, float* params);

@@ -44,3 +44,10 @@ def test_load_rejects_malformed(at, tmp_path):
     p.write_text('{"groupby.slots": "many"}')
     with pytest.raises(frovedis_amd.FhipError):
         at.load(p)
+
+
+def test_batch_size_class(at):
+    assert at.batch_size_class(4097) == 13
+    assert at.batch_size_class(8192) == 13
+    assert at.batch_size_class(8193) == 14
+    assert at.batch_size_class(1 << 20) == 20

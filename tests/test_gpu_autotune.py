@@ -53,3 +53,31 @@ def test_tune_groupby_small_space(fa, tmp_path):
         assert at.tune_get(nm, -1) == (-1 if val is None else val)
     for nm in space:
         at.tune_set(nm, None)
+
+
+def test_tune_batch_per_size_class(fa):
+    # the batched segments' tuner: digit width, then the bucket rows of each
+    # size class; every kept setting reproduces the default's output and the
+    # batch path then runs with it
+    import oracle
+    from frovedis_amd import autotune as at
+    from frovedis_amd import datagen as dg
+    sizes = dg.segment_sizes(nseg=64, total=1 << 20, lo=256, hi=1 << 16)
+    keys, vals, off = dg.batch_inputs_dev(sizes)
+    k = fa.DeviceArray(keys, np.int64)
+    v = fa.DeviceArray(vals, np.float64)
+    o = fa.DeviceArray(off, np.uint64)
+    classes = sorted({at.batch_size_class(int(m)) for m in sizes if m > 4096}, reverse=True)[:2]
+    best_s, setting, results = at.tune_batch(k, v, o, 16, reps=1, classes=classes)
+    assert len(results) == 1 + len(at.BATCH_DIGIT_BITS) + len(classes) * len(at.BATCH_BUCKET_ROWS)
+    assert all(ok for _, _, ok in results), [r for r in results if not r[2]]
+    for nm, val in setting.items():
+        assert at.tune_get(nm, -1) == (-1 if val is None else val)
+    res = fa.batch_sort_groupby(k, v, o, key_bits=16, use_graph=True)
+    kh, oh = keys.cpu().numpy(), off.cpu().numpy()
+    s = int(np.argmax(sizes))
+    a, b = int(oh[s]), int(oh[s + 1])
+    ek, _ = oracle.radix_sort(kh[a:b])
+    assert np.array_equal(res["keys"].numpy()[a:b], ek)
+    for nm in setting:
+        at.tune_set(nm, None)

@@ -176,3 +176,90 @@ def test_model_cardinality_estimate(fa):
     truth = len(torch.unique(cur))
     est = model.cardinality(n)
     assert truth / 2 <= est <= truth * 2, (est, truth)
+
+
+def _best_ms(fn, reps=3):
+    import time
+    import torch
+    best = float("inf")
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best * 1e3
+
+
+def test_network_against_sample_quantile_splitters(fa):
+    # The ablation: the same pipeline with the coarse splitters taken as the
+    # training sample's quantiles b/2048 instead of the network, and with
+    # equal-width ranges.  All three sort exactly.  Measured at 2^24 on the
+    # lognormal mixture: network 58K / quantile 56K / equal-width 16.3M keys
+    # in the largest coarse bucket, 1.32 / 1.32 / 12.4 ms -- the network
+    # matches the sample quantiles (it gains nothing over them on this
+    # input) and both beat equal-width buckets by far.  The figures are
+    # printed (bench.py reports the times at 2^28).
+    import sys
+    import torch
+    from frovedis_amd import datagen as dg
+    n = 1 << 24
+    model = _trained_model(fa, n, dg.SEED_LOGN + 300)
+    keys = dg.lognormal_mixture_dev(n)
+    ref = _plain_sorted(fa, keys)
+    out = {}
+    for coarse in ("network", "quantile", "equal_width"):
+        k = fa.DeviceArray(keys.clone(), np.int64)
+        fb, nb, mc = model.sort(k, coarse=coarse)
+        assert torch.equal(k.t, ref), coarse
+
+        def run():
+            k.t.copy_(keys)
+            model.sort(k, coarse=coarse)
+        out[coarse] = (mc, fb, _best_ms(run))
+    print("coarse splitters: (max coarse bucket, fallback keys, ms)", out, file=sys.stderr)
+    assert out["network"][0] <= 2 * out["quantile"][0], out
+    assert out["network"][0] * 20 < out["equal_width"][0], out
+    assert out["quantile"][0] * 20 < out["equal_width"][0], out
+
+
+def test_model_sizes_group_by_tables(fa):
+    # the model of an earlier input's key column gives the group-by's table
+    # hint; the results equal those without it
+    import torch
+    from frovedis_amd import datagen as dg
+    n = 1 << 22
+    prev = dg.zipf_keys_dev(n, start=n)
+    model = fa.CdfModel(fa.DeviceArray(_plain_sorted(fa, prev), np.int64), steps=300)
+    k = fa.DeviceArray(dg.zipf_keys_dev(n), np.int64)
+    v = fa.DeviceArray(dg.normal_dev(n), np.float64)
+    truth = len(torch.unique(k.t))
+    hint = model.table_hint(n)
+    assert truth / 2 <= hint <= truth * 2, (hint, truth)
+    g0, (s0, c0) = fa.group_by_agg(k, [("sum", v), ("count", None)])
+    g1, (s1, c1) = fa.group_by_agg(k, [("sum", v), ("count", None)], model=model)
+    assert torch.equal(g0.t, g1.t) and torch.equal(c0.t, c1.t)
+    assert torch.allclose(s0.t, s1.t, rtol=1e-12, atol=1e-9)
+
+
+def test_model_picks_join_strategy(fa):
+    # a unique build column (a permutation: the training column's keys are
+    # all distinct) -> AUTO without the sampled duplicate test; a foreign-key
+    # column with repeats -> AUTO with it; the pairs are those of AUTO
+    import torch
+    from frovedis_amd import datagen as dg
+    nl, nr = 1 << 18, 1 << 20
+    lk = fa.DeviceArray(dg.perm_keys_dev(nl), np.int64)
+    rk = fa.DeviceArray(dg.fk_keys_dev(nr, domain=nl), np.int64)
+    li = fa.DeviceArray(torch.arange(nl, dtype=torch.int64, device="cuda"), np.uint64)
+    ri = fa.DeviceArray(torch.arange(nr, dtype=torch.int64, device="cuda"), np.uint64)
+    m_unique = fa.CdfModel(fa.DeviceArray(_plain_sorted(fa, dg.perm_keys_dev(nl, seed=77)),
+                                          np.int64), steps=100)
+    m_fk = fa.CdfModel(fa.DeviceArray(_plain_sorted(fa, dg.fk_keys_dev(nr, domain=nl, seed=78)),
+                                      np.int64), steps=100)
+    assert m_unique.join_strategy(nl) == "auto_unique"
+    assert m_fk.join_strategy(nr) == "auto"
+    for (a, ai, b, bi, m) in [(rk, ri, lk, li, m_unique), (lk, li, rk, ri, m_fk)]:
+        lo0, ro0, u0 = fa.equi_join(a, ai, b, bi)
+        lo1, ro1, u1 = fa.equi_join(a, ai, b, bi, model=m)
+        assert u0 == u1 and torch.equal(lo0.t, lo1.t) and torch.equal(ro0.t, ro1.t)
