@@ -76,29 +76,160 @@ def step_groupby(fa, k, v, iv, model):
                            model=model)
 
 
-def cpu_baseline_groupby(sample_log2=24, reps=3):
-    """The oracle's restatement of the reference's per-rank group_by +
-    sum/count/avg/max lambdas (sort based, x86 fold), one core, on the first
-    2^sample_log2 rows of the same generators."""
+def _host_info():
+    """CPU model, logical CPUs usable by this process and RAM of the host."""
+    model, sockets, per_core = "unknown", set(), None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("physical id"):
+                sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    ram = None
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal"):
+                ram = round(int(line.split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "sockets": len(sockets) or None, "logical_cpus": os.cpu_count(),
+            "usable_cpus": usable, "ram_gib": ram}
+
+
+def _cpu_reps(states, prep, run, reps=5):
+    """One warm-up and `reps` timed repetitions of run(state) for every state
+    at once (one thread per state: the oracle's C calls release the GIL, so
+    the copies run concurrently, one per core, like one single-threaded rank
+    per core).  prep(state) (fresh inputs) runs before each repetition,
+    untimed.  Returns (best, median) wall seconds of a repetition."""
+    import statistics
+    import threading
+    times = []
+    for rep in range(1 + reps):
+        for st in states:
+            prep(st)
+        go = threading.Barrier(len(states) + 1)
+        errs = []
+
+        def work(st):
+            go.wait()
+            try:
+                run(st)
+            except Exception as e:  # reported after the join
+                errs.append(e)
+        th = [threading.Thread(target=work, args=(st,)) for st in states]
+        for t in th:
+            t.start()
+        go.wait()
+        t0 = time.perf_counter()
+        for t in th:
+            t.join()
+        dt = time.perf_counter() - t0
+        if errs:
+            raise errs[0]
+        if rep:
+            times.append(dt)
+    return min(times), statistics.median(times)
+
+
+def cpu_baselines(ncores=None, groupby_log2=24, join_log2=(20, 22), reps=5):
+    """The CPU baseline of record, timed on this host: the oracle's C
+    restatement of the reference's x86 bodies (liboracle.so, built with the
+    reference's -O3 -pthread), one warm-up then `reps` repetitions, best and
+    median, on 1 core (one rank, latency) and on N cores (N copies at once,
+    aggregate throughput), N = the CPUs this process may use, at most 16.
+      radix:   radix_sort(uint64*, size_t*, 2^24) (radix_sort.hpp:197-219),
+               the full 2^24 configuration;
+      groupby: group_by + sum/count/avg/max (dfcolumn_groupby.cc:16-105,
+               dfcolumn_grouped_agg.cc), the first 2^groupby_log2 rows of the
+               2^28-row generators (a bounded sample);
+      join:    the x86 equi_join body (std::unordered_multimap, join.hpp:25-45)
+               on 2^20 unique left keys x 2^22 foreign keys, the join
+               generators at 1/128 scale."""
     import numpy as np
 
     import oracle
     from frovedis_amd import datagen as dg
-    n = 1 << sample_log2
-    k = dg.zipf_keys_np(n)
-    v = dg.normal_np(n)
-    iv = dg.int_uniform_np(n, 1000)
-    best = float("inf")
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        gi, split, keys = oracle.group_by(k)
-        oracle.grouped_agg("sum", v, gi, split)
-        oracle.grouped_agg("count", v, gi, split)
-        oracle.grouped_agg("avg", v, gi, split)
-        oracle.grouped_agg("max", iv, gi, split)
-        best = min(best, time.perf_counter() - t0)
-    return {"value": n / best, "unit": "rows/s", "cores": 1, "kind": "port",
-            "sample": f"first 2^{sample_log2} rows of the same generators, best of {reps}"}
+    info = _host_info()
+    N = ncores or max(1, min(16, info["usable_cpus"] or 1))
+    L = oracle.lib()
+    out = {"host": info, "cores_n": N, "code": "oracle restatement (liboracle.so, -O3 -pthread)",
+           "timing": f"1 warm-up + {reps} repetitions, best and median"}
+
+    def report(name, unit_items, sample, states1, statesN, prep, run):
+        print(f"# cpu baseline: {name}, 1 core", file=sys.stderr, flush=True)
+        b1, m1 = _cpu_reps(states1, prep, run, reps)
+        print(f"# cpu baseline: {name}, {N} cores", file=sys.stderr, flush=True)
+        bn, mn = _cpu_reps(statesN, prep, run, reps)
+        out[name] = {"sample": sample,
+                     "one_core": {"best_per_s": unit_items / b1, "median_per_s": unit_items / m1,
+                                  "best_s": b1, "median_s": m1},
+                     "n_cores": {"cores": N, "best_per_s": N * unit_items / bn,
+                                 "median_per_s": N * unit_items / mn, "best_s": bn, "median_s": mn}}
+
+    # radix: full-range uint64 keys + size_t index, sorted in place
+    n = 1 << 24
+    k0 = dg.splitmix64_np(dg.SEED_RADIX, 0, n)  # the keys of radix_keys_dev
+    v0 = np.arange(n, dtype=np.uint64)
+
+    def mk_radix(_):
+        return {"k": np.empty_like(k0), "v": np.empty_like(v0)}
+
+    def prep_radix(st):
+        np.copyto(st["k"], k0)
+        np.copyto(st["v"], v0)
+
+    def run_radix(st):
+        rc = L.orc_radix_sort(3, oracle._p(st["k"]), 8, oracle._p(st["v"]), n, 0, 0)
+        assert rc == 0
+    report("radix_kv_u64_2^24", n, "2^24 full-range uint64 keys + size_t index (the whole configuration)",
+           [mk_radix(0)], [mk_radix(i) for i in range(N)], prep_radix, run_radix)
+    del k0, v0
+
+    # group-by: the first 2^groupby_log2 rows of the Zipf / normal / int generators
+    g = 1 << groupby_log2
+    gk = dg.zipf_keys_np(g)
+    gv = dg.normal_np(g)
+    gi = dg.int_uniform_np(g, 1000)
+
+    def run_groupby(_):
+        idx, split, _keys = oracle.group_by(gk)
+        oracle.grouped_agg("sum", gv, idx, split)
+        oracle.grouped_agg("count", gv, idx, split)
+        oracle.grouped_agg("avg", gv, idx, split)
+        oracle.grouped_agg("max", gi, idx, split)
+    report("groupby_2^28_sample", g,
+           f"first 2^{groupby_log2} of the 2^28 rows (Zipf(1.1) int64 key, N(0,1) double, U[0,1000) int32)",
+           [0], list(range(N)), lambda st: None, run_groupby)
+    del gk, gv, gi
+
+    # join: x86 unordered_multimap body, unique left keys as the build side
+    nl, nr = 1 << join_log2[0], 1 << join_log2[1]
+    lk = dg.perm_keys_np(nl)
+    rk = dg.fk_keys_np(nr, domain=nl)
+    li = np.arange(nl, dtype=np.uint64)
+    ri = np.arange(nr, dtype=np.uint64)
+
+    def run_join(_):
+        a, b, _used = oracle.equi_join(lk, li, rk, ri, "x86")
+        assert len(a) == len(b)
+    report("join_x86_2^20_x_2^22", nl + nr,
+           "2^20 permutation keys x 2^22 foreign keys (90% hits), the join generators at 1/128 scale; "
+           "unit: input rows (nl + nr)", [0], list(range(N)), lambda st: None, run_join)
+    return out
+
+
+def cpu_baseline_groupby(base):
+    """The headline's cpu_baseline object: the group-by figure on N cores."""
+    gb = base["groupby_2^28_sample"]
+    return {"value": gb["n_cores"]["best_per_s"], "unit": "rows/s", "cores": base["cores_n"],
+            "kind": "port", "sample": gb["sample"] + f"; {base['cores_n']} concurrent copies, "
+            f"{base['timing']} (median {gb['n_cores']['median_per_s']:.4g} rows/s); one core: "
+            f"best {gb['one_core']['best_per_s']:.4g}, median {gb['one_core']['median_per_s']:.4g} rows/s",
+            "host": base["host"]}
 
 
 def bench_radix(fa, log2n, steps=3):
@@ -541,8 +672,10 @@ def main():
         result["extra"]["relational_2^28"] = bench_relational(fa, 28)
         torch.cuda.empty_cache()
     if not args.no_cpu:
-        print("# cpu baseline (oracle, one core)", file=sys.stderr, flush=True)
-        result["cpu_baseline"] = cpu_baseline_groupby()
+        print("# cpu baseline (oracle)", file=sys.stderr, flush=True)
+        base = cpu_baselines()
+        result["cpu_baseline"] = cpu_baseline_groupby(base)
+        result["extra"]["cpu_baselines"] = base
     print(json.dumps(result))
 
 
