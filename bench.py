@@ -264,6 +264,7 @@ def bench_radix(fa, log2n, steps=3):
     kern_s = (pass_ms + hist_ms) / 1e3 / steps
     return {"keys": n, "keys_per_s": n / t, "ms": t * 1e3, "passes": P, "digit_bits": 8,
             "algorithmic_bytes": algo, "kernel_ms": kern_s * 1e3,
+            "hist_ms": hist_ms / steps, "pass_ms": pass_ms / max(passes, 1),
             "hbm_frac": algo / kern_s / HBM_PEAK if kern_s else None}
 
 

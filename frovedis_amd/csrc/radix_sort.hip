@@ -87,18 +87,21 @@ __global__ __launch_bounds__(256) void rs_stats(const K* __restrict__ keys, size
 
 // ---- all-digit histograms in one read -------------------------------------
 // Every thread loads RH_ITEMS keys per step (independent loads in flight),
-// then counts each byte position into LDS; the lanes of a wave that share
-// lane 0's digit, and those that share the digit of the first lane with
-// another digit, add them with one atomic per digit (skewed keys put most
-// of a wave on one or two bins, and same-address atomics of one wave
-// serialise).
+// then counts each byte position into LDS.  Same-address LDS atomics of one
+// wave serialise, and skewed keys (a constant high byte, a heavy key) put
+// most of a wave on one bin, so the lanes holding lane 0's digit add their
+// count with one atomic when there are at least 8 of them.  HM picks how
+// that match is found:
+//   HM 2: lane 0's digit is shuffled per byte;
+//   HM 4: lane 0's key is shuffled once per key and every byte compared
+//         (the default: one shuffle per key instead of one per byte);
+//   HM 3: no match, one atomic per lane (for keys known to be uniform).
 constexpr int RH_ITEMS = 8;
 
-template <class U>
+template <class U, int HM>
 __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, size_t n,
                                                  int mode, int desc, int nbytes,
-                                                 unsigned long long* __restrict__ ghist,
-                                                 int two_leaders) {
+                                                 unsigned long long* __restrict__ ghist) {
   __shared__ unsigned h[sizeof(U) * 256];
   for (int i = threadIdx.x; i < int(sizeof(U)) * 256; i += RS_NT) h[i] = 0;
   __syncthreads();
@@ -117,43 +120,29 @@ __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, siz
       const size_t w0 = base + size_t(j) * RS_NT;  // the wave's first key of item j
       if (w0 >= n) break;                          // the same for the whole wave
       const bool valid = w0 + lane < n;
-      const unsigned long long vmask = __ballot(valid);
       // the key in the sort's order once; the byte loop then shifts by
       // constants (a descending sort counts 255 - digit)
       const U xk = key_xform(k[j], mode);
       const unsigned dmask = desc ? 0xFFu : 0u;
+      U dx = 0;
+      if constexpr (HM == 4) dx = xk ^ __shfl(xk, 0, 64);  // lane 0 is always valid here
 #pragma unroll
       for (int b = 0; b < int(sizeof(U)); b++) {
         if (b >= nbytes) break;
-        // lanes holding lane 0's digit, and then those holding the digit of
-        // the first other lane, are each counted with one atomic (skewed
-        // keys put most of a wave on one or two bins: same-address atomics
-        // from one wave serialise); the rest add one each
         const unsigned d = (unsigned(xk >> (8 * b)) & 0xFFu) ^ dmask;
-        unsigned d0 = __shfl(d, 0, 64);  // lane 0 is always valid here
-        unsigned long long same = __ballot(valid && d == d0);
-        if (two_leaders == 2) {
-          // one atomic for lane 0's digit when at least 8 lanes hold it (a
-          // byte constant over long runs, a heavy key), one per lane else
-          const unsigned c0 = unsigned(__popcll(same));
+        if constexpr (HM == 3) {
+          if (valid) atomicAdd(&h[b * 256 + d], 1u);
+        } else {
+          bool eq0;
+          if constexpr (HM == 4) eq0 = ((dx >> (8 * b)) & U(0xFF)) == U(0);
+          else eq0 = d == unsigned(__shfl(int(d), 0, 64));
+          const unsigned c0 = unsigned(__popcll(__ballot(valid && eq0)));
           if (c0 >= 8u) {
-            if (lane == 0) atomicAdd(&h[b * 256 + d0], c0);
-            if (valid && d != d0) atomicAdd(&h[b * 256 + d], 1u);
+            if (lane == 0) atomicAdd(&h[b * 256 + d], c0);
+            if (valid && !eq0) atomicAdd(&h[b * 256 + d], 1u);
           } else if (valid) {
             atomicAdd(&h[b * 256 + d], 1u);
           }
-          continue;
-        }
-        if (lane == 0) atomicAdd(&h[b * 256 + d0], unsigned(__popcll(same)));
-        const unsigned long long rest = vmask & ~same;
-        if (rest && (two_leaders || (same & (same - 1)))) {
-          const int l1 = __ffsll((long long)rest) - 1;
-          const unsigned d1 = __shfl(d, l1, 64);
-          const unsigned long long same1 = __ballot(valid && d == d1);
-          if (int(lane) == l1) atomicAdd(&h[b * 256 + d1], unsigned(__popcll(same1)));
-          if (valid && d != d0 && d != d1) atomicAdd(&h[b * 256 + d], 1u);
-        } else if (valid && d != d0) {
-          atomicAdd(&h[b * 256 + d], 1u);
         }
       }
     }
@@ -163,6 +152,16 @@ __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, siz
     unsigned v = h[i];
     if (v) atomicAdd(&ghist[i], (unsigned long long)v);
   }
+}
+
+template <class U>
+void launch_hist(int hm, int grid, const void* keys, size_t n, int mode, int desc, int nbytes,
+                 unsigned long long* ghist, hipStream_t st) {
+  const U* k = static_cast<const U*>(keys);
+  if (hm == 3) rs_hist<U, 3><<<grid, RS_NT, 0, st>>>(k, n, mode, desc, nbytes, ghist);
+  else if (hm == 2) rs_hist<U, 2><<<grid, RS_NT, 0, st>>>(k, n, mode, desc, nbytes, ghist);
+  else rs_hist<U, 4><<<grid, RS_NT, 0, st>>>(k, n, mode, desc, nbytes, ghist);
+  FHIP_CHECK(hipGetLastError());
 }
 
 // exclusive scan of each byte's histogram -> global digit bases
@@ -226,19 +225,17 @@ __global__ __launch_bounds__(NT) void rs_onesweep(
     size_t idx = wave_base + size_t(j) * 64 + lane;
     keys[j] = idx < n ? ksrc[idx] : U(0);
   }
-  if constexpr (VB != 0) {
-    const V* vsrc = static_cast<const V*>(vsrc_);
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-      size_t idx = wave_base + size_t(j) * 64 + lane;
-      vals[j] = idx < n ? vsrc[idx] : V(0);
-    }
-  }
+  // payloads: loaded once the look-back is done (their latency overlaps
+  // the key reorder and write), which keeps the registers of the ranking
+  // and look-back phases free of them
 
   // stable in-wave ranking: lanes with equal digits are matched with 8
-  // ballots; ranks count earlier items first, then lower lanes.
-  unsigned rank[ITEMS];
-  unsigned digit[ITEMS];
+  // ballots; ranks count earlier items first, then lower lanes.  Digit and
+  // rank share one register (digit in the low 8 bits), and the LDS
+  // positions below two to a register: the tile's rows, 32 keys and
+  // payloads per lane at 1024 x 16, then fit the registers without spills.
+  static_assert(TILE <= 65536 && ITEMS % 4 == 0, "16-bit tile positions, packed digits");
+  unsigned dr[ITEMS];
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
@@ -252,15 +249,15 @@ __global__ __launch_bounds__(NT) void rs_onesweep(
       unsigned long long bb = __ballot(bit);
       m &= bit ? bb : ~bb;
     }
-    digit[j] = d;
-    rank[j] = 0;
+    unsigned rank = 0;
     if (valid) {
       unsigned cnt = unsigned(__popcll(m));
       unsigned below = unsigned(__popcll(m & lt_mask));
       unsigned prior = s.wcnt[wave][d];
-      rank[j] = prior + below;
+      rank = prior + below;
       if (below == cnt - 1) s.wcnt[wave][d] = prior + cnt;
     }
+    dr[j] = d | (rank << 8);
   }
   __syncthreads();
 
@@ -316,28 +313,40 @@ __global__ __launch_bounds__(NT) void rs_onesweep(
   }
   __syncthreads();
 
+  if constexpr (VB != 0) {
+    const V* vsrc = static_cast<const V*>(vsrc_);
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      size_t idx = wave_base + size_t(j) * 64 + lane;
+      vals[j] = idx < n ? vsrc[idx] : V(0);
+    }
+  }
   // reorder keys by digit in LDS
   U* kbuf = reinterpret_cast<U*>(s.buf);
-  unsigned pos[ITEMS];
+  unsigned pos2[ITEMS / 2];  // tile positions, two 16-bit halves each
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
     size_t idx = wave_base + size_t(j) * 64 + lane;
-    pos[j] = s.tstart[digit[j]] + s.wcnt[wave][digit[j]] + rank[j];
-    if (idx < n) kbuf[pos[j]] = keys[j];
+    const unsigned d = dr[j] & 0xFFu;
+    const unsigned pos = s.tstart[d] + s.wcnt[wave][d] + (dr[j] >> 8);
+    if (j & 1) pos2[j / 2] |= pos << 16;
+    else pos2[j / 2] = pos;
+    if (idx < n) kbuf[pos] = keys[j];
   }
   __syncthreads();
   const unsigned tile_n = unsigned((n - tile_base) < size_t(TILE) ? (n - tile_base) : size_t(TILE));
-  unsigned long long dst[ITEMS];
+  unsigned dig4[ITEMS / 4];  // digits of the written keys, four to a register
 #pragma unroll
   for (int j = 0; j < ITEMS; j++) {
     unsigned i = unsigned(j) * NT + tid;
-    dst[j] = 0;
+    unsigned d = 0;
     if (i < tile_n) {
       U k = kbuf[i];
-      unsigned d = key_digit(k, mode, shift, desc);
-      dst[j] = s.gdst[d] + i;
-      kdst[dst[j]] = k;
+      d = key_digit(k, mode, shift, desc);
+      kdst[s.gdst[d] + i] = k;
     }
+    if (j % 4 == 0) dig4[j / 4] = d;
+    else dig4[j / 4] |= d << (8 * (j % 4));
   }
   if constexpr (VB != 0) {
     V* vbuf = reinterpret_cast<V*>(s.buf);
@@ -346,13 +355,13 @@ __global__ __launch_bounds__(NT) void rs_onesweep(
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
       size_t idx = wave_base + size_t(j) * 64 + lane;
-      if (idx < n) vbuf[pos[j]] = vals[j];
+      if (idx < n) vbuf[(pos2[j / 2] >> (16 * (j & 1))) & 0xFFFFu] = vals[j];
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
       unsigned i = unsigned(j) * NT + tid;
-      if (i < tile_n) vdst[dst[j]] = vbuf[i];
+      if (i < tile_n) vdst[s.gdst[(dig4[j / 4] >> (8 * (j % 4))) & 0xFFu] + i] = vbuf[i];
     }
   }
 }
@@ -402,7 +411,10 @@ struct TileShape {
 constexpr TileShape kTileShapes[] = {{256, 16}, {256, 32}, {512, 16}, {512, 24},
                                      {512, 12},  {1024, 8}, {1024, 16}, {768, 16}};
 // measured on MI355X: 512 x 16 is the fastest key-only shape at 2^24 and
-// 2^28 keys; with a payload, 1024 x 16 at 2^28 (512 x 16 within 2% at 2^24)
+// 2^28 keys; with an 8-byte payload at 2^28 the order of 1024 x 16 and
+// 512 x 16 differs from one MI355X to the next (2.12 against 2.21 ms per
+// pass on one, 2.01 against 1.93 on another), so the payload default stays
+// 1024 x 16 and the autotuner (radix.shape_kv) settles it per machine
 constexpr int kDefaultShape = 2;
 constexpr int kDefaultShapeKV = 6;
 constexpr int kNumShapes = int(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
@@ -467,17 +479,23 @@ size_t bytes_of_max(unsigned long long mx, int kbytes) {
 
 // Core driver: histogram, pass selection and the pass schedule.
 // Launch shape of the histogram pass: workgroups per CU (tunable
-// "radix.hist_bpc") and how a wave counts its digits ("radix.hist_two"):
-// 2 (default) one atomic for lane 0's digit when 8 or more lanes hold it,
-// one per lane otherwise; 1 leaders for lane 0's digit and for the first
-// other lane's, one per remaining lane; 0 the second leader only when lane
-// 0's digit was shared.  Measured on 2^28 64-bit keys: the histogram takes
-// 1.45 ms with 2 against 2.14 ms with 1 (uniform keys) and 1.42 against
-// 1.81 ms (Zipf keys).
-int hist_grid(size_t n, int* two) {
-  const int hbpc = int(std::max(1LL, std::min(64LL, tune_get("radix.hist_bpc", 64))));
-  *two = int(tune_get("radix.hist_two", 2));
-  return int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), size_t(fill_blocks(RS_NT, hbpc))));
+// "radix.hist_bpc") and how a wave matches lane 0's digit ("radix.hist_mode",
+// rs_hist's HM: 4 by default, 2 or 3).
+// Without the tunable, a workgroup counts about 16384 keys and at least 8
+// workgroups per CU run (each ends with one global atomic per bin): at 2^28
+// 64-bit keys 16384 workgroups took 0.73 ms against 0.96 for 2048; at 2^24,
+// 2048 took 0.10 ms against 0.21 for 8192.
+int hist_grid(size_t n, int* hm) {
+  *hm = int(tune_get("radix.hist_mode", 4));
+  const long long bpc = tune_get("radix.hist_bpc", -1);
+  size_t cap;
+  if (bpc > 0) {
+    cap = size_t(fill_blocks(RS_NT, int(std::min(64LL, bpc))));
+  } else {
+    cap = std::min<size_t>(size_t(fill_blocks(RS_NT, 64)),
+                           std::max<size_t>(size_t(fill_blocks(RS_NT, 8)), n / 16384));
+  }
+  return int(std::min<size_t>(ceil_div(n, RS_NT * RH_ITEMS), cap));
 }
 
 // known >= 0: bytes [0, known) are the passes (the caller knows the higher
@@ -499,15 +517,12 @@ void run_lsd(int kbytes, int mode, int desc, int nbytes, const void* kin, void* 
   }
   unsigned long long* gbase = ghist + 8 * 256;
   FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
-  int two = 1;
-  const int grid = hist_grid(n, &two);
+  int hm = 4;
+  const int grid = hist_grid(n, &hm);
   if (nbytes > 0) {
     ProfScope prof("radix_hist", st);
-    if (kbytes == 4)
-      rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, mode, desc, nbytes, ghist, two);
-    else
-      rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, mode, desc, nbytes, ghist, two);
-    FHIP_CHECK(hipGetLastError());
+    if (kbytes == 4) launch_hist<uint32_t>(hm, grid, kin, n, mode, desc, nbytes, ghist, st);
+    else launch_hist<uint64_t>(hm, grid, kin, n, mode, desc, nbytes, ghist, st);
     rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
     FHIP_CHECK(hipGetLastError());
   }
@@ -670,13 +685,10 @@ void sort_pairs_async(int kbytes, int nbytes, const void* kin, void* kout, int v
   zero_u64_kernel<<<grid_for_n(L.lb_words), 256, 0, st>>>(lb0, L.lb_words);
   zero_u64_kernel<<<1, 64, 0, st>>>(reinterpret_cast<unsigned long long*>(ctr), 1);
   FHIP_CHECK(hipGetLastError());
-  int two = 1;
-  const int grid = hist_grid(n, &two);
-  if (kbytes == 4)
-    rs_hist<uint32_t><<<grid, RS_NT, 0, st>>>((const uint32_t*)kin, n, KM_RAW, 0, nbytes, ghist, two);
-  else
-    rs_hist<uint64_t><<<grid, RS_NT, 0, st>>>((const uint64_t*)kin, n, KM_RAW, 0, nbytes, ghist, two);
-  FHIP_CHECK(hipGetLastError());
+  int hm = 4;
+  const int grid = hist_grid(n, &hm);
+  if (kbytes == 4) launch_hist<uint32_t>(hm, grid, kin, n, KM_RAW, 0, nbytes, ghist, st);
+  else launch_hist<uint64_t>(hm, grid, kin, n, KM_RAW, 0, nbytes, ghist, st);
   rs_scan<<<nbytes, 256, 0, st>>>(ghist, gbase);
   FHIP_CHECK(hipGetLastError());
   PassFn fn = pick_pass(kDefaultShape, kbytes, vbytes);  // the workspace's tile count

@@ -91,11 +91,12 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *       fhip_dev_multi_group_by (the column-by-column sorts run instead);
  *   "join.load_pct"     load factor (10 to 50 %, default 50) of the joins' hash table for
  *       integer keys;
- *   "radix.hist_bpc", "radix.hist_two"  workgroups per CU of the radix
- *       histogram pass (1 to 64), and how a wave counts its digits: 2
- *       (default) one atomic for lane 0's digit when at least 8 lanes hold
- *       it, else one per lane; 1 leaders for two digits; 0 the second
- *       leader only when lane 0's digit was shared.
+ *   "radix.hist_bpc", "radix.hist_mode"  workgroups per CU of the radix
+ *       histogram pass (1 to 64; by default about 16384 keys per workgroup,
+ *       8 to 64 workgroups per CU), and how a wave counts its digits: one
+ *       atomic for the lanes holding lane 0's digit when at least 8 do, else
+ *       one per lane, the match found from lane 0's whole key (4, default)
+ *       or its digit per byte (2); 3 one atomic per lane, no match.
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);
