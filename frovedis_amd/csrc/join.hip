@@ -378,39 +378,197 @@ __global__ __launch_bounds__(JT_NT) void jt_probe_kernel(const K* __restrict__ q
   }
 }
 
+// The probe fused with the compaction of its hits (inner joins): a
+// workgroup takes the next tile of JP_NT * JP_ITEMS queries in arrival
+// order, looks every query up (the ITEMS slot reads of a thread in flight
+// together), counts its hits, finds the hits of all earlier tiles by
+// decoupled look-back and writes its pairs (left row id, looked-up value)
+// at once, in query order.  No per-query value or flag array is written and
+// none is read back.  Pairs at positions >= capacity are counted, not
+// written; tile_state (tiles words), ticket and total start at zero.
+constexpr int JP_NT = 1024;
+constexpr int JP_ITEMS = 8;
+
+template <class K>
+__global__ __launch_bounds__(JP_NT) void jt_probe_compact_kernel(
+    const K* __restrict__ q, size_t m, const unsigned long long* __restrict__ slots,
+    unsigned long long mask, int shift, const unsigned long long* __restrict__ side,
+    const size_t* __restrict__ lidx, size_t* __restrict__ lo, size_t* __restrict__ ro,
+    size_t capacity, unsigned long long* tile_state, unsigned* ticket,
+    unsigned long long* total) {
+  constexpr int WAVES = JP_NT / 64;
+  constexpr int COUNTS = JP_ITEMS * WAVES;  // (round, wave) counts, round-major
+  constexpr int PER_LANE = COUNTS / 64;
+  static_assert(COUNTS % 64 == 0, "the counts split evenly over one wave");
+  __shared__ unsigned s_cnt[COUNTS];
+  __shared__ unsigned s_tile;
+  __shared__ unsigned long long s_excl;
+  const unsigned lane = __lane_id();
+  const unsigned wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const unsigned tile = s_tile;
+  const size_t base = size_t(tile) * (JP_NT * JP_ITEMS);
+  unsigned long long k[JP_ITEMS], v[JP_ITEMS];
+  bool in[JP_ITEMS];
+#pragma unroll
+  for (int j = 0; j < JP_ITEMS; j++) {
+    const size_t i = base + size_t(j) * JP_NT + threadIdx.x;
+    in[j] = i < m;
+    k[j] = in[j] ? jt_bits(q[i]) : 0ull;
+  }
+  // home slots of every query read together
+  ulonglong2 e[JP_ITEMS];
+#pragma unroll
+  for (int j = 0; j < JP_ITEMS; j++)
+    e[j] = (in[j] && k[j] != JT_EMPTY)
+               ? *reinterpret_cast<const ulonglong2*>(slots + 2 * jt_home(k[j], shift))
+               : make_ulonglong2(JT_EMPTY, 0ull);
+  unsigned long long hit[JP_ITEMS];
+#pragma unroll
+  for (int j = 0; j < JP_ITEMS; j++) {
+    bool f = false;
+    v[j] = 0;
+    if (in[j]) {
+      if (k[j] == JT_EMPTY) {
+        f = side[0] != 0;
+        v[j] = f ? side[1] : 0ull;
+      } else if (e[j].x == k[j]) {
+        f = true;
+        v[j] = e[j].y;
+      } else if (e[j].x != JT_EMPTY) {  // a collision: walk on from the next slot
+        const ulonglong2 r = jt_probe_rest(slots, mask, k[j], jt_home(k[j], shift));
+        f = r.x != 0;
+        v[j] = r.y;
+      }
+    }
+    hit[j] = __ballot(f);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < JP_ITEMS; j++) s_cnt[j * WAVES + wave] = unsigned(__popcll(hit[j]));
+  }
+  __syncthreads();
+  if (wave == 0) {
+    unsigned c[PER_LANE];
+    unsigned sum = 0;
+#pragma unroll
+    for (int t = 0; t < PER_LANE; t++) {
+      c[t] = s_cnt[lane * PER_LANE + t];
+      sum += c[t];
+    }
+    const unsigned incl = wave_incl_scan(sum);
+    const unsigned run = __shfl(incl, 63, 64);
+    unsigned ex = incl - sum;
+#pragma unroll
+    for (int t = 0; t < PER_LANE; t++) {
+      s_cnt[lane * PER_LANE + t] = ex;
+      ex += c[t];
+    }
+    const unsigned long long excl = compact_lookback(tile_state, tile, run);
+    if (lane == 0) {
+      if (tile == gridDim.x - 1) *total = excl + run;
+      s_excl = excl;
+    }
+  }
+  __syncthreads();
+  const unsigned long long excl = s_excl;
+#pragma unroll
+  for (int j = 0; j < JP_ITEMS; j++) {
+    if ((hit[j] >> lane) & 1ull) {
+      const size_t pos = size_t(excl + s_cnt[j * WAVES + wave] + lane_rank(hit[j]));
+      if (pos < capacity) {
+        lo[pos] = lidx[base + size_t(j) * JP_NT + threadIdx.x];
+        ro[pos] = v[j];
+      }
+    }
+  }
+}
+
+// The joins' table over (keys, vals) for integer keys.
+struct JoinTable {
+  DBuf tab, side;
+  unsigned long long nslots = 0;
+  int bits = 0;
+  bool unique = true;
+};
+
+// Builds the table; unique is false when the keys repeat (the table is
+// then not usable).
+template <class K>
+void jt_build(JoinTable& t, const K* keys, const unsigned long long* vals, size_t n,
+              hipStream_t st) {
+  const long long load_pct = std::max(10LL, std::min(50LL, tune_get("join.load_pct", 50)));
+  int bits = 10;
+  while ((size_t(1) << bits) * size_t(load_pct) < 100 * n) bits++;
+  t.bits = bits;
+  t.nslots = 1ull << bits;
+  t.tab.reset(size_t(t.nslots) * 16);
+  t.side.reset(3 * sizeof(unsigned long long));
+  FHIP_CHECK(hipMemsetAsync(t.tab.get(), 0xFF, size_t(t.nslots) * 16, st));
+  FHIP_CHECK(hipMemsetAsync(t.side.get(), 0, 3 * sizeof(unsigned long long), st));
+  {
+    ProfScope prof("join_build", st);
+    if (n)
+      jt_insert_kernel<K><<<grid_for_n(n), JT_NT, 0, st>>>(
+          keys, vals, n, t.tab.as<unsigned long long>(), t.nslots - 1, 64 - bits,
+          t.side.as<unsigned long long>());
+    FHIP_CHECK(hipGetLastError());
+  }
+  unsigned long long dup = 0;
+  FHIP_CHECK(hipMemcpyAsync(&dup, t.side.as<unsigned long long>() + 2, sizeof(dup),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  t.unique = dup == 0;
+}
+
+// Probes q: vals_out / found per query.
+template <class K>
+void jt_probe(const JoinTable& t, const K* q, size_t m, unsigned long long* vals_out,
+              uint8_t* found, hipStream_t st) {
+  if (!m) return;
+  ProfScope prof("join_probe", st);
+  jt_probe_kernel<K><<<grid_for_n(m), JT_NT, 0, st>>>(
+      q, m, t.tab.as<unsigned long long>(), t.nslots - 1, 64 - t.bits,
+      t.side.as<unsigned long long>(), vals_out, found);
+  FHIP_CHECK(hipGetLastError());
+}
+
+// Probes q and writes the pairs (lidx of the query, looked-up value) of its
+// hits in query order (at most capacity of them); returns the hit count
+// (synchronises the stream).
+template <class K>
+size_t jt_probe_pairs(const JoinTable& t, const K* q, const size_t* lidx, size_t m, size_t* lo,
+                      size_t* ro, size_t capacity, hipStream_t st) {
+  if (!m) return 0;
+  const size_t tiles = ceil_div(m, size_t(JP_NT) * JP_ITEMS);
+  FHIP_REQUIRE(tiles < (size_t(1) << 31), FHIP_ERR_TOO_LARGE, "equi_join: too many probe rows");
+  DBuf ws((tiles + 2) * sizeof(unsigned long long));
+  unsigned long long* state = ws.as<unsigned long long>();
+  FHIP_CHECK(hipMemsetAsync(state, 0, (tiles + 2) * sizeof(unsigned long long), st));
+  {
+    ProfScope prof("join_probe", st);
+    jt_probe_compact_kernel<K><<<unsigned(tiles), JP_NT, 0, st>>>(
+        q, m, t.tab.as<unsigned long long>(), t.nslots - 1, 64 - t.bits,
+        t.side.as<unsigned long long>(), lidx, lo, ro, capacity, state,
+        reinterpret_cast<unsigned*>(state + tiles), state + tiles + 1);
+    FHIP_CHECK(hipGetLastError());
+  }
+  unsigned long long total = 0;
+  FHIP_CHECK(hipMemcpyAsync(&total, state + tiles + 1, sizeof(total), hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  return size_t(total);
+}
+
 // Builds the table over (keys, vals) and probes q with it; returns false
 // (without probing) when the keys repeat.
 template <class K>
 bool jt_build_probe(const K* keys, const unsigned long long* vals, size_t n, const K* q, size_t m,
                     unsigned long long* vals_out, uint8_t* found, hipStream_t st) {
-  const long long load_pct = std::max(10LL, std::min(50LL, tune_get("join.load_pct", 50)));
-  int bits = 10;
-  while ((size_t(1) << bits) * size_t(load_pct) < 100 * n) bits++;
-  const unsigned long long nslots = 1ull << bits;
-  DBuf tab(size_t(nslots) * 16), side(3 * sizeof(unsigned long long));
-  FHIP_CHECK(hipMemsetAsync(tab.get(), 0xFF, size_t(nslots) * 16, st));
-  FHIP_CHECK(hipMemsetAsync(side.get(), 0, 3 * sizeof(unsigned long long), st));
-  {
-    ProfScope prof("join_build", st);
-    if (n)
-      jt_insert_kernel<K><<<grid_for_n(n), JT_NT, 0, st>>>(
-          keys, vals, n, tab.as<unsigned long long>(), nslots - 1, 64 - bits,
-          side.as<unsigned long long>());
-    FHIP_CHECK(hipGetLastError());
-  }
-  unsigned long long dup = 0;
-  FHIP_CHECK(hipMemcpyAsync(&dup, side.as<unsigned long long>() + 2, sizeof(dup),
-                            hipMemcpyDeviceToHost, st));
-  FHIP_CHECK(hipStreamSynchronize(st));
-  if (dup) return false;
-  if (m) {
-    ProfScope prof("join_probe", st);
-    jt_probe_kernel<K><<<grid_for_n(m), JT_NT, 0, st>>>(q, m, tab.as<unsigned long long>(),
-                                                        nslots - 1, 64 - bits,
-                                                        side.as<unsigned long long>(), vals_out,
-                                                        found);
-    FHIP_CHECK(hipGetLastError());
-  }
+  JoinTable t;
+  jt_build<K>(t, keys, vals, n, st);
+  if (!t.unique) return false;
+  jt_probe<K>(t, q, m, vals_out, found, st);
   return true;
 }
 
@@ -487,11 +645,18 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
     dispatch_type(ktype, [&](auto z) {
       using K = decltype(z);
       if constexpr (std::is_integral<K>::value) {
-        DBuf vals(nl * sizeof(uint64_t)), found(nl);
-        uniq = jt_build_probe<K>((const K*)right, reinterpret_cast<const unsigned long long*>(ridx),
-                                 nr, (const K*)left, nl, vals.as<unsigned long long>(),
-                                 found.as<uint8_t>(), st);
-        if (uniq) r = emit(vals, found);
+        JoinTable t;
+        jt_build<K>(t, (const K*)right, reinterpret_cast<const unsigned long long*>(ridx), nr, st);
+        uniq = t.unique;
+        if (uniq && !only_left) {
+          // inner join: the probe writes the pairs itself, in left order
+          r = jt_probe_pairs<K>(t, (const K*)left, lidx, nl, lo, ro, capacity, st);
+        } else if (uniq) {
+          DBuf vals(nl * sizeof(uint64_t)), found(nl);
+          jt_probe<K>(t, (const K*)left, nl, vals.as<unsigned long long>(), found.as<uint8_t>(),
+                      st);
+          r = emit(vals, found);
+        }
         done = true;
       }
     });
