@@ -39,7 +39,10 @@ def lib():
             raise FhipError(
                 f"{LIB_PATH} is not built; run `python -m frovedis_amd.build`")
         L = ctypes.CDLL(LIB_PATH)
-        L.fhip_last_error.restype = ctypes.c_char_p
+        # every entry point gets its prototype from include/frovedis_hip.h:
+        # sizes and pointers then cross as 64-bit values, never as C ints
+        from . import abi
+        abi.apply(L)
         _lib = L
     return _lib
 

@@ -22,7 +22,7 @@ namespace fhip {
 size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, const void* right,
                  const size_t* ridx, size_t nr, int strategy, size_t* lo, size_t* ro,
                  size_t capacity, int* used_hash, size_t* only_left, size_t* n_only,
-                 hipStream_t st);
+                 hipStream_t st, bool inplace);
 namespace {
 
 // the host entry points run on the default stream, ordered after the
@@ -94,24 +94,58 @@ int fhip_ht_all_keys(const fhip_ht* ht, void* out, size_t* out_n) {
   });
 }
 
+}  // extern "C"
+
+namespace fhip {
+namespace {
+// The host forms of equi_join: sorted_back non-null (the in-place form)
+// writes the sorted inputs back when the sort-merge path ran.
+int host_equi_join(int ktype, const void* left, const size_t* left_idx, size_t nl,
+                   const void* right, const size_t* right_idx, size_t nr, int strategy,
+                   size_t* left_out, size_t* right_out, size_t capacity, size_t* nout,
+                   int* used_hash, void* const* sorted_back) {
+  return guarded([&] {
+    FHIP_REQUIRE(nout, FHIP_ERR_ARG, "equi_join: null nout");
+    const int kb = type_bytes(ktype);
+    DBuf dl = to_dev(left, nl * kb), dli = to_dev(left_idx, nl * 8);
+    DBuf dr = to_dev(right, nr * kb), dri = to_dev(right_idx, nr * 8);
+    DBuf dlo(capacity * 8), dro(capacity * 8);
+    int uh = 0;
+    size_t r = equi_join(ktype, dl.get(), dli.as<size_t>(), nl, dr.get(), dri.as<size_t>(), nr,
+                         strategy, dlo.as<size_t>(), dro.as<size_t>(), capacity, &uh, nullptr,
+                         nullptr, nullptr, sorted_back != nullptr);
+    if (used_hash) *used_hash = uh;
+    *nout = r;
+    FHIP_REQUIRE(r <= capacity, FHIP_ERR_TOO_LARGE, "equi_join: output capacity too small");
+    to_host(left_out, dlo.get(), r * 8);
+    to_host(right_out, dro.get(), r * 8);
+    if (sorted_back && !uh) {
+      to_host(sorted_back[0], dl.get(), nl * kb);
+      to_host(sorted_back[1], dli.get(), nl * 8);
+      to_host(sorted_back[2], dr.get(), nr * kb);
+      to_host(sorted_back[3], dri.get(), nr * 8);
+    }
+  });
+}
+}  // namespace
+}  // namespace fhip
+
+extern "C" {
+
 int fhip_equi_join(int ktype, const void* left, const size_t* left_idx, size_t nl,
                    const void* right, const size_t* right_idx, size_t nr, int strategy,
                    size_t* left_out, size_t* right_out, size_t capacity, size_t* nout,
                    int* used_hash) {
-  return fhip::guarded([&] {
-    FHIP_REQUIRE(nout, FHIP_ERR_ARG, "equi_join: null nout");
-    const int kb = fhip::type_bytes(ktype);
-    fhip::DBuf dl = fhip::to_dev(left, nl * kb), dli = fhip::to_dev(left_idx, nl * 8);
-    fhip::DBuf dr = fhip::to_dev(right, nr * kb), dri = fhip::to_dev(right_idx, nr * 8);
-    fhip::DBuf dlo(capacity * 8), dro(capacity * 8);
-    size_t r = fhip::equi_join(ktype, dl.get(), dli.as<size_t>(), nl, dr.get(), dri.as<size_t>(),
-                               nr, strategy, dlo.as<size_t>(), dro.as<size_t>(), capacity,
-                               used_hash, nullptr, nullptr, nullptr);
-    *nout = r;
-    FHIP_REQUIRE(r <= capacity, FHIP_ERR_TOO_LARGE, "equi_join: output capacity too small");
-    fhip::to_host(left_out, dlo.get(), r * 8);
-    fhip::to_host(right_out, dro.get(), r * 8);
-  });
+  return fhip::host_equi_join(ktype, left, left_idx, nl, right, right_idx, nr, strategy, left_out,
+                              right_out, capacity, nout, used_hash, nullptr);
+}
+
+int fhip_equi_join_inplace(int ktype, void* left, size_t* left_idx, size_t nl, void* right,
+                           size_t* right_idx, size_t nr, int strategy, size_t* left_out,
+                           size_t* right_out, size_t capacity, size_t* nout, int* used_hash) {
+  void* back[4] = {left, left_idx, right, right_idx};
+  return fhip::host_equi_join(ktype, left, left_idx, nl, right, right_idx, nr, strategy, left_out,
+                              right_out, capacity, nout, used_hash, back);
 }
 
 int fhip_group_by(int ktype, const void* col, size_t col_n, size_t* idx, size_t n, size_t* split,

@@ -187,19 +187,37 @@ bool sample_has_duplicate(int ktype, const K* right, size_t nr, hipStream_t st) 
 // only_left (may be null): receives the left row ids without a match in
 // stable key order of LEFT (outer_equi_join, join.hpp:352-378), *n_only
 // their count.
+// inplace: the inputs themselves are sorted (with their indices), as the
+// VE body leaves them (join.hpp:320-322); else sorted copies are used.
 template <class K>
 size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, const K* right,
                     const size_t* ridx, size_t nr, size_t* lo, size_t* ro, size_t capacity,
-                    size_t* only_left, size_t* n_only, hipStream_t st) {
+                    size_t* only_left, size_t* n_only, hipStream_t st, bool inplace = false) {
   if (n_only) *n_only = 0;
+  // radix_sort(right, right_idx); radix_sort(left, left_idx) (join.hpp:320-322)
+  if (inplace) {
+    if (nr) sort_pairs(ktype, right, const_cast<K*>(right), 8, ridx, const_cast<size_t*>(ridx),
+                       nr, false, false, st);
+    if (nl) sort_pairs(ktype, left, const_cast<K*>(left), 8, lidx, const_cast<size_t*>(lidx), nl,
+                       false, false, st);
+  }
   if (nl == 0) return 0;
   if (nr == 0 && !only_left) return 0;
-  DBuf lk(nl * sizeof(K)), li(nl * sizeof(size_t)), rk(nr * sizeof(K)), ri(nr * sizeof(size_t));
-  // radix_sort(right, right_idx); radix_sort(left, left_idx) (join.hpp:320-322)
-  if (nr) sort_pairs(ktype, right, rk.get(), 8, ridx, ri.get(), nr, false, false, st);
-  sort_pairs(ktype, left, lk.get(), 8, lidx, li.get(), nl, false, false, st);
+  DBuf lk, li, rk, ri;
+  if (!inplace) {
+    lk.reset(nl * sizeof(K));
+    li.reset(nl * sizeof(size_t));
+    rk.reset(nr * sizeof(K));
+    ri.reset(nr * sizeof(size_t));
+    if (nr) sort_pairs(ktype, right, rk.get(), 8, ridx, ri.get(), nr, false, false, st);
+    sort_pairs(ktype, left, lk.get(), 8, lidx, li.get(), nl, false, false, st);
+  }
+  const K* lks = inplace ? left : lk.as<K>();
+  const size_t* lis = inplace ? lidx : li.as<size_t>();
+  const K* rks = inplace ? right : rk.as<K>();
+  const size_t* ris = inplace ? ridx : ri.as<size_t>();
   if (nr == 0) {
-    FHIP_CHECK(hipMemcpyAsync(only_left, li.get(), nl * sizeof(size_t), hipMemcpyDeviceToDevice,
+    FHIP_CHECK(hipMemcpyAsync(only_left, lis, nl * sizeof(size_t), hipMemcpyDeviceToDevice,
                               st));
     FHIP_CHECK(hipStreamSynchronize(st));
     *n_only = nl;
@@ -210,7 +228,7 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
   {
     ProfScope prof("join_merge", st);
     match_count_kernel<K><<<unsigned(ceil_div(nl, MC_NT)), MC_NT, 0, st>>>(
-        lk.as<K>(), nl, rk.as<K>(), nr, cnt.as<unsigned long long>(),
+        lks, nl, rks, nr, cnt.as<unsigned long long>(),
         first.as<unsigned long long>());
     FHIP_CHECK(hipGetLastError());
   }
@@ -223,11 +241,11 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
     ProfScope prof("join_write", st);
     pair_write_kernel<<<unsigned(ceil_div(nl, PW_NT)), PW_NT, 0, st>>>(
         incl.as<unsigned long long>(), cnt.as<unsigned long long>(),
-        first.as<unsigned long long>(), li.as<size_t>(), ri.as<size_t>(), nl, lo, ro);
+        first.as<unsigned long long>(), lis, ris, nl, lo, ro);
     FHIP_CHECK(hipGetLastError());
   }
   if (only_left)
-    *n_only = compact(NoMatchPred{cnt.as<unsigned long long>()}, RowIdMap{li.as<size_t>()}, nl,
+    *n_only = compact(NoMatchPred{cnt.as<unsigned long long>()}, RowIdMap{lis}, nl,
                       only_left, st);
   return size_t(total);
 }
@@ -606,10 +624,12 @@ struct StarWriter {
 // Inner join, or LEFT outer join when only_left is given (outer_equi_join,
 // join.hpp:328-379): then only_left (nl entries) receives the left row ids
 // without a match and *n_only their count.
+// inplace: on the sort-merge path the (device) inputs are sorted in place
+// with their indices, as the reference's VE body leaves them.
 size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, const void* right,
                  const size_t* ridx, size_t nr, int strategy, size_t* lo, size_t* ro,
                  size_t capacity, int* used_hash, size_t* only_left, size_t* n_only,
-                 hipStream_t st) {
+                 hipStream_t st, bool inplace = false) {
   if (n_only) *n_only = 0;
   FHIP_REQUIRE(strategy >= FHIP_JOIN_AUTO && strategy <= FHIP_JOIN_AUTO_UNIQUE, FHIP_ERR_ARG,
                "equi_join: bad strategy");
@@ -687,7 +707,7 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
   dispatch_type(ktype, [&](auto z) {
     using K = decltype(z);
     r = sort_merge_t<K>(ktype, (const K*)left, lidx, nl, (const K*)right, ridx, nr, lo, ro,
-                        capacity, only_left, n_only, st);
+                        capacity, only_left, n_only, st, inplace);
   });
   return r;
 }

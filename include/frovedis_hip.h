@@ -382,6 +382,15 @@ int fhip_equi_join(int ktype, const void* left, const size_t* left_idx,
                    size_t nl, const void* right, const size_t* right_idx,
                    size_t nr, int strategy, size_t* left_out, size_t* right_out,
                    size_t capacity, size_t* nout, int* used_hash);
+/* fhip_equi_join whose inputs are left as the reference's VE equi_join
+ * leaves them (dataframe/join.hpp:304-326): when the sort-merge path runs
+ * (*used_hash == 0), left / left_idx and right / right_idx come back
+ * stable-sorted by key (join.hpp:320-322), from the device copies the join
+ * sorted, with no second sort; the hash path leaves them unchanged. */
+int fhip_equi_join_inplace(int ktype, void* left, size_t* left_idx, size_t nl,
+                           void* right, size_t* right_idx, size_t nr,
+                           int strategy, size_t* left_out, size_t* right_out,
+                           size_t capacity, size_t* nout, int* used_hash);
 int fhip_group_by(int ktype, const void* col, size_t col_n, size_t* idx,
                   size_t n, size_t* split, size_t* nsplit, void* keys_out);
 int fhip_grouped_agg(int op, int vtype, const void* col, size_t col_n,

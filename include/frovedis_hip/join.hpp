@@ -30,9 +30,13 @@ void equi_join(std::vector<T>& left, std::vector<size_t>& left_idx, std::vector<
     right_idx_out.resize(cap);
     size_t nout = 0;
     int used_hash = 0;
-    int rc = fhip_equi_join(fhip_detail::key_code<T>(), left.data(), left_idx.data(), left.size(),
-                            right.data(), right_idx.data(), right.size(), FHIP_JOIN_AUTO,
-                            left_idx_out.data(), right_idx_out.data(), cap, &nout, &used_hash);
+    // the sort-merge path sorts the inputs in place, as the VE body does;
+    // they come back with the pairs (nothing comes back when the pairs did
+    // not fit, so the retry starts from the inputs as given)
+    int rc = fhip_equi_join_inplace(fhip_detail::key_code<T>(), left.data(), left_idx.data(),
+                                    left.size(), right.data(), right_idx.data(), right.size(),
+                                    FHIP_JOIN_AUTO, left_idx_out.data(), right_idx_out.data(),
+                                    cap, &nout, &used_hash);
     if (rc == FHIP_ERR_TOO_LARGE && nout > cap) {
       cap = nout;
       continue;
@@ -40,10 +44,6 @@ void equi_join(std::vector<T>& left, std::vector<size_t>& left_idx, std::vector<
     fhip_detail::check(rc);
     left_idx_out.resize(nout);
     right_idx_out.resize(nout);
-    if (!used_hash) {  // the VE sort-merge path sorts its inputs in place
-      radix_sort(right, right_idx);
-      radix_sort(left, left_idx);
-    }
     return;
   }
   throw std::runtime_error("equi_join: output did not fit");

@@ -74,6 +74,12 @@ int main() {
   std::vector<size_t> li = {0, 1, 2, 3, 4, 5, 6, 7}, ri = {0, 1, 2, 3}, lo, ro;
   frovedis::equi_join(t1c3, li, t2c1, ri, lo, ro);
   EXPECT(lo.size() == 10);
+  // the right keys repeat: the sort-merge path left both inputs stable-sorted
+  // with their indices (join.hpp:320-322)
+  EXPECT((t1c3 == std::vector<int>{1, 1, 2, 2, 2, 3, 3, 4}));
+  EXPECT((li == std::vector<size_t>{2, 7, 1, 3, 5, 0, 4, 6}));
+  EXPECT((t2c1 == std::vector<int>{1, 2, 2, 3}));
+  EXPECT((ri == std::vector<size_t>{0, 1, 3, 2}));
   // tutorial group_by (tutorial.md:2500-2531)
   std::vector<int> c1 = {1, 3, 3, 2, 2, 2, 1, 1}, c3 = {30, 20, 30, 20, 30, 40, 40, 50};
   std::vector<size_t> idx(8), split, nulls;

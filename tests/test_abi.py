@@ -64,3 +64,25 @@ def test_cpp_dropin_headers_compile_as_cxx11(libpath, tmp_path):
     # the reference builds with -std=c++11 (Makefile.in.x86:3-4)
     _build_shim(tmp_path / "shim_test")
     assert (tmp_path / "shim_test").exists()
+
+
+def test_prototypes_cover_every_declared_entry_point():
+    # each declaration maps to ctypes types (sizes as size_t, pointers as
+    # 64-bit), so no Python integer reaches the library as a 32-bit int
+    from frovedis_amd import abi
+    protos = abi.prototypes()
+    assert sorted(protos) == _declared()
+    res, args = protos["fhip_dev_radix_sort"]
+    assert res is ctypes.c_int
+    assert args == [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                    ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    assert protos["fhip_last_error"] == (ctypes.c_char_p, [])
+
+
+def test_loaded_library_rejects_a_wrongly_typed_argument(libpath):
+    import frovedis_amd
+    L = frovedis_amd.lib()
+    with pytest.raises(ctypes.ArgumentError):
+        L.fhip_dev_radix_sort(3, 1.5, 0, None, 0, 0, 0, None)
+    with pytest.raises(TypeError):  # too few arguments
+        L.fhip_dev_radix_sort(3, None, 0)
