@@ -5,12 +5,17 @@
 // 153-164; the outputs here are those of that per-segment loop.
 //
 // Segment s holds rows [seg_off[s], seg_off[s+1]) with keys in
-// [0, 2^key_bits).  Every row gets the composite key (s << key_bits) | key,
-// one stable LSD sort orders all segments at once (segment-major, then key,
-// ties in input order), and one thread per group folds its run of values in
-// order -- the same additions in the same order as the reference, so the
-// sums are bit-exact.  The whole sequence -- composite keys, histogram,
-// passes, run detection, fold -- is stream-ordered with no host
+// [0, 2^key_bits).  Two paths give the same outputs:
+//   - the work queue (the default, below): small segments and key-range
+//     buckets of large ones are jobs of one persistent launch, each sorted
+//     and folded in LDS;
+//   - the composite key (the fallback when a bucket overflows its job, or
+//     "batch.path" 1): every row gets the key (s << key_bits) | key, one
+//     stable LSD sort orders all segments at once (segment-major, then key,
+//     ties in input order), and one thread per group folds its run.
+// Either way each group's values are added one by one in row order -- the
+// same additions in the same order as the reference, so the sums are
+// bit-exact.  Each sequence is stream-ordered with no host
 // synchronisation, memset or allocation, and is captured once per shape in
 // a HIP graph that later calls replay with a single hipGraphLaunch.
 #include "fhip_device.hpp"

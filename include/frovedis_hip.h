@@ -118,10 +118,12 @@ int fhip_dev_radix_sort(int ktype, void* keys, int vbytes, void* vals,
                         size_t n, int positive_only, int descending,
                         void* stream);
 /* No counterpart in the reference: fhip_dev_radix_sort for callers that
- * cannot allocate device memory (code under HIP graph capture, tight
- * serving loops).  Same arguments and results; it calls no hipMalloc /
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results; it calls no hipMalloc /
  * hipFree: its temporaries live in a file-scope 64 MiB device array, one
- * per GPU, kept until the process ends.  Inputs whose temporaries do not
+ * per GPU, kept until the process ends.  Like fhip_dev_radix_sort it reads
+ * the digit histogram back once (a wait on the stream) to skip the bytes
+ * every key shares, so it is not meant for graph capture.  Inputs whose temporaries do not
  * fit (more than about 4M 8+8-byte pairs, 8M 8-byte keys) return
  * FHIP_ERR_TOO_LARGE.  Every call on a GPU shares that array: calls that
  * overlap on one GPU (two streams at once, two threads) race. */
