@@ -260,4 +260,20 @@ long long tune_get(const char* name, long long dflt);
 // number of workgroups that fill the device for grid-stride kernels
 int fill_blocks(int threads_per_block, int blocks_per_cu = 8);
 
+// The heavy keys of a column known in advance (from a model of the column):
+// key bits (sign-extended for signed keys) and their counts, the heaviest
+// first, counted over `total` rows.
+struct HeavyKeys {
+  const unsigned long long* keys = nullptr;
+  const unsigned long long* counts = nullptr;
+  size_t n = 0;
+  size_t total = 0;
+};
+
+// The fused hash group-by + aggregation (hashagg.hip); heavy (optional)
+// replaces the sample the heavy-key table is planned from.
+void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
+                  void* keys_out, size_t capacity, size_t* ngroups, size_t table_hint,
+                  hipStream_t st, const HeavyKeys* heavy = nullptr);
+
 }  // namespace fhip

@@ -1139,19 +1139,78 @@ struct HotPlan {
   std::vector<unsigned long long> table;
 };
 
+// Two-choice placement of the candidates `runs` ((count, key bits), the
+// heaviest first) into a table of Sh slots: at most 80 % of the slots are
+// filled, a key goes to a free slot of either bucket, or displaces a key of
+// the bucket it was not just evicted from, which then moves on; a key still
+// unplaced after the displacement limit is left out (its rows take the
+// partitioned path).
+HotPlan place_hot(std::vector<std::pair<size_t, unsigned long long>>& runs, int Sh) {
+  HotPlan h;
+  if (runs.size() > size_t(Sh) * 4 / 5) runs.resize(size_t(Sh) * 4 / 5);
+  h.Sh = Sh;
+  h.table.assign(size_t(Sh), HA_EMPTY);
+  const int shift = 64 - (__builtin_ctz(unsigned(Sh)) - 1);
+  for (auto& r : runs) {
+    unsigned long long k = r.second;
+    int from = -1;  // the bucket k was evicted from
+    for (int kick = 0; kick < 256; kick++) {
+      const int c1 = hot_bucket1(k, shift), c2 = hot_bucket2(k, shift);
+      int placed = -1;
+      for (int s : {2 * c1, 2 * c1 + 1, 2 * c2, 2 * c2 + 1})
+        if (h.table[size_t(s)] == HA_EMPTY) {
+          placed = s;
+          break;
+        }
+      if (placed >= 0) {
+        h.table[size_t(placed)] = k;
+        k = HA_EMPTY;
+        break;
+      }
+      const int eb = (from == c1) ? c2 : c1;
+      std::swap(k, h.table[size_t(2 * eb + (kick & 1))]);
+      from = eb;
+    }
+  }
+  h.used = true;
+  return h;
+}
+
+// Table slots of the heavy-key pass for a group-by whose LDS tables have S
+// slots: 2048 by default (the 80 % the plan fills hold the sampled heavy
+// keys of skewed data -- about 1,900 keys of 2^20 Zipf(1.1) values reach 4
+// groups of a 2^17-row sample; the 1,638 most counted are kept -- and two
+// workgroups' tables fit one CU's LDS); 0 when too small
+int hot_slots(int S) {
+  long long want = tune_get("groupby.hot", std::min(S, 2048));
+  if (want < 256) return 0;
+  int Sh = int(std::min<long long>(want, S));
+  while (Sh & (Sh - 1)) Sh &= Sh - 1;
+  return Sh;
+}
+
+// The heavy-key plan from known heavy keys: the same coverage rule as the
+// sampled plan (the kept keys must hold groupby.hot_min_pct % of the rows)
+HotPlan plan_hot_known(const HeavyKeys& hk, size_t n, int S) {
+  const int Sh = hot_slots(S);
+  if (!Sh || hk.n == 0 || hk.total == 0 || n < (size_t(1) << 16)) return HotPlan();
+  std::vector<std::pair<size_t, unsigned long long>> runs(hk.n);
+  for (size_t j = 0; j < hk.n; j++) runs[j] = {size_t(hk.counts[j]), hk.keys[j]};
+  const size_t keep = std::min(runs.size(), size_t(Sh) * 4 / 5);
+  size_t covered = 0;
+  for (size_t j = 0; j < keep; j++) covered += runs[j].first;
+  const long long min_pct = tune_get("groupby.hot_min_pct", 30);
+  if (covered * 100 < size_t(std::max(0LL, min_pct)) * hk.total) return HotPlan();
+  return place_hot(runs, Sh);
+}
+
 HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
   HotPlan h;
-  // 2048 slots by default: the 80 % the plan fills hold the sampled heavy
-  // keys of skewed data (about 1,900 keys of 2^20 Zipf(1.1) values reach 4
-  // groups of a 2^17-row sample; the 1,638 most counted are kept), and two
-  // workgroups' tables fit one CU's LDS
-  long long want = tune_get("groupby.hot", std::min(S, 2048));
+  const int Sh = hot_slots(S);
   // sample of 2^groupby.hot_sample rows (default 2^17)
   const int mbits = int(std::max(12LL, std::min(20LL, tune_get("groupby.hot_sample", 17))));
   const size_t m = size_t(1) << mbits;
-  if (want < 256 || n < 16 * m) return h;
-  int Sh = int(std::min<long long>(want, S));
-  while (Sh & (Sh - 1)) Sh &= Sh - 1;
+  if (!Sh || n < 16 * m) return h;
   // n >= 16 m, so the groups (HOT_GROUP rows every gstep rows) never overlap
   const size_t gstep = n / (m / HOT_GROUP);
   const unsigned tsize = unsigned(2 * m), cap = unsigned(m / HOT_MIN_SEEN);
@@ -1212,43 +1271,14 @@ HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {
     }
   }
   // two-choice buckets of two slots place keys up to a load near 90 %; the
-  // plan fills at most 80 % of the slots
-  if (runs.size() > size_t(Sh) * 4 / 5) runs.resize(size_t(Sh) * 4 / 5);
-  // sample rows holding a kept key, against the sample's m rows
+  // plan fills at most 80 % of the slots.  Sample rows holding a kept key,
+  // against the sample's m rows
+  const size_t keep = std::min(runs.size(), size_t(Sh) * 4 / 5);
   size_t covered = 0;
-  for (size_t j = 0; j < runs.size(); j++) covered += size_t(rows_of[j]);
+  for (size_t j = 0; j < keep; j++) covered += size_t(rows_of[j]);
   const long long min_pct = tune_get("groupby.hot_min_pct", 30);
   if (runs.empty() || covered * 100 < size_t(std::max(0LL, min_pct)) * m) return h;
-  h.Sh = Sh;
-  h.table.assign(size_t(Sh), HA_EMPTY);
-  // two-choice placement, heaviest keys first: a key goes to a free slot of
-  // either bucket, or displaces a key of the bucket it was not just evicted
-  // from, which then moves on; a key still unplaced after the displacement
-  // limit is left out (its rows take the partitioned path)
-  const int shift = 64 - (__builtin_ctz(unsigned(Sh)) - 1);
-  for (auto& r : runs) {
-    unsigned long long k = r.second;
-    int from = -1;  // the bucket k was evicted from
-    for (int kick = 0; kick < 256; kick++) {
-      const int c1 = hot_bucket1(k, shift), c2 = hot_bucket2(k, shift);
-      int placed = -1;
-      for (int s : {2 * c1, 2 * c1 + 1, 2 * c2, 2 * c2 + 1})
-        if (h.table[size_t(s)] == HA_EMPTY) {
-          placed = s;
-          break;
-        }
-      if (placed >= 0) {
-        h.table[size_t(placed)] = k;
-        k = HA_EMPTY;
-        break;
-      }
-      const int eb = (from == c1) ? c2 : c1;
-      std::swap(k, h.table[size_t(2 * eb + (kick & 1))]);
-      from = eb;
-    }
-  }
-  h.used = true;
-  return h;
+  return place_hot(runs, Sh);
 }
 
 static __global__ void ha_iota_kernel(size_t* __restrict__ p, size_t n) {
@@ -1279,9 +1309,11 @@ void group_by_agg_sorted(int ktype, const void* keys, size_t n, const fhip_agg* 
 }
 
 // Sorted-by-key result of a fused group-by; see fhip_dev_group_by_agg.
+// heavy (optional): the column's heavy keys known in advance (a model of the
+// column); then no sample of the input is counted for the heavy-key table
 void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
                   void* keys_out, size_t capacity, size_t* ngroups, size_t table_hint,
-                  hipStream_t st) {
+                  hipStream_t st, const HeavyKeys* heavy) {
   FHIP_REQUIRE(ktype == FHIP_I32 || ktype == FHIP_U32 || ktype == FHIP_I64 || ktype == FHIP_U64,
                FHIP_ERR_TYPE, "group_by_agg: integer keys only (use group_by for FP keys)");
   FHIP_REQUIRE(naggs >= 0 && naggs <= 16, FHIP_ERR_ARG, "group_by_agg: too many aggregations");
@@ -1361,7 +1393,9 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   };
   // ---- heavy keys: aggregated in one pass, the other rows go on ----------
   HotPlan hp;
-  {
+  if (heavy) {
+    hp = plan_hot_known(*heavy, n, S);
+  } else {
     ProfScope prof("groupby_plan", st);
     hp = plan_hot(ktype, keys, n, S, st);
   }

@@ -564,6 +564,25 @@ __global__ __launch_bounds__(256) void count_runs_kernel(const uint64_t* __restr
   if (threadIdx.x == 0) atomicAdd(out, ws[0] + ws[1] + ws[2] + ws[3]);
 }
 
+// the first row of every run of equal keys of a sorted column
+struct RunStartPred {
+  const uint64_t* k;
+  __device__ bool operator()(size_t i) const { return i == 0 || k[i] != k[i - 1]; }
+};
+
+// run j: its key and its length (the next run's start, or n, minus its own)
+__global__ __launch_bounds__(256) void run_len_kernel(const uint64_t* __restrict__ k, size_t n,
+                                                      const size_t* __restrict__ starts, size_t runs,
+                                                      unsigned long long* __restrict__ len,
+                                                      unsigned long long* __restrict__ key) {
+  const size_t stride = size_t(gridDim.x) * 256;
+  for (size_t j = size_t(blockIdx.x) * 256 + threadIdx.x; j < runs; j += stride) {
+    const size_t a = starts[j];
+    len[j] = (j + 1 < runs ? starts[j + 1] : n) - a;
+    key[j] = k[a];
+  }
+}
+
 }  // namespace
 }  // namespace fhip
 
@@ -580,6 +599,10 @@ struct fhip_cdf_model {
   std::vector<uint64_t> sample;      // ordered sample keys (sorted)
   size_t train_n = 0;                // rows of the training input
   size_t train_distinct = 0;         // distinct keys of the training input
+  // the heaviest keys of the training input (raw key bits, the longest runs
+  // of the sorted column first) and their rows, for the group-by's
+  // heavy-key table
+  std::vector<unsigned long long> heavy_keys, heavy_counts;
   double final_loss = 0;
 };
 
@@ -672,6 +695,31 @@ void distill(fhip_cdf_model& m, const std::vector<float>& y) {
 
 }  // namespace
 
+constexpr size_t LS_HEAVY = 4096;  // heavy keys a model keeps
+
+// The model's heavy keys: the LS_HEAVY longest runs of the sorted training
+// column (ties in key order), with their lengths.
+static void learned_heavy(fhip_cdf_model* m, const void* sorted, size_t n, hipStream_t st) {
+  const size_t D = m->train_distinct;
+  if (D == 0 || D == n) return;  // every key distinct: none is heavy
+  const uint64_t* k = static_cast<const uint64_t*>(sorted);
+  DBuf sb(D * sizeof(size_t)), lb(D * 8), kb(D * 8);
+  const size_t got = compact_indices(RunStartPred{k}, n, sb.as<size_t>(), st);
+  FHIP_REQUIRE(got == D, FHIP_ERR_HIP, "learned_train: run count changed");
+  run_len_kernel<<<grid_for_n(D), 256, 0, st>>>(k, n, sb.as<size_t>(), D,
+                                                lb.as<unsigned long long>(),
+                                                kb.as<unsigned long long>());
+  FHIP_CHECK(hipGetLastError());
+  // longest first; the stable sort keeps equal lengths in key order
+  sort_pairs(FHIP_U64, lb.get(), lb.get(), 8, kb.get(), kb.get(), D, false, true, st);
+  const size_t K = std::min(D, LS_HEAVY);
+  m->heavy_counts.resize(K);
+  m->heavy_keys.resize(K);
+  FHIP_CHECK(hipMemcpyAsync(m->heavy_counts.data(), lb.get(), K * 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipMemcpyAsync(m->heavy_keys.data(), kb.get(), K * 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+}
+
 fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t samples, int steps,
                               float lr, hipStream_t st) {
   FHIP_REQUIRE(ktype == FHIP_I64 || ktype == FHIP_U64, FHIP_ERR_TYPE,
@@ -702,6 +750,7 @@ fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t sa
       FHIP_CHECK(hipStreamSynchronize(st));
       m->train_distinct = size_t(d);
     }
+    learned_heavy(m, sorted, n, st);
     const double span = std::log2(1.0 + double(m->umax - m->umin));
     m->inv_span = span > 0 ? 1.0 / span : 1.0;
     // the network trains on at most 2^18 of the samples (full batch)
@@ -1007,6 +1056,38 @@ int fhip_learned_join_strategy(const fhip_cdf_model* m, size_t n_build, int* str
     // (the table build decides, as in the reference); else AUTO with it
     const double d = fhip::learned_cardinality(m, n_build);
     *strategy = d >= 0.999 * double(n_build) ? FHIP_JOIN_AUTO_UNIQUE : FHIP_JOIN_AUTO;
+  });
+}
+
+int fhip_learned_heavy_keys(const fhip_cdf_model* m, size_t* n_keys, double* row_frac) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(m && n_keys && row_frac, FHIP_ERR_ARG, "learned_heavy_keys: null pointer");
+    *n_keys = m->heavy_keys.size();
+    unsigned long long rows = 0;
+    for (unsigned long long c : m->heavy_counts) rows += c;
+    *row_frac = m->train_n ? double(rows) / double(m->train_n) : 0.0;
+  });
+}
+
+int fhip_dev_group_by_agg_model(int ktype, const void* keys, size_t n, const fhip_agg* aggs,
+                                int naggs, void* keys_out, size_t capacity, size_t* ngroups,
+                                const fhip_cdf_model* model, void* stream) {
+  return fhip::guarded([&] {
+    FHIP_REQUIRE(model, FHIP_ERR_ARG, "group_by_agg_model: null model");
+    FHIP_REQUIRE(ngroups && (naggs == 0 || aggs), FHIP_ERR_ARG, "group_by_agg: null pointer");
+    FHIP_REQUIRE(n == 0 || (keys && keys_out), FHIP_ERR_ARG, "group_by_agg: null keys");
+    const size_t hint = size_t(std::ceil(std::max(1.0, fhip::learned_cardinality(model, n))));
+    // the heavy keys apply to a column of the model's own key type
+    fhip::HeavyKeys hk;
+    const bool use = model->ktype == ktype && !model->heavy_keys.empty();
+    if (use) {
+      hk.keys = model->heavy_keys.data();
+      hk.counts = model->heavy_counts.data();
+      hk.n = model->heavy_keys.size();
+      hk.total = model->train_n;
+    }
+    fhip::group_by_agg(ktype, keys, n, aggs, naggs, keys_out, capacity, ngroups, hint,
+                       fhip::as_stream(stream), use ? &hk : nullptr);
   });
 }
 

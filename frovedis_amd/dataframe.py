@@ -70,11 +70,11 @@ def group_by_agg(keys, aggs, table_hint=0, capacity=None, stream=None, model=Non
     (column may be None for count).  Returns (group_keys, [outputs]) as
     device arrays, groups ascending by key like grouped_dftable::select
     (dftable.cc:3724-3772).  ``model``: a CdfModel of the key column
-    (trained on an earlier input) whose cardinality estimate becomes the
-    table hint that sizes the partitioned hash tables."""
+    (trained on an earlier input): its cardinality estimate sizes the
+    partitioned hash tables and its heavy keys fill the heavy-key table
+    (fhip_dev_group_by_agg_model), in place of ``table_hint`` and of the
+    sample the library would count; the result is the same."""
     n = len(keys)
-    if model is not None and not table_hint:
-        table_hint = model.table_hint(n)
     dev = keys.t.device
     cap = int(capacity) if capacity is not None else max(1, min(n, 1 << 22))
     for _ in range(2):
@@ -92,9 +92,14 @@ def group_by_agg(keys, aggs, table_hint=0, capacity=None, stream=None, model=Non
             specs[i].col = col.ptr.value if col is not None else None
             specs[i].out = out.ptr.value
         ng = ctypes.c_size_t(0)
-        rc = lib().fhip_dev_group_by_agg(keys.code, keys.ptr, _sz(n), specs, len(aggs), kout.ptr,
-                                         _sz(cap), ctypes.byref(ng), _sz(table_hint),
-                                         stream or current_stream())
+        if model is not None:
+            rc = lib().fhip_dev_group_by_agg_model(keys.code, keys.ptr, _sz(n), specs, len(aggs),
+                                                   kout.ptr, _sz(cap), ctypes.byref(ng),
+                                                   model._m, stream or current_stream())
+        else:
+            rc = lib().fhip_dev_group_by_agg(keys.code, keys.ptr, _sz(n), specs, len(aggs),
+                                             kout.ptr, _sz(cap), ctypes.byref(ng),
+                                             _sz(table_hint), stream or current_stream())
         if rc == 6 and ng.value > cap:  # FHIP_ERR_TOO_LARGE: resize once
             cap = ng.value
             continue

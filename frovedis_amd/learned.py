@@ -73,6 +73,14 @@ class CdfModel:
         check(lib().fhip_learned_table_hint(self._m, _sz(n), ctypes.byref(h)))
         return h.value
 
+    def heavy_keys(self):
+        """(number of heavy keys kept, fraction of the training rows they
+        hold): the longest runs of the sorted training column."""
+        k = ctypes.c_size_t(0)
+        f = ctypes.c_double(0)
+        check(lib().fhip_learned_heavy_keys(self._m, ctypes.byref(k), ctypes.byref(f)))
+        return k.value, f.value
+
     def join_strategy(self, n_build):
         """"auto_unique" when the model predicts n_build distinct keys on the
         build side, else "auto" (the join's result is the same)."""

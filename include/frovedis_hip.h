@@ -426,6 +426,16 @@ int fhip_grouped_agg(int op, int vtype, const void* col, size_t col_n,
  *  - fhip_learned_join_strategy: FHIP_JOIN_AUTO_UNIQUE when the model
  *    predicts n_build distinct keys, else FHIP_JOIN_AUTO; the join's output
  *    is the same either way.
+ *  - fhip_learned_heavy_keys: how many heavy keys the model kept (the
+ *    longest runs of its sorted training column, at most 4096) and the
+ *    fraction of the training rows they hold.
+ *  - fhip_dev_group_by_agg_model: fhip_dev_group_by_agg with the model of
+ *    the key column (trained on an earlier input) in place of the table
+ *    hint: its cardinality estimate sizes the partitioned tables and, for a
+ *    column of the model's key type, its heavy keys fill the heavy-key
+ *    table, so no sample of the input is counted first.  The result is that
+ *    of fhip_dev_group_by_agg; a model of another distribution only makes
+ *    the step slower.
  *  - fhip_dev_mlp_*: the network's kernels (gradient of the mean squared
  *    error, forward pass, Adam step) on a parameter vector of
  *    fhip_mlp_nparam() floats. */
@@ -446,6 +456,13 @@ int fhip_learned_table_hint(const fhip_cdf_model* m, size_t n,
                             size_t* table_hint);
 int fhip_learned_join_strategy(const fhip_cdf_model* m, size_t n_build,
                                int* strategy);
+int fhip_learned_heavy_keys(const fhip_cdf_model* m, size_t* n_keys,
+                            double* row_frac);
+int fhip_dev_group_by_agg_model(int ktype, const void* keys, size_t n,
+                                const fhip_agg* aggs, int naggs,
+                                void* keys_out, size_t capacity,
+                                size_t* ngroups, const fhip_cdf_model* model,
+                                void* stream);
 int fhip_learned_model_info(const fhip_cdf_model* m, double* final_loss,
                             double* cdf_knots, uint64_t* coarse_splitters);
 int fhip_learned_model_params(const fhip_cdf_model* m, float* params);

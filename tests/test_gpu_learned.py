@@ -263,3 +263,50 @@ def test_model_picks_join_strategy(fa):
         lo0, ro0, u0 = fa.equi_join(a, ai, b, bi)
         lo1, ro1, u1 = fa.equi_join(a, ai, b, bi, model=m)
         assert u0 == u1 and torch.equal(lo0.t, lo1.t) and torch.equal(ro0.t, ro1.t)
+
+
+def _prof_count(fa, name):
+    import ctypes
+    t = ctypes.c_double(0)
+    c = ctypes.c_size_t(0)
+    fa.check(fa.lib().fhip_prof_query(name.encode(), ctypes.byref(t), ctypes.byref(c)))
+    return c.value
+
+
+def test_model_heavy_keys_plan_the_group_by(fa):
+    # the model of an earlier Zipf input keeps its longest runs as heavy
+    # keys; the group-by then fills its heavy-key table from them (no sample
+    # of the input counted: no "groupby_plan" scope) with the same result,
+    # and a model of another distribution changes nothing but the speed
+    import torch
+    from frovedis_amd import datagen as dg
+    n = 1 << 22
+    prev = dg.zipf_keys_dev(n, start=n)
+    model = fa.CdfModel(fa.DeviceArray(_plain_sorted(fa, prev), np.int64), steps=100)
+    nk, frac = model.heavy_keys()
+    assert nk == 4096 and 0.6 < frac < 0.95, (nk, frac)
+    other = fa.CdfModel(fa.DeviceArray(_plain_sorted(fa, dg.lognormal_mixture_dev(n)), np.int64),
+                        steps=100)
+    k = fa.DeviceArray(dg.zipf_keys_dev(n), np.int64)
+    v = fa.DeviceArray(dg.normal_dev(n), np.float64)
+    iv = fa.DeviceArray(dg.int_uniform_dev(n, 1000), np.int32)
+    aggs = [("sum", v), ("count", v), ("max", iv)]
+    fa.check(fa.lib().fhip_prof_enable(1))
+    g0, (s0, c0, m0) = fa.group_by_agg(k, aggs)
+    torch.cuda.synchronize()
+    assert _prof_count(fa, "groupby_plan") == 1
+    fa.check(fa.lib().fhip_prof_enable(0))
+    fa.check(fa.lib().fhip_prof_enable(1))
+    g1, (s1, c1, m1) = fa.group_by_agg(k, aggs, model=model)
+    torch.cuda.synchronize()
+    assert _prof_count(fa, "groupby_plan") == 0
+    fa.check(fa.lib().fhip_prof_enable(0))
+    g2, (s2, c2, m2) = fa.group_by_agg(k, aggs, model=other)
+    for g, s_, c, m in ((g1, s1, c1, m1), (g2, s2, c2, m2)):
+        assert torch.equal(g0.t, g.t) and torch.equal(c0.t, c.t) and torch.equal(m0.t, m.t)
+        assert torch.allclose(s0.t, s_.t, rtol=1e-12, atol=1e-9)
+    # against the oracle
+    gi, split, ukeys = oracle.group_by(k.numpy())
+    assert np.array_equal(g1.numpy(), ukeys)
+    assert np.array_equal(c1.numpy(), oracle.grouped_agg("count", v.numpy(), gi, split))
+    assert np.array_equal(m1.numpy(), oracle.grouped_agg("max", iv.numpy(), gi, split))
