@@ -933,6 +933,31 @@ __global__ void gather_group_keys(const unsigned long long* __restrict__ w, int 
 }
 
 // The aggregations of one fused group-by, for the final write.
+// Dense key range (ordered key bits in [lo, lo + R)): flag[b - lo] = 1 for
+// every group's key; the inclusive scan of the flags then gives each key
+// its rank among the groups, which replaces the sort of the group keys.
+__global__ void dense_mark_kernel(const unsigned long long* __restrict__ w, int stride,
+                                  const size_t* __restrict__ ent, size_t G,
+                                  unsigned long long mask, unsigned long long flip,
+                                  unsigned long long lo, unsigned* __restrict__ flag) {
+  const size_t st = size_t(gridDim.x) * blockDim.x;
+  for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < G; g += st)
+    flag[((w[ent[g] * stride] & mask) ^ flip) - lo] = 1u;
+}
+
+// ent_sorted[rank of group g's key] = ent[g] (ranks from the inclusive scan)
+__global__ void dense_order_kernel(const unsigned long long* __restrict__ w, int stride,
+                                   const size_t* __restrict__ ent, size_t G,
+                                   unsigned long long mask, unsigned long long flip,
+                                   unsigned long long lo, const unsigned* __restrict__ incl,
+                                   size_t* __restrict__ ent_sorted) {
+  const size_t st = size_t(gridDim.x) * blockDim.x;
+  for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < G; g += st) {
+    const size_t e = ent[g];
+    ent_sorted[incl[((w[e * stride] & mask) ^ flip) - lo] - 1u] = e;
+  }
+}
+
 struct EmitSet {
   int n;
   int op[16];
@@ -1525,8 +1550,11 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   // Final assembly over Ptot partitions' tables (the heavy keys are the
   // last one when used): compact, sort the group keys, gather the outputs.
   // low_bytes: the key bytes the group-key sort must run (-1: unknown)
+  // range (may be null): the smallest and largest group key, as ordered key
+  // bits (sign bit flipped for signed keys)
   auto assemble = [&](const unsigned long long* w, const unsigned* fn_dev,
-                      const std::vector<unsigned>& fn, int Ptot, int low_bytes) {
+                      const std::vector<unsigned>& fn, int Ptot, int low_bytes,
+                      const unsigned long long* range) {
     // partition offsets, uploaded from pinned memory (the lambda ends with
     // a synchronisation, so the slot is free again on return)
     unsigned long long* foff =
@@ -1546,18 +1574,49 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     final_index_kernel<<<Ptot, 256, 0, st>>>(fn_dev, fob.as<unsigned long long>(), Ptot, S,
                                               entb.as<size_t>());
     FHIP_CHECK(hipGetLastError());
-    DBuf gk(G * size_t(kb));
-    switch (ktype) {
-      case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int32_t>()); break;
-      case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint32_t>()); break;
-      case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<int64_t>()); break;
-      default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, gk.as<uint64_t>()); break;
+    auto gather_keys = [&](const size_t* ent, void* dst) {
+      switch (ktype) {
+        case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(w, stride, ent, G, static_cast<int32_t*>(dst)); break;
+        case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(w, stride, ent, G, static_cast<uint32_t*>(dst)); break;
+        case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(w, stride, ent, G, static_cast<int64_t*>(dst)); break;
+        default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(w, stride, ent, G, static_cast<uint64_t*>(dst)); break;
+      }
+      FHIP_CHECK(hipGetLastError());
+    };
+    // keys are distinct, so ascending order is the reference's order: when
+    // they fill their range densely enough (at most `groupby.dense_ratio`
+    // possible keys per group, 32 by default; at most 2^28), a key's rank is
+    // the number of group keys at or below it (flags and a scan); else the
+    // group keys are radix-sorted
+    const long long ratio = tune_get("groupby.dense_ratio", 32);
+    const unsigned long long R = range ? range[1] - range[0] + 1 : 0;
+    DBuf ent_sorted;
+    const size_t* ent_out = entb.as<size_t>();
+    if (range && range[0] <= range[1] && R != 0 && ratio > 0 && R <= (size_t(1) << 28) &&
+        R <= (unsigned long long)(ratio) * G) {
+      const unsigned long long mask = kb == 8 ? ~0ull : (1ull << (8 * kb)) - 1;
+      const unsigned long long flip =
+          (ktype == FHIP_I32 || ktype == FHIP_I64) ? (1ull << (8 * kb - 1)) : 0ull;
+      DBuf flag(size_t(R) * 4), incl(size_t(R) * 4);
+      ent_sorted.reset(G * sizeof(size_t));
+      FHIP_CHECK(hipMemsetAsync(flag.get(), 0, size_t(R) * 4, st));
+      dense_mark_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, mask, flip,
+                                                     range[0], flag.as<unsigned>());
+      FHIP_CHECK(hipGetLastError());
+      prefix_sum(FHIP_U32, flag.get(), incl.get(), size_t(R), st);
+      dense_order_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, mask,
+                                                      flip, range[0], incl.as<unsigned>(),
+                                                      ent_sorted.as<size_t>());
+      FHIP_CHECK(hipGetLastError());
+      ent_out = ent_sorted.as<size_t>();
+      gather_keys(ent_out, keys_out);
+    } else {
+      DBuf gk(G * size_t(kb));
+      gather_keys(entb.as<size_t>(), gk.get());
+      sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st,
+                 low_bytes);
+      FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
     }
-    FHIP_CHECK(hipGetLastError());
-    // keys are distinct, so the stable sort gives the reference's ascending order
-    sort_pairs(ktype, gk.get(), gk.get(), 8, entb.get(), entb.get(), G, false, false, st,
-               low_bytes);
-    FHIP_CHECK(hipMemcpyAsync(keys_out, gk.get(), G * size_t(kb), hipMemcpyDeviceToDevice, st));
     if (naggs > 0) {
       EmitSet es{};
       es.n = naggs;
@@ -1567,7 +1626,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
         es.slot[i] = slot_of[i] < 0 ? 0 : slot_of[i];
         es.out[i] = aggs[i].out;
       }
-      emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, entb.as<size_t>(), G, es);
+      emit_kernel<<<grid_for(G), 256, 0, st>>>(w, stride, ent_out, G, es);
       FHIP_CHECK(hipGetLastError());
     }
     FHIP_CHECK(hipStreamSynchronize(st));  // the pinned slot of foff is reused by the next call
@@ -1576,7 +1635,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   auto assemble_hot_only = [&]() {  // every row had a heavy key
     DBuf fnb(sizeof(unsigned));
     FHIP_CHECK(hipMemcpyAsync(fnb.get(), &hot_groups, sizeof(unsigned), hipMemcpyHostToDevice, st));
-    assemble(hot_fin.as<unsigned long long>(), fnb.as<unsigned>(), std::vector<unsigned>{hot_groups}, 1, -1);
+    assemble(hot_fin.as<unsigned long long>(), fnb.as<unsigned>(), std::vector<unsigned>{hot_groups}, 1, -1,
+             nullptr);
   };
   if (hp.used && !cn_pending && cn == 0) {
     assemble_hot_only();
@@ -1759,7 +1819,8 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       const unsigned long long d = rng_h[0] ^ rng_h[1];
       low_bytes = d ? (63 - __builtin_clzll(d)) / 8 + 1 : 0;
     }
-    assemble(fin.as<unsigned long long>(), fnb.as<unsigned>(), fn, Ptot, low_bytes);
+    assemble(fin.as<unsigned long long>(), fnb.as<unsigned>(), fn, Ptot, low_bytes,
+             rng_h[0] <= rng_h[1] ? rng_h : nullptr);
     return;
   }
 }

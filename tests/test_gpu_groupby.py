@@ -394,3 +394,39 @@ def test_fp_max_min_with_signed_zeros_and_nan(fa, dt):
                                    [(op, fa.DeviceArray.from_numpy(v))])
         assert np.array_equal(gk.numpy(), keys)
         assert np.array_equal(h.numpy(), ref), op  # == on values: -0.0 == 0.0
+
+
+@pytest.mark.parametrize("kt", [np.int32, np.uint32, np.int64, np.uint64])
+@pytest.mark.parametrize("dense", [True, False])
+def test_group_by_agg_dense_key_ranks(fa, kt, dense):
+    # keys filling a narrow range (across zero for signed types, near the
+    # top for unsigned ones): the group keys are ordered by ranks from flags
+    # and a scan instead of a sort ("groupby.dense_ratio"; 0 switches it
+    # off); both give the oracle's keys and aggregates, and so does a sparse
+    # column of the same type
+    from frovedis_amd import autotune as at
+    rng = np.random.default_rng(41)
+    n = 1 << 20
+    info = np.iinfo(kt)
+    dt = np.int64 if info.min < 0 else np.uint64
+    lo = -5000 if info.min < 0 else int(info.max) - 300000
+    cols = {"dense": rng.integers(lo, lo + 200000, n, dtype=dt),
+            "edge": rng.integers(int(info.max) - 60000, int(info.max), n, endpoint=True, dtype=dt),
+            "sparse": rng.integers(int(info.min), int(info.max), n, endpoint=True, dtype=dt)}
+    at.tune_set("groupby.dense_ratio", None if dense else 0)
+    try:
+        for name, raw in cols.items():
+            k = np.asarray(raw).astype(kt)
+            k[::5] = k[3]  # a heavy key
+            v = rng.integers(-1000, 1000, n).astype(np.int64)
+            gi, split, keys = oracle.group_by(k)
+            gk, (s, c, mx) = fa.group_by_agg(fa.DeviceArray.from_numpy(k),
+                                             [("sum", fa.DeviceArray.from_numpy(v)),
+                                              ("count", None),
+                                              ("max", fa.DeviceArray.from_numpy(v))])
+            assert np.array_equal(gk.numpy(), keys), name
+            assert np.array_equal(s.numpy(), oracle.grouped_agg("sum", v, gi, split)), name
+            assert np.array_equal(c.numpy(), oracle.grouped_agg("count", v, gi, split)), name
+            assert np.array_equal(mx.numpy(), oracle.grouped_agg("max", v, gi, split)), name
+    finally:
+        at.tune_set("groupby.dense_ratio", None)
