@@ -254,7 +254,8 @@ constexpr int BJ_OFF_WCNT = BJ_OFF_VS + BQ_CAP * 8;           // BJ_WAVES x 256 
 constexpr int BJ_OFF_TSTART = BJ_OFF_WCNT + BJ_WAVES * 256 * 4;
 constexpr int BJ_OFF_SCAN = BJ_OFF_TSTART + 256 * 4;          // BQ_WAVES x 4
 constexpr int BJ_OFF_MISC = BJ_OFF_SCAN + 64;                 // 8 x 8
-constexpr int BJ_LDS = BJ_OFF_MISC + 64;
+constexpr int BJ_OFF_VIN = BJ_OFF_MISC + 64;                  // BQ_CAP x 8
+constexpr int BJ_LDS = BJ_OFF_VIN + BQ_CAP * 8;
 static_assert(2 * BJ_LDS > 160 * 1024, "one job workgroup per CU");
 // LDS (bytes) of the histogram and of the plan
 constexpr int BH_LDS = BQ_FINE * 4;
@@ -524,6 +525,9 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
     return reinterpret_cast<unsigned*>(lds + BJ_OFF_PK0 + (c ? BQ_CAP * 4 : 0));
   };
   double* vs = reinterpret_cast<double*>(lds + BJ_OFF_VS);
+  // the job's values in input order, loaded with the keys (coalesced); the
+  // sorted order reads them from here instead of gathering from memory
+  double* vin = reinterpret_cast<double*>(lds + BJ_OFF_VIN);
   unsigned* wcnt = reinterpret_cast<unsigned*>(lds + BJ_OFF_WCNT);
   unsigned* tstart = reinterpret_cast<unsigned*>(lds + BJ_OFF_TSTART);
   unsigned* scan = reinterpret_cast<unsigned*>(lds + BJ_OFF_SCAN);
@@ -556,10 +560,16 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   bool valid[IT];
   int bad = 0;
   unsigned long long lo = ~0ull, hi = 0;
+  double xv[IT];
 #pragma unroll
   for (int j = 0; j < IT; j++) {
     const unsigned p = wave_base + unsigned(j) * 64 + lane;
     valid[j] = fits && p < m;
+    xv[j] = valid[j] ? vsrc[p] : 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned p = wave_base + unsigned(j) * 64 + lane;
     const int64_t k = valid[j] ? ksrc[p] : 0;
     if (valid[j] && (k < 0 || (unsigned long long)k >= A.key_lim)) bad = 1;
     kv[j] = (unsigned long long)k;
@@ -568,6 +578,9 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
       hi = kv[j] > hi ? kv[j] : hi;
     }
   }
+#pragma unroll
+  for (int j = 0; j < IT; j++)
+    if (valid[j]) vin[wave_base + unsigned(j) * 64 + lane] = xv[j];
   if (small && __ballot(bad) && lane == 0) atomicOr(&A.scal[0], 1ull);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -654,7 +667,7 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   const unsigned mm = ok ? unsigned(m) : 0u;
   for (unsigned i = tid; i < mm; i += BJ_NT) {
     const unsigned v = srt[i];
-    const double x = vsrc[v & (BQ_CAP - 1)];
+    const double x = vin[v & (BQ_CAP - 1)];
     A.keys_out[a + i] = int64_t(kmin + (v >> BQ_IDX_BITS));
     A.vals_out[a + i] = x;
     vs[i] = x;
