@@ -60,7 +60,7 @@ RADIX_SHAPES = ((256, 16), (256, 32), (512, 16), (512, 24), (512, 12), (1024, 8)
 COMPACT_SHAPES = ((256, 16), (1024, 16))
 
 BATCH_DIGIT_BITS = (4, 6, 8)
-BATCH_BUCKET_ROWS = (2048, 3072, 3584)
+BATCH_BUCKET_ROWS = (4096, 6144, 7168)
 
 
 def batch_size_class(rows):

@@ -183,14 +183,14 @@ constexpr int BQ_TILE_ROWS = BQ_NT * BQ_TITEMS;
 // gave wrong rows now and then on the MI355X; one workgroup per CU never did.
 constexpr int BJ_NT = 1024;
 constexpr int BJ_WAVES = BJ_NT / 64;
-constexpr int BQ_ITEMS = 4;                 // rows per thread of a job
+constexpr int BQ_ITEMS = 8;                 // rows per thread of a job
 constexpr int BQ_CAP = BJ_NT * BQ_ITEMS;    // rows one LDS job holds
 constexpr int BQ_FINE_BITS = 12;
 constexpr int BQ_FINE = 1 << BQ_FINE_BITS;
-constexpr int BQ_IDX_BITS = 12;  // row within a job (BQ_CAP = 2^12)
+constexpr int BQ_IDX_BITS = 13;  // row within a job (BQ_CAP = 2^13)
 constexpr unsigned long long BQ_REL_LIMIT = 1ull << (32 - BQ_IDX_BITS);
 constexpr int BQ_EARLY_BITS = 15;  // key span of a job whose group count is published early
-static_assert((1 << BQ_EARLY_BITS) / 8 <= BQ_CAP * 8, "the early bitmap lives in the value buffer");
+static_assert((1 << BQ_EARLY_BITS) / 8 <= BQ_CAP * 4, "the early bitmap lives in the second row buffer");
 static_assert(BQ_CAP == (1 << BQ_IDX_BITS), "job rows must fit the packed row field");
 
 enum : unsigned { BQ_HIST = 0, BQ_TILE = 1, BQ_JOB = 2 };
@@ -249,8 +249,7 @@ constexpr int BT_LDS = BT_OFF_SCAN + 64;
 // LDS (bytes) of a job: two packed-row buffers, the sorted values, ranks
 constexpr int BJ_OFF_PK0 = 0;                                 // BQ_CAP x 4
 constexpr int BJ_OFF_PK1 = BJ_OFF_PK0 + BQ_CAP * 4;           // BQ_CAP x 4
-constexpr int BJ_OFF_VS = BJ_OFF_PK1 + BQ_CAP * 4;            // BQ_CAP x 8
-constexpr int BJ_OFF_WCNT = BJ_OFF_VS + BQ_CAP * 8;           // BJ_WAVES x 256 x 4
+constexpr int BJ_OFF_WCNT = BJ_OFF_PK1 + BQ_CAP * 4;          // BJ_WAVES x 256 x 4
 constexpr int BJ_OFF_TSTART = BJ_OFF_WCNT + BJ_WAVES * 256 * 4;
 constexpr int BJ_OFF_SCAN = BJ_OFF_TSTART + 256 * 4;          // BQ_WAVES x 4
 constexpr int BJ_OFF_MISC = BJ_OFF_SCAN + 64;                 // 8 x 8
@@ -524,9 +523,9 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   auto pk = [lds](int c) {
     return reinterpret_cast<unsigned*>(lds + BJ_OFF_PK0 + (c ? BQ_CAP * 4 : 0));
   };
-  double* vs = reinterpret_cast<double*>(lds + BJ_OFF_VS);
   // the job's values in input order, loaded with the keys (coalesced); the
-  // sorted order reads them from here instead of gathering from memory
+  // sorted rows read them from here (by their row field) instead of
+  // gathering from memory, and so do the folds
   double* vin = reinterpret_cast<double*>(lds + BJ_OFF_VIN);
   unsigned* wcnt = reinterpret_cast<unsigned*>(lds + BJ_OFF_WCNT);
   unsigned* tstart = reinterpret_cast<unsigned*>(lds + BJ_OFF_TSTART);
@@ -600,11 +599,12 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   // sort the packed rows by their key bits (stable LSD passes in LDS)
   const int kbits = range ? 64 - __builtin_clzll(range) : 0;
   // the job's group count up front when its keys span at most
-  // BQ_EARLY_BITS bits (a bitmap of the keys present, in the value buffer):
+  // BQ_EARLY_BITS bits (a bitmap of the keys present, in the second row
+  // buffer, which the sort passes only use after it):
   // publishing it now lets later jobs' look-backs pass this job at once
   const bool early = ok && kbits <= A.early_bits;
   if (early) {
-    unsigned* bits = reinterpret_cast<unsigned*>(vs);
+    unsigned* bits = pk(1);
     for (unsigned w = tid; w < (1u << BQ_EARLY_BITS) / 32; w += BJ_NT) bits[w] = 0;
     __syncthreads();
 #pragma unroll
@@ -662,15 +662,13 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
     }
   }
   __syncthreads();
-  // the sorted rows go out first (they need no group offset); their values
-  // are kept in LDS for the folds
+  // the sorted rows go out first (they need no group offset)
   const unsigned mm = ok ? unsigned(m) : 0u;
   for (unsigned i = tid; i < mm; i += BJ_NT) {
     const unsigned v = srt[i];
     const double x = vin[v & (BQ_CAP - 1)];
     A.keys_out[a + i] = int64_t(kmin + (v >> BQ_IDX_BITS));
     A.vals_out[a + i] = x;
-    vs[i] = x;
   }
   // group starts: a row whose key differs from the row before it
   unsigned long long fl[IT];
@@ -721,7 +719,7 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   for (unsigned g = tid; g < G; g += BJ_NT) {
     const unsigned s = starts[g], e = g + 1 < G ? starts[g + 1] : mm;
     double total = 0;
-    for (unsigned i = s; i < e; i++) total += vs[i];
+    for (unsigned i = s; i < e; i++) total += vin[srt[i] & (BQ_CAP - 1)];
     const unsigned long long o = goff + g;
     if (o < A.capacity) {
       A.gkeys[o] = int64_t(kmin + (srt[s] >> BQ_IDX_BITS));
@@ -964,7 +962,7 @@ unsigned long long bucket_rows_for(size_t m, long long dflt) {
 }
 
 void build_queue(QPlan& q, hipStream_t st) {
-  const long long bucket_rows = tune_get("batch.bucket_rows", 3072);
+  const long long bucket_rows = tune_get("batch.bucket_rows", 7168);
   const long long chunk = std::max(1024LL, tune_get("batch.hist_chunk", 1 << 16));
   const int fshift = std::max(0, q.key_bits - BQ_FINE_BITS);
   std::vector<BqSeg> ls;
@@ -1099,7 +1097,7 @@ bool queue_sort_groupby(const void* keys, const void* vals, size_t n, const size
   for (size_t s = 0; s < nseg; s++)
     FHIP_REQUIRE(q->off[s] <= q->off[s + 1], FHIP_ERR_ARG, "batch: seg_off must not decrease");
   q->tunables[0] = std::max(2LL, std::min(8LL, tune_get("batch.digit_bits", 8)));
-  q->tunables[1] = tune_get("batch.bucket_rows", 3072);
+  q->tunables[1] = tune_get("batch.bucket_rows", 7168);
   q->tunables[2] = tune_get("batch.hist_chunk", 1 << 16);
   q->tunables[3] = std::max(-1LL, std::min((long long)BQ_EARLY_BITS, tune_get("batch.early_bits", BQ_EARLY_BITS)));
   for (int c = 0; c < 64; c++)

@@ -96,7 +96,20 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *       8 to 64 workgroups per CU), and how a wave counts its digits: one
  *       atomic for the lanes holding lane 0's digit when at least 8 do, else
  *       one per lane, the match found from lane 0's whole key (4, default)
- *       or its digit per byte (2); 3 one atomic per lane, no match.
+ *       or its digit per byte (2); 3 one atomic per lane, no match;
+ *   "groupby.dense_ratio"  the group keys are ordered by ranks from flags
+ *       and a scan, not sorted, when their range holds at most this many
+ *       possible keys per group (default 32; 0 always sorts);
+ *   "batch.path"        0 (default) the batched work queue, 1 the
+ *       composite-key sort of fhip_dev_batch_sort_groupby;
+ *   "batch.digit_bits"  bits per LSD pass of a job's LDS sort (2 to 8,
+ *       default 8);
+ *   "batch.bucket_rows" rows per bucket job of a segment larger than one
+ *       job (8192 rows; default 7168), "batch.bucket_rows.c<k>" the same
+ *       for segments of 2^(k-1) to 2^k rows;
+ *   "batch.hist_chunk", "batch.early_bits"  rows per histogram workgroup of
+ *       the large segments (default 65536) and the key span (bits, default
+ *       15) up to which a job publishes its group count before sorting.
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);

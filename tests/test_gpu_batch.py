@@ -51,7 +51,8 @@ def _check_against_oracle(res, keys, vals, off):
 
 
 @pytest.mark.parametrize("sizes", [[1], [256, 300, 1000], [5, 0, 70000, 1, 4096, 0, 3],
-                                   [256] * 64 + [50000, 123457]])
+                                   [256] * 64 + [50000, 123457],
+                                   [8191, 8192, 8193, 16384, 7168, 7169]])
 def test_batch_matches_per_segment_oracle(fa, sizes):
     keys, vals, off = _case(np.array(sizes, np.int64))
     for use_graph in (False, True):
