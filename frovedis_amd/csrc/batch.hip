@@ -394,8 +394,11 @@ __device__ void bq_tile(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   }
   const unsigned tile_n = unsigned(it.b - it.a);
   const unsigned wave_base = wave * 64 * IT;
+  // registers: the tile's keys, then its values; the digit and rank of a
+  // key share one register, the LDS positions two to a register and the
+  // written keys' digits four to a register (no spills at 1024 x 16)
+  static_assert(BQ_TILE_ROWS <= 65536 && IT % 4 == 0, "16-bit tile positions, packed digits");
   int64_t key[IT];
-  double val[IT];
   unsigned dig[IT], rank[IT];
   bool valid[IT];
 #pragma unroll
@@ -403,7 +406,6 @@ __device__ void bq_tile(const BqArgs& A, const BqItem& it, unsigned char* lds) {
     const unsigned p = wave_base + unsigned(j) * 64 + lane;
     valid[j] = p < tile_n;
     key[j] = valid[j] ? A.keys[it.a + p] : 0;
-    val[j] = valid[j] ? A.vals[it.a + p] : 0.0;
   }
   __syncthreads();  // the map is in LDS
 #pragma unroll
@@ -412,6 +414,9 @@ __device__ void bq_tile(const BqArgs& A, const BqItem& it, unsigned char* lds) {
     dig[j] = map[f < BQ_FINE ? unsigned(f) : unsigned(BQ_FINE - 1)];
   }
   const unsigned total = bq_rank<BQ_NT, IT>(dig, valid, 8, wcnt, rank);
+  unsigned dr[IT];
+#pragma unroll
+  for (int j = 0; j < IT; j++) dr[j] = dig[j] | (rank[j] << 8);
   const unsigned ex = block_excl_scan<BQ_NT>(tid < 256 ? total : 0u, scan, (unsigned*)nullptr);
   if (tid < 256) {
     tstart[tid] = ex;
@@ -449,38 +454,49 @@ __device__ void bq_tile(const BqArgs& A, const BqItem& it, unsigned char* lds) {
     const unsigned long long bs = total ? A.bstart[size_t(it.seg) * 257 + tid] : 0ull;
     gdst[tid] = S.start + bs + excl - ex;
   }
-  __syncthreads();
-  int64_t* kbuf = reinterpret_cast<int64_t*>(lds + BT_OFF_BUF);
-  unsigned pos[IT];
+  // the values: loaded now, so that their latency overlaps the key reorder
+  double val[IT];
 #pragma unroll
   for (int j = 0; j < IT; j++) {
-    pos[j] = tstart[dig[j]] + wcnt[wave * 256 + dig[j]] + rank[j];
-    if (valid[j]) kbuf[pos[j]] = key[j];
+    const unsigned p = wave_base + unsigned(j) * 64 + lane;
+    val[j] = valid[j] ? A.vals[it.a + p] : 0.0;
   }
   __syncthreads();
-  unsigned long long dst[IT];
+  int64_t* kbuf = reinterpret_cast<int64_t*>(lds + BT_OFF_BUF);
+  unsigned pos2[IT / 2];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned d = dr[j] & 0xFFu;
+    const unsigned pos = tstart[d] + wcnt[wave * 256 + d] + (dr[j] >> 8);
+    if (j & 1) pos2[j / 2] |= pos << 16;
+    else pos2[j / 2] = pos;
+    if (valid[j]) kbuf[pos] = key[j];
+  }
+  __syncthreads();
+  unsigned dig4[IT / 4];
 #pragma unroll
   for (int j = 0; j < IT; j++) {
     const unsigned i = unsigned(j) * BQ_NT + tid;
-    dst[j] = 0;
+    unsigned d = 0;
     if (i < tile_n) {
       const int64_t k = kbuf[i];
       const unsigned long long f = (unsigned long long)k >> S.fshift;
-      const unsigned d = map[f < BQ_FINE ? unsigned(f) : unsigned(BQ_FINE - 1)];
-      dst[j] = gdst[d] + i;
-      A.tk[dst[j]] = k;
+      d = map[f < BQ_FINE ? unsigned(f) : unsigned(BQ_FINE - 1)];
+      A.tk[gdst[d] + i] = k;
     }
+    if (j % 4 == 0) dig4[j / 4] = d;
+    else dig4[j / 4] |= d << (8 * (j % 4));
   }
   __syncthreads();
   double* vbuf = reinterpret_cast<double*>(lds + BT_OFF_BUF);
 #pragma unroll
   for (int j = 0; j < IT; j++)
-    if (valid[j]) vbuf[pos[j]] = val[j];
+    if (valid[j]) vbuf[(pos2[j / 2] >> (16 * (j & 1))) & 0xFFFFu] = val[j];
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < IT; j++) {
     const unsigned i = unsigned(j) * BQ_NT + tid;
-    if (i < tile_n) A.tv[dst[j]] = vbuf[i];
+    if (i < tile_n) A.tv[gdst[(dig4[j / 4] >> (8 * (j % 4))) & 0xFFu] + i] = vbuf[i];
   }
 }
 
