@@ -382,7 +382,11 @@ def bench_join(fa, steps=3, log2_left=27, log2_right=29):
     ri = fa.DeviceArray(torch.arange(nr, dtype=torch.int64, device="cuda"), np.uint64)
     hits = int((rk.t < nl).sum())
     out = {"left_rows": nl, "right_rows": nr, "pairs": hits}
-    legs = {"hash": (rk, ri, lk, li, "auto", ("join_build", "join_probe", "join_compact")),
+    legs = {"hash": (rk, ri, lk, li, "auto", ("join_dense_stats", "join_dense_build_part",
+                                              "join_dense_build",
+                                              "join_dense_part", "join_dense_probe",
+                                              "join_dense_emit", "join_build", "join_probe",
+                                              "join_compact")),
             "sortmerge": (lk, li, rk, ri, "auto", ("radix_hist", "radix_pass", "join_merge",
                                                     "join_write"))}
     for name, (a, ai, b, bi, strat, kernels) in legs.items():

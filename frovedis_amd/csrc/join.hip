@@ -12,6 +12,7 @@
 // (key, left position, right position) order).
 // FHIP_JOIN_HASH requires unique right keys; FHIP_JOIN_SORTMERGE skips the
 // table build.
+#include "densejoin.hpp"
 #include "fhip_device.hpp"
 #include "hashtable.hpp"
 #include "radix_sort.hpp"
@@ -665,6 +666,21 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
     dispatch_type(ktype, [&](auto z) {
       using K = decltype(z);
       if constexpr (std::is_integral<K>::value) {
+        // a unique key column that fills its range: the partitioned direct
+        // table (densejoin.hip); else the open-addressing table
+        int dres = DJ_NA;
+        if (!only_left) {
+          dres = dense_join_pairs(ktype, left, lidx, nl, right, ridx, nr, lo, ro, capacity, &r, st);
+        } else {
+          DBuf vals, found;
+          dres = dense_join_flags(ktype, left, nl, right, ridx, nr, vals, found, st);
+          if (dres == DJ_DONE) r = emit(vals, found);
+        }
+        if (dres != DJ_NA) {
+          uniq = dres == DJ_DONE;
+          done = true;
+          return;
+        }
         JoinTable t;
         jt_build<K>(t, (const K*)right, reinterpret_cast<const unsigned long long*>(ridx), nr, st);
         uniq = t.unique;

@@ -91,6 +91,13 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *       fhip_dev_multi_group_by (the column-by-column sorts run instead);
  *   "join.load_pct"     load factor (10 to 50 %, default 50) of the joins' hash table for
  *       integer keys;
+ *   "join.dense_min", "join.dense_ratio"  the joins' direct table for integer
+ *       build keys that are unique and fill their range: at least this many
+ *       build rows (default 2^20) and a key range of at most dense_ratio x
+ *       the rows (default 2; 0 switches the direct table off);
+ *   "join.dense_slice_bits", "join.dense_chunks"  key offsets per part of the
+ *       direct table's radix partitioning (2^bits, 10 to 18, default 18) and
+ *       the workgroups that probe one part (default 128);
  *   "radix.hist_bpc", "radix.hist_mode"  workgroups per CU of the radix
  *       histogram pass (1 to 64; by default about 16384 keys per workgroup,
  *       8 to 64 workgroups per CU), and how a wave counts its digits: one
