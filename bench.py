@@ -383,7 +383,8 @@ def bench_join(fa, steps=3, log2_left=27, log2_right=29):
     hits = int((rk.t < nl).sum())
     out = {"left_rows": nl, "right_rows": nr, "pairs": hits}
     legs = {"hash": (rk, ri, lk, li, "auto", ("join_dense_stats", "join_dense_build_part",
-                                              "join_dense_build",
+                                              "join_dense_build", "join_dense_scatter",
+                                              "join_dense_verify",
                                               "join_dense_part", "join_dense_probe",
                                               "join_dense_emit", "join_build", "join_probe",
                                               "join_compact")),

@@ -135,14 +135,14 @@ __global__ __launch_bounds__(64) void dj_stats_final_kernel(const DjStats* __res
 //          out of range.
 // Within a part the rows come in LDS-atomic order; the readers never depend
 // on it.
-template <class K, class E, bool BUILD>
+template <class K, class E, bool BUILD, int MAXP = DJ_MAXP>
 __global__ __launch_bounds__(DJ_NT) void dj_part_kernel(
     const K* __restrict__ keys, size_t n, unsigned long long mn, unsigned long long R, int sbits,
     int P, size_t ntiles, const size_t* __restrict__ ridx, unsigned* __restrict__ o_out,
     E* __restrict__ v_out, unsigned short* __restrict__ dest, unsigned short* __restrict__ starts) {
   constexpr size_t STAGE = size_t(DJ_TILE) * (BUILD ? sizeof(E) : sizeof(unsigned));
   __shared__ __align__(16) unsigned char s_stage[STAGE];
-  __shared__ unsigned s_cnt[DJ_MAXP];
+  __shared__ unsigned s_cnt[MAXP];
   __shared__ unsigned s_tmp[DJ_NT / 64];
   __shared__ unsigned s_total;
   unsigned* stage_o = reinterpret_cast<unsigned*>(s_stage);
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(DJ_NT) void dj_part_kernel(
   // the tile's rows through uniform base pointers and 32-bit offsets
   const K* kt = keys + base;
   const unsigned rows = unsigned(std::min<size_t>(DJ_TILE, n - base));
-  for (int p = threadIdx.x; p < DJ_MAXP; p += DJ_NT) s_cnt[p] = 0;
+  for (int p = threadIdx.x; p < MAXP; p += DJ_NT) s_cnt[p] = 0;
   // o = key - mn, or ~0u for rows out of range (R <= 2^29)
   unsigned o[DJ_ITEMS];
   if (rows == DJ_TILE) {  // a full tile: every load issued at once
@@ -189,17 +189,25 @@ __global__ __launch_bounds__(DJ_NT) void dj_part_kernel(
   }
   __syncthreads();
   {
-    // exclusive scan of the DJ_MAXP counters, two per thread
-    static_assert(DJ_MAXP == 2 * DJ_NT, "two counters per thread");
-    const unsigned a = s_cnt[2 * threadIdx.x], b = s_cnt[2 * threadIdx.x + 1];
+    // exclusive scan of the MAXP counters, PER consecutive ones per thread
+    constexpr int PER = MAXP / DJ_NT;
+    static_assert(PER * DJ_NT == MAXP, "whole counters per thread");
+    unsigned c[PER], sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      c[q] = s_cnt[PER * threadIdx.x + q];
+      sum += c[q];
+    }
     unsigned all;
-    const unsigned e = block_excl_scan<DJ_NT>(a + b, s_tmp, &all);
-    s_cnt[2 * threadIdx.x] = e;
-    s_cnt[2 * threadIdx.x + 1] = e + a;
+    unsigned e = block_excl_scan<DJ_NT>(sum, s_tmp, &all);
     if (threadIdx.x == 0) s_total = all;
-    const int p0 = 2 * int(threadIdx.x);
-    if (p0 < P) starts[size_t(p0) * ntiles + t] = (unsigned short)e;
-    if (p0 + 1 < P) starts[size_t(p0 + 1) * ntiles + t] = (unsigned short)(e + a);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+      const int p = PER * int(threadIdx.x) + q;
+      s_cnt[p] = e;
+      if (p < P) starts[size_t(p) * ntiles + t] = (unsigned short)e;
+      e += c[q];
+    }
     if (threadIdx.x == 0) starts[size_t(P) * ntiles + t] = (unsigned short)all;
   }
   __syncthreads();
@@ -247,7 +255,11 @@ __global__ __launch_bounds__(DJ_NT) void dj_part_kernel(
 // of lengths exceeds r.  The rows of a lane only grow, so each lane keeps a
 // cursor on that tile and moves it forward, the whole wave one step at a
 // time (about two steps per 64 rows).
-template <int U, class F>
+//
+// SEARCH: a lane finds its tile by a binary search over the 64 inclusive
+// sums instead (six shuffles a row, whatever the segment lengths), for
+// segments much shorter than a wave.
+template <int U, bool SEARCH = false, class F>
 __device__ __forceinline__ void dj_for_rows(const unsigned short* __restrict__ starts,
                                             size_t ntiles, int p, size_t t0, size_t t1,
                                             unsigned wave, unsigned nwaves, F&& f) {
@@ -272,13 +284,24 @@ __device__ __forceinline__ void dj_for_rows(const unsigned short* __restrict__ s
 #pragma unroll
       for (int u = 0; u < U; u++) {
         const unsigned r = r0 + unsigned(u) * 64 + lane;
-        while (true) {
-          // every lane takes part in the shuffle (a lane left out of it
-          // would hand 0 to the lanes reading from it)
-          const unsigned nx = unsigned(__shfl(incl, int(cur), 64));
-          const bool adv = cur < 63 && nx <= r;
-          if (!__any(adv)) break;
-          cur += adv ? 1u : 0u;
+        if constexpr (SEARCH) {
+          // lanes before the tile of row r: incl <= r (every lane shuffles)
+          unsigned idx = 0;
+#pragma unroll
+          for (unsigned step = 32; step > 0; step >>= 1) {
+            const unsigned nx = unsigned(__shfl(incl, int(idx + step - 1), 64));
+            if (nx <= r) idx += step;
+          }
+          cur = idx < 63 ? idx : 63u;
+        } else {
+          while (true) {
+            // every lane takes part in the shuffle (a lane left out of it
+            // would hand 0 to the lanes reading from it)
+            const unsigned nx = unsigned(__shfl(incl, int(cur), 64));
+            const bool adv = cur < 63 && nx <= r;
+            if (!__any(adv)) break;
+            cur += adv ? 1u : 0u;
+          }
         }
         in[u] = r < total;
         const unsigned sh = __shfl(shift, int(cur), 64);
@@ -336,16 +359,74 @@ __global__ __launch_bounds__(DJ_NT) void dj_build_kernel(
   }
 }
 
+// One workgroup per part of at most 2^DJ_LSBITS offsets, built in LDS:
+// every slot starts empty, each row claims its slot with an LDS
+// compare-and-swap (a slot already taken is a repeated key: *dup is set),
+// and the slice is then written out in order, empty slots included.  The
+// values are 4-byte row positions, never the empty marker.
+constexpr int DJ_LSBITS = 15;
+constexpr int DJ_LMAXP = 4096;
+
+__global__ __launch_bounds__(DJ_NT) void dj_lds_build_kernel(
+    const unsigned* __restrict__ o_in, const unsigned* __restrict__ v_in,
+    const unsigned short* __restrict__ starts, size_t ntiles, unsigned long long R,
+    unsigned* __restrict__ table, unsigned* __restrict__ dup) {
+  constexpr unsigned S = 1u << DJ_LSBITS;
+  __shared__ __align__(16) unsigned s_slice[S];
+  __shared__ unsigned s_dup;
+  const int p = blockIdx.x;
+  const unsigned long long lo = (unsigned long long)p << DJ_LSBITS;
+  const unsigned len = unsigned(std::min<unsigned long long>(R - lo, S));
+  for (unsigned i = threadIdx.x; i < S; i += DJ_NT) s_slice[i] = ~0u;
+  if (threadIdx.x == 0) s_dup = 0;
+  __syncthreads();
+  unsigned d = 0;
+  dj_for_rows<4, true>(starts, ntiles, p, 0, ntiles, threadIdx.x >> 6, DJ_NT / 64,
+                       [&](const size_t* pos, const bool* in) {
+                         unsigned o[4], v[4];
+#pragma unroll
+                         for (int u = 0; u < 4; u++) {
+                           o[u] = o_in[pos[u]];
+                           v[u] = v_in[pos[u]];
+                         }
+#pragma unroll
+                         for (int u = 0; u < 4; u++) {
+                           if (!in[u]) continue;
+                           const unsigned l = o[u] - unsigned(lo);
+                           d |= unsigned(atomicCAS(&s_slice[l], ~0u, v[u]) != ~0u);
+                         }
+                       });
+  if (d) atomicOr(&s_dup, 1u);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_dup) atomicOr(dup, 1u);
+  unsigned* out = table + lo;
+  if (len == S) {
+    const uint4* src = reinterpret_cast<const uint4*>(s_slice);
+    uint4* dst = reinterpret_cast<uint4*>(out);
+    for (unsigned i = threadIdx.x; i < S / 4; i += DJ_NT) dst[i] = src[i];
+  } else {
+    for (unsigned i = threadIdx.x; i < len; i += DJ_NT) out[i] = s_slice[i];
+  }
+}
+
+// What a slice pass does with each row of its part.
+enum DjMode {
+  DJ_PROBE = 0,    // res[pos] = table[o]
+  DJ_SCATTER = 1,  // table[o] = v[pos]
+  DJ_VERIFY = 2,   // *flag = 1 when table[o] != v[pos] (another row took the slot)
+};
+
 // Workgroup w takes part 8 * (g / C) + w % 8 (g = w / 8) and the tile chunk
 // g % C: the C chunks of one part run on one XCD under the round-robin
-// dispatch of workgroups to XCDs, so that part's table slice is read from
-// that XCD's L2.  (Only the speed depends on the dispatch.)
-template <class E>
-__global__ __launch_bounds__(DJ_PNT) void dj_probe_kernel(const unsigned* __restrict__ o_in,
+// dispatch of workgroups to XCDs, so that part's table slice is read and
+// written in that XCD's L2.  (Only the speed depends on the dispatch.)
+template <class E, int MODE>
+__global__ __launch_bounds__(DJ_PNT) void dj_slice_kernel(const unsigned* __restrict__ o_in,
                                                           const unsigned short* __restrict__ starts,
                                                           size_t ntiles, int P, int C,
-                                                          const E* __restrict__ table,
-                                                          E* __restrict__ res) {
+                                                          E* __restrict__ table,
+                                                          E* __restrict__ vals,
+                                                          unsigned* __restrict__ flag) {
   const unsigned w = blockIdx.x;
   const unsigned g = w >> 3;
   const int p = int((g / unsigned(C)) * 8 + (w & 7));
@@ -354,19 +435,40 @@ __global__ __launch_bounds__(DJ_PNT) void dj_probe_kernel(const unsigned* __rest
   const size_t t0 = size_t(g % unsigned(C)) * chunk;
   const size_t t1 = std::min(ntiles, t0 + chunk);
   if (t0 >= t1) return;
+  unsigned bad = 0;
   dj_for_rows<DJ_PU>(starts, ntiles, p, t0, t1, threadIdx.x >> 6, DJ_PNT / 64,
                      [&](const size_t* pos, const bool* in) {
                        unsigned o[DJ_PU];
 #pragma unroll
                        for (int u = 0; u < DJ_PU; u++)
                          o[u] = __builtin_nontemporal_load(o_in + pos[u]);
-                       E v[DJ_PU];
+                       if constexpr (MODE == DJ_PROBE) {
+                         E v[DJ_PU];
 #pragma unroll
-                       for (int u = 0; u < DJ_PU; u++) v[u] = table[in[u] ? o[u] : 0u];
+                         for (int u = 0; u < DJ_PU; u++) v[u] = table[in[u] ? o[u] : 0u];
 #pragma unroll
-                       for (int u = 0; u < DJ_PU; u++)
-                         if (in[u]) __builtin_nontemporal_store(v[u], res + pos[u]);
+                         for (int u = 0; u < DJ_PU; u++)
+                           if (in[u]) __builtin_nontemporal_store(v[u], vals + pos[u]);
+                       } else {
+                         E v[DJ_PU];
+#pragma unroll
+                         for (int u = 0; u < DJ_PU; u++) v[u] = __builtin_nontemporal_load(vals + pos[u]);
+                         if constexpr (MODE == DJ_SCATTER) {
+#pragma unroll
+                           for (int u = 0; u < DJ_PU; u++)
+                             if (in[u]) table[o[u]] = v[u];
+                         } else {
+                           E c[DJ_PU];
+#pragma unroll
+                           for (int u = 0; u < DJ_PU; u++) c[u] = table[in[u] ? o[u] : 0u];
+#pragma unroll
+                           for (int u = 0; u < DJ_PU; u++) bad |= unsigned(in[u] && c[u] != v[u]);
+                         }
+                       }
                      });
+  if constexpr (MODE == DJ_VERIFY) {
+    if (__any(bad) && __lane_id() == 0) atomicOr(flag, 1u);
+  }
 }
 
 // The right row id behind a stored value.
@@ -375,26 +477,28 @@ __device__ __forceinline__ unsigned long long dj_rid(E v, unsigned long long rba
   return sizeof(E) == 4 ? rbase + (unsigned long long)v : (unsigned long long)v;
 }
 
-// Emit tiles of DJ_ETILE rows (half a partition tile) in order (ticket):
-// the hits of a tile are written as (left_idx, right_idx) pairs after those
-// of every earlier tile (decoupled look-back); pairs at positions >=
-// capacity are counted, not written.  The row ids are loaded with the
-// positions, before the look-back, so that every load of a tile is in
-// flight at once.
+// Partition tiles in order (ticket): the tile's results are first copied
+// into LDS with coalesced loads, so that each row's result is read back
+// through its 16-bit position from LDS, not from L2; the hits are written
+// as (left_idx, right_idx) pairs after those of every earlier tile
+// (decoupled look-back); pairs at positions >= capacity are counted, not
+// written.  The left row ids are loaded with the positions, before the
+// look-back.  Rows go in two rounds of DJ_EITEMS per thread.
 constexpr int DJ_EITEMS = 8;
-constexpr int DJ_ETILE = DJ_NT * DJ_EITEMS;
-static_assert(DJ_TILE % DJ_ETILE == 0, "an emit tile lies in one partition tile");
+constexpr int DJ_EROUNDS = DJ_ITEMS / DJ_EITEMS;
 
 template <class E>
 __global__ __launch_bounds__(DJ_NT) void dj_emit_kernel(
-    const unsigned short* __restrict__ dest, size_t n, const E* __restrict__ res, E empty,
-    int check_empty, unsigned long long rbase, const size_t* __restrict__ lidx,
-    size_t* __restrict__ lo, size_t* __restrict__ ro, size_t capacity,
-    unsigned long long* tile_state, unsigned* ticket, unsigned long long* total) {
+    const unsigned short* __restrict__ dest, size_t n, const E* __restrict__ res,
+    const unsigned short* __restrict__ tile_cnt, E empty, int check_empty,
+    unsigned long long rbase, const size_t* __restrict__ lidx, size_t* __restrict__ lo,
+    size_t* __restrict__ ro, size_t capacity, unsigned long long* tile_state, unsigned* ticket,
+    unsigned long long* total) {
   constexpr int WAVES = DJ_NT / 64;
-  constexpr int COUNTS = DJ_EITEMS * WAVES;
+  constexpr int COUNTS = DJ_ITEMS * WAVES;
   constexpr int PER_LANE = COUNTS / 64;
   static_assert(COUNTS % 64 == 0, "the counts split evenly over one wave");
+  __shared__ __align__(16) E s_res[DJ_TILE];
   __shared__ unsigned s_cnt[COUNTS];
   __shared__ unsigned s_tile;
   __shared__ unsigned long long s_excl;
@@ -403,38 +507,51 @@ __global__ __launch_bounds__(DJ_NT) void dj_emit_kernel(
   if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
   __syncthreads();
   const unsigned tile = s_tile;
-  const size_t base = size_t(tile) * DJ_ETILE;
-  const unsigned rows = unsigned(std::min<size_t>(DJ_ETILE, n - base));
+  const size_t base = size_t(tile) * DJ_TILE;
+  const unsigned rows = unsigned(std::min<size_t>(DJ_TILE, n - base));
   const unsigned short* dt = dest + base;
   const size_t* lt = lidx + base;
-  const E* rt = res + (base / DJ_TILE) * DJ_TILE;  // the partition tile's results
-  unsigned dp[DJ_EITEMS];
-  unsigned long long lid[DJ_EITEMS];
-  if (rows == DJ_ETILE) {
+  unsigned long long hit[DJ_ITEMS];
+  unsigned dpk[DJ_ITEMS / 2];  // two 16-bit positions per word
+  // every position load issued at once (a clamped index, no branch)
+  unsigned short dl[DJ_ITEMS];
 #pragma unroll
-    for (int j = 0; j < DJ_EITEMS; j++) {
-      const unsigned i = unsigned(j) * DJ_NT + threadIdx.x;
-      dp[j] = dt[i];
-      lid[j] = lt[i];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < DJ_EITEMS; j++) {
-      const unsigned i = unsigned(j) * DJ_NT + threadIdx.x;
-      dp[j] = i < rows ? unsigned(dt[i]) : unsigned(DJ_MISS);
-      lid[j] = i < rows ? lt[i] : 0ull;
-    }
+  for (int j = 0; j < DJ_ITEMS; j++) {
+    const unsigned i = unsigned(j) * DJ_NT + threadIdx.x;
+    dl[j] = dt[i < rows ? i : 0u];
   }
-  E v[DJ_EITEMS];
+  {
+    // the tile's results into LDS, 16 bytes per load, all loads in flight
+    constexpr int NV = DJ_TILE * int(sizeof(E)) / 16 / DJ_NT;
+    const unsigned nvec = (unsigned(tile_cnt[tile]) * unsigned(sizeof(E)) + 15u) / 16u;
+    const uint4* rt = reinterpret_cast<const uint4*>(res + base);
+    uint4* sr = reinterpret_cast<uint4*>(s_res);
+    uint4 rv[NV];
 #pragma unroll
-  for (int j = 0; j < DJ_EITEMS; j++) v[j] = rt[dp[j] != DJ_MISS ? dp[j] : 0u];
-  unsigned long long hit[DJ_EITEMS];
+    for (int k = 0; k < NV; k++) {
+      const unsigned i = unsigned(k) * DJ_NT + threadIdx.x;
+      rv[k] = rt[i < nvec ? i : 0u];
+    }
+    // (entries past the tile's results hold copies of its first ones; no
+    // row reads them)
 #pragma unroll
-  for (int j = 0; j < DJ_EITEMS; j++)
-    hit[j] = __ballot(dp[j] != DJ_MISS && !(check_empty && v[j] == empty));
+    for (int k = 0; k < NV; k++) sr[unsigned(k) * DJ_NT + threadIdx.x] = rv[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < DJ_ITEMS; j++) {
+    const unsigned i = unsigned(j) * DJ_NT + threadIdx.x;
+    const unsigned dp = i < rows ? unsigned(dl[j]) : unsigned(DJ_MISS);
+    if (j & 1)
+      dpk[j >> 1] |= dp << 16;
+    else
+      dpk[j >> 1] = dp;
+    const E v = s_res[dp != DJ_MISS ? dp : 0u];
+    hit[j] = __ballot(dp != DJ_MISS && !(check_empty && v == empty));
+  }
   if (lane == 0) {
 #pragma unroll
-    for (int j = 0; j < DJ_EITEMS; j++) s_cnt[j * WAVES + wave] = unsigned(__popcll(hit[j]));
+    for (int j = 0; j < DJ_ITEMS; j++) s_cnt[j * WAVES + wave] = unsigned(__popcll(hit[j]));
   }
   __syncthreads();
   if (wave == 0) {
@@ -462,12 +579,25 @@ __global__ __launch_bounds__(DJ_NT) void dj_emit_kernel(
   __syncthreads();
   const unsigned long long excl = s_excl;
 #pragma unroll
-  for (int j = 0; j < DJ_EITEMS; j++) {
-    if ((hit[j] >> lane) & 1ull) {
-      const size_t pos = size_t(excl + s_cnt[j * WAVES + wave] + lane_rank(hit[j]));
-      if (pos < capacity) {
-        lo[pos] = lid[j];
-        ro[pos] = dj_rid(v[j], rbase);
+  for (int h = 0; h < DJ_EROUNDS; h++) {
+    // the left row ids of this round's hits, all loads issued together
+    unsigned long long lid[DJ_EITEMS];
+#pragma unroll
+    for (int q = 0; q < DJ_EITEMS; q++) {
+      const int j = h * DJ_EITEMS + q;
+      const unsigned i = unsigned(j) * DJ_NT + threadIdx.x;
+      lid[q] = lt[i < rows ? i : 0u];
+    }
+#pragma unroll
+    for (int q = 0; q < DJ_EITEMS; q++) {
+      const int j = h * DJ_EITEMS + q;
+      if ((hit[j] >> lane) & 1ull) {
+        const size_t pos = size_t(excl + s_cnt[j * WAVES + wave] + lane_rank(hit[j]));
+        if (pos < capacity) {
+          const unsigned dp = (dpk[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
+          lo[pos] = lid[q];
+          ro[pos] = dj_rid(s_res[dp], rbase);
+        }
       }
     }
   }
@@ -540,41 +670,91 @@ bool dj_plan(const K* right, const size_t* ridx, size_t nr, DenseTable& t, bool&
   return t.P <= DJ_MAXP;
 }
 
+// Workgroups per part of a slice pass over nt tiles: join.dense_chunks, but
+// at least 64 tiles for each wave of a workgroup.
+inline int dj_chunks(size_t nt) {
+  const long long c = tune_get("join.dense_chunks", 128);
+  const long long most = std::max<long long>(1, (long long)(nt / (64 * (DJ_PNT / 64))));
+  return int(std::max<long long>(1, std::min<long long>(c, most)));
+}
+
 // Builds the table over RIGHT; false when a key repeats.
 template <class K, class E>
 bool dj_build(const K* right, const size_t* ridx, size_t nr, DenseTable& t, hipStream_t st) {
   const size_t nt = ceil_div(nr, size_t(DJ_TILE));
+  // 4-byte positions over a range of at most 2^27 offsets: parts of 2^15
+  // offsets built in LDS; else the probe's parts
+  const bool lds = sizeof(E) == 4 && t.R <= (unsigned long long)DJ_LMAXP << DJ_LSBITS &&
+                   tune_get("join.dense_lds_build", 1) != 0;
+  const int sb = lds ? DJ_LSBITS : t.sbits;
+  const int P = lds ? int(ceil_div(size_t(t.R), size_t(1) << DJ_LSBITS)) : t.P;
   DBuf o(nt * DJ_TILE * sizeof(unsigned)), v(nt * DJ_TILE * sizeof(E));
-  DBuf starts((size_t(t.P) + 1) * nt * sizeof(unsigned short));
+  DBuf starts((size_t(P) + 1) * nt * sizeof(unsigned short));
   {
     ProfScope prof("join_dense_build_part", st);
-    dj_part_kernel<K, E, true><<<unsigned(ceil_div(nt, 8) * 8), DJ_NT, 0, st>>>(
-        right, nr, t.mn, t.R, t.sbits, t.P, nt, ridx, o.as<unsigned>(), v.as<E>(), nullptr,
-        starts.as<unsigned short>());
+    if (lds)
+      dj_part_kernel<K, E, true, DJ_LMAXP><<<unsigned(ceil_div(nt, 8) * 8), DJ_NT, 0, st>>>(
+          right, nr, t.mn, t.R, sb, P, nt, ridx, o.as<unsigned>(), v.as<E>(), nullptr,
+          starts.as<unsigned short>());
+    else
+      dj_part_kernel<K, E, true><<<unsigned(ceil_div(nt, 8) * 8), DJ_NT, 0, st>>>(
+          right, nr, t.mn, t.R, sb, P, nt, ridx, o.as<unsigned>(), v.as<E>(), nullptr,
+          starts.as<unsigned short>());
     FHIP_CHECK(hipGetLastError());
   }
   ProfScope prof("join_dense_build", st);
   t.table.reset(size_t(t.R) * sizeof(E));
   DBuf dup(sizeof(unsigned));
   FHIP_CHECK(hipMemsetAsync(dup.get(), 0, sizeof(unsigned), st));
-  dj_build_kernel<E><<<unsigned(t.P), DJ_NT, 0, st>>>(
-      o.as<unsigned>(), v.as<E>(), starts.as<unsigned short>(), nt, t.sbits, t.R, t.table.as<E>(),
-      E(~E(0)), t.check_empty, dup.as<unsigned>());
-  FHIP_CHECK(hipGetLastError());
+  if constexpr (sizeof(E) == 4) {
+    if (lds) {
+      dj_lds_build_kernel<<<unsigned(P), DJ_NT, 0, st>>>(
+          o.as<unsigned>(), v.as<unsigned>(), starts.as<unsigned short>(), nt, t.R,
+          t.table.as<unsigned>(), dup.as<unsigned>());
+      FHIP_CHECK(hipGetLastError());
+    } else {
+      // 4-byte positions (unique per row): scatter, then check every row
+      // still owns its slot (a repeated key lost one), both XCD-aware like
+      // the probe
+      if (t.check_empty)
+        FHIP_CHECK(hipMemsetAsync(t.table.get(), 0xFF, size_t(t.R) * sizeof(E), st));
+      const int C = dj_chunks(nt);
+      const size_t p8 = ceil_div(size_t(P), 8) * 8;
+      {
+        ProfScope prof2("join_dense_scatter", st);
+        dj_slice_kernel<E, DJ_SCATTER><<<unsigned(p8 * size_t(C)), DJ_PNT, 0, st>>>(
+            o.as<unsigned>(), starts.as<unsigned short>(), nt, P, C, t.table.as<E>(), v.as<E>(),
+            nullptr);
+        FHIP_CHECK(hipGetLastError());
+      }
+      {
+        ProfScope prof2("join_dense_verify", st);
+        dj_slice_kernel<E, DJ_VERIFY><<<unsigned(p8 * size_t(C)), DJ_PNT, 0, st>>>(
+            o.as<unsigned>(), starts.as<unsigned short>(), nt, P, C, t.table.as<E>(), v.as<E>(),
+            dup.as<unsigned>());
+        FHIP_CHECK(hipGetLastError());
+      }
+    }
+  } else {
+    // 8-byte row ids (which may repeat): a repeated key is found by one
+    // workgroup per part marking its offsets in an LDS bitmap
+    dj_build_kernel<E><<<unsigned(P), DJ_NT, 0, st>>>(
+        o.as<unsigned>(), v.as<E>(), starts.as<unsigned short>(), nt, sb, t.R, t.table.as<E>(),
+        E(~E(0)), t.check_empty, dup.as<unsigned>());
+    FHIP_CHECK(hipGetLastError());
+  }
   unsigned* h = static_cast<unsigned*>(pinned_scratch(0, sizeof(unsigned)));
   FHIP_CHECK(hipMemcpyAsync(h, dup.get(), sizeof(unsigned), hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
   return *h == 0;
 }
 
-// Partitions LEFT and probes the table: dest (nl) and res (tiles x DJ_TILE)
-// hold every left row's result.
 template <class K, class E>
 void dj_probe(const DenseTable& t, const K* left, size_t nl, DBuf& dest, DBuf& res,
-              hipStream_t st) {
+              DBuf& starts, hipStream_t st) {
   const size_t nt = ceil_div(nl, size_t(DJ_TILE));
   DBuf o(nt * DJ_TILE * sizeof(unsigned));
-  DBuf starts((size_t(t.P) + 1) * nt * sizeof(unsigned short));
+  starts.reset((size_t(t.P) + 1) * nt * sizeof(unsigned short));
   dest.reset(nl * sizeof(unsigned short));
   res.reset(nt * DJ_TILE * sizeof(E));
   {
@@ -584,13 +764,13 @@ void dj_probe(const DenseTable& t, const K* left, size_t nl, DBuf& dest, DBuf& r
         dest.as<unsigned short>(), starts.as<unsigned short>());
     FHIP_CHECK(hipGetLastError());
   }
-  const long long cdef = tune_get("join.dense_chunks", 128);
-  const int C = int(std::max<long long>(1, std::min<long long>(std::max(1LL, cdef), (long long)nt)));
+  const int C = dj_chunks(nt);
   const size_t p8 = ceil_div(size_t(t.P), 8) * 8;
   {
     ProfScope prof("join_dense_probe", st);
-    dj_probe_kernel<E><<<unsigned(p8 * size_t(C)), DJ_PNT, 0, st>>>(
-        o.as<unsigned>(), starts.as<unsigned short>(), nt, t.P, C, t.table.as<E>(), res.as<E>());
+    dj_slice_kernel<E, DJ_PROBE><<<unsigned(p8 * size_t(C)), DJ_PNT, 0, st>>>(
+        o.as<unsigned>(), starts.as<unsigned short>(), nt, t.P, C, t.table.as<E>(), res.as<E>(),
+        nullptr);
     FHIP_CHECK(hipGetLastError());
   }
 }
@@ -608,8 +788,8 @@ int dj_run(const K* left, const size_t* lidx, size_t nl, const K* right, const s
     }
     return DJ_DONE;
   }
-  DBuf dest, res;
-  dj_probe<K, E>(t, left, nl, dest, res, st);
+  DBuf dest, res, starts;  // starts row P: the results in each tile
+  dj_probe<K, E>(t, left, nl, dest, res, starts, st);
   const E empty = E(~E(0));
   if (vals) {
     vals->reset(nl * sizeof(unsigned long long));
@@ -622,7 +802,7 @@ int dj_run(const K* left, const size_t* lidx, size_t nl, const K* right, const s
     FHIP_CHECK(hipGetLastError());
     return DJ_DONE;
   }
-  const size_t nt = ceil_div(nl, size_t(DJ_ETILE));
+  const size_t nt = ceil_div(nl, size_t(DJ_TILE));
   FHIP_REQUIRE(nt < (size_t(1) << 31), FHIP_ERR_TOO_LARGE, "equi_join: too many probe rows");
   DBuf ws((nt + 2) * sizeof(unsigned long long));
   unsigned long long* state = ws.as<unsigned long long>();
@@ -630,8 +810,9 @@ int dj_run(const K* left, const size_t* lidx, size_t nl, const K* right, const s
   {
     ProfScope prof("join_dense_emit", st);
     dj_emit_kernel<E><<<unsigned(nt), DJ_NT, 0, st>>>(
-        dest.as<unsigned short>(), nl, res.as<E>(), empty, t.check_empty, t.rbase, lidx, lo, ro,
-        capacity, state, reinterpret_cast<unsigned*>(state + nt), state + nt + 1);
+        dest.as<unsigned short>(), nl, res.as<E>(),
+        starts.as<unsigned short>() + size_t(t.P) * nt, empty, t.check_empty, t.rbase, lidx, lo,
+        ro, capacity, state, reinterpret_cast<unsigned*>(state + nt), state + nt + 1);
     FHIP_CHECK(hipGetLastError());
   }
   unsigned long long* h = static_cast<unsigned long long*>(pinned_scratch(0, 8));

@@ -98,6 +98,8 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *   "join.dense_slice_bits", "join.dense_chunks"  key offsets per part of the
  *       direct table's radix partitioning (2^bits, 10 to 18, default 18) and
  *       the workgroups that probe one part (default 128);
+ *       "join.dense_lds_build" 0 builds a table of row positions by scatter
+ *       and check instead of in LDS parts of 2^15 offsets (default 1);
  *   "radix.hist_bpc", "radix.hist_mode"  workgroups per CU of the radix
  *       histogram pass (1 to 64; by default about 16384 keys per workgroup,
  *       8 to 64 workgroups per CU), and how a wave counts its digits: one

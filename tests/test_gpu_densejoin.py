@@ -35,7 +35,7 @@ def dense(fa):
     autotune.tune_set("join.dense_min", 1024)
     yield autotune
     for name in ("join.dense_min", "join.dense_slice_bits", "join.dense_chunks",
-                 "join.dense_ratio"):
+                 "join.dense_ratio", "join.dense_lds_build"):
         autotune.tune_set(name, None)
 
 
@@ -124,12 +124,15 @@ def test_dense_join_build_last_wave_with_few_rows(fa, dense):
     assert np.array_equal(ro.numpy(), ero)
 
 
+@pytest.mark.parametrize("lds_build", [1, 0])
 @pytest.mark.parametrize("sbits,chunks", [(10, 1), (10, 3), (12, 128), (17, 7)])
-def test_dense_join_parts_and_chunks(fa, dense, sbits, chunks):
+def test_dense_join_parts_and_chunks(fa, dense, sbits, chunks, lds_build):
     # many parts (2^10 offsets each) and uneven tile chunks; a hot key on a
-    # third of the probe rows
+    # third of the probe rows; the table built in LDS parts or scattered
+    # into the probe's parts and verified
     dense.tune_set("join.dense_slice_bits", sbits)
     dense.tune_set("join.dense_chunks", chunks)
+    dense.tune_set("join.dense_lds_build", lds_build)
     rng = np.random.default_rng(sbits * 100 + chunks)
     for nr, nl, hot in [(100000, 250000, False), (70000, 200003, True)]:
         l, li, r, ri = _case(rng, np.int64, nr, nl, 1.3, hot=hot)
@@ -140,10 +143,12 @@ def test_dense_join_parts_and_chunks(fa, dense, sbits, chunks):
         assert np.array_equal(ro.numpy(), ero)
 
 
+@pytest.mark.parametrize("lds_build", [1, 0])
 @pytest.mark.parametrize("dt", [np.int32, np.uint64])
-def test_dense_join_repeated_key_falls_back(fa, dense, dt):
+def test_dense_join_repeated_key_falls_back(fa, dense, dt, lds_build):
     # a repeated build key: AUTO takes the sort-merge path (the reference's
     # is_unique_ok), HASH is an error
+    dense.tune_set("join.dense_lds_build", lds_build)
     rng = np.random.default_rng(5)
     l, li, r, ri = _case(rng, dt, 50000, 80000, 1.0, dup=True)
     elo, ero, eh = oracle.equi_join(l, li, r, ri, "ve")
