@@ -602,8 +602,9 @@ def main():
         step_groupby(fa, k, v, iv, model)
     torch.cuda.synchronize()
     breakdown = {}
-    for name in ("groupby_hot", "groupby_hot_merge", "groupby_part_hist",
-                 "groupby_part_scatter", "hashagg", "groupby_merge", "radix_hist", "radix_pass"):
+    for name in ("groupby_hot", "groupby_hot_merge", "groupby_range_part", "groupby_range_agg",
+                 "groupby_range_emit", "groupby_part_hist", "groupby_part_scatter", "hashagg",
+                 "groupby_merge", "radix_hist", "radix_pass"):
         ms, cnt = _prof_ms(fa, name)
         breakdown[name] = {"ms_per_step": ms / args.steps, "launches_per_step": cnt / args.steps}
     fa.check(fa.lib().fhip_prof_enable(0))

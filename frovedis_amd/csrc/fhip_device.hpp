@@ -301,6 +301,73 @@ inline void copy_strided_to_host(const void* src, size_t stride, size_t count, v
   FHIP_CHECK(hipStreamSynchronize(st));
 }
 
+// Rows of a tile-local partition: tile t (TILE rows) holds its rows grouped
+// by part at t * TILE + starts[p * ntiles + t] .. t * TILE + starts[(p + 1) *
+// ntiles + t] - 1.  Calls f(pos[U], in[U]) for the rows of part p in tiles
+// [t0, t1), U rows per lane at a time; pos is 0 where in is false, so that
+// loads through it need no branch.  A wave takes 64 tiles at once (lane l
+// reads tile g + l's segment bounds) and spreads their rows over its lanes:
+// row r (the r-th of the 64 segments together) belongs to the first tile
+// whose inclusive sum of lengths exceeds r.  The rows of a lane only grow, so each lane keeps a
+// cursor on that tile and moves it forward, the whole wave one step at a
+// time (about two steps per 64 rows).
+//
+// SEARCH: a lane finds its tile by a binary search over the 64 inclusive
+// sums instead (six shuffles a row, whatever the segment lengths), for
+// segments much shorter than a wave.
+template <int TILE, int U, bool SEARCH = false, class F>
+__device__ __forceinline__ void segment_rows(const unsigned short* __restrict__ starts,
+                                            size_t ntiles, int p, size_t t0, size_t t1,
+                                            unsigned wave, unsigned nwaves, F&& f) {
+  const unsigned lane = __lane_id();
+  const unsigned short* sp = starts + size_t(p) * ntiles;
+  const unsigned short* ep = sp + ntiles;
+  for (size_t g = t0 + size_t(wave) * 64; g < t1; g += size_t(nwaves) * 64) {
+    const size_t tile = g + lane;
+    unsigned s = 0, len = 0;
+    if (tile < t1) {
+      s = sp[tile];
+      len = unsigned(ep[tile]) - s;
+    }
+    const unsigned incl = wave_incl_scan(len);
+    const unsigned total = __shfl(incl, 63, 64);
+    // row r of the lane's tile sits at tile * TILE + (s - excl) + r
+    const unsigned shift = s - (incl - len);
+    unsigned cur = 0;
+    for (unsigned r0 = 0; r0 < total; r0 += 64 * U) {
+      size_t pos[U];
+      bool in[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const unsigned r = r0 + unsigned(u) * 64 + lane;
+        if constexpr (SEARCH) {
+          // lanes before the tile of row r: incl <= r (every lane shuffles)
+          unsigned idx = 0;
+#pragma unroll
+          for (unsigned step = 32; step > 0; step >>= 1) {
+            const unsigned nx = unsigned(__shfl(incl, int(idx + step - 1), 64));
+            if (nx <= r) idx += step;
+          }
+          cur = idx < 63 ? idx : 63u;
+        } else {
+          while (true) {
+            // every lane takes part in the shuffle (a lane left out of it
+            // would hand 0 to the lanes reading from it)
+            const unsigned nx = unsigned(__shfl(incl, int(cur), 64));
+            const bool adv = cur < 63 && nx <= r;
+            if (!__any(adv)) break;
+            cur += adv ? 1u : 0u;
+          }
+        }
+        in[u] = r < total;
+        const unsigned sh = __shfl(shift, int(cur), 64);
+        pos[u] = in[u] ? (g + cur) * TILE + unsigned(sh + r) : 0;
+      }
+      f(pos, in);
+    }
+  }
+}
+
 template <class F>
 void dispatch_type(int t, F&& f) {
   switch (t) {

@@ -693,7 +693,7 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     const K* __restrict__ keys, size_t n, AccSet accs, const unsigned long long* __restrict__ hot_keys,
     int Sh, unsigned long long* __restrict__ partial, K* __restrict__ cold_keys, Cols cold,
     unsigned long long* __restrict__ cold_n, unsigned long long* __restrict__ seg_n, int cpbits,
-    unsigned long long* __restrict__ ccnt) {
+    unsigned long long* __restrict__ ccnt, unsigned long long* __restrict__ krange) {
   extern __shared__ unsigned long long lds_u64[];
   __shared__ unsigned cold_cnt;  // rows in this workgroup's cold region
   constexpr int na = NA;
@@ -735,10 +735,22 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
         vals[u][a] = r < r1 ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
     }
   };
+  // the smallest and largest key of the rows read (order-preserving bits:
+  // the sign bit flipped for signed keys), for krange
+  constexpr unsigned long long KFLIP = std::is_signed<K>::value ? (1ull << 63) : 0ull;
+  unsigned long long kmn = ~0ull, kmx = 0ull;
   for (size_t t0 = r0; t0 < r1; t0 += STEP) {
     K kv[U];
     unsigned long long vv[U][NA1];
     load_step(t0, kv, vv);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (t0 + size_t(u) * HOT_NT + tid < r1) {
+        const unsigned long long ob = key_bits(kv[u]) ^ KFLIP;
+        kmn = ob < kmn ? ob : kmn;
+        kmx = ob > kmx ? ob : kmx;
+      }
+    }
     int sl[U];
     unsigned long long cm[U];  // lanes of the wave with a cold row, per row of the step
     unsigned ncold = 0;
@@ -787,6 +799,18 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
         }
       }
       base += unsigned(__popcll(cm[u]));
+    }
+  }
+  if (krange) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const unsigned long long a = __shfl_xor(kmn, o, 64), b = __shfl_xor(kmx, o, 64);
+      kmn = a < kmn ? a : kmn;
+      kmx = b > kmx ? b : kmx;
+    }
+    if (lane == 0 && kmn <= kmx) {
+      atomicMin(&krange[0], kmn);
+      atomicMax(&krange[1], kmx);
     }
   }
   __syncthreads();
@@ -1336,6 +1360,407 @@ void group_by_agg_sorted(int ktype, const void* keys, size_t n, const fhip_agg* 
 // Sorted-by-key result of a fused group-by; see fhip_dev_group_by_agg.
 // heavy (optional): the column's heavy keys known in advance (a model of the
 // column); then no sample of the input is counted for the heavy-key table
+namespace {
+
+// ---- range path: group keys in a narrow range ------------------------------
+// When the heavy-key pass finds all keys within R <= min(rows, 2^21)
+// consecutive values, the cold rows are partitioned by key offset instead of
+// by hash, and every part's table is an array indexed by the offset:
+//   rp_part: each 16384-row tile of a cold region ranks its rows by part
+//     (key offset k goes to part k mod P, at index k / P; P a power of two,
+//     at most 1024, parts of at most 2^pkb keys) in LDS and writes the
+//     16-bit index and the value columns in part order to the tile's own
+//     slots, with the start of every part in the tile (part-major, as the
+//     dense join's partition does);
+//   rp_agg: workgroup (part, chunk of tiles) aggregates its rows into LDS
+//     arrays indexed by the index (no hashing, no key compare, no
+//     overflow) and writes them out per chunk;
+//   rp_combine: per key, the chunks' arrays are combined (in chunk order)
+//     into one entry {key, count, accumulators} and flagged when present;
+//     rp_hot then places the heavy keys' entries;
+//   the flags' prefix sum ranks the present keys, which are already in key
+//   order: no sort of the group keys.
+constexpr int RP_NT = 1024;
+constexpr int RP_ITEMS = 16;
+constexpr int RP_TILE = RP_NT * RP_ITEMS;  // 16384 rows: tile positions fit 16 bits
+constexpr int RP_MAXP = 1024;              // parts at most (one counter per thread)
+constexpr size_t RP_TPR = HOT_CHUNK / RP_TILE;  // tiles per heavy-key region
+static_assert(HOT_CHUNK % RP_TILE == 0, "a heavy-key region is whole tiles");
+constexpr int RP_ANT = 512;                // threads of the aggregation kernel
+constexpr int RP_AU = 4;                   // rows per lane in flight (aggregation)
+constexpr size_t RP_LDS = size_t(64) << 10;  // LDS of one part's arrays
+constexpr unsigned long long RP_MAXR = 1ull << 21;
+
+// Tile t covers rows t * RP_TILE .. of the cold buffers; region t / RP_TPR
+// holds seg_n[region] rows, so the tile's rows are the first `rows` of it.
+template <class K, int NA>
+__global__ __launch_bounds__(RP_NT) void rp_part_kernel(
+    const K* __restrict__ ckeys, Cols cold, const unsigned long long* __restrict__ seg_n,
+    size_t ntiles, unsigned long long kmin, int P, unsigned short* __restrict__ lk_out,
+    Cols pout, unsigned short* __restrict__ starts) {
+  __shared__ __align__(16) unsigned char s_stage[size_t(RP_TILE) * 8];
+  __shared__ unsigned s_cnt[RP_MAXP];
+  __shared__ unsigned s_tmp[RP_NT / 64];
+  __shared__ unsigned s_total;
+  // tiles t .. t + per - 1 on one XCD, so neighbouring tiles' start words
+  // (which share cache lines) are combined in one L2
+  const size_t per = (ntiles + 7) / 8;
+  const size_t t = size_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (t >= ntiles) return;
+  const size_t off = (t % RP_TPR) * RP_TILE;
+  const unsigned long long have = seg_n[t / RP_TPR];
+  const unsigned rows = have > off ? unsigned(std::min<unsigned long long>(RP_TILE, have - off)) : 0u;
+  const size_t base = t * RP_TILE;
+  // loads go through clamped indices (no branch); a tile with no rows reads
+  // row 0 of the buffers, which always exists
+  const size_t lbase = rows ? base : 0;
+  const K* kt = ckeys + lbase;
+  for (int p = threadIdx.x; p < RP_MAXP; p += RP_NT) s_cnt[p] = 0;
+  // offset k: part k mod P, index k / P within the part (P a power of two),
+  // so every part holds an even share of each stretch of keys (a skewed
+  // column's densest stretch would otherwise fill a few parts)
+  const unsigned long long pmask = (unsigned long long)(P - 1);
+  const int ppb = __builtin_ctz(unsigned(P));
+  unsigned kl[RP_ITEMS / 2];  // two 16-bit indexes within the part per word
+  unsigned dr[RP_ITEMS];      // part << 16 | rank within the part, ~0u past the rows
+  {
+    unsigned long long kb[RP_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RP_ITEMS; j++) {
+      const unsigned i = unsigned(j) * RP_NT + threadIdx.x;
+      kb[j] = key_bits(kt[i < rows ? i : 0u]) - kmin;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RP_ITEMS; j++) {
+      const unsigned i = unsigned(j) * RP_NT + threadIdx.x;
+      const unsigned l = unsigned(kb[j] >> ppb);
+      if (j & 1) kl[j >> 1] |= l << 16;
+      else kl[j >> 1] = l;
+      dr[j] = ~0u;
+      if (i < rows) {
+        const unsigned d = unsigned(kb[j] & pmask);
+        dr[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const unsigned c = s_cnt[threadIdx.x];
+    unsigned all;
+    const unsigned e = block_excl_scan<RP_NT>(c, s_tmp, &all);
+    s_cnt[threadIdx.x] = e;
+    if (int(threadIdx.x) < P) starts[size_t(threadIdx.x) * ntiles + t] = (unsigned short)e;
+    if (threadIdx.x == 0) {
+      s_total = all;
+      starts[size_t(P) * ntiles + t] = (unsigned short)all;
+    }
+  }
+  __syncthreads();
+  const unsigned cnt = s_total;
+  // the offsets within the parts, then every value column, each staged in
+  // part order and written out in one run
+  {
+    unsigned short* sk = reinterpret_cast<unsigned short*>(s_stage);
+#pragma unroll
+    for (int j = 0; j < RP_ITEMS; j++)
+      if (dr[j] != ~0u)
+        sk[s_cnt[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = (unsigned short)(kl[j >> 1] >> ((j & 1) * 16));
+    __syncthreads();
+    unsigned short* lo = lk_out + base;
+    for (unsigned i = threadIdx.x; i < cnt; i += RP_NT) lo[i] = sk[i];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    if (cold.bytes[a] == 4) {
+      const unsigned* src = static_cast<const unsigned*>(cold.in[a]) + lbase;
+      unsigned* sv = reinterpret_cast<unsigned*>(s_stage);
+      unsigned v[RP_ITEMS];
+#pragma unroll
+      for (int j = 0; j < RP_ITEMS; j++) {
+        const unsigned i = unsigned(j) * RP_NT + threadIdx.x;
+        v[j] = src[i < rows ? i : 0u];
+      }
+#pragma unroll
+      for (int j = 0; j < RP_ITEMS; j++)
+        if (dr[j] != ~0u) sv[s_cnt[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = v[j];
+      __syncthreads();
+      unsigned* dst = static_cast<unsigned*>(pout.out[a]) + base;
+      for (unsigned i = threadIdx.x; i < cnt; i += RP_NT) dst[i] = sv[i];
+    } else {
+      const unsigned long long* src = static_cast<const unsigned long long*>(cold.in[a]) + lbase;
+      unsigned long long* sv = reinterpret_cast<unsigned long long*>(s_stage);
+      unsigned long long v[RP_ITEMS];
+#pragma unroll
+      for (int j = 0; j < RP_ITEMS; j++) {
+        const unsigned i = unsigned(j) * RP_NT + threadIdx.x;
+        v[j] = src[i < rows ? i : 0u];
+      }
+#pragma unroll
+      for (int j = 0; j < RP_ITEMS; j++)
+        if (dr[j] != ~0u) sv[s_cnt[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = v[j];
+      __syncthreads();
+      unsigned long long* dst = static_cast<unsigned long long*>(pout.out[a]) + base;
+      for (unsigned i = threadIdx.x; i < cnt; i += RP_NT) dst[i] = sv[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Workgroup w aggregates part w / C over the tile chunk w % C into LDS
+// arrays indexed by the index within the part (count: 32 bits; one 64-bit
+// word per accumulator) and writes them to chunk w % C's arrays, part-major
+// (pcnt[c * P * 2^pkb + p * 2^pkb + index], pacc likewise per accumulator).
+template <int NA>
+__global__ __launch_bounds__(RP_ANT) void rp_agg_kernel(
+    const unsigned short* __restrict__ lk_in, AccSet accs,
+    const unsigned short* __restrict__ starts, size_t ntiles, int P, int C, int pkb,
+    unsigned* __restrict__ pcnt, unsigned long long* __restrict__ pacc) {
+  extern __shared__ unsigned long long lds_u64[];
+  const int PK = 1 << pkb;
+  unsigned long long* acc = lds_u64;                          // [NA][PK]
+  unsigned* cnt = reinterpret_cast<unsigned*>(acc + size_t(NA) * PK);  // [PK]
+  const int p = int(blockIdx.x) / C, c = int(blockIdx.x) % C;
+  for (int i = threadIdx.x; i < PK; i += RP_ANT) {
+    cnt[i] = 0;
+#pragma unroll
+    for (int a = 0; a < NA; a++) acc[size_t(a) * PK + i] = acc_identity(accs.a[a].kind, accs.a[a].vtype);
+  }
+  __syncthreads();
+  const size_t chunk = (ntiles + size_t(C) - 1) / size_t(C);
+  const size_t t0 = size_t(c) * chunk;
+  const size_t t1 = std::min(ntiles, t0 + chunk);
+  constexpr int NA1 = NA > 0 ? NA : 1;
+  if (t0 < t1)
+    segment_rows<RP_TILE, RP_AU>(starts, ntiles, p, t0, t1, threadIdx.x >> 6, RP_ANT / 64,
+                                 [&](const size_t* pos, const bool* in) {
+                                   unsigned l[RP_AU];
+                                   unsigned long long v[RP_AU][NA1];
+#pragma unroll
+                                   for (int u = 0; u < RP_AU; u++) {
+                                     l[u] = lk_in[pos[u]];
+#pragma unroll
+                                     for (int a = 0; a < NA; a++)
+                                       v[u][a] = load_bits(accs.a[a].col, accs.a[a].vtype, pos[u]);
+                                   }
+#pragma unroll
+                                   for (int u = 0; u < RP_AU; u++) {
+                                     if (!in[u]) continue;
+                                     atomicAdd(&cnt[l[u]], 1u);
+#pragma unroll
+                                     for (int a = 0; a < NA; a++)
+                                       acc_merge(&acc[size_t(a) * PK + l[u]], accs.a[a].kind,
+                                                 accs.a[a].vtype, v[u][a]);
+                                   }
+                                 });
+  __syncthreads();
+  // part-major: slot i of part p goes to [c][p * PK + i] (coalesced); the
+  // combine kernel reads it back by key offset
+  const size_t RP = size_t(P) << pkb;
+  const size_t k0 = size_t(p) << pkb;
+  for (int i = threadIdx.x; i < PK; i += RP_ANT) {
+    pcnt[size_t(c) * RP + k0 + i] = cnt[i];
+#pragma unroll
+    for (int a = 0; a < NA; a++) pacc[(size_t(c) * NA + a) * RP + k0 + i] = acc[size_t(a) * PK + i];
+  }
+}
+
+// Per key offset k < R (slot k / P of part k mod P): the chunks' counts and
+// accumulators combined in chunk order into entry k of `w` ({key bits,
+// count, accumulators}); flag[k] = 1 when the key occurs.
+__global__ __launch_bounds__(256) void rp_combine_kernel(
+    const unsigned* __restrict__ pcnt, const unsigned long long* __restrict__ pacc, int C,
+    unsigned long long R, int P, int pkb, AccSet accs, unsigned long long kmin, int stride,
+    unsigned long long* __restrict__ w, unsigned* __restrict__ flag) {
+  const size_t RP = size_t(P) << pkb;  // slots of one chunk's arrays
+  const int ppb = __builtin_ctz(unsigned(P));
+  for (size_t k = size_t(blockIdx.x) * 256 + threadIdx.x; k < R; k += size_t(gridDim.x) * 256) {
+    const size_t slot = ((k & size_t(P - 1)) << pkb) + (k >> ppb);
+    unsigned long long cn = 0;
+    for (int c = 0; c < C; c++) cn += pcnt[size_t(c) * RP + slot];
+    unsigned long long* e = w + k * size_t(stride);
+    e[0] = kmin + k;
+    e[1] = cn;
+    for (int a = 0; a < accs.n; a++) {
+      unsigned long long x = acc_identity(accs.a[a].kind, accs.a[a].vtype);
+      for (int c = 0; c < C; c++)
+        x = acc_combine(x, pacc[(size_t(c) * accs.n + a) * RP + slot], accs.a[a].kind,
+                        accs.a[a].vtype);
+      e[2 + a] = x;
+    }
+    flag[k] = cn != 0;
+  }
+}
+
+// The heavy keys' entries ({key bits, count, accumulators}, hot_n of them)
+// go to their key's entry of `w`.
+__global__ __launch_bounds__(256) void rp_hot_kernel(const unsigned long long* __restrict__ hot,
+                                                     unsigned hot_n, unsigned long long kmin,
+                                                     int stride, unsigned long long* __restrict__ w,
+                                                     unsigned* __restrict__ flag) {
+  for (unsigned j = blockIdx.x * 256 + threadIdx.x; j < hot_n; j += gridDim.x * 256) {
+    const unsigned long long* src = hot + size_t(j) * stride;
+    const unsigned long long k = src[0] - kmin;
+    for (int q = 0; q < stride; q++) w[k * stride + q] = src[q];
+    flag[k] = 1;
+  }
+}
+
+// ent[rank] = k for every present key offset k (incl: inclusive prefix sum
+// of the flags).
+__global__ __launch_bounds__(256) void rp_ent_kernel(const unsigned* __restrict__ flag,
+                                                     const unsigned* __restrict__ incl,
+                                                     unsigned long long R,
+                                                     size_t* __restrict__ ent) {
+  for (size_t k = size_t(blockIdx.x) * 256 + threadIdx.x; k < R; k += size_t(gridDim.x) * 256)
+    if (flag[k]) ent[incl[k] - 1] = k;
+}
+
+// The range path after the heavy-key pass (its cold regions: nregions of
+// HOT_CHUNK rows, seg_n rows each; the heavy keys' entries in hot_fin).
+// kmin: the smallest key's bits; R: the key range.
+// Offsets per part of the range path with na accumulators: the largest
+// power of two whose arrays fit RP_LDS.
+int rp_part_bits(int na) {
+  const size_t bpk = 4 + 8 * size_t(na);
+  int pkb = 0;
+  while (pkb < 16 && (size_t(2) << pkb) * bpk <= RP_LDS) pkb++;
+  return pkb;
+}
+
+void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /*unused*/,
+                        const fhip_agg* aggs, int naggs, const std::vector<int>& slot_of,
+                        const void* ckeys, const unsigned long long* seg_n, size_t nregions,
+                        const unsigned long long* hot_fin, unsigned hot_groups,
+                        unsigned long long kmin, unsigned long long R, void* keys_out,
+                        size_t capacity, size_t* ngroups, hipStream_t st, const DBuf* cold_cols) {
+  const int na = accs.n;
+  const int stride = 2 + na;
+  const size_t bpk = 4 + 8 * size_t(na);
+  const int pkb = rp_part_bits(na);
+  int P = 1;  // a power of two: key offset k goes to part k mod P
+  while ((size_t(P) << pkb) < size_t(R)) P *= 2;
+  const size_t ntiles = nregions * RP_TPR;
+  const size_t prow = ntiles * RP_TILE;  // slots of the partitioned copy
+  DBuf lk(prow * sizeof(unsigned short)), starts((size_t(P) + 1) * ntiles * sizeof(unsigned short));
+  DBuf pc[HA_MAX_ACC];
+  Cols cold{}, pout{};
+  cold.n = pout.n = na;
+  AccSet paccs = accs;
+  for (int a = 0; a < na; a++) {
+    cold.bytes[a] = pout.bytes[a] = type_bytes(accs.a[a].vtype);
+    cold.in[a] = cold_cols[a].get();
+    pc[a].reset(prow * size_t(pout.bytes[a]));
+    pout.out[a] = pc[a].get();
+    paccs.a[a].col = pc[a].get();
+  }
+  {
+    ProfScope prof("groupby_range_part", st);
+    auto launch = [&](auto kz) {
+      using KK = decltype(kz);
+      const KK* kp = static_cast<const KK*>(ckeys);
+      const unsigned g = unsigned(ceil_div(ntiles, size_t(8)) * 8);
+      switch (na) {
+        case 0: rp_part_kernel<KK, 0><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        case 1: rp_part_kernel<KK, 1><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        case 2: rp_part_kernel<KK, 2><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        case 3: rp_part_kernel<KK, 3><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        default: rp_part_kernel<KK, 4><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+      }
+    };
+    switch (ktype) {
+      case FHIP_I32: launch(int32_t()); break;
+      case FHIP_U32: launch(uint32_t()); break;
+      case FHIP_I64: launch(int64_t()); break;
+      default: launch(uint64_t()); break;
+    }
+    FHIP_CHECK(hipGetLastError());
+  }
+  // chunks per part: about 2048 workgroups, each with at least 64 tiles a wave
+  const long long cdef = tune_get("groupby.range_chunks", 0);
+  int C = cdef > 0 ? int(cdef) : int(std::max<size_t>(1, ceil_div(2048, size_t(P))));
+  C = int(std::max<size_t>(1, std::min<size_t>(size_t(C), ntiles / (64 * (RP_ANT / 64)))));
+  const size_t RP = size_t(P) << pkb;  // slots of one chunk's arrays (>= R)
+  DBuf pcnt(size_t(C) * RP * 4), pacc(size_t(C) * RP * 8 * size_t(std::max(na, 1)));
+  {
+    ProfScope prof("groupby_range_agg", st);
+    const size_t lds = (size_t(1) << pkb) * bpk;
+    static bool attr = false;
+    if (!attr) {
+      for (auto f : {(const void*)rp_agg_kernel<0>, (const void*)rp_agg_kernel<1>,
+                     (const void*)rp_agg_kernel<2>, (const void*)rp_agg_kernel<3>,
+                     (const void*)rp_agg_kernel<4>})
+        FHIP_CHECK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(RP_LDS)));
+      attr = true;
+    }
+    const unsigned g = unsigned(size_t(P) * size_t(C));
+    const unsigned short* sp = starts.as<unsigned short>();
+    unsigned* cp = pcnt.as<unsigned>();
+    unsigned long long* ap = pacc.as<unsigned long long>();
+    switch (na) {
+      case 0: rp_agg_kernel<0><<<g, RP_ANT, lds, st>>>(lk.as<unsigned short>(), paccs, sp, ntiles, P, C, pkb, cp, ap); break;
+      case 1: rp_agg_kernel<1><<<g, RP_ANT, lds, st>>>(lk.as<unsigned short>(), paccs, sp, ntiles, P, C, pkb, cp, ap); break;
+      case 2: rp_agg_kernel<2><<<g, RP_ANT, lds, st>>>(lk.as<unsigned short>(), paccs, sp, ntiles, P, C, pkb, cp, ap); break;
+      case 3: rp_agg_kernel<3><<<g, RP_ANT, lds, st>>>(lk.as<unsigned short>(), paccs, sp, ntiles, P, C, pkb, cp, ap); break;
+      default: rp_agg_kernel<4><<<g, RP_ANT, lds, st>>>(lk.as<unsigned short>(), paccs, sp, ntiles, P, C, pkb, cp, ap); break;
+    }
+    FHIP_CHECK(hipGetLastError());
+  }
+  DBuf w(size_t(R) * stride * 8), flag(size_t(R) * 4), incl(size_t(R) * 4), ent(size_t(R) * sizeof(size_t));
+  {
+    ProfScope prof("groupby_range_emit", st);
+    rp_combine_kernel<<<grid_for(size_t(R)), 256, 0, st>>>(
+        pcnt.as<unsigned>(), pacc.as<unsigned long long>(), C, R, P, pkb, accs, kmin, stride,
+        w.as<unsigned long long>(), flag.as<unsigned>());
+    FHIP_CHECK(hipGetLastError());
+    if (hot_groups) {
+      rp_hot_kernel<<<grid_for(hot_groups), 256, 0, st>>>(hot_fin, hot_groups, kmin, stride,
+                                                         w.as<unsigned long long>(),
+                                                         flag.as<unsigned>());
+      FHIP_CHECK(hipGetLastError());
+    }
+    prefix_sum(FHIP_U32, flag.get(), incl.get(), size_t(R), st);
+    rp_ent_kernel<<<grid_for(size_t(R)), 256, 0, st>>>(flag.as<unsigned>(), incl.as<unsigned>(), R,
+                                                       ent.as<size_t>());
+    FHIP_CHECK(hipGetLastError());
+  }
+  unsigned* gh = static_cast<unsigned*>(pinned_scratch(0, sizeof(unsigned)));
+  FHIP_CHECK(hipMemcpyAsync(gh, incl.as<unsigned>() + (R - 1), sizeof(unsigned),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  const size_t G = *gh;
+  *ngroups = G;  // reported even when it does not fit, so the caller can resize
+  FHIP_REQUIRE(G <= capacity, FHIP_ERR_TOO_LARGE,
+               "group_by_agg: output capacity " + std::to_string(capacity) + " < groups " +
+                   std::to_string(G));
+  if (G == 0) return;
+  const unsigned long long* wp = w.as<unsigned long long>();
+  const size_t* ep = ent.as<size_t>();
+  switch (ktype) {
+    case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<int32_t*>(keys_out)); break;
+    case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<uint32_t*>(keys_out)); break;
+    case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<int64_t*>(keys_out)); break;
+    default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<uint64_t*>(keys_out)); break;
+  }
+  FHIP_CHECK(hipGetLastError());
+  if (naggs > 0) {
+    EmitSet es{};
+    es.n = naggs;
+    for (int i = 0; i < naggs; i++) {
+      es.op[i] = aggs[i].op;
+      es.vtype[i] = aggs[i].vtype;
+      es.slot[i] = slot_of[i] < 0 ? 0 : slot_of[i];
+      es.out[i] = aggs[i].out;
+    }
+    emit_kernel<<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, es);
+    FHIP_CHECK(hipGetLastError());
+  }
+  FHIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
 void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
                   void* keys_out, size_t capacity, size_t* ngroups, size_t table_hint,
                   hipStream_t st, const HeavyKeys* heavy) {
@@ -1424,6 +1849,11 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     ProfScope prof("groupby_plan", st);
     hp = plan_hot(ktype, keys, n, S, st);
   }
+  // the range path (group_by_agg_range) needs the key range, which the
+  // heavy-key pass records; the hash partitions' cold-row counts are then
+  // not made in that pass
+  const bool range_on = hp.used && tune_get("groupby.range", 1) != 0;
+  DBuf hot_range;
   const void* ckeys = keys;  // the rows the partitioned path sees
   size_t cn = n;
   AccSet caccs = accs;
@@ -1483,7 +1913,12 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     unsigned long long* cold_n = hot_cnt.as<unsigned long long>();
     unsigned* hot_n = reinterpret_cast<unsigned*>(cold_n + 1);
     size_t lds = size_t(2 + accs.n) * Sh * 8;
-    if (table_hint) {
+    if (range_on) {
+      hot_range.reset(16);
+      FHIP_CHECK(hipMemsetAsync(hot_range.get(), 0xFF, 8, st));
+      FHIP_CHECK(hipMemsetAsync(hot_range.as<unsigned long long>() + 1, 0, 8, st));
+    }
+    if (table_hint && !range_on) {
       const int cp = pbits_for(table_hint);
       if (lds + (size_t(4) << cp) <= size_t(HOT_MAX_LDS)) {
         hot_cpbits = cp;
@@ -1501,12 +1936,13 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
         unsigned long long* hpart = hot_part.as<unsigned long long>();
         unsigned long long* sg = hot_seg.as<unsigned long long>();
         unsigned long long* cc = hot_cpbits >= 0 ? hot_ccnt.as<unsigned long long>() : nullptr;
+        unsigned long long* kr = range_on ? hot_range.as<unsigned long long>() : nullptr;
         switch (accs.n) {
-          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
-          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
-          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
-          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
-          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc); break;
+          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
         }
       };
       switch (ktype) {
@@ -1536,10 +1972,32 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       // with the partition sizes, so the host does not wait here
       cn_pending = true;
     } else {
-      FHIP_CHECK(hipMemcpyAsync(hc, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
+      unsigned long long* rb = static_cast<unsigned long long*>(pinned_scratch(0, 32));
+      FHIP_CHECK(hipMemcpyAsync(rb, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
+      if (range_on)
+        FHIP_CHECK(hipMemcpyAsync(rb + 2, hot_range.get(), 16, hipMemcpyDeviceToHost, st));
       FHIP_CHECK(hipStreamSynchronize(st));
+      hc[0] = rb[0];
+      hc[1] = rb[1];
       cn = size_t(hc[0]);
       hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
+      if (range_on && cn > 0 && rb[2] <= rb[3]) {
+        // keys within R consecutive values: the range path, when its parts
+        // (2^pkb offsets each) number at most RP_MAXP
+        const unsigned long long R = rb[3] - rb[2] + 1;
+        const unsigned long long rmax =
+            (unsigned long long)std::max(0LL, tune_get("groupby.range_max", (long long)RP_MAXR));
+        const int pkb = rp_part_bits(accs.n);
+        if (R != 0 && R <= n && R <= std::min(rmax, RP_MAXR) &&
+            size_t(R) <= (size_t(RP_MAXP) << pkb)) {
+          const unsigned long long flip =
+              (ktype == FHIP_I32 || ktype == FHIP_I64) ? (1ull << 63) : 0ull;
+          group_by_agg_range(ktype, n, accs, AccSet{}, aggs, naggs, slot_of, cold_k.get(),
+                             hot_seg.as<unsigned long long>(), nparts, hot_fin.as<unsigned long long>(),
+                             hot_groups, rb[2] ^ flip, R, keys_out, capacity, ngroups, st, cold_c);
+          return;
+        }
+      }
     }
     ckeys = cold_k.get();
     for (int a = 0; a < accs.n; a++) caccs.a[a].col = cold_c[a].get();

@@ -287,15 +287,26 @@ def test_group_by_agg_heavy_keys_and_rest(fa):
     at.tune_set("groupby.hot_min_pct", None)
 
 
+@pytest.mark.parametrize("range_path", [1, 0])
 @pytest.mark.parametrize("hint", [0, 16, 1 << 16, 1 << 23])
 @pytest.mark.parametrize("aggs", [("count",), ("sum",), ("sum", "maxi"),
                                   ("sum", "maxv", "minv", "maxi")])
-def test_group_by_agg_heavy_keys_accumulator_counts(fa, aggs, hint):
+def test_group_by_agg_heavy_keys_accumulator_counts(fa, aggs, hint, range_path):
     """The heavy-key pass with zero to four distinct accumulators (its
-    kernel is specialised on that count).  With a table hint the pass also
-    counts the cold rows per partition (used as they are for 2^16 and for
-    the 4096 partitions of 2^23, dropped for the recount after the 16-group
-    hint overflows)."""
+    kernel is specialised on that count).  The keys fill a range of 2^16, so
+    the cold rows take the range path unless it is switched off; on the
+    hash path a table hint makes the pass also count the cold rows per
+    partition (used as they are for 2^16 and for the 4096 partitions of
+    2^23, dropped for the recount after the 16-group hint overflows)."""
+    from frovedis_amd import autotune as at
+    at.tune_set("groupby.range", range_path)
+    try:
+        _heavy_keys_accumulator_counts(fa, aggs, hint)
+    finally:
+        at.tune_set("groupby.range", None)
+
+
+def _heavy_keys_accumulator_counts(fa, aggs, hint):
     from frovedis_amd import datagen as dg
     n = 1 << 21
     k = dg.zipf_keys_np(n, n_values=1 << 16)
