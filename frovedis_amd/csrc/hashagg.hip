@@ -735,22 +735,14 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
         vals[u][a] = r < r1 ? load_bits(accs.a[a].col, accs.a[a].vtype, r) : 0ull;
     }
   };
-  // the smallest and largest key of the rows read (order-preserving bits:
-  // the sign bit flipped for signed keys), for krange
+  // the smallest and largest cold key (order-preserving bits: the sign bit
+  // flipped for signed keys), for krange; the heavy keys' range is known
   constexpr unsigned long long KFLIP = std::is_signed<K>::value ? (1ull << 63) : 0ull;
   unsigned long long kmn = ~0ull, kmx = 0ull;
   for (size_t t0 = r0; t0 < r1; t0 += STEP) {
     K kv[U];
     unsigned long long vv[U][NA1];
     load_step(t0, kv, vv);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      if (t0 + size_t(u) * HOT_NT + tid < r1) {
-        const unsigned long long ob = key_bits(kv[u]) ^ KFLIP;
-        kmn = ob < kmn ? ob : kmn;
-        kmx = ob > kmx ? ob : kmx;
-      }
-    }
     int sl[U];
     unsigned long long cm[U];  // lanes of the wave with a cold row, per row of the step
     unsigned ncold = 0;
@@ -789,6 +781,9 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if ((cm[u] >> lane) & 1ull) {
+        const unsigned long long ob = key_bits(kv[u]) ^ KFLIP;
+        kmn = ob < kmn ? ob : kmn;
+        kmx = ob > kmx ? ob : kmx;
         const size_t pos = r0 + base + unsigned(__popcll(cm[u] & lt));
         if (CP) atomicAdd(&chist[part_of(mix64(key_bits(kv[u])), cpbits)], 1u);
         cold_keys[pos] = kv[u];
@@ -1391,11 +1386,12 @@ constexpr int RP_AU = 4;                   // rows per lane in flight (aggregati
 constexpr size_t RP_LDS = size_t(64) << 10;  // LDS of one part's arrays
 constexpr unsigned long long RP_MAXR = 1ull << 21;
 
-// Tile t covers rows t * RP_TILE .. of the cold buffers; region t / RP_TPR
-// holds seg_n[region] rows, so the tile's rows are the first `rows` of it.
+// Tile t holds the `rows` cold rows from row `src` of the cold buffers
+// (tiles[t] = src << 14 | rows - 1; every tile lies in one cold region and
+// holds at least one row) and writes slots t * RP_TILE .. of the outputs.
 template <class K, int NA>
 __global__ __launch_bounds__(RP_NT) void rp_part_kernel(
-    const K* __restrict__ ckeys, Cols cold, const unsigned long long* __restrict__ seg_n,
+    const K* __restrict__ ckeys, Cols cold, const unsigned long long* __restrict__ tiles,
     size_t ntiles, unsigned long long kmin, int P, unsigned short* __restrict__ lk_out,
     Cols pout, unsigned short* __restrict__ starts) {
   __shared__ __align__(16) unsigned char s_stage[size_t(RP_TILE) * 8];
@@ -1407,13 +1403,11 @@ __global__ __launch_bounds__(RP_NT) void rp_part_kernel(
   const size_t per = (ntiles + 7) / 8;
   const size_t t = size_t(blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (t >= ntiles) return;
-  const size_t off = (t % RP_TPR) * RP_TILE;
-  const unsigned long long have = seg_n[t / RP_TPR];
-  const unsigned rows = have > off ? unsigned(std::min<unsigned long long>(RP_TILE, have - off)) : 0u;
-  const size_t base = t * RP_TILE;
-  // loads go through clamped indices (no branch); a tile with no rows reads
-  // row 0 of the buffers, which always exists
-  const size_t lbase = rows ? base : 0;
+  const unsigned long long td = tiles[t];
+  const unsigned rows = unsigned(td & 0x3FFFull) + 1;
+  const size_t lbase = size_t(td >> 14);  // the tile's first cold row
+  const size_t base = t * RP_TILE;        // its output slots
+  // loads go through clamped indices (no branch)
   const K* kt = ckeys + lbase;
   for (int p = threadIdx.x; p < RP_MAXP; p += RP_NT) s_cnt[p] = 0;
   // offset k: part k mod P, index k / P within the part (P a power of two),
@@ -1631,7 +1625,7 @@ int rp_part_bits(int na) {
 
 void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /*unused*/,
                         const fhip_agg* aggs, int naggs, const std::vector<int>& slot_of,
-                        const void* ckeys, const unsigned long long* seg_n, size_t nregions,
+                        const void* ckeys, const unsigned long long* seg_h, size_t nregions,
                         const unsigned long long* hot_fin, unsigned hot_groups,
                         unsigned long long kmin, unsigned long long R, void* keys_out,
                         size_t capacity, size_t* ngroups, hipStream_t st, const DBuf* cold_cols) {
@@ -1641,7 +1635,17 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
   const int pkb = rp_part_bits(na);
   int P = 1;  // a power of two: key offset k goes to part k mod P
   while ((size_t(P) << pkb) < size_t(R)) P *= 2;
-  const size_t ntiles = nregions * RP_TPR;
+  // the tiles of the cold regions (seg_h: each region's cold rows, on the
+  // host), packed as first row << 14 | rows - 1
+  unsigned long long* tl =
+      static_cast<unsigned long long*>(pinned_scratch(1, nregions * RP_TPR * 8));
+  size_t ntiles = 0;
+  for (size_t b = 0; b < nregions; b++)
+    for (unsigned long long off = 0; off < seg_h[b]; off += RP_TILE)
+      tl[ntiles++] = ((b * HOT_CHUNK + off) << 14) |
+                     (std::min<unsigned long long>(RP_TILE, seg_h[b] - off) - 1);
+  DBuf tiles(ntiles * 8);
+  FHIP_CHECK(hipMemcpyAsync(tiles.get(), tl, ntiles * 8, hipMemcpyHostToDevice, st));
   const size_t prow = ntiles * RP_TILE;  // slots of the partitioned copy
   DBuf lk(prow * sizeof(unsigned short)), starts((size_t(P) + 1) * ntiles * sizeof(unsigned short));
   DBuf pc[HA_MAX_ACC];
@@ -1662,11 +1666,11 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
       const KK* kp = static_cast<const KK*>(ckeys);
       const unsigned g = unsigned(ceil_div(ntiles, size_t(8)) * 8);
       switch (na) {
-        case 0: rp_part_kernel<KK, 0><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
-        case 1: rp_part_kernel<KK, 1><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
-        case 2: rp_part_kernel<KK, 2><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
-        case 3: rp_part_kernel<KK, 3><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
-        default: rp_part_kernel<KK, 4><<<g, RP_NT, 0, st>>>(kp, cold, seg_n, ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        case 0: rp_part_kernel<KK, 0><<<g, RP_NT, 0, st>>>(kp, cold, tiles.as<unsigned long long>(), ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        case 1: rp_part_kernel<KK, 1><<<g, RP_NT, 0, st>>>(kp, cold, tiles.as<unsigned long long>(), ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        case 2: rp_part_kernel<KK, 2><<<g, RP_NT, 0, st>>>(kp, cold, tiles.as<unsigned long long>(), ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        case 3: rp_part_kernel<KK, 3><<<g, RP_NT, 0, st>>>(kp, cold, tiles.as<unsigned long long>(), ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
+        default: rp_part_kernel<KK, 4><<<g, RP_NT, 0, st>>>(kp, cold, tiles.as<unsigned long long>(), ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
       }
     };
     switch (ktype) {
@@ -1972,16 +1976,29 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       // with the partition sizes, so the host does not wait here
       cn_pending = true;
     } else {
-      unsigned long long* rb = static_cast<unsigned long long*>(pinned_scratch(0, 32));
+      // the counts, and for the range path the cold keys' range and every
+      // region's cold rows
+      unsigned long long* rb =
+          static_cast<unsigned long long*>(pinned_scratch(0, 32 + (range_on ? nparts * 8 : 0)));
       FHIP_CHECK(hipMemcpyAsync(rb, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
-      if (range_on)
+      if (range_on) {
         FHIP_CHECK(hipMemcpyAsync(rb + 2, hot_range.get(), 16, hipMemcpyDeviceToHost, st));
+        FHIP_CHECK(hipMemcpyAsync(rb + 4, hot_seg.get(), nparts * 8, hipMemcpyDeviceToHost, st));
+      }
       FHIP_CHECK(hipStreamSynchronize(st));
       hc[0] = rb[0];
       hc[1] = rb[1];
       cn = size_t(hc[0]);
       hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
       if (range_on && cn > 0 && rb[2] <= rb[3]) {
+        // the cold keys' range (from the heavy-key pass) and the heavy keys'
+        const unsigned long long flip0 =
+            (ktype == FHIP_I32 || ktype == FHIP_I64) ? (1ull << 63) : 0ull;
+        for (unsigned long long kb : hp.table) {
+          if (kb == HA_EMPTY) continue;
+          rb[2] = std::min(rb[2], kb ^ flip0);
+          rb[3] = std::max(rb[3], kb ^ flip0);
+        }
         // keys within R consecutive values: the range path, when its parts
         // (2^pkb offsets each) number at most RP_MAXP
         const unsigned long long R = rb[3] - rb[2] + 1;
@@ -1993,7 +2010,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
           const unsigned long long flip =
               (ktype == FHIP_I32 || ktype == FHIP_I64) ? (1ull << 63) : 0ull;
           group_by_agg_range(ktype, n, accs, AccSet{}, aggs, naggs, slot_of, cold_k.get(),
-                             hot_seg.as<unsigned long long>(), nparts, hot_fin.as<unsigned long long>(),
+                             rb + 4, nparts, hot_fin.as<unsigned long long>(),
                              hot_groups, rb[2] ^ flip, R, keys_out, capacity, ngroups, st, cold_c);
           return;
         }
