@@ -156,6 +156,56 @@ __device__ __forceinline__ void acc_merge(unsigned long long* w, int kind, int v
   }
 }
 
+// acc_merge for U rows of one accumulator (slots base[idx[u]], rows with
+// in[u]), the type dispatch done once for all of them.
+template <int U>
+__device__ __forceinline__ void acc_merge_rows(unsigned long long* base, const unsigned* idx,
+                                               const bool* in, int kind, int vtype,
+                                               const unsigned long long* v) {
+#define FHIP_MERGE_ROWS(T, CONV, OP)                                  \
+  {                                                                   \
+    _Pragma("unroll") for (int u = 0; u < U; u++) if (in[u]) {        \
+      T* p = reinterpret_cast<T*>(base + idx[u]);                     \
+      const T x = CONV(v[u]);                                         \
+      OP;                                                             \
+    }                                                                 \
+    return;                                                           \
+  }
+  auto as_i32 = [](unsigned long long b) { return int(unsigned(b)); };
+  auto as_u32 = [](unsigned long long b) { return unsigned(b); };
+  auto as_i64 = [](unsigned long long b) { return (long long)b; };
+  auto as_u64 = [](unsigned long long b) { return b; };
+  auto as_f32 = [](unsigned long long b) { float f; unsigned w = unsigned(b); __builtin_memcpy(&f, &w, 4); return f; };
+  auto as_f64 = [](unsigned long long b) { double d; __builtin_memcpy(&d, &b, 8); return d; };
+  switch (vtype) {
+    case FHIP_F64:
+      if (kind == K_SUM) FHIP_MERGE_ROWS(double, as_f64, unsafeAtomicAdd(p, x))
+      else if (kind == K_MAX) FHIP_MERGE_ROWS(double, as_f64, if (x == x) atomicMax(p, x))
+      else FHIP_MERGE_ROWS(double, as_f64, if (x == x) atomicMin(p, x))
+    case FHIP_F32:
+      if (kind == K_SUM) FHIP_MERGE_ROWS(float, as_f32, unsafeAtomicAdd(p, x))
+      else if (kind == K_MAX) FHIP_MERGE_ROWS(float, as_f32, if (x == x) atomicMax(p, x))
+      else FHIP_MERGE_ROWS(float, as_f32, if (x == x) atomicMin(p, x))
+    case FHIP_I32:
+      if (kind == K_SUM) FHIP_MERGE_ROWS(int, as_i32, atomicAdd(p, x))
+      else if (kind == K_MAX) FHIP_MERGE_ROWS(int, as_i32, atomicMax(p, x))
+      else FHIP_MERGE_ROWS(int, as_i32, atomicMin(p, x))
+    case FHIP_U32:
+      if (kind == K_SUM) FHIP_MERGE_ROWS(unsigned, as_u32, atomicAdd(p, x))
+      else if (kind == K_MAX) FHIP_MERGE_ROWS(unsigned, as_u32, atomicMax(p, x))
+      else FHIP_MERGE_ROWS(unsigned, as_u32, atomicMin(p, x))
+    case FHIP_I64:
+      if (kind == K_SUM) FHIP_MERGE_ROWS(unsigned long long, as_u64, atomicAdd(p, x))
+      else if (kind == K_MAX) FHIP_MERGE_ROWS(long long, as_i64, atomicMax(p, x))
+      else FHIP_MERGE_ROWS(long long, as_i64, atomicMin(p, x))
+    default:
+      if (kind == K_SUM) FHIP_MERGE_ROWS(unsigned long long, as_u64, atomicAdd(p, x))
+      else if (kind == K_MAX) FHIP_MERGE_ROWS(unsigned long long, as_u64, atomicMax(p, x))
+      else FHIP_MERGE_ROWS(unsigned long long, as_u64, atomicMin(p, x))
+  }
+#undef FHIP_MERGE_ROWS
+}
+
 // ---- pass 1: partition histogram -------------------------------------------
 // partition of a key: the top pbits of its hash; LDS slots use the low bits
 __device__ __forceinline__ unsigned part_of(unsigned long long h, int pbits) {
@@ -796,6 +846,9 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
       base += unsigned(__popcll(cm[u]));
     }
   }
+  // the workgroup's cold-key range goes to krange[2b], krange[2b + 1] (no
+  // atomics: the host reduces it with the regions' row counts)
+  __shared__ unsigned long long s_rng[2 * (HOT_NT / 64)];
   if (krange) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -803,13 +856,21 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
       kmn = a < kmn ? a : kmn;
       kmx = b > kmx ? b : kmx;
     }
-    if (lane == 0 && kmn <= kmx) {
-      atomicMin(&krange[0], kmn);
-      atomicMax(&krange[1], kmx);
+    if (lane == 0) {
+      s_rng[2 * (tid >> 6)] = kmn;
+      s_rng[2 * (tid >> 6) + 1] = kmx;
     }
   }
   __syncthreads();
   if (tid == 0) {
+    if (krange) {
+      for (int w = 1; w < HOT_NT / 64; w++) {
+        kmn = s_rng[2 * w] < kmn ? s_rng[2 * w] : kmn;
+        kmx = s_rng[2 * w + 1] > kmx ? s_rng[2 * w + 1] : kmx;
+      }
+      krange[2 * blockIdx.x] = kmn;
+      krange[2 * blockIdx.x + 1] = kmx;
+    }
     seg_n[blockIdx.x] = cold_cnt;
     if (cold_cnt) atomicAdd(cold_n, (unsigned long long)cold_cnt);
   }
@@ -1382,7 +1443,7 @@ constexpr int RP_MAXP = 1024;              // parts at most (one counter per thr
 constexpr size_t RP_TPR = HOT_CHUNK / RP_TILE;  // tiles per heavy-key region
 static_assert(HOT_CHUNK % RP_TILE == 0, "a heavy-key region is whole tiles");
 constexpr int RP_ANT = 512;                // threads of the aggregation kernel
-constexpr int RP_AU = 4;                   // rows per lane in flight (aggregation)
+constexpr int RP_AU = 8;                   // rows per lane in flight (aggregation)
 constexpr size_t RP_LDS = size_t(64) << 10;  // LDS of one part's arrays
 constexpr unsigned long long RP_MAXR = 1ull << 21;
 
@@ -1530,23 +1591,33 @@ __global__ __launch_bounds__(RP_ANT) void rp_agg_kernel(
     segment_rows<RP_TILE, RP_AU>(starts, ntiles, p, t0, t1, threadIdx.x >> 6, RP_ANT / 64,
                                  [&](const size_t* pos, const bool* in) {
                                    unsigned l[RP_AU];
-                                   unsigned long long v[RP_AU][NA1];
+                                   unsigned long long v[NA1][RP_AU];
 #pragma unroll
-                                   for (int u = 0; u < RP_AU; u++) {
-                                     l[u] = lk_in[pos[u]];
+                                   for (int u = 0; u < RP_AU; u++) l[u] = lk_in[pos[u]];
+                                   // one branch per column (its width is
+                                   // uniform), so that all its loads issue
+                                   // together
 #pragma unroll
-                                     for (int a = 0; a < NA; a++)
-                                       v[u][a] = load_bits(accs.a[a].col, accs.a[a].vtype, pos[u]);
+                                   for (int a = 0; a < NA; a++) {
+                                     const int vt = accs.a[a].vtype;
+                                     if (vt == FHIP_I32 || vt == FHIP_U32 || vt == FHIP_F32) {
+                                       const unsigned* c = static_cast<const unsigned*>(accs.a[a].col);
+#pragma unroll
+                                       for (int u = 0; u < RP_AU; u++) v[a][u] = c[pos[u]];
+                                     } else {
+                                       const unsigned long long* c =
+                                           static_cast<const unsigned long long*>(accs.a[a].col);
+#pragma unroll
+                                       for (int u = 0; u < RP_AU; u++) v[a][u] = c[pos[u]];
+                                     }
                                    }
 #pragma unroll
-                                   for (int u = 0; u < RP_AU; u++) {
-                                     if (!in[u]) continue;
-                                     atomicAdd(&cnt[l[u]], 1u);
+                                   for (int u = 0; u < RP_AU; u++)
+                                     if (in[u]) atomicAdd(&cnt[l[u]], 1u);
 #pragma unroll
-                                     for (int a = 0; a < NA; a++)
-                                       acc_merge(&acc[size_t(a) * PK + l[u]], accs.a[a].kind,
-                                                 accs.a[a].vtype, v[u][a]);
-                                   }
+                                   for (int a = 0; a < NA; a++)
+                                     acc_merge_rows<RP_AU>(acc + size_t(a) * PK, l, in,
+                                                           accs.a[a].kind, accs.a[a].vtype, v[a]);
                                  });
   __syncthreads();
   // part-major: slot i of part p goes to [c][p * PK + i] (coalesced); the
@@ -1681,9 +1752,10 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
     }
     FHIP_CHECK(hipGetLastError());
   }
-  // chunks per part: about 2048 workgroups, each with at least 64 tiles a wave
+  // chunks per part: at least 512 workgroups (two a CU), each with at least
+  // 64 tiles a wave; more chunks cost the combine a pass over their arrays
   const long long cdef = tune_get("groupby.range_chunks", 0);
-  int C = cdef > 0 ? int(cdef) : int(std::max<size_t>(1, ceil_div(2048, size_t(P))));
+  int C = cdef > 0 ? int(cdef) : int(std::max<size_t>(1, ceil_div(512, size_t(P))));
   C = int(std::max<size_t>(1, std::min<size_t>(size_t(C), ntiles / (64 * (RP_ANT / 64)))));
   const size_t RP = size_t(P) << pkb;  // slots of one chunk's arrays (>= R)
   DBuf pcnt(size_t(C) * RP * 4), pacc(size_t(C) * RP * 8 * size_t(std::max(na, 1)));
@@ -1917,11 +1989,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     unsigned long long* cold_n = hot_cnt.as<unsigned long long>();
     unsigned* hot_n = reinterpret_cast<unsigned*>(cold_n + 1);
     size_t lds = size_t(2 + accs.n) * Sh * 8;
-    if (range_on) {
-      hot_range.reset(16);
-      FHIP_CHECK(hipMemsetAsync(hot_range.get(), 0xFF, 8, st));
-      FHIP_CHECK(hipMemsetAsync(hot_range.as<unsigned long long>() + 1, 0, 8, st));
-    }
+    if (range_on) hot_range.reset(nparts * 16);
     if (table_hint && !range_on) {
       const int cp = pbits_for(table_hint);
       if (lds + (size_t(4) << cp) <= size_t(HOT_MAX_LDS)) {
@@ -1978,16 +2046,25 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     } else {
       // the counts, and for the range path the cold keys' range and every
       // region's cold rows
-      unsigned long long* rb =
-          static_cast<unsigned long long*>(pinned_scratch(0, 32 + (range_on ? nparts * 8 : 0)));
+      unsigned long long* rb = static_cast<unsigned long long*>(
+          pinned_scratch(0, 32 + (range_on ? nparts * 24 : 0)));
       FHIP_CHECK(hipMemcpyAsync(rb, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
       if (range_on) {
-        FHIP_CHECK(hipMemcpyAsync(rb + 2, hot_range.get(), 16, hipMemcpyDeviceToHost, st));
         FHIP_CHECK(hipMemcpyAsync(rb + 4, hot_seg.get(), nparts * 8, hipMemcpyDeviceToHost, st));
+        FHIP_CHECK(hipMemcpyAsync(rb + 4 + nparts, hot_range.get(), nparts * 16,
+                                  hipMemcpyDeviceToHost, st));
       }
       FHIP_CHECK(hipStreamSynchronize(st));
       hc[0] = rb[0];
       hc[1] = rb[1];
+      if (range_on) {  // the cold keys' range over the workgroups' ranges
+        rb[2] = ~0ull;
+        rb[3] = 0ull;
+        for (size_t b = 0; b < nparts; b++) {
+          rb[2] = std::min(rb[2], rb[4 + nparts + 2 * b]);
+          rb[3] = std::max(rb[3], rb[4 + nparts + 2 * b + 1]);
+        }
+      }
       cn = size_t(hc[0]);
       hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
       if (range_on && cn > 0 && rb[2] <= rb[3]) {
