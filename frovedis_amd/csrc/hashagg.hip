@@ -1423,11 +1423,11 @@ namespace {
 // consecutive values, the cold rows are partitioned by key offset instead of
 // by hash, and every part's table is an array indexed by the offset:
 //   rp_part: each 16384-row tile of a cold region ranks its rows by part
-//     (key offset k goes to part k mod P, at index k / P; P a power of two,
-//     at most 1024, parts of at most 2^pkb keys) in LDS and writes the
-//     16-bit index and the value columns in part order to the tile's own
-//     slots, with the start of every part in the tile (part-major, as the
-//     dense join's partition does);
+//     (key offset k goes to part (k / 16) mod P, in runs of 16 keys; P a
+//     power of two, at most 1024, parts of at most 2^pkb keys) in LDS and
+//     writes the 16-bit index within the part and the value columns in part
+//     order to the tile's own slots, with the start of every part in the
+//     tile (part-major, as the dense join's partition does);
 //   rp_agg: workgroup (part, chunk of tiles) aggregates its rows into LDS
 //     arrays indexed by the index (no hashing, no key compare, no
 //     overflow) and writes them out per chunk;
@@ -1444,8 +1444,14 @@ constexpr size_t RP_TPR = HOT_CHUNK / RP_TILE;  // tiles per heavy-key region
 static_assert(HOT_CHUNK % RP_TILE == 0, "a heavy-key region is whole tiles");
 constexpr int RP_ANT = 512;                // threads of the aggregation kernel
 constexpr int RP_AU = 8;                   // rows per lane in flight (aggregation)
-constexpr size_t RP_LDS = size_t(64) << 10;  // LDS of one part's arrays
+constexpr size_t RP_LDS = size_t(80) << 10;  // LDS of one part's arrays (two workgroups a CU)
 constexpr unsigned long long RP_MAXR = 1ull << 21;
+constexpr int RP_BLK = 4;                  // keys go to the parts in blocks of 2^RP_BLK
+
+// Index of key offset k within its part (ppb = log2 of the part count).
+__host__ __device__ __forceinline__ unsigned rp_index(unsigned long long k, int ppb) {
+  return unsigned((((k >> RP_BLK) >> ppb) << RP_BLK) | (k & ((1ull << RP_BLK) - 1)));
+}
 
 // Tile t holds the `rows` cold rows from row `src` of the cold buffers
 // (tiles[t] = src << 14 | rows - 1; every tile lies in one cold region and
@@ -1471,9 +1477,11 @@ __global__ __launch_bounds__(RP_NT) void rp_part_kernel(
   // loads go through clamped indices (no branch)
   const K* kt = ckeys + lbase;
   for (int p = threadIdx.x; p < RP_MAXP; p += RP_NT) s_cnt[p] = 0;
-  // offset k: part k mod P, index k / P within the part (P a power of two),
-  // so every part holds an even share of each stretch of keys (a skewed
-  // column's densest stretch would otherwise fill a few parts)
+  // offset k in block k / 16: part (k / 16) mod P, index
+  // (k / 16 / P) * 16 + k mod 16 within the part (P a power of two), so
+  // every part holds an even share of each stretch of keys (a skewed
+  // column's densest stretch would otherwise fill a few parts) and a part's
+  // keys still come in runs of 16
   const unsigned long long pmask = (unsigned long long)(P - 1);
   const int ppb = __builtin_ctz(unsigned(P));
   unsigned kl[RP_ITEMS / 2];  // two 16-bit indexes within the part per word
@@ -1489,12 +1497,12 @@ __global__ __launch_bounds__(RP_NT) void rp_part_kernel(
 #pragma unroll
     for (int j = 0; j < RP_ITEMS; j++) {
       const unsigned i = unsigned(j) * RP_NT + threadIdx.x;
-      const unsigned l = unsigned(kb[j] >> ppb);
+      const unsigned l = rp_index(kb[j], ppb);
       if (j & 1) kl[j >> 1] |= l << 16;
       else kl[j >> 1] = l;
       dr[j] = ~0u;
       if (i < rows) {
-        const unsigned d = unsigned(kb[j] & pmask);
+        const unsigned d = unsigned((kb[j] >> RP_BLK) & pmask);
         dr[j] = (d << 16) | atomicAdd(&s_cnt[d], 1u);
       }
     }
@@ -1631,9 +1639,10 @@ __global__ __launch_bounds__(RP_ANT) void rp_agg_kernel(
   }
 }
 
-// Per key offset k < R (slot k / P of part k mod P): the chunks' counts and
-// accumulators combined in chunk order into entry k of `w` ({key bits,
-// count, accumulators}); flag[k] = 1 when the key occurs.
+// Per key offset k < R (index rp_index(k) of part (k / 2^RP_BLK) mod P):
+// the chunks' counts and accumulators combined in chunk order into entry k
+// of `w` ({key bits, count, accumulators}); flag[k] = 1 when the key
+// occurs.
 __global__ __launch_bounds__(256) void rp_combine_kernel(
     const unsigned* __restrict__ pcnt, const unsigned long long* __restrict__ pacc, int C,
     unsigned long long R, int P, int pkb, AccSet accs, unsigned long long kmin, int stride,
@@ -1641,7 +1650,7 @@ __global__ __launch_bounds__(256) void rp_combine_kernel(
   const size_t RP = size_t(P) << pkb;  // slots of one chunk's arrays
   const int ppb = __builtin_ctz(unsigned(P));
   for (size_t k = size_t(blockIdx.x) * 256 + threadIdx.x; k < R; k += size_t(gridDim.x) * 256) {
-    const size_t slot = ((k & size_t(P - 1)) << pkb) + (k >> ppb);
+    const size_t slot = (((k >> RP_BLK) & size_t(P - 1)) << pkb) + rp_index(k, ppb);
     unsigned long long cn = 0;
     for (int c = 0; c < C; c++) cn += pcnt[size_t(c) * RP + slot];
     unsigned long long* e = w + k * size_t(stride);
@@ -1704,7 +1713,7 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
   const int stride = 2 + na;
   const size_t bpk = 4 + 8 * size_t(na);
   const int pkb = rp_part_bits(na);
-  int P = 1;  // a power of two: key offset k goes to part k mod P
+  int P = 1;  // a power of two: key offset k goes to part (k / 2^RP_BLK) mod P
   while ((size_t(P) << pkb) < size_t(R)) P *= 2;
   // the tiles of the cold regions (seg_h: each region's cold rows, on the
   // host), packed as first row << 14 | rows - 1
