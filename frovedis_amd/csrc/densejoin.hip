@@ -135,12 +135,17 @@ __global__ __launch_bounds__(64) void dj_stats_final_kernel(const DjStats* __res
 //          out of range.
 // Within a part the rows come in LDS-atomic order; the readers never depend
 // on it.
-template <class K, class E, bool BUILD, int MAXP = DJ_MAXP>
+//   PAIRS (BUILD, 4-byte values): offset and value are written together as
+//          one 8-byte pair per row to o_out (two words a row), so a reader
+//          of short segments fetches one run, not two.
+template <class K, class E, bool BUILD, int MAXP = DJ_MAXP, bool PAIRS = false>
 __global__ __launch_bounds__(DJ_NT) void dj_part_kernel(
     const K* __restrict__ keys, size_t n, unsigned long long mn, unsigned long long R, int sbits,
     int P, size_t ntiles, const size_t* __restrict__ ridx, unsigned* __restrict__ o_out,
     E* __restrict__ v_out, unsigned short* __restrict__ dest, unsigned short* __restrict__ starts) {
-  constexpr size_t STAGE = size_t(DJ_TILE) * (BUILD ? sizeof(E) : sizeof(unsigned));
+  static_assert(!PAIRS || (BUILD && sizeof(E) == 4), "pairs of 4-byte offset and value");
+  constexpr size_t STAGE =
+      size_t(DJ_TILE) * (PAIRS ? 8 : BUILD ? sizeof(E) : sizeof(unsigned));
   __shared__ __align__(16) unsigned char s_stage[STAGE];
   __shared__ unsigned s_cnt[MAXP];
   __shared__ unsigned s_tmp[DJ_NT / 64];
@@ -211,6 +216,21 @@ __global__ __launch_bounds__(DJ_NT) void dj_part_kernel(
     if (threadIdx.x == 0) starts[size_t(P) * ntiles + t] = (unsigned short)all;
   }
   __syncthreads();
+  if constexpr (PAIRS) {
+    uint2* stage_p = reinterpret_cast<uint2*>(s_stage);
+#pragma unroll
+    for (int j = 0; j < DJ_ITEMS; j++) {
+      if (dr[j] != ~0u) {
+        const unsigned i = unsigned(j) * DJ_NT + threadIdx.x;
+        stage_p[s_cnt[dr[j] >> 16] + (dr[j] & 0xFFFFu)] = make_uint2(o[j], unsigned(base + i));
+      }
+    }
+    __syncthreads();
+    const unsigned cnt = s_total;
+    uint2* pt = reinterpret_cast<uint2*>(o_out) + base;
+    for (unsigned i = threadIdx.x; i < cnt; i += DJ_NT) pt[i] = stage_p[i];
+    return;
+  }
   unsigned short* dt = BUILD ? nullptr : dest + base;
 #pragma unroll
   for (int j = 0; j < DJ_ITEMS; j++) {
@@ -310,9 +330,8 @@ constexpr int DJ_LSBITS = 15;
 constexpr int DJ_LMAXP = 4096;
 
 __global__ __launch_bounds__(DJ_NT) void dj_lds_build_kernel(
-    const unsigned* __restrict__ o_in, const unsigned* __restrict__ v_in,
-    const unsigned short* __restrict__ starts, size_t ntiles, unsigned long long R,
-    unsigned* __restrict__ table, unsigned* __restrict__ dup) {
+    const uint2* __restrict__ ov, const unsigned short* __restrict__ starts, size_t ntiles,
+    unsigned long long R, unsigned* __restrict__ table, unsigned* __restrict__ dup) {
   constexpr unsigned S = 1u << DJ_LSBITS;
   __shared__ __align__(16) unsigned s_slice[S];
   __shared__ unsigned s_dup;
@@ -328,8 +347,9 @@ __global__ __launch_bounds__(DJ_NT) void dj_lds_build_kernel(
                          unsigned o[4], v[4];
 #pragma unroll
                          for (int u = 0; u < 4; u++) {
-                           o[u] = o_in[pos[u]];
-                           v[u] = v_in[pos[u]];
+                           const uint2 x = ov[pos[u]];
+                           o[u] = x.x;
+                           v[u] = x.y;
                          }
 #pragma unroll
                          for (int u = 0; u < 4; u++) {
@@ -630,15 +650,18 @@ bool dj_build(const K* right, const size_t* ridx, size_t nr, DenseTable& t, hipS
                    tune_get("join.dense_lds_build", 1) != 0;
   const int sb = lds ? DJ_LSBITS : t.sbits;
   const int P = lds ? int(ceil_div(size_t(t.R), size_t(1) << DJ_LSBITS)) : t.P;
-  DBuf o(nt * DJ_TILE * sizeof(unsigned)), v(nt * DJ_TILE * sizeof(E));
+  // (the LDS build reads offset-value pairs from o: two words a row)
+  DBuf o(nt * DJ_TILE * (lds ? 8 : sizeof(unsigned))), v(lds ? 0 : nt * DJ_TILE * sizeof(E));
   DBuf starts((size_t(P) + 1) * nt * sizeof(unsigned short));
   {
     ProfScope prof("join_dense_build_part", st);
-    if (lds)
-      dj_part_kernel<K, E, true, DJ_LMAXP><<<unsigned(ceil_div(nt, 8) * 8), DJ_NT, 0, st>>>(
-          right, nr, t.mn, t.R, sb, P, nt, ridx, o.as<unsigned>(), v.as<E>(), nullptr,
-          starts.as<unsigned short>());
-    else
+    if constexpr (sizeof(E) == 4) {
+      if (lds)
+        dj_part_kernel<K, E, true, DJ_LMAXP, true><<<unsigned(ceil_div(nt, 8) * 8), DJ_NT, 0, st>>>(
+            right, nr, t.mn, t.R, sb, P, nt, ridx, o.as<unsigned>(), nullptr, nullptr,
+            starts.as<unsigned short>());
+    }
+    if (!lds)
       dj_part_kernel<K, E, true><<<unsigned(ceil_div(nt, 8) * 8), DJ_NT, 0, st>>>(
           right, nr, t.mn, t.R, sb, P, nt, ridx, o.as<unsigned>(), v.as<E>(), nullptr,
           starts.as<unsigned short>());
@@ -651,8 +674,8 @@ bool dj_build(const K* right, const size_t* ridx, size_t nr, DenseTable& t, hipS
   if constexpr (sizeof(E) == 4) {
     if (lds) {
       dj_lds_build_kernel<<<unsigned(P), DJ_NT, 0, st>>>(
-          o.as<unsigned>(), v.as<unsigned>(), starts.as<unsigned short>(), nt, t.R,
-          t.table.as<unsigned>(), dup.as<unsigned>());
+          o.as<uint2>(), starts.as<unsigned short>(), nt, t.R, t.table.as<unsigned>(),
+          dup.as<unsigned>());
       FHIP_CHECK(hipGetLastError());
     } else {
       // 4-byte positions (unique per row): scatter, then check every row
