@@ -267,11 +267,13 @@ __global__ __launch_bounds__(DJ_NT) void dj_part_kernel(
   }
 }
 
+// Positions are 32-bit: the dense join takes at most 2^32 - DJ_TILE rows
+// a side.
 template <int U, bool SEARCH = false, class F>
 __device__ __forceinline__ void dj_for_rows(const unsigned short* __restrict__ starts,
                                             size_t ntiles, int p, size_t t0, size_t t1,
                                             unsigned wave, unsigned nwaves, F&& f) {
-  segment_rows<DJ_TILE, U, SEARCH>(starts, ntiles, p, t0, t1, wave, nwaves, f);
+  segment_rows<DJ_TILE, U, SEARCH, unsigned>(starts, ntiles, p, t0, t1, wave, nwaves, f);
 }
 
 // One workgroup per part: the part's offsets are marked in an LDS bitmap
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(DJ_NT) void dj_build_kernel(
   __syncthreads();
   unsigned d = 0;
   dj_for_rows<DJ_BU>(starts, ntiles, p, 0, ntiles, threadIdx.x >> 6, DJ_NT / 64,
-                     [&](const size_t* pos, const bool* in) {
+                     [&](const unsigned* pos, const bool* in) {
                        unsigned o[DJ_BU];
                        E v[DJ_BU];
 #pragma unroll
@@ -343,7 +345,7 @@ __global__ __launch_bounds__(DJ_NT) void dj_lds_build_kernel(
   __syncthreads();
   unsigned d = 0;
   dj_for_rows<4, true>(starts, ntiles, p, 0, ntiles, threadIdx.x >> 6, DJ_NT / 64,
-                       [&](const size_t* pos, const bool* in) {
+                       [&](const unsigned* pos, const bool* in) {
                          unsigned o[4], v[4];
 #pragma unroll
                          for (int u = 0; u < 4; u++) {
@@ -382,7 +384,7 @@ enum DjMode {
 // g % C: the C chunks of one part run on one XCD under the round-robin
 // dispatch of workgroups to XCDs, so that part's table slice is read and
 // written in that XCD's L2.  (Only the speed depends on the dispatch.)
-template <class E, int MODE>
+template <class E, int MODE, int PU = DJ_PU>
 __global__ __launch_bounds__(DJ_PNT) void dj_slice_kernel(const unsigned* __restrict__ o_in,
                                                           const unsigned short* __restrict__ starts,
                                                           size_t ntiles, int P, int C,
@@ -398,33 +400,33 @@ __global__ __launch_bounds__(DJ_PNT) void dj_slice_kernel(const unsigned* __rest
   const size_t t1 = std::min(ntiles, t0 + chunk);
   if (t0 >= t1) return;
   unsigned bad = 0;
-  dj_for_rows<DJ_PU>(starts, ntiles, p, t0, t1, threadIdx.x >> 6, DJ_PNT / 64,
-                     [&](const size_t* pos, const bool* in) {
-                       unsigned o[DJ_PU];
+  dj_for_rows<PU>(starts, ntiles, p, t0, t1, threadIdx.x >> 6, DJ_PNT / 64,
+                     [&](const unsigned* pos, const bool* in) {
+                       unsigned o[PU];
 #pragma unroll
-                       for (int u = 0; u < DJ_PU; u++)
+                       for (int u = 0; u < PU; u++)
                          o[u] = __builtin_nontemporal_load(o_in + pos[u]);
                        if constexpr (MODE == DJ_PROBE) {
-                         E v[DJ_PU];
+                         E v[PU];
 #pragma unroll
-                         for (int u = 0; u < DJ_PU; u++) v[u] = table[in[u] ? o[u] : 0u];
+                         for (int u = 0; u < PU; u++) v[u] = table[in[u] ? o[u] : 0u];
 #pragma unroll
-                         for (int u = 0; u < DJ_PU; u++)
+                         for (int u = 0; u < PU; u++)
                            if (in[u]) __builtin_nontemporal_store(v[u], vals + pos[u]);
                        } else {
-                         E v[DJ_PU];
+                         E v[PU];
 #pragma unroll
-                         for (int u = 0; u < DJ_PU; u++) v[u] = __builtin_nontemporal_load(vals + pos[u]);
+                         for (int u = 0; u < PU; u++) v[u] = __builtin_nontemporal_load(vals + pos[u]);
                          if constexpr (MODE == DJ_SCATTER) {
 #pragma unroll
-                           for (int u = 0; u < DJ_PU; u++)
+                           for (int u = 0; u < PU; u++)
                              if (in[u]) table[o[u]] = v[u];
                          } else {
-                           E c[DJ_PU];
+                           E c[PU];
 #pragma unroll
-                           for (int u = 0; u < DJ_PU; u++) c[u] = table[in[u] ? o[u] : 0u];
+                           for (int u = 0; u < PU; u++) c[u] = table[in[u] ? o[u] : 0u];
 #pragma unroll
-                           for (int u = 0; u < DJ_PU; u++) bad |= unsigned(in[u] && c[u] != v[u]);
+                           for (int u = 0; u < PU; u++) bad |= unsigned(in[u] && c[u] != v[u]);
                          }
                        }
                      });
@@ -733,9 +735,27 @@ void dj_probe(const DenseTable& t, const K* left, size_t nl, DBuf& dest, DBuf& r
   const size_t p8 = ceil_div(size_t(t.P), 8) * 8;
   {
     ProfScope prof("join_dense_probe", st);
-    dj_slice_kernel<E, DJ_PROBE><<<unsigned(p8 * size_t(C)), DJ_PNT, 0, st>>>(
-        o.as<unsigned>(), starts.as<unsigned short>(), nt, t.P, C, t.table.as<E>(), res.as<E>(),
-        nullptr);
+    // rows per lane in flight: join.dense_probe_u (4, 8, 16 or 32)
+    // (measured for 2^29 probe rows: 4.32, 3.67-3.80, 3.36-3.65 and 3.25 ms
+    // for 4, 8, 16 and 32: the probe waits on its L2 reads)
+    const long long pu = tune_get("join.dense_probe_u", 32);
+    const unsigned g = unsigned(p8 * size_t(C));
+    if (pu <= 4)
+      dj_slice_kernel<E, DJ_PROBE, 4><<<g, DJ_PNT, 0, st>>>(
+          o.as<unsigned>(), starts.as<unsigned short>(), nt, t.P, C, t.table.as<E>(),
+          res.as<E>(), nullptr);
+    else if (pu >= 32)
+      dj_slice_kernel<E, DJ_PROBE, 32><<<g, DJ_PNT, 0, st>>>(
+          o.as<unsigned>(), starts.as<unsigned short>(), nt, t.P, C, t.table.as<E>(),
+          res.as<E>(), nullptr);
+    else if (pu >= 16)
+      dj_slice_kernel<E, DJ_PROBE, 16><<<g, DJ_PNT, 0, st>>>(
+          o.as<unsigned>(), starts.as<unsigned short>(), nt, t.P, C, t.table.as<E>(),
+          res.as<E>(), nullptr);
+    else
+      dj_slice_kernel<E, DJ_PROBE><<<g, DJ_PNT, 0, st>>>(
+          o.as<unsigned>(), starts.as<unsigned short>(), nt, t.P, C, t.table.as<E>(),
+          res.as<E>(), nullptr);
     FHIP_CHECK(hipGetLastError());
   }
 }
@@ -793,6 +813,8 @@ int dense_join_t(const K* left, const size_t* lidx, size_t nl, const K* right,
                  size_t* npairs, DBuf* vals, DBuf* found, hipStream_t st) {
   DenseTable t;
   bool wide = false;
+  // 32-bit row positions within the partitioned copies
+  if (nl > (size_t(1) << 32) - DJ_TILE || nr > (size_t(1) << 32) - DJ_TILE) return DJ_NA;
   if (!dj_plan<K>(right, ridx, nr, t, wide, st)) return DJ_NA;
   if (wide)
     return dj_run<K, unsigned long long>(left, lidx, nl, right, ridx, nr, t, lo, ro, capacity,

@@ -315,7 +315,10 @@ inline void copy_strided_to_host(const void* src, size_t stride, size_t count, v
 // SEARCH: a lane finds its tile by a binary search over the 64 inclusive
 // sums instead (six shuffles a row, whatever the segment lengths), for
 // segments much shorter than a wave.
-template <int TILE, int U, bool SEARCH = false, class F>
+//
+// PosT: the type of the positions (unsigned when every position of the
+// tiles fits 32 bits, which saves a register per row in flight).
+template <int TILE, int U, bool SEARCH = false, class PosT = size_t, class F>
 __device__ __forceinline__ void segment_rows(const unsigned short* __restrict__ starts,
                                             size_t ntiles, int p, size_t t0, size_t t1,
                                             unsigned wave, unsigned nwaves, F&& f) {
@@ -335,7 +338,7 @@ __device__ __forceinline__ void segment_rows(const unsigned short* __restrict__ 
     const unsigned shift = s - (incl - len);
     unsigned cur = 0;
     for (unsigned r0 = 0; r0 < total; r0 += 64 * U) {
-      size_t pos[U];
+      PosT pos[U];
       bool in[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -361,7 +364,7 @@ __device__ __forceinline__ void segment_rows(const unsigned short* __restrict__ 
         }
         in[u] = r < total;
         const unsigned sh = __shfl(shift, int(cur), 64);
-        pos[u] = in[u] ? (g + cur) * TILE + unsigned(sh + r) : 0;
+        pos[u] = in[u] ? PosT((g + cur) * TILE + unsigned(sh + r)) : PosT(0);
       }
       f(pos, in);
     }

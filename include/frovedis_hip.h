@@ -100,6 +100,13 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *       the workgroups that probe one part (default 128);
  *       "join.dense_lds_build" 0 builds a table of row positions by scatter
  *       and check instead of in LDS parts of 2^15 offsets (default 1);
+ *       "join.dense_probe_u" rows per lane in flight in its probe (4, 8, 16
+ *       or 32, default 32);
+ *   "groupby.range", "groupby.range_max", "groupby.range_chunks"  0 switches
+ *       off the fused group-by's range path (keys within a narrow range:
+ *       parts by key offset, LDS arrays indexed by it); the widest key range
+ *       it takes (at most 2^21, and at most the row count); workgroups per
+ *       part of its aggregation (default: enough for 512 workgroups);
  *   "radix.hist_bpc", "radix.hist_mode"  workgroups per CU of the radix
  *       histogram pass (1 to 64; by default about 16384 keys per workgroup,
  *       8 to 64 workgroups per CU), and how a wave counts its digits: one
