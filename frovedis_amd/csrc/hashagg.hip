@@ -1810,38 +1810,43 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
                                                        ent.as<size_t>());
     FHIP_CHECK(hipGetLastError());
   }
+  // the group count stays on the device (the last inclusive sum): the
+  // gather and the emit take at most min(count, capacity) groups from it, so
+  // the host waits once, at the end
+  const unsigned* gdev = incl.as<unsigned>() + (R - 1);
+  const size_t gmax = std::min<size_t>(size_t(R), capacity);
+  const unsigned long long* wp = w.as<unsigned long long>();
+  const size_t* ep = ent.as<size_t>();
+  if (gmax > 0) {
+    const int g = grid_for(gmax);
+    switch (ktype) {
+      case FHIP_I32: gather_group_keys<int32_t><<<g, 256, 0, st>>>(wp, stride, ep, gmax, static_cast<int32_t*>(keys_out), gdev); break;
+      case FHIP_U32: gather_group_keys<uint32_t><<<g, 256, 0, st>>>(wp, stride, ep, gmax, static_cast<uint32_t*>(keys_out), gdev); break;
+      case FHIP_I64: gather_group_keys<int64_t><<<g, 256, 0, st>>>(wp, stride, ep, gmax, static_cast<int64_t*>(keys_out), gdev); break;
+      default: gather_group_keys<uint64_t><<<g, 256, 0, st>>>(wp, stride, ep, gmax, static_cast<uint64_t*>(keys_out), gdev); break;
+    }
+    FHIP_CHECK(hipGetLastError());
+    if (naggs > 0) {
+      EmitSet es{};
+      es.n = naggs;
+      for (int i = 0; i < naggs; i++) {
+        es.op[i] = aggs[i].op;
+        es.vtype[i] = aggs[i].vtype;
+        es.slot[i] = slot_of[i] < 0 ? 0 : slot_of[i];
+        es.out[i] = aggs[i].out;
+      }
+      emit_kernel<<<g, 256, 0, st>>>(wp, stride, ep, gmax, es, gdev);
+      FHIP_CHECK(hipGetLastError());
+    }
+  }
   unsigned* gh = static_cast<unsigned*>(pinned_scratch(0, sizeof(unsigned)));
-  FHIP_CHECK(hipMemcpyAsync(gh, incl.as<unsigned>() + (R - 1), sizeof(unsigned),
-                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipMemcpyAsync(gh, gdev, sizeof(unsigned), hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
   const size_t G = *gh;
   *ngroups = G;  // reported even when it does not fit, so the caller can resize
   FHIP_REQUIRE(G <= capacity, FHIP_ERR_TOO_LARGE,
                "group_by_agg: output capacity " + std::to_string(capacity) + " < groups " +
                    std::to_string(G));
-  if (G == 0) return;
-  const unsigned long long* wp = w.as<unsigned long long>();
-  const size_t* ep = ent.as<size_t>();
-  switch (ktype) {
-    case FHIP_I32: gather_group_keys<int32_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<int32_t*>(keys_out)); break;
-    case FHIP_U32: gather_group_keys<uint32_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<uint32_t*>(keys_out)); break;
-    case FHIP_I64: gather_group_keys<int64_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<int64_t*>(keys_out)); break;
-    default: gather_group_keys<uint64_t><<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, static_cast<uint64_t*>(keys_out)); break;
-  }
-  FHIP_CHECK(hipGetLastError());
-  if (naggs > 0) {
-    EmitSet es{};
-    es.n = naggs;
-    for (int i = 0; i < naggs; i++) {
-      es.op[i] = aggs[i].op;
-      es.vtype[i] = aggs[i].vtype;
-      es.slot[i] = slot_of[i] < 0 ? 0 : slot_of[i];
-      es.out[i] = aggs[i].out;
-    }
-    emit_kernel<<<grid_for(G), 256, 0, st>>>(wp, stride, ep, G, es);
-    FHIP_CHECK(hipGetLastError());
-  }
-  FHIP_CHECK(hipStreamSynchronize(st));
 }
 
 }  // namespace
