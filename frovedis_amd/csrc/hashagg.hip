@@ -1004,10 +1004,14 @@ __global__ void final_index_kernel(const unsigned* __restrict__ final_n,
     ent[o + j] = size_t(p) * (S + 1) + j;
 }
 
+// g_dev (optional): a group count on the device; at most that many of the
+// G groups are written
 template <class K>
 __global__ void gather_group_keys(const unsigned long long* __restrict__ w, int stride,
-                                  const size_t* __restrict__ ent, size_t G, K* __restrict__ keys) {
+                                  const size_t* __restrict__ ent, size_t G, K* __restrict__ keys,
+                                  const unsigned* __restrict__ g_dev = nullptr) {
   const size_t st = size_t(gridDim.x) * blockDim.x;
+  if (g_dev) G = G < size_t(*g_dev) ? G : size_t(*g_dev);
   for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < G; g += st)
     keys[g] = K(w[ent[g] * stride]);
 }
@@ -1049,8 +1053,10 @@ struct EmitSet {
 // Writes every aggregate in sorted-group order (one thread per group reads
 // the group's entry once for all of them).
 __global__ void emit_kernel(const unsigned long long* __restrict__ w, int stride,
-                            const size_t* __restrict__ ent, size_t G, EmitSet es) {
+                            const size_t* __restrict__ ent, size_t G, EmitSet es,
+                            const unsigned* __restrict__ g_dev = nullptr) {
   const size_t st = size_t(gridDim.x) * blockDim.x;
+  if (g_dev) G = G < size_t(*g_dev) ? G : size_t(*g_dev);
   for (size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x; g < G; g += st) {
     const unsigned long long* e = w + ent[g] * stride;
     const unsigned long long cnt = e[1];
