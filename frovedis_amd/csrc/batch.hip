@@ -162,8 +162,10 @@ __global__ void seg_groups_kernel(const CT* __restrict__ comp, const size_t* __r
 //   bq_jobs  the persistent launch: workgroups take jobs from the queue by
 //            ticket, in queue order -- a small segment, or one bucket of a
 //            large one: its rows sorted in LDS by key (rows packed as
-//            (key - min) << 13 | row, LSD passes of at most
-//            `batch.digit_bits` bits, wave64 ballot ranks: stable), written
+//            (key - min) << 13 | row; keys spanning at most 4096 values by
+//            counting, with each row placed within its key by the packed
+//            rows below it; else LSD passes of at most `batch.digit_bits`
+//            bits, wave64 ballot ranks: stable), written
 //            out, and every group's values folded in row order by one thread
 //            (the reference's sequential fold, so the sums are bit-exact);
 //            each job's group count goes through a decoupled look-back over
@@ -192,6 +194,10 @@ constexpr unsigned long long BQ_REL_LIMIT = 1ull << (32 - BQ_IDX_BITS);
 constexpr int BQ_EARLY_BITS = 15;  // key span of a job whose group count is published early
 static_assert((1 << BQ_EARLY_BITS) / 8 <= BQ_CAP * 4, "the early bitmap lives in the second row buffer");
 static_assert(BQ_CAP == (1 << BQ_IDX_BITS), "job rows must fit the packed row field");
+// key span of a job sorted by counting: one counter per key in the LSD
+// passes' digit counts, four per thread
+constexpr int BQ_FAST_SPAN = BJ_WAVES * 256;
+static_assert(BQ_FAST_SPAN == 4 * BJ_NT && BQ_CAP < 65536, "counting sort: 4 keys per thread, 16-bit starts");
 
 enum : unsigned { BQ_HIST = 0, BQ_TILE = 1, BQ_JOB = 2 };
 constexpr unsigned BQ_SMALL = 0xFFFFFFFFu;
@@ -234,6 +240,7 @@ struct BqArgs {
   unsigned long long key_lim;    // keys must lie in [0, key_lim)
   int digit_bits;
   int early_bits;                // jobs spanning at most this many key bits publish early
+  unsigned rank_max;             // largest group the counting sort ranks by comparisons
   unsigned* ticket;              // [0] tiles, [1] jobs
   unsigned long long* scal;      // [0] key error, [1] groups, [2] overflow
 };
@@ -567,7 +574,9 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   if (tid == 0) {
     misc[0] = ~0ull;  // min
     misc[1] = 0;      // max
+    misc[3] = 0;      // a bucket of the counting sort larger than rank_max
   }
+  for (unsigned i = tid; i < BQ_FAST_SPAN; i += BJ_NT) wcnt[i] = 0;  // counting-sort counts
   __syncthreads();
   const bool fits = m <= BQ_CAP;
   const unsigned wave_base = wave * 64 * IT;
@@ -612,13 +621,22 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   const unsigned long long range = m ? misc[1] - kmin : 0;
   const bool ok = fits && range < BQ_REL_LIMIT;
   if (!ok && tid == 0) atomicOr(&A.scal[2], 1ull);
-  // sort the packed rows by their key bits (stable LSD passes in LDS)
   const int kbits = range ? 64 - __builtin_clzll(range) : 0;
+  // a key span of at most BQ_FAST_SPAN values: a counting sort (below),
+  // which gives the group count and starts from its counts
+  const bool fast = ok && range < BQ_FAST_SPAN;
+  unsigned pv[IT];
+#pragma unroll
+  for (int j = 0; j < IT; j++) {
+    const unsigned p = wave_base + unsigned(j) * 64 + lane;
+    pv[j] = valid[j] && ok ? unsigned(((kv[j] - kmin) << BQ_IDX_BITS) | p) : 0u;
+    if (!ok) valid[j] = false;
+  }
   // the job's group count up front when its keys span at most
   // BQ_EARLY_BITS bits (a bitmap of the keys present, in the second row
   // buffer, which the sort passes only use after it):
   // publishing it now lets later jobs' look-backs pass this job at once
-  const bool early = ok && kbits <= A.early_bits;
+  const bool early = ok && !fast && kbits <= A.early_bits;
   if (early) {
     unsigned* bits = pk(1);
     for (unsigned w = tid; w < (1u << BQ_EARLY_BITS) / 32; w += BJ_NT) bits[w] = 0;
@@ -626,7 +644,7 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
 #pragma unroll
     for (int j = 0; j < IT; j++)
       if (valid[j]) {
-        const unsigned r = unsigned(kv[j] - kmin);
+        const unsigned r = pv[j] >> BQ_IDX_BITS;
         atomicOr(&bits[r >> 5], 1u << (r & 31));
       }
     __syncthreads();
@@ -637,46 +655,105 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
     if (tid == 0)
       bq_lb_store(A.job_state + it.idx, (it.idx == 0 ? CP_FLAG_P : CP_FLAG_A) | (unsigned long long)ge);
   }
-  const int npass = ok ? (kbits + A.digit_bits - 1) / A.digit_bits : 0;
-  const int dbits = npass ? (kbits + npass - 1) / npass : 0;
-  unsigned pv[IT];
+  // Counting sort: per-key counts by LDS atomics; one block scan gives every
+  // key's bucket start and, over the keys present, the group numbers (the
+  // group count is published at once); rows go to their bucket in the
+  // atomics' order, and then each row to its place within the bucket: the
+  // number of the bucket's packed rows smaller than its own.  Packed rows
+  // are unique, so that order is the stable one, whatever order the atomics
+  // took.  A bucket of more than rank_max rows (where counting costs the
+  // square of its size) leaves the ordering to the LSD passes.
+  unsigned cs[4] = {0u, 0u, 0u, 0u}, sv[4] = {0u, 0u, 0u, 0u}, g0 = 0, G = 0;
+  bool ranked = false;
+  if (fast) {
+    unsigned slot[IT];
 #pragma unroll
-  for (int j = 0; j < IT; j++) {
-    const unsigned p = wave_base + unsigned(j) * 64 + lane;
-    pv[j] = valid[j] && ok ? unsigned(((kv[j] - kmin) << BQ_IDX_BITS) | p) : 0u;
-    if (!ok) valid[j] = false;
+    for (int j = 0; j < IT; j++) slot[j] = valid[j] ? atomicAdd(&wcnt[pv[j] >> BQ_IDX_BITS], 1u) : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; i++) cs[i] = wcnt[4 * tid + i];
+    unsigned sum = 0, nz = 0, mx = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      sum += cs[i];
+      nz += cs[i] ? 1u : 0u;
+      mx = cs[i] > mx ? cs[i] : mx;
+    }
+    // rows (at most 2^13) in the low half, keys present in the high half
+    unsigned all = 0;
+    const unsigned ex = block_excl_scan<BJ_NT>(sum | (nz << 16), scan, &all);
+    unsigned st = ex & 0xFFFFu;
+    g0 = ex >> 16;
+    G = all >> 16;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      sv[i] = st;
+      st += cs[i];
+    }
+    // (start, count) of every key
+#pragma unroll
+    for (int i = 0; i < 4; i++) wcnt[4 * tid + i] = sv[i] | (cs[i] << 16);
+    if (mx > A.rank_max) misc[3] = 1;
+    if (tid == 0)
+      bq_lb_store(A.job_state + it.idx, (it.idx == 0 ? CP_FLAG_P : CP_FLAG_A) | (unsigned long long)G);
+    __syncthreads();
+    if (misc[3] == 0) {
+      unsigned* un = pk(1);
+      unsigned ent[IT];
+#pragma unroll
+      for (int j = 0; j < IT; j++) {
+        ent[j] = wcnt[pv[j] >> BQ_IDX_BITS];  // pv is 0 for a row past the end
+        if (valid[j]) un[(ent[j] & 0xFFFFu) + slot[j]] = pv[j];
+      }
+      __syncthreads();
+      unsigned* out = pk(0);
+#pragma unroll
+      for (int j = 0; j < IT; j++) {
+        const unsigned s = ent[j] & 0xFFFFu, c = valid[j] ? ent[j] >> 16 : 0u;
+        unsigned r = 0;
+        for (unsigned i = 0; i < c; i++) r += un[s + i] < pv[j] ? 1u : 0u;
+        if (valid[j]) out[s + r] = pv[j];
+      }
+      ranked = true;
+    }
   }
   int cur = 0;
-  for (int q = 0; q < npass; q++) {
-    const int sh = BQ_IDX_BITS + q * dbits;
-    const unsigned dm = (1u << dbits) - 1;
-    unsigned dig[IT], rank[IT];
+  if (!ranked) {
+    // stable LSD passes in LDS
+    const int npass = ok ? (kbits + A.digit_bits - 1) / A.digit_bits : 0;
+    const int dbits = npass ? (kbits + npass - 1) / npass : 0;
+    for (int q = 0; q < npass; q++) {
+      const int sh = BQ_IDX_BITS + q * dbits;
+      const unsigned dm = (1u << dbits) - 1;
+      unsigned dig[IT], rank[IT];
 #pragma unroll
-    for (int j = 0; j < IT; j++) dig[j] = (pv[j] >> sh) & dm;
-    const unsigned total = bq_rank<BJ_NT, IT>(dig, valid, dbits, wcnt, rank);
-    const unsigned ex = block_excl_scan<BJ_NT>(tid < 256 ? total : 0u, scan, (unsigned*)nullptr);
-    if (tid < 256) tstart[tid] = ex;
-    __syncthreads();
-    unsigned* out = pk(cur ^ 1);
+      for (int j = 0; j < IT; j++) dig[j] = (pv[j] >> sh) & dm;
+      const unsigned total = bq_rank<BJ_NT, IT>(dig, valid, dbits, wcnt, rank);
+      const unsigned ex = block_excl_scan<BJ_NT>(tid < 256 ? total : 0u, scan, (unsigned*)nullptr);
+      if (tid < 256) tstart[tid] = ex;
+      __syncthreads();
+      unsigned* out = pk(cur ^ 1);
 #pragma unroll
-    for (int j = 0; j < IT; j++)
-      if (valid[j]) out[tstart[dig[j]] + wcnt[wave * 256 + dig[j]] + rank[j]] = pv[j];
-    __syncthreads();
-    cur ^= 1;
+      for (int j = 0; j < IT; j++)
+        if (valid[j]) out[tstart[dig[j]] + wcnt[wave * 256 + dig[j]] + rank[j]] = pv[j];
+      __syncthreads();
+      cur ^= 1;
 #pragma unroll
-    for (int j = 0; j < IT; j++) {
-      const unsigned p = wave_base + unsigned(j) * 64 + lane;
-      if (valid[j]) pv[j] = out[p];
+      for (int j = 0; j < IT; j++) {
+        const unsigned p = wave_base + unsigned(j) * 64 + lane;
+        if (valid[j]) pv[j] = out[p];
+      }
+    }
+    if (npass == 0) {
+      unsigned* srt0 = pk(0);
+#pragma unroll
+      for (int j = 0; j < IT; j++) {
+        const unsigned p = wave_base + unsigned(j) * 64 + lane;
+        if (valid[j]) srt0[p] = pv[j];
+      }
     }
   }
   unsigned* srt = pk(cur);
-  if (npass == 0) {
-#pragma unroll
-    for (int j = 0; j < IT; j++) {
-      const unsigned p = wave_base + unsigned(j) * 64 + lane;
-      if (valid[j]) srt[p] = pv[j];
-    }
-  }
   __syncthreads();
   // the sorted rows go out first (they need no group offset)
   const unsigned mm = ok ? unsigned(m) : 0u;
@@ -686,34 +763,44 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
     A.keys_out[a + i] = int64_t(kmin + (v >> BQ_IDX_BITS));
     A.vals_out[a + i] = x;
   }
-  // group starts: a row whose key differs from the row before it
-  unsigned long long fl[IT];
-  unsigned wtot = 0;
-#pragma unroll
-  for (int j = 0; j < IT; j++) {
-    const unsigned p = wave_base + unsigned(j) * 64 + lane;
-    const bool st = valid[j] && (p == 0 || (srt[p] >> BQ_IDX_BITS) != (srt[p - 1] >> BQ_IDX_BITS));
-    fl[j] = __ballot(st);
-    wtot += unsigned(__popcll(fl[j]));
-  }
-  unsigned* wsum = tstart;  // per-wave group counts
-  if (lane == 0) wsum[wave] = wtot;
-  __syncthreads();
-  unsigned before = 0, G = 0;
-#pragma unroll
-  for (int w = 0; w < BJ_WAVES; w++) {
-    const unsigned c = wsum[w];
-    if (unsigned(w) < wave) before += c;
-    G += c;
-  }
-  // group offset: decoupled look-back over the jobs (wave 0)
-  if (wave == 0) {
-    const unsigned long long off = early ? bq_job_resolve(A.job_state, it.idx, G)
-                                         : compact_lookback(A.job_state, it.idx, G);
-    if (lane == 0) misc[2] = off;
-  }
   unsigned* starts = pk(cur ^ 1);  // G entries (the end of the last group is m)
-  {
+  if (fast) {
+    // group starts from the counts
+    if (wave == 0) {
+      const unsigned long long off = bq_job_resolve(A.job_state, it.idx, G);
+      if (lane == 0) misc[2] = off;
+    }
+    unsigned g = g0;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      if (cs[i]) starts[g++] = sv[i];
+  } else {
+    // group starts: a row whose key differs from the row before it
+    unsigned long long fl[IT];
+    unsigned wtot = 0;
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const unsigned p = wave_base + unsigned(j) * 64 + lane;
+      const bool st = valid[j] && (p == 0 || (srt[p] >> BQ_IDX_BITS) != (srt[p - 1] >> BQ_IDX_BITS));
+      fl[j] = __ballot(st);
+      wtot += unsigned(__popcll(fl[j]));
+    }
+    unsigned* wsum = tstart;  // per-wave group counts
+    if (lane == 0) wsum[wave] = wtot;
+    __syncthreads();
+    unsigned before = 0;
+#pragma unroll
+    for (int w = 0; w < BJ_WAVES; w++) {
+      const unsigned c = wsum[w];
+      if (unsigned(w) < wave) before += c;
+      G += c;
+    }
+    // group offset: decoupled look-back over the jobs (wave 0)
+    if (wave == 0) {
+      const unsigned long long off = early ? bq_job_resolve(A.job_state, it.idx, G)
+                                           : compact_lookback(A.job_state, it.idx, G);
+      if (lane == 0) misc[2] = off;
+    }
     unsigned run = before;
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
 #pragma unroll
@@ -943,7 +1030,7 @@ struct QPlan {
        *gcount = nullptr, *gseg_off = nullptr;
   size_t capacity = 0;
   std::vector<size_t> off;  // seg_off contents
-  long long tunables[4] = {0, 0, 0, 0};
+  long long tunables[5] = {0, 0, 0, 0, 0};
   std::vector<long long> class_rows;  // batch.bucket_rows.c<k> for every k (-1: unset)
   unsigned nitems = 0, njobs = 0, nlarge = 0, nchunks = 0, ntiles = 0;
   size_t lb_words = 0;
@@ -959,9 +1046,7 @@ struct QPlan {
            nseg == o.nseg && key_bits == o.key_bits && keys_out == o.keys_out &&
            vals_out == o.vals_out && gkeys == o.gkeys && gsum == o.gsum && gcount == o.gcount &&
            gseg_off == o.gseg_off && capacity == o.capacity && off == o.off &&
-           tunables[0] == o.tunables[0] && tunables[1] == o.tunables[1] &&
-           tunables[2] == o.tunables[2] && tunables[3] == o.tunables[3] &&
-           class_rows == o.class_rows;
+           std::equal(tunables, tunables + 5, o.tunables) && class_rows == o.class_rows;
   }
 };
 
@@ -1069,6 +1154,7 @@ void enqueue_queue(QPlan& q, hipStream_t st) {
   A.key_lim = q.key_bits >= 64 ? ~0ull : (1ull << q.key_bits);
   A.digit_bits = int(q.tunables[0]);
   A.early_bits = int(q.tunables[3]);
+  A.rank_max = unsigned(q.tunables[4]);
   A.ticket = reinterpret_cast<unsigned*>(scal + 4);
   A.scal = scal;
   {
@@ -1116,6 +1202,7 @@ bool queue_sort_groupby(const void* keys, const void* vals, size_t n, const size
   q->tunables[1] = tune_get("batch.bucket_rows", 7168);
   q->tunables[2] = tune_get("batch.hist_chunk", 1 << 16);
   q->tunables[3] = std::max(-1LL, std::min((long long)BQ_EARLY_BITS, tune_get("batch.early_bits", BQ_EARLY_BITS)));
+  q->tunables[4] = std::max(0LL, std::min((long long)BQ_CAP, tune_get("batch.rank_max", 64)));
   for (int c = 0; c < 64; c++)
     q->class_rows.push_back(tune_get(("batch.bucket_rows.c" + std::to_string(c)).c_str(), -1));
   if (!(g_qplan && g_qplan->same(*q))) {

@@ -125,7 +125,10 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *       for segments of 2^(k-1) to 2^k rows;
  *   "batch.hist_chunk", "batch.early_bits"  rows per histogram workgroup of
  *       the large segments (default 65536) and the key span (bits, default
- *       15) up to which a job publishes its group count before sorting.
+ *       15) up to which a job publishes its group count before sorting;
+ *   "batch.rank_max"    a job whose keys span at most 4096 values is sorted
+ *       by counting when no key has more than this many rows (default 64;
+ *       0: always the LSD passes).
  * A negative value removes the setting (back to the built-in default).  No
  * counterpart in the reference. */
 int fhip_tune_set(const char* name, long long value);

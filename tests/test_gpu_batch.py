@@ -156,3 +156,30 @@ def test_batch_composite_path_matches_oracle(fa):
             _run_and_check(fa, keys, vals, off, 18, use_graph)
     finally:
         at.tune_set("batch.path", None)
+
+
+@pytest.mark.parametrize("rank_max", [None, 0, 5, 8192])
+def test_batch_counting_and_lsd_orders(fa, rank_max):
+    # jobs whose keys span at most 4096 values are sorted by counting, unless
+    # a key holds more than batch.rank_max rows; wider spans take the LSD
+    # passes (with and without the early group count): every mix must give
+    # the reference's order and folds
+    from frovedis_amd import autotune as at
+    rng = np.random.default_rng(11 if rank_max is None else rank_max + 12)
+    parts = [rng.integers(0, 4096, 8192),          # span 4096: counting, full job
+             np.r_[0, 4095, rng.integers(0, 4096, 5000)],   # range 4095: counting
+             np.r_[0, 4096, rng.integers(0, 4097, 5000)],   # range 4096: LSD
+             rng.integers(0, 10, 3000),            # ~300 rows per key
+             np.full(777, 12345),                  # one key
+             rng.integers(0, 40000, 6000),         # past the early bitmap's span
+             rng.integers(1000, 1200, 900) * 16,   # sparse keys, span 3200
+             rng.integers(0, 3000, 30000)]         # a large segment in buckets
+    keys = np.concatenate(parts).astype(np.int64)
+    vals = rng.standard_normal(len(keys))
+    off = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.uint64)
+    at.tune_set("batch.rank_max", rank_max)
+    try:
+        for use_graph in (False, True):
+            _run_and_check(fa, keys, vals, off, 18, use_graph)
+    finally:
+        at.tune_set("batch.rank_max", None)
