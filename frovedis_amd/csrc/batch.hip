@@ -198,6 +198,11 @@ static_assert(BQ_CAP == (1 << BQ_IDX_BITS), "job rows must fit the packed row fi
 // passes' digit counts, four per thread
 constexpr int BQ_FAST_SPAN = BJ_WAVES * 256;
 static_assert(BQ_FAST_SPAN == 4 * BJ_NT && BQ_CAP < 65536, "counting sort: 4 keys per thread, 16-bit starts");
+// the counting sort's largest bucket ranked by comparisons, and the
+// sentinel words after the rows that its rounds of four may read
+constexpr int BQ_RANK_MAX = 64;
+constexpr int BQ_RANK_PAD = 128;
+static_assert(BQ_RANK_MAX + 3 < BQ_RANK_PAD, "rank rounds stay within the sentinels");
 
 enum : unsigned { BQ_HIST = 0, BQ_TILE = 1, BQ_JOB = 2 };
 constexpr unsigned BQ_SMALL = 0xFFFFFFFFu;
@@ -255,8 +260,8 @@ constexpr int BT_OFF_SCAN = BT_OFF_MAP + BQ_FINE;             // BQ_WAVES x 4
 constexpr int BT_LDS = BT_OFF_SCAN + 64;
 // LDS (bytes) of a job: two packed-row buffers, the sorted values, ranks
 constexpr int BJ_OFF_PK0 = 0;                                 // BQ_CAP x 4
-constexpr int BJ_OFF_PK1 = BJ_OFF_PK0 + BQ_CAP * 4;           // BQ_CAP x 4
-constexpr int BJ_OFF_WCNT = BJ_OFF_PK1 + BQ_CAP * 4;          // BJ_WAVES x 256 x 4
+constexpr int BJ_OFF_PK1 = BJ_OFF_PK0 + BQ_CAP * 4;           // (BQ_CAP + BQ_RANK_PAD) x 4
+constexpr int BJ_OFF_WCNT = BJ_OFF_PK1 + (BQ_CAP + BQ_RANK_PAD) * 4;  // BJ_WAVES x 256 x 4
 constexpr int BJ_OFF_TSTART = BJ_OFF_WCNT + BJ_WAVES * 256 * 4;
 constexpr int BJ_OFF_SCAN = BJ_OFF_TSTART + 256 * 4;          // BQ_WAVES x 4
 constexpr int BJ_OFF_MISC = BJ_OFF_SCAN + 64;                 // 8 x 8
@@ -705,15 +710,30 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
         ent[j] = wcnt[pv[j] >> BQ_IDX_BITS];  // pv is 0 for a row past the end
         if (valid[j]) un[(ent[j] & 0xFFFFu) + slot[j]] = pv[j];
       }
+      if (tid < unsigned(BQ_RANK_PAD)) un[m + tid] = ~0u;  // above every packed row
       __syncthreads();
-      unsigned* out = pk(0);
+      // four bucket rows of every item per round, as many rounds as the
+      // wave's largest bucket needs: a read past a bucket's end finds a
+      // larger key or a sentinel, so it counts nothing
+      unsigned r[IT], cmax = 0;
 #pragma unroll
       for (int j = 0; j < IT; j++) {
-        const unsigned s = ent[j] & 0xFFFFu, c = valid[j] ? ent[j] >> 16 : 0u;
-        unsigned r = 0;
-        for (unsigned i = 0; i < c; i++) r += un[s + i] < pv[j] ? 1u : 0u;
-        if (valid[j]) out[s + r] = pv[j];
+        if (!valid[j]) ent[j] &= 0xFFFFu;  // count 0
+        r[j] = 0;
+        cmax = (ent[j] >> 16) > cmax ? ent[j] >> 16 : cmax;
       }
+      for (unsigned i = 0; __ballot(i < cmax); i += 4) {
+#pragma unroll
+        for (int j = 0; j < IT; j++) {
+          const unsigned* b = un + (ent[j] & 0xFFFFu) + i;
+#pragma unroll
+          for (int u = 0; u < 4; u++) r[j] += b[u] < pv[j] ? 1u : 0u;
+        }
+      }
+      unsigned* out = pk(0);
+#pragma unroll
+      for (int j = 0; j < IT; j++)
+        if (valid[j]) out[(ent[j] & 0xFFFFu) + r[j]] = pv[j];
       ranked = true;
     }
   }
@@ -757,11 +777,24 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   __syncthreads();
   // the sorted rows go out first (they need no group offset)
   const unsigned mm = ok ? unsigned(m) : 0u;
-  for (unsigned i = tid; i < mm; i += BJ_NT) {
-    const unsigned v = srt[i];
-    const double x = vin[v & (BQ_CAP - 1)];
-    A.keys_out[a + i] = int64_t(kmin + (v >> BQ_IDX_BITS));
-    A.vals_out[a + i] = x;
+  {
+    unsigned v[IT];
+    double x[IT];
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const unsigned i = unsigned(j) * BJ_NT + tid;
+      v[j] = srt[i < mm ? i : 0u];
+    }
+#pragma unroll
+    for (int j = 0; j < IT; j++) x[j] = vin[v[j] & (BQ_CAP - 1)];
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
+      const unsigned i = unsigned(j) * BJ_NT + tid;
+      if (i < mm) {
+        A.keys_out[a + i] = int64_t(kmin + (v[j] >> BQ_IDX_BITS));
+        A.vals_out[a + i] = x[j];
+      }
+    }
   }
   unsigned* starts = pk(cur ^ 1);  // G entries (the end of the last group is m)
   if (fast) {
@@ -822,7 +855,18 @@ __device__ void bq_job(const BqArgs& A, const BqItem& it, unsigned char* lds) {
   for (unsigned g = tid; g < G; g += BJ_NT) {
     const unsigned s = starts[g], e = g + 1 < G ? starts[g + 1] : mm;
     double total = 0;
-    for (unsigned i = s; i < e; i++) total += vin[srt[i] & (BQ_CAP - 1)];
+    // four rows per round (their loads together), added in row order
+    for (unsigned i = s; i < e; i += 4) {
+      unsigned v[4];
+      double x[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) v[u] = srt[i + u < e ? i + u : e - 1];
+#pragma unroll
+      for (int u = 0; u < 4; u++) x[u] = vin[v[u] & (BQ_CAP - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (i + u < e) total += x[u];
+    }
     const unsigned long long o = goff + g;
     if (o < A.capacity) {
       A.gkeys[o] = int64_t(kmin + (srt[s] >> BQ_IDX_BITS));
@@ -1202,7 +1246,7 @@ bool queue_sort_groupby(const void* keys, const void* vals, size_t n, const size
   q->tunables[1] = tune_get("batch.bucket_rows", 7168);
   q->tunables[2] = tune_get("batch.hist_chunk", 1 << 16);
   q->tunables[3] = std::max(-1LL, std::min((long long)BQ_EARLY_BITS, tune_get("batch.early_bits", BQ_EARLY_BITS)));
-  q->tunables[4] = std::max(0LL, std::min((long long)BQ_CAP, tune_get("batch.rank_max", 64)));
+  q->tunables[4] = std::max(0LL, std::min((long long)BQ_RANK_MAX, tune_get("batch.rank_max", BQ_RANK_MAX)));
   for (int c = 0; c < 64; c++)
     q->class_rows.push_back(tune_get(("batch.bucket_rows.c" + std::to_string(c)).c_str(), -1));
   if (!(g_qplan && g_qplan->same(*q))) {
