@@ -44,7 +44,6 @@ constexpr int L2_NT = 512;
 constexpr int L2_ITEMS = 16;
 constexpr int L2_CHUNK = L2_NT * L2_ITEMS;  // keys per second-level workgroup
 constexpr int LS_NT = 512;
-constexpr int LS_ITEMS = 32;       // keys per thread in the L1 scatter
 constexpr int L3_NT = 512;
 constexpr int L3_CAP = 8192;       // keys a fine bucket may hold (LDS)
 constexpr int L3_ITEMS = L3_CAP / L3_NT;
@@ -140,52 +139,76 @@ __global__ __launch_bounds__(LS_NT) void l1_count_kernel(const uint64_t* __restr
     if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
 }
 
-__global__ __launch_bounds__(LS_NT) void l1_scatter_kernel(const uint64_t* __restrict__ keys,
-                                                           size_t n, int is_signed,
-                                                           const uint64_t* __restrict__ coarse,
-                                                           unsigned long long* __restrict__ cursor,
-                                                           uint64_t* __restrict__ out) {
+// The tile's keys are staged in LDS in bucket order before they are
+// written, so every bucket's keys of the tile leave as one contiguous run
+// (the key-only sort need not be stable: a key's place within its bucket's
+// run is its LDS atomic's ticket).  One thread per coarse bucket for the
+// tile starts (block scan) and the global reservation (one atomic).
+constexpr int L1S_NT = 1024;
+constexpr int L1S_ITEMS = 12;
+constexpr int L1S_TILE = L1S_NT * L1S_ITEMS;
+constexpr int L1S_LDS = L1S_TILE * (8 + 2);  // staged keys, their buckets
+static_assert(L1S_NT == LS_B1 && L1S_TILE < 65536, "one coarse bucket per thread");
+
+__global__ __launch_bounds__(L1S_NT) void l1_scatter_kernel(const uint64_t* __restrict__ keys,
+                                                            size_t n, int is_signed,
+                                                            const uint64_t* __restrict__ coarse,
+                                                            unsigned long long* __restrict__ cursor,
+                                                            uint64_t* __restrict__ out) {
+  extern __shared__ unsigned long long l1s_lds[];
   __shared__ uint64_t spl[LS_B1];
-  __shared__ unsigned cnt[LS_B1];
-  __shared__ unsigned long long base[LS_B1];
-  for (int i = threadIdx.x; i < LS_B1; i += LS_NT) {
-    spl[i] = coarse[i];
-    cnt[i] = 0;
-  }
+  __shared__ unsigned cnt[LS_B1];             // counts, then tile starts
+  __shared__ unsigned long long base[LS_B1];  // global slot of tile position 0 of the bucket
+  __shared__ unsigned scan_tmp[L1S_NT / 64];
+  uint64_t* buf = reinterpret_cast<uint64_t*>(l1s_lds);
+  unsigned short* bid = reinterpret_cast<unsigned short*>(buf + L1S_TILE);
+  const unsigned tid = threadIdx.x;
+  spl[tid] = coarse[tid];
+  cnt[tid] = 0;
   __syncthreads();
-  const size_t tile = size_t(blockIdx.x) * LS_NT * LS_ITEMS;
-  uint64_t k[LS_ITEMS];
-  int b[LS_ITEMS];
+  const size_t tile = size_t(blockIdx.x) * L1S_TILE;
+  const unsigned tn = unsigned(std::min<size_t>(L1S_TILE, n - tile));
+  uint64_t k[L1S_ITEMS];
+  int b[L1S_ITEMS];
+  unsigned r[L1S_ITEMS];
 #pragma unroll
-  for (int j = 0; j < LS_ITEMS; j++) {
-    const size_t i = tile + size_t(j) * LS_NT + threadIdx.x;
-    k[j] = i < n ? keys[i] : 0;
+  for (int j = 0; j < L1S_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L1S_NT + tid;
+    k[j] = keys[tile + (i < tn ? i : 0u)];
     b[j] = 0;
   }
 #pragma unroll
   for (int st = LS_B1 / 2; st >= 1; st >>= 1) {
 #pragma unroll
-    for (int j = 0; j < LS_ITEMS; j++)
+    for (int j = 0; j < L1S_ITEMS; j++)
       if (spl[b[j] + st] <= to_ordered(k[j], is_signed)) b[j] += st;
   }
 #pragma unroll
-  for (int j = 0; j < LS_ITEMS; j++) {
-    const size_t i = tile + size_t(j) * LS_NT + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[b[j]], 1u);
-    else b[j] = -1;
+  for (int j = 0; j < L1S_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L1S_NT + tid;
+    r[j] = i < tn ? atomicAdd(&cnt[b[j]], 1u) : 0u;
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < LS_B1; i += LS_NT) {
-    unsigned c = cnt[i];
-    base[i] = c ? atomicAdd(&cursor[i], (unsigned long long)c) : 0;
-    cnt[i] = 0;
+  const unsigned c = cnt[tid];
+  const unsigned ts = block_excl_scan<L1S_NT>(c, scan_tmp, (unsigned*)nullptr);
+  // modulo 2^64: base + (tile position) is the bucket's global slot
+  base[tid] = c ? atomicAdd(&cursor[tid], (unsigned long long)c) - ts : 0ull;
+  cnt[tid] = ts;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < L1S_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L1S_NT + tid;
+    if (i < tn) {
+      const unsigned pos = cnt[b[j]] + r[j];
+      buf[pos] = k[j];
+      bid[pos] = (unsigned short)b[j];
+    }
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < LS_ITEMS; j++) {
-    if (b[j] < 0) continue;
-    unsigned r = atomicAdd(&cnt[b[j]], 1u);
-    out[base[b[j]] + r] = k[j];
+  for (int j = 0; j < L1S_ITEMS; j++) {
+    const unsigned i = unsigned(j) * L1S_NT + tid;
+    if (i < tn) out[base[bid[i]] + i] = buf[i];
   }
 }
 
@@ -901,8 +924,14 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
                             hipMemcpyHostToDevice, st));
   {
     ProfScope prof("learned_scatter", st);
-    const unsigned grid = unsigned(ceil_div(n, size_t(LS_NT) * LS_ITEMS));
-    l1_scatter_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, cursor, t1.as<uint64_t>());
+    static bool attr = false;
+    if (!attr) {
+      FHIP_CHECK(hipFuncSetAttribute((const void*)l1_scatter_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, L1S_LDS));
+      attr = true;
+    }
+    const unsigned grid = unsigned(ceil_div(n, size_t(L1S_TILE)));
+    l1_scatter_kernel<<<grid, L1S_NT, L1S_LDS, st>>>(k, n, sgn, dco, cursor, t1.as<uint64_t>());
     FHIP_CHECK(hipGetLastError());
   }
   {
