@@ -55,6 +55,26 @@ class DeviceArray:
                ("f", 8): torch.float64}[(dt.kind, dt.itemsize)]
         return cls(torch.empty(int(n), dtype=tdt, device=device), dt)
 
+    @classmethod
+    def empty_many(cls, n, dtypes, device="cuda"):
+        """Arrays of n elements of each dtype, carved from one device
+        allocation (256-byte aligned parts): one allocator call for the
+        several outputs of one operation."""
+        torch = _torch()
+        dts = [np.dtype(d) for d in dtypes]
+        offs, total = [], 0
+        for dt in dts:
+            offs.append(total)
+            total += (int(n) * dt.itemsize + 255) // 256 * 256
+        buf = torch.empty(max(total, 1), dtype=torch.uint8, device=device)
+        out = []
+        for dt, o in zip(dts, offs):
+            tdt = {("i", 4): torch.int32, ("u", 4): torch.int32, ("i", 8): torch.int64,
+                   ("u", 8): torch.int64, ("f", 4): torch.float32,
+                   ("f", 8): torch.float64}[(dt.kind, dt.itemsize)]
+            out.append(cls(buf[o:o + int(n) * dt.itemsize].view(tdt), dt))
+        return out
+
     @property
     def code(self):
         return TYPE_CODE[self.dtype]

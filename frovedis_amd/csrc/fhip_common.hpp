@@ -263,11 +263,27 @@ int fill_blocks(int threads_per_block, int blocks_per_cu = 8);
 // The heavy keys of a column known in advance (from a model of the column):
 // key bits (sign-extended for signed keys) and their counts, the heaviest
 // first, counted over `total` rows.
+struct HeavyKeys;
+
+// The heavy-key table placed from a HeavyKeys list, kept by the list's owner
+// (a column model) so that repeated group-bys do not place it again.  The
+// table depends on the list and on the slot count and coverage rule in force
+// (both tunables), which are the cache key.
+struct HotPlanCache {
+  std::mutex mu;
+  bool valid = false;
+  int Sh = 0;               // slots (0: no plan)
+  long long min_pct = -1;   // coverage rule the plan was made with
+  bool used = false;        // the plan passed the coverage rule
+  std::vector<unsigned long long> table;
+};
+
 struct HeavyKeys {
   const unsigned long long* keys = nullptr;
   const unsigned long long* counts = nullptr;
   size_t n = 0;
   size_t total = 0;
+  HotPlanCache* cache = nullptr;  // optional, owned by the caller
 };
 
 // The fused hash group-by + aggregation (hashagg.hip); heavy (optional)

@@ -1302,17 +1302,42 @@ int hot_slots(int S) {
 
 // The heavy-key plan from known heavy keys: the same coverage rule as the
 // sampled plan (the kept keys must hold groupby.hot_min_pct % of the rows)
+// The plan depends on the list, the slot count and the coverage rule only, so
+// a list with a cache (a column model's) is placed once per (Sh, rule).
 HotPlan plan_hot_known(const HeavyKeys& hk, size_t n, int S) {
   const int Sh = hot_slots(S);
   if (!Sh || hk.n == 0 || hk.total == 0 || n < (size_t(1) << 16)) return HotPlan();
+  const long long min_pct = tune_get("groupby.hot_min_pct", 30);
+  std::unique_lock<std::mutex> lock;
+  if (hk.cache) {
+    lock = std::unique_lock<std::mutex>(hk.cache->mu);
+    const HotPlanCache& c = *hk.cache;
+    if (c.valid && c.Sh == Sh && c.min_pct == min_pct) {
+      HotPlan h;
+      if (c.used) {
+        h.used = true;
+        h.Sh = c.Sh;
+        h.table = c.table;
+      }
+      return h;
+    }
+  }
   std::vector<std::pair<size_t, unsigned long long>> runs(hk.n);
   for (size_t j = 0; j < hk.n; j++) runs[j] = {size_t(hk.counts[j]), hk.keys[j]};
   const size_t keep = std::min(runs.size(), size_t(Sh) * 4 / 5);
   size_t covered = 0;
   for (size_t j = 0; j < keep; j++) covered += runs[j].first;
-  const long long min_pct = tune_get("groupby.hot_min_pct", 30);
-  if (covered * 100 < size_t(std::max(0LL, min_pct)) * hk.total) return HotPlan();
-  return place_hot(runs, Sh);
+  HotPlan h;
+  if (covered * 100 >= size_t(std::max(0LL, min_pct)) * hk.total) h = place_hot(runs, Sh);
+  if (hk.cache) {
+    HotPlanCache& c = *hk.cache;
+    c.valid = true;
+    c.Sh = Sh;
+    c.min_pct = min_pct;
+    c.used = h.used;
+    c.table = h.table;
+  }
+  return h;
 }
 
 HotPlan plan_hot(int ktype, const void* keys, size_t n, int S, hipStream_t st) {

@@ -627,6 +627,13 @@ struct fhip_cdf_model {
   // heavy-key table
   std::vector<unsigned long long> heavy_keys, heavy_counts;
   double final_loss = 0;
+  // the group-by's heavy-key table placed from heavy_keys, and the last
+  // cardinality estimate (n, distinct): both are pure functions of the
+  // model, kept so that repeated queries skip the host work
+  fhip::HotPlanCache hot_plan;
+  std::mutex card_mu;
+  size_t card_n = 0;
+  double card_val = -1.0;
 };
 
 namespace fhip {
@@ -1000,7 +1007,7 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
 // Expected number of distinct keys among n draws from the model's CDF:
 // sum over knot intervals of w * (1 - exp(-n * mass / w)) with w the number
 // of integer keys in the interval.
-double learned_cardinality(const fhip_cdf_model* m, size_t n) {
+double learned_cardinality_uncached(const fhip_cdf_model* m, size_t n) {
   // a training column whose keys were all distinct (row ids, surrogate
   // keys): its draws were not independent, and n new rows of it are
   // expected to be distinct too
@@ -1020,6 +1027,17 @@ double learned_cardinality(const fhip_cdf_model* m, size_t n) {
     prev_c = c;
   }
   return std::min(total, double(n));
+}
+
+// 4095 knots of exp2 and exp per estimate: the last (n, estimate) is kept
+double learned_cardinality(const fhip_cdf_model* m, size_t n) {
+  auto* mm = const_cast<fhip_cdf_model*>(m);
+  std::lock_guard<std::mutex> lock(mm->card_mu);
+  if (mm->card_val < 0.0 || mm->card_n != n) {
+    mm->card_val = learned_cardinality_uncached(m, n);
+    mm->card_n = n;
+  }
+  return mm->card_val;
 }
 
 }  // namespace fhip
@@ -1114,6 +1132,7 @@ int fhip_dev_group_by_agg_model(int ktype, const void* keys, size_t n, const fhi
       hk.counts = model->heavy_counts.data();
       hk.n = model->heavy_keys.size();
       hk.total = model->train_n;
+      hk.cache = const_cast<fhip::HotPlanCache*>(&model->hot_plan);
     }
     fhip::group_by_agg(ktype, keys, n, aggs, naggs, keys_out, capacity, ngroups, hint,
                        fhip::as_stream(stream), use ? &hk : nullptr);

@@ -125,12 +125,24 @@ __global__ __launch_bounds__(RS_NT) void rs_hist(const U* __restrict__ keys, siz
       const U xk = key_xform(k[j], mode);
       const unsigned dmask = desc ? 0xFFu : 0u;
       U dx = 0;
-      if constexpr (HM == 4) dx = xk ^ __shfl(xk, 0, 64);  // lane 0 is always valid here
+      bool few = false;  // fewer than 8 lanes share any counted byte with lane 0
+      if constexpr (HM == 4) {
+        dx = xk ^ __shfl(xk, 0, 64);  // lane 0 is always valid here
+        // a lane that matches lane 0 in some byte has a zero byte in dx
+        // (bytes past nbytes forced non-zero); when fewer than 8 lanes do,
+        // no byte reaches the 8 matches of the one-atomic case, and every
+        // byte takes one atomic per lane without its ballot
+        constexpr U ONES = U(~U(0)) / U(0xFF);  // 0x01 in every byte
+        const U hi = nbytes >= int(sizeof(U)) ? U(0) : U(~U(0)) << (8 * nbytes);
+        const U z = dx | hi;
+        const bool has0 = ((z - ONES) & ~z & (ONES << 7)) != U(0);
+        few = __popcll(__ballot(valid && has0)) < 8;
+      }
 #pragma unroll
       for (int b = 0; b < int(sizeof(U)); b++) {
         if (b >= nbytes) break;
         const unsigned d = (unsigned(xk >> (8 * b)) & 0xFFu) ^ dmask;
-        if constexpr (HM == 3) {
+        if (HM == 3 || few) {
           if (valid) atomicAdd(&h[b * 256 + d], 1u);
         } else {
           bool eq0;

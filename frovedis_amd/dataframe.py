@@ -77,16 +77,17 @@ def group_by_agg(keys, aggs, table_hint=0, capacity=None, stream=None, model=Non
     n = len(keys)
     dev = keys.t.device
     cap = int(capacity) if capacity is not None else max(1, min(n, 1 << 22))
+    odts = []
+    for op, col in aggs:
+        if op not in AGG_OPS:
+            raise FhipError(f"unknown aggregation {op!r}")
+        odts.append({"count": np.uint64, "avg": np.float64}.get(op, None if col is None else col.dtype))
     for _ in range(2):
-        kout = DeviceArray.empty(cap, keys.dtype, dev)
-        outs = []
+        # the group keys and every output in one allocation
+        kout, *outs = DeviceArray.empty_many(cap, [keys.dtype] + odts, dev)
         specs = (_Agg * max(1, len(aggs)))()
         for i, (op, col) in enumerate(aggs):
-            if op not in AGG_OPS:
-                raise FhipError(f"unknown aggregation {op!r}")
-            odt = {"count": np.uint64, "avg": np.float64}.get(op, None if col is None else col.dtype)
-            out = DeviceArray.empty(cap, odt, dev)
-            outs.append(out)
+            out = outs[i]
             specs[i].op = AGG_OPS[op]
             specs[i].vtype = col.code if col is not None else 0
             specs[i].col = col.ptr.value if col is not None else None
