@@ -290,6 +290,6 @@ struct HeavyKeys {
 // replaces the sample the heavy-key table is planned from.
 void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
                   void* keys_out, size_t capacity, size_t* ngroups, size_t table_hint,
-                  hipStream_t st, const HeavyKeys* heavy = nullptr);
+                  hipStream_t st, const HeavyKeys* heavy = nullptr, bool narrow_cold = true);
 
 }  // namespace fhip

@@ -738,10 +738,11 @@ __device__ __forceinline__ int hot_slot(const unsigned long long* hk, int shift,
 // sized to the accumulators actually loaded)
 // Up to two accumulators, the registers fit two 1024-thread workgroups per
 // CU (8 waves per SIMD); the heavy-key table of 2048 slots leaves LDS for both.
-template <class K, int NA>
+// CK: the cold keys as written, K or (for the range path) the low 32 bits.
+template <class K, int NA, class CK>
 __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
     const K* __restrict__ keys, size_t n, AccSet accs, const unsigned long long* __restrict__ hot_keys,
-    int Sh, unsigned long long* __restrict__ partial, K* __restrict__ cold_keys, Cols cold,
+    int Sh, unsigned long long* __restrict__ partial, CK* __restrict__ cold_keys, Cols cold,
     unsigned long long* __restrict__ cold_n, unsigned long long* __restrict__ seg_n, int cpbits,
     unsigned long long* __restrict__ ccnt, unsigned long long* __restrict__ krange) {
   extern __shared__ unsigned long long lds_u64[];
@@ -836,7 +837,7 @@ __global__ __launch_bounds__(HOT_NT, NA <= 2 ? 8 : 4) void hot_agg_kernel(
         kmx = ob > kmx ? ob : kmx;
         const size_t pos = r0 + base + unsigned(__popcll(cm[u] & lt));
         if (CP) atomicAdd(&chist[part_of(mix64(key_bits(kv[u])), cpbits)], 1u);
-        cold_keys[pos] = kv[u];
+        cold_keys[pos] = CK(kv[u]);
 #pragma unroll
         for (int a = 0; a < NA; a++) {
           if (cold.bytes[a] == 4) static_cast<unsigned*>(cold.out[a])[pos] = unsigned(vv[u][a]);
@@ -1522,7 +1523,9 @@ __global__ __launch_bounds__(RP_NT) void rp_part_kernel(
 #pragma unroll
     for (int j = 0; j < RP_ITEMS; j++) {
       const unsigned i = unsigned(j) * RP_NT + threadIdx.x;
-      kb[j] = key_bits(kt[i < rows ? i : 0u]) - kmin;
+      // the range spans less than 2^32 keys: the offset is the low 32 bits
+      // of the difference (cold keys kept as their low 32 bits give the same)
+      kb[j] = (unsigned long long)unsigned(key_bits(kt[i < rows ? i : 0u]) - kmin);
     }
     __syncthreads();
 #pragma unroll
@@ -1698,13 +1701,15 @@ __global__ __launch_bounds__(256) void rp_combine_kernel(
   }
 }
 
-// The heavy keys' entries ({key bits, count, accumulators}, hot_n of them)
-// go to their key's entry of `w`.
+// The heavy keys' entries ({key bits, count, accumulators}, *hot_n of them,
+// a count on the device) go to their key's entry of `w`.
 __global__ __launch_bounds__(256) void rp_hot_kernel(const unsigned long long* __restrict__ hot,
-                                                     unsigned hot_n, unsigned long long kmin,
+                                                     const unsigned* __restrict__ hot_n,
+                                                     unsigned long long kmin,
                                                      int stride, unsigned long long* __restrict__ w,
                                                      unsigned* __restrict__ flag) {
-  for (unsigned j = blockIdx.x * 256 + threadIdx.x; j < hot_n; j += gridDim.x * 256) {
+  const unsigned m = *hot_n;
+  for (unsigned j = blockIdx.x * 256 + threadIdx.x; j < m; j += gridDim.x * 256) {
     const unsigned long long* src = hot + size_t(j) * stride;
     const unsigned long long k = src[0] - kmin;
     for (int q = 0; q < stride; q++) w[k * stride + q] = src[q];
@@ -1734,11 +1739,13 @@ int rp_part_bits(int na) {
   return pkb;
 }
 
-void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /*unused*/,
+// cktype: the type of the cold keys as the heavy-key pass wrote them (the
+// key type, or FHIP_U32 for 64-bit keys kept as their low 32 bits)
+void group_by_agg_range(int ktype, int cktype, size_t n, const AccSet& accs,
                         const fhip_agg* aggs, int naggs, const std::vector<int>& slot_of,
                         const void* ckeys, const unsigned long long* seg_h, size_t nregions,
-                        const unsigned long long* hot_fin, unsigned hot_groups,
-                        unsigned long long kmin, unsigned long long R, void* keys_out,
+                        const unsigned long long* hot_fin, unsigned hot_max,
+                        const unsigned* hot_n, unsigned long long kmin, unsigned long long R, void* keys_out,
                         size_t capacity, size_t* ngroups, hipStream_t st, const DBuf* cold_cols) {
   const int na = accs.n;
   const int stride = 2 + na;
@@ -1784,7 +1791,7 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
         default: rp_part_kernel<KK, 4><<<g, RP_NT, 0, st>>>(kp, cold, tiles.as<unsigned long long>(), ntiles, kmin, P, lk.as<unsigned short>(), pout, starts.as<unsigned short>()); break;
       }
     };
-    switch (ktype) {
+    switch (cktype) {
       case FHIP_I32: launch(int32_t()); break;
       case FHIP_U32: launch(uint32_t()); break;
       case FHIP_I64: launch(int64_t()); break;
@@ -1830,8 +1837,8 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
         pcnt.as<unsigned>(), pacc.as<unsigned long long>(), C, R, P, pkb, accs, kmin, stride,
         w.as<unsigned long long>(), flag.as<unsigned>());
     FHIP_CHECK(hipGetLastError());
-    if (hot_groups) {
-      rp_hot_kernel<<<grid_for(hot_groups), 256, 0, st>>>(hot_fin, hot_groups, kmin, stride,
+    if (hot_max) {
+      rp_hot_kernel<<<grid_for(hot_max), 256, 0, st>>>(hot_fin, hot_n, kmin, stride,
                                                          w.as<unsigned long long>(),
                                                          flag.as<unsigned>());
       FHIP_CHECK(hipGetLastError());
@@ -1884,7 +1891,7 @@ void group_by_agg_range(int ktype, size_t n, const AccSet& accs, const AccSet& /
 
 void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, int naggs,
                   void* keys_out, size_t capacity, size_t* ngroups, size_t table_hint,
-                  hipStream_t st, const HeavyKeys* heavy) {
+                  hipStream_t st, const HeavyKeys* heavy, bool narrow_cold) {
   FHIP_REQUIRE(ktype == FHIP_I32 || ktype == FHIP_U32 || ktype == FHIP_I64 || ktype == FHIP_U64,
                FHIP_ERR_TYPE, "group_by_agg: integer keys only (use group_by for FP keys)");
   FHIP_REQUIRE(naggs >= 0 && naggs <= 16, FHIP_ERR_ARG, "group_by_agg: too many aggregations");
@@ -1974,11 +1981,32 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
   // heavy-key pass records; the hash partitions' cold-row counts are then
   // not made in that pass
   const bool range_on = hp.used && tune_get("groupby.range", 1) != 0;
-  DBuf hot_range;
+  // 64-bit keys whose heavy keys lie within the range path's reach: the
+  // heavy-key pass keeps the cold keys' low 32 bits only (the range path
+  // needs no more: its keys span less than 2^32 values, so a key is its
+  // range's smallest key plus the low bits' difference mod 2^32); if the
+  // range path does not apply after all, the group-by runs again with whole
+  // cold keys
+  bool cold32 = false;
+  if (range_on && narrow_cold && kb == 8 && tune_get("groupby.cold32", 1) != 0) {
+    const unsigned long long flip =
+        (ktype == FHIP_I32 || ktype == FHIP_I64) ? (1ull << 63) : 0ull;
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (unsigned long long kbits : hp.table) {
+      if (kbits == HA_EMPTY) continue;
+      lo = std::min(lo, kbits ^ flip);
+      hi = std::max(hi, kbits ^ flip);
+    }
+    cold32 = lo <= hi && hi - lo < RP_MAXR;
+  }
   const void* ckeys = keys;  // the rows the partitioned path sees
   size_t cn = n;
   AccSet caccs = accs;
-  DBuf hot_tab, cold_k, cold_c[HA_MAX_ACC], hot_part, hot_cnt, hot_fin, hot_seg, hot_ccnt;
+  // hot_cnt: {cold rows, heavy groups}, then every region's cold rows (seg),
+  // then with the range path every region's cold-key range: one readback
+  DBuf hot_tab, cold_k, cold_c[HA_MAX_ACC], hot_part, hot_cnt, hot_fin, hot_ccnt;
+  unsigned long long* hot_seg = nullptr;
+  unsigned long long* hot_range = nullptr;
   int hot_cpbits = -1;  // partition bits of the cold-row counts made by the heavy-key pass
   // with the heavy-key pass, the cold rows of hot workgroup b sit at rows
   // b * HOT_CHUNK .. b * HOT_CHUNK + seg[b] - 1 of the cold buffers, and the
@@ -1997,11 +2025,18 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       const int mx = HOT_MAX_LDS;
       auto set_all = [&](auto kz) {
         using KK = decltype(kz);
-        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 3>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
-        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 0, KK>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 1, KK>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 2, KK>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 3, KK>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 4, KK>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        if constexpr (sizeof(KK) == 8) {
+          FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 0, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+          FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 1, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+          FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 2, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+          FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 3, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+          FHIP_CHECK(hipFuncSetAttribute((const void*)hot_agg_kernel<KK, 4, uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, mx));
+        }
       };
       set_all(int32_t());
       set_all(uint32_t());
@@ -2017,7 +2052,7 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     void* tab_h = pinned_scratch(3, size_t(Sh) * 8);
     std::memcpy(tab_h, hp.table.data(), size_t(Sh) * 8);
     FHIP_CHECK(hipMemcpyAsync(hot_tab.get(), tab_h, size_t(Sh) * 8, hipMemcpyHostToDevice, st));
-    cold_k.reset(n * size_t(kb));
+    cold_k.reset(n * size_t(cold32 ? 4 : kb));
     Cols cold{};
     cold.n = accs.n;
     for (int a = 0; a < accs.n; a++) {
@@ -2028,13 +2063,13 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     }
     const size_t nparts = ceil_div(n, HOT_CHUNK);
     hot_part.reset(nparts * size_t(1 + accs.n) * Sh * 8);
-    hot_cnt.reset(16);
+    hot_cnt.reset(16 + nparts * 8 + (range_on ? nparts * 16 : 0));
     FHIP_CHECK(hipMemsetAsync(hot_cnt.get(), 0, 16, st));
-    hot_seg.reset(nparts * 8);
+    hot_seg = hot_cnt.as<unsigned long long>() + 2;
+    if (range_on) hot_range = hot_seg + nparts;
     unsigned long long* cold_n = hot_cnt.as<unsigned long long>();
     unsigned* hot_n = reinterpret_cast<unsigned*>(cold_n + 1);
     size_t lds = size_t(2 + accs.n) * Sh * 8;
-    if (range_on) hot_range.reset(nparts * 16);
     if (table_hint && !range_on) {
       const int cp = pbits_for(table_hint);
       if (lds + (size_t(4) << cp) <= size_t(HOT_MAX_LDS)) {
@@ -2045,30 +2080,46 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
     }
     {
       ProfScope prof("groupby_hot", st);
-      auto launch = [&](auto kz) {
+      auto launch = [&](auto kz, auto cz) {
         using KK = decltype(kz);
+        using CK = decltype(cz);
         const KK* kp = static_cast<const KK*>(keys);
-        KK* ck = cold_k.as<KK>();
+        CK* ck = cold_k.as<CK>();
         unsigned long long* ht = hot_tab.as<unsigned long long>();
         unsigned long long* hpart = hot_part.as<unsigned long long>();
-        unsigned long long* sg = hot_seg.as<unsigned long long>();
+        unsigned long long* sg = hot_seg;
         unsigned long long* cc = hot_cpbits >= 0 ? hot_ccnt.as<unsigned long long>() : nullptr;
-        unsigned long long* kr = range_on ? hot_range.as<unsigned long long>() : nullptr;
+        unsigned long long* kr = hot_range;
         switch (accs.n) {
-          case 0: hot_agg_kernel<KK, 0><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
-          case 1: hot_agg_kernel<KK, 1><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
-          case 2: hot_agg_kernel<KK, 2><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
-          case 3: hot_agg_kernel<KK, 3><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
-          default: hot_agg_kernel<KK, 4><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          case 0: hot_agg_kernel<KK, 0, CK><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          case 1: hot_agg_kernel<KK, 1, CK><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          case 2: hot_agg_kernel<KK, 2, CK><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          case 3: hot_agg_kernel<KK, 3, CK><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
+          default: hot_agg_kernel<KK, 4, CK><<<unsigned(nparts), HOT_NT, lds, st>>>(kp, n, accs, ht, Sh, hpart, ck, cold, cold_n, sg, hot_cpbits, cc, kr); break;
         }
       };
       switch (ktype) {
-        case FHIP_I32: launch(int32_t()); break;
-        case FHIP_U32: launch(uint32_t()); break;
-        case FHIP_I64: launch(int64_t()); break;
-        default: launch(uint64_t()); break;
+        case FHIP_I32: launch(int32_t(), int32_t()); break;
+        case FHIP_U32: launch(uint32_t(), uint32_t()); break;
+        case FHIP_I64: if (cold32) launch(int64_t(), uint32_t()); else launch(int64_t(), int64_t()); break;
+        default: if (cold32) launch(uint64_t(), uint32_t()); else launch(uint64_t(), uint64_t()); break;
       }
       FHIP_CHECK(hipGetLastError());
+    }
+    // with the range path the host decides from the cold rows' count, every
+    // region's cold rows and cold-key range, which are final after the
+    // heavy-key pass: one copy, and the host waits for it (an event) while
+    // the heavy tables merge.  rb[2], rb[3]: the counts; rb + 4: the regions'
+    // rows; rb + 4 + nparts: their ranges
+    unsigned long long* rb = nullptr;
+    hipEvent_t meta_ready = nullptr;
+    if (hot_cpbits < 0) {
+      rb = static_cast<unsigned long long*>(pinned_scratch(0, 32 + (range_on ? nparts * 24 : 0)));
+      if (range_on) {
+        FHIP_CHECK(hipMemcpyAsync(rb + 2, hot_cnt.get(), 16 + nparts * 24, hipMemcpyDeviceToHost, st));
+        meta_ready = thread_event();
+        FHIP_CHECK(hipEventRecord(meta_ready, st));
+      }
     }
     hot_fin.reset(size_t(S + 1) * stride * 8);
     // two levels: groups of 16 tables, then the groups' tables
@@ -2089,19 +2140,16 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
       // with the partition sizes, so the host does not wait here
       cn_pending = true;
     } else {
-      // the counts, and for the range path the cold keys' range and every
-      // region's cold rows
-      unsigned long long* rb = static_cast<unsigned long long*>(
-          pinned_scratch(0, 32 + (range_on ? nparts * 24 : 0)));
-      FHIP_CHECK(hipMemcpyAsync(rb, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
-      if (range_on) {
-        FHIP_CHECK(hipMemcpyAsync(rb + 4, hot_seg.get(), nparts * 8, hipMemcpyDeviceToHost, st));
-        FHIP_CHECK(hipMemcpyAsync(rb + 4 + nparts, hot_range.get(), nparts * 16,
-                                  hipMemcpyDeviceToHost, st));
-      }
-      FHIP_CHECK(hipStreamSynchronize(st));
-      hc[0] = rb[0];
-      hc[1] = rb[1];
+      // the heavy groups' count is final once the tables have merged: the
+      // range path reads it on the device, the other paths wait for it
+      auto read_counts = [&] {
+        FHIP_CHECK(hipMemcpyAsync(rb + 2, hot_cnt.get(), 16, hipMemcpyDeviceToHost, st));
+        FHIP_CHECK(hipStreamSynchronize(st));
+        hot_groups = unsigned(rb[3] & 0xFFFFFFFFull);
+      };
+      if (range_on) FHIP_CHECK(hipEventSynchronize(meta_ready));
+      else read_counts();
+      hc[0] = rb[2];
       if (range_on) {  // the cold keys' range over the workgroups' ranges
         rb[2] = ~0ull;
         rb[3] = 0ull;
@@ -2111,7 +2159,6 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
         }
       }
       cn = size_t(hc[0]);
-      hot_groups = unsigned(hc[1] & 0xFFFFFFFFull);
       if (range_on && cn > 0 && rb[2] <= rb[3]) {
         // the cold keys' range (from the heavy-key pass) and the heavy keys'
         const unsigned long long flip0 =
@@ -2131,16 +2178,25 @@ void group_by_agg(int ktype, const void* keys, size_t n, const fhip_agg* aggs, i
             size_t(R) <= (size_t(RP_MAXP) << pkb)) {
           const unsigned long long flip =
               (ktype == FHIP_I32 || ktype == FHIP_I64) ? (1ull << 63) : 0ull;
-          group_by_agg_range(ktype, n, accs, AccSet{}, aggs, naggs, slot_of, cold_k.get(),
-                             rb + 4, nparts, hot_fin.as<unsigned long long>(),
-                             hot_groups, rb[2] ^ flip, R, keys_out, capacity, ngroups, st, cold_c);
+          group_by_agg_range(ktype, cold32 ? FHIP_U32 : ktype, n, accs, aggs, naggs, slot_of, cold_k.get(),
+                             rb + 4, nparts, hot_fin.as<unsigned long long>(), unsigned(Sh),
+                             hot_n, rb[2] ^ flip, R, keys_out, capacity, ngroups, st, cold_c);
           return;
         }
+      }
+      if (range_on) read_counts();
+      hc[1] = hot_groups;
+      if (cold32 && cn > 0) {
+        // the cold keys were kept as their low 32 bits for a range path that
+        // does not apply: again, with whole cold keys
+        group_by_agg(ktype, keys, n, aggs, naggs, keys_out, capacity, ngroups, table_hint, st,
+                     heavy, false);
+        return;
       }
     }
     ckeys = cold_k.get();
     for (int a = 0; a < accs.n; a++) caccs.a[a].col = cold_c[a].get();
-    seg_n = hot_seg.as<unsigned long long>();
+    seg_n = hot_seg;
     seg_spans = nparts;
   }
 

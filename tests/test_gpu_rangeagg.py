@@ -185,3 +185,46 @@ def test_range_path_chunks_and_capacity(fa, tune):
     gk, (c,) = fa.group_by_agg(fa.DeviceArray.from_numpy(k), [("count", None)], capacity=4)
     assert np.array_equal(gk.numpy(), keys)
     assert np.array_equal(c.numpy(), oracle.grouped_agg("count", k, gi, split))
+
+
+@pytest.mark.parametrize("kt,base", [(np.uint64, (1 << 33) - 5000), (np.int64, -(1 << 32) - 3000),
+                                     (np.int64, -20000)])
+def test_range_path_low_bits_across_2_32(fa, kt, base):
+    # 64-bit keys whose range crosses a multiple of 2^32 (or zero): the
+    # heavy-key pass keeps the cold keys' low 32 bits, and the range path
+    # rebuilds every key from the range's smallest key
+    from frovedis_amd import datagen as dg
+    n = (1 << 21) + 5
+    z = dg.zipf_keys_np(n, n_values=1 << 16).astype(np.int64)
+    k = (z + base).astype(kt) if kt == np.int64 else (z.astype(np.uint64) + np.uint64(base))
+    v = dg.normal_np(n)
+    fa.check(fa.lib().fhip_prof_enable(1))
+    gk, outs = fa.group_by_agg(fa.DeviceArray.from_numpy(k),
+                               [("count", None), ("sum", fa.DeviceArray.from_numpy(v))])
+    hot, ran = _prof(fa, "groupby_hot"), _prof(fa, "groupby_range_part")
+    fa.check(fa.lib().fhip_prof_enable(0))
+    assert (hot, ran) == (1, 1)
+    _check(k, gk.numpy(), [o.numpy() for o in outs], [("count", None), ("sum", "v")], {"v": v})
+
+
+@pytest.mark.parametrize("kt", [np.int64, np.uint64])
+def test_narrow_heavy_keys_with_far_cold_keys(fa, kt):
+    # heavy keys within a few thousand values, a few cold keys 2^40 away:
+    # the heavy-key pass keeps the cold keys' low 32 bits for the range path,
+    # finds the range too wide, and the group-by runs again with whole keys
+    # on the hash path (two heavy-key passes, no range path)
+    from frovedis_amd import datagen as dg
+    n = 1 << 21
+    k = dg.zipf_keys_np(n, n_values=1 << 12).astype(np.int64) + 100
+    rng = np.random.default_rng(12)
+    far = rng.choice(n, 300, replace=False)
+    k[far] = (1 << 40) + rng.integers(0, 1 << 20, 300)
+    k = k.astype(kt)
+    iv = rng.integers(-50, 50, n).astype(np.int32)
+    fa.check(fa.lib().fhip_prof_enable(1))
+    gk, outs = fa.group_by_agg(fa.DeviceArray.from_numpy(k),
+                               [("count", None), ("max", fa.DeviceArray.from_numpy(iv))])
+    hot, ran = _prof(fa, "groupby_hot"), _prof(fa, "groupby_range_part")
+    fa.check(fa.lib().fhip_prof_enable(0))
+    assert (hot, ran) == (2, 0)
+    _check(k, gk.numpy(), [o.numpy() for o in outs], [("count", None), ("max", "i")], {"i": iv})
