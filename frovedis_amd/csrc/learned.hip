@@ -82,6 +82,35 @@ __device__ __forceinline__ int coarse_of(const uint64_t* spl, uint64_t u) {
 
 constexpr int L1C_ITEMS = 8;  // keys per thread per step of the count kernel
 
+// ---- cell table: the coarse bucket without the full binary search ---------
+// An ordered key u above `base` maps to the cell code of d = u - base (keys
+// at or below base to cell 0): d itself below 2^(CT_M+1), else its bit
+// length and the CT_M bits below its leading one, like a float with CT_M
+// mantissa bits -- non-decreasing in u.  tab[cell] holds the coarse buckets
+// of the cell's smallest and largest key (first | count << 16), so a key's
+// bucket is found among those few splitters: the same index as coarse_of.
+constexpr int CT_M = 6;
+constexpr int CT_CELLS = (65 - CT_M) << CT_M;  // 3776
+
+__host__ __device__ __forceinline__ unsigned cell_code(uint64_t d) {
+  if (d < (uint64_t(1) << (CT_M + 1))) return unsigned(d);
+  const int s = 63 - __builtin_clzll(d) - CT_M;  // >= 1
+  return unsigned((uint64_t(s) << CT_M) + (d >> s));
+}
+
+__device__ __forceinline__ int coarse_by_cell(const uint64_t* spl, const unsigned* tab,
+                                              uint64_t base, uint64_t u) {
+  const unsigned e = tab[cell_code(u > base ? u - base : 0ull)];
+  int lo = int(e & 0xFFFFu), hi = lo + int(e >> 16);
+  // spl[lo] <= u; the last splitter <= u within [lo, hi]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (spl[mid] <= u) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // bucket_of over m splitters (sp[0] <= every key) for N keys at once: the
 // steps of the N searches interleave, so their LDS reads overlap
 template <int N>
@@ -103,13 +132,17 @@ __device__ __forceinline__ void leaf_search(const uint64_t* sp, unsigned m, cons
 __global__ __launch_bounds__(LS_NT) void l1_count_kernel(const uint64_t* __restrict__ keys,
                                                          size_t n, int is_signed,
                                                          const uint64_t* __restrict__ coarse,
+                                                         const unsigned* __restrict__ ctab,
+                                                         uint64_t cbase,
                                                          unsigned long long* __restrict__ hist) {
   __shared__ uint64_t spl[LS_B1];
   __shared__ unsigned h[LS_B1];
+  __shared__ unsigned tab[CT_CELLS];
   for (int i = threadIdx.x; i < LS_B1; i += LS_NT) {
     spl[i] = coarse[i];
     h[i] = 0;
   }
+  for (int i = threadIdx.x; i < CT_CELLS; i += LS_NT) tab[i] = ctab[i];
   __syncthreads();
   const size_t step = size_t(gridDim.x) * LS_NT * L1C_ITEMS;
   for (size_t base = size_t(blockIdx.x) * LS_NT * L1C_ITEMS; base < n; base += step) {
@@ -121,13 +154,7 @@ __global__ __launch_bounds__(LS_NT) void l1_count_kernel(const uint64_t* __restr
     }
     int pos[L1C_ITEMS];
 #pragma unroll
-    for (int j = 0; j < L1C_ITEMS; j++) pos[j] = 0;
-#pragma unroll
-    for (int st = LS_B1 / 2; st >= 1; st >>= 1) {
-#pragma unroll
-      for (int j = 0; j < L1C_ITEMS; j++)
-        if (spl[pos[j] + st] <= u[j]) pos[j] += st;
-    }
+    for (int j = 0; j < L1C_ITEMS; j++) pos[j] = coarse_by_cell(spl, tab, cbase, u[j]);
 #pragma unroll
     for (int j = 0; j < L1C_ITEMS; j++) {
       const size_t i = base + size_t(j) * LS_NT + threadIdx.x;
@@ -153,10 +180,13 @@ static_assert(L1S_NT == LS_B1 && L1S_TILE < 65536, "one coarse bucket per thread
 __global__ __launch_bounds__(L1S_NT) void l1_scatter_kernel(const uint64_t* __restrict__ keys,
                                                             size_t n, int is_signed,
                                                             const uint64_t* __restrict__ coarse,
+                                                            const unsigned* __restrict__ ctab,
+                                                            uint64_t cbase,
                                                             unsigned long long* __restrict__ cursor,
                                                             uint64_t* __restrict__ out) {
   extern __shared__ unsigned long long l1s_lds[];
   __shared__ uint64_t spl[LS_B1];
+  __shared__ unsigned tab[CT_CELLS];
   __shared__ unsigned cnt[LS_B1];             // counts, then tile starts
   __shared__ unsigned long long base[LS_B1];  // global slot of tile position 0 of the bucket
   __shared__ unsigned scan_tmp[L1S_NT / 64];
@@ -165,6 +195,7 @@ __global__ __launch_bounds__(L1S_NT) void l1_scatter_kernel(const uint64_t* __re
   const unsigned tid = threadIdx.x;
   spl[tid] = coarse[tid];
   cnt[tid] = 0;
+  for (int i = tid; i < CT_CELLS; i += L1S_NT) tab[i] = ctab[i];
   __syncthreads();
   const size_t tile = size_t(blockIdx.x) * L1S_TILE;
   const unsigned tn = unsigned(std::min<size_t>(L1S_TILE, n - tile));
@@ -175,14 +206,9 @@ __global__ __launch_bounds__(L1S_NT) void l1_scatter_kernel(const uint64_t* __re
   for (int j = 0; j < L1S_ITEMS; j++) {
     const unsigned i = unsigned(j) * L1S_NT + tid;
     k[j] = keys[tile + (i < tn ? i : 0u)];
-    b[j] = 0;
   }
 #pragma unroll
-  for (int st = LS_B1 / 2; st >= 1; st >>= 1) {
-#pragma unroll
-    for (int j = 0; j < L1S_ITEMS; j++)
-      if (spl[b[j] + st] <= to_ordered(k[j], is_signed)) b[j] += st;
-  }
+  for (int j = 0; j < L1S_ITEMS; j++) b[j] = coarse_by_cell(spl, tab, cbase, to_ordered(k[j], is_signed));
 #pragma unroll
   for (int j = 0; j < L1S_ITEMS; j++) {
     const unsigned i = unsigned(j) * L1S_NT + tid;
@@ -853,6 +879,33 @@ fhip_cdf_model* learned_train(int ktype, const void* sorted, size_t n, size_t sa
 // radix fallback (n if the whole input did).  equal_width: replace the
 // model's coarse splitters by equal-width ranges of the ordered key space
 // (the same pipeline without the network).
+// tab[cell] = first | (last - first) << 16: the coarse buckets (last
+// splitter <= key, as coarse_of) of the cell's smallest and largest key
+std::vector<unsigned> cell_table(const std::vector<uint64_t>& coarse, uint64_t base) {
+  auto bucket = [&](uint64_t u) {
+    return int(std::upper_bound(coarse.begin(), coarse.end(), u) - coarse.begin()) - 1;
+  };
+  std::vector<unsigned> tab(CT_CELLS);
+  for (unsigned c = 0; c < unsigned(CT_CELLS); c++) {
+    uint64_t lo, hi;
+    if (c < (1u << (CT_M + 1))) {
+      lo = hi = c;
+    } else {
+      const int sh = int(c >> CT_M) - 1;
+      const uint64_t top = uint64_t(c) - (uint64_t(sh) << CT_M);
+      lo = top << sh;
+      hi = lo + ((uint64_t(1) << sh) - 1);
+    }
+    // keys of the cell: base + [lo, hi] (saturated), and every key at or
+    // below base for cell 0
+    const uint64_t klo = c == 0 ? 0ull : (lo > ~0ull - base ? ~0ull : base + lo);
+    const uint64_t khi = hi > ~0ull - base ? ~0ull : base + hi;
+    const int b0 = std::max(0, bucket(klo)), b1 = std::max(0, bucket(khi));
+    tab[c] = unsigned(b0) | (unsigned(b1 - b0) << 16);
+  }
+  return tab;
+}
+
 size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mode,
                     size_t* fallback_buckets, size_t* max_coarse, hipStream_t st) {
   if (fallback_buckets) *fallback_buckets = 0;
@@ -879,13 +932,19 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
     snap_coarse(want, m->fine, coarse, kidx);
   }
   const size_t nf = LS_NF;
-  DBuf sb((LS_B1 + nf) * 8 + (LS_B1 + 1) * 4);
+  // the cell table of these coarse splitters (keys at or below the training
+  // minimum share cell 0)
+  const uint64_t cbase = m->umin;
+  std::vector<unsigned> ctab = cell_table(coarse, cbase);
+  DBuf sb((LS_B1 + nf) * 8 + (LS_B1 + 1) * 4 + size_t(CT_CELLS) * 4);
   uint64_t* dco = sb.as<uint64_t>();
   uint64_t* dfi = dco + LS_B1;
   unsigned* dk = reinterpret_cast<unsigned*>(dfi + nf);
+  unsigned* dct = dk + (LS_B1 + 1);
   FHIP_CHECK(hipMemcpyAsync(dco, coarse.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
   FHIP_CHECK(hipMemcpyAsync(dfi, m->fine.data(), nf * 8, hipMemcpyHostToDevice, st));
   FHIP_CHECK(hipMemcpyAsync(dk, kidx.data(), (LS_B1 + 1) * 4, hipMemcpyHostToDevice, st));
+  FHIP_CHECK(hipMemcpyAsync(dct, ctab.data(), size_t(CT_CELLS) * 4, hipMemcpyHostToDevice, st));
   DBuf hb((LS_B1 + 1) * 8 * 3), fob((nf + 1) * 8), t1(n * 8), t2(n * 8), ovb((nf + 1) * 4),
       lcb(nf * 8 * 2);
   unsigned long long* hist = hb.as<unsigned long long>();
@@ -902,7 +961,7 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
   {
     ProfScope prof("learned_count", st);
     const int grid = int(std::min<size_t>(ceil_div(n, LS_NT * 8), size_t(fill_blocks(LS_NT, 2))));
-    l1_count_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, hist);
+    l1_count_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, dct, cbase, hist);
     FHIP_CHECK(hipGetLastError());
   }
   // exclusive offsets of the coarse buckets, and the second-level chunks
@@ -938,7 +997,8 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
       attr = true;
     }
     const unsigned grid = unsigned(ceil_div(n, size_t(L1S_TILE)));
-    l1_scatter_kernel<<<grid, L1S_NT, L1S_LDS, st>>>(k, n, sgn, dco, cursor, t1.as<uint64_t>());
+    l1_scatter_kernel<<<grid, L1S_NT, L1S_LDS, st>>>(k, n, sgn, dco, dct, cbase, cursor,
+                                                     t1.as<uint64_t>());
     FHIP_CHECK(hipGetLastError());
   }
   {
