@@ -129,6 +129,57 @@ __device__ __forceinline__ void leaf_search(const uint64_t* sp, unsigned m, cons
   }
 }
 
+// Leaf search through a cell table of the chunk's coarse bucket: the key
+// range [sp[0], sp[m-1]) of its m leaf splitters (in LDS) is cut into at
+// most L2_CELLS cells of 2^sh keys, and tab[cell] holds the leaves of the
+// cell's first and last key (first | count << 16); a key at or above
+// sp[m-1] is in the last leaf.  Gives leaf_search's answer (the last
+// splitter <= key) with a short search among the cell's splitters.
+constexpr int L2_CELLS = 1024;
+
+struct LeafCells {
+  uint64_t lo, last;  // sp[0], sp[m - 1]
+  int sh, m;
+};
+
+// Fills tab (block-wide; the caller synchronises afterwards).
+template <int NT>
+__device__ __forceinline__ LeafCells leaf_cells_build(const uint64_t* sp, unsigned m,
+                                                      unsigned* tab) {
+  LeafCells lc;
+  lc.lo = sp[0];
+  lc.last = sp[m - 1];
+  lc.m = int(m);
+  const uint64_t R = lc.last - lc.lo;  // keys below sp[m-1]: offsets [0, R)
+  const int bits = R ? 64 - __builtin_clzll(R - 1 ? R - 1 : 1) : 0;
+  lc.sh = bits > 10 ? bits - 10 : 0;
+  const unsigned ncells = R ? unsigned(((R - 1) >> lc.sh) + 1) : 0u;
+  for (unsigned j = threadIdx.x; j < ncells; j += NT) {
+    const uint64_t klo = lc.lo + (uint64_t(j) << lc.sh);
+    const uint64_t span = (uint64_t(1) << lc.sh) - 1;
+    const uint64_t khi = (R - 1 - (uint64_t(j) << lc.sh)) < span ? lc.last - 1 : klo + span;
+    int p[2];
+    const uint64_t q[2] = {klo, khi};
+    leaf_search<2>(sp, m, q, p);
+    tab[j] = unsigned(p[0]) | (unsigned(p[1] - p[0]) << 16);
+  }
+  return lc;
+}
+
+__device__ __forceinline__ int leaf_by_cell(const uint64_t* sp, const unsigned* tab,
+                                            const LeafCells& lc, uint64_t u) {
+  if (u >= lc.last) return lc.m - 1;
+  if (u < lc.lo) return 0;  // below every splitter (leaf_search's answer too)
+  const unsigned e = tab[unsigned((u - lc.lo) >> lc.sh)];
+  int lo = int(e & 0xFFFFu), hi = lo + int(e >> 16);
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sp[mid] <= u) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(LS_NT) void l1_count_kernel(const uint64_t* __restrict__ keys,
                                                          size_t n, int is_signed,
                                                          const uint64_t* __restrict__ coarse,
@@ -256,6 +307,7 @@ __global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restric
                                                          unsigned long long* __restrict__ leaf_cnt) {
   __shared__ uint64_t spl[L2_LDS_FAN];
   __shared__ unsigned cnt[L2_LDS_FAN];
+  __shared__ unsigned tab[L2_CELLS];
   const L2Item it = items[blockIdx.x];
   const unsigned f0 = kidx[it.c], m = kidx[it.c + 1] - f0;
   if (m == 0 || it.s == it.e) return;
@@ -271,7 +323,6 @@ __global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restric
       cnt[j] = 0;
     }
   }
-  __syncthreads();
   const unsigned len = unsigned(it.e - it.s);
   uint64_t u[L2_ITEMS];
   int f[L2_ITEMS];
@@ -280,7 +331,15 @@ __global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restric
     const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
     u[j] = i < len ? to_ordered(in[it.s + i], is_signed) : ~0ull;
   }
-  leaf_search<L2_ITEMS>(sp, m, u, f);
+  __syncthreads();
+  if (in_lds) {
+    const LeafCells lc = leaf_cells_build<L2_NT>(spl, m, tab);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < L2_ITEMS; j++) f[j] = leaf_by_cell(spl, tab, lc, u[j]);
+  } else {
+    leaf_search<L2_ITEMS>(sp, m, u, f);
+  }
 #pragma unroll
   for (int j = 0; j < L2_ITEMS; j++) {
     const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
@@ -304,6 +363,7 @@ __global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restr
   __shared__ uint64_t spl[L2_LDS_FAN];
   __shared__ unsigned cnt[L2_LDS_FAN];
   __shared__ unsigned long long base[L2_LDS_FAN];
+  __shared__ unsigned tab[L2_CELLS];
   const L2Item it = items[blockIdx.x];
   const unsigned f0 = kidx[it.c], m = kidx[it.c + 1] - f0;
   if (m == 0 || it.s == it.e) return;
@@ -323,7 +383,6 @@ __global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restr
       cnt[j] = 0;
     }
   }
-  __syncthreads();
   uint64_t k[L2_ITEMS], u[L2_ITEMS];
   int b[L2_ITEMS];
 #pragma unroll
@@ -332,7 +391,15 @@ __global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restr
     k[j] = i < len ? in[it.s + i] : 0ull;
     u[j] = i < len ? to_ordered(k[j], is_signed) : ~0ull;
   }
-  leaf_search<L2_ITEMS>(sp, m, u, b);
+  __syncthreads();
+  if (in_lds) {
+    const LeafCells lc = leaf_cells_build<L2_NT>(spl, m, tab);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < L2_ITEMS; j++) b[j] = leaf_by_cell(spl, tab, lc, u[j]);
+  } else {
+    leaf_search<L2_ITEMS>(sp, m, u, b);
+  }
 #pragma unroll
   for (int j = 0; j < L2_ITEMS; j++) {
     const unsigned i = unsigned(j) * L2_NT + threadIdx.x;
