@@ -25,6 +25,7 @@
 #include "radix_sort.hpp"
 
 #include <cmath>
+#include <memory>
 
 namespace fhip {
 
@@ -299,7 +300,52 @@ struct L2Item {
   unsigned c, pad;
 };
 
+// One workgroup, one thread per coarse bucket: the buckets' exclusive
+// offsets (off1, and the scatter's cursors), their second-level chunks in
+// bucket order (items, *nitems of them) and the largest bucket, from the
+// first level's counts -- on the device, so the host does not wait.
+__global__ __launch_bounds__(LS_B1) void l2_items_kernel(const unsigned long long* __restrict__ hist,
+                                                         unsigned long long* __restrict__ off1,
+                                                         unsigned long long* __restrict__ cursor,
+                                                         L2Item* __restrict__ items,
+                                                         unsigned* __restrict__ nitems,
+                                                         unsigned long long* __restrict__ maxc) {
+  __shared__ unsigned long long tmp[LS_B1 / 64];
+  __shared__ unsigned tmpc[LS_B1 / 64];
+  __shared__ unsigned long long mx[LS_B1 / 64];
+  const unsigned b = threadIdx.x, lane = __lane_id();
+  const unsigned long long h = hist[b];
+  const unsigned nch = unsigned((h + L2_CHUNK - 1) / L2_CHUNK);
+  unsigned long long all;
+  unsigned call;
+  const unsigned long long o = block_excl_scan<LS_B1>(h, tmp, &all);
+  const unsigned ci = block_excl_scan<LS_B1>(nch, tmpc, &call);
+  off1[b] = o;
+  cursor[b] = o;
+  if (b == LS_B1 - 1) {
+    off1[LS_B1] = all;
+    *nitems = call;
+  }
+  for (unsigned q = 0; q < nch; q++) {
+    const unsigned long long c = o + (unsigned long long)q * L2_CHUNK;
+    items[ci + q] = L2Item{c, c + L2_CHUNK < o + h ? c + L2_CHUNK : o + h, b, 0u};
+  }
+  unsigned long long v = h;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const unsigned long long w = __shfl_xor(v, d, 64);
+    v = w > v ? w : v;
+  }
+  if (lane == 0) mx[b >> 6] = v;
+  __syncthreads();
+  if (b == 0) {
+    for (int w = 1; w < LS_B1 / 64; w++) v = mx[w] > v ? mx[w] : v;
+    *maxc = v;
+  }
+}
+
 __global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restrict__ items,
+                                                         const unsigned* __restrict__ nitems,
                                                          const uint64_t* __restrict__ in,
                                                          const unsigned* __restrict__ kidx,
                                                          int is_signed,
@@ -308,6 +354,7 @@ __global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restric
   __shared__ uint64_t spl[L2_LDS_FAN];
   __shared__ unsigned cnt[L2_LDS_FAN];
   __shared__ unsigned tab[L2_CELLS];
+  if (blockIdx.x >= *nitems) return;  // the grid covers the largest item count
   const L2Item it = items[blockIdx.x];
   const unsigned f0 = kidx[it.c], m = kidx[it.c + 1] - f0;
   if (m == 0 || it.s == it.e) return;
@@ -354,6 +401,7 @@ __global__ __launch_bounds__(L2_NT) void l2_count_kernel(const L2Item* __restric
 }
 
 __global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restrict__ items,
+                                                           const unsigned* __restrict__ nitems,
                                                            const uint64_t* __restrict__ in,
                                                            const unsigned* __restrict__ kidx,
                                                            int is_signed,
@@ -364,6 +412,7 @@ __global__ __launch_bounds__(L2_NT) void l2_scatter_kernel(const L2Item* __restr
   __shared__ unsigned cnt[L2_LDS_FAN];
   __shared__ unsigned long long base[L2_LDS_FAN];
   __shared__ unsigned tab[L2_CELLS];
+  if (blockIdx.x >= *nitems) return;
   const L2Item it = items[blockIdx.x];
   const unsigned f0 = kidx[it.c], m = kidx[it.c + 1] - f0;
   if (m == 0 || it.s == it.e) return;
@@ -562,95 +611,108 @@ __global__ __launch_bounds__(L3_NT) void l3_kernel(const uint64_t* __restrict__ 
 // The leaves that failed the interpolation check (at most L3_CAP keys):
 // one workgroup per listed leaf, LSD radix sort in LDS of (key - min) with
 // 8-bit digits and wave64 ballot ranking.
+// The workgroups walk the list (*nleaves entries) in strides of the grid;
+// every listed leaf's key range goes to rng[2q], rng[2q + 1], and a leaf of
+// more than L3_CAP keys is left for the caller.
 __global__ __launch_bounds__(LSD_NT) void l3_lsd_kernel(const uint64_t* __restrict__ in,
                                                        const unsigned long long* __restrict__ fine_off,
                                                        const unsigned* __restrict__ leaves,
+                                                       const unsigned* __restrict__ nleaves,
+                                                       unsigned long long* __restrict__ rng,
                                                        int is_signed, uint64_t* __restrict__ out) {
   __shared__ uint64_t buf[2][L3_CAP];
   __shared__ unsigned wcnt[LSD_NT / 64][256];
   __shared__ unsigned tstart[256];
   __shared__ unsigned scan_tmp[LSD_NT / 64];
   __shared__ uint64_t red[2][LSD_NT / 64];
-  const unsigned f = leaves[blockIdx.x];
-  const size_t s = fine_off[f], e = fine_off[f + 1];
-  const unsigned m = unsigned(e - s);  // at most L3_CAP (checked by the caller)
-  if (m == 0) return;
-  const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
-  // load, min/max of the ordered keys
-  uint64_t mn = ~0ull, mx = 0;
-  for (unsigned i = tid; i < m; i += LSD_NT) {
-    uint64_t u = to_ordered(in[s + i], is_signed);
-    buf[0][i] = u;
-    mn = u < mn ? u : mn;
-    mx = u > mx ? u : mx;
-  }
+  const unsigned nl = *nleaves;
+  for (unsigned q = blockIdx.x; q < nl; q += gridDim.x) {
+    const unsigned f = leaves[q];
+    const size_t s = fine_off[f], e = fine_off[f + 1];
+    if (threadIdx.x == 0) {
+      rng[2 * size_t(q)] = s;
+      rng[2 * size_t(q) + 1] = e;
+    }
+    const unsigned m = unsigned(e - s);
+    if (m == 0 || e - s > size_t(L3_CAP)) continue;  // uniform over the workgroup
+    const unsigned tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    // load, min/max of the ordered keys
+    uint64_t mn = ~0ull, mx = 0;
+    for (unsigned i = tid; i < m; i += LSD_NT) {
+      uint64_t u = to_ordered(in[s + i], is_signed);
+      buf[0][i] = u;
+      mn = u < mn ? u : mn;
+      mx = u > mx ? u : mx;
+    }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    uint64_t a = __shfl_xor(mn, o, 64), c = __shfl_xor(mx, o, 64);
-    mn = a < mn ? a : mn;
-    mx = c > mx ? c : mx;
-  }
-  if (lane == 0) {
-    red[0][wave] = mn;
-    red[1][wave] = mx;
-  }
-  __syncthreads();
-  mn = red[0][0];
-  mx = red[1][0];
-  for (int w = 1; w < LSD_NT / 64; w++) {
-    mn = red[0][w] < mn ? red[0][w] : mn;
-    mx = red[1][w] > mx ? red[1][w] : mx;
-  }
-  const uint64_t range = mx - mn;
-  const int bits = range ? 64 - __clzll(range) : 0;
-  const int passes = (bits + 7) / 8;
-  int cur = 0;
-  const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-  for (int p = 0; p < passes; p++) {
-    const int sh = 8 * p;
-    for (unsigned i = tid; i < (LSD_NT / 64) * 256; i += LSD_NT) (&wcnt[0][0])[i] = 0;
+    for (int o = 32; o > 0; o >>= 1) {
+      uint64_t a = __shfl_xor(mn, o, 64), c = __shfl_xor(mx, o, 64);
+      mn = a < mn ? a : mn;
+      mx = c > mx ? c : mx;
+    }
+    if (lane == 0) {
+      red[0][wave] = mn;
+      red[1][wave] = mx;
+    }
     __syncthreads();
-    unsigned rank[LSD_ITEMS], dig[LSD_ITEMS];
+    mn = red[0][0];
+    mx = red[1][0];
+    for (int w = 1; w < LSD_NT / 64; w++) {
+      mn = red[0][w] < mn ? red[0][w] : mn;
+      mx = red[1][w] > mx ? red[1][w] : mx;
+    }
+    const uint64_t range = mx - mn;
+    const int bits = range ? 64 - __clzll(range) : 0;
+    const int passes = (bits + 7) / 8;
+    int cur = 0;
+    const unsigned long long lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int p = 0; p < passes; p++) {
+      const int sh = 8 * p;
+      for (unsigned i = tid; i < (LSD_NT / 64) * 256; i += LSD_NT) (&wcnt[0][0])[i] = 0;
+      __syncthreads();
+      unsigned rank[LSD_ITEMS], dig[LSD_ITEMS];
 #pragma unroll
-    for (int j = 0; j < LSD_ITEMS; j++) {
-      const unsigned idx = wave * (64 * LSD_ITEMS) + unsigned(j) * 64 + lane;
-      const bool valid = idx < m;
-      const unsigned d = valid ? unsigned(((buf[cur][idx] - mn) >> sh) & 0xFF) : 0u;
-      unsigned long long mk = __ballot(valid);
+      for (int j = 0; j < LSD_ITEMS; j++) {
+        const unsigned idx = wave * (64 * LSD_ITEMS) + unsigned(j) * 64 + lane;
+        const bool valid = idx < m;
+        const unsigned d = valid ? unsigned(((buf[cur][idx] - mn) >> sh) & 0xFF) : 0u;
+        unsigned long long mk = __ballot(valid);
 #pragma unroll
-      for (int bb = 0; bb < 8; bb++) {
-        const bool bit = (d >> bb) & 1u;
-        unsigned long long bl = __ballot(bit);
-        mk &= bit ? bl : ~bl;
+        for (int bb = 0; bb < 8; bb++) {
+          const bool bit = (d >> bb) & 1u;
+          unsigned long long bl = __ballot(bit);
+          mk &= bit ? bl : ~bl;
+        }
+        dig[j] = d;
+        rank[j] = 0;
+        if (valid) {
+          const unsigned c = unsigned(__popcll(mk)), below = unsigned(__popcll(mk & lt_mask));
+          const unsigned prior = wcnt[wave][d];
+          rank[j] = prior + below;
+          if (below == c - 1) wcnt[wave][d] = prior + c;
+        }
       }
-      dig[j] = d;
-      rank[j] = 0;
-      if (valid) {
-        const unsigned c = unsigned(__popcll(mk)), below = unsigned(__popcll(mk & lt_mask));
-        const unsigned prior = wcnt[wave][d];
-        rank[j] = prior + below;
-        if (below == c - 1) wcnt[wave][d] = prior + c;
+      __syncthreads();
+      unsigned total = 0;
+#pragma unroll
+      for (int w = 0; w < LSD_NT / 64; w++) {
+        const unsigned c = wcnt[w][tid];
+        wcnt[w][tid] = total;
+        total += c;
       }
-    }
-    __syncthreads();
-    unsigned total = 0;
+      tstart[tid] = block_excl_scan<LSD_NT>(total, scan_tmp, (unsigned*)nullptr);
+      __syncthreads();
 #pragma unroll
-    for (int w = 0; w < LSD_NT / 64; w++) {
-      const unsigned c = wcnt[w][tid];
-      wcnt[w][tid] = total;
-      total += c;
+      for (int j = 0; j < LSD_ITEMS; j++) {
+        const unsigned idx = wave * (64 * LSD_ITEMS) + unsigned(j) * 64 + lane;
+        if (idx < m) buf[cur ^ 1][tstart[dig[j]] + wcnt[wave][dig[j]] + rank[j]] = buf[cur][idx];
+      }
+      __syncthreads();
+      cur ^= 1;
     }
-    tstart[tid] = block_excl_scan<LSD_NT>(total, scan_tmp, (unsigned*)nullptr);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < LSD_ITEMS; j++) {
-      const unsigned idx = wave * (64 * LSD_ITEMS) + unsigned(j) * 64 + lane;
-      if (idx < m) buf[cur ^ 1][tstart[dig[j]] + wcnt[wave][dig[j]] + rank[j]] = buf[cur][idx];
-    }
-    __syncthreads();
-    cur ^= 1;
+    for (unsigned i = tid; i < m; i += LSD_NT) out[s + i] = from_ordered(buf[cur][i], is_signed);
+    __syncthreads();  // buf and red are reused by the next listed leaf
   }
-  for (unsigned i = tid; i < m; i += LSD_NT) out[s + i] = from_ordered(buf[cur][i], is_signed);
 }
 
 __global__ void to_feature_kernel(const uint64_t* __restrict__ sorted, size_t n, size_t step,
@@ -724,6 +786,14 @@ struct fhip_cdf_model {
   // cardinality estimate (n, distinct): both are pure functions of the
   // model, kept so that repeated queries skip the host work
   fhip::HotPlanCache hot_plan;
+  // the learned sort's splitters, leaf indices and cell table on a device,
+  // per coarse-splitter mode
+  struct DevConst {
+    int dev = 0, mode = 0;
+    fhip::DBuf buf;
+  };
+  std::mutex dev_mu;
+  std::vector<std::unique_ptr<DevConst>> dev_consts;
   std::mutex card_mu;
   size_t card_n = 0;
   double card_val = -1.0;
@@ -999,24 +1069,48 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
     snap_coarse(want, m->fine, coarse, kidx);
   }
   const size_t nf = LS_NF;
-  // the cell table of these coarse splitters (keys at or below the training
-  // minimum share cell 0)
+  // the splitters, leaf indices and cell table of this mode on the device
+  // (kept by the model, one copy per device and mode)
   const uint64_t cbase = m->umin;
-  std::vector<unsigned> ctab = cell_table(coarse, cbase);
-  DBuf sb((LS_B1 + nf) * 8 + (LS_B1 + 1) * 4 + size_t(CT_CELLS) * 4);
-  uint64_t* dco = sb.as<uint64_t>();
+  int dev = 0;
+  FHIP_CHECK(hipGetDevice(&dev));
+  const size_t cbytes = (LS_B1 + nf) * 8 + (LS_B1 + 1) * 4 + size_t(CT_CELLS) * 4;
+  uint64_t* dco = nullptr;
+  {
+    auto* mm = const_cast<fhip_cdf_model*>(m);
+    std::lock_guard<std::mutex> lock(mm->dev_mu);
+    for (auto& c : mm->dev_consts)
+      if (c->dev == dev && c->mode == coarse_mode) dco = c->buf.as<uint64_t>();
+    if (!dco) {
+      std::unique_ptr<fhip_cdf_model::DevConst> c(new fhip_cdf_model::DevConst);
+      c->dev = dev;
+      c->mode = coarse_mode;
+      c->buf.reset(cbytes);
+      std::vector<unsigned> ctab = cell_table(coarse, cbase);
+      uint64_t* d = c->buf.as<uint64_t>();
+      unsigned* dkk = reinterpret_cast<unsigned*>(d + LS_B1 + nf);
+      FHIP_CHECK(hipMemcpyAsync(d, coarse.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
+      FHIP_CHECK(hipMemcpyAsync(d + LS_B1, m->fine.data(), nf * 8, hipMemcpyHostToDevice, st));
+      FHIP_CHECK(hipMemcpyAsync(dkk, kidx.data(), (LS_B1 + 1) * 4, hipMemcpyHostToDevice, st));
+      FHIP_CHECK(hipMemcpyAsync(dkk + LS_B1 + 1, ctab.data(), size_t(CT_CELLS) * 4,
+                                hipMemcpyHostToDevice, st));
+      FHIP_CHECK(hipStreamSynchronize(st));  // the host vectors are locals
+      dco = d;
+      mm->dev_consts.push_back(std::move(c));
+    }
+  }
   uint64_t* dfi = dco + LS_B1;
   unsigned* dk = reinterpret_cast<unsigned*>(dfi + nf);
   unsigned* dct = dk + (LS_B1 + 1);
-  FHIP_CHECK(hipMemcpyAsync(dco, coarse.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
-  FHIP_CHECK(hipMemcpyAsync(dfi, m->fine.data(), nf * 8, hipMemcpyHostToDevice, st));
-  FHIP_CHECK(hipMemcpyAsync(dk, kidx.data(), (LS_B1 + 1) * 4, hipMemcpyHostToDevice, st));
-  FHIP_CHECK(hipMemcpyAsync(dct, ctab.data(), size_t(CT_CELLS) * 4, hipMemcpyHostToDevice, st));
-  DBuf hb((LS_B1 + 1) * 8 * 3), fob((nf + 1) * 8), t1(n * 8), t2(n * 8), ovb((nf + 1) * 4),
-      lcb(nf * 8 * 2);
+  // second-level chunks: at most one per L2_CHUNK keys plus one per bucket
+  const size_t max_items = ceil_div(n, size_t(L2_CHUNK)) + LS_B1;
+  DBuf hb((LS_B1 + 1) * 8 * 3 + 16), fob((nf + 1) * 8), t1(n * 8), t2(n * 8), ovb((nf + 1) * 4),
+      lcb(nf * 8 * 2), itb(max_items * sizeof(L2Item)), rngb((nf + 1) * 16);
   unsigned long long* hist = hb.as<unsigned long long>();
   unsigned long long* off1 = hist + (LS_B1 + 1);
   unsigned long long* cursor = off1 + (LS_B1 + 1);
+  unsigned long long* maxc = cursor + (LS_B1 + 1);
+  unsigned* nitems = reinterpret_cast<unsigned*>(maxc + 1);
   unsigned long long* fine_off = fob.as<unsigned long long>();
   unsigned long long* leaf_cnt = lcb.as<unsigned long long>();
   unsigned long long* leaf_cur = leaf_cnt + nf;
@@ -1031,30 +1125,10 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
     l1_count_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, dct, cbase, hist);
     FHIP_CHECK(hipGetLastError());
   }
-  // exclusive offsets of the coarse buckets, and the second-level chunks
-  std::vector<L2Item> items;
-  {
-    std::vector<unsigned long long> h(LS_B1 + 1), o(LS_B1 + 1);
-    FHIP_CHECK(hipMemcpyAsync(h.data(), hist, LS_B1 * 8, hipMemcpyDeviceToHost, st));
-    FHIP_CHECK(hipStreamSynchronize(st));
-    unsigned long long run = 0, mx = 0;
-    for (int b = 0; b < LS_B1; b++) {
-      o[b] = run;
-      for (unsigned long long c = run; c < run + h[b]; c += L2_CHUNK)
-        items.push_back(L2Item{c, std::min<unsigned long long>(c + L2_CHUNK, run + h[b]),
-                               unsigned(b), 0u});
-      run += h[b];
-      mx = std::max(mx, h[b]);
-    }
-    o[LS_B1] = run;
-    if (max_coarse) *max_coarse = size_t(mx);
-    FHIP_CHECK(hipMemcpyAsync(off1, o.data(), (LS_B1 + 1) * 8, hipMemcpyHostToDevice, st));
-    FHIP_CHECK(hipMemcpyAsync(cursor, o.data(), LS_B1 * 8, hipMemcpyHostToDevice, st));
-    FHIP_CHECK(hipStreamSynchronize(st));  // o/h are host locals
-  }
-  DBuf itb(std::max<size_t>(1, items.size()) * sizeof(L2Item));
-  FHIP_CHECK(hipMemcpyAsync(itb.get(), items.data(), items.size() * sizeof(L2Item),
-                            hipMemcpyHostToDevice, st));
+  // exclusive offsets of the coarse buckets and the second-level chunks,
+  // made on the device
+  l2_items_kernel<<<1, LS_B1, 0, st>>>(hist, off1, cursor, itb.as<L2Item>(), nitems, maxc);
+  FHIP_CHECK(hipGetLastError());
   {
     ProfScope prof("learned_scatter", st);
     static bool attr = false;
@@ -1070,14 +1144,14 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
   }
   {
     ProfScope prof("learned_split", st);
-    const unsigned ni = unsigned(items.size());
-    l2_count_kernel<<<ni, L2_NT, 0, st>>>(itb.as<L2Item>(), t1.as<uint64_t>(), dk, sgn, dfi,
-                                          leaf_cnt);
+    const unsigned ni = unsigned(max_items);
+    l2_count_kernel<<<ni, L2_NT, 0, st>>>(itb.as<L2Item>(), nitems, t1.as<uint64_t>(), dk, sgn,
+                                          dfi, leaf_cnt);
     FHIP_CHECK(hipGetLastError());
     prefix_sum(FHIP_U64, leaf_cnt, fine_off + 1, nf, st);  // fine_off[nf] = n
     FHIP_CHECK(hipMemcpyAsync(leaf_cur, fine_off, nf * 8, hipMemcpyDeviceToDevice, st));
-    l2_scatter_kernel<<<ni, L2_NT, 0, st>>>(itb.as<L2Item>(), t1.as<uint64_t>(), dk, sgn, dfi,
-                                            leaf_cur, t2.as<uint64_t>());
+    l2_scatter_kernel<<<ni, L2_NT, 0, st>>>(itb.as<L2Item>(), nitems, t1.as<uint64_t>(), dk, sgn,
+                                            dfi, leaf_cur, t2.as<uint64_t>());
     FHIP_CHECK(hipGetLastError());
   }
   {
@@ -1085,27 +1159,39 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
     l3_kernel<<<unsigned(nf), L3_NT, 0, st>>>(t2.as<uint64_t>(), fine_off, dfi, int(nf), sgn, k,
                                               ovb.as<unsigned>() + 1, ovb.as<unsigned>());
     FHIP_CHECK(hipGetLastError());
+    // the leaves that failed the check and fit in LDS: an LSD radix sort,
+    // listed leaves taken in strides of the grid; every listed leaf's key
+    // range is recorded for the host
+    l3_lsd_kernel<<<256, LSD_NT, 0, st>>>(t2.as<uint64_t>(), fine_off, ovb.as<unsigned>() + 1,
+                                          ovb.as<unsigned>(), rngb.as<unsigned long long>(), sgn, k);
+    FHIP_CHECK(hipGetLastError());
   }
-  unsigned nov = 0;
-  FHIP_CHECK(hipMemcpyAsync(&nov, ovb.get(), 4, hipMemcpyDeviceToHost, st));
+  // one readback: the missed leaves' count, their first kFirst ranges and the
+  // largest coarse bucket
+  constexpr size_t kFirst = 256;
+  unsigned long long* hb_h =
+      static_cast<unsigned long long*>(pinned_scratch(0, (2 + 2 * kFirst) * 8));
+  FHIP_CHECK(hipMemcpyAsync(hb_h, maxc, 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipMemcpyAsync(hb_h + 1, ovb.get(), 4, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipMemcpyAsync(hb_h + 2, rngb.get(), 2 * kFirst * 8, hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
+  if (max_coarse) *max_coarse = size_t(hb_h[0]);
+  const unsigned nov = unsigned(hb_h[1] & 0xFFFFFFFFull);
   if (fallback_buckets) *fallback_buckets = nov;
   if (nov == 0) return 0;
-  // radix fallback for the leaves that did not fit in LDS
-  std::vector<unsigned> ids(nov);
-  std::vector<unsigned long long> fo(nf + 1);
-  FHIP_CHECK(hipMemcpyAsync(ids.data(), ovb.as<unsigned>() + 1, nov * 4, hipMemcpyDeviceToHost, st));
-  FHIP_CHECK(hipMemcpyAsync(fo.data(), fine_off, (nf + 1) * 8, hipMemcpyDeviceToHost, st));
-  FHIP_CHECK(hipStreamSynchronize(st));
-  // leaves that fit in LDS: one batched launch of the LSD leaf sort; larger
-  // ones: the device radix sort, leaf by leaf
-  std::vector<unsigned> small;
+  std::vector<unsigned long long> rg(2 * size_t(nov));
+  std::copy(hb_h + 2, hb_h + 2 + 2 * std::min<size_t>(nov, kFirst), rg.begin());
+  if (nov > kFirst) {
+    FHIP_CHECK(hipMemcpyAsync(rg.data() + 2 * kFirst, rngb.as<unsigned long long>() + 2 * kFirst,
+                              2 * (nov - kFirst) * 8, hipMemcpyDeviceToHost, st));
+    FHIP_CHECK(hipStreamSynchronize(st));
+  }
+  // leaves larger than LDS: the device radix sort, leaf by leaf
   size_t fb_keys = 0, big_keys = 0, nbig = 0;
-  for (unsigned f : ids) {
-    const size_t len = size_t(fo[f + 1] - fo[f]);
+  for (unsigned q = 0; q < nov; q++) {
+    const size_t len = size_t(rg[2 * q + 1] - rg[2 * q]);
     fb_keys += len;
-    if (len <= size_t(L3_CAP)) small.push_back(f);
-    else {
+    if (len > size_t(L3_CAP)) {
       big_keys += len;
       nbig++;
     }
@@ -1115,16 +1201,8 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
     sort_pairs(m->ktype, t2.get(), k, 0, nullptr, nullptr, n, false, false, st);
     return n;
   }
-  if (!small.empty()) {
-    DBuf lb(small.size() * 4);
-    FHIP_CHECK(hipMemcpyAsync(lb.get(), small.data(), small.size() * 4, hipMemcpyHostToDevice, st));
-    l3_lsd_kernel<<<unsigned(small.size()), LSD_NT, 0, st>>>(t2.as<uint64_t>(), fine_off,
-                                                              lb.as<unsigned>(), sgn, k);
-    FHIP_CHECK(hipGetLastError());
-    FHIP_CHECK(hipStreamSynchronize(st));  // `small` is a host local
-  }
-  for (unsigned f : ids) {
-    const size_t a = size_t(fo[f]), z = size_t(fo[f + 1]);
+  for (unsigned q = 0; q < nov; q++) {
+    const size_t a = size_t(rg[2 * q]), z = size_t(rg[2 * q + 1]);
     if (z - a <= size_t(L3_CAP)) continue;
     sort_pairs(m->ktype, t2.as<uint64_t>() + a, k + a, 0, nullptr, nullptr, z - a, false, false, st);
   }
