@@ -244,7 +244,11 @@ def bench_radix(fa, log2n, steps=3):
     v0 = torch.arange(n, dtype=torch.int64, device="cuda")
     keys = fa.DeviceArray(k0.clone(), np.uint64)
     vals = fa.DeviceArray(v0.clone(), np.uint64)
-    fa.radix_sort(keys, vals)
+    # the tile shape the autotuner picks for this sort on this machine
+    # (radix.shape_kv; every shape's output is checked against the default's)
+    from frovedis_amd import autotune as at
+    _, shape, tried = at.tune_radix(keys, vals, reps=2)
+    default_ms = next((r[0] * 1e3 for r in tried if r[1] == at.RADIX_SHAPES.index((1024, 16))), None)
     times = []
     fa.check(fa.lib().fhip_prof_enable(1))
     for _ in range(steps):
@@ -263,6 +267,8 @@ def bench_radix(fa, log2n, steps=3):
     algo = n * (8 + 32 * P)
     kern_s = (pass_ms + hist_ms) / 1e3 / steps
     return {"keys": n, "keys_per_s": n / t, "ms": t * 1e3, "passes": P, "digit_bits": 8,
+            "tile_shape": list(at.RADIX_SHAPES[shape]) if shape is not None else None,
+            "default_shape_ms": default_ms,
             "algorithmic_bytes": algo, "kernel_ms": kern_s * 1e3,
             "hist_ms": hist_ms / steps, "pass_ms": pass_ms / max(passes, 1),
             "hbm_frac": algo / kern_s / HBM_PEAK if kern_s else None}
@@ -299,7 +305,11 @@ def bench_learned(fa, log2n, steps=3):
             best = min(best, time.perf_counter() - t)
         return best, r
 
-    fa.radix_sort(keys)
+    # the plain sort with the tile shape the autotuner picks on these keys
+    # (radix.shape_k; outside the timed runs)
+    from frovedis_amd import autotune as at
+    keys.t.copy_(k0)
+    _, shape_k, _ = at.tune_radix(keys, reps=2)
     plain_s, _ = timed(lambda: fa.radix_sort(keys))
     ref = keys.t.clone()
     fa.check(fa.lib().fhip_prof_enable(1))
@@ -326,7 +336,8 @@ def bench_learned(fa, log2n, steps=3):
             "quantile_max_coarse_bucket": fb_quant[2],
             "network_gain_over_quantile": quant_s / learned_s,
             "plain_keys_per_s": n / plain_s, "learned_keys_per_s": n / learned_s,
-            "train_s": train_s, "model_loss": model.info()["loss"], "stages": stages}
+            "train_s": train_s, "model_loss": model.info()["loss"], "stages": stages,
+            "plain_tile_shape": list(at.RADIX_SHAPES[shape_k]) if shape_k is not None else None}
 
 
 def bench_batch(fa, steps=3):
