@@ -58,10 +58,14 @@ struct RowIdMap {
 };
 
 // number of right matches of every sorted left key, and where they start.
-// One workgroup per block of MC_NT sorted left keys: the block's right range
-// comes from two binary searches over all of RIGHT (its first and last
-// key); each key then searches only that range, which stays in cache.
+// One workgroup per block of MC_NT sorted left keys.  The blocks' right
+// ranges (a lower bound of the block's first key and an upper bound of its
+// last, over all of RIGHT) come first, one thread per block, so those long
+// searches run side by side instead of holding each block; a block whose
+// range has at most MC_CAP keys stages it in LDS and every key searches
+// there, a wider one searches its range in memory.
 constexpr int MC_NT = 256;
+constexpr int MC_CAP = 4096;
 
 template <class K>
 __device__ __forceinline__ size_t lower_in(const K* r, size_t lo, size_t hi, K x) {
@@ -82,29 +86,50 @@ __device__ __forceinline__ size_t upper_in(const K* r, size_t lo, size_t hi, K x
 }
 
 template <class K>
+__global__ __launch_bounds__(256) void match_range_kernel(const K* __restrict__ lk, size_t nl,
+                                                          const K* __restrict__ rk, size_t nr,
+                                                          size_t nblocks,
+                                                          unsigned long long* __restrict__ rng) {
+  const size_t b = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (b >= nblocks) return;
+  const size_t b0 = b * MC_NT;
+  const size_t b1 = b0 + MC_NT < nl ? b0 + MC_NT : nl;
+  rng[2 * b] = lower_in(rk, 0, nr, lk[b0]);
+  rng[2 * b + 1] = upper_in(rk, 0, nr, lk[b1 - 1]);
+}
+
+template <class K>
 __global__ __launch_bounds__(MC_NT) void match_count_kernel(const K* __restrict__ lk, size_t nl,
                                                             const K* __restrict__ rk, size_t nr,
+                                                            const unsigned long long* __restrict__ rng,
                                                             unsigned long long* __restrict__ cnt,
                                                             unsigned long long* __restrict__ first) {
-  __shared__ size_t range[2];
+  __shared__ K sr[MC_CAP];
   const size_t b0 = size_t(blockIdx.x) * MC_NT;
   const size_t b1 = b0 + MC_NT < nl ? b0 + MC_NT : nl;
+  size_t r0 = 0, r1 = nr;
+  // floating keys (NaN, -0.0) are not totally ordered by <: search all
   if constexpr (std::is_integral<K>::value) {
-    if (threadIdx.x == 0) range[0] = lower_in(rk, 0, nr, lk[b0]);
-    if (threadIdx.x == 64) range[1] = upper_in(rk, 0, nr, lk[b1 - 1]);
-  } else {
-    // floating keys (NaN, -0.0) are not totally ordered by <: search all
-    if (threadIdx.x == 0) {
-      range[0] = 0;
-      range[1] = nr;
-    }
+    r0 = size_t(rng[2 * size_t(blockIdx.x)]);
+    r1 = size_t(rng[2 * size_t(blockIdx.x) + 1]);
   }
+  const bool staged = std::is_integral<K>::value && r1 - r0 <= size_t(MC_CAP);
+  if (staged)
+    for (size_t j = threadIdx.x; j < r1 - r0; j += MC_NT) sr[j] = rk[r0 + j];
   __syncthreads();
   const size_t i = b0 + threadIdx.x;
   if (i >= b1) return;
   const K x = lk[i];
-  const size_t a = lower_in(rk, range[0], range[1], x);
-  const size_t e = upper_in(rk, a, range[1], x);
+  size_t a, e;
+  if (staged) {
+    a = lower_in(sr, size_t(0), r1 - r0, x);
+    e = upper_in(sr, a, r1 - r0, x);
+    a += r0;
+    e += r0;
+  } else {
+    a = lower_in(rk, r0, r1, x);
+    e = upper_in(rk, a, r1, x);
+  }
   cnt[i] = e - a;
   first[i] = a;
 }
@@ -228,8 +253,15 @@ size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, con
       incl(nl * sizeof(unsigned long long));
   {
     ProfScope prof("join_merge", st);
-    match_count_kernel<K><<<unsigned(ceil_div(nl, MC_NT)), MC_NT, 0, st>>>(
-        lks, nl, rks, nr, cnt.as<unsigned long long>(),
+    const size_t nb = ceil_div(nl, size_t(MC_NT));
+    DBuf rng(2 * nb * 8);
+    if (std::is_integral<K>::value) {
+      match_range_kernel<K><<<unsigned(ceil_div(nb, size_t(256))), 256, 0, st>>>(
+          lks, nl, rks, nr, nb, rng.as<unsigned long long>());
+      FHIP_CHECK(hipGetLastError());
+    }
+    match_count_kernel<K><<<unsigned(nb), MC_NT, 0, st>>>(
+        lks, nl, rks, nr, rng.as<unsigned long long>(), cnt.as<unsigned long long>(),
         first.as<unsigned long long>());
     FHIP_CHECK(hipGetLastError());
   }
