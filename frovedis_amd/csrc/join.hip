@@ -172,6 +172,47 @@ __global__ __launch_bounds__(PW_NT) void pair_write_kernel(
   }
 }
 
+// The pairs of the narrow sort-merge path: as pair_write_kernel, with the
+// sorted sides given as rows (0..n-1) of consecutive row ids ids[0] + row.
+__global__ __launch_bounds__(PW_NT) void pair_write_rows_kernel(
+    const unsigned long long* __restrict__ incl, const unsigned long long* __restrict__ cnt,
+    const unsigned long long* __restrict__ first, const uint32_t* __restrict__ lrow,
+    const uint32_t* __restrict__ rrow, const size_t* __restrict__ lidx,
+    const size_t* __restrict__ ridx, size_t nl, size_t* __restrict__ lo, size_t* __restrict__ ro) {
+  __shared__ unsigned long long s_incl[PW_NT];
+  __shared__ unsigned long long s_first[PW_NT];
+  __shared__ size_t s_l[PW_NT];
+  const size_t b0 = size_t(blockIdx.x) * PW_NT;
+  const unsigned m = unsigned(b0 + PW_NT < nl ? PW_NT : nl - b0);
+  const unsigned t = threadIdx.x;
+  const size_t l0 = lidx[0], r0 = ridx[0];
+  if (t < m) {
+    s_incl[t] = incl[b0 + t];
+    s_first[t] = first[b0 + t];
+    s_l[t] = l0 + lrow[b0 + t];
+  }
+  __syncthreads();
+  const unsigned long long out0 = s_incl[0] - cnt[b0];
+  const unsigned long long out1 = s_incl[m - 1];
+  for (unsigned long long p = out0 + t; p < out1; p += PW_NT) {
+    unsigned a = 0, z = m;
+    while (a < z) {
+      const unsigned mid = (a + z) >> 1;
+      if (s_incl[mid] > p) z = mid; else a = mid + 1;
+    }
+    const unsigned long long row_start = a ? s_incl[a - 1] : out0;
+    lo[p] = s_l[a];
+    ro[p] = r0 + rrow[s_first[a] + (p - row_start)];
+  }
+}
+
+// i -> ids[0] + rows[i] (a sorted side of the narrow path as row ids)
+struct RowOffMap {
+  const size_t* ids;
+  const uint32_t* rows;
+  __device__ size_t operator()(size_t i) const { return ids[0] + rows[i]; }
+};
+
 // an evenly spaced sample of the right keys
 template <class K>
 __global__ void sample_kernel(const K* __restrict__ keys, size_t step, size_t m,
@@ -215,11 +256,86 @@ bool sample_has_duplicate(int ktype, const K* right, size_t nr, hipStream_t st) 
 // their count.
 // inplace: the inputs themselves are sorted (with their indices), as the
 // VE body leaves them (join.hpp:320-322); else sorted copies are used.
+// The sort-merge join when both sides' row ids are consecutive and each
+// side's keys share their bits above the low 32: both sides sorted as (low
+// 32 key bits, 32-bit row) pairs (sort_narrow: half the bytes per pass of
+// the key/row-id sort), merged on the 32-bit keys, and the pairs written as
+// ids[0] + row.  Same pairs in the same (key, left, right) order as the
+// general path.  Returns false, having written nothing the caller keeps,
+// when a side does not qualify.
+template <class K>
+bool sort_merge_narrow(int ktype, const K* left, const size_t* lidx, size_t nl, const K* right,
+                       const size_t* ridx, size_t nr, size_t* lo, size_t* ro, size_t capacity,
+                       size_t* only_left, size_t* n_only, hipStream_t st, size_t* total_out) {
+  DBuf lk(nl * 4), lr(nl * 4), rk(nr * 4), rr(nr * 4), bad(sizeof(unsigned));
+  FHIP_CHECK(hipMemsetAsync(bad.get(), 0, sizeof(unsigned), st));
+  unsigned hl = 0, hr = 0;
+  // radix_sort(right, right_idx); radix_sort(left, left_idx) (join.hpp:320-322)
+  if (!sort_narrow(ktype, right, ridx, nr, rk.as<uint32_t>(), rr.as<uint32_t>(), &hr,
+                   bad.as<unsigned>(), st))
+    return false;
+  if (!sort_narrow(ktype, left, lidx, nl, lk.as<uint32_t>(), lr.as<uint32_t>(), &hl,
+                   bad.as<unsigned>(), st))
+    return false;
+  DBuf cnt(nl * sizeof(unsigned long long)), first(nl * sizeof(unsigned long long)),
+      incl(nl * sizeof(unsigned long long));
+  unsigned long long* rb = static_cast<unsigned long long*>(pinned_scratch(0, 16));
+  rb[0] = 0;
+  if (hl == hr) {
+    {
+      ProfScope prof("join_merge", st);
+      const size_t nb = ceil_div(nl, size_t(MC_NT));
+      DBuf rng(2 * nb * 8);
+      match_range_kernel<uint32_t><<<unsigned(ceil_div(nb, size_t(256))), 256, 0, st>>>(
+          lk.as<uint32_t>(), nl, rk.as<uint32_t>(), nr, nb, rng.as<unsigned long long>());
+      FHIP_CHECK(hipGetLastError());
+      match_count_kernel<uint32_t><<<unsigned(nb), MC_NT, 0, st>>>(
+          lk.as<uint32_t>(), nl, rk.as<uint32_t>(), nr, rng.as<unsigned long long>(),
+          cnt.as<unsigned long long>(), first.as<unsigned long long>());
+      FHIP_CHECK(hipGetLastError());
+    }
+    prefix_sum(FHIP_U64, cnt.get(), incl.get(), nl, st);
+    FHIP_CHECK(hipMemcpyAsync(rb, incl.as<unsigned long long>() + (nl - 1), 8,
+                              hipMemcpyDeviceToHost, st));
+  } else {
+    // the sides differ above the low 32 bits: no key matches
+    FHIP_CHECK(hipMemsetAsync(cnt.get(), 0, nl * sizeof(unsigned long long), st));
+  }
+  FHIP_CHECK(hipMemcpyAsync(rb + 1, bad.get(), sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  if ((rb[1] & 0xFFFFFFFFull) != 0) return false;  // row ids not consecutive
+  const unsigned long long total = rb[0];
+  if (total && size_t(total) <= capacity) {
+    ProfScope prof("join_write", st);
+    pair_write_rows_kernel<<<unsigned(ceil_div(nl, PW_NT)), PW_NT, 0, st>>>(
+        incl.as<unsigned long long>(), cnt.as<unsigned long long>(),
+        first.as<unsigned long long>(), lr.as<uint32_t>(), rr.as<uint32_t>(), lidx, ridx, nl,
+        lo, ro);
+    FHIP_CHECK(hipGetLastError());
+  }
+  if (only_left)
+    *n_only = compact(NoMatchPred{cnt.as<unsigned long long>()},
+                      RowOffMap{lidx, lr.as<uint32_t>()}, nl, only_left, st);
+  *total_out = size_t(total);
+  return true;
+}
+
 template <class K>
 size_t sort_merge_t(int ktype, const K* left, const size_t* lidx, size_t nl, const K* right,
                     const size_t* ridx, size_t nr, size_t* lo, size_t* ro, size_t capacity,
                     size_t* only_left, size_t* n_only, hipStream_t st, bool inplace = false) {
   if (n_only) *n_only = 0;
+  if constexpr (std::is_integral<K>::value) {
+    // the narrow path (both row-id columns consecutive, each side's keys
+    // within one 2^32 stretch); the general one below otherwise
+    size_t total = 0;
+    if (!inplace && nl > 0 && nr > 0 && nl < (size_t(1) << 32) && nr < (size_t(1) << 32) &&
+        tune_get("join.narrow", 1) != 0 &&
+        sort_merge_narrow<K>(ktype, left, lidx, nl, right, ridx, nr, lo, ro, capacity, only_left,
+                             n_only, st, &total))
+      return total;
+    if (n_only) *n_only = 0;
+  }
   // radix_sort(right, right_idx); radix_sort(left, left_idx) (join.hpp:320-322)
   if (inplace) {
     if (nr) sort_pairs(ktype, right, const_cast<K*>(right), 8, ridx, const_cast<size_t*>(ridx),

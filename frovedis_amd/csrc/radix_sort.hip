@@ -202,13 +202,19 @@ struct PassLds {
   alignas(16) unsigned char buf[NT * ITEMS * EB];
 };
 
-template <class U, int VB, int NT, int ITEMS>
+// UO, IOTA: the first pass of a narrow sort (sort_narrow) -- the payload is
+// the key's row (32 bits), the caller's row ids (vsrc_, size_t) are only
+// checked to be consecutive (*iota_bad set otherwise), and the keys are
+// written as UO in the sort's order (flipped), so the later passes are
+// plain 32-bit ones.
+template <class U, int VB, int NT, int ITEMS, class UO = U, bool IOTA = false>
 __global__ __launch_bounds__(NT) void rs_onesweep(
-    const U* __restrict__ ksrc, U* __restrict__ kdst, const void* __restrict__ vsrc_,
+    const U* __restrict__ ksrc, UO* __restrict__ kdst, const void* __restrict__ vsrc_,
     void* __restrict__ vdst_, size_t n, int shift, int mode, int desc,
     const unsigned long long* __restrict__ gbase, unsigned long long* lookback,
     unsigned long long* lookback_next, unsigned* tile_counter,
-    unsigned* tile_counter_next) {
+    unsigned* tile_counter_next, unsigned* __restrict__ iota_bad = nullptr) {
+  static_assert(!IOTA || VB == 4, "the narrow sort's payload is a 32-bit row");
   using V = typename UInt<(VB ? VB : 4)>::T;
   constexpr int WAVES = NT / 64;
   constexpr int TILE = NT * ITEMS;
@@ -325,7 +331,18 @@ __global__ __launch_bounds__(NT) void rs_onesweep(
   }
   __syncthreads();
 
-  if constexpr (VB != 0) {
+  if constexpr (IOTA) {
+    const size_t* ids = static_cast<const size_t*>(vsrc_);
+    const size_t id0 = ids[0];
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      size_t idx = wave_base + size_t(j) * 64 + lane;
+      vals[j] = V(idx);
+      if (idx < n && ids[idx] != id0 + idx) bad = true;
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(iota_bad, 1u);
+  } else if constexpr (VB != 0) {
     const V* vsrc = static_cast<const V*>(vsrc_);
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
@@ -355,7 +372,8 @@ __global__ __launch_bounds__(NT) void rs_onesweep(
     if (i < tile_n) {
       U k = kbuf[i];
       d = key_digit(k, mode, shift, desc);
-      kdst[s.gdst[d] + i] = k;
+      if constexpr (IOTA) kdst[s.gdst[d] + i] = UO(key_xform(k, mode));
+      else kdst[s.gdst[d] + i] = k;
     }
     if (j % 4 == 0) dig4[j / 4] = d;
     else dig4[j / 4] |= d << (8 * (j % 4));
@@ -669,6 +687,40 @@ AsyncLayout async_layout(size_t n, int kbytes, int vbytes) {
   return L;
 }
 
+
+// ---- the narrow sort (sort_narrow) -----------------------------------------
+// All keys equal: the low 32 bits in the sort's order and each row, with the
+// row ids' check.
+template <class U>
+__global__ void narrow_one_kernel(const U* __restrict__ keys, const size_t* __restrict__ ids,
+                                  size_t n, int mode, uint32_t* __restrict__ k32,
+                                  uint32_t* __restrict__ row, unsigned* __restrict__ iota_bad) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  const size_t id0 = ids[0];
+  bool bad = false;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    k32[i] = uint32_t(key_xform(keys[i], mode));
+    row[i] = uint32_t(i);
+    if (ids[i] != id0 + i) bad = true;
+  }
+  if (__ballot(bad) && __lane_id() == 0) atomicOr(iota_bad, 1u);
+}
+
+using NarrowFirstFn = void (*)(const void*, void*, const void*, void*, size_t, int, int,
+                               const unsigned long long*, unsigned long long*, unsigned long long*,
+                               unsigned*, unsigned*, unsigned*, unsigned, hipStream_t);
+
+template <class U, int NT, int ITEMS>
+void launch_narrow_first(const void* ks, void* kd, const void* ids, void* rd, size_t n, int shift,
+                         int mode, const unsigned long long* gbase, unsigned long long* lb,
+                         unsigned long long* lb_next, unsigned* ctr, unsigned* ctr_next,
+                         unsigned* iota_bad, unsigned tiles, hipStream_t st) {
+  rs_onesweep<U, 4, NT, ITEMS, uint32_t, true><<<tiles, NT, 0, st>>>(
+      (const U*)ks, (uint32_t*)kd, ids, rd, n, shift, mode, 0, gbase, lb, lb_next, ctr, ctr_next,
+      iota_bad);
+  FHIP_CHECK(hipGetLastError());
+}
+
 }  // namespace
 
 size_t sort_async_workspace_bytes(size_t n, int kbytes, int vbytes) {
@@ -727,6 +779,100 @@ size_t sort_scratch_bytes(size_t n, int kbytes, int vbytes) {
   auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
   return up(16) + up(size_t(2) * 8 * 256 * 8) + up(n * size_t(kbytes)) + up(n * size_t(vbytes)) +
          up(2 * tiles * 256 * 8 + 256);
+}
+
+// Narrow sort for the sort-merge join (see radix_sort.hpp).
+bool sort_narrow(int ktype, const void* keys, const size_t* ids, size_t n, uint32_t* k32,
+                 uint32_t* row, unsigned* hi, unsigned* iota_bad, hipStream_t st) {
+  FHIP_REQUIRE(ktype == FHIP_I32 || ktype == FHIP_U32 || ktype == FHIP_I64 || ktype == FHIP_U64,
+               FHIP_ERR_TYPE, "sort_narrow: integer keys only");
+  FHIP_REQUIRE(n < (size_t(1) << 32), FHIP_ERR_TOO_LARGE, "sort_narrow: too many keys");
+  *hi = 0;
+  if (n == 0) return true;
+  const int kbytes = type_bytes(ktype);
+  const int mode = (ktype == FHIP_I32 || ktype == FHIP_I64) ? KM_FLIPTOP : KM_RAW;
+  DBuf hist(size_t(2) * 8 * 256 * sizeof(unsigned long long));
+  unsigned long long* ghist = hist.as<unsigned long long>();
+  unsigned long long* gbase = ghist + 8 * 256;
+  FHIP_CHECK(hipMemsetAsync(ghist, 0, size_t(8) * 256 * sizeof(unsigned long long), st));
+  int hm = 4;
+  const int grid = hist_grid(n, &hm);
+  {
+    ProfScope prof("radix_hist", st);
+    if (kbytes == 4) launch_hist<uint32_t>(hm, grid, keys, n, mode, 0, kbytes, ghist, st);
+    else launch_hist<uint64_t>(hm, grid, keys, n, mode, 0, kbytes, ghist, st);
+    rs_scan<<<kbytes, 256, 0, st>>>(ghist, gbase);
+    FHIP_CHECK(hipGetLastError());
+  }
+  unsigned long long* h =
+      static_cast<unsigned long long*>(pinned_scratch(1, size_t(kbytes) * 256 * 8));
+  FHIP_CHECK(hipMemcpyAsync(h, ghist, size_t(kbytes) * 256 * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  int passes[4];
+  int P = 0;
+  for (int b = 0; b < kbytes; b++) {
+    int dmax = 0;
+    unsigned long long mx = 0;
+    for (int d = 0; d < 256; d++)
+      if (h[size_t(b) * 256 + d] > mx) {
+        mx = h[size_t(b) * 256 + d];
+        dmax = d;
+      }
+    if (mx == n) {  // a byte shared by every key
+      if (b >= 4) *hi |= unsigned(dmax) << (8 * (b - 4));
+    } else if (b >= 4) {
+      return false;  // the keys differ above their low 32 bits
+    } else {
+      passes[P++] = b;
+    }
+  }
+  if (P == 0) {
+    if (kbytes == 4)
+      narrow_one_kernel<uint32_t><<<grid_for_n(n), 256, 0, st>>>(
+          static_cast<const uint32_t*>(keys), ids, n, mode, k32, row, iota_bad);
+    else
+      narrow_one_kernel<uint64_t><<<grid_for_n(n), 256, 0, st>>>(
+          static_cast<const uint64_t*>(keys), ids, n, mode, k32, row, iota_bad);
+    FHIP_CHECK(hipGetLastError());
+    return true;
+  }
+  // the last pass lands in (k32, row); 1024 x 16 tiles (512 x 16 when the
+  // tuned key/value shape is that one)
+  const int shape = chosen_shape(4) == 2 ? 2 : 6;
+  const size_t tile_keys = size_t(kTileShapes[shape].nt) * kTileShapes[shape].items;
+  const unsigned tiles = unsigned(ceil_div(n, tile_keys));
+  DBuf tk(n * 4), tr(n * 4);
+  const size_t lb_words = size_t(tiles) * 256;
+  DBuf lbuf(2 * lb_words * sizeof(unsigned long long) + 256);
+  unsigned long long* lb0 = lbuf.as<unsigned long long>();
+  unsigned long long* lb1 = lb0 + lb_words;
+  unsigned* ctr = reinterpret_cast<unsigned*>(lb1 + lb_words);
+  FHIP_CHECK(hipMemsetAsync(lb0, 0, lb_words * sizeof(unsigned long long), st));
+  FHIP_CHECK(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), st));
+  NarrowFirstFn first;
+  if (shape == 2) first = kbytes == 4 ? launch_narrow_first<uint32_t, 512, 16> : launch_narrow_first<uint64_t, 512, 16>;
+  else first = kbytes == 4 ? launch_narrow_first<uint32_t, 1024, 16> : launch_narrow_first<uint64_t, 1024, 16>;
+  PassFn rest = pick_pass(shape, 4, 4);
+  const void* ks = keys;
+  const void* vs = ids;
+  for (int i = 0; i < P; i++) {
+    const bool to_out = (P - 1 - i) % 2 == 0;
+    void* kd = to_out ? static_cast<void*>(k32) : tk.get();
+    void* vd = to_out ? static_cast<void*>(row) : tr.get();
+    unsigned long long* lb = (i & 1) ? lb1 : lb0;
+    unsigned long long* lbn = (i & 1) ? lb0 : lb1;
+    ProfScope prof("radix_pass", st);
+    if (i == 0)
+      first(ks, kd, vs, vd, n, 8 * passes[i], mode, gbase + size_t(passes[i]) * 256, lb, lbn,
+            ctr, ctr + 1, iota_bad, tiles, st);
+    else
+      rest(ks, kd, vs, vd, n, 8 * passes[i], KM_RAW, 0, gbase + size_t(passes[i]) * 256, lb, lbn,
+           ctr + (i & 1), ctr + ((i + 1) & 1), tiles, st);
+    ks = kd;
+    vs = vd;
+  }
+  return true;
 }
 
 void sort_pairs(int ktype, const void* kin, void* kout, int vbytes, const void* vin,

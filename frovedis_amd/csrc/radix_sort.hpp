@@ -58,6 +58,19 @@ void sort_pairs_bits(int kbytes, const void* kin, void* kout, int vbytes,
                      const void* vin, void* vout, size_t n, int key_bits,
                      hipStream_t stream);
 
+// Narrow stable sort (the sort-merge join's): integer keys whose bits above
+// the low 32 -- in the sort's order, the sign bit of signed keys flipped --
+// are shared by every key, with row ids (ids, size_t) that are consecutive.
+// Writes each key's low 32 bits in the sort's order (k32, ascending as
+// unsigned) and its row 0..n-1 (row): half the bytes of a key/row-id sort
+// per pass.  Returns false, having written nothing, when the keys differ
+// above their low 32 bits; *hi gets the shared high bits (in the sort's
+// order).  The row ids are checked on the device: *iota_bad (a device word
+// the caller zeroes) becomes non-zero when ids[i] != ids[0] + i for some i,
+// and the result must then not be used.
+bool sort_narrow(int ktype, const void* keys, const size_t* ids, size_t n, uint32_t* k32,
+                 uint32_t* row, unsigned* hi, unsigned* iota_bad, hipStream_t st);
+
 // Graph-capturable variant (no host synchronisation, memset or allocation):
 // stable sort of unsigned kbytes-wide keys on bytes [0, nbytes) into
 // (kout, vout), using a caller-provided workspace of

@@ -190,3 +190,39 @@ def test_hashtable_float_signed_zero_and_nan_conflicts(fa, with_negative):
     assert np.array_equal(gv.numpy(), ev)
     assert np.array_equal(ht.check_existence(fa.DeviceArray.from_numpy(q)).numpy(),
                           ref.check_existence(q))
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.uint32, np.int64, np.uint64])
+@pytest.mark.parametrize("case", ["low", "signed_span", "high", "apart"])
+def test_sort_merge_narrow_path_order(fa, dt, case):
+    # consecutive row ids on both sides: the sort-merge join sorts (low 32
+    # key bits, 32-bit row) pairs when each side's keys share their higher
+    # bits ("high": a 2^32 stretch far from zero; "apart": the two sides in
+    # different stretches, so nothing matches), and the general key/row-id
+    # sort otherwise ("signed_span": 64-bit keys on both sides of zero).
+    # Pairs in the reference's sort-merge order (join.hpp:304-326), the left
+    # rows without a match in stable key order (:352-378), bit for bit.
+    rng = np.random.default_rng(71)
+    nl, nr = 40000, 90000
+    wide = np.dtype(dt).itemsize == 8
+    signed = np.dtype(dt).kind == "i"
+    base = {"low": 0, "signed_span": -20000 if signed else 0,
+            "high": (1 << 40) if wide else (1 << 30), "apart": (1 << 40) if wide else 0}[case]
+    l = rng.integers(0, 30000, nl).astype(np.int64) + base
+    r = rng.integers(0, 30000, nr).astype(np.int64) + base
+    if case == "apart":
+        r = r + ((1 << 33) if wide else 50000)
+    l, r = l.astype(dt), r.astype(dt)
+    li = np.arange(nl, dtype=np.uint64) + np.uint64(5)
+    ri = np.arange(nr, dtype=np.uint64) + np.uint64(1000)
+    dl, dr = fa.DeviceArray.from_numpy(l), fa.DeviceArray.from_numpy(r)
+    dli, dri = fa.DeviceArray.from_numpy(li), fa.DeviceArray.from_numpy(ri)
+    elo, ero, _ = oracle.equi_join(l, li, r, ri, "ve_sortmerge")
+    lo, ro, used = fa.equi_join(dl, dli, dr, dri, strategy="sortmerge")
+    assert not used
+    assert np.array_equal(lo.numpy(), elo) and np.array_equal(ro.numpy(), ero)
+    olo, oro, only, oused = oracle.outer_equi_join(l, li, r, ri, "ve")
+    assert not oused  # the right side repeats keys: the sort-merge body
+    glo, gro, gonly, gused = fa.outer_equi_join(dl, dli, dr, dri, strategy="sortmerge")
+    assert np.array_equal(glo.numpy(), olo) and np.array_equal(gro.numpy(), oro)
+    assert np.array_equal(gonly.numpy(), only)
