@@ -837,9 +837,12 @@ bool sort_narrow(int ktype, const void* keys, const size_t* ids, size_t n, uint3
     FHIP_CHECK(hipGetLastError());
     return true;
   }
-  // the last pass lands in (k32, row); 1024 x 16 tiles (512 x 16 when the
-  // tuned key/value shape is that one)
-  const int shape = chosen_shape(4) == 2 ? 2 : 6;
+  // the last pass lands in (k32, row).  Tiles of 512 x 24 (two workgroups
+  // a CU): at 2^29 28-bit keys with 32-bit rows 13.5 ms against 15.9 ms for
+  // 1024 x 16 and 14.8 ms for 512 x 16 (the tunable radix.shape_narrow picks
+  // 512 x 16 (2), 512 x 24 (3) or 1024 x 16 (6))
+  const long long want = tune_get("radix.shape_narrow", 3);
+  const int shape = (want == 2 || want == 6) ? int(want) : 3;
   const size_t tile_keys = size_t(kTileShapes[shape].nt) * kTileShapes[shape].items;
   const unsigned tiles = unsigned(ceil_div(n, tile_keys));
   DBuf tk(n * 4), tr(n * 4);
@@ -852,6 +855,7 @@ bool sort_narrow(int ktype, const void* keys, const size_t* ids, size_t n, uint3
   FHIP_CHECK(hipMemsetAsync(ctr, 0, 2 * sizeof(unsigned), st));
   NarrowFirstFn first;
   if (shape == 2) first = kbytes == 4 ? launch_narrow_first<uint32_t, 512, 16> : launch_narrow_first<uint64_t, 512, 16>;
+  else if (shape == 3) first = kbytes == 4 ? launch_narrow_first<uint32_t, 512, 24> : launch_narrow_first<uint64_t, 512, 24>;
   else first = kbytes == 4 ? launch_narrow_first<uint32_t, 1024, 16> : launch_narrow_first<uint64_t, 1024, 16>;
   PassFn rest = pick_pass(shape, 4, 4);
   const void* ks = keys;

@@ -75,6 +75,11 @@ int fhip_prof_query(const char* name, double* total_ms, size_t* launches);
  *   "radix.shape_k", "radix.shape_kv"  tile shape of the radix sort pass
  *       (index: 0 256x16, 1 256x32, 2 512x16, 3 512x24, 4 512x12, 5 1024x8,
  *       6 1024x16, 7 768x16 threads x keys) for key-only / key-value sorts;
+ *   "radix.shape_narrow"  tile shape of the sort-merge join's narrow sort
+ *       (32-bit keys and rows): 2, 3 (default) or 6;
+ *   "join.narrow"       0 switches off the sort-merge join's narrow path;
+ *   "groupby.cold32"    0 keeps whole 64-bit cold keys in the group-by's
+ *       heavy-key pass (else their low 32 bits when the range path applies);
  *   "groupby.slots"     LDS hash table slots of the fused group-by (power of 2);
  *   "groupby.fill_pct"  expected groups per partition as a percentage of the
  *       slots (the hash load factor);
