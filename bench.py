@@ -274,6 +274,36 @@ def bench_radix(fa, log2n, steps=3):
             "hbm_frac": algo / kern_s / HBM_PEAK if kern_s else None}
 
 
+def bench_radix_host(fa, log2n=24, steps=3):
+    """The host-vector drop-in: radix_sort(std::vector<uint64_t>&,
+    std::vector<size_t>&) (radix_sort.hpp:489-496) through fhip_radix_sort on
+    host memory -- copies in and out through pinned staging included, so the
+    rate is PCIe-inclusive (not the device-resident value of the other legs).
+    Same keys as the key/value leg; the output is checked sorted and a
+    permutation of the input rows."""
+    import ctypes
+
+    import numpy as np
+
+    from frovedis_amd import datagen as dg
+    n = 1 << log2n
+    k0 = dg.radix_keys_dev(n).cpu().numpy().view(np.uint64)
+    keys = np.empty_like(k0)
+    vals = np.empty(n, dtype=np.uint64)
+    best = float("inf")
+    L = fa.lib()
+    for _ in range(steps + 1):
+        keys[:] = k0
+        vals[:] = np.arange(n, dtype=np.uint64)
+        t0 = time.perf_counter()
+        fa.check(L.fhip_radix_sort(fa.TYPE_CODE[np.dtype(np.uint64)], keys.ctypes.data_as(ctypes.c_void_p),
+                                   8, vals.ctypes.data_as(ctypes.c_void_p), n, 0, 0))
+        best = min(best, time.perf_counter() - t0)
+    ok = bool(np.all(keys[1:] >= keys[:-1])) and bool(np.array_equal(k0[vals], keys))
+    return {"keys": n, "ms": best * 1e3, "keys_per_s": n / best, "checked": ok,
+            "note": "host pointers: pinned-staged PCIe copies in and out included"}
+
+
 def bench_learned(fa, log2n, steps=3):
     """Learned-CDF partition sort vs the plain radix sort on the same keys:
     2^log2n int64 keys from the 8-component lognormal mixture with 5%
@@ -571,7 +601,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the side measurements")
     ap.add_argument("--leg", default="all",
                     choices=["all", "groupby", "radix", "batch", "join", "learned", "copy",
-                             "relational"],
+                             "relational", "radix_host"],
                     help="run one measurement alone (for profiling); 'copy' is a plain "
                          "device copy of a known size, the counter calibration")
     args = ap.parse_args()
@@ -591,6 +621,7 @@ def main():
                 "join": lambda: bench_join(fa),
                 "learned": lambda: bench_learned(fa, args.rows),
                 "relational": lambda: bench_relational(fa, args.rows),
+                "radix_host": lambda: bench_radix_host(fa, min(args.rows, 26)),
                 "copy": lambda: bench_copy(args.rows)}
         print(f"# leg {args.leg}", file=sys.stderr, flush=True)
         print(json.dumps({"leg": args.leg, **legs[args.leg]()}))
@@ -677,6 +708,7 @@ def main():
         print("# radix sort side measurements", file=sys.stderr, flush=True)
         result["extra"]["radix_kv_u64_2^24"] = bench_radix(fa, 24)
         result["extra"]["radix_kv_u64_2^28"] = bench_radix(fa, 28)
+        result["extra"]["radix_kv_u64_2^24_host_vectors"] = bench_radix_host(fa, 24)
         torch.cuda.empty_cache()
         print("# batched segments", file=sys.stderr, flush=True)
         result["extra"]["batched_segments_2^28"] = bench_batch(fa)
