@@ -448,6 +448,11 @@ __device__ __forceinline__ unsigned long long dj_rid(E v, unsigned long long rba
 // (decoupled look-back); pairs at positions >= capacity are counted, not
 // written.  The left row ids are loaded with the positions, before the
 // look-back.  Rows go in two rounds of DJ_EITEMS per thread.
+// only_left (the outer join; else null): the left row ids without a
+// partner, in left order.  Every tile before this one is full, so the misses
+// before it are its first row minus the hits before it, and a row's misses
+// within the tile are its index minus the hits before it: no second
+// look-back.
 constexpr int DJ_EITEMS = 8;
 constexpr int DJ_EROUNDS = DJ_ITEMS / DJ_EITEMS;
 
@@ -457,7 +462,7 @@ __global__ __launch_bounds__(DJ_NT) void dj_emit_kernel(
     const unsigned short* __restrict__ tile_cnt, E empty, int check_empty,
     unsigned long long rbase, const size_t* __restrict__ lidx, size_t* __restrict__ lo,
     size_t* __restrict__ ro, size_t capacity, unsigned long long* tile_state, unsigned* ticket,
-    unsigned long long* total) {
+    unsigned long long* total, size_t* __restrict__ only_left) {
   constexpr int WAVES = DJ_NT / 64;
   constexpr int COUNTS = DJ_ITEMS * WAVES;
   constexpr int PER_LANE = COUNTS / 64;
@@ -555,34 +560,19 @@ __global__ __launch_bounds__(DJ_NT) void dj_emit_kernel(
 #pragma unroll
     for (int q = 0; q < DJ_EITEMS; q++) {
       const int j = h * DJ_EITEMS + q;
+      const unsigned before = s_cnt[j * WAVES + wave] + lane_rank(hit[j]);  // hits in the tile
       if ((hit[j] >> lane) & 1ull) {
-        const size_t pos = size_t(excl + s_cnt[j * WAVES + wave] + lane_rank(hit[j]));
+        const size_t pos = size_t(excl + before);
         if (pos < capacity) {
           const unsigned dp = (dpk[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu;
           lo[pos] = lid[q];
           ro[pos] = dj_rid(s_res[dp], rbase);
         }
+      } else if (only_left) {
+        const unsigned i = unsigned(j) * DJ_NT + threadIdx.x;
+        if (i < rows) only_left[size_t(base - excl) + (i - before)] = lid[q];
       }
     }
-  }
-}
-
-// Per left row: found[row] and vals[row] (the right row id, 0 for a miss),
-// for the outer join.
-template <class E>
-__global__ __launch_bounds__(256) void dj_flags_kernel(const unsigned short* __restrict__ dest,
-                                                       size_t n, const E* __restrict__ res,
-                                                       E empty, int check_empty,
-                                                       unsigned long long rbase,
-                                                       unsigned long long* __restrict__ vals,
-                                                       uint8_t* __restrict__ found) {
-  for (size_t row = size_t(blockIdx.x) * 256 + threadIdx.x; row < n;
-       row += size_t(gridDim.x) * 256) {
-    const unsigned dp = dest[row];
-    const E v = res[(row / DJ_TILE) * DJ_TILE + (dp != DJ_MISS ? dp : 0u)];
-    const bool h = dp != DJ_MISS && !(check_empty && v == empty);
-    vals[row] = h ? dj_rid(v, rbase) : 0ull;
-    found[row] = h ? 1 : 0;
   }
 }
 
@@ -763,30 +753,14 @@ void dj_probe(const DenseTable& t, const K* left, size_t nl, DBuf& dest, DBuf& r
 template <class K, class E>
 int dj_run(const K* left, const size_t* lidx, size_t nl, const K* right, const size_t* ridx,
            size_t nr, DenseTable& t, size_t* lo, size_t* ro, size_t capacity, size_t* npairs,
-           DBuf* vals, DBuf* found, hipStream_t st) {
+           size_t* only_left, size_t* n_only, hipStream_t st) {
   if (!dj_build<K, E>(right, ridx, nr, t, st)) return DJ_DUP;
-  if (npairs) *npairs = 0;
-  if (nl == 0) {
-    if (vals) {
-      vals->reset(0);
-      found->reset(0);
-    }
-    return DJ_DONE;
-  }
+  *npairs = 0;
+  if (n_only) *n_only = 0;
+  if (nl == 0) return DJ_DONE;
   DBuf dest, res, starts;  // starts row P: the results in each tile
   dj_probe<K, E>(t, left, nl, dest, res, starts, st);
   const E empty = E(~E(0));
-  if (vals) {
-    vals->reset(nl * sizeof(unsigned long long));
-    found->reset(nl);
-    ProfScope prof("join_dense_emit", st);
-    dj_flags_kernel<E><<<grid_for_n(nl), 256, 0, st>>>(dest.as<unsigned short>(), nl, res.as<E>(),
-                                                       empty, t.check_empty, t.rbase,
-                                                       vals->as<unsigned long long>(),
-                                                       found->as<uint8_t>());
-    FHIP_CHECK(hipGetLastError());
-    return DJ_DONE;
-  }
   const size_t nt = ceil_div(nl, size_t(DJ_TILE));
   FHIP_REQUIRE(nt < (size_t(1) << 31), FHIP_ERR_TOO_LARGE, "equi_join: too many probe rows");
   DBuf ws((nt + 2) * sizeof(unsigned long long));
@@ -797,20 +771,21 @@ int dj_run(const K* left, const size_t* lidx, size_t nl, const K* right, const s
     dj_emit_kernel<E><<<unsigned(nt), DJ_NT, 0, st>>>(
         dest.as<unsigned short>(), nl, res.as<E>(),
         starts.as<unsigned short>() + size_t(t.P) * nt, empty, t.check_empty, t.rbase, lidx, lo,
-        ro, capacity, state, reinterpret_cast<unsigned*>(state + nt), state + nt + 1);
+        ro, capacity, state, reinterpret_cast<unsigned*>(state + nt), state + nt + 1, only_left);
     FHIP_CHECK(hipGetLastError());
   }
   unsigned long long* h = static_cast<unsigned long long*>(pinned_scratch(0, 8));
   FHIP_CHECK(hipMemcpyAsync(h, state + nt + 1, 8, hipMemcpyDeviceToHost, st));
   FHIP_CHECK(hipStreamSynchronize(st));
   *npairs = size_t(*h);
+  if (n_only) *n_only = nl - size_t(*h);
   return DJ_DONE;
 }
 
 template <class K>
 int dense_join_t(const K* left, const size_t* lidx, size_t nl, const K* right,
                  const size_t* ridx, size_t nr, size_t* lo, size_t* ro, size_t capacity,
-                 size_t* npairs, DBuf* vals, DBuf* found, hipStream_t st) {
+                 size_t* npairs, size_t* only_left, size_t* n_only, hipStream_t st) {
   DenseTable t;
   bool wide = false;
   // 32-bit row positions within the partitioned copies
@@ -818,9 +793,9 @@ int dense_join_t(const K* left, const size_t* lidx, size_t nl, const K* right,
   if (!dj_plan<K>(right, ridx, nr, t, wide, st)) return DJ_NA;
   if (wide)
     return dj_run<K, unsigned long long>(left, lidx, nl, right, ridx, nr, t, lo, ro, capacity,
-                                         npairs, vals, found, st);
-  return dj_run<K, unsigned>(left, lidx, nl, right, ridx, nr, t, lo, ro, capacity, npairs, vals,
-                             found, st);
+                                         npairs, only_left, n_only, st);
+  return dj_run<K, unsigned>(left, lidx, nl, right, ridx, nr, t, lo, ro, capacity, npairs,
+                             only_left, n_only, st);
 }
 
 template <class F>
@@ -837,20 +812,12 @@ int dj_dispatch(int ktype, F&& f) {
 
 int dense_join_pairs(int ktype, const void* left, const size_t* lidx, size_t nl,
                      const void* right, const size_t* ridx, size_t nr, size_t* lo, size_t* ro,
-                     size_t capacity, size_t* npairs, hipStream_t st) {
+                     size_t capacity, size_t* npairs, size_t* only_left, size_t* n_only,
+                     hipStream_t st) {
   return dj_dispatch(ktype, [&](auto z) {
     using K = decltype(z);
     return dense_join_t<K>((const K*)left, lidx, nl, (const K*)right, ridx, nr, lo, ro, capacity,
-                           npairs, nullptr, nullptr, st);
-  });
-}
-
-int dense_join_flags(int ktype, const void* left, size_t nl, const void* right,
-                     const size_t* ridx, size_t nr, DBuf& vals, DBuf& found, hipStream_t st) {
-  return dj_dispatch(ktype, [&](auto z) {
-    using K = decltype(z);
-    return dense_join_t<K>((const K*)left, nullptr, nl, (const K*)right, ridx, nr, nullptr,
-                           nullptr, 0, nullptr, &vals, &found, st);
+                           npairs, only_left, n_only, st);
   });
 }
 

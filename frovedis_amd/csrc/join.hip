@@ -816,14 +816,8 @@ size_t equi_join(int ktype, const void* left, const size_t* lidx, size_t nl, con
       if constexpr (std::is_integral<K>::value) {
         // a unique key column that fills its range: the partitioned direct
         // table (densejoin.hip); else the open-addressing table
-        int dres = DJ_NA;
-        if (!only_left) {
-          dres = dense_join_pairs(ktype, left, lidx, nl, right, ridx, nr, lo, ro, capacity, &r, st);
-        } else {
-          DBuf vals, found;
-          dres = dense_join_flags(ktype, left, nl, right, ridx, nr, vals, found, st);
-          if (dres == DJ_DONE) r = emit(vals, found);
-        }
+        const int dres = dense_join_pairs(ktype, left, lidx, nl, right, ridx, nr, lo, ro,
+                                          capacity, &r, only_left, n_only, st);
         if (dres != DJ_NA) {
           uniq = dres == DJ_DONE;
           done = true;
