@@ -756,6 +756,119 @@ __global__ __launch_bounds__(256) void sj_and_kernel(uint8_t* __restrict__ acc,
   for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) acc[i] &= f[i];
 }
 
+// A dimension whose unique keys fill their range (at most twice the rows,
+// at most 2^24 keys) and whose row ids are ids[0] + i is probed through an
+// array indexed by key - min that holds each key's row (4 bytes a key: at
+// most 64 MB, 4 MB for 2^20 keys, so the probes hit the cache).
+constexpr unsigned SJ_EMPTY = ~0u;
+
+template <class K>
+__device__ __forceinline__ unsigned long long sj_ordered(K k) {
+  if constexpr (std::is_signed<K>::value) return (unsigned long long)(long long)k ^ (1ull << 63);
+  else return (unsigned long long)k;
+}
+
+// st[0] min, st[1] max of the ordered keys, st[2] != 0 when the ids are
+// not ids[0] + i; the caller sets st to {~0, 0, 0} first
+template <class K>
+__global__ __launch_bounds__(256) void sj_stats_kernel(const K* __restrict__ keys,
+                                                       const size_t* __restrict__ ids, size_t n,
+                                                       unsigned long long* __restrict__ st) {
+  unsigned long long mn = ~0ull, mx = 0ull;
+  bool bad = false;
+  const size_t id0 = ids[0];
+  const size_t stride = size_t(gridDim.x) * 256;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride) {
+    const unsigned long long u = sj_ordered(keys[i]);
+    mn = u < mn ? u : mn;
+    mx = u > mx ? u : mx;
+    if (ids[i] != id0 + i) bad = true;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  const bool anybad = __ballot(bad) != 0ull;
+  if (__lane_id() == 0) {
+    atomicMin(&st[0], mn);
+    atomicMax(&st[1], mx);
+    if (anybad) atomicOr(&st[2], 1ull);
+  }
+}
+
+__global__ void sj_init_kernel(unsigned long long* __restrict__ st, unsigned* __restrict__ tab,
+                               size_t R) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < R; i += stride)
+    tab[i] = SJ_EMPTY;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st[3] = 0ull;
+}
+
+// st[3] != 0 when a key repeats
+template <class K>
+__global__ __launch_bounds__(256) void sj_build_kernel(const K* __restrict__ keys, size_t n,
+                                                       unsigned long long mn,
+                                                       unsigned* __restrict__ tab,
+                                                       unsigned long long* __restrict__ st) {
+  const size_t stride = size_t(gridDim.x) * 256;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += stride)
+    if (atomicCAS(&tab[sj_ordered(keys[i]) - mn], SJ_EMPTY, unsigned(i)) != SJ_EMPTY)
+      atomicOr(&st[3], 1ull);
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void sj_probe_kernel(const K* __restrict__ q, size_t m,
+                                                       unsigned long long mn, unsigned long long R,
+                                                       const unsigned* __restrict__ tab,
+                                                       const size_t* __restrict__ ids,
+                                                       unsigned long long* __restrict__ vals,
+                                                       uint8_t* __restrict__ found) {
+  const size_t id0 = ids[0];
+  const size_t stride = size_t(gridDim.x) * 256;
+  for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < m; i += stride) {
+    const unsigned long long o = sj_ordered(q[i]) - mn;  // wraps below mn
+    const unsigned pos = o < R ? tab[o] : SJ_EMPTY;
+    vals[i] = pos != SJ_EMPTY ? id0 + pos : 0ull;
+    found[i] = pos != SJ_EMPTY ? 1 : 0;
+  }
+}
+
+// The direct-array build and probe of one dimension; returns -1 when it
+// does not apply (then the caller uses the hash table), else 1 for unique
+// keys and 0 for a repeated key.
+template <class K>
+int sj_direct(const K* keys, const size_t* ids, size_t n, const K* q, size_t m,
+              unsigned long long* vals, uint8_t* found, hipStream_t st) {
+  if (n < (size_t(1) << 16) || n > (size_t(1) << 24) || tune_get("join.star_direct", 1) == 0)
+    return -1;
+  DBuf sb(4 * sizeof(unsigned long long));
+  unsigned long long* stv = sb.as<unsigned long long>();
+  const unsigned long long init[3] = {~0ull, 0ull, 0ull};
+  unsigned long long* h = static_cast<unsigned long long*>(pinned_scratch(0, 4 * 8));
+  std::memcpy(h, init, sizeof(init));
+  FHIP_CHECK(hipMemcpyAsync(stv, h, sizeof(init), hipMemcpyHostToDevice, st));
+  sj_stats_kernel<K><<<grid_for_n(n), 256, 0, st>>>(keys, ids, n, stv);
+  FHIP_CHECK(hipGetLastError());
+  FHIP_CHECK(hipMemcpyAsync(h, stv, 3 * 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  const unsigned long long mn = h[0], mx = h[1];
+  if (h[2] != 0 || mx < mn || mx - mn >= 2 * (unsigned long long)n) return -1;
+  const unsigned long long R = mx - mn + 1;
+  DBuf tab(size_t(R) * sizeof(unsigned));
+  sj_init_kernel<<<grid_for_n(size_t(R)), 256, 0, st>>>(stv, tab.as<unsigned>(), size_t(R));
+  sj_build_kernel<K><<<grid_for_n(n), 256, 0, st>>>(keys, n, mn, tab.as<unsigned>(), stv);
+  FHIP_CHECK(hipGetLastError());
+  FHIP_CHECK(hipMemcpyAsync(h + 3, stv + 3, 8, hipMemcpyDeviceToHost, st));
+  FHIP_CHECK(hipStreamSynchronize(st));
+  if (h[3] != 0) return 0;
+  sj_probe_kernel<K><<<grid_for_n(m), 256, 0, st>>>(q, m, mn, R, tab.as<unsigned>(), ids, vals,
+                                                    found);
+  FHIP_CHECK(hipGetLastError());
+  return 1;
+}
+
 struct StarWriter {
   const size_t* lidx;  // null: the left rows are 0..nl-1
   size_t* lo;
@@ -956,9 +1069,18 @@ size_t star_join(int nd, const int* ktypes, const void* const* left_cols, const 
     dispatch_type(ktypes[d], [&](auto z) {
       using K = decltype(z);
       if constexpr (std::is_integral<K>::value) {
-        const bool uniq = jt_build_probe<K>(
-            (const K*)right_cols[d], reinterpret_cast<const unsigned long long*>(ridx[d]), nr[d],
-            (const K*)q, nl, vals[d].as<unsigned long long>(), f, st);
+        int direct = -1;
+        {
+          ProfScope prof("join_probe", st);
+          direct = sj_direct<K>((const K*)right_cols[d], ridx[d], nr[d], (const K*)q, nl,
+                                vals[d].as<unsigned long long>(), f, st);
+        }
+        const bool uniq =
+            direct >= 0 ? direct == 1
+                        : jt_build_probe<K>((const K*)right_cols[d],
+                                            reinterpret_cast<const unsigned long long*>(ridx[d]),
+                                            nr[d], (const K*)q, nl,
+                                            vals[d].as<unsigned long long>(), f, st);
         FHIP_REQUIRE(uniq, FHIP_ERR_ARG, "star_join: right column is not unique");
         done = true;
       }

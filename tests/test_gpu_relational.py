@@ -377,3 +377,37 @@ def test_star_join_errors_and_empty_dimension(fa):
     lo, ros = fa.star_join([left], [(_d(fa, np.zeros(0, np.int64)),
                                      _d(fa, np.zeros(0, np.uint64)))])
     assert len(lo) == 0 and len(ros[0]) == 0
+
+
+@pytest.mark.parametrize("t", [np.int32, np.uint32, np.int64, np.uint64])
+def test_star_join_dense_dimensions(fa, t):
+    # dimensions of 2^17 unique keys within twice their rows, with row ids
+    # ids[0] + i: probed through a direct array (negative keys for signed
+    # types, keys near the top of the range for unsigned ones); a dimension
+    # with shuffled ids or a wide key range takes the hash table; the same
+    # rows as the oracle either way, and a repeated key still raises
+    rng = np.random.default_rng(91)
+    nl, nr = 300000, 1 << 17
+    signed = np.dtype(t).kind == "i"
+    base = -(1 << 16) if signed else np.iinfo(t).max - 3 * nr
+    dims, left = [], []
+    def shift(a):  # a + base in the key type (two's complement wrap for uint32)
+        if signed:
+            return (a.astype(np.int64) + base).astype(t)
+        return (a.astype(np.uint64) + np.uint64(base)).astype(t)
+    for kind in ("dense", "dense", "shuffled_ids", "wide"):
+        span = 2 * nr if kind != "wide" else 64 * nr
+        keys = shift(rng.choice(span, nr, replace=False))
+        ids = (np.arange(nr, dtype=np.uint64) + np.uint64(11) if kind != "shuffled_ids"
+               else rng.permutation(nr).astype(np.uint64))
+        dims.append((keys, ids))
+        left.append(shift(rng.integers(0, span + 100, nl) - 50))  # some below the keys
+    rlo, rros = oracle.star_join(left, dims, None)
+    lo, ros = fa.star_join([_d(fa, c) for c in left], [(_d(fa, k), _d(fa, i)) for k, i in dims])
+    np.testing.assert_array_equal(lo.numpy(), rlo)
+    for r, rr in zip(ros, rros):
+        np.testing.assert_array_equal(r.numpy(), rr)
+    keys = shift(np.arange(nr))
+    keys[7] = keys[9]
+    with pytest.raises(fa.FhipError, match="not unique"):
+        fa.star_join([_d(fa, left[0])], [(_d(fa, keys), _d(fa, np.arange(nr, dtype=np.uint64)))])
