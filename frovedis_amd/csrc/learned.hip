@@ -45,7 +45,7 @@ constexpr int L2_NT = 512;
 constexpr int L2_ITEMS = 16;
 constexpr int L2_CHUNK = L2_NT * L2_ITEMS;  // keys per second-level workgroup
 constexpr int LS_NT = 512;
-constexpr int L3_NT = 512;
+constexpr int L3_NT = 1024;
 constexpr int L3_CAP = 8192;       // keys a fine bucket may hold (LDS)
 constexpr int L3_ITEMS = L3_CAP / L3_NT;
 constexpr int L3_B = 2048;         // interpolation sub-buckets per leaf
