@@ -41,8 +41,8 @@ namespace {
 constexpr int LS_B1 = 1024;        // coarse buckets (routed by the network)
 constexpr int LS_NF = 65536;       // leaves (fine buckets, sample quantiles)
 constexpr int L2_LDS_FAN = 2048;   // leaves per coarse bucket kept in LDS
-constexpr int L2_NT = 512;
-constexpr int L2_ITEMS = 16;
+constexpr int L2_NT = 1024;
+constexpr int L2_ITEMS = 8;
 constexpr int L2_CHUNK = L2_NT * L2_ITEMS;  // keys per second-level workgroup
 constexpr int LS_NT = 512;
 constexpr int L3_NT = 1024;
