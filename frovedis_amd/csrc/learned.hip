@@ -1121,7 +1121,7 @@ size_t learned_sort(const fhip_cdf_model* m, void* keys, size_t n, int coarse_mo
   uint64_t* k = static_cast<uint64_t*>(keys);
   {
     ProfScope prof("learned_count", st);
-    const int grid = int(std::min<size_t>(ceil_div(n, LS_NT * 8), size_t(fill_blocks(LS_NT, 2))));
+    const int grid = int(std::min<size_t>(ceil_div(n, LS_NT * 8), size_t(fill_blocks(LS_NT, 4))));
     l1_count_kernel<<<grid, LS_NT, 0, st>>>(k, n, sgn, dco, dct, cbase, hist);
     FHIP_CHECK(hipGetLastError());
   }
